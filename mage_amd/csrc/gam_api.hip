@@ -1,0 +1,729 @@
+// gam_api.hip -- the C ABI (include/gamhip.h): context, layout conversion
+// and the step driver.  Host code only; kernels live in gam_step.hip and
+// gam_grid.hip.
+#include "gamhip.h"
+#include "gam_kernels.h"
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <vector>
+#include <new>
+#include <stddef.h>
+
+static thread_local char g_err[512] = "";
+
+static int fail(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(call)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (call);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return fail(GAM_E_HIP, "%s failed: %s", #call, hipGetErrorString(e_));    \
+    } while (0)
+
+enum { KG_PREDICT = 0, KG_MOM2FACE, KG_EMF, KG_FLUX, KG_CT, KG_GAS, KG_BC, KG_BFLD };
+static const char *kg_names[GAM_KG_COUNT] = {"predict", "mom2face", "emf", "flux",
+                                             "ct_b", "gas_dt", "bcs_finalize", "bflux2fld"};
+
+struct gam_ctx {
+    gam_dims dims;
+    gam_model model;
+    MParm mp;
+    Geo g;
+    hipStream_t stream;
+    double *st[2];        // State / oState buffers (S_NPL planes each)
+    int cur;              // which of st[] is State
+    double *H;            // half state + primitive planes
+    double *met;          // metric planes
+    double *bg;           // background planes (+9 scratch)
+    double *gflx, *mflx, *vf;
+    double *stage;        // one-plane staging buffer for layout conversion
+    double *part;         // reduction partials
+    Clock *clk;
+    double *wind;         // device wind series
+    WindDev wd;
+    int haveGrid, haveBG, haveWind;
+    // profiling
+    int prof;
+    hipEvent_t ev[2 * GAM_KG_COUNT * 8];
+    int nev;
+    double kg_ms[GAM_KG_COUNT];
+    int kg_n[GAM_KG_COUNT];
+    std::vector<int> pend;   // group id of each pending event pair
+};
+
+// ------------------------------------------------------------ geometry
+static Box mkbox(int i0, int i1, int j0, int j1, int k0, int k1)
+{
+    Box b = {i0, i1, j0, j1, k0, k1};
+    return b;
+}
+static Box face_box(const Geo &g) { return mkbox(g.isg, g.ieg + 1, g.jsg, g.jeg + 1, g.ksg, g.keg + 1); }
+static Box cell_box(const Geo &g) { return mkbox(g.isg, g.ieg, g.jsg, g.jeg, g.ksg, g.keg); }
+static Box act_box(const Geo &g) { return mkbox(g.is, g.ie, g.js, g.je, g.ks, g.ke); }
+static Box dt_box(const Geo &g) { return mkbox(g.is - 1, g.ie + 1, g.js - 1, g.je + 1, g.ks - 1, g.ke + 1); }
+static void edge_boxes(const Geo &g, Box *e)
+{   // active EMF edges (fields.F90:100-175)
+    e[0] = mkbox(g.is, g.ie, g.js, g.je + 1, g.ks, g.ke + 1);
+    e[1] = mkbox(g.is, g.ie + 1, g.js, g.je, g.ks, g.ke + 1);
+    e[2] = mkbox(g.is, g.ie + 1, g.js, g.je + 1, g.ks, g.ke);
+}
+
+static double *S_cur(gam_ctx *c) { return c->st[c->cur]; }
+static double *S_old(gam_ctx *c) { return c->st[1 - c->cur]; }
+
+// ------------------------------------------------------------ profiling
+static int prof_flush(gam_ctx *c);
+static const int PROF_CAP = (int)(sizeof(((gam_ctx *)0)->ev) / sizeof(hipEvent_t));
+static void prof_begin(gam_ctx *c, int grp)
+{
+    if (!c->prof) return;
+    if (c->nev + 2 > PROF_CAP) prof_flush(c);
+    hipEventRecord(c->ev[c->nev], c->stream);
+    c->pend.push_back(grp);
+}
+static void prof_end(gam_ctx *c, int grp)
+{
+    (void)grp;
+    if (!c->prof) return;
+    hipEventRecord(c->ev[c->nev + 1], c->stream);
+    c->nev += 2;
+}
+static int prof_flush(gam_ctx *c)
+{
+    if (!c->prof || c->nev == 0) return GAM_OK;
+    HIPCHK(hipEventSynchronize(c->ev[c->nev - 1]));
+    for (int q = 0; q < c->nev / 2; q++) {
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev[2 * q], c->ev[2 * q + 1]));
+        c->kg_ms[c->pend[q]] += ms;
+        c->kg_n[c->pend[q]] += 1;
+    }
+    c->nev = 0;
+    c->pend.clear();
+    return GAM_OK;
+}
+
+// ------------------------------------------------------------ create
+extern "C" const char *gam_last_error(void) { return g_err; }
+
+extern "C" int gam_create(gam_ctx **out, const gam_dims *d, const gam_model *m)
+{
+    if (!out || !d || !m) return fail(GAM_E_ARG, "gam_create: NULL argument");
+    *out = nullptr;
+    if (d->nG != 4) return fail(GAM_E_ARG, "gam_create: nG must be 4 (got %d)", d->nG);
+    if (d->Nip < 16) return fail(GAM_E_ARG, "Grid I-size smaller than VECLEN! (Nip=%d < 16)", d->Nip);
+    if (d->Nip % 8 != 0) return fail(GAM_E_ARG, "Grid divisibility error, Grid I-size not divisible by 8 (Nip=%d)", d->Nip);
+    if (d->Njp < 8 || d->Nkp < 8) return fail(GAM_E_ARG, "gam_create: Njp and Nkp must be >= 8 (got %d, %d)", d->Njp, d->Nkp);
+    for (int s = 0; s < 6; s++) {
+        int k = m->bc[s];
+        if (k < GAM_BC_PERIODIC || k > GAM_BC_WIND) return fail(GAM_E_ARG, "gam_create: bad BC kind %d on side %d", k, s);
+        if (k == GAM_BC_SPHIN && s != 0) return fail(GAM_E_ARG, "gam_create: sphere-in BC only on the inner I face");
+        if ((k == GAM_BC_SPHOUT || k == GAM_BC_WIND) && s != 1) return fail(GAM_E_ARG, "gam_create: sphere-out/wind BC only on the outer I face");
+    }
+    if (m->rmeth != GAM_RECON_8CENT && m->rmeth != GAM_RECON_7UP) return fail(GAM_E_ARG, "Uknown sim/rmeth (%d)", m->rmeth);
+    if (!(m->gamma > 1.0)) return fail(GAM_E_ARG, "gam_create: gamma must exceed 1");
+
+    gam_ctx *c = (gam_ctx *)calloc(1, sizeof(gam_ctx));
+    if (!c) return fail(GAM_E_NOMEM, "gam_create: host allocation failed");
+    new (&c->pend) std::vector<int>();
+    c->dims = *d;
+    c->model = *m;
+    MParm &p = c->mp;
+    p.gamma = m->gamma; p.CFL = m->CFL; p.Vd0 = m->Vd0; p.Ca = m->Ca; p.cHogH = m->cHogH;
+    p.cHogM = m->cHogM; p.pdmb = m->pdmb; p.dFloor = m->dFloor; p.pFloor = m->pFloor;
+    p.tFin = m->tFin; p.dt0 = m->dt0; p.limDT0 = m->limDT0;
+    p.doMHD = m->doMHD; p.doBoris = m->doBoris; p.doBG = m->doBackground; p.doLFM = m->doLFMLim;
+    p.rmeth = m->rmeth;
+
+    Geo &g = c->g;
+    g.Nip = d->Nip; g.Njp = d->Njp; g.Nkp = d->Nkp;
+    g.Ni = d->Nip + 8; g.Nj = d->Njp + 8; g.Nk = d->Nkp + 8;
+    g.is = 1; g.ie = d->Nip; g.js = 1; g.je = d->Njp; g.ks = 1; g.ke = d->Nkp;
+    g.isg = -3; g.ieg = g.ie + 4; g.jsg = -3; g.jeg = g.je + 4; g.ksg = -3; g.keg = g.ke + 4;
+    g.P = ((g.Ni + 1 + 4) + 7) / 8 * 8;
+    g.PP = (long)g.P * (g.Nj + 1);
+    // i=isg lands at offset 4 of its row, so i=is (=isg+4) is 64-byte aligned
+    g.c0 = 4 - g.isg - (long)g.P * g.jsg - g.PP * g.ksg;
+    long n3 = g.PP * (g.Nk + 1);
+    g.N3 = (n3 + 63) / 64 * 64;
+    const size_t pl = (size_t)g.N3 * sizeof(double);
+
+    int rc = GAM_OK;
+    hipError_t e = hipSuccess;
+#define ALLOC(ptr, nplanes)                                                   \
+    if (e == hipSuccess) {                                                    \
+        e = hipMalloc((void **)&(ptr), (size_t)(nplanes) * pl);              \
+        if (e == hipSuccess) e = hipMemset((ptr), 0, (size_t)(nplanes) * pl); \
+    }
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    ALLOC(c->st[0], S_NPL);
+    ALLOC(c->st[1], S_NPL);
+    ALLOC(c->H, H_NPL);
+    ALLOC(c->met, M_NPL);
+    ALLOC(c->bg, B_NPL + 9);
+    ALLOC(c->gflx, 15);
+    ALLOC(c->mflx, 9);
+    ALLOC(c->vf, 3);
+    ALLOC(c->stage, 1);
+    ALLOC(c->part, 1);
+#undef ALLOC
+    if (e == hipSuccess) e = hipMalloc((void **)&c->clk, sizeof(Clock));
+    if (e == hipSuccess) {
+        Clock ck;
+        memset(&ck, 0, sizeof(ck));
+        double h = GAM_HUGE;
+        memcpy(&ck.dtBits, &h, sizeof(double));
+        e = hipMemcpy(c->clk, &ck, sizeof(Clock), hipMemcpyHostToDevice);
+    }
+    for (int q = 0; e == hipSuccess && q < (int)(sizeof(c->ev) / sizeof(c->ev[0])); q++)
+        e = hipEventCreate(&c->ev[q]);
+    if (e != hipSuccess) {
+        rc = fail(GAM_E_HIP, "gam_create: %s", hipGetErrorString(e));
+        gam_destroy(c);
+        return rc;
+    }
+    // Grid defaults (face=1, edge=1, Teb=1) until metrics arrive
+    c->cur = 0;
+    *out = c;
+    return GAM_OK;
+}
+
+extern "C" int gam_destroy(gam_ctx *c)
+{
+    if (!c) return GAM_OK;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    double *bufs[] = {c->st[0], c->st[1], c->H, c->met, c->bg, c->gflx, c->mflx, c->vf, c->stage, c->part, c->wind};
+    for (double *b : bufs) if (b) hipFree(b);
+    if (c->clk) hipFree(c->clk);
+    for (int q = 0; q < (int)(sizeof(c->ev) / sizeof(c->ev[0])); q++) if (c->ev[q]) hipEventDestroy(c->ev[q]);
+    if (c->stream) hipStreamDestroy(c->stream);
+    c->pend.~vector<int>();
+    free(c);
+    return GAM_OK;
+}
+
+// ------------------------------------------------------------ transfers
+// Upload ncomp consecutive Fortran planes of (ni,nj,nk) into device planes.
+static int upload(gam_ctx *c, const double *src, int face, int ncomp, double *dst)
+{
+    const Geo &g = c->g;
+    const int ni = g.Ni + face, nj = g.Nj + face, nk = g.Nk + face;
+    const size_t n = (size_t)ni * nj * nk;
+    for (int q = 0; q < ncomp; q++) {
+        HIPCHK(hipMemcpyAsync(c->stage, src + q * n, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        launch_repack(c->stream, 1, g, ni, nj, nk, c->stage, dst + (size_t)q * g.N3);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return GAM_OK;
+}
+
+static int download(gam_ctx *c, const double *src, int face, int ncomp, double *dst)
+{
+    const Geo &g = c->g;
+    const int ni = g.Ni + face, nj = g.Nj + face, nk = g.Nk + face;
+    const size_t n = (size_t)ni * nj * nk;
+    for (int q = 0; q < ncomp; q++) {
+        launch_repack(c->stream, 0, g, ni, nj, nk, src + (size_t)q * g.N3, c->stage);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(dst + q * n, c->stage, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    return GAM_OK;
+}
+
+extern "C" int gam_set_grid(gam_ctx *c, const gam_grid_arrays *a)
+{
+    if (!c || !a) return fail(GAM_E_ARG, "gam_set_grid: NULL argument");
+    if (!a->xyzcc || !a->xfc || !a->di || !a->dj || !a->dk || !a->volume || !a->Tf || !a->Te ||
+        !a->Teb || !a->face || !a->edge)
+        return fail(GAM_E_ARG, "gam_set_grid: every metric array is required");
+    const long N3 = c->g.N3;
+    int rc;
+    if ((rc = upload(c, a->xyzcc, 0, 3, c->met + M_XYZCC * N3))) return rc;
+    if ((rc = upload(c, a->xfc, 1, 9, c->met + M_XFC * N3))) return rc;
+    if ((rc = upload(c, a->di, 0, 1, c->met + M_DI * N3))) return rc;
+    if ((rc = upload(c, a->dj, 0, 1, c->met + M_DJ * N3))) return rc;
+    if ((rc = upload(c, a->dk, 0, 1, c->met + M_DK * N3))) return rc;
+    if ((rc = upload(c, a->volume, 0, 1, c->met + M_VOL * N3))) return rc;
+    if ((rc = upload(c, a->Tf, 1, 27, c->met + M_TF * N3))) return rc;
+    if ((rc = upload(c, a->Te, 1, 27, c->met + M_TE * N3))) return rc;
+    if ((rc = upload(c, a->Teb, 1, 12, c->met + M_TEB * N3))) return rc;
+    if ((rc = upload(c, a->face, 1, 3, c->met + M_FACE * N3))) return rc;
+    if ((rc = upload(c, a->edge, 1, 3, c->met + M_EDGE * N3))) return rc;
+    c->haveGrid = 1;
+    return GAM_OK;
+}
+
+extern "C" int gam_grid_from_corners(gam_ctx *c, const double *xyz)
+{
+    if (!c || !xyz) return fail(GAM_E_ARG, "gam_grid_from_corners: NULL argument");
+    int rc = upload(c, xyz, 1, 3, c->met + (long)M_XYZ * c->g.N3);
+    if (rc) return rc;
+    launch_grid_from_corners(c->stream, c->g, c->met);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->haveGrid = 1;
+    return GAM_OK;
+}
+
+extern "C" int gam_get_grid(gam_ctx *c, const gam_grid_arrays *a)
+{
+    if (!c || !a) return fail(GAM_E_ARG, "gam_get_grid: NULL argument");
+    const long N3 = c->g.N3;
+    int rc = GAM_OK;
+    if (a->xyzcc && (rc = download(c, c->met + M_XYZCC * N3, 0, 3, a->xyzcc))) return rc;
+    if (a->xfc && (rc = download(c, c->met + M_XFC * N3, 1, 9, a->xfc))) return rc;
+    if (a->di && (rc = download(c, c->met + M_DI * N3, 0, 1, a->di))) return rc;
+    if (a->dj && (rc = download(c, c->met + M_DJ * N3, 0, 1, a->dj))) return rc;
+    if (a->dk && (rc = download(c, c->met + M_DK * N3, 0, 1, a->dk))) return rc;
+    if (a->volume && (rc = download(c, c->met + M_VOL * N3, 0, 1, a->volume))) return rc;
+    if (a->Tf && (rc = download(c, c->met + M_TF * N3, 1, 27, a->Tf))) return rc;
+    if (a->Te && (rc = download(c, c->met + M_TE * N3, 1, 27, a->Te))) return rc;
+    if (a->Teb && (rc = download(c, c->met + M_TEB * N3, 1, 12, a->Teb))) return rc;
+    if (a->face && (rc = download(c, c->met + M_FACE * N3, 1, 3, a->face))) return rc;
+    if (a->edge && (rc = download(c, c->met + M_EDGE * N3, 1, 3, a->edge))) return rc;
+    return GAM_OK;
+}
+
+extern "C" int gam_set_background(gam_ctx *c, const gam_background_arrays *b)
+{
+    if (!c || !b) return fail(GAM_E_ARG, "gam_set_background: NULL argument");
+    if (!b->fcB0 || !b->edgB0 || !b->B0 || !b->bFlux0 || !b->dpB0)
+        return fail(GAM_E_ARG, "gam_set_background: every background array is required");
+    const long N3 = c->g.N3;
+    int rc;
+    if ((rc = upload(c, b->fcB0, 0, 9, c->bg + B_FCB0 * N3))) return rc;
+    if ((rc = upload(c, b->edgB0, 0, 6, c->bg + B_EDGB0 * N3))) return rc;
+    if ((rc = upload(c, b->B0, 0, 3, c->bg + B_B0 * N3))) return rc;
+    if ((rc = upload(c, b->bFlux0, 0, 3, c->bg + B_BFLUX0 * N3))) return rc;
+    if ((rc = upload(c, b->dpB0, 0, 3, c->bg + B_DPB0 * N3))) return rc;
+    c->haveBG = 1;
+    return GAM_OK;
+}
+
+extern "C" int gam_add_dipole_b0(gam_ctx *c, double M0)
+{
+    if (!c) return fail(GAM_E_ARG, "gam_add_dipole_b0: NULL context");
+    if (!c->haveGrid) return fail(GAM_E_STATE, "gam_add_dipole_b0: grid metrics must be set first");
+    launch_dipole_b0(c->stream, c->g, M0, c->met, c->bg, c->bg + (long)B_NPL * c->g.N3);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->haveBG = 1;
+    return GAM_OK;
+}
+
+extern "C" int gam_get_background(gam_ctx *c, const gam_background_arrays *b)
+{
+    if (!c || !b) return fail(GAM_E_ARG, "gam_get_background: NULL argument");
+    const long N3 = c->g.N3;
+    int rc = GAM_OK;
+    if (b->fcB0 && (rc = download(c, c->bg + B_FCB0 * N3, 0, 9, b->fcB0))) return rc;
+    if (b->edgB0 && (rc = download(c, c->bg + B_EDGB0 * N3, 0, 6, b->edgB0))) return rc;
+    if (b->B0 && (rc = download(c, c->bg + B_B0 * N3, 0, 3, b->B0))) return rc;
+    if (b->bFlux0 && (rc = download(c, c->bg + B_BFLUX0 * N3, 0, 3, b->bFlux0))) return rc;
+    if (b->dpB0 && (rc = download(c, c->bg + B_DPB0 * N3, 0, 3, b->dpB0))) return rc;
+    return GAM_OK;
+}
+
+extern "C" int gam_set_wind(gam_ctx *c, int n, const double *tW, const double *Q, const double *B,
+                            double Bx0, double ByC, double BzC, const double *xyzW)
+{
+    if (!c || !tW || !Q || !B) return fail(GAM_E_ARG, "gam_set_wind: NULL argument");
+    if (n < 2) return fail(GAM_E_ARG, "gam_set_wind: need at least 2 samples (got %d)", n);
+    // tMin/tMax/dtW as readWind (wind.F90:685-699)
+    double tMin = tW[0], tMax = tW[0], dtMin = 1e300, dtMax = -1e300;
+    for (int q = 0; q < n; q++) { tMin = fmin(tMin, tW[q]); tMax = fmax(tMax, tW[q]); }
+    for (int q = 1; q < n; q++) { dtMin = fmin(dtMin, tW[q] - tW[q - 1]); dtMax = fmax(dtMax, tW[q] - tW[q - 1]); }
+    if (!(dtMin > 0.0)) return fail(GAM_E_ARG, "gam_set_wind: times must increase");
+    if (fabs(dtMin - dtMax) > 1.0e-6 * sqrt(dtMin * dtMin + dtMax * dtMax))
+        return fail(GAM_E_ARG, "gam_set_wind: wind time series must be uniformly spaced");
+    if (c->wind) { hipFree(c->wind); c->wind = nullptr; }
+    HIPCHK(hipMalloc((void **)&c->wind, sizeof(double) * (size_t)n * 9));
+    std::vector<double> h((size_t)n * 9);
+    memcpy(h.data(), tW, sizeof(double) * n);
+    memcpy(h.data() + n, Q, sizeof(double) * n * 5);
+    memcpy(h.data() + 6 * n, B, sizeof(double) * n * 3);
+    HIPCHK(hipMemcpy(c->wind, h.data(), sizeof(double) * n * 9, hipMemcpyHostToDevice));
+    WindDev &w = c->wd;
+    w.tW = c->wind; w.Q = c->wind + n; w.B = c->wind + 6 * n; w.n = n;
+    w.tMin = tMin; w.tMax = tMax; w.dtW = dtMin;
+    w.Bx0 = Bx0; w.ByC = ByC; w.BzC = BzC;
+    w.xW = xyzW ? xyzW[0] : 0.0; w.yW = xyzW ? xyzW[1] : 0.0; w.zW = xyzW ? xyzW[2] : 0.0;
+    c->haveWind = 1;
+    return GAM_OK;
+}
+
+static double *state_ptr(gam_ctx *c, int which)
+{
+    if (which == GAM_STATE) return S_cur(c);
+    if (which == GAM_OSTATE) return S_old(c);
+    return nullptr;
+}
+
+extern "C" int gam_put_state(gam_ctx *c, int which, const double *Gas, const double *magFlux,
+                             const double *Bxyz, const double *Efld, double time)
+{
+    if (!c) return fail(GAM_E_ARG, "gam_put_state: NULL context");
+    double *S = state_ptr(c, which);
+    if (!S) return fail(GAM_E_ARG, "gam_put_state: which must be GAM_STATE or GAM_OSTATE");
+    const long N3 = c->g.N3;
+    int rc;
+    if (Gas && (rc = upload(c, Gas, 0, 5, S + S_GAS * N3))) return rc;
+    if (magFlux && (rc = upload(c, magFlux, 1, 3, S + S_MF * N3))) return rc;
+    if (Bxyz && (rc = upload(c, Bxyz, 0, 3, S + S_B * N3))) return rc;
+    if (Efld) {
+        if ((rc = upload(c, Efld, 0, 3, S + S_E * N3))) return rc;
+        Box eb[3];
+        edge_boxes(c->g, eb);
+        launch_mask_e(c->stream, c->g, cell_box(c->g), eb, S + S_E * N3);
+        HIPCHK(hipGetLastError());
+    }
+    // State%time / oState%time
+    const size_t off = (which == GAM_STATE) ? offsetof(Clock, st) : offsetof(Clock, ost);
+    HIPCHK(hipMemcpyAsync((char *)c->clk + off, &time, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return GAM_OK;
+}
+
+extern "C" int gam_get_state(gam_ctx *c, int which, double *Gas, double *magFlux, double *Bxyz,
+                             double *Efld, double *time)
+{
+    if (!c) return fail(GAM_E_ARG, "gam_get_state: NULL context");
+    const long N3 = c->g.N3;
+    double *S = state_ptr(c, which);
+    int rc;
+    if (which == GAM_HFSTATE) {
+        if (Gas && (rc = download(c, c->H + H_GAS * N3, 0, 5, Gas))) return rc;
+        if (magFlux && (rc = download(c, c->H + H_MF * N3, 1, 3, magFlux))) return rc;
+        if (Bxyz && (rc = download(c, c->H + H_B * N3, 0, 3, Bxyz))) return rc;
+        if (Efld) memset(Efld, 0, sizeof(double) * (size_t)c->g.Ni * c->g.Nj * c->g.Nk * 3);
+        if (time) *time = 0.0;
+        return GAM_OK;
+    }
+    if (!S) return fail(GAM_E_ARG, "gam_get_state: bad selector %d", which);
+    if (Gas && (rc = download(c, S + S_GAS * N3, 0, 5, Gas))) return rc;
+    if (magFlux && (rc = download(c, S + S_MF * N3, 1, 3, magFlux))) return rc;
+    if (Bxyz && (rc = download(c, S + S_B * N3, 0, 3, Bxyz))) return rc;
+    if (Efld && (rc = download(c, S + S_E * N3, 0, 3, Efld))) return rc;
+    if (time) {
+        Clock ck;
+        HIPCHK(hipMemcpy(&ck, c->clk, sizeof(Clock), hipMemcpyDeviceToHost));
+        *time = (which == GAM_STATE) ? ck.st : ck.ost;
+    }
+    return GAM_OK;
+}
+
+extern "C" int gam_get_solver(gam_ctx *c, int which, double *out)
+{
+    if (!c || !out) return fail(GAM_E_ARG, "gam_get_solver: NULL argument");
+    if (which == 0) return download(c, c->gflx, 0, 15, out);
+    if (which == 1) return download(c, c->mflx, 0, 9, out);
+    return fail(GAM_E_ARG, "gam_get_solver: which must be 0 (gFlx) or 1 (mFlx)");
+}
+
+extern "C" int gam_set_time(gam_ctx *c, double t, double dt, int ts)
+{
+    if (!c) return fail(GAM_E_ARG, "gam_set_time: NULL context");
+    Clock ck;
+    HIPCHK(hipMemcpy(&ck, c->clk, sizeof(Clock), hipMemcpyDeviceToHost));
+    ck.t = t; ck.dt = dt; ck.dtA = dt; ck.ts = ts;
+    HIPCHK(hipMemcpy(c->clk, &ck, sizeof(Clock), hipMemcpyHostToDevice));
+    return GAM_OK;
+}
+
+extern "C" int gam_get_time(gam_ctx *c, double *t, double *dt, int *ts)
+{
+    if (!c) return fail(GAM_E_ARG, "gam_get_time: NULL context");
+    Clock ck;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(&ck, c->clk, sizeof(Clock), hipMemcpyDeviceToHost));
+    if (t) *t = ck.t;
+    if (dt) *dt = ck.dt;
+    if (ts) *ts = ck.ts;
+    return GAM_OK;
+}
+
+// ------------------------------------------------------------ step pieces
+static int check_ready(gam_ctx *c, const char *who)
+{
+    if (!c) return fail(GAM_E_ARG, "%s: NULL context", who);
+    if (!c->haveGrid) return fail(GAM_E_STATE, "%s: grid metrics not set", who);
+    if (c->mp.doBG && !c->haveBG) return fail(GAM_E_STATE, "%s: doBackground but no background field", who);
+    for (int s = 0; s < 6; s++)
+        if (c->model.bc[s] == GAM_BC_WIND && !c->haveWind) return fail(GAM_E_STATE, "%s: wind BC without a wind series", who);
+    return GAM_OK;
+}
+
+static void enq_bflux2fld(gam_ctx *c, double *S)
+{
+    const long N3 = c->g.N3;
+    launch_bflux2fld(c->stream, c->g, cell_box(c->g), S + S_MF * N3, S + S_B * N3, c->met);
+}
+
+static void enq_bcs(gam_ctx *c, double *S)
+{
+    for (int side = 0; side < 6; side++)
+        launch_bc(c->stream, side, c->model.bc[side], c->g, c->mp, c->clk, &c->wd, S, c->met);
+}
+
+// AdvanceMHD on the device (mhdgroup.F90:16-76); swaps State/oState.
+static void enq_advance(gam_ctx *c, int finmode)
+{
+    const Geo &g = c->g;
+    const long N3 = g.N3;
+    hipStream_t s = c->stream;
+    double *S = S_cur(c), *O = S_old(c);
+    double *Sn = O;   // becomes State
+
+    prof_begin(c, KG_PREDICT);
+    launch_predict(s, g, c->mp, face_box(g), c->clk, S, O, c->met, c->H);
+    prof_end(c, KG_PREDICT);
+
+    Box eb[3];
+    edge_boxes(g, eb);
+    if (c->mp.doMHD) {
+        for (int eD = 0; eD < 3; eD++) {
+            // Mom2Face ranges (fields.F90:119-175)
+            Box mb = cell_box(g);
+            int dT1;
+            if (eD == 0) { dT1 = 1; mb.j0 = g.js; mb.j1 = g.je + 1; }
+            else if (eD == 1) { dT1 = 2; mb.k0 = g.ks; mb.k1 = g.ke + 1; }
+            else { dT1 = 0; mb.i0 = g.is; mb.i1 = g.ie + 1; }
+            prof_begin(c, KG_MOM2FACE);
+            launch_mom2face(s, dT1, g, c->mp, mb, c->H, c->met, c->vf);
+            prof_end(c, KG_MOM2FACE);
+            prof_begin(c, KG_EMF);
+            launch_emf(s, eD, g, c->mp, eb[eD], c->clk, c->H, S, c->vf, c->met, c->bg, Sn + S_E * N3);
+            prof_end(c, KG_EMF);
+        }
+    }
+    prof_begin(c, KG_FLUX);
+    for (int d = 0; d < 3; d++) {
+        Box fb = act_box(g);
+        if (d == 0) fb.i1 += 1; else if (d == 1) fb.j1 += 1; else fb.k1 += 1;
+        launch_flux(s, d, g, c->mp, fb, c->H, S, c->met, c->bg, c->gflx, c->mflx);
+    }
+    prof_end(c, KG_FLUX);
+
+    prof_begin(c, KG_CT);
+    Box ctb = mkbox(g.is, g.ie + 1, g.js, g.je + 1, g.ks, g.ke + 1);
+    launch_ct_b(s, g, c->mp, face_box(g), ctb, eb, c->clk, S, Sn, c->met);
+    prof_end(c, KG_CT);
+
+    prof_begin(c, KG_GAS);
+    launch_gas(s, g, c->mp, cell_box(g), act_box(g), dt_box(g), c->clk, S, Sn, c->gflx, c->mflx, c->met, c->bg);
+    prof_end(c, KG_GAS);
+
+    launch_finalize(s, c->mp, c->clk, finmode);
+    c->cur = 1 - c->cur;
+}
+
+static void enq_step(gam_ctx *c)
+{
+    enq_advance(c, FIN_ADVANCE | FIN_MODEL | FIN_CALCDT | FIN_SETDT);
+    double *S = S_cur(c);
+    prof_begin(c, KG_BC);
+    enq_bcs(c, S);
+    prof_end(c, KG_BC);
+    prof_begin(c, KG_BFLD);
+    enq_bflux2fld(c, S);
+    prof_end(c, KG_BFLD);
+}
+
+static int read_clock(gam_ctx *c, Clock *ck)
+{
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(ck, c->clk, sizeof(Clock), hipMemcpyDeviceToHost));
+    return GAM_OK;
+}
+
+static int status_code(int st)
+{
+    if (st >= 2) return fail(GAM_E_DT_DISASTER, "<Timestep too small or no timestep below HUGE>");
+    if (st == 1) return fail(GAM_E_DT_BAD, "<Drop in timestep by 10x or below limDT0>");
+    return GAM_OK;
+}
+
+extern "C" int gam_init_run(gam_ctx *c, double *dt)
+{
+    int rc = check_ready(c, "gam_init_run");
+    if (rc) return rc;
+    double *S = S_cur(c), *O = S_old(c);
+    const size_t bytes = (size_t)S_NPL * c->g.N3 * sizeof(double);
+    if (c->mp.doMHD) enq_bflux2fld(c, S);
+    enq_bcs(c, S);
+    HIPCHK(hipMemcpyAsync(O, S, bytes, hipMemcpyDeviceToDevice, c->stream));
+    launch_calc_dt(c->stream, c->g, c->mp, dt_box(c->g), c->clk, S, c->met, c->bg);
+    // oState = State, then oState%time = State%time - dt (init.F90:228-244)
+    Clock ck;
+    launch_finalize(c->stream, c->mp, c->clk, FIN_CALCDT | FIN_SETDT | FIN_INIT);
+    HIPCHK(hipGetLastError());
+    if ((rc = read_clock(c, &ck))) return rc;
+    if (dt) *dt = ck.dt;
+    return status_code(ck.lastStatus);
+}
+
+extern "C" int gam_advance_mhd(gam_ctx *c, double dt)
+{
+    int rc = check_ready(c, "gam_advance_mhd");
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync((char *)c->clk + offsetof(Clock, dtA), &dt, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    enq_advance(c, FIN_ADVANCE);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return prof_flush(c);
+}
+
+extern "C" int gam_calc_dt(gam_ctx *c, double *dt)
+{
+    int rc = check_ready(c, "gam_calc_dt");
+    if (rc) return rc;
+    launch_calc_dt(c->stream, c->g, c->mp, dt_box(c->g), c->clk, S_cur(c), c->met, c->bg);
+    launch_finalize(c->stream, c->mp, c->clk, FIN_CALCDT);
+    HIPCHK(hipGetLastError());
+    Clock ck;
+    if ((rc = read_clock(c, &ck))) return rc;
+    if (dt) *dt = ck.dtRet;
+    return status_code(ck.lastStatus);
+}
+
+extern "C" int gam_enforce_bcs(gam_ctx *c)
+{
+    int rc = check_ready(c, "gam_enforce_bcs");
+    if (rc) return rc;
+    enq_bcs(c, S_cur(c));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return GAM_OK;
+}
+
+extern "C" int gam_bflux2fld(gam_ctx *c, int which)
+{
+    int rc = check_ready(c, "gam_bflux2fld");
+    if (rc) return rc;
+    double *S = state_ptr(c, which);
+    if (!S) return fail(GAM_E_ARG, "gam_bflux2fld: which must be GAM_STATE or GAM_OSTATE");
+    enq_bflux2fld(c, S);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return GAM_OK;
+}
+
+extern "C" int gam_step(gam_ctx *c, double *dt_next)
+{
+    int rc = check_ready(c, "gam_step");
+    if (rc) return rc;
+    enq_step(c);
+    HIPCHK(hipGetLastError());
+    if (!dt_next) return GAM_OK;
+    Clock ck;
+    if ((rc = read_clock(c, &ck))) return rc;
+    if ((rc = prof_flush(c))) return rc;
+    *dt_next = ck.dt;
+    return status_code(ck.lastStatus);
+}
+
+extern "C" int gam_run(gam_ctx *c, int nsteps)
+{
+    int rc = check_ready(c, "gam_run");
+    if (rc) return rc;
+    if (nsteps < 0) return fail(GAM_E_ARG, "gam_run: nsteps < 0");
+    for (int n = 0; n < nsteps; n++) {
+        enq_step(c);
+    }
+    HIPCHK(hipGetLastError());
+    Clock ck;
+    if ((rc = read_clock(c, &ck))) return rc;
+    if ((rc = prof_flush(c))) return rc;
+    return status_code(ck.status);
+}
+
+extern "C" int gam_sync(gam_ctx *c)
+{
+    if (!c) return fail(GAM_E_ARG, "gam_sync: NULL context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return prof_flush(c);
+}
+
+extern "C" int gam_divb(gam_ctx *c, double *tot)
+{
+    int rc = check_ready(c, "gam_divb");
+    if (rc) return rc;
+    if (!tot) return fail(GAM_E_ARG, "gam_divb: NULL output");
+    int np = 0;
+    launch_divb(c->stream, c->g, act_box(c->g), S_cur(c) + S_MF * c->g.N3, c->bg, c->mp.doBG, c->part, &np);
+    HIPCHK(hipGetLastError());
+    std::vector<double> h(np);
+    HIPCHK(hipMemcpyAsync(h.data(), c->part, sizeof(double) * np, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double s = 0.0;
+    for (int q = 0; q < np; q++) s += h[q];
+    *tot = s;
+    return GAM_OK;
+}
+
+extern "C" int gam_time_steps(gam_ctx *c, int nsteps, double *ms)
+{
+    int rc = check_ready(c, "gam_time_steps");
+    if (rc) return rc;
+    if (nsteps < 1 || !ms) return fail(GAM_E_ARG, "gam_time_steps: need nsteps >= 1 and an output");
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipEventRecord(a, c->stream));
+    for (int n = 0; n < nsteps; n++) enq_step(c);
+    HIPCHK(hipEventRecord(b, c->stream));
+    HIPCHK(hipEventSynchronize(b));
+    float f = 0.0f;
+    HIPCHK(hipEventElapsedTime(&f, a, b));
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    *ms = f;
+    Clock ck;
+    if ((rc = read_clock(c, &ck))) return rc;
+    if ((rc = prof_flush(c))) return rc;
+    return status_code(ck.status);
+}
+
+extern "C" int gam_set_profiling(gam_ctx *c, int on)
+{
+    if (!c) return fail(GAM_E_ARG, "gam_set_profiling: NULL context");
+    int rc = gam_sync(c);
+    if (rc) return rc;
+    c->prof = on ? 1 : 0;
+    for (int q = 0; q < GAM_KG_COUNT; q++) { c->kg_ms[q] = 0.0; c->kg_n[q] = 0; }
+    c->nev = 0;
+    c->pend.clear();
+    return GAM_OK;
+}
+
+extern "C" int gam_kernel_times(gam_ctx *c, double *ms, int *counts)
+{
+    if (!c) return fail(GAM_E_ARG, "gam_kernel_times: NULL context");
+    int rc = gam_sync(c);
+    if (rc) return rc;
+    for (int q = 0; q < GAM_KG_COUNT; q++) {
+        if (ms) ms[q] = c->kg_ms[q];
+        if (counts) counts[q] = c->kg_n[q];
+    }
+    return GAM_OK;
+}
+
+extern "C" const char *gam_kernel_group_name(int grp)
+{
+    if (grp < 0 || grp >= GAM_KG_COUNT) return "";
+    return kg_names[grp];
+}

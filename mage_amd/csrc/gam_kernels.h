@@ -1,0 +1,53 @@
+// gam_kernels.h -- host-side launchers of the gfx950 kernels.
+#pragma once
+#include "gam_common.h"
+
+// k_finalize modes
+enum {
+    FIN_ADVANCE = 1,   // State/oState times after applyUpdate
+    FIN_MODEL = 2,     // Model%ts += 1, Model%t += Model%dt
+    FIN_CALCDT = 4,    // evaluate CalcDT from the reduction
+    FIN_SETDT = 8,     // Model%dt = CalcDT result
+    FIN_INIT = 16      // oState%time = State%time - dt (init.F90:238-244)
+};
+
+struct WindDev {
+    const double *tW, *Q, *B;  // n, (n,5), (n,3) column-major
+    int n;
+    double tMin, tMax, dtW;
+    double Bx0, ByC, BzC;
+    double xW, yW, zW;
+};
+
+void launch_predict(hipStream_t s, const Geo &g, const MParm &m, const Box &fb, const Clock *clk,
+                    const double *S, const double *O, const double *met, double *H);
+void launch_mom2face(hipStream_t s, int dT, const Geo &g, const MParm &m, const Box &b,
+                     const double *H, const double *met, double *Vf);
+void launch_emf(hipStream_t s, int eD, const Geo &g, const MParm &m, const Box &b, const Clock *clk,
+                const double *H, const double *S, const double *Vf, const double *met,
+                const double *bg, double *E);
+void launch_flux(hipStream_t s, int d, const Geo &g, const MParm &m, const Box &b, const double *H,
+                 const double *S, const double *met, const double *bg, double *gf, double *mf);
+void launch_ct_b(hipStream_t s, const Geo &g, const MParm &m, const Box &fb, const Box &ctb,
+                 const Box *eboxes, const Clock *clk, const double *So, double *Sn,
+                 const double *met);
+void launch_gas(hipStream_t s, const Geo &g, const MParm &m, const Box &cb, const Box &ab,
+                const Box &dtb, Clock *clk, const double *So, double *Sn, const double *gf,
+                const double *mf, const double *met, const double *bg);
+void launch_calc_dt(hipStream_t s, const Geo &g, const MParm &m, const Box &dtb, Clock *clk,
+                    const double *S, const double *met, const double *bg);
+void launch_finalize(hipStream_t s, const MParm &m, Clock *clk, int mode);
+void launch_bflux2fld(hipStream_t s, const Geo &g, const Box &cb, const double *mf, double *B,
+                      const double *met);
+void launch_bc(hipStream_t s, int side, int kind, const Geo &g, const MParm &m, const Clock *clk,
+               const WindDev *w, double *S, const double *met);
+void launch_divb(hipStream_t s, const Geo &g, const Box &ab, const double *mf, const double *bg,
+                 int useBG, double *part, int *nparts);
+void launch_repack(hipStream_t s, int dir_in, const Geo &g, int ni, int nj, int nk,
+                   const double *src, double *dst);
+void launch_mask_e(hipStream_t s, const Geo &g, const Box &all, const Box *eboxes, double *E);
+
+// Grid pre-compute (gam_grid.hip)
+void launch_grid_from_corners(hipStream_t s, const Geo &g, double *met);
+void launch_dipole_b0(hipStream_t s, const Geo &g, double M0, const double *met, double *bg,
+                      double *scratch9);

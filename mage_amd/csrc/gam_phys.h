@@ -1,0 +1,200 @@
+// gam_phys.h -- per-point FP64 physics of the GAMERA step, written for
+// device code.  Each function states the reference routine it reproduces;
+// operation order follows the reference so that results agree with it to
+// rounding.
+#pragma once
+#include "gam_common.h"
+
+#define GAM_PI 3.141592653589793115997963468544185161590576171875
+
+enum { DEN = 0, MOMX = 1, MOMY = 2, MOMZ = 3, ENER = 4 };
+enum { PRS = 4 };
+
+// ---------------------------------------------------------- conversions
+// CellC2P (gamutils.F90:301-321): velocity from the unfloored density.
+__device__ __forceinline__ void c2p_cell(const MParm &m, const double *U, double *W)
+{
+    double rho = U[DEN];
+    double vx = U[MOMX] / rho, vy = U[MOMY] / rho, vz = U[MOMZ] / rho;
+    double ke = 0.5 * rho * (vx * vx + vy * vy + vz * vz);
+    W[DEN] = dmax(rho, m.dFloor);
+    W[1] = vx; W[2] = vy; W[3] = vz;
+    W[PRS] = dmax((m.gamma - 1) * (U[ENER] - ke), m.pFloor);
+}
+
+// CellP2C (gamutils.F90:362-383)
+__device__ __forceinline__ void p2c_cell(const MParm &m, const double *W, double *U)
+{
+    double rho = dmax(W[DEN], m.dFloor);
+    double vx = W[1], vy = W[2], vz = W[3];
+    double ie = dmax(W[PRS], m.pFloor) / (m.gamma - 1);
+    double ke = 0.5 * rho * (vx * vx + vy * vy + vz * vz);
+    U[DEN] = rho; U[MOMX] = rho * vx; U[MOMY] = rho * vy; U[MOMZ] = rho * vz;
+    U[ENER] = ie + ke;
+}
+
+// BlockCon2Prim (recon.F90:75-90): density floored before dividing.
+__device__ __forceinline__ void c2p_block(const MParm &m, const double *U, double *W)
+{
+    double d = dmax(U[DEN], m.dFloor);
+    double mx = U[MOMX], my = U[MOMY], mz = U[MOMZ];
+    double ke = 0.5 * (mx * mx + my * my + mz * mz) / d;
+    W[DEN] = d; W[1] = mx / d; W[2] = my / d; W[3] = mz / d;
+    W[PRS] = dmax((m.gamma - 1) * (U[ENER] - ke), m.pFloor);
+}
+
+// ---------------------------------------------------------- limiter
+// PDM (recon.F90:275-301)
+__device__ __forceinline__ double pdm_one(double b, double q0, double q1, double q2, double qI)
+{
+    double qN = dmax(dmin(q1, q2), dmin(qI, dmax(q1, q2)));
+    double dq0 = b * (q1 - q0), dq1 = b * (q2 - q1);
+    double s1 = copysign(1.0, dq1);
+    double s01 = fabs(copysign(1.0, dq0) + s1);
+    return qN - s1 * dmax(0.0, fabs(qN - q1) - s01 * fabs(dq0));
+}
+
+// pdmLR (recon.F90:231-272)
+__device__ __forceinline__ void pdm_pair(double b, double v0, double v1, double v2, double v3,
+                                         double qi, double &ql, double &qr)
+{
+    double vN = dmax(dmin(v1, v2), dmin(qi, dmax(v1, v2)));
+    double dv0 = b * (v1 - v0), dv1 = b * (v2 - v1), dv2 = b * (v3 - v2);
+    double s0 = copysign(1.0, dv0), s1 = copysign(1.0, dv1), s2 = copysign(1.0, dv2);
+    double a0 = fabs(s0 + s1), a1 = fabs(s1 + s2);
+    ql = vN - s1 * dmax(0.0, fabs(vN - v1) - a0 * fabs(dv0));
+    qr = vN + s1 * dmax(0.0, fabs(v2 - vN) - a1 * fabs(dv2));
+}
+
+// Volume-weighted reconstruction + limiting for one variable at one face:
+// Cen8LRs (recon.F90:125-158) or Up7LRs (162-198).  vol[8] the weights
+// (cells i-4..i+3), Vi their central-8 interpolant, q[8] the values,
+// nq[4] the limiter values (cells i-2..i+1).
+template <int RM>
+__device__ __forceinline__ void recon_lr(double pdmb, const double *vol, double Vi,
+                                         const double *nq, const double *q,
+                                         double &ql, double &qr)
+{
+    double qv[8];
+#pragma unroll
+    for (int n = 0; n < 8; n++) qv[n] = vol[n] * q[n];
+    if (RM == 1) {
+        double l = qv[0] * UP7(0) + qv[1] * UP7(1) + qv[2] * UP7(2) + qv[3] * UP7(3)
+                 + qv[4] * UP7(4) + qv[5] * UP7(5) + qv[6] * UP7(6);
+        double r = qv[7] * UP7(0) + qv[6] * UP7(1) + qv[5] * UP7(2) + qv[4] * UP7(3)
+                 + qv[3] * UP7(4) + qv[2] * UP7(5) + qv[1] * UP7(6);
+        ql = pdm_one(pdmb, nq[0], nq[1], nq[2], l / Vi);
+        qr = pdm_one(pdmb, nq[3], nq[2], nq[1], r / Vi);
+    } else {
+        double qi = wsum8([&](int n) { return qv[n]; }) / Vi;
+        pdm_pair(pdmb, nq[0], nq[1], nq[2], nq[3], qi, ql, qr);
+    }
+}
+
+// ---------------------------------------------------------- fluxes
+// One side (s = -1 left, +1 right) of the Lyon gas-kinetic flux,
+// GasKinFlux (stress.F90:432-511).  T = face triad (N, T1, T2).
+__device__ __forceinline__ void gas_kin(double gam, double s, const double *W, const double *T,
+                                        double *gF, double *dW)
+{
+    double d = W[DEN], p = W[PRS], vx = W[1], vy = W[2], vz = W[3];
+    double e = 0.5 * d * (vx * vx + vy * vy + vz * vz) + p / (gam - 1);
+    double lam = d / (2 * p);
+    double vn = T[0] * vx + T[1] * vy + T[2] * vz;
+    double vt1 = T[3] * vx + T[4] * vy + T[5] * vz;
+    double vt2 = T[6] * vx + T[7] * vy + T[8] * vz;
+    double vn0 = 0.5 * erfc(s * sqrt(lam) * vn);
+    double vn1 = vn * vn0 - 0.5 * s * exp(-(lam * vn * vn)) / sqrt(GAM_PI * lam);
+    gF[DEN] = gF[DEN] + d * vn1;
+    gF[ENER] = gF[ENER] + (e + 0.5 * p) * vn1 + 0.5 * p * vn0 * vn;
+    double fn = d * vn * vn1 + p * vn0;
+    double ft1 = d * vt1 * vn1, ft2 = d * vt2 * vn1;
+    gF[MOMX] = gF[MOMX] + (T[0] * fn + T[3] * ft1 + T[6] * ft2);
+    gF[MOMY] = gF[MOMY] + (T[1] * fn + T[4] * ft1 + T[7] * ft2);
+    gF[MOMZ] = gF[MOMZ] + (T[2] * fn + T[5] * ft1 + T[8] * ft2);
+    dW[DEN] = dW[DEN] + s * d;
+    dW[MOMX] = dW[MOMX] + s * d * vx;
+    dW[MOMY] = dW[MOMY] + s * d * vy;
+    dW[MOMZ] = dW[MOMZ] + s * d * vz;
+    dW[ENER] = dW[ENER] + s * e;
+}
+
+// One side of the magnetic kinetic flux, MagKinFlux (stress.F90:513-599),
+// with the background-field cross terms (589-596).
+template <bool BG, bool BORIS>
+__device__ __forceinline__ void mag_kin(const MParm &m, double s, const double *W, const double *B,
+                                        double Bn, const double *T, const double *B0, double B0n,
+                                        double *mF, double &bbD, double &VaD)
+{
+    double d = W[DEN], p = W[PRS];
+    double bx = B[0], by = B[1], bz = B[2];
+    double dPb = 0.5 * (bx * bx + by * by + bz * bz);
+    double va2;
+    if (BG) {
+        double tx = bx + B0[0], ty = by + B0[1], tz = bz + B0[2];
+        double tt = tx * tx + ty * ty + tz * tz;
+        va2 = tt / d;
+        bbD = bbD + 0.5 * tt;
+    } else {
+        va2 = 2 * dPb / d;
+        bbD = bbD + 0.5 * (bx * bx + by * by + bz * bz);
+    }
+    double vac2 = BORIS ? va2 / (1 + (va2 / (m.Ca * m.Ca))) : va2;
+    double nx = T[0], ny = T[1], nz = T[2];
+    double vn = nx * W[1] + ny * W[2] + nz * W[3];
+    double lam = 1 / (2 * p / d + vac2);
+    double vn0 = 0.5 * erfc(s * sqrt(lam) * vn);
+    VaD = VaD + 0.5 * sqrt(vac2);
+    mF[0] = mF[0] + vn0 * (dPb * nx - bx * Bn);
+    mF[1] = mF[1] + vn0 * (dPb * ny - by * Bn);
+    mF[2] = mF[2] + vn0 * (dPb * nz - bz * Bn);
+    if (BG) {
+        double bdb0 = B0[0] * bx + B0[1] * by + B0[2] * bz;
+        mF[0] = mF[0] + vn0 * (-(B0[0] * Bn) - bx * B0n + bdb0 * nx);
+        mF[1] = mF[1] + vn0 * (-(B0[1] * Bn) - by * B0n + bdb0 * ny);
+        mF[2] = mF[2] + vn0 * (-(B0[2] * Bn) - bz * B0n + bdb0 * nz);
+    }
+}
+
+// ---------------------------------------------------------- cell B
+// gamCellBxyz (gridutils.F90:648-680) given the six face fluxes.
+__device__ __forceinline__ void cell_b(const Geo &g, const double *met, long c,
+                                       double fI, double fJ, double fK,
+                                       double fIp, double fJp, double fKp, double *B)
+{
+    const long N3 = g.N3;
+    double dV = met[M_VOL * N3 + c];
+    double div = fIp - fI + fJp - fJ + fKp - fK;
+#pragma unroll
+    for (int x = 0; x < 3; x++) {
+        double xI = met[(M_XFC + x + 0) * N3 + c];
+        double xJ = met[(M_XFC + x + 3) * N3 + c];
+        double xK = met[(M_XFC + x + 6) * N3 + c];
+        double xIp = met[(M_XFC + x + 0) * N3 + c + 1];
+        double xJp = met[(M_XFC + x + 3) * N3 + c + g.P];
+        double xKp = met[(M_XFC + x + 6) * N3 + c + g.PP];
+        double b = (fIp * xIp + fJp * xJp + fKp * xKp - fI * xI - fJ * xJ - fK * xK) / dV;
+        B[x] = b - div * met[(M_XYZCC + x) * N3 + c] / dV;
+    }
+}
+
+// ---------------------------------------------------------- timestep
+// CellDT (gridutils.F90:49-106) on conserved U and total B.
+template <bool BORIS>
+__device__ __forceinline__ double cell_dt(const MParm &m, const double *U, double bx, double by,
+                                          double bz, double dl)
+{
+    double rho = U[DEN];
+    double vx = U[MOMX] / rho, vy = U[MOMY] / rho, vz = U[MOMZ] / rho;
+    double mv = sqrt(vx * vx + vy * vy + vz * vz);
+    double ke = 0.5 * rho * (mv * mv);
+    double p = (m.gamma - 1) * (U[ENER] - ke);
+    double cs = sqrt(m.gamma * p / rho);
+    double va = 0.0;
+    if (m.doMHD) {
+        va = sqrt(bx * bx + by * by + bz * bz) / sqrt(rho);
+        if (BORIS) va = m.Ca * va / sqrt(m.Ca * m.Ca + va * va);
+    }
+    double vcfl = mv + sqrt(cs * cs + va * va) + 0.0;
+    return m.CFL * dl / vcfl;
+}

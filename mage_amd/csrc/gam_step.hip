@@ -1,0 +1,1177 @@
+// gam_step.hip -- gfx950 kernels of one GAMERA MHD step (FP64).
+//
+// Kernel sequence of one stepGamera (gamapp.F90:73-113):
+//   k_predict    Predictor: AB2 face fluxes, half-step B and gas, plus the
+//                primitives used by the flux sweeps           (K1-K3)
+//   k_mom2face   face velocities along dT1, per EMF direction (K4-K6)
+//   k_emf        edge EMFs                                    (K7)
+//   k_flux       Lyon gas/magnetic fluxes, one per direction  (K8-K10)
+//   k_ct_b       CT face-flux update + cell B of the new state (K14,K15)
+//   k_gas        flux divergence, Reynolds, Maxwell/Boris, and the CFL
+//                min-reduction of the new state               (K11,K13,K16,K17)
+//   k_finalize   time bookkeeping and CalcDT checks (1 thread)
+//   k_bc_*       boundary conditions                          (K18)
+//   k_bflux2fld  cell B after the BCs                          (K19)
+// State and oState are two buffers that swap roles every step, which
+// replaces CopyState (mhdgroup.F90:204-255): the update reads the current
+// buffer and writes every point of the other.
+#include "gam_phys.h"
+#include "gam_kernels.h"
+
+#define BX 64
+#define BY 4
+
+__device__ __forceinline__ bool in_box(const Box &b, int i, int j, int k)
+{
+    return i >= b.i0 && i <= b.i1 && j >= b.j0 && j <= b.j1 && k >= b.k0 && k <= b.k1;
+}
+
+#define THREAD_IJK(b)                                   \
+    const int i = (b).i0 + blockIdx.x * BX + threadIdx.x; \
+    const int j = (b).j0 + blockIdx.y * BY + threadIdx.y; \
+    const int k = (b).k0 + blockIdx.z;                  \
+    if (i > (b).i1 || j > (b).j1) return;
+
+static inline dim3 grid_of(const Box &b)
+{
+    return dim3((unsigned)((b.i1 - b.i0 + BX) / BX), (unsigned)((b.j1 - b.j0 + BY) / BY),
+                (unsigned)(b.k1 - b.k0 + 1));
+}
+
+// AB2 extrapolation (ExtrapAB2, mhdgroup.F90:709-717)
+__device__ __forceinline__ double ab2(double q, double oq, double odt, double pdt)
+{
+    double dq = (q - oq) / odt;
+    return q + pdt * dq;
+}
+
+// ====================================================================
+// Predictor (mhdgroup.F90:444-561) fused with bFlux2Fld of the half state
+// (gridutils.F90:281-301) and the primitive conversions of the flux
+// stencils (BlockCon2Prim, recon.F90:75-90) for both the half state and
+// the limiter state n.
+// Domain: all face points isg..ieg+1 (x) jsg..jeg+1 (x) ksg..keg+1.
+// ====================================================================
+__global__ __launch_bounds__(256) void k_predict(Geo g, MParm m, Box fb, const Clock *clk,
+                                                 const double *__restrict__ S,
+                                                 const double *__restrict__ O,
+                                                 const double *__restrict__ met,
+                                                 double *__restrict__ H)
+{
+    THREAD_IJK(fb);
+    const long N3 = g.N3;
+    const long c = IX(g, i, j, k);
+    const double st = clk->st, dtM = clk->dt;
+    double ost = clk->ost;
+    if (fabs(ost - st) < GAM_TINY) ost = st - dtM;   // mhdgroup.F90:26-28
+    const double odt = st - ost;
+    const double pdt = 0.5 * clk->dtA;
+
+    double f[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        f[d] = m.doMHD ? ab2(S[(S_MF + d) * N3 + c], O[(S_MF + d) * N3 + c], odt, pdt)
+                       : H[(H_MF + d) * N3 + c];
+        H[(H_MF + d) * N3 + c] = f[d];
+    }
+    if (i > g.ieg || j > g.jeg || k > g.keg) return;
+
+    // half-state cell B from the six extrapolated face fluxes
+    double fIp, fJp, fKp;
+    if (m.doMHD) {
+        fIp = ab2(S[(S_MF + 0) * N3 + c + 1], O[(S_MF + 0) * N3 + c + 1], odt, pdt);
+        fJp = ab2(S[(S_MF + 1) * N3 + c + g.P], O[(S_MF + 1) * N3 + c + g.P], odt, pdt);
+        fKp = ab2(S[(S_MF + 2) * N3 + c + g.PP], O[(S_MF + 2) * N3 + c + g.PP], odt, pdt);
+    } else {
+        fIp = H[(H_MF + 0) * N3 + c + 1];
+        fJp = H[(H_MF + 1) * N3 + c + g.P];
+        fKp = H[(H_MF + 2) * N3 + c + g.PP];
+    }
+    double B[3];
+    cell_b(g, met, c, f[0], f[1], f[2], fIp, fJp, fKp, B);
+#pragma unroll
+    for (int x = 0; x < 3; x++) H[(H_B + x) * N3 + c] = B[x];
+
+    // PredictCell (mhdgroup.F90:528-561)
+    double Q[5], oQ[5], W[5], oW[5], pW[5], pQ[5], Wb[5];
+#pragma unroll
+    for (int v = 0; v < 5; v++) { Q[v] = S[(S_GAS + v) * N3 + c]; oQ[v] = O[(S_GAS + v) * N3 + c]; }
+    c2p_cell(m, oQ, oW);
+    c2p_cell(m, Q, W);
+#pragma unroll
+    for (int v = 0; v < 5; v++) pW[v] = ab2(W[v], oW[v], odt, pdt);
+    pW[DEN] = dmax(pW[DEN], m.dFloor);
+    pW[PRS] = dmax(pW[PRS], m.pFloor);
+    p2c_cell(m, pW, pQ);
+#pragma unroll
+    for (int v = 0; v < 5; v++) H[(H_GAS + v) * N3 + c] = pQ[v];
+    // primitives of the reconstruction (half state) and limiter (state n) stencils
+    c2p_block(m, pQ, Wb);
+#pragma unroll
+    for (int v = 0; v < 5; v++) H[(H_WH + v) * N3 + c] = Wb[v];
+    c2p_block(m, Q, Wb);
+#pragma unroll
+    for (int v = 0; v < 5; v++) H[(H_WN + v) * N3 + c] = Wb[v];
+}
+
+// ====================================================================
+// Mom2Face (fields.F90:235-285): volume-weighted 8-point interpolation of
+// rho*v/rho to the dT faces.
+// ====================================================================
+template <int DT>
+__global__ __launch_bounds__(256) void k_mom2face(Geo g, MParm m, Box b,
+                                                  const double *__restrict__ H,
+                                                  const double *__restrict__ met,
+                                                  double *__restrict__ Vf)
+{
+    THREAD_IJK(b);
+    const long N3 = g.N3;
+    const long c = IX(g, i, j, k);
+    const long s = STRIDE<DT>(g);
+    double vol[8], vel[3][8];
+#pragma unroll
+    for (int n = 0; n < 8; n++) {
+        const long q = c + (n - 4) * s;
+        double v = met[M_VOL * N3 + q];
+        double dn = dmax(H[(H_GAS + DEN) * N3 + q], m.dFloor);
+        vol[n] = v;
+#pragma unroll
+        for (int d = 0; d < 3; d++) vel[d][n] = v * H[(H_GAS + MOMX + d) * N3 + q] / dn;
+    }
+    double dV = dmax(wsum8([&](int n) { return vol[n]; }), GAM_TINY);
+#pragma unroll
+    for (int d = 0; d < 3; d++) Vf[d * N3 + c] = wsum8([&](int n) { return vel[d][n]; }) / dV;
+}
+
+// ====================================================================
+// Edge EMF: GetCornerV/GetCornerD/GetCornerB/EdgeLRs + VxB
+// (fields.F90:187-224, 288-517).  One thread per active edge of ED.
+// ====================================================================
+template <int FD, int SD, int RM>
+__device__ __forceinline__ void edge_lr(const Geo &g, const MParm &m, long c,
+                                        const double *__restrict__ met,
+                                        const double *__restrict__ mfH,
+                                        const double *__restrict__ mfL,
+                                        double &bL, double &bR, double &fA)
+{
+    // EdgeLRs (fields.F90:288-362) for FD faces stacked along SD, area weighted
+    const long N3 = g.N3;
+    const long s = STRIDE<SD>(g);
+    double A[8], q[8], nq[4];
+#pragma unroll
+    for (int n = 0; n < 8; n++) {
+        const long p = c + (n - 4) * s;
+        A[n] = met[(M_FACE + FD) * N3 + p];
+        q[n] = mfH[FD * N3 + p] / dmax(A[n], GAM_TINY);
+    }
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+        const long p = c + (n - 2) * s;
+        nq[n] = mfL[FD * N3 + p] / dmax(met[(M_FACE + FD) * N3 + p], GAM_TINY);
+    }
+    double Vi = wsum8([&](int n) { return A[n]; });
+    double l, r;
+    recon_lr<RM>(m.pdmb, A, Vi, nq, q, l, r);
+    fA = Vi;
+    bL = l * fA;
+    bR = r * fA;
+}
+
+template <int ED, int RM, bool BG, bool BORIS>
+__global__ __launch_bounds__(256) void k_emf(Geo g, MParm m, Box b, const Clock *clk,
+                                             const double *__restrict__ H,
+                                             const double *__restrict__ S,
+                                             const double *__restrict__ Vf,
+                                             const double *__restrict__ met,
+                                             const double *__restrict__ bg,
+                                             double *__restrict__ E)
+{
+    THREAD_IJK(b);
+    constexpr int T1 = (ED == 0) ? 1 : (ED == 1) ? 2 : 0;
+    constexpr int T2 = (ED == 0) ? 2 : (ED == 1) ? 0 : 1;
+    // getNormalPlane (metric.F90:151-167)
+    constexpr int E1 = (ED == 0) ? 1 : 0;
+    constexpr int E2 = (ED == 2) ? 1 : 2;
+    const long N3 = g.N3;
+    const long c = IX(g, i, j, k);
+    const long s2 = STRIDE<T2>(g);
+
+    // GetCornerV (fields.F90:457-517), doVdA
+    double A[8];
+#pragma unroll
+    for (int n = 0; n < 8; n++) A[n] = met[(M_FACE + T1) * N3 + c + (n - 4) * s2];
+    const double dA = dmax(wsum8([&](int n) { return A[n]; }), GAM_TINY);
+    double vc[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+        vc[d] = wsum8([&](int n) { return A[n] * Vf[d * N3 + c + (n - 4) * s2]; }) / dA;
+    const double *Te = met + (long)(M_TE + 9 * ED) * N3 + c;
+    const double v1 = vc[0] * Te[3 * N3] + vc[1] * Te[4 * N3] + vc[2] * Te[5 * N3];
+    const double v2 = vc[0] * Te[6 * N3] + vc[1] * Te[7 * N3] + vc[2] * Te[8 * N3];
+    double vDiff = sqrt(v1 * v1 + v2 * v2);
+
+    // GetCornerD (fields.F90:427-454)
+    const long se1 = STRIDE<E1>(g), se2 = STRIDE<E2>(g);
+    const double *Dh = H + (H_GAS + DEN) * N3;
+    const double Dc = 0.25 * (Dh[c] + Dh[c - se1] + Dh[c - se2] + Dh[c - se1 - se2]);
+
+    // GetCornerB (fields.F90:365-424)
+    const double *mfH = H + H_MF * N3;
+    const double *mfL = m.doLFM ? S + S_MF * N3 : mfH;
+    double bT1L, bT1R, fA1, bT2L, bT2R, fA2;
+    edge_lr<T1, T2, RM>(g, m, c, met, mfH, mfL, bT1L, bT1R, fA1);
+    edge_lr<T2, T1, RM>(g, m, c, met, mfH, mfL, bT2L, bT2R, fA2);
+    const double *Tb = met + (long)(M_TEB + 4 * ED) * N3 + c;
+    const double xni = Tb[0], yni = Tb[N3], xnj = Tb[2 * N3], ynj = Tb[3 * N3];
+    const double detT = 1.0 / (xni * ynj - xnj * yni);
+    const double bT1 = 0.5 * (bT1L + bT1R) / fA1;
+    const double bT2 = 0.5 * (bT2L + bT2R) / fA2;
+    const double db1 = (bT1R - bT1L) / fA1;
+    const double db2 = (bT2R - bT2L) / fA2;
+    const double Jd = db2 - db1;
+    double b1 = detT * (ynj * bT1 - yni * bT2);
+    double b2 = detT * (-(xnj * bT1) + xni * bT2);
+    if (BG) {
+        b1 = b1 + bg[(B_EDGB0 + 0 + 2 * ED) * N3 + c];
+        b2 = b2 + bg[(B_EDGB0 + 1 + 2 * ED) * N3 + c];
+    }
+    const double edge = met[(M_EDGE + ED) * N3 + c];
+    double vA = sqrt(b1 * b1 + b2 * b2) / sqrt(Dc);
+    if (BORIS) vA = m.Ca * vA / sqrt(m.Ca * m.Ca + vA * vA);
+    vDiff = vDiff + vA;
+    vDiff = dmin(vDiff, m.CFL * edge / clk->dt);
+    const double e = -(v1 * b2 - v2 * b1) + m.Vd0 * vDiff * Jd;
+    E[ED * N3 + c] = e * edge;
+}
+
+// ====================================================================
+// Interface fluxes of one direction (Fluxes, stress.F90:204-429)
+// ====================================================================
+template <int D, int RM, bool BG, bool BORIS>
+__global__ __launch_bounds__(256) void k_flux(Geo g, MParm m, Box b,
+                                              const double *__restrict__ H,
+                                              const double *__restrict__ S,
+                                              const double *__restrict__ met,
+                                              const double *__restrict__ bg,
+                                              double *__restrict__ gflx,
+                                              double *__restrict__ mflx)
+{
+    THREAD_IJK(b);
+    const long N3 = g.N3;
+    const long c = IX(g, i, j, k);
+    const long s = STRIDE<D>(g);
+
+    double vol[8];
+#pragma unroll
+    for (int n = 0; n < 8; n++) vol[n] = met[M_VOL * N3 + c + (n - 4) * s];
+    const double Vi = wsum8([&](int n) { return vol[n]; });
+
+    // L/R primitives (BlockStateLRs, recon.F90:39-93)
+    const double *Wh = H + H_WH * N3;
+    const double *Wl = m.doLFM ? H + H_WN * N3 : Wh;
+    double WL[5], WR[5];
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+        double q[8], nq[4];
+#pragma unroll
+        for (int n = 0; n < 8; n++) q[n] = Wh[v * N3 + c + (n - 4) * s];
+#pragma unroll
+        for (int n = 0; n < 4; n++) nq[n] = Wl[v * N3 + c + (n - 2) * s];
+        recon_lr<RM>(m.pdmb, vol, Vi, nq, q, WL[v], WR[v]);
+    }
+    double T[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) T[q] = met[(M_TF + q + 9 * D) * N3 + c];
+    const double faB = met[(M_FACE + D) * N3 + c];
+
+    double VaD = 0.0;
+    if (m.doMHD) {
+        const double *Bh = H + H_B * N3;
+        const double *Bl = m.doLFM ? S + S_B * N3 : Bh;
+        double BL[3], BR[3];
+#pragma unroll
+        for (int x = 0; x < 3; x++) {
+            double q[8], nq[4];
+#pragma unroll
+            for (int n = 0; n < 8; n++) q[n] = Bh[x * N3 + c + (n - 4) * s];
+#pragma unroll
+            for (int n = 0; n < 4; n++) nq[n] = Bl[x * N3 + c + (n - 2) * s];
+            recon_lr<RM>(m.pdmb, vol, Vi, nq, q, BL[x], BR[x]);
+        }
+        const double Bn = H[(H_MF + D) * N3 + c] / faB;
+        double B0[3] = {0.0, 0.0, 0.0}, B0n = 0.0;
+        if (BG) {
+#pragma unroll
+            for (int x = 0; x < 3; x++) B0[x] = bg[(B_FCB0 + x + 3 * D) * N3 + c];
+            B0n = bg[(B_BFLUX0 + D) * N3 + c] / faB;
+        }
+        double mF[3] = {0.0, 0.0, 0.0}, bbD = 0.0;
+        mag_kin<BG, BORIS>(m, -1.0, WL, BL, Bn, T, B0, B0n, mF, bbD, VaD);
+        mag_kin<BG, BORIS>(m, +1.0, WR, BR, Bn, T, B0, B0n, mF, bbD, VaD);
+        if (BORIS) {
+#pragma unroll
+            for (int x = 0; x < 3; x++) {
+                double deeP = bbD * (WR[1 + x] - WL[1 + x]) / (m.Ca * m.Ca);
+                mF[x] = mF[x] - m.cHogM * VaD * deeP;
+            }
+        }
+#pragma unroll
+        for (int x = 0; x < 3; x++) mflx[(x + 3 * D) * N3 + c] = faB * mF[x];
+    }
+
+    double gF[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, dW[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    gas_kin(m.gamma, -1.0, WL, T, gF, dW);
+    gas_kin(m.gamma, +1.0, WR, T, gF, dW);
+#pragma unroll
+    for (int v = 0; v < 5; v++) gF[v] = gF[v] - m.cHogH * VaD * dW[v];
+    if (BORIS) {   // "nuclear option" (stress.F90:399-417)
+        double vL = sqrt(WL[1] * WL[1] + WL[2] * WL[2] + WL[3] * WL[3]);
+        double vR = sqrt(WR[1] * WR[1] + WR[2] * WR[2] + WR[3] * WR[3]);
+        double vn = sqrt(vL * vL + vR * vR) / sqrt(2.0);
+        if (vn >= 1.5 * m.Ca) {
+#pragma unroll
+            for (int v = 0; v < 5; v++) {
+                double dU = H[(H_GAS + v) * N3 + c] - H[(H_GAS + v) * N3 + c - s];
+                gF[v] = gF[v] - m.Ca * dU;
+            }
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < 5; v++) gflx[(v + 5 * D) * N3 + c] = faB * gF[v];
+}
+
+// ====================================================================
+// CT update (E2Flux, fields.F90:18-51) + bFlux2Fld of the new state.
+// Domain: all face points.  Faces outside is..ie+1 x js..je+1 x ks..ke+1
+// keep their old flux; edges outside the active EMF boxes read as zero,
+// as CalcElecField zeroes E before filling active edges (fields.F90:89).
+// ====================================================================
+struct EdgeBoxes { Box e[3]; };
+
+__device__ __forceinline__ double e_at(const Geo &g, const EdgeBoxes &eb, const double *E, int d,
+                                       int i, int j, int k)
+{
+    return in_box(eb.e[d], i, j, k) ? E[d * g.N3 + IX(g, i, j, k)] : 0.0;
+}
+
+__device__ __forceinline__ double ct_face(const Geo &g, const EdgeBoxes &eb, const Box &ctb,
+                                          const double *E, const double *mfOld, double dt,
+                                          int d, int i, int j, int k)
+{
+    const double old = mfOld[d * g.N3 + IX(g, i, j, k)];
+    if (!in_box(ctb, i, j, k)) return old;
+    double df;
+    if (d == 0)
+        df = e_at(g, eb, E, 2, i, j + 1, k) - e_at(g, eb, E, 2, i, j, k)
+           - e_at(g, eb, E, 1, i, j, k + 1) + e_at(g, eb, E, 1, i, j, k);
+    else if (d == 1)
+        df = e_at(g, eb, E, 0, i, j, k + 1) - e_at(g, eb, E, 0, i, j, k)
+           - e_at(g, eb, E, 2, i + 1, j, k) + e_at(g, eb, E, 2, i, j, k);
+    else
+        df = e_at(g, eb, E, 1, i + 1, j, k) - e_at(g, eb, E, 1, i, j, k)
+           - e_at(g, eb, E, 0, i, j + 1, k) + e_at(g, eb, E, 0, i, j, k);
+    return old - dt * df;
+}
+
+__global__ __launch_bounds__(256) void k_ct_b(Geo g, MParm m, Box fb, Box ctb, EdgeBoxes eb,
+                                              const Clock *clk, const double *__restrict__ So,
+                                              double *__restrict__ Sn,
+                                              const double *__restrict__ met)
+{
+    THREAD_IJK(fb);
+    const long N3 = g.N3;
+    const long c = IX(g, i, j, k);
+    const double dt = clk->dtA;
+    const double *E = Sn + S_E * N3;
+    const double *mfo = So + S_MF * N3;
+    double f[3];
+    for (int d = 0; d < 3; d++) {
+        f[d] = m.doMHD ? ct_face(g, eb, ctb, E, mfo, dt, d, i, j, k) : mfo[d * N3 + c];
+        Sn[(S_MF + d) * N3 + c] = f[d];
+    }
+    if (i > g.ieg || j > g.jeg || k > g.keg) return;
+    double fIp, fJp, fKp;
+    if (m.doMHD) {
+        fIp = ct_face(g, eb, ctb, E, mfo, dt, 0, i + 1, j, k);
+        fJp = ct_face(g, eb, ctb, E, mfo, dt, 1, i, j + 1, k);
+        fKp = ct_face(g, eb, ctb, E, mfo, dt, 2, i, j, k + 1);
+    } else {
+        fIp = mfo[0 * N3 + c + 1]; fJp = mfo[1 * N3 + c + g.P]; fKp = mfo[2 * N3 + c + g.PP];
+    }
+    double B[3];
+    cell_b(g, met, c, f[0], f[1], f[2], fIp, fJp, fKp, B);
+    for (int x = 0; x < 3; x++) Sn[(S_B + x) * N3 + c] = B[x];
+}
+
+// ====================================================================
+// Block min-reduction of positive doubles into clk->dtBits.
+// ====================================================================
+__device__ __forceinline__ void block_min_dt(double v, Clock *clk)
+{
+    __shared__ double red[256 / 64];
+    for (int off = 32; off > 0; off >>= 1) v = dmin(v, __shfl_xor(v, off, 64));
+    const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+    if ((tid & 63) == 0) red[tid >> 6] = v;
+    __syncthreads();
+    if (tid == 0) {
+        double r = red[0];
+        for (int w = 1; w < (int)(blockDim.x * blockDim.y) / 64; w++) r = dmin(r, red[w]);
+        if (r > 0.0 && r < GAM_HUGE) atomicMin(&clk->dtBits, (unsigned long long)__double_as_longlong(r));
+    }
+}
+
+// ====================================================================
+// Gas update: Flux2Deltas (stress.F90:152-194), Reynolds (mhdgroup.F90:
+// 114-125, with Model%dt), Maxwell/Boris (171-195, 299-441); ghost cells
+// copy the old state (CopyState).  Cells of the CalcDT domain contribute
+// CellDT of the new state to the dt reduction (step.F90:91-101).
+// Domain: all cells (block covers every cell; out-of-range threads feed
+// +HUGE to the reduction).
+// ====================================================================
+template <bool BG, bool BORIS>
+__global__ __launch_bounds__(256) void k_gas(Geo g, MParm m, Box cb, Box ab, Box dtb,
+                                             Clock *clk, const double *__restrict__ So,
+                                             double *__restrict__ Sn,
+                                             const double *__restrict__ gflx,
+                                             const double *__restrict__ mflx,
+                                             const double *__restrict__ met,
+                                             const double *__restrict__ bg)
+{
+    const int i = cb.i0 + blockIdx.x * BX + threadIdx.x;
+    const int j = cb.j0 + blockIdx.y * BY + threadIdx.y;
+    const int k = cb.k0 + blockIdx.z;
+    const bool live = (i <= cb.i1 && j <= cb.j1);
+    double dtc = GAM_HUGE;
+    if (live) {
+        const long N3 = g.N3;
+        const long c = IX(g, i, j, k);
+        double U[5];
+#pragma unroll
+        for (int v = 0; v < 5; v++) U[v] = So[(S_GAS + v) * N3 + c];
+        if (in_box(ab, i, j, k)) {
+            const double dV = met[M_VOL * N3 + c];
+            const double dtM = clk->dt, dt = clk->dtA;
+            double dH[5], oU[5];
+#pragma unroll
+            for (int v = 0; v < 5; v++) {
+                const double *gv = gflx + (long)v * N3;
+                dH[v] = (gv[c] - gv[c + 1] + gv[5 * N3 + c] - gv[5 * N3 + c + g.P]
+                         + gv[10 * N3 + c] - gv[10 * N3 + c + g.PP]) / dV;
+                oU[v] = U[v];
+                U[v] = U[v] + dtM * dH[v];
+            }
+            if (m.doMHD) {
+                double dM[3], B[3], oB[3];
+#pragma unroll
+                for (int x = 0; x < 3; x++) {
+                    const double *mv = mflx + (long)x * N3;
+                    double r = (mv[c] - mv[c + 1] + mv[3 * N3 + c] - mv[3 * N3 + c + g.P]
+                                + mv[6 * N3 + c] - mv[6 * N3 + c + g.PP]) / dV;
+                    if (BG) r = r + bg[(B_DPB0 + x) * N3 + c];
+                    dM[x] = r;
+                    B[x] = Sn[(S_B + x) * N3 + c];
+                    oB[x] = So[(S_B + x) * N3 + c];
+                    if (BG) { B[x] = B[x] + bg[(B_B0 + x) * N3 + c]; oB[x] = oB[x] + bg[(B_B0 + x) * N3 + c]; }
+                }
+                if (BORIS) {
+                    // CellBoris (mhdgroup.F90:336-441), single fluid
+                    double W2[5];
+                    c2p_cell(m, U, W2);
+                    const double D2 = dmax(W2[DEN], m.dFloor);
+                    const double P2 = dmax(W2[PRS], m.pFloor);
+                    const double D0 = dmax(oU[DEN], m.dFloor);
+                    double V0[3] = {oU[MOMX] / D0, oU[MOMY] / D0, oU[MOMZ] / D0};
+                    double B1[3] = {0.5 * (B[0] + oB[0]), 0.5 * (B[1] + oB[1]), 0.5 * (B[2] + oB[2])};
+                    const double alpha = (B1[0] * B1[0] + B1[1] * B1[1] + B1[2] * B1[2]) / (D2 * m.Ca * m.Ca);
+                    const double dRho = dt * dH[DEN];
+                    double dPG[3] = {dt * dH[MOMX], dt * dH[MOMY], dt * dH[MOMZ]};
+                    double dPM[3] = {dt * dM[0], dt * dM[1], dt * dM[2]};
+                    const double pCon = 1.0 / (1.0 + alpha);
+                    double dMom[3];
+                    if (alpha > GAM_TINY) {
+                        const double nb = sqrt(B1[0] * B1[0] + B1[1] * B1[1] + B1[2] * B1[2]);
+                        double bh[3];
+                        if (nb <= GAM_TINY) { bh[0] = bh[1] = bh[2] = 0.0; }
+                        else { bh[0] = B1[0] / nb; bh[1] = B1[1] / nb; bh[2] = B1[2] / nb; }
+                        double sGM[3] = {dPG[0] + dPM[0], dPG[1] + dPM[1], dPG[2] + dPM[2]};
+                        const double pG = dPG[0] * bh[0] + dPG[1] * bh[1] + dPG[2] * bh[2];
+                        const double pS = sGM[0] * bh[0] + sGM[1] * bh[1] + sGM[2] * bh[2];
+                        const double pV = V0[0] * bh[0] + V0[1] * bh[1] + V0[2] * bh[2];
+#pragma unroll
+                        for (int x = 0; x < 3; x++)
+                            dMom[x] = pG * bh[x] + pCon * (sGM[x] - pS * bh[x]) + alpha * pCon * dRho * (V0[x] - pV * bh[x]);
+                    } else {
+#pragma unroll
+                        for (int x = 0; x < 3; x++) dMom[x] = dPG[x] + dPM[x];
+                    }
+                    double Wn[5];
+                    Wn[DEN] = D2;
+#pragma unroll
+                    for (int x = 0; x < 3; x++) Wn[1 + x] = (D0 * V0[x] + dMom[x]) / D2;
+                    Wn[PRS] = P2;
+                    p2c_cell(m, Wn, U);
+                } else {
+                    // CellMaxwell (mhdgroup.F90:299-329)
+                    const double d = dmax(U[DEN], m.dFloor);
+                    double mx = U[MOMX], my = U[MOMY], mz = U[MOMZ];
+                    double ke = 0.5 * (mx * mx + my * my + mz * mz) / d;
+                    const double p = dmax((m.gamma - 1) * (U[ENER] - ke), m.pFloor);
+                    mx = mx + dt * dM[0];
+                    my = my + dt * dM[1];
+                    mz = mz + dt * dM[2];
+                    ke = 0.5 * (mx * mx + my * my + mz * mz) / d;
+                    U[DEN] = d; U[MOMX] = mx; U[MOMY] = my; U[MOMZ] = mz;
+                    U[ENER] = ke + p / (m.gamma - 1);
+                }
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 5; v++) Sn[(S_GAS + v) * N3 + c] = U[v];
+        if (in_box(dtb, i, j, k)) {
+            double bx = Sn[(S_B + 0) * N3 + c], by = Sn[(S_B + 1) * N3 + c], bz = Sn[(S_B + 2) * N3 + c];
+            if (BG) { bx = bx + bg[(B_B0 + 0) * N3 + c]; by = by + bg[(B_B0 + 1) * N3 + c]; bz = bz + bg[(B_B0 + 2) * N3 + c]; }
+            const double dl = dmin(dmin(met[M_DI * N3 + c], met[M_DJ * N3 + c]), met[M_DK * N3 + c]);
+            dtc = cell_dt<BORIS>(m, U, bx, by, bz, dl);
+            if (!(dtc == dtc)) dtc = GAM_HUGE;
+        }
+    }
+    block_min_dt(dtc, clk);
+}
+
+// CalcDT over the DT box on a state (standalone CalcDT, step.F90:58-101)
+template <bool BG, bool BORIS>
+__global__ __launch_bounds__(256) void k_calc_dt(Geo g, MParm m, Box dtb, Clock *clk,
+                                                 const double *__restrict__ S,
+                                                 const double *__restrict__ met,
+                                                 const double *__restrict__ bg)
+{
+    const int i = dtb.i0 + blockIdx.x * BX + threadIdx.x;
+    const int j = dtb.j0 + blockIdx.y * BY + threadIdx.y;
+    const int k = dtb.k0 + blockIdx.z;
+    double dtc = GAM_HUGE;
+    if (i <= dtb.i1 && j <= dtb.j1) {
+        const long N3 = g.N3, c = IX(g, i, j, k);
+        double U[5];
+        for (int v = 0; v < 5; v++) U[v] = S[(S_GAS + v) * N3 + c];
+        double bx = S[(S_B + 0) * N3 + c], by = S[(S_B + 1) * N3 + c], bz = S[(S_B + 2) * N3 + c];
+        if (BG) { bx = bx + bg[(B_B0 + 0) * N3 + c]; by = by + bg[(B_B0 + 1) * N3 + c]; bz = bz + bg[(B_B0 + 2) * N3 + c]; }
+        const double dl = dmin(dmin(met[M_DI * N3 + c], met[M_DJ * N3 + c]), met[M_DK * N3 + c]);
+        dtc = cell_dt<BORIS>(m, U, bx, by, bz, dl);
+        if (!(dtc == dtc)) dtc = GAM_HUGE;
+    }
+    block_min_dt(dtc, clk);
+}
+
+// ====================================================================
+// Time bookkeeping and CalcDT checks (step.F90:103-166); one thread.
+// ====================================================================
+__global__ void k_finalize(MParm m, Clock *clk, int mode)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    Clock c = *clk;
+    if (mode & FIN_ADVANCE) {          // applyUpdate: oState%time=State%time; State%time+=dt
+        c.ost = c.st;
+        c.st = c.st + c.dtA;
+    }
+    if (mode & FIN_MODEL) {            // UpdateStateData (gamapp.F90:108-110)
+        c.ts = c.ts + 1;
+        c.t = c.t + c.dt;
+    }
+    if (mode & FIN_CALCDT) {
+        const double dtMin = __longlong_as_double((long long)c.dtBits);
+        int st = 0;
+        if (dtMin == GAM_HUGE) st = 2;
+        double dt = dtMin;
+        if (c.ts > 0) {
+            if (c.dt / dtMin >= 10) st = st > 1 ? st : 1;
+            if ((m.dt0 > GAM_TINY) && (dtMin / m.dt0 <= m.limDT0)) st = st > 1 ? st : 1;
+            if (dtMin <= GAM_TINY) st = 2;
+        }
+        if ((c.t + dt) > m.tFin) dt = dmax(m.tFin - c.t, GAM_TINY);
+        if (m.dt0 > GAM_TINY && dt > (10.0 * m.dt0)) dt = 10.0 * m.dt0;
+        c.dtRet = dt;
+        c.lastStatus = st;
+        c.status = c.status > st ? c.status : st;
+        if (mode & FIN_SETDT) { c.dt = dt; c.dtA = dt; }
+        if (mode & FIN_INIT) c.ost = c.st - c.dt;
+    }
+    c.dtBits = (unsigned long long)__double_as_longlong(GAM_HUGE);
+    *clk = c;
+}
+
+// ====================================================================
+// bFlux2Fld (gridutils.F90:281-301) over all cells
+// ====================================================================
+__global__ __launch_bounds__(256) void k_bflux2fld(Geo g, Box cb, const double *__restrict__ mf,
+                                                   double *__restrict__ B,
+                                                   const double *__restrict__ met)
+{
+    THREAD_IJK(cb);
+    const long N3 = g.N3, c = IX(g, i, j, k);
+    double b[3];
+    cell_b(g, met, c, mf[c], mf[N3 + c], mf[2 * N3 + c], mf[c + 1], mf[N3 + c + g.P],
+           mf[2 * N3 + c + g.PP], b);
+    for (int x = 0; x < 3; x++) B[x * N3 + c] = b[x];
+}
+
+// ====================================================================
+// Boundary conditions.  One thread per transverse ghost column; the n loop
+// runs inside as in the reference.  Kernels of the six faces run in the
+// reference order INI, OUTI, INJ, OUTJ, INK, OUTK.
+// ====================================================================
+__device__ __forceinline__ void copy_cell(double *S, long N3, long dst, long src)
+{
+    for (int v = 0; v < 5; v++) S[(S_GAS + v) * N3 + dst] = S[(S_GAS + v) * N3 + src];
+}
+__device__ __forceinline__ void copy_b(double *S, long N3, long dst, long src)
+{
+    for (int x = 0; x < 3; x++) S[(S_B + x) * N3 + dst] = S[(S_B + x) * N3 + src];
+}
+__device__ __forceinline__ bool is_cell(const Geo &g, int i, int j, int k)
+{
+    return i >= g.isg && i <= g.ieg && j >= g.jsg && j <= g.jeg && k >= g.ksg && k <= g.keg;
+}
+
+// periodic (bcs.F90:279-450).  side: 0..5.  (a,b) = transverse indices
+template <int SIDE>
+__global__ void k_bc_periodic(Geo g, Box t, double *__restrict__ S)
+{
+    const int a = t.i0 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = t.j0 + blockIdx.y;
+    if (a > t.i1 || b > t.j1) return;
+    const long N3 = g.N3;
+    double *mf = S + S_MF * N3;
+    for (int n = 1; n <= 4; n++) {
+        if (SIDE == 0) {        // inner I: a=j, b=k
+            const int ig = g.is - n, sr = g.ie - n + 1;
+            const long d = IX(g, ig, a, b), s = IX(g, sr, a, b);
+            if (is_cell(g, ig, a, b)) { copy_cell(S, N3, d, s); copy_b(S, N3, d, s); }
+            for (int x = 0; x < 3; x++) mf[x * N3 + d] = mf[x * N3 + s];
+        } else if (SIDE == 1) { // outer I
+            const int ig = g.ie + n, sr = g.is + n - 1;
+            const long d = IX(g, ig, a, b), s = IX(g, sr, a, b);
+            if (is_cell(g, ig, a, b)) { copy_cell(S, N3, d, s); copy_b(S, N3, d, s); }
+            mf[1 * N3 + d] = mf[1 * N3 + s];
+            mf[2 * N3 + d] = mf[2 * N3 + s];
+            mf[IX(g, ig + 1, a, b)] = mf[IX(g, g.is + n, a, b)];
+        } else if (SIDE == 2) { // inner J: a=i, b=k
+            const int jg = g.js - n, sr = g.je - n + 1;
+            const long d = IX(g, a, jg, b), s = IX(g, a, sr, b);
+            if (is_cell(g, a, jg, b)) { copy_cell(S, N3, d, s); copy_b(S, N3, d, s); }
+            for (int x = 0; x < 3; x++) mf[x * N3 + d] = mf[x * N3 + s];
+        } else if (SIDE == 3) { // outer J
+            const int jg = g.je + n, sr = g.js + n - 1;
+            const long d = IX(g, a, jg, b), s = IX(g, a, sr, b);
+            if (is_cell(g, a, jg, b)) { copy_cell(S, N3, d, s); copy_b(S, N3, d, s); }
+            mf[0 * N3 + d] = mf[0 * N3 + s];
+            mf[2 * N3 + d] = mf[2 * N3 + s];
+            mf[1 * N3 + IX(g, a, jg + 1, b)] = mf[1 * N3 + IX(g, a, g.js + n, b)];
+        } else if (SIDE == 4) { // inner K: a=i, b=j
+            const int kg = g.ks - n, sr = g.ke - n + 1;
+            const long d = IX(g, a, b, kg), s = IX(g, a, b, sr);
+            if (is_cell(g, a, b, kg)) { copy_cell(S, N3, d, s); copy_b(S, N3, d, s); }
+            for (int x = 0; x < 3; x++) mf[x * N3 + d] = mf[x * N3 + s];
+        } else {                // outer K
+            const int kg = g.ke + n, sr = g.ks + n - 1;
+            const long d = IX(g, a, b, kg), s = IX(g, a, b, sr);
+            if (is_cell(g, a, b, kg)) { copy_cell(S, N3, d, s); copy_b(S, N3, d, s); }
+            mf[0 * N3 + d] = mf[0 * N3 + s];
+            mf[1 * N3 + d] = mf[1 * N3 + s];
+            mf[2 * N3 + IX(g, a, b, kg + 1)] = mf[2 * N3 + IX(g, a, b, g.ks + n)];
+        }
+    }
+}
+
+// zero-gradient (bcs.F90:799-930), sphere in/out (934-982)
+template <int SIDE, int KIND>
+__global__ void k_bc_copy(Geo g, Box t, double *__restrict__ S, const double *__restrict__ met)
+{
+    const int a = t.i0 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = t.j0 + blockIdx.y;
+    if (a > t.i1 || b > t.j1) return;
+    const long N3 = g.N3;
+    double *mf = S + S_MF * N3;
+    if (SIDE == 0) {            // inner I: a=j, b=k
+        const long s = IX(g, g.is, a, b);
+        for (int n = 1; n <= 4; n++) {
+            const long d = IX(g, g.is - n, a, b);
+            copy_cell(S, N3, d, s);
+            for (int x = 0; x < 3; x++) mf[x * N3 + d] = mf[x * N3 + s];
+            if (KIND == 2) copy_b(S, N3, d, s);      // SphIn also copies Bxyz
+        }
+    } else if (SIDE == 1) {     // outer I
+        const long s = IX(g, g.ie, a, b);
+        if (KIND == 3) {        // SphOut
+            for (int n = 1; n <= 4; n++) {
+                const int ig = g.ie + n;
+                const long d = IX(g, ig, a, b);
+                copy_cell(S, N3, d, s);
+                mf[IX(g, ig + 1, a, b)] = mf[s];
+                mf[N3 + d] = mf[N3 + s];
+                mf[2 * N3 + d] = mf[2 * N3 + s];
+                copy_b(S, N3, d, s);
+            }
+        } else {                // zeroGrad_obcI: fluxes of the last physical cell's B
+            double B[3];
+            cell_b(g, met, s, mf[s], mf[N3 + s], mf[2 * N3 + s], mf[s + 1], mf[N3 + s + g.P],
+                   mf[2 * N3 + s + g.PP], B);
+            for (int n = 1; n <= 4; n++) {
+                const int ig = g.ie + n;
+                const long d = IX(g, ig, a, b);
+                copy_cell(S, N3, d, s);
+                const long fi[3] = {IX(g, ig + 1, a, b), d, d};
+                for (int x = 0; x < 3; x++) {
+                    const long f = fi[x];
+                    const double *nrm = met + (long)(M_TF + 9 * x) * N3 + f;
+                    mf[x * N3 + f] = met[(M_FACE + x) * N3 + f] * (B[0] * nrm[0] + B[1] * nrm[N3] + B[2] * nrm[2 * N3]);
+                }
+            }
+        }
+    } else if (SIDE == 2) {     // inner J: a=i, b=k
+        const long s = IX(g, a, g.js, b);
+        for (int n = 1; n <= 4; n++) {
+            const long d = IX(g, a, g.js - n, b);
+            copy_cell(S, N3, d, s);
+            for (int x = 0; x < 3; x++) mf[x * N3 + d] = mf[x * N3 + s];
+        }
+    } else if (SIDE == 3) {     // outer J
+        const long s = IX(g, a, g.je, b);
+        for (int n = 1; n <= 4; n++) {
+            const long d = IX(g, a, g.je + n, b);
+            copy_cell(S, N3, d, s);
+            mf[d] = mf[s];
+            mf[2 * N3 + d] = mf[2 * N3 + s];
+            mf[N3 + IX(g, a, g.je + n + 1, b)] = mf[N3 + IX(g, a, g.je + 1, b)];
+        }
+    } else if (SIDE == 4) {     // inner K: a=i, b=j
+        const long s = IX(g, a, b, g.ks);
+        for (int n = 1; n <= 4; n++) {
+            const long d = IX(g, a, b, g.ks - n);
+            copy_cell(S, N3, d, s);
+            for (int x = 0; x < 3; x++) mf[x * N3 + d] = mf[x * N3 + s];
+        }
+    } else {                    // outer K; the k+1 face takes the J flux (bcs.F90:925)
+        const long s = IX(g, a, b, g.ke);
+        for (int n = 1; n <= 4; n++) {
+            const long d = IX(g, a, b, g.ke + n);
+            copy_cell(S, N3, d, s);
+            mf[d] = mf[s];
+            mf[N3 + d] = mf[N3 + s];
+            mf[2 * N3 + IX(g, a, b, g.ke + n + 1)] = mf[N3 + IX(g, a, b, g.ke + 1)];
+        }
+    }
+}
+
+// ---------------------------------------------------------------- wind
+
+__device__ void wind_interp(const WindDev &w, double t, double *rho, double *pr, double *V, double *B)
+{   // InterpWind/InterpWgts with discrete walls (wind.F90:766-868)
+    int i0;
+    if (t >= w.tMax) i0 = w.n - 1;
+    else if (t <= w.tMin) i0 = 0;
+    else {
+        i0 = (int)floor((t - w.tMin) / w.dtW);
+        if (i0 + 1 > w.n - 1) i0 = w.n - 2;
+    }
+    *rho = w.Q[i0];
+    *pr = w.Q[i0 + 4 * w.n];
+    V[0] = w.Q[i0 + 1 * w.n]; V[1] = w.Q[i0 + 2 * w.n]; V[2] = w.Q[i0 + 3 * w.n];
+    B[1] = w.B[i0 + w.n]; B[2] = w.B[i0 + 2 * w.n];
+    B[0] = w.Bx0 + w.ByC * B[1] + w.BzC * B[2];
+    if (t <= w.tMin) { B[0] = 0.0; B[1] = 0.0; B[2] = 0.0; }
+}
+
+__device__ void wind_front(const WindDev &w, double t, double *vF)
+{   // RefreshWind (wind.F90:114-155)
+    double r, p, V[3], B[3];
+    wind_interp(w, t, &r, &p, V, B);
+    const double vs = -sqrt(V[0] * V[0] + V[1] * V[1] + V[2] * V[2]);
+    const double bcc = sqrt(w.ByC * w.ByC + w.BzC * w.BzC);
+    const double btt = sqrt(1 + bcc * bcc);
+    if (bcc > GAM_TINY) {
+        const double cB = w.ByC / bcc, sB = w.BzC / bcc, cF = 1.0 / btt, sF = bcc / btt;
+        vF[0] = vs * (cF * cF);
+        vF[1] = -vs * cF * sF * cB;
+        vF[2] = -vs * cF * sF * sB;
+    } else { vF[0] = vs; vF[1] = 0.0; vF[2] = 0.0; }
+}
+
+__device__ void wind_at(const WindDev &w, const double *vF, const double *x, double t,
+                        double *rho, double *pr, double *V, double *B)
+{   // GetWindAt (wind.F90:157-175)
+    const double vm = sqrt(vF[0] * vF[0] + vF[1] * vF[1] + vF[2] * vF[2]);
+    const double dl = ((x[0] - w.xW) * vF[0] + (x[1] - w.yW) * vF[1] + (x[2] - w.zW) * vF[2]) / (vm * vm);
+    wind_interp(w, t - dl, rho, pr, V, B);
+}
+
+__device__ double wind_wgt(const WindDev &w, const double *vF, const double *x, double t)
+{   // wgtWind (wind.F90:180-203) with RampDown(dTh,90,90) (math.F90:97-111)
+    double nh[3], vh[3], r, p, V[3], B[3];
+    double na = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+    if (na <= GAM_TINY) { nh[0] = nh[1] = nh[2] = 0.0; } else { nh[0] = x[0] / na; nh[1] = x[1] / na; nh[2] = x[2] / na; }
+    wind_at(w, vF, x, t, &r, &p, V, B);
+    double nv = sqrt(V[0] * V[0] + V[1] * V[1] + V[2] * V[2]);
+    if (nv <= GAM_TINY) { vh[0] = vh[1] = vh[2] = 0.0; } else { vh[0] = V[0] / nv; vh[1] = V[1] / nv; vh[2] = V[2] / nv; }
+    const double dTh = acos(-vh[0] * nh[0] + -vh[1] * nh[1] + -vh[2] * nh[2]) * 180.0 / GAM_PI;
+    if (dTh <= 90.0) return 1.0;
+    if (dTh >= 180.0) return 0.0;
+    const double rs = (dTh - 90.0) / 90.0;
+    const double ea = -1.0 / (1.0 - rs * rs);
+    return exp(ea + 1.0);
+}
+
+// WindBC part 1 (wind.F90:226-335): ghost gas and blended face fluxes.
+// Transverse: a = j in jsg..jeg+1, b = k in ksg..keg+1.
+__global__ void k_bc_wind(Geo g, MParm m, Box t, const Clock *clk, WindDev w,
+                          double *__restrict__ S, const double *__restrict__ met)
+{
+    const int j = t.i0 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = t.j0 + blockIdx.y;
+    if (j > t.i1 || k > t.j1) return;
+    const long N3 = g.N3;
+    double *mf = S + S_MF * N3;
+    const double tm = clk->t;
+    double vF[3];
+    wind_front(w, tm, vF);
+    const int ip = g.ie;
+    for (int n = 1; n <= 4; n++) {
+        const int ig = g.ie + n;
+        if (k <= g.keg && j <= g.jeg) {
+            const long cg = IX(g, ig, j, k), cp = IX(g, ip, j, k);
+            const double xc[3] = {met[(M_XYZCC + 0) * N3 + cg], met[(M_XYZCC + 1) * N3 + cg], met[(M_XYZCC + 2) * N3 + cg]};
+            const double wSW = wind_wgt(w, vF, xc, tm), wMHD = 1.0 - wSW;
+            double gs[5] = {0, 0, 0, 0, 0}, gi[5] = {0, 0, 0, 0, 0};
+            if (wSW > GAM_TINY) {
+                double r, p, V[3], B[3];
+                wind_at(w, vF, xc, tm, &r, &p, V, B);
+                gs[0] = r; gs[1] = V[0]; gs[2] = V[1]; gs[3] = V[2]; gs[4] = p;
+            }
+            if (wMHD > GAM_TINY) {
+                double U[5], Bi[3];
+                for (int v = 0; v < 5; v++) U[v] = S[(S_GAS + v) * N3 + cp];
+                for (int x = 0; x < 3; x++) Bi[x] = S[(S_B + x) * N3 + cp];
+                c2p_cell(m, U, gi);
+                // Con2Fast (gamutils.F90:338-360)
+                const double rho = dmax(U[DEN], m.dFloor);
+                const double vx = U[MOMX] / rho, vy = U[MOMY] / rho, vz = U[MOMZ] / rho;
+                const double ke = 0.5 * rho * (vx * vx + vy * vy + vz * vz);
+                const double P = dmax((m.gamma - 1) * (U[ENER] - ke), m.pFloor);
+                const double Cs = sqrt(m.gamma * P / rho);
+                double Va = sqrt(Bi[0] * Bi[0] + Bi[1] * Bi[1] + Bi[2] * Bi[2]) / sqrt(U[DEN]);
+                if (m.doBoris) Va = m.Ca * Va / sqrt(m.Ca * m.Ca + Va * Va);
+                const double Cx = sqrt(Cs * Cs + Va * Va);
+                const long fp = IX(g, ip + 1, j, k);
+                const double nH[3] = {met[(M_TF + 0) * N3 + fp], met[(M_TF + 1) * N3 + fp], met[(M_TF + 2) * N3 + fp]};
+                double V[3] = {gi[1], gi[2], gi[3]};
+                const double vp = V[0] * nH[0] + V[1] * nH[1] + V[2] * nH[2];
+                if (vp < 1.0 * Cx) {
+                    for (int x = 0; x < 3; x++) V[x] = V[x] - vp * nH[x] + 1.0 * Cx * nH[x];
+                }
+                gi[1] = V[0]; gi[2] = V[1]; gi[3] = V[2];
+            }
+            double gw[5], U[5];
+            for (int v = 0; v < 5; v++) gw[v] = wSW * gs[v] + wMHD * gi[v];
+            p2c_cell(m, gw, U);
+            for (int v = 0; v < 5; v++) S[(S_GAS + v) * N3 + cg] = U[v];
+        }
+        const int fi[3] = {ig + 1, ig, ig};
+        const int ii[3] = {ip + 1, ip, ip};
+        for (int d = 0; d < 3; d++) {
+            const long f = IX(g, fi[d], j, k);
+            const double xf[3] = {met[(M_XFC + 0 + 3 * d) * N3 + f], met[(M_XFC + 1 + 3 * d) * N3 + f], met[(M_XFC + 2 + 3 * d) * N3 + f]};
+            const double wSW = wind_wgt(w, vF, xf, tm);
+            double r, p, V[3], B[3];
+            wind_at(w, vF, xf, tm, &r, &p, V, B);
+            const double *nrm = met + (long)(M_TF + 9 * d) * N3 + f;
+            const double sw = met[(M_FACE + d) * N3 + f] * (nrm[0] * B[0] + nrm[N3] * B[1] + nrm[2 * N3] * B[2]);
+            const double in = mf[d * N3 + IX(g, ii[d], j, k)];
+            mf[d * N3 + f] = wSW * sw + (1 - wSW) * in;
+        }
+    }
+}
+
+// WindBC part 2 (wind.F90:338-348): ghost Bxyz from the blended fluxes with
+// the periodic k mapping of lfmIJKcc (ringutils.F90:482-518).
+__global__ void k_bc_wind_b(Geo g, Box t, double *__restrict__ S, const double *__restrict__ met)
+{
+    const int j = t.i0 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = t.j0 + blockIdx.y;
+    if (j > t.i1 || k > t.j1) return;
+    const long N3 = g.N3;
+    const double *mf = S + S_MF * N3;
+    int kp = k;
+    if (k < g.ks) kp = k + g.Nkp;
+    else if (k > g.ke) kp = k - g.Nkp;
+    for (int n = 1; n <= 4; n++) {
+        const int ig = g.ie + n;
+        const long s = IX(g, ig, j, kp);
+        double B[3];
+        cell_b(g, met, s, mf[s], mf[N3 + s], mf[2 * N3 + s], mf[s + 1], mf[N3 + s + g.P], mf[2 * N3 + s + g.PP], B);
+        const long d = IX(g, ig, j, k);
+        for (int x = 0; x < 3; x++) S[(S_B + x) * N3 + d] = B[x];
+    }
+}
+
+// DivB partial sums (gridutils.F90:717-770)
+__global__ void k_divb(Geo g, Box ab, const double *__restrict__ mf, const double *__restrict__ bg,
+                       int useBG, double *__restrict__ part)
+{
+    const int i = ab.i0 + blockIdx.x * BX + threadIdx.x;
+    const int j = ab.j0 + blockIdx.y * BY + threadIdx.y;
+    const int k = ab.k0 + blockIdx.z;
+    double v = 0.0;
+    if (i <= ab.i1 && j <= ab.j1) {
+        const long N3 = g.N3, c = IX(g, i, j, k);
+        double dFi = mf[c] - mf[c + 1], dFj = mf[N3 + c] - mf[N3 + c + g.P], dFk = mf[2 * N3 + c] - mf[2 * N3 + c + g.PP];
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+        if (useBG) {
+            const double *b0 = bg + B_BFLUX0 * N3;
+            a0 = b0[c] - b0[c + 1]; a1 = b0[N3 + c] - b0[N3 + c + g.P]; a2 = b0[2 * N3 + c] - b0[2 * N3 + c + g.PP];
+        }
+        v = fabs(dFi + dFj + dFk + a0 + a1 + a2);
+    }
+    __shared__ double red[BX * BY];
+    const int tid = threadIdx.x + threadIdx.y * BX;
+    red[tid] = v;
+    __syncthreads();
+    for (int s = BX * BY / 2; s > 0; s >>= 1) {
+        if (tid < s) red[tid] += red[tid + s];
+        __syncthreads();
+    }
+    if (tid == 0) part[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = red[0];
+}
+
+// Layout conversion between Fortran host arrays and the padded device index
+// space.  src/dst plane of an (ni,nj,nk) Fortran array starting at isg.
+__global__ void k_repack_in(Geo g, int ni, int nj, int nk, const double *__restrict__ src,
+                            double *__restrict__ dst)
+{
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long tot = (long)ni * nj * nk;
+    if (q >= tot) return;
+    const int ii = (int)(q % ni);
+    const long r = q / ni;
+    const int jj = (int)(r % nj);
+    const int kk = (int)(r / nj);
+    dst[IX(g, g.isg + ii, g.jsg + jj, g.ksg + kk)] = src[q];
+}
+
+__global__ void k_repack_out(Geo g, int ni, int nj, int nk, const double *__restrict__ src,
+                             double *__restrict__ dst)
+{
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long tot = (long)ni * nj * nk;
+    if (q >= tot) return;
+    const int ii = (int)(q % ni);
+    const long r = q / ni;
+    const int jj = (int)(r % nj);
+    const int kk = (int)(r / nj);
+    dst[q] = src[IX(g, g.isg + ii, g.jsg + jj, g.ksg + kk)];
+}
+
+// Zero the E entries outside the active EMF boxes (CalcElecField zeroes E
+// before filling the active edges, fields.F90:89).
+__global__ void k_mask_e(Geo g, Box all, EdgeBoxes eb, double *__restrict__ E)
+{
+    THREAD_IJK(all);
+    const long c = IX(g, i, j, k);
+    for (int d = 0; d < 3; d++)
+        if (!in_box(eb.e[d], i, j, k)) E[d * g.N3 + c] = 0.0;
+}
+
+// ====================================================================
+// Host-side launchers
+// ====================================================================
+#define LAUNCH(kern, grid, block, stream, ...) hipLaunchKernelGGL(kern, grid, block, 0, stream, __VA_ARGS__)
+
+void launch_predict(hipStream_t s, const Geo &g, const MParm &m, const Box &fb, const Clock *clk,
+                    const double *S, const double *O, const double *met, double *H)
+{
+    LAUNCH(k_predict, grid_of(fb), dim3(BX, BY), s, g, m, fb, clk, S, O, met, H);
+}
+
+void launch_mom2face(hipStream_t s, int dT, const Geo &g, const MParm &m, const Box &b,
+                     const double *H, const double *met, double *Vf)
+{
+    if (dT == 0) LAUNCH(k_mom2face<0>, grid_of(b), dim3(BX, BY), s, g, m, b, H, met, Vf);
+    else if (dT == 1) LAUNCH(k_mom2face<1>, grid_of(b), dim3(BX, BY), s, g, m, b, H, met, Vf);
+    else LAUNCH(k_mom2face<2>, grid_of(b), dim3(BX, BY), s, g, m, b, H, met, Vf);
+}
+
+template <int ED, int RM, bool BG, bool BO>
+static void emf_t(hipStream_t s, const Geo &g, const MParm &m, const Box &b, const Clock *clk,
+                  const double *H, const double *S, const double *Vf, const double *met,
+                  const double *bg, double *E)
+{
+    LAUNCH((k_emf<ED, RM, BG, BO>), grid_of(b), dim3(BX, BY), s, g, m, b, clk, H, S, Vf, met, bg, E);
+}
+
+template <int ED>
+static void emf_d(hipStream_t s, const Geo &g, const MParm &m, const Box &b, const Clock *clk,
+                  const double *H, const double *S, const double *Vf, const double *met,
+                  const double *bg, double *E)
+{
+    const int sel = (m.rmeth ? 4 : 0) + (m.doBG ? 2 : 0) + (m.doBoris ? 1 : 0);
+    switch (sel) {
+    case 0: emf_t<ED, 0, false, false>(s, g, m, b, clk, H, S, Vf, met, bg, E); break;
+    case 1: emf_t<ED, 0, false, true>(s, g, m, b, clk, H, S, Vf, met, bg, E); break;
+    case 2: emf_t<ED, 0, true, false>(s, g, m, b, clk, H, S, Vf, met, bg, E); break;
+    case 3: emf_t<ED, 0, true, true>(s, g, m, b, clk, H, S, Vf, met, bg, E); break;
+    case 4: emf_t<ED, 1, false, false>(s, g, m, b, clk, H, S, Vf, met, bg, E); break;
+    case 5: emf_t<ED, 1, false, true>(s, g, m, b, clk, H, S, Vf, met, bg, E); break;
+    case 6: emf_t<ED, 1, true, false>(s, g, m, b, clk, H, S, Vf, met, bg, E); break;
+    default: emf_t<ED, 1, true, true>(s, g, m, b, clk, H, S, Vf, met, bg, E); break;
+    }
+}
+
+void launch_emf(hipStream_t s, int eD, const Geo &g, const MParm &m, const Box &b, const Clock *clk,
+                const double *H, const double *S, const double *Vf, const double *met,
+                const double *bg, double *E)
+{
+    if (eD == 0) emf_d<0>(s, g, m, b, clk, H, S, Vf, met, bg, E);
+    else if (eD == 1) emf_d<1>(s, g, m, b, clk, H, S, Vf, met, bg, E);
+    else emf_d<2>(s, g, m, b, clk, H, S, Vf, met, bg, E);
+}
+
+template <int D, int RM, bool BG, bool BO>
+static void flux_t(hipStream_t s, const Geo &g, const MParm &m, const Box &b, const double *H,
+                   const double *S, const double *met, const double *bg, double *gf, double *mf)
+{
+    LAUNCH((k_flux<D, RM, BG, BO>), grid_of(b), dim3(BX, BY), s, g, m, b, H, S, met, bg, gf, mf);
+}
+
+template <int D>
+static void flux_d(hipStream_t s, const Geo &g, const MParm &m, const Box &b, const double *H,
+                   const double *S, const double *met, const double *bg, double *gf, double *mf)
+{
+    const int sel = (m.rmeth ? 4 : 0) + (m.doBG ? 2 : 0) + (m.doBoris ? 1 : 0);
+    switch (sel) {
+    case 0: flux_t<D, 0, false, false>(s, g, m, b, H, S, met, bg, gf, mf); break;
+    case 1: flux_t<D, 0, false, true>(s, g, m, b, H, S, met, bg, gf, mf); break;
+    case 2: flux_t<D, 0, true, false>(s, g, m, b, H, S, met, bg, gf, mf); break;
+    case 3: flux_t<D, 0, true, true>(s, g, m, b, H, S, met, bg, gf, mf); break;
+    case 4: flux_t<D, 1, false, false>(s, g, m, b, H, S, met, bg, gf, mf); break;
+    case 5: flux_t<D, 1, false, true>(s, g, m, b, H, S, met, bg, gf, mf); break;
+    case 6: flux_t<D, 1, true, false>(s, g, m, b, H, S, met, bg, gf, mf); break;
+    default: flux_t<D, 1, true, true>(s, g, m, b, H, S, met, bg, gf, mf); break;
+    }
+}
+
+void launch_flux(hipStream_t s, int d, const Geo &g, const MParm &m, const Box &b, const double *H,
+                 const double *S, const double *met, const double *bg, double *gf, double *mf)
+{
+    if (d == 0) flux_d<0>(s, g, m, b, H, S, met, bg, gf, mf);
+    else if (d == 1) flux_d<1>(s, g, m, b, H, S, met, bg, gf, mf);
+    else flux_d<2>(s, g, m, b, H, S, met, bg, gf, mf);
+}
+
+void launch_ct_b(hipStream_t s, const Geo &g, const MParm &m, const Box &fb, const Box &ctb,
+                 const Box *eboxes, const Clock *clk, const double *So, double *Sn, const double *met)
+{
+    EdgeBoxes eb;
+    for (int d = 0; d < 3; d++) eb.e[d] = eboxes[d];
+    LAUNCH(k_ct_b, grid_of(fb), dim3(BX, BY), s, g, m, fb, ctb, eb, clk, So, Sn, met);
+}
+
+void launch_gas(hipStream_t s, const Geo &g, const MParm &m, const Box &cb, const Box &ab,
+                const Box &dtb, Clock *clk, const double *So, double *Sn, const double *gf,
+                const double *mf, const double *met, const double *bg)
+{
+    const int sel = (m.doBG ? 2 : 0) + (m.doBoris ? 1 : 0);
+    switch (sel) {
+    case 0: LAUNCH((k_gas<false, false>), grid_of(cb), dim3(BX, BY), s, g, m, cb, ab, dtb, clk, So, Sn, gf, mf, met, bg); break;
+    case 1: LAUNCH((k_gas<false, true>), grid_of(cb), dim3(BX, BY), s, g, m, cb, ab, dtb, clk, So, Sn, gf, mf, met, bg); break;
+    case 2: LAUNCH((k_gas<true, false>), grid_of(cb), dim3(BX, BY), s, g, m, cb, ab, dtb, clk, So, Sn, gf, mf, met, bg); break;
+    default: LAUNCH((k_gas<true, true>), grid_of(cb), dim3(BX, BY), s, g, m, cb, ab, dtb, clk, So, Sn, gf, mf, met, bg); break;
+    }
+}
+
+void launch_calc_dt(hipStream_t s, const Geo &g, const MParm &m, const Box &dtb, Clock *clk,
+                    const double *S, const double *met, const double *bg)
+{
+    const int sel = (m.doBG ? 2 : 0) + (m.doBoris ? 1 : 0);
+    switch (sel) {
+    case 0: LAUNCH((k_calc_dt<false, false>), grid_of(dtb), dim3(BX, BY), s, g, m, dtb, clk, S, met, bg); break;
+    case 1: LAUNCH((k_calc_dt<false, true>), grid_of(dtb), dim3(BX, BY), s, g, m, dtb, clk, S, met, bg); break;
+    case 2: LAUNCH((k_calc_dt<true, false>), grid_of(dtb), dim3(BX, BY), s, g, m, dtb, clk, S, met, bg); break;
+    default: LAUNCH((k_calc_dt<true, true>), grid_of(dtb), dim3(BX, BY), s, g, m, dtb, clk, S, met, bg); break;
+    }
+}
+
+void launch_finalize(hipStream_t s, const MParm &m, Clock *clk, int mode)
+{
+    LAUNCH(k_finalize, dim3(1), dim3(64), s, m, clk, mode);
+}
+
+void launch_bflux2fld(hipStream_t s, const Geo &g, const Box &cb, const double *mf, double *B,
+                      const double *met)
+{
+    LAUNCH(k_bflux2fld, grid_of(cb), dim3(BX, BY), s, g, cb, mf, B, met);
+}
+
+static inline dim3 grid2(const Box &t) { return dim3((unsigned)((t.i1 - t.i0 + 64) / 64), (unsigned)(t.j1 - t.j0 + 1)); }
+
+void launch_bc(hipStream_t s, int side, int kind, const Geo &g, const MParm &m, const Clock *clk,
+               const WindDev *w, double *S, const double *met)
+{
+    // Transverse boxes (a,b) per side as in the reference loops
+    Box t;
+    const bool pi = (kind == 0);
+    if (side <= 1) { t.i0 = g.jsg; t.i1 = g.jeg + (pi ? 1 : 0); t.j0 = g.ksg; t.j1 = g.keg + (pi ? 1 : 0); }
+    else if (side <= 3) { t.i0 = g.isg; t.i1 = g.ieg + (pi ? 1 : 0); t.j0 = g.ksg; t.j1 = g.keg + (pi ? 1 : 0); }
+    else { t.i0 = g.isg; t.i1 = g.ieg + (pi ? 1 : 0); t.j0 = g.jsg; t.j1 = g.jeg + (pi ? 1 : 0); }
+    t.k0 = t.k1 = 0;
+    dim3 gr = grid2(t), bl(64);
+    if (kind == 0) {
+        switch (side) {
+        case 0: LAUNCH(k_bc_periodic<0>, gr, bl, s, g, t, S); break;
+        case 1: LAUNCH(k_bc_periodic<1>, gr, bl, s, g, t, S); break;
+        case 2: LAUNCH(k_bc_periodic<2>, gr, bl, s, g, t, S); break;
+        case 3: LAUNCH(k_bc_periodic<3>, gr, bl, s, g, t, S); break;
+        case 4: LAUNCH(k_bc_periodic<4>, gr, bl, s, g, t, S); break;
+        default: LAUNCH(k_bc_periodic<5>, gr, bl, s, g, t, S); break;
+        }
+    } else if (kind == 1) {
+        switch (side) {
+        case 0: LAUNCH((k_bc_copy<0, 1>), gr, bl, s, g, t, S, met); break;
+        case 1: LAUNCH((k_bc_copy<1, 1>), gr, bl, s, g, t, S, met); break;
+        case 2: LAUNCH((k_bc_copy<2, 1>), gr, bl, s, g, t, S, met); break;
+        case 3: LAUNCH((k_bc_copy<3, 1>), gr, bl, s, g, t, S, met); break;
+        case 4: LAUNCH((k_bc_copy<4, 1>), gr, bl, s, g, t, S, met); break;
+        default: LAUNCH((k_bc_copy<5, 1>), gr, bl, s, g, t, S, met); break;
+        }
+    } else if (kind == 2) {
+        LAUNCH((k_bc_copy<0, 2>), gr, bl, s, g, t, S, met);
+    } else if (kind == 3) {
+        LAUNCH((k_bc_copy<1, 3>), gr, bl, s, g, t, S, met);
+    } else if (kind == 4) {
+        Box tw = t;
+        tw.i1 = g.jeg + 1; tw.j1 = g.keg + 1;
+        LAUNCH(k_bc_wind, grid2(tw), bl, s, g, m, tw, clk, *w, S, met);
+        LAUNCH(k_bc_wind_b, gr, bl, s, g, t, S, met);
+    }
+}
+
+void launch_divb(hipStream_t s, const Geo &g, const Box &ab, const double *mf, const double *bg,
+                 int useBG, double *part, int *nparts)
+{
+    dim3 gr = grid_of(ab);
+    *nparts = (int)(gr.x * gr.y * gr.z);
+    LAUNCH(k_divb, gr, dim3(BX, BY), s, g, ab, mf, bg, useBG, part);
+}
+
+void launch_repack(hipStream_t s, int dir_in, const Geo &g, int ni, int nj, int nk,
+                   const double *src, double *dst)
+{
+    const long tot = (long)ni * nj * nk;
+    dim3 gr((unsigned)((tot + 255) / 256));
+    if (dir_in) LAUNCH(k_repack_in, gr, dim3(256), s, g, ni, nj, nk, src, dst);
+    else LAUNCH(k_repack_out, gr, dim3(256), s, g, ni, nj, nk, src, dst);
+}
+
+void launch_mask_e(hipStream_t s, const Geo &g, const Box &all, const Box *eboxes, double *E)
+{
+    EdgeBoxes eb;
+    for (int d = 0; d < 3; d++) eb.e[d] = eboxes[d];
+    LAUNCH(k_mask_e, grid_of(all), dim3(BX, BY), s, g, all, eb, E);
+}

@@ -1,0 +1,308 @@
+"""Python binding of libgamhip (include/gamhip.h) over ctypes.
+
+Mirrors the reference's operator interface for the GAMERA advance step
+(rgorby/MAGE src/gamera): a ``GamHip`` context holds one Grid/State/Solver
+set on the GPU and exposes AdvanceMHD, CalcDT, EnforceBCs, bFlux2Fld and
+stepGamera.  Host arrays are numpy arrays in the reference's Fortran
+layout: cell arrays (Ni,Nj,Nk[,comp..]) and face arrays
+(Ni+1,Nj+1,Nk+1[,comp..]), order='F', with Ni = Nip + 2*nG.
+
+There is no CPU fallback: if the library or a GPU is missing, calls raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libgamhip.so")
+
+GAM_OK = 0
+GAM_STATE, GAM_OSTATE, GAM_HFSTATE = 0, 1, 2
+BC = {"periodic": 0, "zerograd": 1, "sphin": 2, "sphout": 3, "wind": 4}
+RECON = {"8CENT": 0, "7UP": 1}
+ERRORS = {-1: "GAM_E_ARG", -2: "GAM_E_HIP", -3: "GAM_E_STATE", -4: "GAM_E_DT_BAD",
+          -5: "GAM_E_DT_DISASTER", -6: "GAM_E_NOMEM"}
+
+# Every symbol include/gamhip.h declares
+EXPORTS = (
+    "gam_last_error", "gam_create", "gam_destroy", "gam_set_grid", "gam_grid_from_corners",
+    "gam_get_grid", "gam_set_background", "gam_add_dipole_b0", "gam_get_background",
+    "gam_set_wind", "gam_put_state", "gam_get_state", "gam_set_time", "gam_get_time",
+    "gam_init_run", "gam_advance_mhd", "gam_calc_dt", "gam_enforce_bcs", "gam_bflux2fld",
+    "gam_step", "gam_run", "gam_sync", "gam_divb", "gam_time_steps", "gam_set_profiling",
+    "gam_kernel_times", "gam_kernel_group_name", "gam_get_solver",
+)
+
+DP = C.POINTER(C.c_double)
+
+
+class GamError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (%d): %s" % (ERRORS.get(code, "GAM_E?"), code, msg))
+        self.code = code
+
+
+class Dims(C.Structure):
+    _fields_ = [("Nip", C.c_int), ("Njp", C.c_int), ("Nkp", C.c_int), ("nG", C.c_int)]
+
+
+class Model(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "gamma", "CFL", "Vd0", "Ca", "cHogH", "cHogM", "pdmb", "dFloor", "pFloor",
+        "tFin", "dt0", "limDT0")] + [(n, C.c_int) for n in (
+            "doMHD", "doBoris", "doBackground", "doLFMLim", "rmeth")] + [("bc", C.c_int * 6)]
+
+
+GRID_FIELDS = ("xyzcc", "xfc", "di", "dj", "dk", "volume", "Tf", "Te", "Teb", "face", "edge")
+BG_FIELDS = ("fcB0", "edgB0", "B0", "bFlux0", "dpB0")
+
+
+class GridArrays(C.Structure):
+    _fields_ = [(n, DP) for n in GRID_FIELDS]
+
+
+class BgArrays(C.Structure):
+    _fields_ = [(n, DP) for n in BG_FIELDS]
+
+
+_lib = None
+
+
+def load():
+    """Load libgamhip.so (built by mage_amd.build)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise OSError("libgamhip.so is not built (run __graft_entry__.build())")
+    L = C.CDLL(LIBPATH)
+    vp = C.c_void_p
+    L.gam_last_error.restype = C.c_char_p
+    L.gam_kernel_group_name.restype = C.c_char_p
+    L.gam_kernel_group_name.argtypes = [C.c_int]
+    L.gam_create.argtypes = [C.POINTER(vp), C.POINTER(Dims), C.POINTER(Model)]
+    L.gam_destroy.argtypes = [vp]
+    L.gam_set_grid.argtypes = [vp, C.POINTER(GridArrays)]
+    L.gam_get_grid.argtypes = [vp, C.POINTER(GridArrays)]
+    L.gam_grid_from_corners.argtypes = [vp, DP]
+    L.gam_set_background.argtypes = [vp, C.POINTER(BgArrays)]
+    L.gam_get_background.argtypes = [vp, C.POINTER(BgArrays)]
+    L.gam_add_dipole_b0.argtypes = [vp, C.c_double]
+    L.gam_set_wind.argtypes = [vp, C.c_int, DP, DP, DP, C.c_double, C.c_double, C.c_double, DP]
+    L.gam_put_state.argtypes = [vp, C.c_int, DP, DP, DP, DP, C.c_double]
+    L.gam_get_state.argtypes = [vp, C.c_int, DP, DP, DP, DP, DP]
+    L.gam_set_time.argtypes = [vp, C.c_double, C.c_double, C.c_int]
+    L.gam_get_time.argtypes = [vp, DP, DP, C.POINTER(C.c_int)]
+    L.gam_init_run.argtypes = [vp, DP]
+    L.gam_advance_mhd.argtypes = [vp, C.c_double]
+    L.gam_calc_dt.argtypes = [vp, DP]
+    L.gam_enforce_bcs.argtypes = [vp]
+    L.gam_bflux2fld.argtypes = [vp, C.c_int]
+    L.gam_step.argtypes = [vp, DP]
+    L.gam_run.argtypes = [vp, C.c_int]
+    L.gam_sync.argtypes = [vp]
+    L.gam_divb.argtypes = [vp, DP]
+    L.gam_time_steps.argtypes = [vp, C.c_int, DP]
+    L.gam_set_profiling.argtypes = [vp, C.c_int]
+    L.gam_kernel_times.argtypes = [vp, DP, C.POINTER(C.c_int)]
+    L.gam_get_solver.argtypes = [vp, C.c_int, DP]
+    _lib = L
+    return L
+
+
+def _p(a):
+    if a is None:
+        return DP()
+    if a.dtype != np.float64 or not a.flags.f_contiguous:
+        raise ValueError("arrays must be float64 and Fortran-contiguous")
+    return a.ctypes.data_as(DP)
+
+
+def model_struct(p):
+    m = Model()
+    for k in ("gamma", "CFL", "Vd0", "Ca", "cHogH", "cHogM", "pdmb", "dFloor", "pFloor",
+              "tFin", "dt0", "limDT0"):
+        setattr(m, k, float(p[k]))
+    for k in ("doMHD", "doBoris", "doBackground", "doLFMLim", "rmeth"):
+        setattr(m, k, int(p[k]))
+    for s in range(6):
+        b = p["bc"][s]
+        m.bc[s] = BC[b] if isinstance(b, str) else int(b)
+    return m
+
+
+class GamHip:
+    """One device context: Grid, State, oState, StateHf and solver storage."""
+
+    def __init__(self, Nip, Njp, Nkp, params):
+        self.L = load()
+        self.Nip, self.Njp, self.Nkp = Nip, Njp, Nkp
+        self.Ni, self.Nj, self.Nk = Nip + 8, Njp + 8, Nkp + 8
+        self.params = dict(params)
+        self.ctx = C.c_void_p()
+        d = Dims(Nip, Njp, Nkp, 4)
+        m = model_struct(params)
+        self._chk(self.L.gam_create(C.byref(self.ctx), C.byref(d), C.byref(m)))
+
+    def _chk(self, rc):
+        if rc != GAM_OK:
+            raise GamError(rc, self.L.gam_last_error().decode())
+        return rc
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.L.gam_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- shapes
+    @property
+    def cshape(self):
+        return (self.Ni, self.Nj, self.Nk)
+
+    @property
+    def fshape(self):
+        return (self.Ni + 1, self.Nj + 1, self.Nk + 1)
+
+    # ---- grid
+    def grid_from_corners(self, xyz):
+        self._chk(self.L.gam_grid_from_corners(self.ctx, _p(np.asfortranarray(xyz, dtype=np.float64))))
+
+    def set_grid(self, g):
+        a = GridArrays()
+        keep = {}
+        for k in GRID_FIELDS:
+            keep[k] = np.asfortranarray(g[k], dtype=np.float64)
+            setattr(a, k, _p(keep[k]))
+        self._chk(self.L.gam_set_grid(self.ctx, C.byref(a)))
+
+    def get_grid(self):
+        c, f = self.cshape, self.fshape
+        shapes = {"xyzcc": c + (3,), "xfc": f + (3, 3), "di": c, "dj": c, "dk": c, "volume": c,
+                  "Tf": f + (9, 3), "Te": f + (9, 3), "Teb": f + (4, 3), "face": f + (3,), "edge": f + (3,)}
+        out = {k: np.zeros(s, order="F") for k, s in shapes.items()}
+        a = GridArrays()
+        for k in GRID_FIELDS:
+            setattr(a, k, _p(out[k]))
+        self._chk(self.L.gam_get_grid(self.ctx, C.byref(a)))
+        return out
+
+    def add_dipole_b0(self, M0):
+        self._chk(self.L.gam_add_dipole_b0(self.ctx, float(M0)))
+
+    def set_background(self, b):
+        a = BgArrays()
+        keep = {}
+        for k in BG_FIELDS:
+            keep[k] = np.asfortranarray(b[k], dtype=np.float64)
+            setattr(a, k, _p(keep[k]))
+        self._chk(self.L.gam_set_background(self.ctx, C.byref(a)))
+
+    def get_background(self):
+        c = self.cshape
+        shapes = {"fcB0": c + (3, 3), "edgB0": c + (2, 3), "B0": c + (3,), "bFlux0": c + (3,), "dpB0": c + (3,)}
+        out = {k: np.zeros(s, order="F") for k, s in shapes.items()}
+        a = BgArrays()
+        for k in BG_FIELDS:
+            setattr(a, k, _p(out[k]))
+        self._chk(self.L.gam_get_background(self.ctx, C.byref(a)))
+        return out
+
+    def set_wind(self, wind):
+        tW = np.asfortranarray(wind["tW"], dtype=np.float64)
+        Q = np.asfortranarray(wind["Q"], dtype=np.float64)
+        B = np.asfortranarray(wind["B"], dtype=np.float64)
+        xyzW = np.asfortranarray(wind.get("xyzW", (0.0, 0.0, 0.0)), dtype=np.float64)
+        self._chk(self.L.gam_set_wind(self.ctx, len(tW), _p(tW), _p(Q), _p(B),
+                                      float(wind.get("Bx0", 0.0)), float(wind.get("ByC", 0.0)),
+                                      float(wind.get("BzC", 0.0)), _p(xyzW)))
+
+    # ---- state
+    def put_state(self, st, which=GAM_STATE):
+        arrs = {k: (np.asfortranarray(st[k], dtype=np.float64) if st.get(k) is not None else None)
+                for k in ("Gas", "magFlux", "Bxyz", "Efld")}
+        self._chk(self.L.gam_put_state(self.ctx, which, _p(arrs["Gas"]), _p(arrs["magFlux"]),
+                                       _p(arrs["Bxyz"]), _p(arrs["Efld"]), float(st.get("time", 0.0))))
+
+    def get_state(self, which=GAM_STATE):
+        c, f = self.cshape, self.fshape
+        out = {"Gas": np.zeros(c + (5,), order="F"), "magFlux": np.zeros(f + (3,), order="F"),
+               "Bxyz": np.zeros(c + (3,), order="F"), "Efld": np.zeros(c + (3,), order="F")}
+        t = C.c_double()
+        self._chk(self.L.gam_get_state(self.ctx, which, _p(out["Gas"]), _p(out["magFlux"]),
+                                       _p(out["Bxyz"]), _p(out["Efld"]), C.byref(t)))
+        out["time"] = t.value
+        return out
+
+    def get_solver(self, which):
+        c = self.cshape
+        out = np.zeros(c + ((5, 3) if which == 0 else (3, 3)), order="F")
+        self._chk(self.L.gam_get_solver(self.ctx, which, _p(out)))
+        return out
+
+    def set_time(self, t, dt, ts):
+        self._chk(self.L.gam_set_time(self.ctx, float(t), float(dt), int(ts)))
+
+    def get_time(self):
+        t, dt, ts = C.c_double(), C.c_double(), C.c_int()
+        self._chk(self.L.gam_get_time(self.ctx, C.byref(t), C.byref(dt), C.byref(ts)))
+        return t.value, dt.value, ts.value
+
+    # ---- operators
+    def init_run(self):
+        dt = C.c_double()
+        self._chk(self.L.gam_init_run(self.ctx, C.byref(dt)))
+        return dt.value
+
+    def advance_mhd(self, dt):
+        self._chk(self.L.gam_advance_mhd(self.ctx, float(dt)))
+
+    def calc_dt(self):
+        dt = C.c_double()
+        self._chk(self.L.gam_calc_dt(self.ctx, C.byref(dt)))
+        return dt.value
+
+    def enforce_bcs(self):
+        self._chk(self.L.gam_enforce_bcs(self.ctx))
+
+    def bflux2fld(self, which=GAM_STATE):
+        self._chk(self.L.gam_bflux2fld(self.ctx, which))
+
+    def step(self, sync=True):
+        if sync:
+            dt = C.c_double()
+            self._chk(self.L.gam_step(self.ctx, C.byref(dt)))
+            return dt.value
+        self._chk(self.L.gam_step(self.ctx, DP()))
+        return None
+
+    def run(self, nsteps):
+        self._chk(self.L.gam_run(self.ctx, int(nsteps)))
+
+    def sync(self):
+        self._chk(self.L.gam_sync(self.ctx))
+
+    def divb(self):
+        v = C.c_double()
+        self._chk(self.L.gam_divb(self.ctx, C.byref(v)))
+        return v.value
+
+    def time_steps(self, nsteps):
+        ms = C.c_double()
+        self._chk(self.L.gam_time_steps(self.ctx, int(nsteps), C.byref(ms)))
+        return ms.value
+
+    def set_profiling(self, on):
+        self._chk(self.L.gam_set_profiling(self.ctx, 1 if on else 0))
+
+    def kernel_times(self):
+        ms = (C.c_double * 8)()
+        n = (C.c_int * 8)()
+        self._chk(self.L.gam_kernel_times(self.ctx, ms, n))
+        return {self.L.gam_kernel_group_name(q).decode(): (ms[q], n[q]) for q in range(8)}

@@ -1,0 +1,261 @@
+"""Grids, initial conditions and model settings of the benchmark workloads.
+
+Host-side input producers (the reference's genGridXML / prob.F90 role,
+init.F90:564-790): corner coordinates, seeded initial states and Model
+knobs.  All arrays are numpy arrays in the reference's Fortran layout with
+ghost lower bound isg = 1 - nG (nG = 4).
+
+Workloads
+  periodic_box  64^3 uniform Cartesian periodic box (Fourier-mode rho, v, A)
+  shell         log-r spherical shell r in [2.5, 30], theta in
+                [0.03pi, 0.97pi], phi periodic, 5% seeded warp of the theta
+                lines, dipole background, inflow on the outer shell
+Random numbers come from SplitMix64 (Steele, Lea & Flood 2014) seeded from
+31337 (kdefs.F90:28), so every run is reproducible bit for bit.
+"""
+import math
+
+import numpy as np
+
+NG = 4
+SEED0 = 31337
+MASK64 = (1 << 64) - 1
+
+# 12-point Gauss-Legendre nodes/weights (quadrature.F90:9-26)
+GA = np.array([0.1252334085114689154724, 0.3678314989981801937527, 0.5873179542866174472967,
+               0.7699026741943046870369, 0.9041172563704748566785, 0.9815606342467192506906,
+               -0.1252334085114689154724, -0.3678314989981801937527, -0.5873179542866174472967,
+               -0.7699026741943046870369, -0.9041172563704748566785, -0.9815606342467192506906])
+GW = np.array([0.2491470458134027850006, 0.233492536538354808761, 0.203167426723065921749,
+               0.160078328543346226335, 0.1069393259953184309603, 0.0471753363865118271946] * 2)
+
+
+class SplitMix64:
+    """SplitMix64 generator; uniform() returns doubles in [0,1) from the top 53 bits."""
+
+    def __init__(self, seed):
+        self.s = seed & MASK64
+
+    def next(self):
+        self.s = (self.s + 0x9E3779B97F4A7C15) & MASK64
+        z = self.s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
+        return z ^ (z >> 31)
+
+    def uniform(self):
+        return (self.next() >> 11) * (1.0 / 9007199254740992.0)
+
+    def randint(self, lo, hi):
+        """Integer in [lo, hi]."""
+        return lo + int(self.uniform() * (hi - lo + 1))
+
+
+def model_params(**over):
+    """Model_T knobs with the reference defaults (init.F90:312-484)."""
+    p = dict(gamma=5.0 / 3.0, CFL=0.3, Vd0=0.5, Ca=1.0e12, cHogH=0.25, cHogM=0.25, pdmb=0.75,
+             dFloor=1.0e-12, pFloor=1.0e-12, tFin=1.0e30, dt0=0.0, limDT0=1.0e-3,
+             doMHD=1, doBoris=0, doBackground=0, doLFMLim=1, rmeth=0,
+             bc=["periodic"] * 6)
+    p.update(over)
+    return p
+
+
+def index_ranges(N):
+    """Ghost-inclusive corner indices 1-nG .. N+nG+1."""
+    return np.arange(1 - NG, N + NG + 2)
+
+
+# -------------------------------------------------------------- grids
+def cartesian_corners(Nip, Njp, Nkp, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0)):
+    """Uniform Cartesian corners, x = min + (i-1)*dx (init.F90:663-709)."""
+    dx = [(hi[q] - lo[q]) / n for q, n in enumerate((Nip, Njp, Nkp))]
+    I = index_ranges(Nip)
+    J = index_ranges(Njp)
+    K = index_ranges(Nkp)
+    x = lo[0] + (I - 1) * dx[0]
+    y = lo[1] + (J - 1) * dx[1]
+    z = lo[2] + (K - 1) * dx[2]
+    xyz = np.zeros((len(I), len(J), len(K), 3), order="F")
+    xyz[..., 0] = x[:, None, None]
+    xyz[..., 1] = y[None, :, None]
+    xyz[..., 2] = z[None, None, :]
+    return xyz
+
+
+def shell_corners(Nip, Njp, Nkp, r0=2.5, r1=30.0, th0=0.03 * math.pi, th1=0.97 * math.pi,
+                  warp=0.05, seed=SEED0):
+    """Warped spherical-shell corners.
+
+    r is log-spaced over [r0, r1]; theta is uniform over [th0, th1] in the
+    active range and continues exponentially towards the poles in the ghost
+    layers so that no ghost cell crosses the axis; phi covers [0, 2pi).  The
+    theta lines get a seeded non-orthogonal displacement
+        dtheta = warp*(th1-th0)*sin(pi*s)*sin(2pi*u + phi + psi)
+    with s, u the normalised theta and log-r indices.
+    """
+    rng = SplitMix64(seed)
+    psi = 2.0 * math.pi * rng.uniform()
+    I = index_ranges(Nip)
+    J = index_ranges(Njp)
+    K = index_ranges(Nkp)
+    u = (I - 1) / Nip
+    r = r0 * (r1 / r0) ** u
+    s = (J - 1) / Njp
+    span = th1 - th0
+    th = th0 + s * span
+    lo = s < 0
+    hi = s > 1
+    th[lo] = th0 * np.exp(s[lo] * span / th0)
+    th[hi] = math.pi - (math.pi - th1) * np.exp((1 - s[hi]) * span / (math.pi - th1))
+    ph = 2.0 * math.pi * (K - 1) / Nkp
+    R = r[:, None, None]
+    U = u[:, None, None]
+    S = s[None, :, None]
+    TH = th[None, :, None] + warp * span * np.sin(math.pi * S) * np.sin(2 * math.pi * U + ph[None, None, :] + psi)
+    PH = ph[None, None, :]
+    xyz = np.zeros((len(I), len(J), len(K), 3), order="F")
+    xyz[..., 0] = R * np.sin(TH) * np.cos(PH)
+    xyz[..., 1] = R * np.sin(TH) * np.sin(PH)
+    xyz[..., 2] = R * np.cos(TH)
+    return xyz
+
+
+# -------------------------------------------------------------- states
+def empty_state(Nip, Njp, Nkp):
+    c = (Nip + 2 * NG, Njp + 2 * NG, Nkp + 2 * NG)
+    f = (c[0] + 1, c[1] + 1, c[2] + 1)
+    return {"Gas": np.zeros(c + (5,), order="F"), "magFlux": np.zeros(f + (3,), order="F"),
+            "Bxyz": np.zeros(c + (3,), order="F"), "Efld": np.zeros(c + (3,), order="F"),
+            "time": 0.0}
+
+
+def prim2con(gamma, rho, vx, vy, vz, p):
+    """CellP2C (gamutils.F90:362-383) on arrays."""
+    ke = 0.5 * rho * (vx * vx + vy * vy + vz * vz)
+    return np.stack([rho, rho * vx, rho * vy, rho * vz, p / (gamma - 1) + ke], axis=-1)
+
+
+def fourier_modes(rng, n, kmax=4):
+    """n seeded modes: integer wavevector 0<|k|<=kmax, phase, unit direction."""
+    modes = []
+    while len(modes) < n:
+        k = np.array([rng.randint(-kmax, kmax) for _ in range(3)], dtype=float)
+        nk = np.linalg.norm(k)
+        if nk == 0 or nk > kmax:
+            continue
+        ph = 2 * math.pi * rng.uniform()
+        d = np.array([rng.uniform() - 0.5 for _ in range(3)])
+        d = d / np.linalg.norm(d)
+        modes.append((k, ph, d))
+    return modes
+
+
+def _modesum(modes, X, Y, Z, scale=1.0):
+    out = np.zeros(np.broadcast(X, Y, Z).shape)
+    for k, ph, _ in modes:
+        out += np.cos(2 * math.pi * (k[0] * X + k[1] * Y + k[2] * Z) / scale + ph)
+    return out
+
+
+def _vecsum(modes, X, Y, Z, amp):
+    out = [np.zeros(np.broadcast(X, Y, Z).shape) for _ in range(3)]
+    for k, ph, d in modes:
+        c = amp * np.cos(2 * math.pi * (k[0] * X + k[1] * Y + k[2] * Z) + ph)
+        for q in range(3):
+            out[q] += d[q] * c
+    return out
+
+
+def vecpot_fluxes(xyz, Afun):
+    """Face fluxes from edge line integrals of A (VectorPot2Flux,
+    gridutils.F90:193-237) with the 12-point Gauss edge rule
+    (GaussianEdgeIntegral, quadrature.F90:347-369)."""
+    ni, nj, nk, _ = xyz.shape          # corner counts = Ni+1, ...
+    Ni, Nj, Nk = ni - 1, nj - 1, nk - 1
+    lA = np.zeros((Ni, Nj, Nk, 3), order="F")
+    e0 = xyz[:-2, :-2, :-2]
+    for d in range(3):
+        sl = [slice(0, -2)] * 3
+        sl[d] = slice(1, -1)
+        e1 = xyz[tuple(sl)]
+        dR2 = 0.5 * (e0 - e1)
+        xb = 0.5 * (e0 + e1)
+        acc = np.zeros(e0.shape)
+        for q in range(12):
+            xp = xb + GA[q] * dR2
+            A = Afun(xp[..., 0], xp[..., 1], xp[..., 2])
+            for c in range(3):
+                acc[..., c] += 0.5 * GW[q] * A[c]
+        lA[:-1, :-1, :-1, d] = np.sum(acc * (e1 - e0), axis=-1)
+    mf = np.zeros((Ni + 1, Nj + 1, Nk + 1, 3), order="F")
+    a = lA
+    s = (slice(0, Ni - 1), slice(0, Nj - 1), slice(0, Nk - 1))
+    ip = (slice(1, Ni), slice(0, Nj - 1), slice(0, Nk - 1))
+    jp = (slice(0, Ni - 1), slice(1, Nj), slice(0, Nk - 1))
+    kp = (slice(0, Ni - 1), slice(0, Nj - 1), slice(1, Nk))
+    mf[s + (0,)] = a[s + (1,)] - a[kp + (1,)] + a[jp + (2,)] - a[s + (2,)]
+    mf[s + (1,)] = -(a[s + (0,)] - a[kp + (0,)] + a[ip + (2,)] - a[s + (2,)])
+    mf[s + (2,)] = a[s + (0,)] - a[jp + (0,)] + a[ip + (1,)] - a[s + (1,)]
+    return mf
+
+
+def periodic_box_state(Nip, Njp, Nkp, xyzcc, xyz, gamma=5.0 / 3.0, seed=SEED0):
+    """Seeded periodic-box IC on cell centres xyzcc(Ni,Nj,Nk,3):
+    rho = 1 + 0.1*sum_8 cos(2pi k.x + phi), p = 1, v = 8 modes of amplitude
+    0.1, B = curl A with A = 8 modes of amplitude 0.05 (face fluxes by edge
+    integration, so div B vanishes to round-off).  Point values at the
+    volume centroids."""
+    rng = SplitMix64(seed)
+    rmodes = fourier_modes(rng, 8)
+    vmodes = fourier_modes(rng, 8)
+    amodes = fourier_modes(rng, 8)
+    st = empty_state(Nip, Njp, Nkp)
+    X, Y, Z = xyzcc[..., 0], xyzcc[..., 1], xyzcc[..., 2]
+    rho = 1.0 + 0.1 * _modesum(rmodes, X, Y, Z)
+    vx, vy, vz = _vecsum(vmodes, X, Y, Z, 0.1)
+    p = np.ones_like(rho)
+    st["Gas"][...] = prim2con(gamma, rho, vx, vy, vz, p)
+    st["magFlux"][...] = vecpot_fluxes(xyz, lambda x, y, z: _vecsum(amodes, x, y, z, 0.05))
+    return st
+
+
+def shell_state(Nip, Njp, Nkp, xyzcc, gamma=5.0 / 3.0, seed=SEED0, p0=1.0e-3):
+    """Shell IC: rho = 1, v = 0, residual B = 0, p = p0*(1 + 0.5*f) with f a
+    seeded smooth field in [-1,1] (8 modes on the 60-wide box)."""
+    rng = SplitMix64(seed + 1)
+    modes = fourier_modes(rng, 8)
+    st = empty_state(Nip, Njp, Nkp)
+    X, Y, Z = xyzcc[..., 0], xyzcc[..., 1], xyzcc[..., 2]
+    f = _modesum(modes, X, Y, Z, scale=60.0) / 8.0
+    rho = np.ones_like(X)
+    p = p0 * (1.0 + 0.5 * f)
+    z = np.zeros_like(X)
+    st["Gas"][...] = prim2con(gamma, rho, z, z, z, p)
+    return st
+
+
+def constant_wind(rho=1.0, vx=-0.4, p=1.0e-3, bz=-0.05):
+    """A two-sample constant inflow series covering every time the runs reach."""
+    tW = np.array([-1.0e3, 1.0e3])
+    Q = np.zeros((2, 5), order="F")
+    Q[:, 0] = rho
+    Q[:, 1] = vx
+    Q[:, 4] = p
+    B = np.zeros((2, 3), order="F")
+    B[:, 2] = bz
+    return {"tW": tW, "Q": Q, "B": B, "Bx0": 0.0, "ByC": 0.0, "BzC": 0.0, "xyzW": (0.0, 0.0, 0.0)}
+
+
+# Dipole moment giving |B0| = 1 at r = 2.5 on the equator, oriented like the
+# Earth's (equatorial B0 along +z): M0 = -(2.5)^3.
+SHELL_M0 = -(2.5 ** 3)
+
+
+def box_params(rmeth="7UP"):
+    return model_params(rmeth={"8CENT": 0, "7UP": 1}[rmeth], bc=["periodic"] * 6)
+
+
+def shell_params(rmeth="7UP"):
+    return model_params(rmeth={"8CENT": 0, "7UP": 1}[rmeth], doBackground=1, doBoris=1, Ca=10.0,
+                        bc=["zerograd", "wind", "zerograd", "zerograd", "periodic", "periodic"])
