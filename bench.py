@@ -1,0 +1,188 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X GAMERA MHD step (FP64 cell-updates/s).
+
+Workload (one GPU): the 192x192x256 warped log-r spherical shell with the
+dipole background, Boris correction (Ca=10), 7th-order upwind + PDM
+reconstruction and the inflow/outflow outer boundary, stepped with the full
+stepGamera sequence (AdvanceMHD, CalcDT, EnforceBCs, bFlux2Fld).  Inputs are
+synthetic and resident in HBM before the timed region.
+
+Prints ONE JSON line.  value = Nip*Njp*Nkp*steps / seconds (the kZCs
+definition of output.F90:69-70, times 1e3), timed with hipEvents on the
+library's stream around exactly --steps steps after --warmup steps.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 vector
+# Algorithmic FP64 words per cell-update of the whole step (state 19 read +
+# 14 written, metrics 79, dipole background 24): 136 words = 1088 B.
+STEP_BYTES_PER_CELL = 8 * (19 + 14 + 79 + 24)
+# Algorithmic words of one interface-flux launch, per face: stencil inputs
+# counted once per cell (half-state 5, limiter state 5, half B 3, limiter B
+# 3, volume 1), face data (Tf 9, area 1, half-state flux 1, fcB0 3,
+# bFlux0 1) and outputs (gas 5, magnetic 3): 40 words = 320 B.
+FLUX_BYTES_PER_FACE = 8 * 40
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def setup_shell(gamhip, pb, N, rmeth="7UP", full_grid=False):
+    Nip, Njp, Nkp = N
+    p = pb.shell_params(rmeth)
+    g = gamhip.GamHip(Nip, Njp, Nkp, p)
+    xyz = pb.shell_corners(Nip, Njp, Nkp)
+    g.grid_from_corners(xyz)
+    g.add_dipole_b0(pb.SHELL_M0)
+    wind = pb.constant_wind()
+    g.set_wind(wind)
+    if full_grid:
+        grid = g.get_grid()
+        grid.update(g.get_background())
+        grid["xyz"] = xyz
+    else:
+        grid = g.get_grid(fields=("xyzcc",))
+    st = pb.shell_state(Nip, Njp, Nkp, grid["xyzcc"])
+    g.put_state(st)
+    g.init_run()
+    return g, p, grid, st, wind
+
+
+def cpu_baseline(gamhip, pb, min_seconds=10.0, max_steps=20):
+    """Time the oracle (C/OpenMP restatement of the reference step) on a
+    96x96x128 sample of the same shell workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    N = (96, 96, 128)
+    g, p, grid, st, wind = setup_shell(gamhip, pb, N, full_grid=True)
+    g.close()
+    o = po.Oracle(*N, p, grid, wind=wind)
+    o.set_state(st)
+    o.init_run()
+    o.step()                      # warm-up (first touch of the work arrays)
+    n = 0
+    t0 = time.perf_counter()
+    while n < max_steps:
+        o.step()
+        n += 1
+        if time.perf_counter() - t0 >= min_seconds:
+            break
+    sec = time.perf_counter() - t0
+    cells = N[0] * N[1] * N[2]
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    return {"value": cells * n / sec, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": "oracle (C/OpenMP restatement of stepGamera), %d steps of the %dx%dx%d shell "
+                      "with the same physics, %.1f s" % (n, N[0], N[1], N[2], sec)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--grid", type=str, default="192x192x256")
+    ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported "
+              "(the multi-GPU halo exchange is a design, not a build target)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    from mage_amd import gamhip, problems as pb
+    N = tuple(int(x) for x in args.grid.split("x"))
+    cells = N[0] * N[1] * N[2]
+    t0 = time.time()
+    g, p, grid, st, wind = setup_shell(gamhip, pb, N)
+    log("setup %.1f s" % (time.time() - t0))
+
+    g.run(args.warmup)
+    ms = g.time_steps(args.steps)
+    ms_step = ms / args.steps
+    value = cells * args.steps / (ms / 1e3)
+    log("timed %d steps: %.3f ms/step, %.4e cell-updates/s" % (args.steps, ms_step, value))
+
+    # Per-kernel timing pass (hipEvents around every launch on the stream)
+    g.set_profiling(True)
+    g.run(args.profile_steps)
+    kt = g.kernel_times()
+    g.set_profiling(False)
+    kernels = {k: {"ms_total": v[0], "launches": v[1],
+                   "ms_per_launch": (v[0] / v[1]) if v[1] else None} for k, v in kt.items()}
+    flux = kt["flux"]
+    flux_ms = flux[0] / flux[1]
+    faces = ((N[0] + 1) * N[1] * N[2] + N[0] * (N[1] + 1) * N[2] + N[0] * N[1] * (N[2] + 1)) / 3.0
+    flux_gbs = FLUX_BYTES_PER_FACE * faces / (flux_ms * 1e-3) / 1e9
+    step_gbs = STEP_BYTES_PER_CELL * value / 1e9
+    t, dt, ts = g.get_time()
+    log("kernel ms/launch: " + ", ".join("%s=%.3f" % (k, v["ms_per_launch"] or 0) for k, v in kernels.items()))
+    g.close()
+
+    cpu = None
+    if not args.no_cpu:
+        t1 = time.time()
+        cpu = cpu_baseline(gamhip, pb)
+        log("cpu baseline %.4e cell-updates/s on %d threads (%.1f s)" % (cpu["value"], cpu["cores"], time.time() - t1))
+
+    out = {
+        "metric": "FP64 MHD cell-updates/s vs host CPU; ensemble gain; adjoint/fwd cost",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "curvilinear single run: %dx%dx%d warped log-r spherical shell r=2.5..30, "
+                        "theta 0.03pi..0.97pi, phi periodic; dipole B0 |B0|=1 at r=2.5; rho=1, "
+                        "p=1e-3 seeded; inflow rho=1 vx=-0.4 Bz=-0.05; 7UP+PDM, Boris Ca=10, "
+                        "full stepGamera" % N,
+            "grid": list(N),
+            "cells": cells,
+            "reconstruction": "7UP",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_flux (one interface direction per launch)",
+            "achieved": flux_gbs,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": flux_gbs / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_launch": FLUX_BYTES_PER_FACE * faces,
+            "ms_per_launch": flux_ms,
+        },
+        "step_roofline": {
+            "bound": "hbm",
+            "bytes_per_cell_update": STEP_BYTES_PER_CELL,
+            "achieved": step_gbs,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": step_gbs / HBM_PEAK_GBS,
+        },
+        "kernels": kernels,
+        "final": {"t": t, "dt": dt, "ts": ts},
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["gpu_over_cpu"] = value / cpu["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -509,13 +509,13 @@ static void enq_advance(gam_ctx *c, int finmode)
             prof_end(c, KG_EMF);
         }
     }
-    prof_begin(c, KG_FLUX);
     for (int d = 0; d < 3; d++) {
         Box fb = act_box(g);
         if (d == 0) fb.i1 += 1; else if (d == 1) fb.j1 += 1; else fb.k1 += 1;
+        prof_begin(c, KG_FLUX);
         launch_flux(s, d, g, c->mp, fb, c->H, S, c->met, c->bg, c->gflx, c->mflx);
+        prof_end(c, KG_FLUX);
     }
-    prof_end(c, KG_FLUX);
 
     prof_begin(c, KG_CT);
     Box ctb = mkbox(g.is, g.ie + 1, g.js, g.je + 1, g.ks, g.ke + 1);

@@ -182,13 +182,13 @@ class GamHip:
             setattr(a, k, _p(keep[k]))
         self._chk(self.L.gam_set_grid(self.ctx, C.byref(a)))
 
-    def get_grid(self):
+    def get_grid(self, fields=GRID_FIELDS):
         c, f = self.cshape, self.fshape
         shapes = {"xyzcc": c + (3,), "xfc": f + (3, 3), "di": c, "dj": c, "dk": c, "volume": c,
                   "Tf": f + (9, 3), "Te": f + (9, 3), "Teb": f + (4, 3), "face": f + (3,), "edge": f + (3,)}
-        out = {k: np.zeros(s, order="F") for k, s in shapes.items()}
+        out = {k: np.zeros(shapes[k], order="F") for k in fields}
         a = GridArrays()
-        for k in GRID_FIELDS:
+        for k in fields:
             setattr(a, k, _p(out[k]))
         self._chk(self.L.gam_get_grid(self.ctx, C.byref(a)))
         return out

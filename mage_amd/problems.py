@@ -92,7 +92,7 @@ def shell_corners(Nip, Njp, Nkp, r0=2.5, r1=30.0, th0=0.03 * math.pi, th1=0.97 *
     layers so that no ghost cell crosses the axis; phi covers [0, 2pi).  The
     theta lines get a seeded non-orthogonal displacement
         dtheta = warp*(th1-th0)*sin(pi*s)*sin(2pi*u + phi + psi)
-    with s, u the normalised theta and log-r indices.
+    with s, u the normalised theta and log-r indices (s clipped to [0,1]).
     """
     rng = SplitMix64(seed)
     psi = 2.0 * math.pi * rng.uniform()
@@ -112,7 +112,10 @@ def shell_corners(Nip, Njp, Nkp, r0=2.5, r1=30.0, th0=0.03 * math.pi, th1=0.97 *
     R = r[:, None, None]
     U = u[:, None, None]
     S = s[None, :, None]
-    TH = th[None, :, None] + warp * span * np.sin(math.pi * S) * np.sin(2 * math.pi * U + ph[None, None, :] + psi)
+    # the warp vanishes outside the active theta range so ghost lines near the
+    # poles stay monotone
+    TH = th[None, :, None] + warp * span * np.sin(math.pi * np.clip(S, 0.0, 1.0)) * np.sin(
+        2 * math.pi * U + ph[None, None, :] + psi)
     PH = ph[None, None, :]
     xyz = np.zeros((len(I), len(J), len(K), 3), order="F")
     xyz[..., 0] = R * np.sin(TH) * np.cos(PH)
