@@ -31,9 +31,9 @@ static int fail(int code, const char *fmt, ...)
             return fail(GAM_E_HIP, "%s failed: %s", #call, hipGetErrorString(e_));    \
     } while (0)
 
-enum { KG_PREDICT = 0, KG_MOM2FACE, KG_EMF, KG_FLUX, KG_CT, KG_GAS, KG_BC, KG_BFLD };
+enum { KG_PREDICT = 0, KG_MOM2FACE, KG_EMF, KG_FLUX, KG_UPDATE, KG_BC, KG_BFLD, KG_SPARE };
 static const char *kg_names[GAM_KG_COUNT] = {"predict", "mom2face", "emf", "flux",
-                                             "ct_b", "gas_dt", "bcs_finalize", "bflux2fld"};
+                                             "update_dt", "bcs", "bflux2fld_ghosts", "unused"};
 
 struct gam_ctx {
     gam_dims dims;
@@ -466,10 +466,15 @@ static int check_ready(gam_ctx *c, const char *who)
     return GAM_OK;
 }
 
-static void enq_bflux2fld(gam_ctx *c, double *S)
+static Box empty_box() { return mkbox(1, 0, 1, 0, 1, 0); }
+
+// bFlux2Fld over all cells, or over the ghost cells only when the active
+// cells' faces are known to be unchanged since their B was computed.
+static void enq_bflux2fld(gam_ctx *c, double *S, bool ghosts_only = false)
 {
     const long N3 = c->g.N3;
-    launch_bflux2fld(c->stream, c->g, cell_box(c->g), S + S_MF * N3, S + S_B * N3, c->met);
+    launch_bflux2fld(c->stream, c->g, cell_box(c->g), ghosts_only ? act_box(c->g) : empty_box(),
+                     S + S_MF * N3, S + S_B * N3, c->met);
 }
 
 static void enq_bcs(gam_ctx *c, double *S)
@@ -517,14 +522,11 @@ static void enq_advance(gam_ctx *c, int finmode)
         prof_end(c, KG_FLUX);
     }
 
-    prof_begin(c, KG_CT);
+    prof_begin(c, KG_UPDATE);
     Box ctb = mkbox(g.is, g.ie + 1, g.js, g.je + 1, g.ks, g.ke + 1);
-    launch_ct_b(s, g, c->mp, face_box(g), ctb, eb, c->clk, S, Sn, c->met);
-    prof_end(c, KG_CT);
-
-    prof_begin(c, KG_GAS);
-    launch_gas(s, g, c->mp, cell_box(g), act_box(g), dt_box(g), c->clk, S, Sn, c->gflx, c->mflx, c->met, c->bg);
-    prof_end(c, KG_GAS);
+    launch_update(s, g, c->mp, face_box(g), ctb, eb, act_box(g), dt_box(g), c->clk, S, Sn, c->gflx,
+                  c->mflx, c->met, c->bg);
+    prof_end(c, KG_UPDATE);
 
     launch_finalize(s, c->mp, c->clk, finmode);
     c->cur = 1 - c->cur;
@@ -538,7 +540,7 @@ static void enq_step(gam_ctx *c)
     enq_bcs(c, S);
     prof_end(c, KG_BC);
     prof_begin(c, KG_BFLD);
-    enq_bflux2fld(c, S);
+    enq_bflux2fld(c, S, true);
     prof_end(c, KG_BFLD);
 }
 

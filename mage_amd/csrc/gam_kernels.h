@@ -28,17 +28,14 @@ void launch_emf(hipStream_t s, int eD, const Geo &g, const MParm &m, const Box &
                 const double *bg, double *E);
 void launch_flux(hipStream_t s, int d, const Geo &g, const MParm &m, const Box &b, const double *H,
                  const double *S, const double *met, const double *bg, double *gf, double *mf);
-void launch_ct_b(hipStream_t s, const Geo &g, const MParm &m, const Box &fb, const Box &ctb,
-                 const Box *eboxes, const Clock *clk, const double *So, double *Sn,
-                 const double *met);
-void launch_gas(hipStream_t s, const Geo &g, const MParm &m, const Box &cb, const Box &ab,
-                const Box &dtb, Clock *clk, const double *So, double *Sn, const double *gf,
-                const double *mf, const double *met, const double *bg);
+void launch_update(hipStream_t s, const Geo &g, const MParm &m, const Box &fb, const Box &ctb,
+                   const Box *eboxes, const Box &ab, const Box &dtb, Clock *clk, const double *So,
+                   double *Sn, const double *gf, const double *mf, const double *met, const double *bg);
 void launch_calc_dt(hipStream_t s, const Geo &g, const MParm &m, const Box &dtb, Clock *clk,
                     const double *S, const double *met, const double *bg);
 void launch_finalize(hipStream_t s, const MParm &m, Clock *clk, int mode);
-void launch_bflux2fld(hipStream_t s, const Geo &g, const Box &cb, const double *mf, double *B,
-                      const double *met);
+void launch_bflux2fld(hipStream_t s, const Geo &g, const Box &cb, const Box &skip, const double *mf,
+                      double *B, const double *met);
 void launch_bc(hipStream_t s, int side, int kind, const Geo &g, const MParm &m, const Clock *clk,
                const WindDev *w, double *S, const double *met);
 void launch_divb(hipStream_t s, const Geo &g, const Box &ab, const double *mf, const double *bg,

@@ -6,12 +6,13 @@
 //   k_mom2face   face velocities along dT1, per EMF direction (K4-K6)
 //   k_emf        edge EMFs                                    (K7)
 //   k_flux       Lyon gas/magnetic fluxes, one per direction  (K8-K10)
-//   k_ct_b       CT face-flux update + cell B of the new state (K14,K15)
-//   k_gas        flux divergence, Reynolds, Maxwell/Boris, and the CFL
-//                min-reduction of the new state               (K11,K13,K16,K17)
+//   k_update     CT face-flux update, cell B of the new state, flux
+//                divergence, Reynolds, Maxwell/Boris and the CFL
+//                min-reduction of the new state  (K11,K13-K17)
 //   k_finalize   time bookkeeping and CalcDT checks (1 thread)
 //   k_bc_*       boundary conditions                          (K18)
-//   k_bflux2fld  cell B after the BCs                          (K19)
+//   k_bflux2fld  cell B after the BCs; active cells keep theirs,
+//                since no BC writes a face of an active cell  (K19)
 // State and oState are two buffers that swap roles every step, which
 // replaces CopyState (mhdgroup.F90:204-255): the update reads the current
 // buffer and writes every point of the other.
@@ -31,6 +32,53 @@ __device__ __forceinline__ bool in_box(const Box &b, int i, int j, int k)
     const int j = (b).j0 + blockIdx.y * BY + threadIdx.y; \
     const int k = (b).k0 + blockIdx.z;                  \
     if (i > (b).i1 || j > (b).j1) return;
+
+// XCD-aware tiling.  A launch covers box b with 64 x 4 tiles: 64 points
+// along i and 4 along axis ax1 (1 = j, 2 = k); tiles are numbered i-tile
+// fastest, then ax1-chunk, then the remaining axis.  The hardware hands
+// workgroup w to XCD w % 8, so logical tile L = (w % 8) * per + w / 8 gives
+// every XCD one contiguous run of tiles: tiles that share stencil planes
+// along ax1 then run back to back on the same L2.
+struct Tile {
+    Box b;
+    int ax1;
+    int n0, n1, n2, nb;
+};
+
+__device__ __forceinline__ bool tile_point(const Tile &t, int &i, int &j, int &k)
+{
+    const int w = blockIdx.x;
+    const int per = (t.nb + 7) >> 3;
+    const int L = (w & 7) * per + (w >> 3);
+    if (L >= t.nb) { i = j = k = 0x40000000; return false; }
+    const int t0 = L % t.n0;
+    const int r = L / t.n0;
+    const int t1 = r % t.n1;
+    const int t2 = r / t.n1;
+    i = t.b.i0 + t0 * BX + threadIdx.x;
+    const int a = t1 * BY + threadIdx.y;
+    if (t.ax1 == 2) { k = t.b.k0 + a; j = t.b.j0 + t2; }
+    else { j = t.b.j0 + a; k = t.b.k0 + t2; }
+    return i <= t.b.i1 && j <= t.b.j1 && k <= t.b.k1;
+}
+
+#define TILE_IJK(t)                  \
+    int i, j, k;                     \
+    if (!tile_point((t), i, j, k)) return;
+
+static inline Tile tile_of(const Box &b, int ax1)
+{
+    Tile t;
+    t.b = b;
+    t.ax1 = ax1;
+    const int ni = b.i1 - b.i0 + 1, nj = b.j1 - b.j0 + 1, nk = b.k1 - b.k0 + 1;
+    t.n0 = (ni + BX - 1) / BX;
+    t.n1 = ((ax1 == 2 ? nk : nj) + BY - 1) / BY;
+    t.n2 = (ax1 == 2 ? nj : nk);
+    t.nb = t.n0 * t.n1 * t.n2;
+    return t;
+}
+static inline dim3 tgrid(const Tile &t) { return dim3((unsigned)(((t.nb + 7) / 8) * 8)); }
 
 static inline dim3 grid_of(const Box &b)
 {
@@ -52,13 +100,13 @@ __device__ __forceinline__ double ab2(double q, double oq, double odt, double pd
 // the limiter state n.
 // Domain: all face points isg..ieg+1 (x) jsg..jeg+1 (x) ksg..keg+1.
 // ====================================================================
-__global__ __launch_bounds__(256) void k_predict(Geo g, MParm m, Box fb, const Clock *clk,
+__global__ __launch_bounds__(256) void k_predict(Geo g, MParm m, Tile fb, const Clock *clk,
                                                  const double *__restrict__ S,
                                                  const double *__restrict__ O,
                                                  const double *__restrict__ met,
                                                  double *__restrict__ H)
 {
-    THREAD_IJK(fb);
+    TILE_IJK(fb);
     const long N3 = g.N3;
     const long c = IX(g, i, j, k);
     const double st = clk->st, dtM = clk->dt;
@@ -119,12 +167,12 @@ __global__ __launch_bounds__(256) void k_predict(Geo g, MParm m, Box fb, const C
 // rho*v/rho to the dT faces.
 // ====================================================================
 template <int DT>
-__global__ __launch_bounds__(256) void k_mom2face(Geo g, MParm m, Box b,
+__global__ __launch_bounds__(256) void k_mom2face(Geo g, MParm m, Tile b,
                                                   const double *__restrict__ H,
                                                   const double *__restrict__ met,
                                                   double *__restrict__ Vf)
 {
-    THREAD_IJK(b);
+    TILE_IJK(b);
     const long N3 = g.N3;
     const long c = IX(g, i, j, k);
     const long s = STRIDE<DT>(g);
@@ -178,7 +226,7 @@ __device__ __forceinline__ void edge_lr(const Geo &g, const MParm &m, long c,
 }
 
 template <int ED, int RM, bool BG, bool BORIS>
-__global__ __launch_bounds__(256) void k_emf(Geo g, MParm m, Box b, const Clock *clk,
+__global__ __launch_bounds__(256) void k_emf(Geo g, MParm m, Tile b, const Clock *clk,
                                              const double *__restrict__ H,
                                              const double *__restrict__ S,
                                              const double *__restrict__ Vf,
@@ -186,7 +234,7 @@ __global__ __launch_bounds__(256) void k_emf(Geo g, MParm m, Box b, const Clock 
                                              const double *__restrict__ bg,
                                              double *__restrict__ E)
 {
-    THREAD_IJK(b);
+    TILE_IJK(b);
     constexpr int T1 = (ED == 0) ? 1 : (ED == 1) ? 2 : 0;
     constexpr int T2 = (ED == 0) ? 2 : (ED == 1) ? 0 : 1;
     // getNormalPlane (metric.F90:151-167)
@@ -248,7 +296,7 @@ __global__ __launch_bounds__(256) void k_emf(Geo g, MParm m, Box b, const Clock 
 // Interface fluxes of one direction (Fluxes, stress.F90:204-429)
 // ====================================================================
 template <int D, int RM, bool BG, bool BORIS>
-__global__ __launch_bounds__(256) void k_flux(Geo g, MParm m, Box b,
+__global__ __launch_bounds__(256) void k_flux(Geo g, MParm m, Tile b,
                                               const double *__restrict__ H,
                                               const double *__restrict__ S,
                                               const double *__restrict__ met,
@@ -256,7 +304,7 @@ __global__ __launch_bounds__(256) void k_flux(Geo g, MParm m, Box b,
                                               double *__restrict__ gflx,
                                               double *__restrict__ mflx)
 {
-    THREAD_IJK(b);
+    TILE_IJK(b);
     const long N3 = g.N3;
     const long c = IX(g, i, j, k);
     const long s = STRIDE<D>(g);
@@ -373,36 +421,6 @@ __device__ __forceinline__ double ct_face(const Geo &g, const EdgeBoxes &eb, con
     return old - dt * df;
 }
 
-__global__ __launch_bounds__(256) void k_ct_b(Geo g, MParm m, Box fb, Box ctb, EdgeBoxes eb,
-                                              const Clock *clk, const double *__restrict__ So,
-                                              double *__restrict__ Sn,
-                                              const double *__restrict__ met)
-{
-    THREAD_IJK(fb);
-    const long N3 = g.N3;
-    const long c = IX(g, i, j, k);
-    const double dt = clk->dtA;
-    const double *E = Sn + S_E * N3;
-    const double *mfo = So + S_MF * N3;
-    double f[3];
-    for (int d = 0; d < 3; d++) {
-        f[d] = m.doMHD ? ct_face(g, eb, ctb, E, mfo, dt, d, i, j, k) : mfo[d * N3 + c];
-        Sn[(S_MF + d) * N3 + c] = f[d];
-    }
-    if (i > g.ieg || j > g.jeg || k > g.keg) return;
-    double fIp, fJp, fKp;
-    if (m.doMHD) {
-        fIp = ct_face(g, eb, ctb, E, mfo, dt, 0, i + 1, j, k);
-        fJp = ct_face(g, eb, ctb, E, mfo, dt, 1, i, j + 1, k);
-        fKp = ct_face(g, eb, ctb, E, mfo, dt, 2, i, j, k + 1);
-    } else {
-        fIp = mfo[0 * N3 + c + 1]; fJp = mfo[1 * N3 + c + g.P]; fKp = mfo[2 * N3 + c + g.PP];
-    }
-    double B[3];
-    cell_b(g, met, c, f[0], f[1], f[2], fIp, fJp, fKp, B);
-    for (int x = 0; x < 3; x++) Sn[(S_B + x) * N3 + c] = B[x];
-}
-
 // ====================================================================
 // Block min-reduction of positive doubles into clk->dtBits.
 // ====================================================================
@@ -421,30 +439,61 @@ __device__ __forceinline__ void block_min_dt(double v, Clock *clk)
 }
 
 // ====================================================================
-// Gas update: Flux2Deltas (stress.F90:152-194), Reynolds (mhdgroup.F90:
-// 114-125, with Model%dt), Maxwell/Boris (171-195, 299-441); ghost cells
-// copy the old state (CopyState).  Cells of the CalcDT domain contribute
-// CellDT of the new state to the dt reduction (step.F90:91-101).
-// Domain: all cells (block covers every cell; out-of-range threads feed
-// +HUGE to the reduction).
+// applyUpdate (mhdgroup.F90:79-258) in one pass over all face points:
+//  * CT: E2Flux (fields.F90:18-51) on the point's three faces; faces
+//    outside is..ie+1 x js..je+1 x ks..ke+1 keep the old flux, edges
+//    outside the active EMF boxes read as zero (E is zeroed before the
+//    active edges are filled, fields.F90:89);
+//  * bFlux2Fld of the new state (the three upper faces are re-derived);
+//  * Flux2Deltas (stress.F90:152-194), Reynolds with Model%dt (114-125)
+//    and Maxwell/Boris (171-195, 299-441) on active cells; ghost cells copy
+//    the old state (CopyState);
+//  * CellDT of the new state on the CalcDT domain, min-reduced
+//    (step.F90:91-101).
+// Every thread reaches the block reduction.
 // ====================================================================
 template <bool BG, bool BORIS>
-__global__ __launch_bounds__(256) void k_gas(Geo g, MParm m, Box cb, Box ab, Box dtb,
-                                             Clock *clk, const double *__restrict__ So,
-                                             double *__restrict__ Sn,
-                                             const double *__restrict__ gflx,
-                                             const double *__restrict__ mflx,
-                                             const double *__restrict__ met,
-                                             const double *__restrict__ bg)
+__global__ __launch_bounds__(256) void k_update(Geo g, MParm m, Tile ft, Box ctb, EdgeBoxes eb,
+                                                Box ab, Box dtb, Clock *clk,
+                                                const double *__restrict__ So,
+                                                double *__restrict__ Sn,
+                                                const double *__restrict__ gflx,
+                                                const double *__restrict__ mflx,
+                                                const double *__restrict__ met,
+                                                const double *__restrict__ bg)
 {
-    const int i = cb.i0 + blockIdx.x * BX + threadIdx.x;
-    const int j = cb.j0 + blockIdx.y * BY + threadIdx.y;
-    const int k = cb.k0 + blockIdx.z;
-    const bool live = (i <= cb.i1 && j <= cb.j1);
+    int i, j, k;
+    const bool live = tile_point(ft, i, j, k);
     double dtc = GAM_HUGE;
+    const long N3 = g.N3;
+    const long c = IX(g, i, j, k);
+    const bool cell = live && i <= g.ieg && j <= g.jeg && k <= g.keg;
+    double Bn[3] = {0.0, 0.0, 0.0};
     if (live) {
-        const long N3 = g.N3;
-        const long c = IX(g, i, j, k);
+        const double dt = clk->dtA;
+        const double *E = Sn + S_E * N3;
+        const double *mfo = So + S_MF * N3;
+        double f[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            f[d] = m.doMHD ? ct_face(g, eb, ctb, E, mfo, dt, d, i, j, k) : mfo[d * N3 + c];
+            Sn[(S_MF + d) * N3 + c] = f[d];
+        }
+        if (cell) {
+            double fIp, fJp, fKp;
+            if (m.doMHD) {
+                fIp = ct_face(g, eb, ctb, E, mfo, dt, 0, i + 1, j, k);
+                fJp = ct_face(g, eb, ctb, E, mfo, dt, 1, i, j + 1, k);
+                fKp = ct_face(g, eb, ctb, E, mfo, dt, 2, i, j, k + 1);
+            } else {
+                fIp = mfo[0 * N3 + c + 1]; fJp = mfo[1 * N3 + c + g.P]; fKp = mfo[2 * N3 + c + g.PP];
+            }
+            cell_b(g, met, c, f[0], f[1], f[2], fIp, fJp, fKp, Bn);
+#pragma unroll
+            for (int x = 0; x < 3; x++) Sn[(S_B + x) * N3 + c] = Bn[x];
+        }
+    }
+    if (cell) {
         double U[5];
 #pragma unroll
         for (int v = 0; v < 5; v++) U[v] = So[(S_GAS + v) * N3 + c];
@@ -469,7 +518,7 @@ __global__ __launch_bounds__(256) void k_gas(Geo g, MParm m, Box cb, Box ab, Box
                                 + mv[6 * N3 + c] - mv[6 * N3 + c + g.PP]) / dV;
                     if (BG) r = r + bg[(B_DPB0 + x) * N3 + c];
                     dM[x] = r;
-                    B[x] = Sn[(S_B + x) * N3 + c];
+                    B[x] = Bn[x];
                     oB[x] = So[(S_B + x) * N3 + c];
                     if (BG) { B[x] = B[x] + bg[(B_B0 + x) * N3 + c]; oB[x] = oB[x] + bg[(B_B0 + x) * N3 + c]; }
                 }
@@ -528,7 +577,7 @@ __global__ __launch_bounds__(256) void k_gas(Geo g, MParm m, Box cb, Box ab, Box
 #pragma unroll
         for (int v = 0; v < 5; v++) Sn[(S_GAS + v) * N3 + c] = U[v];
         if (in_box(dtb, i, j, k)) {
-            double bx = Sn[(S_B + 0) * N3 + c], by = Sn[(S_B + 1) * N3 + c], bz = Sn[(S_B + 2) * N3 + c];
+            double bx = Bn[0], by = Bn[1], bz = Bn[2];
             if (BG) { bx = bx + bg[(B_B0 + 0) * N3 + c]; by = by + bg[(B_B0 + 1) * N3 + c]; bz = bz + bg[(B_B0 + 2) * N3 + c]; }
             const double dl = dmin(dmin(met[M_DI * N3 + c], met[M_DJ * N3 + c]), met[M_DK * N3 + c]);
             dtc = cell_dt<BORIS>(m, U, bx, by, bz, dl);
@@ -540,16 +589,15 @@ __global__ __launch_bounds__(256) void k_gas(Geo g, MParm m, Box cb, Box ab, Box
 
 // CalcDT over the DT box on a state (standalone CalcDT, step.F90:58-101)
 template <bool BG, bool BORIS>
-__global__ __launch_bounds__(256) void k_calc_dt(Geo g, MParm m, Box dtb, Clock *clk,
+__global__ __launch_bounds__(256) void k_calc_dt(Geo g, MParm m, Tile dtb, Clock *clk,
                                                  const double *__restrict__ S,
                                                  const double *__restrict__ met,
                                                  const double *__restrict__ bg)
 {
-    const int i = dtb.i0 + blockIdx.x * BX + threadIdx.x;
-    const int j = dtb.j0 + blockIdx.y * BY + threadIdx.y;
-    const int k = dtb.k0 + blockIdx.z;
+    int i, j, k;
+    const bool live = tile_point(dtb, i, j, k);
     double dtc = GAM_HUGE;
-    if (i <= dtb.i1 && j <= dtb.j1) {
+    if (live) {
         const long N3 = g.N3, c = IX(g, i, j, k);
         double U[5];
         for (int v = 0; v < 5; v++) U[v] = S[(S_GAS + v) * N3 + c];
@@ -602,11 +650,12 @@ __global__ void k_finalize(MParm m, Clock *clk, int mode)
 // ====================================================================
 // bFlux2Fld (gridutils.F90:281-301) over all cells
 // ====================================================================
-__global__ __launch_bounds__(256) void k_bflux2fld(Geo g, Box cb, const double *__restrict__ mf,
+__global__ __launch_bounds__(256) void k_bflux2fld(Geo g, Tile cb, Box skip, const double *__restrict__ mf,
                                                    double *__restrict__ B,
                                                    const double *__restrict__ met)
 {
-    THREAD_IJK(cb);
+    TILE_IJK(cb);
+    if (in_box(skip, i, j, k)) return;
     const long N3 = g.N3, c = IX(g, i, j, k);
     double b[3];
     cell_b(g, met, c, mf[c], mf[N3 + c], mf[2 * N3 + c], mf[c + 1], mf[N3 + c + g.P],
@@ -987,15 +1036,17 @@ __global__ void k_mask_e(Geo g, Box all, EdgeBoxes eb, double *__restrict__ E)
 void launch_predict(hipStream_t s, const Geo &g, const MParm &m, const Box &fb, const Clock *clk,
                     const double *S, const double *O, const double *met, double *H)
 {
-    LAUNCH(k_predict, grid_of(fb), dim3(BX, BY), s, g, m, fb, clk, S, O, met, H);
+    const Tile t = tile_of(fb, 1);
+    LAUNCH(k_predict, tgrid(t), dim3(BX, BY), s, g, m, t, clk, S, O, met, H);
 }
 
 void launch_mom2face(hipStream_t s, int dT, const Geo &g, const MParm &m, const Box &b,
                      const double *H, const double *met, double *Vf)
 {
-    if (dT == 0) LAUNCH(k_mom2face<0>, grid_of(b), dim3(BX, BY), s, g, m, b, H, met, Vf);
-    else if (dT == 1) LAUNCH(k_mom2face<1>, grid_of(b), dim3(BX, BY), s, g, m, b, H, met, Vf);
-    else LAUNCH(k_mom2face<2>, grid_of(b), dim3(BX, BY), s, g, m, b, H, met, Vf);
+    const Tile t = tile_of(b, dT == 2 ? 2 : 1);
+    if (dT == 0) LAUNCH(k_mom2face<0>, tgrid(t), dim3(BX, BY), s, g, m, t, H, met, Vf);
+    else if (dT == 1) LAUNCH(k_mom2face<1>, tgrid(t), dim3(BX, BY), s, g, m, t, H, met, Vf);
+    else LAUNCH(k_mom2face<2>, tgrid(t), dim3(BX, BY), s, g, m, t, H, met, Vf);
 }
 
 template <int ED, int RM, bool BG, bool BO>
@@ -1003,7 +1054,9 @@ static void emf_t(hipStream_t s, const Geo &g, const MParm &m, const Box &b, con
                   const double *H, const double *S, const double *Vf, const double *met,
                   const double *bg, double *E)
 {
-    LAUNCH((k_emf<ED, RM, BG, BO>), grid_of(b), dim3(BX, BY), s, g, m, b, clk, H, S, Vf, met, bg, E);
+    // stencils run along T2 and T1; tile along k whenever one of them is k
+    const Tile t = tile_of(b, ED == 2 ? 1 : 2);
+    LAUNCH((k_emf<ED, RM, BG, BO>), tgrid(t), dim3(BX, BY), s, g, m, t, clk, H, S, Vf, met, bg, E);
 }
 
 template <int ED>
@@ -1037,7 +1090,8 @@ template <int D, int RM, bool BG, bool BO>
 static void flux_t(hipStream_t s, const Geo &g, const MParm &m, const Box &b, const double *H,
                    const double *S, const double *met, const double *bg, double *gf, double *mf)
 {
-    LAUNCH((k_flux<D, RM, BG, BO>), grid_of(b), dim3(BX, BY), s, g, m, b, H, S, met, bg, gf, mf);
+    const Tile t = tile_of(b, D == 2 ? 2 : 1);
+    LAUNCH((k_flux<D, RM, BG, BO>), tgrid(t), dim3(BX, BY), s, g, m, t, H, S, met, bg, gf, mf);
 }
 
 template <int D>
@@ -1065,24 +1119,19 @@ void launch_flux(hipStream_t s, int d, const Geo &g, const MParm &m, const Box &
     else flux_d<2>(s, g, m, b, H, S, met, bg, gf, mf);
 }
 
-void launch_ct_b(hipStream_t s, const Geo &g, const MParm &m, const Box &fb, const Box &ctb,
-                 const Box *eboxes, const Clock *clk, const double *So, double *Sn, const double *met)
+void launch_update(hipStream_t s, const Geo &g, const MParm &m, const Box &fb, const Box &ctb,
+                   const Box *eboxes, const Box &ab, const Box &dtb, Clock *clk, const double *So,
+                   double *Sn, const double *gf, const double *mf, const double *met, const double *bg)
 {
     EdgeBoxes eb;
     for (int d = 0; d < 3; d++) eb.e[d] = eboxes[d];
-    LAUNCH(k_ct_b, grid_of(fb), dim3(BX, BY), s, g, m, fb, ctb, eb, clk, So, Sn, met);
-}
-
-void launch_gas(hipStream_t s, const Geo &g, const MParm &m, const Box &cb, const Box &ab,
-                const Box &dtb, Clock *clk, const double *So, double *Sn, const double *gf,
-                const double *mf, const double *met, const double *bg)
-{
     const int sel = (m.doBG ? 2 : 0) + (m.doBoris ? 1 : 0);
+    const Tile t = tile_of(fb, 1);
     switch (sel) {
-    case 0: LAUNCH((k_gas<false, false>), grid_of(cb), dim3(BX, BY), s, g, m, cb, ab, dtb, clk, So, Sn, gf, mf, met, bg); break;
-    case 1: LAUNCH((k_gas<false, true>), grid_of(cb), dim3(BX, BY), s, g, m, cb, ab, dtb, clk, So, Sn, gf, mf, met, bg); break;
-    case 2: LAUNCH((k_gas<true, false>), grid_of(cb), dim3(BX, BY), s, g, m, cb, ab, dtb, clk, So, Sn, gf, mf, met, bg); break;
-    default: LAUNCH((k_gas<true, true>), grid_of(cb), dim3(BX, BY), s, g, m, cb, ab, dtb, clk, So, Sn, gf, mf, met, bg); break;
+    case 0: LAUNCH((k_update<false, false>), tgrid(t), dim3(BX, BY), s, g, m, t, ctb, eb, ab, dtb, clk, So, Sn, gf, mf, met, bg); break;
+    case 1: LAUNCH((k_update<false, true>), tgrid(t), dim3(BX, BY), s, g, m, t, ctb, eb, ab, dtb, clk, So, Sn, gf, mf, met, bg); break;
+    case 2: LAUNCH((k_update<true, false>), tgrid(t), dim3(BX, BY), s, g, m, t, ctb, eb, ab, dtb, clk, So, Sn, gf, mf, met, bg); break;
+    default: LAUNCH((k_update<true, true>), tgrid(t), dim3(BX, BY), s, g, m, t, ctb, eb, ab, dtb, clk, So, Sn, gf, mf, met, bg); break;
     }
 }
 
@@ -1090,11 +1139,12 @@ void launch_calc_dt(hipStream_t s, const Geo &g, const MParm &m, const Box &dtb,
                     const double *S, const double *met, const double *bg)
 {
     const int sel = (m.doBG ? 2 : 0) + (m.doBoris ? 1 : 0);
+    const Tile t = tile_of(dtb, 1);
     switch (sel) {
-    case 0: LAUNCH((k_calc_dt<false, false>), grid_of(dtb), dim3(BX, BY), s, g, m, dtb, clk, S, met, bg); break;
-    case 1: LAUNCH((k_calc_dt<false, true>), grid_of(dtb), dim3(BX, BY), s, g, m, dtb, clk, S, met, bg); break;
-    case 2: LAUNCH((k_calc_dt<true, false>), grid_of(dtb), dim3(BX, BY), s, g, m, dtb, clk, S, met, bg); break;
-    default: LAUNCH((k_calc_dt<true, true>), grid_of(dtb), dim3(BX, BY), s, g, m, dtb, clk, S, met, bg); break;
+    case 0: LAUNCH((k_calc_dt<false, false>), tgrid(t), dim3(BX, BY), s, g, m, t, clk, S, met, bg); break;
+    case 1: LAUNCH((k_calc_dt<false, true>), tgrid(t), dim3(BX, BY), s, g, m, t, clk, S, met, bg); break;
+    case 2: LAUNCH((k_calc_dt<true, false>), tgrid(t), dim3(BX, BY), s, g, m, t, clk, S, met, bg); break;
+    default: LAUNCH((k_calc_dt<true, true>), tgrid(t), dim3(BX, BY), s, g, m, t, clk, S, met, bg); break;
     }
 }
 
@@ -1103,10 +1153,11 @@ void launch_finalize(hipStream_t s, const MParm &m, Clock *clk, int mode)
     LAUNCH(k_finalize, dim3(1), dim3(64), s, m, clk, mode);
 }
 
-void launch_bflux2fld(hipStream_t s, const Geo &g, const Box &cb, const double *mf, double *B,
-                      const double *met)
+void launch_bflux2fld(hipStream_t s, const Geo &g, const Box &cb, const Box &skip, const double *mf,
+                      double *B, const double *met)
 {
-    LAUNCH(k_bflux2fld, grid_of(cb), dim3(BX, BY), s, g, cb, mf, B, met);
+    const Tile t = tile_of(cb, 1);
+    LAUNCH(k_bflux2fld, tgrid(t), dim3(BX, BY), s, g, t, skip, mf, B, met);
 }
 
 static inline dim3 grid2(const Box &t) { return dim3((unsigned)((t.i1 - t.i0 + 64) / 64), (unsigned)(t.j1 - t.j0 + 1)); }
