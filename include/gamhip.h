@@ -143,7 +143,7 @@ int gam_get_time(gam_ctx *ctx, double *t, double *dt, int *ts);
  * EnforceBCs, oState=State, dt=CalcDT, oState%time=t-dt. */
 int gam_init_run(gam_ctx *ctx, double *dt);
 
-/* Operator-level entry points */
+/* Operator-level entry points of the GAMERA advance step */
 int gam_advance_mhd(gam_ctx *ctx, double dt);      /* AdvanceMHD   mhdgroup.F90:16-76 */
 int gam_calc_dt(gam_ctx *ctx, double *dt);         /* CalcDT       step.F90:58-168 */
 int gam_enforce_bcs(gam_ctx *ctx);                 /* EnforceBCs   step.F90:17-55  */
@@ -155,6 +155,10 @@ int gam_step(gam_ctx *ctx, double *dt_next);
 /* nsteps x stepGamera without host synchronisation between steps; returns
  * the first CalcDT failure code seen, if any. */
 int gam_run(gam_ctx *ctx, int nsteps);
+/* gamAdvanceModel (gamtypessub.F90:74-90): one stepGamera, then more while
+ * Model%t < t0 + dt.  *nsteps (may be NULL) receives the step count.
+ * Synchronises after every step (the loop tests Model%t on the host). */
+int gam_advance_model(gam_ctx *ctx, double dt, int *nsteps);
 int gam_sync(gam_ctx *ctx);
 
 /* Total |div B| over active cells, background included

@@ -143,6 +143,8 @@ extern "C" int gam_create(gam_ctx **out, const gam_dims *d, const gam_model *m)
     p.gamma = m->gamma; p.CFL = m->CFL; p.Vd0 = m->Vd0; p.Ca = m->Ca; p.cHogH = m->cHogH;
     p.cHogM = m->cHogM; p.pdmb = m->pdmb; p.dFloor = m->dFloor; p.pFloor = m->pFloor;
     p.tFin = m->tFin; p.dt0 = m->dt0; p.limDT0 = m->limDT0;
+    p.igm1 = 1.0 / (m->gamma - 1.0);
+    p.iCa2 = 1.0 / (m->Ca * m->Ca);
     p.doMHD = m->doMHD; p.doBoris = m->doBoris; p.doBG = m->doBackground; p.doLFM = m->doLFMLim;
     p.rmeth = m->rmeth;
 
@@ -262,6 +264,9 @@ extern "C" int gam_set_grid(gam_ctx *c, const gam_grid_arrays *a)
     if ((rc = upload(c, a->Teb, 1, 12, c->met + M_TEB * N3))) return rc;
     if ((rc = upload(c, a->face, 1, 3, c->met + M_FACE * N3))) return rc;
     if ((rc = upload(c, a->edge, 1, 3, c->met + M_EDGE * N3))) return rc;
+    launch_inv_face(c->stream, c->g, c->met);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
     c->haveGrid = 1;
     return GAM_OK;
 }
@@ -650,6 +655,31 @@ extern "C" int gam_run(gam_ctx *c, int nsteps)
     if ((rc = read_clock(c, &ck))) return rc;
     if ((rc = prof_flush(c))) return rc;
     return status_code(ck.status);
+}
+
+// gamAdvanceModel (gamtypessub.F90:74-90): one stepGamera, then more while
+// Model%t < t0 + dt.  The loop condition needs Model%t, so every step ends
+// with a 64-byte clock read; a CalcDT failure ends the loop with its code,
+// where the reference prints and stops (step.F90:136-151).
+extern "C" int gam_advance_model(gam_ctx *c, double dt, int *nsteps)
+{
+    int rc = check_ready(c, "gam_advance_model");
+    if (rc) return rc;
+    if (!(dt >= 0.0)) return fail(GAM_E_ARG, "gam_advance_model: dt must be >= 0");
+    Clock ck;
+    if ((rc = read_clock(c, &ck))) return rc;
+    const double target = ck.t + dt;
+    int n = 0;
+    do {
+        enq_step(c);
+        HIPCHK(hipGetLastError());
+        n++;
+        if ((rc = read_clock(c, &ck))) break;
+        if ((rc = status_code(ck.lastStatus))) break;
+    } while (ck.t < target);
+    if (nsteps) *nsteps = n;
+    if (rc) return rc;
+    return prof_flush(c);
 }
 
 extern "C" int gam_sync(gam_ctx *c)

@@ -6,16 +6,22 @@
 #include "gam_common.h"
 
 #define GAM_PI 3.141592653589793115997963468544185161590576171875
+#define GAM_RSQRT_PI 0.56418958354775628694807945156077258584405062932900  // 1/sqrt(pi)
 
 enum { DEN = 0, MOMX = 1, MOMY = 2, MOMZ = 3, ENER = 4 };
 enum { PRS = 4 };
 
 // ---------------------------------------------------------- conversions
+// Divisions by a common divisor are done as one reciprocal and products
+// throughout this file (last-bit differences from the reference's
+// per-component divisions, well inside the FP64 parity tolerance).
+
 // CellC2P (gamutils.F90:301-321): velocity from the unfloored density.
 __device__ __forceinline__ void c2p_cell(const MParm &m, const double *U, double *W)
 {
     double rho = U[DEN];
-    double vx = U[MOMX] / rho, vy = U[MOMY] / rho, vz = U[MOMZ] / rho;
+    double ir = 1.0 / rho;
+    double vx = U[MOMX] * ir, vy = U[MOMY] * ir, vz = U[MOMZ] * ir;
     double ke = 0.5 * rho * (vx * vx + vy * vy + vz * vz);
     W[DEN] = dmax(rho, m.dFloor);
     W[1] = vx; W[2] = vy; W[3] = vz;
@@ -27,7 +33,7 @@ __device__ __forceinline__ void p2c_cell(const MParm &m, const double *W, double
 {
     double rho = dmax(W[DEN], m.dFloor);
     double vx = W[1], vy = W[2], vz = W[3];
-    double ie = dmax(W[PRS], m.pFloor) / (m.gamma - 1);
+    double ie = dmax(W[PRS], m.pFloor) * m.igm1;
     double ke = 0.5 * rho * (vx * vx + vy * vy + vz * vz);
     U[DEN] = rho; U[MOMX] = rho * vx; U[MOMY] = rho * vy; U[MOMZ] = rho * vz;
     U[ENER] = ie + ke;
@@ -37,9 +43,10 @@ __device__ __forceinline__ void p2c_cell(const MParm &m, const double *W, double
 __device__ __forceinline__ void c2p_block(const MParm &m, const double *U, double *W)
 {
     double d = dmax(U[DEN], m.dFloor);
+    double id = 1.0 / d;
     double mx = U[MOMX], my = U[MOMY], mz = U[MOMZ];
-    double ke = 0.5 * (mx * mx + my * my + mz * mz) / d;
-    W[DEN] = d; W[1] = mx / d; W[2] = my / d; W[3] = mz / d;
+    double ke = 0.5 * (mx * mx + my * my + mz * mz) * id;
+    W[DEN] = d; W[1] = mx * id; W[2] = my * id; W[3] = mz * id;
     W[PRS] = dmax((m.gamma - 1) * (U[ENER] - ke), m.pFloor);
 }
 
@@ -69,9 +76,9 @@ __device__ __forceinline__ void pdm_pair(double b, double v0, double v1, double 
 // Volume-weighted reconstruction + limiting for one variable at one face:
 // Cen8LRs (recon.F90:125-158) or Up7LRs (162-198).  vol[8] the weights
 // (cells i-4..i+3), Vi their central-8 interpolant, q[8] the values,
-// nq[4] the limiter values (cells i-2..i+1).
+// nq[4] the limiter values (cells i-2..i+1).  rVi = 1/Vi.
 template <int RM>
-__device__ __forceinline__ void recon_lr(double pdmb, const double *vol, double Vi,
+__device__ __forceinline__ void recon_lr(double pdmb, const double *vol, double rVi,
                                          const double *nq, const double *q,
                                          double &ql, double &qr)
 {
@@ -83,10 +90,10 @@ __device__ __forceinline__ void recon_lr(double pdmb, const double *vol, double 
                  + qv[4] * UP7(4) + qv[5] * UP7(5) + qv[6] * UP7(6);
         double r = qv[7] * UP7(0) + qv[6] * UP7(1) + qv[5] * UP7(2) + qv[4] * UP7(3)
                  + qv[3] * UP7(4) + qv[2] * UP7(5) + qv[1] * UP7(6);
-        ql = pdm_one(pdmb, nq[0], nq[1], nq[2], l / Vi);
-        qr = pdm_one(pdmb, nq[3], nq[2], nq[1], r / Vi);
+        ql = pdm_one(pdmb, nq[0], nq[1], nq[2], l * rVi);
+        qr = pdm_one(pdmb, nq[3], nq[2], nq[1], r * rVi);
     } else {
-        double qi = wsum8([&](int n) { return qv[n]; }) / Vi;
+        double qi = wsum8([&](int n) { return qv[n]; }) * rVi;
         pdm_pair(pdmb, nq[0], nq[1], nq[2], nq[3], qi, ql, qr);
     }
 }
@@ -94,17 +101,19 @@ __device__ __forceinline__ void recon_lr(double pdmb, const double *vol, double 
 // ---------------------------------------------------------- fluxes
 // One side (s = -1 left, +1 right) of the Lyon gas-kinetic flux,
 // GasKinFlux (stress.F90:432-511).  T = face triad (N, T1, T2).
-__device__ __forceinline__ void gas_kin(double gam, double s, const double *W, const double *T,
+// sqrt(lam) and 1/sqrt(pi*lam) come from one reciprocal square root.
+__device__ __forceinline__ void gas_kin(double igm1, double s, const double *W, const double *T,
                                         double *gF, double *dW)
 {
     double d = W[DEN], p = W[PRS], vx = W[1], vy = W[2], vz = W[3];
-    double e = 0.5 * d * (vx * vx + vy * vy + vz * vz) + p / (gam - 1);
+    double e = 0.5 * d * (vx * vx + vy * vy + vz * vz) + p * igm1;
     double lam = d / (2 * p);
+    double rl = rsqrt(lam);
     double vn = T[0] * vx + T[1] * vy + T[2] * vz;
     double vt1 = T[3] * vx + T[4] * vy + T[5] * vz;
     double vt2 = T[6] * vx + T[7] * vy + T[8] * vz;
-    double vn0 = 0.5 * erfc(s * sqrt(lam) * vn);
-    double vn1 = vn * vn0 - 0.5 * s * exp(-(lam * vn * vn)) / sqrt(GAM_PI * lam);
+    double vn0 = 0.5 * erfc(s * (lam * rl) * vn);
+    double vn1 = vn * vn0 - 0.5 * s * exp(-(lam * vn * vn)) * (rl * GAM_RSQRT_PI);
     gF[DEN] = gF[DEN] + d * vn1;
     gF[ENER] = gF[ENER] + (e + 0.5 * p) * vn1 + 0.5 * p * vn0 * vn;
     double fn = d * vn * vn1 + p * vn0;
@@ -127,22 +136,23 @@ __device__ __forceinline__ void mag_kin(const MParm &m, double s, const double *
                                         double *mF, double &bbD, double &VaD)
 {
     double d = W[DEN], p = W[PRS];
+    double id = 1.0 / d;
     double bx = B[0], by = B[1], bz = B[2];
     double dPb = 0.5 * (bx * bx + by * by + bz * bz);
     double va2;
     if (BG) {
         double tx = bx + B0[0], ty = by + B0[1], tz = bz + B0[2];
         double tt = tx * tx + ty * ty + tz * tz;
-        va2 = tt / d;
+        va2 = tt * id;
         bbD = bbD + 0.5 * tt;
     } else {
-        va2 = 2 * dPb / d;
+        va2 = 2 * dPb * id;
         bbD = bbD + 0.5 * (bx * bx + by * by + bz * bz);
     }
-    double vac2 = BORIS ? va2 / (1 + (va2 / (m.Ca * m.Ca))) : va2;
+    double vac2 = BORIS ? va2 / (1 + va2 * m.iCa2) : va2;
     double nx = T[0], ny = T[1], nz = T[2];
     double vn = nx * W[1] + ny * W[2] + nz * W[3];
-    double lam = 1 / (2 * p / d + vac2);
+    double lam = 1 / (2 * p * id + vac2);
     double vn0 = 0.5 * erfc(s * sqrt(lam) * vn);
     VaD = VaD + 0.5 * sqrt(vac2);
     mF[0] = mF[0] + vn0 * (dPb * nx - bx * Bn);
@@ -163,7 +173,7 @@ __device__ __forceinline__ void cell_b(const Geo &g, const double *met, long c,
                                        double fIp, double fJp, double fKp, double *B)
 {
     const long N3 = g.N3;
-    double dV = met[M_VOL * N3 + c];
+    double idV = 1.0 / met[M_VOL * N3 + c];
     double div = fIp - fI + fJp - fJ + fKp - fK;
 #pragma unroll
     for (int x = 0; x < 3; x++) {
@@ -173,8 +183,8 @@ __device__ __forceinline__ void cell_b(const Geo &g, const double *met, long c,
         double xIp = met[(M_XFC + x + 0) * N3 + c + 1];
         double xJp = met[(M_XFC + x + 3) * N3 + c + g.P];
         double xKp = met[(M_XFC + x + 6) * N3 + c + g.PP];
-        double b = (fIp * xIp + fJp * xJp + fKp * xKp - fI * xI - fJ * xJ - fK * xK) / dV;
-        B[x] = b - div * met[(M_XYZCC + x) * N3 + c] / dV;
+        double b = (fIp * xIp + fJp * xJp + fKp * xKp - fI * xI - fJ * xJ - fK * xK) * idV;
+        B[x] = b - div * met[(M_XYZCC + x) * N3 + c] * idV;
     }
 }
 
@@ -185,11 +195,12 @@ __device__ __forceinline__ double cell_dt(const MParm &m, const double *U, doubl
                                           double bz, double dl)
 {
     double rho = U[DEN];
-    double vx = U[MOMX] / rho, vy = U[MOMY] / rho, vz = U[MOMZ] / rho;
+    double ir = 1.0 / rho;
+    double vx = U[MOMX] * ir, vy = U[MOMY] * ir, vz = U[MOMZ] * ir;
     double mv = sqrt(vx * vx + vy * vy + vz * vz);
     double ke = 0.5 * rho * (mv * mv);
     double p = (m.gamma - 1) * (U[ENER] - ke);
-    double cs = sqrt(m.gamma * p / rho);
+    double cs = sqrt(m.gamma * p * ir);
     double va = 0.0;
     if (m.doMHD) {
         va = sqrt(bx * bx + by * by + bz * bz) / sqrt(rho);

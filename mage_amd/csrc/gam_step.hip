@@ -86,11 +86,11 @@ static inline dim3 grid_of(const Box &b)
                 (unsigned)(b.k1 - b.k0 + 1));
 }
 
-// AB2 extrapolation (ExtrapAB2, mhdgroup.F90:709-717)
-__device__ __forceinline__ double ab2(double q, double oq, double odt, double pdt)
+// AB2 extrapolation (ExtrapAB2, mhdgroup.F90:709-717): q + pdt*(q-oq)/odt
+// with w = pdt/odt formed once per thread.
+__device__ __forceinline__ double ab2(double q, double oq, double w)
 {
-    double dq = (q - oq) / odt;
-    return q + pdt * dq;
+    return q + w * (q - oq);
 }
 
 // ====================================================================
@@ -114,11 +114,12 @@ __global__ __launch_bounds__(256) void k_predict(Geo g, MParm m, Tile fb, const 
     if (fabs(ost - st) < GAM_TINY) ost = st - dtM;   // mhdgroup.F90:26-28
     const double odt = st - ost;
     const double pdt = 0.5 * clk->dtA;
+    const double w = pdt / odt;
 
     double f[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        f[d] = m.doMHD ? ab2(S[(S_MF + d) * N3 + c], O[(S_MF + d) * N3 + c], odt, pdt)
+        f[d] = m.doMHD ? ab2(S[(S_MF + d) * N3 + c], O[(S_MF + d) * N3 + c], w)
                        : H[(H_MF + d) * N3 + c];
         H[(H_MF + d) * N3 + c] = f[d];
     }
@@ -127,9 +128,9 @@ __global__ __launch_bounds__(256) void k_predict(Geo g, MParm m, Tile fb, const 
     // half-state cell B from the six extrapolated face fluxes
     double fIp, fJp, fKp;
     if (m.doMHD) {
-        fIp = ab2(S[(S_MF + 0) * N3 + c + 1], O[(S_MF + 0) * N3 + c + 1], odt, pdt);
-        fJp = ab2(S[(S_MF + 1) * N3 + c + g.P], O[(S_MF + 1) * N3 + c + g.P], odt, pdt);
-        fKp = ab2(S[(S_MF + 2) * N3 + c + g.PP], O[(S_MF + 2) * N3 + c + g.PP], odt, pdt);
+        fIp = ab2(S[(S_MF + 0) * N3 + c + 1], O[(S_MF + 0) * N3 + c + 1], w);
+        fJp = ab2(S[(S_MF + 1) * N3 + c + g.P], O[(S_MF + 1) * N3 + c + g.P], w);
+        fKp = ab2(S[(S_MF + 2) * N3 + c + g.PP], O[(S_MF + 2) * N3 + c + g.PP], w);
     } else {
         fIp = H[(H_MF + 0) * N3 + c + 1];
         fJp = H[(H_MF + 1) * N3 + c + g.P];
@@ -147,7 +148,7 @@ __global__ __launch_bounds__(256) void k_predict(Geo g, MParm m, Tile fb, const 
     c2p_cell(m, oQ, oW);
     c2p_cell(m, Q, W);
 #pragma unroll
-    for (int v = 0; v < 5; v++) pW[v] = ab2(W[v], oW[v], odt, pdt);
+    for (int v = 0; v < 5; v++) pW[v] = ab2(W[v], oW[v], w);
     pW[DEN] = dmax(pW[DEN], m.dFloor);
     pW[PRS] = dmax(pW[PRS], m.pFloor);
     p2c_cell(m, pW, pQ);
@@ -164,7 +165,8 @@ __global__ __launch_bounds__(256) void k_predict(Geo g, MParm m, Tile fb, const 
 
 // ====================================================================
 // Mom2Face (fields.F90:235-285): volume-weighted 8-point interpolation of
-// rho*v/rho to the dT faces.
+// rho*v/max(rho,dFloor) to the dT faces.  The velocities rho*v/max(rho,
+// dFloor) are the half-state primitives the predictor already stored.
 // ====================================================================
 template <int DT>
 __global__ __launch_bounds__(256) void k_mom2face(Geo g, MParm m, Tile b,
@@ -181,14 +183,13 @@ __global__ __launch_bounds__(256) void k_mom2face(Geo g, MParm m, Tile b,
     for (int n = 0; n < 8; n++) {
         const long q = c + (n - 4) * s;
         double v = met[M_VOL * N3 + q];
-        double dn = dmax(H[(H_GAS + DEN) * N3 + q], m.dFloor);
         vol[n] = v;
 #pragma unroll
-        for (int d = 0; d < 3; d++) vel[d][n] = v * H[(H_GAS + MOMX + d) * N3 + q] / dn;
+        for (int d = 0; d < 3; d++) vel[d][n] = v * H[(H_WH + 1 + d) * N3 + q];
     }
-    double dV = dmax(wsum8([&](int n) { return vol[n]; }), GAM_TINY);
+    double rdV = 1.0 / dmax(wsum8([&](int n) { return vol[n]; }), GAM_TINY);
 #pragma unroll
-    for (int d = 0; d < 3; d++) Vf[d * N3 + c] = wsum8([&](int n) { return vel[d][n]; }) / dV;
+    for (int d = 0; d < 3; d++) Vf[d * N3 + c] = wsum8([&](int n) { return vel[d][n]; }) * rdV;
 }
 
 // ====================================================================
@@ -212,14 +213,12 @@ __device__ __forceinline__ void edge_lr(const Geo &g, const MParm &m, long c,
         A[n] = met[(M_FACE + FD) * N3 + p];
         q[n] = mfH[FD * N3 + p] / dmax(A[n], GAM_TINY);
     }
+    // the limiter faces are the middle four of the stencil
 #pragma unroll
-    for (int n = 0; n < 4; n++) {
-        const long p = c + (n - 2) * s;
-        nq[n] = mfL[FD * N3 + p] / dmax(met[(M_FACE + FD) * N3 + p], GAM_TINY);
-    }
+    for (int n = 0; n < 4; n++) nq[n] = mfL[FD * N3 + c + (n - 2) * s] / dmax(A[n + 2], GAM_TINY);
     double Vi = wsum8([&](int n) { return A[n]; });
     double l, r;
-    recon_lr<RM>(m.pdmb, A, Vi, nq, q, l, r);
+    recon_lr<RM>(m.pdmb, A, 1.0 / Vi, nq, q, l, r);
     fA = Vi;
     bL = l * fA;
     bR = r * fA;
@@ -248,11 +247,11 @@ __global__ __launch_bounds__(256) void k_emf(Geo g, MParm m, Tile b, const Clock
     double A[8];
 #pragma unroll
     for (int n = 0; n < 8; n++) A[n] = met[(M_FACE + T1) * N3 + c + (n - 4) * s2];
-    const double dA = dmax(wsum8([&](int n) { return A[n]; }), GAM_TINY);
+    const double rdA = 1.0 / dmax(wsum8([&](int n) { return A[n]; }), GAM_TINY);
     double vc[3];
 #pragma unroll
     for (int d = 0; d < 3; d++)
-        vc[d] = wsum8([&](int n) { return A[n] * Vf[d * N3 + c + (n - 4) * s2]; }) / dA;
+        vc[d] = wsum8([&](int n) { return A[n] * Vf[d * N3 + c + (n - 4) * s2]; }) * rdA;
     const double *Te = met + (long)(M_TE + 9 * ED) * N3 + c;
     const double v1 = vc[0] * Te[3 * N3] + vc[1] * Te[4 * N3] + vc[2] * Te[5 * N3];
     const double v2 = vc[0] * Te[6 * N3] + vc[1] * Te[7 * N3] + vc[2] * Te[8 * N3];
@@ -272,10 +271,11 @@ __global__ __launch_bounds__(256) void k_emf(Geo g, MParm m, Tile b, const Clock
     const double *Tb = met + (long)(M_TEB + 4 * ED) * N3 + c;
     const double xni = Tb[0], yni = Tb[N3], xnj = Tb[2 * N3], ynj = Tb[3 * N3];
     const double detT = 1.0 / (xni * ynj - xnj * yni);
-    const double bT1 = 0.5 * (bT1L + bT1R) / fA1;
-    const double bT2 = 0.5 * (bT2L + bT2R) / fA2;
-    const double db1 = (bT1R - bT1L) / fA1;
-    const double db2 = (bT2R - bT2L) / fA2;
+    const double rA1 = 1.0 / fA1, rA2 = 1.0 / fA2;
+    const double bT1 = 0.5 * (bT1L + bT1R) * rA1;
+    const double bT2 = 0.5 * (bT2L + bT2R) * rA2;
+    const double db1 = (bT1R - bT1L) * rA1;
+    const double db2 = (bT2R - bT2L) * rA2;
     const double Jd = db2 - db1;
     double b1 = detT * (ynj * bT1 - yni * bT2);
     double b2 = detT * (-(xnj * bT1) + xni * bT2);
@@ -312,7 +312,7 @@ __global__ __launch_bounds__(256) void k_flux(Geo g, MParm m, Tile b,
     double vol[8];
 #pragma unroll
     for (int n = 0; n < 8; n++) vol[n] = met[M_VOL * N3 + c + (n - 4) * s];
-    const double Vi = wsum8([&](int n) { return vol[n]; });
+    const double rVi = 1.0 / wsum8([&](int n) { return vol[n]; });
 
     // L/R primitives (BlockStateLRs, recon.F90:39-93)
     const double *Wh = H + H_WH * N3;
@@ -325,7 +325,7 @@ __global__ __launch_bounds__(256) void k_flux(Geo g, MParm m, Tile b,
         for (int n = 0; n < 8; n++) q[n] = Wh[v * N3 + c + (n - 4) * s];
 #pragma unroll
         for (int n = 0; n < 4; n++) nq[n] = Wl[v * N3 + c + (n - 2) * s];
-        recon_lr<RM>(m.pdmb, vol, Vi, nq, q, WL[v], WR[v]);
+        recon_lr<RM>(m.pdmb, vol, rVi, nq, q, WL[v], WR[v]);
     }
     double T[9];
 #pragma unroll
@@ -344,14 +344,15 @@ __global__ __launch_bounds__(256) void k_flux(Geo g, MParm m, Tile b,
             for (int n = 0; n < 8; n++) q[n] = Bh[x * N3 + c + (n - 4) * s];
 #pragma unroll
             for (int n = 0; n < 4; n++) nq[n] = Bl[x * N3 + c + (n - 2) * s];
-            recon_lr<RM>(m.pdmb, vol, Vi, nq, q, BL[x], BR[x]);
+            recon_lr<RM>(m.pdmb, vol, rVi, nq, q, BL[x], BR[x]);
         }
-        const double Bn = H[(H_MF + D) * N3 + c] / faB;
+        const double iFaB = 1.0 / faB;
+        const double Bn = H[(H_MF + D) * N3 + c] * iFaB;
         double B0[3] = {0.0, 0.0, 0.0}, B0n = 0.0;
         if (BG) {
 #pragma unroll
             for (int x = 0; x < 3; x++) B0[x] = bg[(B_FCB0 + x + 3 * D) * N3 + c];
-            B0n = bg[(B_BFLUX0 + D) * N3 + c] / faB;
+            B0n = bg[(B_BFLUX0 + D) * N3 + c] * iFaB;
         }
         double mF[3] = {0.0, 0.0, 0.0}, bbD = 0.0;
         mag_kin<BG, BORIS>(m, -1.0, WL, BL, Bn, T, B0, B0n, mF, bbD, VaD);
@@ -359,7 +360,7 @@ __global__ __launch_bounds__(256) void k_flux(Geo g, MParm m, Tile b,
         if (BORIS) {
 #pragma unroll
             for (int x = 0; x < 3; x++) {
-                double deeP = bbD * (WR[1 + x] - WL[1 + x]) / (m.Ca * m.Ca);
+                double deeP = bbD * (WR[1 + x] - WL[1 + x]) * m.iCa2;
                 mF[x] = mF[x] - m.cHogM * VaD * deeP;
             }
         }
@@ -368,14 +369,14 @@ __global__ __launch_bounds__(256) void k_flux(Geo g, MParm m, Tile b,
     }
 
     double gF[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, dW[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    gas_kin(m.gamma, -1.0, WL, T, gF, dW);
-    gas_kin(m.gamma, +1.0, WR, T, gF, dW);
+    gas_kin(m.igm1, -1.0, WL, T, gF, dW);
+    gas_kin(m.igm1, +1.0, WR, T, gF, dW);
 #pragma unroll
     for (int v = 0; v < 5; v++) gF[v] = gF[v] - m.cHogH * VaD * dW[v];
     if (BORIS) {   // "nuclear option" (stress.F90:399-417)
         double vL = sqrt(WL[1] * WL[1] + WL[2] * WL[2] + WL[3] * WL[3]);
         double vR = sqrt(WR[1] * WR[1] + WR[2] * WR[2] + WR[3] * WR[3]);
-        double vn = sqrt(vL * vL + vR * vR) / sqrt(2.0);
+        double vn = sqrt(vL * vL + vR * vR) * 0.70710678118654752440084436210484903928483593768847;
         if (vn >= 1.5 * m.Ca) {
 #pragma unroll
             for (int v = 0; v < 5; v++) {
@@ -498,14 +499,14 @@ __global__ __launch_bounds__(256) void k_update(Geo g, MParm m, Tile ft, Box ctb
 #pragma unroll
         for (int v = 0; v < 5; v++) U[v] = So[(S_GAS + v) * N3 + c];
         if (in_box(ab, i, j, k)) {
-            const double dV = met[M_VOL * N3 + c];
+            const double idV = 1.0 / met[M_VOL * N3 + c];
             const double dtM = clk->dt, dt = clk->dtA;
             double dH[5], oU[5];
 #pragma unroll
             for (int v = 0; v < 5; v++) {
                 const double *gv = gflx + (long)v * N3;
                 dH[v] = (gv[c] - gv[c + 1] + gv[5 * N3 + c] - gv[5 * N3 + c + g.P]
-                         + gv[10 * N3 + c] - gv[10 * N3 + c + g.PP]) / dV;
+                         + gv[10 * N3 + c] - gv[10 * N3 + c + g.PP]) * idV;
                 oU[v] = U[v];
                 U[v] = U[v] + dtM * dH[v];
             }
@@ -515,7 +516,7 @@ __global__ __launch_bounds__(256) void k_update(Geo g, MParm m, Tile ft, Box ctb
                 for (int x = 0; x < 3; x++) {
                     const double *mv = mflx + (long)x * N3;
                     double r = (mv[c] - mv[c + 1] + mv[3 * N3 + c] - mv[3 * N3 + c + g.P]
-                                + mv[6 * N3 + c] - mv[6 * N3 + c + g.PP]) / dV;
+                                + mv[6 * N3 + c] - mv[6 * N3 + c + g.PP]) * idV;
                     if (BG) r = r + bg[(B_DPB0 + x) * N3 + c];
                     dM[x] = r;
                     B[x] = Bn[x];
@@ -529,9 +530,10 @@ __global__ __launch_bounds__(256) void k_update(Geo g, MParm m, Tile ft, Box ctb
                     const double D2 = dmax(W2[DEN], m.dFloor);
                     const double P2 = dmax(W2[PRS], m.pFloor);
                     const double D0 = dmax(oU[DEN], m.dFloor);
-                    double V0[3] = {oU[MOMX] / D0, oU[MOMY] / D0, oU[MOMZ] / D0};
+                    const double iD0 = 1.0 / D0, iD2 = 1.0 / D2;
+                    double V0[3] = {oU[MOMX] * iD0, oU[MOMY] * iD0, oU[MOMZ] * iD0};
                     double B1[3] = {0.5 * (B[0] + oB[0]), 0.5 * (B[1] + oB[1]), 0.5 * (B[2] + oB[2])};
-                    const double alpha = (B1[0] * B1[0] + B1[1] * B1[1] + B1[2] * B1[2]) / (D2 * m.Ca * m.Ca);
+                    const double alpha = (B1[0] * B1[0] + B1[1] * B1[1] + B1[2] * B1[2]) * (iD2 * m.iCa2);
                     const double dRho = dt * dH[DEN];
                     double dPG[3] = {dt * dH[MOMX], dt * dH[MOMY], dt * dH[MOMZ]};
                     double dPM[3] = {dt * dM[0], dt * dM[1], dt * dM[2]};
@@ -541,7 +543,7 @@ __global__ __launch_bounds__(256) void k_update(Geo g, MParm m, Tile ft, Box ctb
                         const double nb = sqrt(B1[0] * B1[0] + B1[1] * B1[1] + B1[2] * B1[2]);
                         double bh[3];
                         if (nb <= GAM_TINY) { bh[0] = bh[1] = bh[2] = 0.0; }
-                        else { bh[0] = B1[0] / nb; bh[1] = B1[1] / nb; bh[2] = B1[2] / nb; }
+                        else { const double inb = 1.0 / nb; bh[0] = B1[0] * inb; bh[1] = B1[1] * inb; bh[2] = B1[2] * inb; }
                         double sGM[3] = {dPG[0] + dPM[0], dPG[1] + dPM[1], dPG[2] + dPM[2]};
                         const double pG = dPG[0] * bh[0] + dPG[1] * bh[1] + dPG[2] * bh[2];
                         const double pS = sGM[0] * bh[0] + sGM[1] * bh[1] + sGM[2] * bh[2];
@@ -556,21 +558,22 @@ __global__ __launch_bounds__(256) void k_update(Geo g, MParm m, Tile ft, Box ctb
                     double Wn[5];
                     Wn[DEN] = D2;
 #pragma unroll
-                    for (int x = 0; x < 3; x++) Wn[1 + x] = (D0 * V0[x] + dMom[x]) / D2;
+                    for (int x = 0; x < 3; x++) Wn[1 + x] = (D0 * V0[x] + dMom[x]) * iD2;
                     Wn[PRS] = P2;
                     p2c_cell(m, Wn, U);
                 } else {
                     // CellMaxwell (mhdgroup.F90:299-329)
                     const double d = dmax(U[DEN], m.dFloor);
                     double mx = U[MOMX], my = U[MOMY], mz = U[MOMZ];
-                    double ke = 0.5 * (mx * mx + my * my + mz * mz) / d;
+                    const double id = 1.0 / d;
+                    double ke = 0.5 * (mx * mx + my * my + mz * mz) * id;
                     const double p = dmax((m.gamma - 1) * (U[ENER] - ke), m.pFloor);
                     mx = mx + dt * dM[0];
                     my = my + dt * dM[1];
                     mz = mz + dt * dM[2];
-                    ke = 0.5 * (mx * mx + my * my + mz * mz) / d;
+                    ke = 0.5 * (mx * mx + my * my + mz * mz) * id;
                     U[DEN] = d; U[MOMX] = mx; U[MOMY] = my; U[MOMZ] = mz;
-                    U[ENER] = ke + p / (m.gamma - 1);
+                    U[ENER] = ke + p * m.igm1;
                 }
             }
         }

@@ -156,6 +156,12 @@ module gamhip
             type(c_ptr), value :: ctx
             integer(c_int), value :: nsteps
         end function
+        integer(c_int) function gam_advance_model(ctx, dt, nsteps) bind(C, name="gam_advance_model")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), value :: dt
+            integer(c_int) :: nsteps
+        end function
         integer(c_int) function gam_divb(ctx, tot) bind(C, name="gam_divb")
             import :: c_int, c_ptr, c_double
             type(c_ptr), value :: ctx
@@ -262,6 +268,16 @@ module gamhip
         dt = 0.0_rp
         call gamhip_check(gam_step(ctx%p, c_loc(dt)), 'gam_step')
     end function gamhip_step
+
+    !gamAdvanceModel (gamtypessub.F90:74-90): returns the number of steps
+    function gamhip_advance_model(ctx, dt) result(nsteps)
+        type(gamhip_ctx), intent(in) :: ctx
+        real(rp), intent(in) :: dt
+        integer :: nsteps
+        integer(c_int) :: n
+        call gamhip_check(gam_advance_model(ctx%p, dt, n), 'gam_advance_model')
+        nsteps = n
+    end function gamhip_advance_model
 
     subroutine gamhip_get_time(ctx, t, dt, ts)
         type(gamhip_ctx), intent(in) :: ctx

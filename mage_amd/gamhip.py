@@ -30,7 +30,7 @@ EXPORTS = (
     "gam_get_grid", "gam_set_background", "gam_add_dipole_b0", "gam_get_background",
     "gam_set_wind", "gam_put_state", "gam_get_state", "gam_set_time", "gam_get_time",
     "gam_init_run", "gam_advance_mhd", "gam_calc_dt", "gam_enforce_bcs", "gam_bflux2fld",
-    "gam_step", "gam_run", "gam_sync", "gam_divb", "gam_time_steps", "gam_set_profiling",
+    "gam_step", "gam_run", "gam_advance_model", "gam_sync", "gam_divb", "gam_time_steps", "gam_set_profiling",
     "gam_kernel_times", "gam_kernel_group_name", "gam_get_solver",
 )
 
@@ -101,6 +101,7 @@ def load():
     L.gam_bflux2fld.argtypes = [vp, C.c_int]
     L.gam_step.argtypes = [vp, DP]
     L.gam_run.argtypes = [vp, C.c_int]
+    L.gam_advance_model.argtypes = [vp, C.c_double, C.POINTER(C.c_int)]
     L.gam_sync.argtypes = [vp]
     L.gam_divb.argtypes = [vp, DP]
     L.gam_time_steps.argtypes = [vp, C.c_int, DP]
@@ -284,6 +285,13 @@ class GamHip:
 
     def run(self, nsteps):
         self._chk(self.L.gam_run(self.ctx, int(nsteps)))
+
+    def advance_model(self, dt):
+        """gamAdvanceModel: step until Model%t >= t0 + dt (at least once);
+        returns the number of steps taken."""
+        n = C.c_int()
+        self._chk(self.L.gam_advance_model(self.ctx, float(dt), C.byref(n)))
+        return n.value
 
     def sync(self):
         self._chk(self.L.gam_sync(self.ctx))

@@ -262,3 +262,129 @@ def box_params(rmeth="7UP"):
 def shell_params(rmeth="7UP"):
     return model_params(rmeth={"8CENT": 0, "7UP": 1}[rmeth], doBackground=1, doBoris=1, Ca=10.0,
                         bc=["zerograd", "wind", "zerograd", "zerograd", "periodic", "periodic"])
+
+
+# -------------------------------------------------------------- reference case decks
+# The reference's whole-step case tests (tests/cases/*.xml, checked by
+# tests/cases/testCases.pf) with the initial states of prob.F90.  Decks with
+# one cell along j or k run here with 8 uniform cells along that axis and
+# periodic walls: every flux and EMF difference along it is then exactly
+# zero, and the dt bound of the collapsed axis stays above the in-plane one.
+def cfl_from_pdmb(pdmb, max_cfl=0.3):
+    """Default CFL from the PDM parameter (init.F90:348-353)."""
+    return min(0.5 / (pdmb + 0.5), max_cfl)
+
+
+CASES = {
+    # name: cells, domain lo/hi, tFin, Model_T overrides
+    "adv1d": ((128, 8, 8), (-1.0, -0.5, -0.5), (1.0, 0.5, 0.5), 2.0,
+              dict(gamma=1.4, pdmb=4.0, doMHD=0, rmeth=1)),
+    "adv1d_8cent": ((128, 8, 8), (-1.0, -0.5, -0.5), (1.0, 0.5, 0.5), 2.0,
+                    dict(gamma=1.4, pdmb=4.0, doMHD=0, rmeth=0)),
+    "adv2d": ((256, 256, 8), (-0.5, -0.5, -0.5), (0.5, 0.5, 0.5), 1.0, dict(pdmb=4.0)),
+    "loop2d": ((256, 256, 8), (-0.5, -0.5, -0.5), (0.5, 0.5, 0.5), 1.0, dict(pdmb=4.0)),
+    "alfven": ((128, 256, 8), (0.0, 0.0, -0.5), (2.0, 1.15470053838, 0.5), 1.0, {}),
+    "ot2d": ((128, 128, 8), (0.0, 0.0, -0.5), (1.0, 1.0, 0.5), 1.0, {}),
+}
+
+
+def case_params(name):
+    N, lo, hi, tFin, over = CASES[name]
+    p = model_params(tFin=tFin, bc=["periodic"] * 6)
+    p.update(over)
+    p["CFL"] = cfl_from_pdmb(p["pdmb"])
+    return p
+
+
+def case_corners(name):
+    N, lo, hi, _, _ = CASES[name]
+    return cartesian_corners(*N, lo=lo, hi=hi)
+
+
+def _gauss_box_average(xyz, fun, nq=12):
+    """Cell averages of fun(x, y) over axis-aligned cells by the nq x nq
+    Gauss rule in the plane (GaussianVolumeIntegral/volume, quadrature.F90,
+    for functions that do not depend on z).  fun returns a tuple of arrays."""
+    x0, x1 = xyz[:-1, 0, 0, 0], xyz[1:, 0, 0, 0]
+    y0, y1 = xyz[0, :-1, 0, 1], xyz[0, 1:, 0, 1]
+    acc = None
+    for a in range(nq):
+        xq = 0.5 * (x0 + x1) + 0.5 * GA[a] * (x1 - x0)
+        for b in range(nq):
+            yq = 0.5 * (y0 + y1) + 0.5 * GA[b] * (y1 - y0)
+            vals = fun(xq[:, None], yq[None, :])
+            w = 0.25 * GW[a] * GW[b]
+            if acc is None:
+                acc = [w * v for v in vals]
+            else:
+                for q, v in enumerate(vals):
+                    acc[q] = acc[q] + w * v
+    return acc
+
+
+def case_state(name, grid, xyz):
+    """Initial State of a reference case deck (prob.F90)."""
+    p = case_params(name)
+    gam = p["gamma"]
+    st = empty_state(*[n - 2 * NG for n in grid["xyzcc"].shape[:3]])
+    xc, yc = grid["xyzcc"][..., 0], grid["xyzcc"][..., 1]
+    zero = np.zeros_like(xc)
+    pi = math.pi
+    if name.startswith("adv1d"):
+        # initADV1D (prob.F90:671-736): four shapes on rho0 = 0.1, vx = 1, P = 1
+        r1 = np.exp(-((xc + 0.7) / 0.05) ** 2)
+        r2 = np.where(np.abs(xc + 0.3) <= 0.1, 1.0, 0.0)
+        r3 = np.where(np.abs(xc - 0.1) <= 0.1, 1.0 - np.abs(10.0 * (xc - 0.1)), 0.0)
+        arg = 0.1 ** 2 - (xc - 0.5) ** 2
+        r4 = np.where(arg >= 0, 10.0 * np.sqrt(np.maximum(arg, 0.0)), 0.0)
+        rho = 0.1 + r1 + r2 + r3 + r4
+        st["Gas"][...] = prim2con(gam, rho, zero + 1.0, zero, zero, zero + 1.0)
+    elif name in ("adv2d", "loop2d"):
+        # initADV2D (prob.F90:739-796) / initLoop (prob.F90:436-500)
+        V0, alpha = 1.41421356237, 45.0
+        vx = V0 * math.sin(alpha * pi / 180.0)
+        vy = V0 * math.cos(alpha * pi / 180.0)
+        rho = np.ones_like(xc)
+        if name == "adv2d":
+            rho = np.where(np.sqrt(xc ** 2 + yc ** 2) < 0.3, 2.0, 1.0)
+        st["Gas"][...] = prim2con(gam, rho, zero + vx, zero + vy, zero, zero + 1.0)
+        if name == "loop2d":
+            def A(x, y, z):
+                r = np.sqrt(x ** 2 + y ** 2)
+                return (0 * x, 0 * x, np.maximum(0.0, 1.0e-3 * (0.3 - r)))
+            st["magFlux"][...] = vecpot_fluxes(xyz, A)
+    elif name == "alfven":
+        # initALFVEN (prob.F90:1027-1083): GasIC2State + VectorPot2Flux
+        alp, Bp = 1.0471975512, 1.0
+        ca, sa = math.cos(alp), math.sin(alp)
+
+        def W(x, y):
+            ph = 2 * pi * (x * ca + y * sa)
+            return (1.0 + 0 * x, -0.1 * np.sin(ph) * sa, 0.1 * np.sin(ph) * ca, 0.1 * np.cos(ph),
+                    0.1 + 0 * x)
+        D, vx, vy, vz, P = _gauss_box_average(xyz, W)
+        mx, my, mz = D * vx, D * vy, D * vz
+        g = st["Gas"]
+        g[..., 0] = D[:, :, None]
+        g[..., 1] = mx[:, :, None]
+        g[..., 2] = my[:, :, None]
+        g[..., 3] = mz[:, :, None]
+        g[..., 4] = (P / (gam - 1.0) + 0.5 * (mx ** 2 + my ** 2 + mz ** 2) / D)[:, :, None]
+
+        def A(x, y, z):
+            ph = 2 * pi * (x * ca + y * sa)
+            return (Bp * sa * z, -1.0 * Bp * ca * z + 0.1 * np.sin(ph) / (2 * pi * ca),
+                    0.1 * np.cos(ph) / (2 * pi))
+        st["magFlux"][...] = vecpot_fluxes(xyz, A)
+    elif name == "ot2d":
+        # initOT2D (prob.F90:74-119)
+        rho0, P0, B0 = 25.0 / (36.0 * pi), 5.0 / (12.0 * pi), 1.0 / math.sqrt(4.0 * pi)
+        vx, vy = -np.sin(2 * pi * yc), np.sin(2 * pi * xc)
+        st["Gas"][...] = prim2con(gam, zero + rho0, vx, vy, zero, zero + P0)
+        f = st["magFlux"]
+        face = grid["face"]
+        f[:-1, :-1, :-1, 0] = (-B0 * np.sin(2 * pi * yc)) * face[:-1, :-1, :-1, 0]
+        f[:-1, :-1, :-1, 1] = (B0 * np.sin(4 * pi * xc)) * face[:-1, :-1, :-1, 1]
+    else:
+        raise KeyError(name)
+    return st
