@@ -30,6 +30,10 @@ STEP_BYTES_PER_CELL = 8 * (19 + 14 + 79 + 24)
 # 3, volume 1), face data (Tf 9, area 1, half-state flux 1, fcB0 3,
 # bFlux0 1) and outputs (gas 5, magnetic 3): 40 words = 320 B.
 FLUX_BYTES_PER_FACE = 8 * 40
+# rocprofv3 FETCH_SIZE/WRITE_SIZE per dispatch of the flux kernel on this
+# workload, with the copy-kernel calibration (written by the profiling run
+# recorded in that file).
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "shell_192x192x256_step_kernels.md")
 
 
 def log(*a):
@@ -57,7 +61,23 @@ def setup_shell(gamhip, pb, N, rmeth="7UP", full_grid=False):
     return g, p, grid, st, wind
 
 
-def cpu_baseline(gamhip, pb, min_seconds=10.0, max_steps=20):
+def flux_traffic(path=PMC_SUMMARY):
+    """Calibrated HBM bytes per k_flux dispatch from the committed PMC
+    summary (mean over the three directions), or None without one."""
+    if not os.path.exists(path):
+        return None
+    vals = []
+    for line in open(path):
+        cols = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cols) >= 7 and cols[0].startswith("`k_flux"):
+            try:
+                vals.append(float(cols[5]))
+            except ValueError:
+                pass
+    return sum(vals) / len(vals) if vals else None
+
+
+def cpu_baseline(gamhip, pb, min_seconds=12.0, max_steps=400):
     """Time the oracle (C/OpenMP restatement of the reference step) on a
     96x96x128 sample of the same shell workload."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -124,6 +144,9 @@ def main():
     faces = ((N[0] + 1) * N[1] * N[2] + N[0] * (N[1] + 1) * N[2] + N[0] * N[1] * (N[2] + 1)) / 3.0
     flux_gbs = FLUX_BYTES_PER_FACE * faces / (flux_ms * 1e-3) / 1e9
     step_gbs = STEP_BYTES_PER_CELL * value / 1e9
+    # AdvanceMHD alone (the reference's "Gamera" clock): the advance kernels
+    adv_ms = sum(kt[k][0] for k in ("predict", "mom2face", "emf", "flux", "update_dt")) / args.profile_steps
+    mhd_value = cells / (adv_ms * 1e-3)
     t, dt, ts = g.get_time()
     log("kernel ms/launch: " + ", ".join("%s=%.3f" % (k, v["ms_per_launch"] or 0) for k, v in kernels.items()))
     g.close()
@@ -163,7 +186,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": flux_gbs / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": flux_traffic(),
             "bytes_per_launch": FLUX_BYTES_PER_FACE * faces,
             "ms_per_launch": flux_ms,
         },
@@ -175,6 +198,7 @@ def main():
             "unit": "GB/s",
             "frac": step_gbs / HBM_PEAK_GBS,
         },
+        "mhd_cell_updates_per_s": mhd_value,
         "kernels": kernels,
         "final": {"t": t, "dt": dt, "ts": ts},
         "cpu_baseline": cpu,
