@@ -176,7 +176,7 @@ extern "C" int gam_create(gam_ctx **out, const gam_dims *d, const gam_model *m)
     ALLOC(c->bg, B_NPL + 9);
     ALLOC(c->gflx, 15);
     ALLOC(c->mflx, 9);
-    ALLOC(c->vf, 3);
+    ALLOC(c->vf, 9);
     ALLOC(c->stage, 1);
     ALLOC(c->part, 1);
 #undef ALLOC
@@ -478,14 +478,18 @@ static Box empty_box() { return mkbox(1, 0, 1, 0, 1, 0); }
 static void enq_bflux2fld(gam_ctx *c, double *S, bool ghosts_only = false)
 {
     const long N3 = c->g.N3;
-    launch_bflux2fld(c->stream, c->g, cell_box(c->g), ghosts_only ? act_box(c->g) : empty_box(),
-                     S + S_MF * N3, S + S_B * N3, c->met);
+    if (ghosts_only)
+        launch_bflux2fld_ghosts(c->stream, c->g, S + S_MF * N3, S + S_B * N3, c->met);
+    else
+        launch_bflux2fld(c->stream, c->g, cell_box(c->g), empty_box(), S + S_MF * N3, S + S_B * N3, c->met);
 }
 
-static void enq_bcs(gam_ctx *c, double *S)
+// ghostB: whether BCs also set ghost-cell B (EnforceBCs on its own does;
+// inside stepGamera the bFlux2Fld that follows rewrites every ghost cell)
+static void enq_bcs(gam_ctx *c, double *S, bool ghostB = true)
 {
     for (int side = 0; side < 6; side++)
-        launch_bc(c->stream, side, c->model.bc[side], c->g, c->mp, c->clk, &c->wd, S, c->met);
+        launch_bc(c->stream, side, c->model.bc[side], c->g, c->mp, c->clk, &c->wd, S, c->met, ghostB);
 }
 
 // AdvanceMHD on the device (mhdgroup.F90:16-76); swaps State/oState.
@@ -504,18 +508,16 @@ static void enq_advance(gam_ctx *c, int finmode)
     Box eb[3];
     edge_boxes(g, eb);
     if (c->mp.doMHD) {
+        // Mom2Face of every face direction (ranges of fields.F90:119-175)
+        prof_begin(c, KG_MOM2FACE);
+        launch_vface(s, g, c->mp, c->H, c->met, c->vf);
+        prof_end(c, KG_MOM2FACE);
         for (int eD = 0; eD < 3; eD++) {
-            // Mom2Face ranges (fields.F90:119-175)
-            Box mb = cell_box(g);
-            int dT1;
-            if (eD == 0) { dT1 = 1; mb.j0 = g.js; mb.j1 = g.je + 1; }
-            else if (eD == 1) { dT1 = 2; mb.k0 = g.ks; mb.k1 = g.ke + 1; }
-            else { dT1 = 0; mb.i0 = g.is; mb.i1 = g.ie + 1; }
-            prof_begin(c, KG_MOM2FACE);
-            launch_mom2face(s, dT1, g, c->mp, mb, c->H, c->met, c->vf);
-            prof_end(c, KG_MOM2FACE);
+            // edge direction eD takes the face velocities along dT1
+            const int dT1 = (eD == 0) ? 1 : (eD == 1) ? 2 : 0;
             prof_begin(c, KG_EMF);
-            launch_emf(s, eD, g, c->mp, eb[eD], c->clk, c->H, S, c->vf, c->met, c->bg, Sn + S_E * N3);
+            launch_emf(s, eD, g, c->mp, eb[eD], c->clk, c->H, S, c->vf + (long)3 * dT1 * N3, c->met,
+                       c->bg, Sn + S_E * N3);
             prof_end(c, KG_EMF);
         }
     }
@@ -542,7 +544,7 @@ static void enq_step(gam_ctx *c)
     enq_advance(c, FIN_ADVANCE | FIN_MODEL | FIN_CALCDT | FIN_SETDT);
     double *S = S_cur(c);
     prof_begin(c, KG_BC);
-    enq_bcs(c, S);
+    enq_bcs(c, S, false);
     prof_end(c, KG_BC);
     prof_begin(c, KG_BFLD);
     enq_bflux2fld(c, S, true);

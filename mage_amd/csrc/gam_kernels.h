@@ -21,8 +21,9 @@ struct WindDev {
 
 void launch_predict(hipStream_t s, const Geo &g, const MParm &m, const Box &fb, const Clock *clk,
                     const double *S, const double *O, const double *met, double *H);
-void launch_mom2face(hipStream_t s, int dT, const Geo &g, const MParm &m, const Box &b,
-                     const double *H, const double *met, double *Vf);
+// Mom2Face of the three face directions: Vf planes [3*dT1 + comp]
+void launch_vface(hipStream_t s, const Geo &g, const MParm &m, const double *H, const double *met,
+                  double *Vf);
 void launch_emf(hipStream_t s, int eD, const Geo &g, const MParm &m, const Box &b, const Clock *clk,
                 const double *H, const double *S, const double *Vf, const double *met,
                 const double *bg, double *E);
@@ -36,8 +37,10 @@ void launch_calc_dt(hipStream_t s, const Geo &g, const MParm &m, const Box &dtb,
 void launch_finalize(hipStream_t s, const MParm &m, Clock *clk, int mode);
 void launch_bflux2fld(hipStream_t s, const Geo &g, const Box &cb, const Box &skip, const double *mf,
                       double *B, const double *met);
+void launch_bflux2fld_ghosts(hipStream_t s, const Geo &g, const double *mf, double *B, const double *met);
+// ghostB = false skips ghost-cell B that a following ghost bFlux2Fld rewrites
 void launch_bc(hipStream_t s, int side, int kind, const Geo &g, const MParm &m, const Clock *clk,
-               const WindDev *w, double *S, const double *met);
+               const WindDev *w, double *S, const double *met, bool ghostB);
 void launch_divb(hipStream_t s, const Geo &g, const Box &ab, const double *mf, const double *bg,
                  int useBG, double *part, int *nparts);
 void launch_repack(hipStream_t s, int dir_in, const Geo &g, int ni, int nj, int nk,

@@ -2,17 +2,17 @@
 //
 // Kernel sequence of one stepGamera (gamapp.F90:73-113):
 //   k_predict    Predictor: AB2 face fluxes, half-step B and gas, plus the
-//                primitives used by the flux sweeps           (K1-K3)
-//   k_mom2face   face velocities along dT1, per EMF direction (K4-K6)
-//   k_emf        edge EMFs                                    (K7)
-//   k_flux       Lyon gas/magnetic fluxes, one per direction  (K8-K10)
+//                primitives used by the flux sweeps
+//   k_vface      face velocities of all three face directions
+//   k_emf        edge EMFs
+//   k_flux       Lyon gas/magnetic fluxes, one per direction
 //   k_update     CT face-flux update, cell B of the new state, flux
 //                divergence, Reynolds, Maxwell/Boris and the CFL
-//                min-reduction of the new state  (K11,K13-K17)
+//                min-reduction of the new state
 //   k_finalize   time bookkeeping and CalcDT checks (1 thread)
-//   k_bc_*       boundary conditions                          (K18)
+//   k_bc_*       boundary conditions
 //   k_bflux2fld  cell B after the BCs; active cells keep theirs,
-//                since no BC writes a face of an active cell  (K19)
+//                since no BC writes a face of an active cell
 // State and oState are two buffers that swap roles every step, which
 // replaces CopyState (mhdgroup.F90:204-255): the update reads the current
 // buffer and writes every point of the other.
@@ -164,32 +164,128 @@ __global__ __launch_bounds__(256) void k_predict(Geo g, MParm m, Tile fb, const 
 }
 
 // ====================================================================
-// Mom2Face (fields.F90:235-285): volume-weighted 8-point interpolation of
-// rho*v/max(rho,dFloor) to the dT faces.  The velocities rho*v/max(rho,
-// dFloor) are the half-state primitives the predictor already stored.
+// Column tiles for kernels that march along k: a workgroup owns 64 (i) x 4
+// (j) columns over KC consecutive k, so k-direction stencils slide through
+// registers instead of being re-read.  Same XCD-aware numbering as Tile:
+// i-tile fastest, then j-tile, then k-chunk.
 // ====================================================================
-template <int DT>
-__global__ __launch_bounds__(256) void k_mom2face(Geo g, MParm m, Tile b,
-                                                  const double *__restrict__ H,
-                                                  const double *__restrict__ met,
-                                                  double *__restrict__ Vf)
+#define KC 32
+struct ColTile {
+    Box b;
+    int n0, n1, nk, nb;
+};
+
+static inline ColTile coltile_of(const Box &b)
 {
-    TILE_IJK(b);
+    ColTile t;
+    t.b = b;
+    t.n0 = (b.i1 - b.i0 + BX) / BX;
+    t.n1 = (b.j1 - b.j0 + BY) / BY;
+    t.nk = (b.k1 - b.k0 + KC) / KC;
+    t.nb = t.n0 * t.n1 * t.nk;
+    return t;
+}
+static inline dim3 cgrid(const ColTile &t) { return dim3((unsigned)(((t.nb + 7) / 8) * 8)); }
+
+// Column of this thread: (i, j) and the k range [k0, k1]; false if none.
+__device__ __forceinline__ bool col_point(const ColTile &t, int &i, int &j, int &k0, int &k1)
+{
+    const int w = blockIdx.x;
+    const int per = (t.nb + 7) >> 3;
+    const int L = (w & 7) * per + (w >> 3);
+    if (L >= t.nb) return false;
+    const int t0 = L % t.n0;
+    const int r = L / t.n0;
+    const int t1 = r % t.n1;
+    const int t2 = r / t.n1;
+    i = t.b.i0 + t0 * BX + threadIdx.x;
+    j = t.b.j0 + t1 * BY + threadIdx.y;
+    k0 = t.b.k0 + t2 * KC;
+    k1 = min(k0 + KC - 1, t.b.k1);
+    return i <= t.b.i1 && j <= t.b.j1;
+}
+
+// ====================================================================
+// Mom2Face (fields.F90:235-285) for all three face directions in one pass:
+// Vf[3*dT1 + c] = <V * rho v / max(rho, dFloor)>_8 / <V>_8 along dT1, on
+// the face ranges of CalcElecField (fields.F90:119-175): dT1 = I on
+// i in is..ie+1, J on j in js..je+1, K on k in ks..ke+1 (other indices over
+// all cells).  The velocities rho*v/max(rho,dFloor) are the half-state
+// primitives the predictor stored.  The k stencil slides along the column.
+// ====================================================================
+__global__ __launch_bounds__(256) void k_vface(Geo g, MParm m, ColTile t,
+                                               const double *__restrict__ H,
+                                               const double *__restrict__ met,
+                                               double *__restrict__ Vf)
+{
+    int i, j, k0, k1;
+    if (!col_point(t, i, j, k0, k1)) return;
     const long N3 = g.N3;
-    const long c = IX(g, i, j, k);
-    const long s = STRIDE<DT>(g);
-    double vol[8], vel[3][8];
+    const double *vol = met + (long)M_VOL * N3;
+    const double *wv = H + (long)(H_WH + 1) * N3;
+    // k window: slot n holds cell k-4+n (zero outside the cell range)
+    double zv[8], zx[8], zy[8], zz[8];
 #pragma unroll
-    for (int n = 0; n < 8; n++) {
-        const long q = c + (n - 4) * s;
-        double v = met[M_VOL * N3 + q];
-        vol[n] = v;
-#pragma unroll
-        for (int d = 0; d < 3; d++) vel[d][n] = v * H[(H_WH + 1 + d) * N3 + q];
+    for (int n = 0; n < 7; n++) {
+        const int kk = k0 - 4 + n;
+        const bool in = kk >= g.ksg && kk <= g.keg;
+        const long q = IX(g, i, j, in ? kk : g.ksg);
+        zv[n] = in ? vol[q] : 0.0;
+        zx[n] = in ? wv[q] : 0.0;
+        zy[n] = in ? wv[N3 + q] : 0.0;
+        zz[n] = in ? wv[2 * N3 + q] : 0.0;
     }
-    double rdV = 1.0 / dmax(wsum8([&](int n) { return vol[n]; }), GAM_TINY);
+    const bool iI = i >= g.is && i <= g.ie + 1;
+    const bool iJ = j >= g.js && j <= g.je + 1;
+#pragma unroll 1
+    for (int kl = k0; kl <= k1; kl++) {
+        int k = kl;
+        asm volatile("" : "+v"(k));   // one address base per iteration
+        {
+            const int kk = k + 3;
+            const bool in = kk <= g.keg;
+            long q = IX(g, i, j, in ? kk : g.keg);
+            asm volatile("" : "+v"(q));
+            zv[7] = in ? vol[q] : 0.0;
+            zx[7] = in ? wv[q] : 0.0;
+            zy[7] = in ? wv[N3 + q] : 0.0;
+            zz[7] = in ? wv[2 * N3 + q] : 0.0;
+        }
+        long c = IX(g, i, j, k);
+        asm volatile("" : "+v"(c));
+        if (k >= g.ks && k <= g.ke + 1) {
+            const double r = 1.0 / dmax(wsum8([&](int n) { return zv[n]; }), GAM_TINY);
+            Vf[(6 + 0) * N3 + c] = wsum8([&](int n) { return zv[n] * zx[n]; }) * r;
+            Vf[(6 + 1) * N3 + c] = wsum8([&](int n) { return zv[n] * zy[n]; }) * r;
+            Vf[(6 + 2) * N3 + c] = wsum8([&](int n) { return zv[n] * zz[n]; }) * r;
+        }
+        if (iI) {
+            double v8[8], x8[8], y8[8], z8[8];
 #pragma unroll
-    for (int d = 0; d < 3; d++) Vf[d * N3 + c] = wsum8([&](int n) { return vel[d][n]; }) * rdV;
+            for (int n = 0; n < 8; n++) {
+                const long q = c + (n - 4);
+                v8[n] = vol[q]; x8[n] = wv[q]; y8[n] = wv[N3 + q]; z8[n] = wv[2 * N3 + q];
+            }
+            const double r = 1.0 / dmax(wsum8([&](int n) { return v8[n]; }), GAM_TINY);
+            Vf[0 * N3 + c] = wsum8([&](int n) { return v8[n] * x8[n]; }) * r;
+            Vf[1 * N3 + c] = wsum8([&](int n) { return v8[n] * y8[n]; }) * r;
+            Vf[2 * N3 + c] = wsum8([&](int n) { return v8[n] * z8[n]; }) * r;
+        }
+        if (iJ) {
+            double v8[8], x8[8], y8[8], z8[8];
+#pragma unroll
+            for (int n = 0; n < 8; n++) {
+                const long q = c + (long)(n - 4) * g.P;
+                v8[n] = vol[q]; x8[n] = wv[q]; y8[n] = wv[N3 + q]; z8[n] = wv[2 * N3 + q];
+            }
+            const double r = 1.0 / dmax(wsum8([&](int n) { return v8[n]; }), GAM_TINY);
+            Vf[3 * N3 + c] = wsum8([&](int n) { return v8[n] * x8[n]; }) * r;
+            Vf[4 * N3 + c] = wsum8([&](int n) { return v8[n] * y8[n]; }) * r;
+            Vf[5 * N3 + c] = wsum8([&](int n) { return v8[n] * z8[n]; }) * r;
+        }
+#pragma unroll
+        for (int n = 0; n < 7; n++) { zv[n] = zv[n + 1]; zx[n] = zx[n + 1]; zy[n] = zy[n + 1]; zz[n] = zz[n + 1]; }
+    }
 }
 
 // ====================================================================
@@ -666,6 +762,38 @@ __global__ __launch_bounds__(256) void k_bflux2fld(Geo g, Tile cb, Box skip, con
     for (int x = 0; x < 3; x++) B[x * N3 + c] = b[x];
 }
 
+// bFlux2Fld over the ghost shell only: up to six boxes (the i, j and k
+// ghost slabs, disjoint), each cut into 64 x 4 (i, j) tiles; workgroup w
+// finds its box from the running tile counts.
+struct GhostBoxes {
+    Box b[6];
+    int start[7];     // first tile of each box; start[n] = total
+    int n0[6], n1[6];
+    int n;
+};
+
+__global__ __launch_bounds__(256) void k_bflux2fld_ghosts(Geo g, GhostBoxes gb, const double *__restrict__ mf,
+                                                          double *__restrict__ B,
+                                                          const double *__restrict__ met)
+{
+    const int w = blockIdx.x;
+    int q = 0;
+    while (q + 1 < gb.n && w >= gb.start[q + 1]) q++;
+    const int L = w - gb.start[q];
+    const Box &b = gb.b[q];
+    const int t0 = L % gb.n0[q];
+    const int r = L / gb.n0[q];
+    const int i = b.i0 + t0 * BX + threadIdx.x;
+    const int j = b.j0 + (r % gb.n1[q]) * BY + threadIdx.y;
+    const int k = b.k0 + r / gb.n1[q];
+    if (i > b.i1 || j > b.j1 || k > b.k1) return;
+    const long N3 = g.N3, c = IX(g, i, j, k);
+    double bb[3];
+    cell_b(g, met, c, mf[c], mf[N3 + c], mf[2 * N3 + c], mf[c + 1], mf[N3 + c + g.P],
+           mf[2 * N3 + c + g.PP], bb);
+    for (int x = 0; x < 3; x++) B[x * N3 + c] = bb[x];
+}
+
 // ====================================================================
 // Boundary conditions.  One thread per transverse ghost column; the n loop
 // runs inside as in the reference.  Kernels of the six faces run in the
@@ -1043,13 +1171,12 @@ void launch_predict(hipStream_t s, const Geo &g, const MParm &m, const Box &fb, 
     LAUNCH(k_predict, tgrid(t), dim3(BX, BY), s, g, m, t, clk, S, O, met, H);
 }
 
-void launch_mom2face(hipStream_t s, int dT, const Geo &g, const MParm &m, const Box &b,
-                     const double *H, const double *met, double *Vf)
+void launch_vface(hipStream_t s, const Geo &g, const MParm &m, const double *H, const double *met,
+                  double *Vf)
 {
-    const Tile t = tile_of(b, dT == 2 ? 2 : 1);
-    if (dT == 0) LAUNCH(k_mom2face<0>, tgrid(t), dim3(BX, BY), s, g, m, t, H, met, Vf);
-    else if (dT == 1) LAUNCH(k_mom2face<1>, tgrid(t), dim3(BX, BY), s, g, m, t, H, met, Vf);
-    else LAUNCH(k_mom2face<2>, tgrid(t), dim3(BX, BY), s, g, m, t, H, met, Vf);
+    const Box b = {g.isg, g.ieg, g.jsg, g.jeg, g.ksg, g.keg};
+    const ColTile t = coltile_of(b);
+    LAUNCH(k_vface, cgrid(t), dim3(BX, BY), s, g, m, t, H, met, Vf);
 }
 
 template <int ED, int RM, bool BG, bool BO>
@@ -1163,10 +1290,33 @@ void launch_bflux2fld(hipStream_t s, const Geo &g, const Box &cb, const Box &ski
     LAUNCH(k_bflux2fld, tgrid(t), dim3(BX, BY), s, g, t, skip, mf, B, met);
 }
 
+void launch_bflux2fld_ghosts(hipStream_t s, const Geo &g, const double *mf, double *B, const double *met)
+{
+    GhostBoxes gb;
+    const Box bx[6] = {
+        {g.isg, g.ieg, g.jsg, g.jeg, g.ksg, g.ks - 1},      // k ghosts, all i, j
+        {g.isg, g.ieg, g.jsg, g.jeg, g.ke + 1, g.keg},
+        {g.isg, g.ieg, g.jsg, g.js - 1, g.ks, g.ke},        // j ghosts, active k
+        {g.isg, g.ieg, g.je + 1, g.jeg, g.ks, g.ke},
+        {g.isg, g.is - 1, g.js, g.je, g.ks, g.ke},          // i ghosts, active j, k
+        {g.ie + 1, g.ieg, g.js, g.je, g.ks, g.ke}};
+    int tot = 0;
+    for (int q = 0; q < 6; q++) {
+        gb.b[q] = bx[q];
+        gb.start[q] = tot;
+        gb.n0[q] = (bx[q].i1 - bx[q].i0 + BX) / BX;
+        gb.n1[q] = (bx[q].j1 - bx[q].j0 + BY) / BY;
+        tot += gb.n0[q] * gb.n1[q] * (bx[q].k1 - bx[q].k0 + 1);
+    }
+    gb.start[6] = tot;
+    gb.n = 6;
+    LAUNCH(k_bflux2fld_ghosts, dim3((unsigned)tot), dim3(BX, BY), s, g, gb, mf, B, met);
+}
+
 static inline dim3 grid2(const Box &t) { return dim3((unsigned)((t.i1 - t.i0 + 64) / 64), (unsigned)(t.j1 - t.j0 + 1)); }
 
 void launch_bc(hipStream_t s, int side, int kind, const Geo &g, const MParm &m, const Clock *clk,
-               const WindDev *w, double *S, const double *met)
+               const WindDev *w, double *S, const double *met, bool ghostB)
 {
     // Transverse boxes (a,b) per side as in the reference loops
     Box t;
@@ -1202,7 +1352,9 @@ void launch_bc(hipStream_t s, int side, int kind, const Geo &g, const MParm &m, 
         Box tw = t;
         tw.i1 = g.jeg + 1; tw.j1 = g.keg + 1;
         LAUNCH(k_bc_wind, grid2(tw), bl, s, g, m, tw, clk, *w, S, met);
-        LAUNCH(k_bc_wind_b, gr, bl, s, g, t, S, met);
+        // ghost-cell B of the wind BC (wind.F90:338-348); skipped when a
+        // bFlux2Fld over every ghost cell follows, which overwrites it
+        if (ghostB) LAUNCH(k_bc_wind_b, gr, bl, s, g, t, S, met);
     }
 }
 
