@@ -4,6 +4,7 @@
 // rounding.
 #pragma once
 #include "gam_common.h"
+#include "gam_erfc.h"
 
 #define GAM_PI 3.141592653589793115997963468544185161590576171875
 #define GAM_RSQRT_PI 0.56418958354775628694807945156077258584405062932900  // 1/sqrt(pi)
@@ -98,6 +99,24 @@ __device__ __forceinline__ void recon_lr(double pdmb, const double *vol, double 
     }
 }
 
+// recon_lr on values already multiplied by their weights: qv[n] = V_n q_n.
+template <int RM>
+__device__ __forceinline__ void recon_lr_w(double pdmb, double rVi, const double *nq,
+                                           const double *qv, double &ql, double &qr)
+{
+    if (RM == 1) {
+        double l = qv[0] * UP7(0) + qv[1] * UP7(1) + qv[2] * UP7(2) + qv[3] * UP7(3)
+                 + qv[4] * UP7(4) + qv[5] * UP7(5) + qv[6] * UP7(6);
+        double r = qv[7] * UP7(0) + qv[6] * UP7(1) + qv[5] * UP7(2) + qv[4] * UP7(3)
+                 + qv[3] * UP7(4) + qv[2] * UP7(5) + qv[1] * UP7(6);
+        ql = pdm_one(pdmb, nq[0], nq[1], nq[2], l * rVi);
+        qr = pdm_one(pdmb, nq[3], nq[2], nq[1], r * rVi);
+    } else {
+        double qi = wsum8([&](int n) { return qv[n]; }) * rVi;
+        pdm_pair(pdmb, nq[0], nq[1], nq[2], nq[3], qi, ql, qr);
+    }
+}
+
 // ---------------------------------------------------------- fluxes
 // One side (s = -1 left, +1 right) of the Lyon gas-kinetic flux,
 // GasKinFlux (stress.F90:432-511).  T = face triad (N, T1, T2).
@@ -112,8 +131,10 @@ __device__ __forceinline__ void gas_kin(double igm1, double s, const double *W, 
     double vn = T[0] * vx + T[1] * vy + T[2] * vz;
     double vt1 = T[3] * vx + T[4] * vy + T[5] * vz;
     double vt2 = T[6] * vx + T[7] * vy + T[8] * vz;
-    double vn0 = 0.5 * erfc(s * (lam * rl) * vn);
-    double vn1 = vn * vn0 - 0.5 * s * exp(-(lam * vn * vn)) * (rl * GAM_RSQRT_PI);
+    // erfc(s sqrt(lam) vn) and exp(-lam vn^2) from one evaluation
+    double ez;
+    double vn0 = 0.5 * gam_erfc_exp(s * (lam * rl) * vn, ez);
+    double vn1 = vn * vn0 - 0.5 * s * ez * (rl * GAM_RSQRT_PI);
     gF[DEN] = gF[DEN] + d * vn1;
     gF[ENER] = gF[ENER] + (e + 0.5 * p) * vn1 + 0.5 * p * vn0 * vn;
     double fn = d * vn * vn1 + p * vn0;
@@ -153,7 +174,8 @@ __device__ __forceinline__ void mag_kin(const MParm &m, double s, const double *
     double nx = T[0], ny = T[1], nz = T[2];
     double vn = nx * W[1] + ny * W[2] + nz * W[3];
     double lam = 1 / (2 * p * id + vac2);
-    double vn0 = 0.5 * erfc(s * sqrt(lam) * vn);
+    double ez;
+    double vn0 = 0.5 * gam_erfc_exp(s * sqrt(lam) * vn, ez);
     VaD = VaD + 0.5 * sqrt(vac2);
     mF[0] = mF[0] + vn0 * (dPb * nx - bx * Bn);
     mF[1] = mF[1] + vn0 * (dPb * ny - by * Bn);

@@ -33,16 +33,19 @@ __device__ __forceinline__ bool in_box(const Box &b, int i, int j, int k)
     const int k = (b).k0 + blockIdx.z;                  \
     if (i > (b).i1 || j > (b).j1) return;
 
-// XCD-aware tiling.  A launch covers box b with 64 x 4 tiles: 64 points
-// along i and 4 along axis ax1 (1 = j, 2 = k); tiles are numbered i-tile
-// fastest, then ax1-chunk, then the remaining axis.  The hardware hands
-// workgroup w to XCD w % 8, so logical tile L = (w % 8) * per + w / 8 gives
-// every XCD one contiguous run of tiles: tiles that share stencil planes
-// along ax1 then run back to back on the same L2.
+// XCD-aware tiling.  A launch covers box b plane by plane: the points of
+// each (i, ax1) plane (ax1 = 1: j, 2: k) are numbered i fastest and cut
+// into chunks of 256 consecutive points, one per workgroup, so no lane idles
+// however the row length falls against 64; chunks are numbered within the
+// plane first, then plane by plane along the remaining axis.  The hardware
+// hands workgroup w to XCD w % 8, so logical chunk L = (w % 8) * per + w / 8
+// gives every XCD one contiguous run: chunks that share stencil rows along
+// ax1 run back to back on the same L2.
 struct Tile {
     Box b;
     int ax1;
-    int n0, n1, n2, nb;
+    int ni, na;       // extents along i and ax1
+    int nc, nb;       // chunks per plane, chunks in all
 };
 
 __device__ __forceinline__ bool tile_point(const Tile &t, int &i, int &j, int &k)
@@ -50,16 +53,17 @@ __device__ __forceinline__ bool tile_point(const Tile &t, int &i, int &j, int &k
     const int w = blockIdx.x;
     const int per = (t.nb + 7) >> 3;
     const int L = (w & 7) * per + (w >> 3);
-    if (L >= t.nb) { i = j = k = 0x40000000; return false; }
-    const int t0 = L % t.n0;
-    const int r = L / t.n0;
-    const int t1 = r % t.n1;
-    const int t2 = r / t.n1;
-    i = t.b.i0 + t0 * BX + threadIdx.x;
-    const int a = t1 * BY + threadIdx.y;
+    i = j = k = 0x40000000;
+    if (L >= t.nb) return false;
+    const int ch = L % t.nc;
+    const int t2 = L / t.nc;
+    const unsigned p = (unsigned)ch * (BX * BY) + threadIdx.x + BX * threadIdx.y;
+    if (p >= (unsigned)(t.ni * t.na)) return false;
+    const int a = (int)(p / (unsigned)t.ni);
+    i = t.b.i0 + (int)(p - (unsigned)a * t.ni);
     if (t.ax1 == 2) { k = t.b.k0 + a; j = t.b.j0 + t2; }
     else { j = t.b.j0 + a; k = t.b.k0 + t2; }
-    return i <= t.b.i1 && j <= t.b.j1 && k <= t.b.k1;
+    return true;
 }
 
 #define TILE_IJK(t)                  \
@@ -72,10 +76,10 @@ static inline Tile tile_of(const Box &b, int ax1)
     t.b = b;
     t.ax1 = ax1;
     const int ni = b.i1 - b.i0 + 1, nj = b.j1 - b.j0 + 1, nk = b.k1 - b.k0 + 1;
-    t.n0 = (ni + BX - 1) / BX;
-    t.n1 = ((ax1 == 2 ? nk : nj) + BY - 1) / BY;
-    t.n2 = (ax1 == 2 ? nj : nk);
-    t.nb = t.n0 * t.n1 * t.n2;
+    t.ni = ni;
+    t.na = (ax1 == 2 ? nk : nj);
+    t.nc = (ni * t.na + BX * BY - 1) / (BX * BY);
+    t.nb = t.nc * (ax1 == 2 ? nj : nk);
     return t;
 }
 static inline dim3 tgrid(const Tile &t) { return dim3((unsigned)(((t.nb + 7) / 8) * 8)); }
@@ -154,35 +158,43 @@ __global__ __launch_bounds__(256) void k_predict(Geo g, MParm m, Tile fb, const 
     p2c_cell(m, pW, pQ);
 #pragma unroll
     for (int v = 0; v < 5; v++) H[(H_GAS + v) * N3 + c] = pQ[v];
-    // primitives of the reconstruction (half state) and limiter (state n) stencils
+    // primitives of the reconstruction stencils (half state), stored
+    // multiplied by the cell volume: every use is the volume-weighted
+    // interpolation <V q>/<V> (recon.F90:125-198, fields.F90:262-270), and
+    // the product is the one those formulas would form.  The limiter
+    // stencils take plain primitives: of state n (doLFMLim), else of the
+    // half state.
     c2p_block(m, pQ, Wb);
+    const double vol = met[M_VOL * N3 + c];
 #pragma unroll
-    for (int v = 0; v < 5; v++) H[(H_WH + v) * N3 + c] = Wb[v];
-    c2p_block(m, Q, Wb);
+    for (int v = 0; v < 5; v++) H[(H_WH + v) * N3 + c] = vol * Wb[v];
+    if (m.doLFM) c2p_block(m, Q, Wb);
 #pragma unroll
     for (int v = 0; v < 5; v++) H[(H_WN + v) * N3 + c] = Wb[v];
 }
 
 // ====================================================================
-// Column tiles for kernels that march along k: a workgroup owns 64 (i) x 4
-// (j) columns over KC consecutive k, so k-direction stencils slide through
-// registers instead of being re-read.  Same XCD-aware numbering as Tile:
-// i-tile fastest, then j-tile, then k-chunk.
+// Column tiles for kernels that march along k: a workgroup owns 256
+// consecutive (i, j) columns of the plane (i fastest) over KC consecutive k,
+// so k-direction stencils slide through registers instead of being re-read.
+// Same XCD-aware numbering as Tile: column chunk fastest, then k-chunk.
 // ====================================================================
 #define KC 32
 struct ColTile {
     Box b;
-    int n0, n1, nk, nb;
+    int ni, nj;       // extents along i and j
+    int nc, nk, nb;   // 256-column chunks of the (i, j) plane, k chunks, total
 };
 
 static inline ColTile coltile_of(const Box &b)
 {
     ColTile t;
     t.b = b;
-    t.n0 = (b.i1 - b.i0 + BX) / BX;
-    t.n1 = (b.j1 - b.j0 + BY) / BY;
+    t.ni = b.i1 - b.i0 + 1;
+    t.nj = b.j1 - b.j0 + 1;
+    t.nc = (t.ni * t.nj + BX * BY - 1) / (BX * BY);
     t.nk = (b.k1 - b.k0 + KC) / KC;
-    t.nb = t.n0 * t.n1 * t.nk;
+    t.nb = t.nc * t.nk;
     return t;
 }
 static inline dim3 cgrid(const ColTile &t) { return dim3((unsigned)(((t.nb + 7) / 8) * 8)); }
@@ -194,15 +206,16 @@ __device__ __forceinline__ bool col_point(const ColTile &t, int &i, int &j, int 
     const int per = (t.nb + 7) >> 3;
     const int L = (w & 7) * per + (w >> 3);
     if (L >= t.nb) return false;
-    const int t0 = L % t.n0;
-    const int r = L / t.n0;
-    const int t1 = r % t.n1;
-    const int t2 = r / t.n1;
-    i = t.b.i0 + t0 * BX + threadIdx.x;
-    j = t.b.j0 + t1 * BY + threadIdx.y;
+    const int ch = L % t.nc;
+    const int t2 = L / t.nc;
+    const unsigned p = (unsigned)ch * (BX * BY) + threadIdx.x + BX * threadIdx.y;
+    if (p >= (unsigned)(t.ni * t.nj)) return false;
+    const int a = (int)(p / (unsigned)t.ni);
+    i = t.b.i0 + (int)(p - (unsigned)a * t.ni);
+    j = t.b.j0 + a;
     k0 = t.b.k0 + t2 * KC;
     k1 = min(k0 + KC - 1, t.b.k1);
-    return i <= t.b.i1 && j <= t.b.j1;
+    return true;
 }
 
 // ====================================================================
@@ -222,7 +235,7 @@ __global__ __launch_bounds__(256) void k_vface(Geo g, MParm m, ColTile t,
     if (!col_point(t, i, j, k0, k1)) return;
     const long N3 = g.N3;
     const double *vol = met + (long)M_VOL * N3;
-    const double *wv = H + (long)(H_WH + 1) * N3;
+    const double *wv = H + (long)(H_WH + 1) * N3;   // V * velocity
     // k window: slot n holds cell k-4+n (zero outside the cell range)
     double zv[8], zx[8], zy[8], zz[8];
 #pragma unroll
@@ -255,9 +268,9 @@ __global__ __launch_bounds__(256) void k_vface(Geo g, MParm m, ColTile t,
         asm volatile("" : "+v"(c));
         if (k >= g.ks && k <= g.ke + 1) {
             const double r = 1.0 / dmax(wsum8([&](int n) { return zv[n]; }), GAM_TINY);
-            Vf[(6 + 0) * N3 + c] = wsum8([&](int n) { return zv[n] * zx[n]; }) * r;
-            Vf[(6 + 1) * N3 + c] = wsum8([&](int n) { return zv[n] * zy[n]; }) * r;
-            Vf[(6 + 2) * N3 + c] = wsum8([&](int n) { return zv[n] * zz[n]; }) * r;
+            Vf[(6 + 0) * N3 + c] = wsum8([&](int n) { return zx[n]; }) * r;
+            Vf[(6 + 1) * N3 + c] = wsum8([&](int n) { return zy[n]; }) * r;
+            Vf[(6 + 2) * N3 + c] = wsum8([&](int n) { return zz[n]; }) * r;
         }
         if (iI) {
             double v8[8], x8[8], y8[8], z8[8];
@@ -267,9 +280,9 @@ __global__ __launch_bounds__(256) void k_vface(Geo g, MParm m, ColTile t,
                 v8[n] = vol[q]; x8[n] = wv[q]; y8[n] = wv[N3 + q]; z8[n] = wv[2 * N3 + q];
             }
             const double r = 1.0 / dmax(wsum8([&](int n) { return v8[n]; }), GAM_TINY);
-            Vf[0 * N3 + c] = wsum8([&](int n) { return v8[n] * x8[n]; }) * r;
-            Vf[1 * N3 + c] = wsum8([&](int n) { return v8[n] * y8[n]; }) * r;
-            Vf[2 * N3 + c] = wsum8([&](int n) { return v8[n] * z8[n]; }) * r;
+            Vf[0 * N3 + c] = wsum8([&](int n) { return x8[n]; }) * r;
+            Vf[1 * N3 + c] = wsum8([&](int n) { return y8[n]; }) * r;
+            Vf[2 * N3 + c] = wsum8([&](int n) { return z8[n]; }) * r;
         }
         if (iJ) {
             double v8[8], x8[8], y8[8], z8[8];
@@ -279,9 +292,9 @@ __global__ __launch_bounds__(256) void k_vface(Geo g, MParm m, ColTile t,
                 v8[n] = vol[q]; x8[n] = wv[q]; y8[n] = wv[N3 + q]; z8[n] = wv[2 * N3 + q];
             }
             const double r = 1.0 / dmax(wsum8([&](int n) { return v8[n]; }), GAM_TINY);
-            Vf[3 * N3 + c] = wsum8([&](int n) { return v8[n] * x8[n]; }) * r;
-            Vf[4 * N3 + c] = wsum8([&](int n) { return v8[n] * y8[n]; }) * r;
-            Vf[5 * N3 + c] = wsum8([&](int n) { return v8[n] * z8[n]; }) * r;
+            Vf[3 * N3 + c] = wsum8([&](int n) { return x8[n]; }) * r;
+            Vf[4 * N3 + c] = wsum8([&](int n) { return y8[n]; }) * r;
+            Vf[5 * N3 + c] = wsum8([&](int n) { return z8[n]; }) * r;
         }
 #pragma unroll
         for (int n = 0; n < 7; n++) { zv[n] = zv[n + 1]; zx[n] = zx[n + 1]; zy[n] = zy[n + 1]; zz[n] = zz[n + 1]; }
@@ -410,18 +423,18 @@ __global__ __launch_bounds__(256) void k_flux(Geo g, MParm m, Tile b,
     for (int n = 0; n < 8; n++) vol[n] = met[M_VOL * N3 + c + (n - 4) * s];
     const double rVi = 1.0 / wsum8([&](int n) { return vol[n]; });
 
-    // L/R primitives (BlockStateLRs, recon.F90:39-93)
-    const double *Wh = H + H_WH * N3;
-    const double *Wl = m.doLFM ? H + H_WN * N3 : Wh;
+    // L/R primitives (BlockStateLRs, recon.F90:39-93); Wv holds V*W
+    const double *Wv = H + H_WH * N3;
+    const double *Wl = H + H_WN * N3;
     double WL[5], WR[5];
 #pragma unroll
     for (int v = 0; v < 5; v++) {
-        double q[8], nq[4];
+        double qv[8], nq[4];
 #pragma unroll
-        for (int n = 0; n < 8; n++) q[n] = Wh[v * N3 + c + (n - 4) * s];
+        for (int n = 0; n < 8; n++) qv[n] = Wv[v * N3 + c + (n - 4) * s];
 #pragma unroll
         for (int n = 0; n < 4; n++) nq[n] = Wl[v * N3 + c + (n - 2) * s];
-        recon_lr<RM>(m.pdmb, vol, rVi, nq, q, WL[v], WR[v]);
+        recon_lr_w<RM>(m.pdmb, rVi, nq, qv, WL[v], WR[v]);
     }
     double T[9];
 #pragma unroll
