@@ -264,9 +264,6 @@ extern "C" int gam_set_grid(gam_ctx *c, const gam_grid_arrays *a)
     if ((rc = upload(c, a->Teb, 1, 12, c->met + M_TEB * N3))) return rc;
     if ((rc = upload(c, a->face, 1, 3, c->met + M_FACE * N3))) return rc;
     if ((rc = upload(c, a->edge, 1, 3, c->met + M_EDGE * N3))) return rc;
-    launch_inv_face(c->stream, c->g, c->met);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(c->stream));
     c->haveGrid = 1;
     return GAM_OK;
 }

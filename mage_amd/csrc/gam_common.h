@@ -59,8 +59,7 @@ enum { H_GAS = 0, H_MF = 5, H_B = 8, H_WH = 11, H_WN = 16, H_NPL = 21 };
 enum {
     M_XYZCC = 0, M_XFC = 3, M_DI = 12, M_DJ = 13, M_DK = 14, M_VOL = 15,
     M_TF = 16, M_TE = 43, M_TEB = 70, M_FACE = 82, M_EDGE = 85, M_XYZ = 88,
-    M_IFACE = 91,   // 1/max(face, TINY), derived from M_FACE
-    M_NPL = 94
+    M_NPL = 91
 };
 // Background planes
 enum { B_FCB0 = 0, B_EDGB0 = 9, B_B0 = 15, B_BFLUX0 = 18, B_DPB0 = 21, B_NPL = 24 };

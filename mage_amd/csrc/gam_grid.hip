@@ -372,21 +372,6 @@ static void fill(hipStream_t s, double *p, long n, double v)
     hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, v, p);
 }
 
-// Reciprocal face areas 1/max(face, TINY): the divisor of every Phi/A in
-// the EMF and flux kernels, computed once per grid.
-__global__ void k_inv_face(long n, const double *__restrict__ face, double *__restrict__ iface)
-{
-    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < n) iface[q] = 1.0 / dmax(face[q], GAM_TINY);
-}
-
-void launch_inv_face(hipStream_t s, const Geo &g, double *met)
-{
-    const long n = 3 * g.N3;
-    hipLaunchKernelGGL(k_inv_face, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n,
-                       met + (long)M_FACE * g.N3, met + (long)M_IFACE * g.N3);
-}
-
 void launch_grid_from_corners(hipStream_t s, const Geo &g, double *met)
 {
     const long N3 = g.N3;
@@ -420,7 +405,6 @@ void launch_grid_from_corners(hipStream_t s, const Geo &g, double *met)
     hipLaunchKernelGGL((k_teb<1, 0, 2, 2>), gr3(b), dim3(64), 0, s, g, b, met);
     hipLaunchKernelGGL((k_teb<2, 0, 1, 0>), gr3(b), dim3(64), 0, s, g, b, met);
     hipLaunchKernelGGL((k_teb<2, 1, 0, 2>), gr3(b), dim3(64), 0, s, g, b, met);
-    launch_inv_face(s, g, met);
 }
 
 void launch_dipole_b0(hipStream_t s, const Geo &g, double M0, const double *met, double *bg,

@@ -49,6 +49,5 @@ void launch_mask_e(hipStream_t s, const Geo &g, const Box &all, const Box *eboxe
 
 // Grid pre-compute (gam_grid.hip)
 void launch_grid_from_corners(hipStream_t s, const Geo &g, double *met);
-void launch_inv_face(hipStream_t s, const Geo &g, double *met);
 void launch_dipole_b0(hipStream_t s, const Geo &g, double M0, const double *met, double *bg,
                       double *scratch9);
