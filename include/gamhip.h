@@ -101,6 +101,8 @@ const char *gam_last_error(void);
  * Replaces the allocations of Hatch/initSolver (init.F90:38-59, 486-508). */
 int gam_create(gam_ctx **ctx, const gam_dims *dims, const gam_model *model);
 int gam_destroy(gam_ctx *ctx);
+/* The sizes the context was created with. */
+int gam_get_dims(gam_ctx *ctx, gam_dims *dims);
 
 /* Upload Grid_T metrics computed by the host (the output of Corners2Grid,
  * init.F90:793-1072). */
@@ -137,6 +139,10 @@ int gam_get_state(gam_ctx *ctx, int which, double *Gas, double *magFlux,
 /* Model_T time fields (t, dt, ts) */
 int gam_set_time(gam_ctx *ctx, double t, double dt, int ts);
 int gam_get_time(gam_ctx *ctx, double *t, double *dt, int *ts);
+/* Model%dt0, the reference step of the CalcDT checks (step.F90:120-124,
+ * 164-166); set from a restart file by readH5Restart (gioH5.F90:814-820). */
+int gam_set_dt0(gam_ctx *ctx, double dt0);
+int gam_get_dt0(gam_ctx *ctx, double *dt0);
 
 /* Finish initialisation after the IC and background are in place: the
  * tail of CalcGridInfo (init.F90:200-244): bFlux2Fld, oState=State,

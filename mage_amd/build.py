@@ -55,5 +55,27 @@ def build_lib(verbose=False):
     return LIB
 
 
+# HDF5 C library used by the restart library (the image ships it under
+# /opt/conda); without it libgamres is not built and restarts are unavailable.
+HDF5_PREFIX = os.environ.get("GAM_HDF5_PREFIX", "/opt/conda")
+RES_LIB = os.path.join(HERE, "libgamres.so")
+
+
+def build_restart_lib():
+    """Compile libgamres.so (restart files over HDF5) if HDF5 is present;
+    returns its path or None."""
+    inc = os.path.join(HDF5_PREFIX, "include")
+    lib = os.path.join(HDF5_PREFIX, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5_hl.h")):
+        return None
+    tmp = RES_LIB + ".tmp"
+    _run(["gcc", "-O2", "-std=c11", "-shared", "-fPIC", "-Wall", "-I" + INCLUDE, "-I" + inc,
+          os.path.join(CSRC, "gam_restart.c"), "-o", tmp, "-L" + HERE, "-lgamhip",
+          "-L" + lib, "-lhdf5_hl", "-lhdf5", "-Wl,-rpath," + "$ORIGIN", "-Wl,-rpath," + lib])
+    os.replace(tmp, RES_LIB)
+    return RES_LIB
+
+
 if __name__ == "__main__":
     print(build_lib(verbose="-v" in sys.argv))
+    print(build_restart_lib())

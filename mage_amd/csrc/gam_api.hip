@@ -351,6 +351,7 @@ extern "C" int gam_set_wind(gam_ctx *c, int n, const double *tW, const double *Q
     if (!(dtMin > 0.0)) return fail(GAM_E_ARG, "gam_set_wind: times must increase");
     if (fabs(dtMin - dtMax) > 1.0e-6 * sqrt(dtMin * dtMin + dtMax * dtMax))
         return fail(GAM_E_ARG, "gam_set_wind: wind time series must be uniformly spaced");
+    HIPCHK(hipStreamSynchronize(c->stream));   // no pending step may still read the old series
     if (c->wind) { hipFree(c->wind); c->wind = nullptr; }
     HIPCHK(hipMalloc((void **)&c->wind, sizeof(double) * (size_t)n * 9));
     std::vector<double> h((size_t)n * 9);
@@ -438,10 +439,35 @@ extern "C" int gam_get_solver(gam_ctx *c, int which, double *out)
 extern "C" int gam_set_time(gam_ctx *c, double t, double dt, int ts)
 {
     if (!c) return fail(GAM_E_ARG, "gam_set_time: NULL context");
+    HIPCHK(hipStreamSynchronize(c->stream));
     Clock ck;
     HIPCHK(hipMemcpy(&ck, c->clk, sizeof(Clock), hipMemcpyDeviceToHost));
     ck.t = t; ck.dt = dt; ck.dtA = dt; ck.ts = ts;
     HIPCHK(hipMemcpy(c->clk, &ck, sizeof(Clock), hipMemcpyHostToDevice));
+    return GAM_OK;
+}
+
+extern "C" int gam_get_dims(gam_ctx *c, gam_dims *d)
+{
+    if (!c || !d) return fail(GAM_E_ARG, "gam_get_dims: NULL argument");
+    *d = c->dims;
+    return GAM_OK;
+}
+
+extern "C" int gam_set_dt0(gam_ctx *c, double dt0)
+{
+    if (!c) return fail(GAM_E_ARG, "gam_set_dt0: NULL context");
+    if (!(dt0 >= 0.0)) return fail(GAM_E_ARG, "gam_set_dt0: dt0 must be >= 0");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->mp.dt0 = dt0;
+    c->model.dt0 = dt0;
+    return GAM_OK;
+}
+
+extern "C" int gam_get_dt0(gam_ctx *c, double *dt0)
+{
+    if (!c || !dt0) return fail(GAM_E_ARG, "gam_get_dt0: NULL argument");
+    *dt0 = c->mp.dt0;
     return GAM_OK;
 }
 

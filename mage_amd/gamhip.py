@@ -26,9 +26,9 @@ ERRORS = {-1: "GAM_E_ARG", -2: "GAM_E_HIP", -3: "GAM_E_STATE", -4: "GAM_E_DT_BAD
 
 # Every symbol include/gamhip.h declares
 EXPORTS = (
-    "gam_last_error", "gam_create", "gam_destroy", "gam_set_grid", "gam_grid_from_corners",
+    "gam_last_error", "gam_create", "gam_destroy", "gam_get_dims", "gam_set_grid", "gam_grid_from_corners",
     "gam_get_grid", "gam_set_background", "gam_add_dipole_b0", "gam_get_background",
-    "gam_set_wind", "gam_put_state", "gam_get_state", "gam_set_time", "gam_get_time",
+    "gam_set_wind", "gam_put_state", "gam_get_state", "gam_set_time", "gam_get_time", "gam_set_dt0", "gam_get_dt0",
     "gam_init_run", "gam_advance_mhd", "gam_calc_dt", "gam_enforce_bcs", "gam_bflux2fld",
     "gam_step", "gam_run", "gam_advance_model", "gam_sync", "gam_divb", "gam_time_steps", "gam_set_profiling",
     "gam_kernel_times", "gam_kernel_group_name", "gam_get_solver",
@@ -94,6 +94,9 @@ def load():
     L.gam_get_state.argtypes = [vp, C.c_int, DP, DP, DP, DP, DP]
     L.gam_set_time.argtypes = [vp, C.c_double, C.c_double, C.c_int]
     L.gam_get_time.argtypes = [vp, DP, DP, C.POINTER(C.c_int)]
+    L.gam_get_dims.argtypes = [vp, C.POINTER(Dims)]
+    L.gam_set_dt0.argtypes = [vp, C.c_double]
+    L.gam_get_dt0.argtypes = [vp, DP]
     L.gam_init_run.argtypes = [vp, DP]
     L.gam_advance_mhd.argtypes = [vp, C.c_double]
     L.gam_calc_dt.argtypes = [vp, DP]
@@ -250,6 +253,14 @@ class GamHip:
     def set_time(self, t, dt, ts):
         self._chk(self.L.gam_set_time(self.ctx, float(t), float(dt), int(ts)))
 
+    def set_dt0(self, dt0):
+        self._chk(self.L.gam_set_dt0(self.ctx, float(dt0)))
+
+    def get_dt0(self):
+        v = C.c_double()
+        self._chk(self.L.gam_get_dt0(self.ctx, C.byref(v)))
+        return v.value
+
     def get_time(self):
         t, dt, ts = C.c_double(), C.c_double(), C.c_int()
         self._chk(self.L.gam_get_time(self.ctx, C.byref(t), C.byref(dt), C.byref(ts)))
@@ -314,3 +325,84 @@ class GamHip:
         n = (C.c_int * 8)()
         self._chk(self.L.gam_kernel_times(self.ctx, ms, n))
         return {self.L.gam_kernel_group_name(q).decode(): (ms[q], n[q]) for q in range(8)}
+
+
+# ---------------------------------------------------------------- restart files
+RES_LIBPATH = os.path.join(HERE, "libgamres.so")
+NG = 4
+RES_EXPORTS = ("gam_res_last_error", "gam_res_write", "gam_res_read", "gam_write_restart",
+               "gam_read_restart")
+
+
+class ResMeta(C.Structure):
+    """gam_res_meta: Model_T/State_T scalars of a restart."""
+    _fields_ = [("nOut", C.c_int), ("nRes", C.c_int), ("ts", C.c_int), ("t", C.c_double),
+                ("ot", C.c_double), ("dt", C.c_double), ("dt0", C.c_double)]
+
+
+_reslib = None
+
+
+def load_restart():
+    """Load libgamres.so (restart files over HDF5); raises if it was not built."""
+    global _reslib
+    if _reslib is not None:
+        return _reslib
+    load()
+    if not os.path.exists(RES_LIBPATH):
+        raise OSError("libgamres.so is not built (HDF5 missing when __graft_entry__.build() ran)")
+    L = C.CDLL(RES_LIBPATH)
+    vp = C.c_void_p
+    L.gam_res_last_error.restype = C.c_char_p
+    L.gam_res_write.argtypes = [C.c_char_p, C.POINTER(Dims), C.POINTER(ResMeta)] + [DP] * 7
+    L.gam_res_read.argtypes = [C.c_char_p, C.POINTER(Dims), C.POINTER(ResMeta)] + [DP] * 4
+    L.gam_write_restart.argtypes = [vp, C.c_char_p, DP, C.c_int, C.c_int]
+    L.gam_read_restart.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), DP]
+    _reslib = L
+    return L
+
+
+def _res_chk(L, rc):
+    if rc != 0:
+        raise GamError(rc, L.gam_res_last_error().decode())
+
+
+def res_write(path, dims, meta, xyz, st, ost):
+    """Write a restart from host arrays: st/ost dicts with Gas, magFlux and
+    optionally Bxyz; meta a dict of ResMeta fields."""
+    L = load_restart()
+    d = Dims(*dims, NG)
+    m = ResMeta(**meta)
+    _res_chk(L, L.gam_res_write(str(path).encode(), C.byref(d), C.byref(m), _p(xyz),
+                                _p(st["Gas"]), _p(st["magFlux"]), _p(st.get("Bxyz")),
+                                _p(ost["Gas"]), _p(ost["magFlux"]), _p(ost.get("Bxyz"))))
+
+
+def res_read(path, dims):
+    """Read a restart into host arrays: (meta dict, state dict, oState dict)."""
+    L = load_restart()
+    Nip, Njp, Nkp = dims
+    c = (Nip + 2 * NG, Njp + 2 * NG, Nkp + 2 * NG)
+    f = (c[0] + 1, c[1] + 1, c[2] + 1)
+    st = {"Gas": np.zeros(c + (5,), order="F"), "magFlux": np.zeros(f + (3,), order="F")}
+    ost = {"Gas": np.zeros(c + (5,), order="F"), "magFlux": np.zeros(f + (3,), order="F")}
+    d = Dims(Nip, Njp, Nkp, NG)
+    m = ResMeta()
+    _res_chk(L, L.gam_res_read(str(path).encode(), C.byref(d), C.byref(m), _p(st["Gas"]),
+                               _p(st["magFlux"]), _p(ost["Gas"]), _p(ost["magFlux"])))
+    return {k: getattr(m, k) for k, _ in ResMeta._fields_}, st, ost
+
+
+def write_restart(g, path, xyz=None, nOut=0, nRes=0):
+    """WriteRestart of a GamHip context."""
+    L = load_restart()
+    _res_chk(L, L.gam_write_restart(g.ctx, str(path).encode(), _p(xyz), int(nOut), int(nRes)))
+
+
+def read_restart(g, path):
+    """Restart a GamHip context from a file: returns (nOut, nRes, dt)."""
+    L = load_restart()
+    nOut, nRes, dt = C.c_int(), C.c_int(), C.c_double()
+    _res_chk(L, L.gam_read_restart(g.ctx, str(path).encode(), C.byref(nOut), C.byref(nRes),
+                                   C.byref(dt)))
+    return nOut.value, nRes.value, dt.value

@@ -27,6 +27,19 @@ def test_header_symbols_exported():
     assert sorted(gamhip.EXPORTS) == names
 
 
+def test_restart_library_exports():
+    """libgamres exports every symbol include/gamres.h declares."""
+    if not os.path.exists(gamhip.RES_LIBPATH):
+        pytest.skip("libgamres.so not built (no HDF5)")
+    src = open(os.path.join(ROOT, "include", "gamres.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = sorted(set(re.findall(r"\b(gam_[a-z0-9_]+)\s*\(", src)))
+    L = gamhip.load_restart()
+    for n in names:
+        assert hasattr(L, n), n
+    assert sorted(gamhip.RES_EXPORTS) == names
+
+
 def _create(Nip, Njp, Nkp, **over):
     L = gamhip.load()
     ctx = C.c_void_p()
