@@ -114,10 +114,13 @@ def test_wrong_grid_and_missing_file(tmp_path):
 
 @pytest.mark.gpu
 def test_restarted_context_continues_the_run(tmp_path):
-    """Run A: 10 steps, write a restart, 10 more.  Run B: a new context on
-    the same grid reads the restart and runs 10 steps.  The restart carries
-    State, oState, t, ts and dt0 exactly; dt is recomputed from the restored
-    state (init.F90:238), so B matches A to round-off."""
+    """Run A: 10 steps, write a restart.  Run B: a new context on the same
+    grid reads it.  The restart carries State, oState, t, ts and dt0
+    exactly; Model%dt is recomputed from the restored state (init.F90:238),
+    i.e. after the boundary pass, where the step computed it before
+    (gamapp.F90:81-86), so the two can differ in the last digits.  A then
+    takes the same CalcDT in place, and both run 10 more steps: the runs
+    must agree."""
     from helpers import shell_case
     gA, o, grid, st, p = shell_case((24, 24, 32))
     gA.init_run()
@@ -125,6 +128,8 @@ def test_restarted_context_continues_the_run(tmp_path):
     f = tmp_path / "shell.gam.Res.00000.h5"
     gamhip.write_restart(gA, f, grid["xyz"], nOut=0, nRes=0)
     a10 = gA.get_state(), gA.get_state(gamhip.GAM_OSTATE), gA.get_time()
+    dtA = gA.calc_dt()
+    gA.set_time(a10[2][0], dtA, a10[2][2])
     gA.run(10)
     a20 = gA.get_state()
 
@@ -141,11 +146,12 @@ def test_restarted_context_continues_the_run(tmp_path):
         assert np.array_equal(bo10[k], a10[1][k]), k
     t, dtB, ts = gB.get_time()
     assert t == a10[2][0] and ts == a10[2][2]
-    assert abs(dtB - a10[2][1]) <= 1e-12 * a10[2][1]
+    assert dtB == dt == dtA
+    assert abs(dtB - a10[2][1]) <= 1e-5 * a10[2][1]
     gB.run(10)
     b20 = gB.get_state()
     for k in ("Gas", "magFlux", "Bxyz"):
         den = np.max(np.abs(a20[k]))
-        assert np.max(np.abs(b20[k] - a20[k])) <= 1e-10 * den, k
+        assert np.max(np.abs(b20[k] - a20[k])) <= 1e-13 * den, k
     gA.close()
     gB.close()
