@@ -33,19 +33,23 @@ __device__ __forceinline__ bool in_box(const Box &b, int i, int j, int k)
     const int k = (b).k0 + blockIdx.z;                  \
     if (i > (b).i1 || j > (b).j1) return;
 
-// XCD-aware tiling.  A launch covers box b plane by plane: the points of
-// each (i, ax1) plane (ax1 = 1: j, 2: k) are numbered i fastest and cut
-// into chunks of 256 consecutive points, one per workgroup, so no lane idles
-// however the row length falls against 64; chunks are numbered within the
-// plane first, then plane by plane along the remaining axis.  The hardware
-// hands workgroup w to XCD w % 8, so logical chunk L = (w % 8) * per + w / 8
-// gives every XCD one contiguous run: chunks that share stencil rows along
-// ax1 run back to back on the same L2.
+// XCD-aware tiling.  A launch covers box b row by row (a row = the points
+// of one (j, k) along i) and cuts the rows, i fastest, into chunks of 256
+// consecutive points, one per workgroup, so no lane idles however the row
+// length falls against 64.  Row order, by ax1:
+//   1: j fastest, then k;   2: k fastest, then j;
+//   3: slabs of RB j-rows: j fastest within the slab, then k, then the
+//      next slab -- the rows at j+1 and at k+1 are both at most RB rows
+//      away, for kernels that read neighbours along j and along k.
+// The hardware hands workgroup w to XCD w % 8, so logical chunk
+// L = (w % 8) * per + w / 8 gives every XCD one contiguous run of rows:
+// chunks that share stencil rows run close together on the same L2.
+#define RB 8
 struct Tile {
     Box b;
     int ax1;
-    int ni, na;       // extents along i and ax1
-    int nc, nb;       // chunks per plane, chunks in all
+    int ni, nj, nk;   // extents
+    int nb;           // chunks in all
 };
 
 __device__ __forceinline__ bool tile_point(const Tile &t, int &i, int &j, int &k)
@@ -55,14 +59,32 @@ __device__ __forceinline__ bool tile_point(const Tile &t, int &i, int &j, int &k
     const int L = (w & 7) * per + (w >> 3);
     i = j = k = 0x40000000;
     if (L >= t.nb) return false;
-    const int ch = L % t.nc;
-    const int t2 = L / t.nc;
-    const unsigned p = (unsigned)ch * (BX * BY) + threadIdx.x + BX * threadIdx.y;
-    if (p >= (unsigned)(t.ni * t.na)) return false;
-    const int a = (int)(p / (unsigned)t.ni);
-    i = t.b.i0 + (int)(p - (unsigned)a * t.ni);
-    if (t.ax1 == 2) { k = t.b.k0 + a; j = t.b.j0 + t2; }
-    else { j = t.b.j0 + a; k = t.b.k0 + t2; }
+    const unsigned p = (unsigned)L * (BX * BY) + threadIdx.x + BX * threadIdx.y;
+    if (p >= (unsigned)t.ni * (unsigned)(t.nj * t.nk)) return false;
+    const unsigned R = p / (unsigned)t.ni;
+    i = t.b.i0 + (int)(p - R * (unsigned)t.ni);
+    int jj, kk;
+    if (t.ax1 == 1) {
+        kk = (int)(R / (unsigned)t.nj); jj = (int)(R - (unsigned)kk * t.nj);
+    } else if (t.ax1 == 2) {
+        jj = (int)(R / (unsigned)t.nk); kk = (int)(R - (unsigned)jj * t.nk);
+    } else {
+        const unsigned slab = (unsigned)RB * t.nk;
+        const unsigned nfull = (unsigned)(t.nj / RB);
+        const unsigned sb = R / slab;
+        if (sb < nfull) {
+            const unsigned r = R - sb * slab;
+            kk = (int)(r / RB);
+            jj = (int)(sb * RB + (r % RB));
+        } else {
+            const unsigned r = R - nfull * slab;
+            const unsigned rj = (unsigned)t.nj - nfull * RB;
+            kk = (int)(r / rj);
+            jj = (int)(nfull * RB + (r - (unsigned)kk * rj));
+        }
+    }
+    j = t.b.j0 + jj;
+    k = t.b.k0 + kk;
     return true;
 }
 
@@ -75,11 +97,11 @@ static inline Tile tile_of(const Box &b, int ax1)
     Tile t;
     t.b = b;
     t.ax1 = ax1;
-    const int ni = b.i1 - b.i0 + 1, nj = b.j1 - b.j0 + 1, nk = b.k1 - b.k0 + 1;
-    t.ni = ni;
-    t.na = (ax1 == 2 ? nk : nj);
-    t.nc = (ni * t.na + BX * BY - 1) / (BX * BY);
-    t.nb = t.nc * (ax1 == 2 ? nj : nk);
+    t.ni = b.i1 - b.i0 + 1;
+    t.nj = b.j1 - b.j0 + 1;
+    t.nk = b.k1 - b.k0 + 1;
+    const long pts = (long)t.ni * t.nj * t.nk;
+    t.nb = (int)((pts + BX * BY - 1) / (BX * BY));
     return t;
 }
 static inline dim3 tgrid(const Tile &t) { return dim3((unsigned)(((t.nb + 7) / 8) * 8)); }
@@ -825,6 +847,13 @@ __device__ __forceinline__ bool is_cell(const Geo &g, int i, int j, int k)
     return i >= g.isg && i <= g.ieg && j >= g.jsg && j <= g.jeg && k >= g.ksg && k <= g.keg;
 }
 
+// Ghost layers n = 1..4: the i-side kernels (one thread per (j, k) row of
+// 4-wide ghost slabs, uncoalesced along i) take n from blockIdx.z, one
+// layer per thread, for four times the parallelism; the j- and k-side
+// kernels (threads along i, coalesced) loop over n.
+#define NL0(side) (((side) <= 1) ? 1 + (int)blockIdx.z : 1)
+#define NL1(side) (((side) <= 1) ? 1 + (int)blockIdx.z : 4)
+
 // periodic (bcs.F90:279-450).  side: 0..5.  (a,b) = transverse indices
 template <int SIDE>
 __global__ void k_bc_periodic(Geo g, Box t, double *__restrict__ S)
@@ -834,7 +863,7 @@ __global__ void k_bc_periodic(Geo g, Box t, double *__restrict__ S)
     if (a > t.i1 || b > t.j1) return;
     const long N3 = g.N3;
     double *mf = S + S_MF * N3;
-    for (int n = 1; n <= 4; n++) {
+    for (int n = NL0(SIDE); n <= NL1(SIDE); n++) {
         if (SIDE == 0) {        // inner I: a=j, b=k
             const int ig = g.is - n, sr = g.ie - n + 1;
             const long d = IX(g, ig, a, b), s = IX(g, sr, a, b);
@@ -886,7 +915,7 @@ __global__ void k_bc_copy(Geo g, Box t, double *__restrict__ S, const double *__
     double *mf = S + S_MF * N3;
     if (SIDE == 0) {            // inner I: a=j, b=k
         const long s = IX(g, g.is, a, b);
-        for (int n = 1; n <= 4; n++) {
+        for (int n = NL0(SIDE); n <= NL1(SIDE); n++) {
             const long d = IX(g, g.is - n, a, b);
             copy_cell(S, N3, d, s);
             for (int x = 0; x < 3; x++) mf[x * N3 + d] = mf[x * N3 + s];
@@ -895,7 +924,7 @@ __global__ void k_bc_copy(Geo g, Box t, double *__restrict__ S, const double *__
     } else if (SIDE == 1) {     // outer I
         const long s = IX(g, g.ie, a, b);
         if (KIND == 3) {        // SphOut
-            for (int n = 1; n <= 4; n++) {
+            for (int n = NL0(SIDE); n <= NL1(SIDE); n++) {
                 const int ig = g.ie + n;
                 const long d = IX(g, ig, a, b);
                 copy_cell(S, N3, d, s);
@@ -908,7 +937,7 @@ __global__ void k_bc_copy(Geo g, Box t, double *__restrict__ S, const double *__
             double B[3];
             cell_b(g, met, s, mf[s], mf[N3 + s], mf[2 * N3 + s], mf[s + 1], mf[N3 + s + g.P],
                    mf[2 * N3 + s + g.PP], B);
-            for (int n = 1; n <= 4; n++) {
+            for (int n = NL0(SIDE); n <= NL1(SIDE); n++) {
                 const int ig = g.ie + n;
                 const long d = IX(g, ig, a, b);
                 copy_cell(S, N3, d, s);
@@ -922,14 +951,14 @@ __global__ void k_bc_copy(Geo g, Box t, double *__restrict__ S, const double *__
         }
     } else if (SIDE == 2) {     // inner J: a=i, b=k
         const long s = IX(g, a, g.js, b);
-        for (int n = 1; n <= 4; n++) {
+        for (int n = NL0(SIDE); n <= NL1(SIDE); n++) {
             const long d = IX(g, a, g.js - n, b);
             copy_cell(S, N3, d, s);
             for (int x = 0; x < 3; x++) mf[x * N3 + d] = mf[x * N3 + s];
         }
     } else if (SIDE == 3) {     // outer J
         const long s = IX(g, a, g.je, b);
-        for (int n = 1; n <= 4; n++) {
+        for (int n = NL0(SIDE); n <= NL1(SIDE); n++) {
             const long d = IX(g, a, g.je + n, b);
             copy_cell(S, N3, d, s);
             mf[d] = mf[s];
@@ -938,14 +967,14 @@ __global__ void k_bc_copy(Geo g, Box t, double *__restrict__ S, const double *__
         }
     } else if (SIDE == 4) {     // inner K: a=i, b=j
         const long s = IX(g, a, b, g.ks);
-        for (int n = 1; n <= 4; n++) {
+        for (int n = NL0(SIDE); n <= NL1(SIDE); n++) {
             const long d = IX(g, a, b, g.ks - n);
             copy_cell(S, N3, d, s);
             for (int x = 0; x < 3; x++) mf[x * N3 + d] = mf[x * N3 + s];
         }
     } else {                    // outer K; the k+1 face takes the J flux (bcs.F90:925)
         const long s = IX(g, a, b, g.ke);
-        for (int n = 1; n <= 4; n++) {
+        for (int n = NL0(SIDE); n <= NL1(SIDE); n++) {
             const long d = IX(g, a, b, g.ke + n);
             copy_cell(S, N3, d, s);
             mf[d] = mf[s];
@@ -1027,7 +1056,7 @@ __global__ void k_bc_wind(Geo g, MParm m, Box t, const Clock *clk, WindDev w,
     double vF[3];
     wind_front(w, tm, vF);
     const int ip = g.ie;
-    for (int n = 1; n <= 4; n++) {
+    for (int n = NL0(1); n <= NL1(1); n++) {
         const int ig = g.ie + n;
         if (k <= g.keg && j <= g.jeg) {
             const long cg = IX(g, ig, j, k), cp = IX(g, ip, j, k);
@@ -1095,7 +1124,7 @@ __global__ void k_bc_wind_b(Geo g, Box t, double *__restrict__ S, const double *
     int kp = k;
     if (k < g.ks) kp = k + g.Nkp;
     else if (k > g.ke) kp = k - g.Nkp;
-    for (int n = 1; n <= 4; n++) {
+    for (int n = NL0(1); n <= NL1(1); n++) {
         const int ig = g.ie + n;
         const long s = IX(g, ig, j, kp);
         double B[3];
@@ -1180,7 +1209,7 @@ __global__ void k_mask_e(Geo g, Box all, EdgeBoxes eb, double *__restrict__ E)
 void launch_predict(hipStream_t s, const Geo &g, const MParm &m, const Box &fb, const Clock *clk,
                     const double *S, const double *O, const double *met, double *H)
 {
-    const Tile t = tile_of(fb, 1);
+    const Tile t = tile_of(fb, 3);
     LAUNCH(k_predict, tgrid(t), dim3(BX, BY), s, g, m, t, clk, S, O, met, H);
 }
 
@@ -1197,8 +1226,9 @@ static void emf_t(hipStream_t s, const Geo &g, const MParm &m, const Box &b, con
                   const double *H, const double *S, const double *Vf, const double *met,
                   const double *bg, double *E)
 {
-    // stencils run along T2 and T1; tile along k whenever one of them is k
-    const Tile t = tile_of(b, ED == 2 ? 1 : 2);
+    // stencils run along T2 and T1: rows along j for K edges, along k for J
+    // edges (both stencils along i or k), j-slabs for I edges (j and k)
+    const Tile t = tile_of(b, ED == 2 ? 1 : (ED == 0 ? 3 : 2));
     LAUNCH((k_emf<ED, RM, BG, BO>), tgrid(t), dim3(BX, BY), s, g, m, t, clk, H, S, Vf, met, bg, E);
 }
 
@@ -1269,7 +1299,7 @@ void launch_update(hipStream_t s, const Geo &g, const MParm &m, const Box &fb, c
     EdgeBoxes eb;
     for (int d = 0; d < 3; d++) eb.e[d] = eboxes[d];
     const int sel = (m.doBG ? 2 : 0) + (m.doBoris ? 1 : 0);
-    const Tile t = tile_of(fb, 1);
+    const Tile t = tile_of(fb, 3);
     switch (sel) {
     case 0: LAUNCH((k_update<false, false>), tgrid(t), dim3(BX, BY), s, g, m, t, ctb, eb, ab, dtb, clk, So, Sn, gf, mf, met, bg); break;
     case 1: LAUNCH((k_update<false, true>), tgrid(t), dim3(BX, BY), s, g, m, t, ctb, eb, ab, dtb, clk, So, Sn, gf, mf, met, bg); break;
@@ -1339,6 +1369,7 @@ void launch_bc(hipStream_t s, int side, int kind, const Geo &g, const MParm &m, 
     else { t.i0 = g.isg; t.i1 = g.ieg + (pi ? 1 : 0); t.j0 = g.jsg; t.j1 = g.jeg + (pi ? 1 : 0); }
     t.k0 = t.k1 = 0;
     dim3 gr = grid2(t), bl(64);
+    if (side <= 1) gr.z = 4;    // one ghost layer per blockIdx.z
     if (kind == 0) {
         switch (side) {
         case 0: LAUNCH(k_bc_periodic<0>, gr, bl, s, g, t, S); break;
@@ -1364,7 +1395,9 @@ void launch_bc(hipStream_t s, int side, int kind, const Geo &g, const MParm &m, 
     } else if (kind == 4) {
         Box tw = t;
         tw.i1 = g.jeg + 1; tw.j1 = g.keg + 1;
-        LAUNCH(k_bc_wind, grid2(tw), bl, s, g, m, tw, clk, *w, S, met);
+        dim3 gw = grid2(tw);
+        gw.z = 4;
+        LAUNCH(k_bc_wind, gw, bl, s, g, m, tw, clk, *w, S, met);
         // ghost-cell B of the wind BC (wind.F90:338-348); skipped when a
         // bFlux2Fld over every ghost cell follows, which overwrites it
         if (ghostB) LAUNCH(k_bc_wind_b, gr, bl, s, g, t, S, met);
