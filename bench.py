@@ -112,6 +112,7 @@ def main():
     ap.add_argument("--grid", type=str, default="192x192x256")
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-box", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
@@ -150,6 +151,25 @@ def main():
     t, dt, ts = g.get_time()
     log("kernel ms/launch: " + ", ".join("%s=%.3f" % (k, v["ms_per_launch"] or 0) for k, v in kernels.items()))
     g.close()
+
+    # The plain-step workload: 64^3 periodic box, 100 steps (8CENT, no B0)
+    box = None
+    if not args.no_box:
+        Nb = (64, 64, 64)
+        xyzb = pb.cartesian_corners(*Nb)
+        gb = gamhip.GamHip(*Nb, pb.box_params("8CENT"))
+        gb.grid_from_corners(xyzb)
+        stb = pb.periodic_box_state(*Nb, gb.get_grid(fields=("xyzcc",))["xyzcc"], xyzb)
+        gb.put_state(stb)
+        gb.init_run()
+        gb.run(5)
+        msb = gb.time_steps(100)
+        gb.close()
+        box = {"workload": "64x64x64 periodic box, seeded modes, 8CENT, 100 steps",
+               "ms_per_step": msb / 100, "value": 64 ** 3 * 100 / (msb * 1e-3),
+               "unit": "cell-updates/s",
+               "step_frac_of_hbm_roofline": 8 * (19 + 14 + 79) * 64 ** 3 * 100 / (msb * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        log("box 64^3: %.3f ms/step, %.4e cell-updates/s" % (box["ms_per_step"], box["value"]))
 
     cpu = None
     if not args.no_cpu:
@@ -199,6 +219,7 @@ def main():
             "frac": step_gbs / HBM_PEAK_GBS,
         },
         "mhd_cell_updates_per_s": mhd_value,
+        "box_64cubed": box,
         "kernels": kernels,
         "final": {"t": t, "dt": dt, "ts": ts},
         "cpu_baseline": cpu,

@@ -139,6 +139,27 @@ def test_box_100_steps_and_divb():
     g.close()
 
 
+@pytest.mark.parametrize("rmeth", ["8CENT", "7UP"])
+def test_box_64_cubed_100_steps(rmeth):
+    """The plain-step workload itself: the 64^3 periodic box with seeded
+    Fourier-mode density, velocity and vector potential, 100 steps, against
+    the oracle at the multi-step bar, with div B at round-off."""
+    print("box 64^3 %s: 100 steps on device and oracle" % rmeth, flush=True)
+    g, o, grid, st, p = box_case((64, 64, 64), rmeth)
+    g.init_run()
+    o.init_run()
+    g.run(100)
+    for _ in range(100):
+        o.step()
+    e = compare_state(g, o)
+    print("box 64^3 %s: rel Linf %s" % (rmeth, {k: "%.2e" % v for k, v in e.items()}), flush=True)
+    assert max(e.values()) <= LONG_TOL, e
+    mf = g.get_state()["magFlux"]
+    scale = float(np.sum(np.abs(mf[4:69, 4:69, 4:69])))
+    assert g.divb() <= 1e-12 * scale
+    g.close()
+
+
 def test_shell_100_steps_and_divb():
     g, o, grid, st, p = shell_case((16, 16, 24), "7UP")
     g.init_run()
