@@ -74,6 +74,30 @@ template <int D> __host__ __device__ inline long STRIDE(const Geo &g)
     return D == 0 ? 1L : (D == 1 ? (long)g.P : g.PP);
 }
 
+// Load the double at byte offset `off` (32 bits: planes are far below
+// 4 GiB) from a plane base that is uniform across the wavefront.  Written
+// this way the compiler emits global loads with a scalar base and a 32-bit
+// vector offset: one offset register serves every plane of a stencil
+// point, with no 64-bit address arithmetic per load.
+__device__ __forceinline__ double ldp(const double *__restrict__ plane, unsigned off)
+{
+    return *(const double *)((const char *)plane + off);
+}
+__device__ __forceinline__ void stp(double *__restrict__ plane, unsigned off, double v)
+{
+    *(double *)((char *)plane + off) = v;
+}
+// Point c + d*s of a plane whose stencil runs along axis AX (stride s):
+// along i (AX = 0) plain indexing, which the compiler turns into wide loads
+// of contiguous points; along j and k a 32-bit byte offset from the plane
+// base (see ldp), the same for every plane at that point.
+template <int AX>
+__device__ __forceinline__ double ldx(const double *__restrict__ plane, long c, long s, int d)
+{
+    if (AX == 0) return plane[c + d];
+    return ldp(plane, (unsigned)((c + d * s) * 8));
+}
+
 __device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
 __device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
 

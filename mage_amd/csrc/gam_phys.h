@@ -195,18 +195,20 @@ __device__ __forceinline__ void cell_b(const Geo &g, const double *met, long c,
                                        double fIp, double fJp, double fKp, double *B)
 {
     const long N3 = g.N3;
-    double idV = 1.0 / met[M_VOL * N3 + c];
+    const unsigned oc = (unsigned)(c * 8), oI = oc + 8, oJ = oc + 8 * g.P,
+                   oK = oc + (unsigned)(8 * g.PP);
+    double idV = 1.0 / ldp(met + M_VOL * N3, oc);
     double div = fIp - fI + fJp - fJ + fKp - fK;
 #pragma unroll
     for (int x = 0; x < 3; x++) {
-        double xI = met[(M_XFC + x + 0) * N3 + c];
-        double xJ = met[(M_XFC + x + 3) * N3 + c];
-        double xK = met[(M_XFC + x + 6) * N3 + c];
-        double xIp = met[(M_XFC + x + 0) * N3 + c + 1];
-        double xJp = met[(M_XFC + x + 3) * N3 + c + g.P];
-        double xKp = met[(M_XFC + x + 6) * N3 + c + g.PP];
+        double xI = ldp(met + (M_XFC + x + 0) * N3, oc);
+        double xJ = ldp(met + (M_XFC + x + 3) * N3, oc);
+        double xK = ldp(met + (M_XFC + x + 6) * N3, oc);
+        double xIp = ldp(met + (M_XFC + x + 0) * N3, oI);
+        double xJp = ldp(met + (M_XFC + x + 3) * N3, oJ);
+        double xKp = ldp(met + (M_XFC + x + 6) * N3, oK);
         double b = (fIp * xIp + fJp * xJp + fKp * xKp - fI * xI - fJ * xJ - fK * xK) * idV;
-        B[x] = b - div * met[(M_XYZCC + x) * N3 + c] * idV;
+        B[x] = b - div * ldp(met + (M_XYZCC + x) * N3, oc) * idV;
     }
 }
 

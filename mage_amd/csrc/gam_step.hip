@@ -135,6 +135,7 @@ __global__ __launch_bounds__(256) void k_predict(Geo g, MParm m, Tile fb, const 
     TILE_IJK(fb);
     const long N3 = g.N3;
     const long c = IX(g, i, j, k);
+    const unsigned oc = (unsigned)(c * 8);    // byte offset of the point
     const double st = clk->st, dtM = clk->dt;
     double ost = clk->ost;
     if (fabs(ost - st) < GAM_TINY) ost = st - dtM;   // mhdgroup.F90:26-28
@@ -145,32 +146,33 @@ __global__ __launch_bounds__(256) void k_predict(Geo g, MParm m, Tile fb, const 
     double f[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        f[d] = m.doMHD ? ab2(S[(S_MF + d) * N3 + c], O[(S_MF + d) * N3 + c], w)
-                       : H[(H_MF + d) * N3 + c];
-        H[(H_MF + d) * N3 + c] = f[d];
+        f[d] = m.doMHD ? ab2(ldp(S + (S_MF + d) * N3, oc), ldp(O + (S_MF + d) * N3, oc), w)
+                       : ldp(H + (H_MF + d) * N3, oc);
+        stp(H + (H_MF + d) * N3, oc, f[d]);
     }
     if (i > g.ieg || j > g.jeg || k > g.keg) return;
 
     // half-state cell B from the six extrapolated face fluxes
     double fIp, fJp, fKp;
+    const unsigned oI = oc + 8, oJ = oc + 8 * g.P, oK = oc + (unsigned)(8 * g.PP);
     if (m.doMHD) {
-        fIp = ab2(S[(S_MF + 0) * N3 + c + 1], O[(S_MF + 0) * N3 + c + 1], w);
-        fJp = ab2(S[(S_MF + 1) * N3 + c + g.P], O[(S_MF + 1) * N3 + c + g.P], w);
-        fKp = ab2(S[(S_MF + 2) * N3 + c + g.PP], O[(S_MF + 2) * N3 + c + g.PP], w);
+        fIp = ab2(ldp(S + (S_MF + 0) * N3, oI), ldp(O + (S_MF + 0) * N3, oI), w);
+        fJp = ab2(ldp(S + (S_MF + 1) * N3, oJ), ldp(O + (S_MF + 1) * N3, oJ), w);
+        fKp = ab2(ldp(S + (S_MF + 2) * N3, oK), ldp(O + (S_MF + 2) * N3, oK), w);
     } else {
-        fIp = H[(H_MF + 0) * N3 + c + 1];
-        fJp = H[(H_MF + 1) * N3 + c + g.P];
-        fKp = H[(H_MF + 2) * N3 + c + g.PP];
+        fIp = ldp(H + (H_MF + 0) * N3, oI);
+        fJp = ldp(H + (H_MF + 1) * N3, oJ);
+        fKp = ldp(H + (H_MF + 2) * N3, oK);
     }
     double B[3];
     cell_b(g, met, c, f[0], f[1], f[2], fIp, fJp, fKp, B);
 #pragma unroll
-    for (int x = 0; x < 3; x++) H[(H_B + x) * N3 + c] = B[x];
+    for (int x = 0; x < 3; x++) stp(H + (H_B + x) * N3, oc, B[x]);
 
     // PredictCell (mhdgroup.F90:528-561)
     double Q[5], oQ[5], W[5], oW[5], pW[5], pQ[5], Wb[5];
 #pragma unroll
-    for (int v = 0; v < 5; v++) { Q[v] = S[(S_GAS + v) * N3 + c]; oQ[v] = O[(S_GAS + v) * N3 + c]; }
+    for (int v = 0; v < 5; v++) { Q[v] = ldp(S + (S_GAS + v) * N3, oc); oQ[v] = ldp(O + (S_GAS + v) * N3, oc); }
     c2p_cell(m, oQ, oW);
     c2p_cell(m, Q, W);
 #pragma unroll
@@ -179,7 +181,7 @@ __global__ __launch_bounds__(256) void k_predict(Geo g, MParm m, Tile fb, const 
     pW[PRS] = dmax(pW[PRS], m.pFloor);
     p2c_cell(m, pW, pQ);
 #pragma unroll
-    for (int v = 0; v < 5; v++) H[(H_GAS + v) * N3 + c] = pQ[v];
+    for (int v = 0; v < 5; v++) stp(H + (H_GAS + v) * N3, oc, pQ[v]);
     // primitives of the reconstruction stencils (half state), stored
     // multiplied by the cell volume: every use is the volume-weighted
     // interpolation <V q>/<V> (recon.F90:125-198, fields.F90:262-270), and
@@ -187,12 +189,12 @@ __global__ __launch_bounds__(256) void k_predict(Geo g, MParm m, Tile fb, const 
     // stencils take plain primitives: of state n (doLFMLim), else of the
     // half state.
     c2p_block(m, pQ, Wb);
-    const double vol = met[M_VOL * N3 + c];
+    const double vol = ldp(met + M_VOL * N3, oc);
 #pragma unroll
-    for (int v = 0; v < 5; v++) H[(H_WH + v) * N3 + c] = vol * Wb[v];
+    for (int v = 0; v < 5; v++) stp(H + (H_WH + v) * N3, oc, vol * Wb[v]);
     if (m.doLFM) c2p_block(m, Q, Wb);
 #pragma unroll
-    for (int v = 0; v < 5; v++) H[(H_WN + v) * N3 + c] = Wb[v];
+    for (int v = 0; v < 5; v++) stp(H + (H_WN + v) * N3, oc, Wb[v]);
 }
 
 // ====================================================================
@@ -340,13 +342,12 @@ __device__ __forceinline__ void edge_lr(const Geo &g, const MParm &m, long c,
     double A[8], q[8], nq[4];
 #pragma unroll
     for (int n = 0; n < 8; n++) {
-        const long p = c + (n - 4) * s;
-        A[n] = met[(M_FACE + FD) * N3 + p];
-        q[n] = mfH[FD * N3 + p] / dmax(A[n], GAM_TINY);
+        A[n] = ldx<SD>(met + (M_FACE + FD) * N3, c, s, n - 4);
+        q[n] = ldx<SD>(mfH + FD * N3, c, s, n - 4) / dmax(A[n], GAM_TINY);
     }
     // the limiter faces are the middle four of the stencil
 #pragma unroll
-    for (int n = 0; n < 4; n++) nq[n] = mfL[FD * N3 + c + (n - 2) * s] / dmax(A[n + 2], GAM_TINY);
+    for (int n = 0; n < 4; n++) nq[n] = ldx<SD>(mfL + FD * N3, c, s, n - 2) / dmax(A[n + 2], GAM_TINY);
     double Vi = wsum8([&](int n) { return A[n]; });
     double l, r;
     recon_lr<RM>(m.pdmb, A, 1.0 / Vi, nq, q, l, r);
@@ -377,12 +378,12 @@ __global__ __launch_bounds__(256) void k_emf(Geo g, MParm m, Tile b, const Clock
     // GetCornerV (fields.F90:457-517), doVdA
     double A[8];
 #pragma unroll
-    for (int n = 0; n < 8; n++) A[n] = met[(M_FACE + T1) * N3 + c + (n - 4) * s2];
+    for (int n = 0; n < 8; n++) A[n] = ldx<T2>(met + (M_FACE + T1) * N3, c, s2, n - 4);
     const double rdA = 1.0 / dmax(wsum8([&](int n) { return A[n]; }), GAM_TINY);
     double vc[3];
 #pragma unroll
     for (int d = 0; d < 3; d++)
-        vc[d] = wsum8([&](int n) { return A[n] * Vf[d * N3 + c + (n - 4) * s2]; }) * rdA;
+        vc[d] = wsum8([&](int n) { return A[n] * ldx<T2>(Vf + d * N3, c, s2, n - 4); }) * rdA;
     const double *Te = met + (long)(M_TE + 9 * ED) * N3 + c;
     const double v1 = vc[0] * Te[3 * N3] + vc[1] * Te[4 * N3] + vc[2] * Te[5 * N3];
     const double v2 = vc[0] * Te[6 * N3] + vc[1] * Te[7 * N3] + vc[2] * Te[8 * N3];
@@ -439,10 +440,21 @@ __global__ __launch_bounds__(256) void k_flux(Geo g, MParm m, Tile b,
     const long N3 = g.N3;
     const long c = IX(g, i, j, k);
     const long s = STRIDE<D>(g);
+    // byte offsets of the 8 stencil points (cells c-4s .. c+3s), shared by
+    // every plane; o[4] is the face point c itself.  Along i the points are
+    // contiguous and the compiler merges them into wide loads from one
+    // address, so that direction keeps plain indexing.
+    unsigned o[8];
+#pragma unroll
+    for (int n = 0; n < 8; n++) o[n] = (unsigned)((c + (n - 4) * s) * 8);
+    auto LD = [&](const double *__restrict__ plane, int n) -> double {
+        if (D == 0) return plane[c + (n - 4)];
+        return ldp(plane, o[n]);
+    };
 
     double vol[8];
 #pragma unroll
-    for (int n = 0; n < 8; n++) vol[n] = met[M_VOL * N3 + c + (n - 4) * s];
+    for (int n = 0; n < 8; n++) vol[n] = LD(met + M_VOL * N3, n);
     const double rVi = 1.0 / wsum8([&](int n) { return vol[n]; });
 
     // L/R primitives (BlockStateLRs, recon.F90:39-93); Wv holds V*W
@@ -453,15 +465,15 @@ __global__ __launch_bounds__(256) void k_flux(Geo g, MParm m, Tile b,
     for (int v = 0; v < 5; v++) {
         double qv[8], nq[4];
 #pragma unroll
-        for (int n = 0; n < 8; n++) qv[n] = Wv[v * N3 + c + (n - 4) * s];
+        for (int n = 0; n < 8; n++) qv[n] = LD(Wv + v * N3, n);
 #pragma unroll
-        for (int n = 0; n < 4; n++) nq[n] = Wl[v * N3 + c + (n - 2) * s];
+        for (int n = 0; n < 4; n++) nq[n] = LD(Wl + v * N3, n + 2);
         recon_lr_w<RM>(m.pdmb, rVi, nq, qv, WL[v], WR[v]);
     }
     double T[9];
 #pragma unroll
-    for (int q = 0; q < 9; q++) T[q] = met[(M_TF + q + 9 * D) * N3 + c];
-    const double faB = met[(M_FACE + D) * N3 + c];
+    for (int q = 0; q < 9; q++) T[q] = LD(met + (M_TF + q + 9 * D) * N3, 4);
+    const double faB = LD(met + (M_FACE + D) * N3, 4);
 
     double VaD = 0.0;
     if (m.doMHD) {
@@ -472,18 +484,18 @@ __global__ __launch_bounds__(256) void k_flux(Geo g, MParm m, Tile b,
         for (int x = 0; x < 3; x++) {
             double q[8], nq[4];
 #pragma unroll
-            for (int n = 0; n < 8; n++) q[n] = Bh[x * N3 + c + (n - 4) * s];
+            for (int n = 0; n < 8; n++) q[n] = LD(Bh + x * N3, n);
 #pragma unroll
-            for (int n = 0; n < 4; n++) nq[n] = Bl[x * N3 + c + (n - 2) * s];
+            for (int n = 0; n < 4; n++) nq[n] = LD(Bl + x * N3, n + 2);
             recon_lr<RM>(m.pdmb, vol, rVi, nq, q, BL[x], BR[x]);
         }
         const double iFaB = 1.0 / faB;
-        const double Bn = H[(H_MF + D) * N3 + c] * iFaB;
+        const double Bn = LD(H + (H_MF + D) * N3, 4) * iFaB;
         double B0[3] = {0.0, 0.0, 0.0}, B0n = 0.0;
         if (BG) {
 #pragma unroll
-            for (int x = 0; x < 3; x++) B0[x] = bg[(B_FCB0 + x + 3 * D) * N3 + c];
-            B0n = bg[(B_BFLUX0 + D) * N3 + c] * iFaB;
+            for (int x = 0; x < 3; x++) B0[x] = LD(bg + (B_FCB0 + x + 3 * D) * N3, 4);
+            B0n = LD(bg + (B_BFLUX0 + D) * N3, 4) * iFaB;
         }
         double mF[3] = {0.0, 0.0, 0.0}, bbD = 0.0;
         mag_kin<BG, BORIS>(m, -1.0, WL, BL, Bn, T, B0, B0n, mF, bbD, VaD);
@@ -496,7 +508,7 @@ __global__ __launch_bounds__(256) void k_flux(Geo g, MParm m, Tile b,
             }
         }
 #pragma unroll
-        for (int x = 0; x < 3; x++) mflx[(x + 3 * D) * N3 + c] = faB * mF[x];
+        for (int x = 0; x < 3; x++) stp(mflx + (x + 3 * D) * N3, o[4], faB * mF[x]);
     }
 
     double gF[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, dW[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -511,13 +523,13 @@ __global__ __launch_bounds__(256) void k_flux(Geo g, MParm m, Tile b,
         if (vn >= 1.5 * m.Ca) {
 #pragma unroll
             for (int v = 0; v < 5; v++) {
-                double dU = H[(H_GAS + v) * N3 + c] - H[(H_GAS + v) * N3 + c - s];
+                double dU = LD(H + (H_GAS + v) * N3, 4) - LD(H + (H_GAS + v) * N3, 3);
                 gF[v] = gF[v] - m.Ca * dU;
             }
         }
     }
 #pragma unroll
-    for (int v = 0; v < 5; v++) gflx[(v + 5 * D) * N3 + c] = faB * gF[v];
+    for (int v = 0; v < 5; v++) stp(gflx + (v + 5 * D) * N3, o[4], faB * gF[v]);
 }
 
 // ====================================================================
@@ -529,27 +541,29 @@ __global__ __launch_bounds__(256) void k_flux(Geo g, MParm m, Tile b,
 struct EdgeBoxes { Box e[3]; };
 
 __device__ __forceinline__ double e_at(const Geo &g, const EdgeBoxes &eb, const double *E, int d,
-                                       int i, int j, int k)
+                                       int i, int j, int k, unsigned off)
 {
-    return in_box(eb.e[d], i, j, k) ? E[d * g.N3 + IX(g, i, j, k)] : 0.0;
+    return in_box(eb.e[d], i, j, k) ? ldp(E + d * g.N3, off) : 0.0;
 }
 
+// New flux of face d at (i,j,k), whose byte offset is oc.
 __device__ __forceinline__ double ct_face(const Geo &g, const EdgeBoxes &eb, const Box &ctb,
                                           const double *E, const double *mfOld, double dt,
-                                          int d, int i, int j, int k)
+                                          int d, int i, int j, int k, unsigned oc)
 {
-    const double old = mfOld[d * g.N3 + IX(g, i, j, k)];
+    const double old = ldp(mfOld + d * g.N3, oc);
     if (!in_box(ctb, i, j, k)) return old;
+    const unsigned oI = oc + 8, oJ = oc + 8 * g.P, oK = oc + (unsigned)(8 * g.PP);
     double df;
     if (d == 0)
-        df = e_at(g, eb, E, 2, i, j + 1, k) - e_at(g, eb, E, 2, i, j, k)
-           - e_at(g, eb, E, 1, i, j, k + 1) + e_at(g, eb, E, 1, i, j, k);
+        df = e_at(g, eb, E, 2, i, j + 1, k, oJ) - e_at(g, eb, E, 2, i, j, k, oc)
+           - e_at(g, eb, E, 1, i, j, k + 1, oK) + e_at(g, eb, E, 1, i, j, k, oc);
     else if (d == 1)
-        df = e_at(g, eb, E, 0, i, j, k + 1) - e_at(g, eb, E, 0, i, j, k)
-           - e_at(g, eb, E, 2, i + 1, j, k) + e_at(g, eb, E, 2, i, j, k);
+        df = e_at(g, eb, E, 0, i, j, k + 1, oK) - e_at(g, eb, E, 0, i, j, k, oc)
+           - e_at(g, eb, E, 2, i + 1, j, k, oI) + e_at(g, eb, E, 2, i, j, k, oc);
     else
-        df = e_at(g, eb, E, 1, i + 1, j, k) - e_at(g, eb, E, 1, i, j, k)
-           - e_at(g, eb, E, 0, i, j + 1, k) + e_at(g, eb, E, 0, i, j, k);
+        df = e_at(g, eb, E, 1, i + 1, j, k, oI) - e_at(g, eb, E, 1, i, j, k, oc)
+           - e_at(g, eb, E, 0, i, j + 1, k, oJ) + e_at(g, eb, E, 0, i, j, k, oc);
     return old - dt * df;
 }
 
@@ -599,6 +613,7 @@ __global__ __launch_bounds__(256) void k_update(Geo g, MParm m, Tile ft, Box ctb
     double dtc = GAM_HUGE;
     const long N3 = g.N3;
     const long c = IX(g, i, j, k);
+    const unsigned oc = (unsigned)(c * 8);    // byte offset of the point
     const bool cell = live && i <= g.ieg && j <= g.jeg && k <= g.keg;
     double Bn[3] = {0.0, 0.0, 0.0};
     if (live) {
@@ -608,36 +623,37 @@ __global__ __launch_bounds__(256) void k_update(Geo g, MParm m, Tile ft, Box ctb
         double f[3];
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            f[d] = m.doMHD ? ct_face(g, eb, ctb, E, mfo, dt, d, i, j, k) : mfo[d * N3 + c];
-            Sn[(S_MF + d) * N3 + c] = f[d];
+            f[d] = m.doMHD ? ct_face(g, eb, ctb, E, mfo, dt, d, i, j, k, oc) : ldp(mfo + d * N3, oc);
+            stp(Sn + (S_MF + d) * N3, oc, f[d]);
         }
         if (cell) {
             double fIp, fJp, fKp;
             if (m.doMHD) {
-                fIp = ct_face(g, eb, ctb, E, mfo, dt, 0, i + 1, j, k);
-                fJp = ct_face(g, eb, ctb, E, mfo, dt, 1, i, j + 1, k);
-                fKp = ct_face(g, eb, ctb, E, mfo, dt, 2, i, j, k + 1);
+                fIp = ct_face(g, eb, ctb, E, mfo, dt, 0, i + 1, j, k, oc + 8);
+                fJp = ct_face(g, eb, ctb, E, mfo, dt, 1, i, j + 1, k, oc + 8 * g.P);
+                fKp = ct_face(g, eb, ctb, E, mfo, dt, 2, i, j, k + 1, oc + (unsigned)(8 * g.PP));
             } else {
                 fIp = mfo[0 * N3 + c + 1]; fJp = mfo[1 * N3 + c + g.P]; fKp = mfo[2 * N3 + c + g.PP];
             }
             cell_b(g, met, c, f[0], f[1], f[2], fIp, fJp, fKp, Bn);
 #pragma unroll
-            for (int x = 0; x < 3; x++) Sn[(S_B + x) * N3 + c] = Bn[x];
+            for (int x = 0; x < 3; x++) stp(Sn + (S_B + x) * N3, oc, Bn[x]);
         }
     }
     if (cell) {
         double U[5];
 #pragma unroll
-        for (int v = 0; v < 5; v++) U[v] = So[(S_GAS + v) * N3 + c];
+        for (int v = 0; v < 5; v++) U[v] = ldp(So + (S_GAS + v) * N3, oc);
         if (in_box(ab, i, j, k)) {
-            const double idV = 1.0 / met[M_VOL * N3 + c];
+            const double idV = 1.0 / ldp(met + M_VOL * N3, oc);
+            const unsigned oI = oc + 8, oJ = oc + 8 * g.P, oK = oc + (unsigned)(8 * g.PP);
             const double dtM = clk->dt, dt = clk->dtA;
             double dH[5], oU[5];
 #pragma unroll
             for (int v = 0; v < 5; v++) {
                 const double *gv = gflx + (long)v * N3;
-                dH[v] = (gv[c] - gv[c + 1] + gv[5 * N3 + c] - gv[5 * N3 + c + g.P]
-                         + gv[10 * N3 + c] - gv[10 * N3 + c + g.PP]) * idV;
+                dH[v] = (ldp(gv, oc) - ldp(gv, oI) + ldp(gv + 5 * N3, oc) - ldp(gv + 5 * N3, oJ)
+                         + ldp(gv + 10 * N3, oc) - ldp(gv + 10 * N3, oK)) * idV;
                 oU[v] = U[v];
                 U[v] = U[v] + dtM * dH[v];
             }
@@ -646,13 +662,17 @@ __global__ __launch_bounds__(256) void k_update(Geo g, MParm m, Tile ft, Box ctb
 #pragma unroll
                 for (int x = 0; x < 3; x++) {
                     const double *mv = mflx + (long)x * N3;
-                    double r = (mv[c] - mv[c + 1] + mv[3 * N3 + c] - mv[3 * N3 + c + g.P]
-                                + mv[6 * N3 + c] - mv[6 * N3 + c + g.PP]) * idV;
-                    if (BG) r = r + bg[(B_DPB0 + x) * N3 + c];
+                    double r = (ldp(mv, oc) - ldp(mv, oI) + ldp(mv + 3 * N3, oc) - ldp(mv + 3 * N3, oJ)
+                                + ldp(mv + 6 * N3, oc) - ldp(mv + 6 * N3, oK)) * idV;
+                    if (BG) r = r + ldp(bg + (B_DPB0 + x) * N3, oc);
                     dM[x] = r;
                     B[x] = Bn[x];
-                    oB[x] = So[(S_B + x) * N3 + c];
-                    if (BG) { B[x] = B[x] + bg[(B_B0 + x) * N3 + c]; oB[x] = oB[x] + bg[(B_B0 + x) * N3 + c]; }
+                    oB[x] = ldp(So + (S_B + x) * N3, oc);
+                    if (BG) {
+                        const double b0 = ldp(bg + (B_B0 + x) * N3, oc);
+                        B[x] = B[x] + b0;
+                        oB[x] = oB[x] + b0;
+                    }
                 }
                 if (BORIS) {
                     // CellBoris (mhdgroup.F90:336-441), single fluid
@@ -709,11 +729,15 @@ __global__ __launch_bounds__(256) void k_update(Geo g, MParm m, Tile ft, Box ctb
             }
         }
 #pragma unroll
-        for (int v = 0; v < 5; v++) Sn[(S_GAS + v) * N3 + c] = U[v];
+        for (int v = 0; v < 5; v++) stp(Sn + (S_GAS + v) * N3, oc, U[v]);
         if (in_box(dtb, i, j, k)) {
             double bx = Bn[0], by = Bn[1], bz = Bn[2];
-            if (BG) { bx = bx + bg[(B_B0 + 0) * N3 + c]; by = by + bg[(B_B0 + 1) * N3 + c]; bz = bz + bg[(B_B0 + 2) * N3 + c]; }
-            const double dl = dmin(dmin(met[M_DI * N3 + c], met[M_DJ * N3 + c]), met[M_DK * N3 + c]);
+            if (BG) {
+                bx = bx + ldp(bg + (B_B0 + 0) * N3, oc);
+                by = by + ldp(bg + (B_B0 + 1) * N3, oc);
+                bz = bz + ldp(bg + (B_B0 + 2) * N3, oc);
+            }
+            const double dl = dmin(dmin(ldp(met + M_DI * N3, oc), ldp(met + M_DJ * N3, oc)), ldp(met + M_DK * N3, oc));
             dtc = cell_dt<BORIS>(m, U, bx, by, bz, dl);
             if (!(dtc == dtc)) dtc = GAM_HUGE;
         }
