@@ -425,7 +425,83 @@ __global__ __launch_bounds__(256) void k_emf(Geo g, MParm m, Tile b, const Clock
 }
 
 // ====================================================================
-// Interface fluxes of one direction (Fluxes, stress.F90:204-429)
+// The fluxes of one face from its reconstructed L/R states (the part of
+// Fluxes, stress.F90:204-429, after BlockStateLRs): magnetic kinetic flux
+// with the B0 terms, Boris mag-HOGS, gas kinetic flux, HOGS and the nuke
+// term, times the face area, stored to gFlx/mFlx of direction D.  AT(p)
+// reads plane p at the face point, AT_M(p) one cell below it along D;
+// RECON_B(BL, BR) fills the magnetic L/R states (called only with MHD on,
+// after the metric loads, which keeps the register peak of the fused
+// kernel at that of the gas part); oc is the face's byte offset for the
+// stores.
+// ====================================================================
+template <int D, bool BG, bool BORIS, class AtF, class AtMF, class ReconB>
+__device__ __forceinline__ void face_flux_lr(const Geo &g, const MParm &m,
+                                             const double *__restrict__ H,
+                                             const double *__restrict__ met,
+                                             const double *__restrict__ bg, AtF AT, AtMF AT_M,
+                                             ReconB RECON_B, const double *WL,
+                                             const double *WR, double *BL, double *BR,
+                                             unsigned oc,
+                                             double *__restrict__ gflx,
+                                             double *__restrict__ mflx)
+{
+    const long N3 = g.N3;
+    double T[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) T[q] = AT(met + (M_TF + q + 9 * D) * N3);
+    const double faB = AT(met + (M_FACE + D) * N3);
+
+    double VaD = 0.0;
+    if (m.doMHD) {
+        RECON_B(BL, BR);
+        const double iFaB = 1.0 / faB;
+        const double Bn = AT(H + (H_MF + D) * N3) * iFaB;
+        double B0[3] = {0.0, 0.0, 0.0}, B0n = 0.0;
+        if (BG) {
+#pragma unroll
+            for (int x = 0; x < 3; x++) B0[x] = AT(bg + (B_FCB0 + x + 3 * D) * N3);
+            B0n = AT(bg + (B_BFLUX0 + D) * N3) * iFaB;
+        }
+        double mF[3] = {0.0, 0.0, 0.0}, bbD = 0.0;
+        mag_kin<BG, BORIS>(m, -1.0, WL, BL, Bn, T, B0, B0n, mF, bbD, VaD);
+        mag_kin<BG, BORIS>(m, +1.0, WR, BR, Bn, T, B0, B0n, mF, bbD, VaD);
+        if (BORIS) {
+#pragma unroll
+            for (int x = 0; x < 3; x++) {
+                double deeP = bbD * (WR[1 + x] - WL[1 + x]) * m.iCa2;
+                mF[x] = mF[x] - m.cHogM * VaD * deeP;
+            }
+        }
+#pragma unroll
+        for (int x = 0; x < 3; x++) stp(mflx + (x + 3 * D) * N3, oc, faB * mF[x]);
+    }
+
+    double gF[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, dW[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    gas_kin(m.igm1, -1.0, WL, T, gF, dW);
+    gas_kin(m.igm1, +1.0, WR, T, gF, dW);
+#pragma unroll
+    for (int v = 0; v < 5; v++) gF[v] = gF[v] - m.cHogH * VaD * dW[v];
+    if (BORIS) {   // "nuclear option" (stress.F90:399-417)
+        double vL = sqrt(WL[1] * WL[1] + WL[2] * WL[2] + WL[3] * WL[3]);
+        double vR = sqrt(WR[1] * WR[1] + WR[2] * WR[2] + WR[3] * WR[3]);
+        double vn = sqrt(vL * vL + vR * vR) * 0.70710678118654752440084436210484903928483593768847;
+        if (vn >= 1.5 * m.Ca) {
+#pragma unroll
+            for (int v = 0; v < 5; v++) {
+                double dU = AT(H + (H_GAS + v) * N3) - AT_M(H + (H_GAS + v) * N3);
+                gF[v] = gF[v] - m.Ca * dU;
+            }
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < 5; v++) stp(gflx + (v + 5 * D) * N3, oc, faB * gF[v]);
+}
+
+// ====================================================================
+// Interface fluxes of one direction (Fluxes, stress.F90:204-429), every
+// stencil point read from global memory (used along i, where the 8 points
+// are contiguous and merge into wide loads)
 // ====================================================================
 template <int D, int RM, bool BG, bool BORIS>
 __global__ __launch_bounds__(256) void k_flux(Geo g, MParm m, Tile b,
@@ -470,16 +546,9 @@ __global__ __launch_bounds__(256) void k_flux(Geo g, MParm m, Tile b,
         for (int n = 0; n < 4; n++) nq[n] = LD(Wl + v * N3, n + 2);
         recon_lr_w<RM>(m.pdmb, rVi, nq, qv, WL[v], WR[v]);
     }
-    double T[9];
-#pragma unroll
-    for (int q = 0; q < 9; q++) T[q] = LD(met + (M_TF + q + 9 * D) * N3, 4);
-    const double faB = LD(met + (M_FACE + D) * N3, 4);
-
-    double VaD = 0.0;
-    if (m.doMHD) {
+    auto recon_b = [&](double *BL, double *BR) {
         const double *Bh = H + H_B * N3;
         const double *Bl = m.doLFM ? S + S_B * N3 : Bh;
-        double BL[3], BR[3];
 #pragma unroll
         for (int x = 0; x < 3; x++) {
             double q[8], nq[4];
@@ -489,47 +558,147 @@ __global__ __launch_bounds__(256) void k_flux(Geo g, MParm m, Tile b,
             for (int n = 0; n < 4; n++) nq[n] = LD(Bl + x * N3, n + 2);
             recon_lr<RM>(m.pdmb, vol, rVi, nq, q, BL[x], BR[x]);
         }
-        const double iFaB = 1.0 / faB;
-        const double Bn = LD(H + (H_MF + D) * N3, 4) * iFaB;
-        double B0[3] = {0.0, 0.0, 0.0}, B0n = 0.0;
-        if (BG) {
-#pragma unroll
-            for (int x = 0; x < 3; x++) B0[x] = LD(bg + (B_FCB0 + x + 3 * D) * N3, 4);
-            B0n = LD(bg + (B_BFLUX0 + D) * N3, 4) * iFaB;
-        }
-        double mF[3] = {0.0, 0.0, 0.0}, bbD = 0.0;
-        mag_kin<BG, BORIS>(m, -1.0, WL, BL, Bn, T, B0, B0n, mF, bbD, VaD);
-        mag_kin<BG, BORIS>(m, +1.0, WR, BR, Bn, T, B0, B0n, mF, bbD, VaD);
-        if (BORIS) {
-#pragma unroll
-            for (int x = 0; x < 3; x++) {
-                double deeP = bbD * (WR[1 + x] - WL[1 + x]) * m.iCa2;
-                mF[x] = mF[x] - m.cHogM * VaD * deeP;
-            }
-        }
-#pragma unroll
-        for (int x = 0; x < 3; x++) stp(mflx + (x + 3 * D) * N3, o[4], faB * mF[x]);
-    }
+    };
+    double BL[3], BR[3];
+    face_flux_lr<D, BG, BORIS>(
+        g, m, H, met, bg, [&](const double *p) { return LD(p, 4); },
+        [&](const double *p) { return LD(p, 3); }, recon_b, WL, WR, BL, BR, o[4], gflx, mflx);
+}
 
-    double gF[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, dW[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    gas_kin(m.igm1, -1.0, WL, T, gF, dW);
-    gas_kin(m.igm1, +1.0, WR, T, gF, dW);
+// ====================================================================
+// Interface fluxes along j or k (D = 1, 2) with the stencil planes staged
+// in LDS.  A workgroup owns 64 (i) x 4 (along D) faces; the rows of the
+// 8-point stencil (faces' cells a-4 .. a+3, 11 rows for the 4 face rows)
+// and of the 4-point limiter stencil (a-2 .. a+1, 7 rows) of one variable
+// at a time go to LDS with coalesced loads, double-buffered so the next
+// variable's rows are in flight while the current one is reconstructed.
+// Each row is loaded once per workgroup instead of once per stencil point.
+// ====================================================================
+#define SR8 (BY + 7)          // rows of an 8-point stage
+#define SR4 (BY + 3)          // rows of a 4-point stage
+#define SRW (SR8 + SR4)       // rows of one variable's stage
+#define SLD ((SRW * BX + BX * BY - 1) / (BX * BY))   // loads per thread per stage
+
+struct FluxTile {
+    int i0, i1, a0, a1, b0, b1;   // faces along i, along D, along the third axis
+    int nti, nta, nb;
+};
+
+template <int D, int RM, bool BG, bool BORIS>
+__global__ __launch_bounds__(256) void k_flux_s(Geo g, MParm m, FluxTile t,
+                                                const double *__restrict__ H,
+                                                const double *__restrict__ S,
+                                                const double *__restrict__ met,
+                                                const double *__restrict__ bg,
+                                                double *__restrict__ gflx,
+                                                double *__restrict__ mflx)
+{
+    __shared__ double vs[SR8][BX];
+    __shared__ double buf[2][SRW][BX];
+    const int w = blockIdx.x;
+    const int per = (t.nb + 7) >> 3;
+    const int L = (w & 7) * per + (w >> 3);
+    if (L >= t.nb) return;                   // whole workgroup
+    const int ti = L % t.nti;
+    const int r = L / t.nti;
+    const int ta = r % t.nta;
+    const int bb = t.b0 + r / t.nta;
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + BX * ty;
+    const int i0 = t.i0 + ti * BX, a0 = t.a0 + ta * BY;
+    const int i = i0 + tx, a = a0 + ty;
+    const bool act = i <= t.i1 && a <= t.a1;
+    const long N3 = g.N3;
+    const long s = STRIDE<D>(g);
+    // point (i', a', bb) of the box's index space
+    auto at = [&](int ii, int aa) -> long {
+        return D == 1 ? IX(g, ii, aa, bb) : IX(g, ii, bb, aa);
+    };
+    // rows that exist: the stencils of the active faces reach a1 + 3
+    const int aTop = t.a1 + 3;
+    // stage loaders: element e of a stage -> (row, column)
+    auto fetch = [&](const double *p8, const double *p4, double *reg) {
 #pragma unroll
-    for (int v = 0; v < 5; v++) gF[v] = gF[v] - m.cHogH * VaD * dW[v];
-    if (BORIS) {   // "nuclear option" (stress.F90:399-417)
-        double vL = sqrt(WL[1] * WL[1] + WL[2] * WL[2] + WL[3] * WL[3]);
-        double vR = sqrt(WR[1] * WR[1] + WR[2] * WR[2] + WR[3] * WR[3]);
-        double vn = sqrt(vL * vL + vR * vR) * 0.70710678118654752440084436210484903928483593768847;
-        if (vn >= 1.5 * m.Ca) {
-#pragma unroll
-            for (int v = 0; v < 5; v++) {
-                double dU = LD(H + (H_GAS + v) * N3, 4) - LD(H + (H_GAS + v) * N3, 3);
-                gF[v] = gF[v] - m.Ca * dU;
+        for (int q = 0; q < SLD; q++) {
+            const int e = tid + q * BX * BY;
+            const int row = e / BX, col = e % BX;
+            const int ii = i0 + col;
+            double v = 0.0;
+            if (row < SRW && ii <= t.i1) {
+                const int aa = row < SR8 ? a0 - 4 + row : a0 - 2 + (row - SR8);
+                const double *p = row < SR8 ? p8 : p4;
+                if (aa <= aTop) v = p[at(ii, aa)];
             }
+            reg[q] = v;
+        }
+    };
+    auto store = [&](int bsel, const double *reg) {
+#pragma unroll
+        for (int q = 0; q < SLD; q++) {
+            const int e = tid + q * BX * BY;
+            if (e < SRW * BX) buf[bsel][e / BX][e % BX] = reg[q];
+        }
+    };
+    // volumes (8-point rows) and the first stage
+    {
+        const double *pv = met + M_VOL * N3;
+        for (int e = tid; e < SR8 * BX; e += BX * BY) {
+            const int row = e / BX, col = e % BX;
+            const int ii = i0 + col, aa = a0 - 4 + row;
+            vs[row][col] = (ii <= t.i1 && aa <= aTop) ? pv[at(ii, aa)] : 0.0;
         }
     }
+    const int nst = m.doMHD ? 8 : 5;
+    const double *Wv = H + H_WH * N3, *Wl = H + H_WN * N3;
+    const double *Bh = H + H_B * N3, *Bl = m.doLFM ? S + S_B * N3 : Bh;
+    auto planes = [&](int st, const double *&p8, const double *&p4) {
+        if (st < 5) { p8 = Wv + st * N3; p4 = Wl + st * N3; }
+        else { p8 = Bh + (st - 5) * N3; p4 = Bl + (st - 5) * N3; }
+    };
+    double reg[SLD];
+    {
+        const double *p8, *p4;
+        planes(0, p8, p4);
+        fetch(p8, p4, reg);
+        store(0, reg);
+    }
+    __syncthreads();
+    const double rVi = 1.0 / wsum8([&](int n) { return vs[ty + n][tx]; });
+    double WL[5], WR[5], BL[3] = {0.0, 0.0, 0.0}, BR[3] = {0.0, 0.0, 0.0};
+    // kept a loop: unrolled, the compiler hoists later stages' loads and
+    // the register count doubles
+#pragma unroll 1
+    for (int st = 0; st < 8; st++) {
+        if (st >= nst) break;
+        const int cur = st & 1;
+        if (st + 1 < nst) {
+            const double *p8, *p4;
+            planes(st + 1, p8, p4);
+            fetch(p8, p4, reg);
+        }
+        double q[8], nq[4];
 #pragma unroll
-    for (int v = 0; v < 5; v++) stp(gflx + (v + 5 * D) * N3, o[4], faB * gF[v]);
+        for (int n = 0; n < 8; n++) q[n] = buf[cur][ty + n][tx];
+#pragma unroll
+        for (int n = 0; n < 4; n++) nq[n] = buf[cur][SR8 + ty + n][tx];
+        if (st < 5) {
+            recon_lr_w<RM>(m.pdmb, rVi, nq, q, WL[st], WR[st]);
+        } else {
+            // cell B is not volume-weighted in H: weight it here (LDS volumes)
+#pragma unroll
+            for (int n = 0; n < 8; n++) q[n] = vs[ty + n][tx] * q[n];
+            recon_lr_w<RM>(m.pdmb, rVi, nq, q, BL[st - 5], BR[st - 5]);
+        }
+        if (st + 1 < nst) store(cur ^ 1, reg);
+        __syncthreads();
+    }
+    if (act) {
+        const long c = at(i, a);
+        const unsigned oc = (unsigned)(c * 8);
+        face_flux_lr<D, BG, BORIS>(
+            g, m, H, met, bg, [&](const double *p) { return ldp(p, oc); },
+            [&](const double *p) { return p[c - s]; },
+            [](double *, double *) {}, WL, WR, BL, BR, oc, gflx, mflx);
+    }
 }
 
 // ====================================================================
@@ -1287,8 +1456,21 @@ template <int D, int RM, bool BG, bool BO>
 static void flux_t(hipStream_t s, const Geo &g, const MParm &m, const Box &b, const double *H,
                    const double *S, const double *met, const double *bg, double *gf, double *mf)
 {
-    const Tile t = tile_of(b, D == 2 ? 2 : 1);
-    LAUNCH((k_flux<D, RM, BG, BO>), tgrid(t), dim3(BX, BY), s, g, m, t, H, S, met, bg, gf, mf);
+    if constexpr (D == 0) {
+        const Tile t = tile_of(b, 1);
+        LAUNCH((k_flux<D, RM, BG, BO>), tgrid(t), dim3(BX, BY), s, g, m, t, H, S, met, bg, gf, mf);
+        return;
+    } else {
+    FluxTile t;
+    t.i0 = b.i0; t.i1 = b.i1;
+    if (D == 1) { t.a0 = b.j0; t.a1 = b.j1; t.b0 = b.k0; t.b1 = b.k1; }
+    else { t.a0 = b.k0; t.a1 = b.k1; t.b0 = b.j0; t.b1 = b.j1; }
+    t.nti = (t.i1 - t.i0 + BX) / BX;
+    t.nta = (t.a1 - t.a0 + BY) / BY;
+    t.nb = t.nti * t.nta * (t.b1 - t.b0 + 1);
+    LAUNCH((k_flux_s<D, RM, BG, BO>), dim3((unsigned)(((t.nb + 7) / 8) * 8)), dim3(BX, BY), s, g, m, t,
+           H, S, met, bg, gf, mf);
+    }
 }
 
 template <int D>
