@@ -77,6 +77,47 @@ def flux_traffic(path=PMC_SUMMARY):
     return sum(vals) / len(vals) if vals else None
 
 
+def setup_member(gamhip, pb, spec, rmeth="7UP"):
+    """One ensemble member: its own shell size, warp/IC seed and inflow."""
+    Nip, Njp, Nkp = spec["N"]
+    g = gamhip.GamHip(Nip, Njp, Nkp, pb.shell_params(rmeth))
+    xyz = pb.shell_corners(Nip, Njp, Nkp, seed=spec["seed"])
+    g.grid_from_corners(xyz)
+    g.add_dipole_b0(pb.SHELL_M0)
+    g.set_wind(pb.constant_wind(vx=spec["vx"], bz=spec["bz"]))
+    xyzcc = g.get_grid(fields=("xyzcc",))["xyzcc"]
+    g.put_state(pb.shell_state(Nip, Njp, Nkp, xyzcc, seed=spec["seed"]))
+    g.init_run()
+    return g
+
+
+def ensemble_gain(gamhip, pb, n=32, steps=100, warmup=2):
+    """Batched ensemble vs. the same members stepped one after another with
+    plain launches: aggregate cell-updates/s of each, HIP-event timed."""
+    specs = pb.ensemble_members(n=n)
+    members = [setup_member(gamhip, pb, s) for s in specs]
+    cells = sum(s["N"][0] * s["N"][1] * s["N"][2] for s in specs)
+    for g in members:
+        g.run(warmup)
+    seq_ms = sum(g.time_steps(steps) for g in members)
+    ens = gamhip.GamEnsemble(members)
+    ens.run(warmup)
+    ens_ms = ens.time_steps(steps)
+    launches = ens.launches_per_step()
+    ens.close()
+    for g in members:
+        g.close()
+    seq = cells * steps / (seq_ms * 1e-3)
+    bat = cells * steps / (ens_ms * 1e-3)
+    return {"workload": "%d shell members of uneven size (multiples of 8 in [48,96]x[48,96]x[64,128], "
+                        "seeded), own warp, IC and inflow each, %d steps each" % (n, steps),
+            "members": n, "cells": cells, "steps": steps,
+            "sizes": [list(s["N"]) for s in specs],
+            "sequential_ms": seq_ms, "batched_ms": ens_ms,
+            "sequential_cell_updates_per_s": seq, "batched_cell_updates_per_s": bat,
+            "gain": seq_ms / ens_ms, "launches_per_step": launches}
+
+
 def cpu_baseline(gamhip, pb, min_seconds=12.0, max_steps=400):
     """Time the oracle (C/OpenMP restatement of the reference step) on a
     96x96x128 sample of the same shell workload."""
@@ -113,6 +154,9 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-box", action="store_true")
+    ap.add_argument("--no-ens", action="store_true")
+    ap.add_argument("--ens-members", type=int, default=32)
+    ap.add_argument("--ens-steps", type=int, default=100)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
@@ -171,6 +215,13 @@ def main():
                "step_frac_of_hbm_roofline": 8 * (19 + 14 + 79) * 64 ** 3 * 100 / (msb * 1e-3) / 1e9 / HBM_PEAK_GBS}
         log("box 64^3: %.3f ms/step, %.4e cell-updates/s" % (box["ms_per_step"], box["value"]))
 
+    ens = None
+    if not args.no_ens:
+        t1 = time.time()
+        ens = ensemble_gain(gamhip, pb, n=args.ens_members, steps=args.ens_steps)
+        log("ensemble %d members: sequential %.1f ms, batched %.1f ms, gain %.2f (%.1f s)"
+            % (ens["members"], ens["sequential_ms"], ens["batched_ms"], ens["gain"], time.time() - t1))
+
     cpu = None
     if not args.no_cpu:
         t1 = time.time()
@@ -220,6 +271,7 @@ def main():
         },
         "mhd_cell_updates_per_s": mhd_value,
         "box_64cubed": box,
+        "ensemble": ens,
         "kernels": kernels,
         "final": {"t": t, "dt": dt, "ts": ts},
         "cpu_baseline": cpu,

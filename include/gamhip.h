@@ -188,6 +188,37 @@ const char *gam_kernel_group_name(int group);
  * 1 mFlx (Ni,Nj,Nk,3,3).  Values are those of the last AdvanceMHD. */
 int gam_get_solver(gam_ctx *ctx, int which, double *out);
 
+/* ---------------------------------------------------------------------
+ * Ensembles: many independent runs stepped together.  The reference runs
+ * one gamApp_T per process and has no batched form; an ensemble here is a
+ * set of contexts (each one stepGamera instance, gamapp.F90:73-113, with
+ * its own grid size, grid, background, inflow series, state and clock)
+ * whose steps are issued as one launch per kernel covering every member,
+ * and whose whole step is one captured hipGraph.  Every member's results
+ * are those of stepping it on its own.
+ * ------------------------------------------------------------------- */
+typedef struct gam_ens gam_ens;
+
+#define GAM_ENS_MAX 256
+/* Batch n (1..GAM_ENS_MAX) contexts, each ready to step (grid, background,
+ * wind as its BCs need, state, gam_init_run).  Members must share the
+ * kernel variant: rmeth, doMHD, doBoris, doBackground, doLFMLim and the six
+ * BC kinds (GAM_E_ARG otherwise).  The context handles stay valid and
+ * owned by the caller; state transfers (gam_get_state, ...) on a member
+ * show the ensemble's progress.  A member may also be stepped on its own
+ * between ensemble runs only an even number of times (the recorded graphs
+ * fix which of its two state buffers is current; GAM_E_STATE otherwise). */
+int gam_ens_create(gam_ens **out, int n, gam_ctx *const *members);
+/* nsteps x stepGamera of every member; synchronises at the end and returns
+ * the first member's CalcDT failure code, if any (gam_last_error names the
+ * member). */
+int gam_ens_run(gam_ens *e, int nsteps);
+/* Same, bracketed by hipEvents on the ensemble's stream; ms = elapsed time. */
+int gam_ens_time_steps(gam_ens *e, int nsteps, double *ms);
+/* Kernel launches per step of every member (= graph nodes per step). */
+int gam_ens_launches_per_step(gam_ens *e, int *n);
+int gam_ens_destroy(gam_ens *e);
+
 #ifdef __cplusplus
 }
 #endif

@@ -784,3 +784,247 @@ extern "C" const char *gam_kernel_group_name(int grp)
     if (grp < 0 || grp >= GAM_KG_COUNT) return "";
     return kg_names[grp];
 }
+
+// ------------------------------------------------------------ ensembles
+// Every member's step is recorded (not issued) as its list of step-kernel
+// launches, twice: once from each of its State/oState roles.  Launch p of
+// all members becomes one batched launch (k_ens) over the members'
+// concatenated grids with a device table of their argument records; the
+// batched launches of one step are captured as a hipGraph per role.
+struct EnsPhase {
+    void (*launch)(hipStream_t, const void *, const void *, unsigned, dim3);
+    dim3 block;
+    unsigned nwg;
+    size_t hdr_off, rec_off;   // byte offsets into gam_ens::dev
+};
+
+struct gam_ens {
+    int n;
+    gam_ctx **m;
+    int *cur0;                       // each member's State buffer at recording
+    int parity;                      // steps taken mod 2 since recording
+    hipStream_t stream;
+    unsigned char *dev;              // headers and argument records
+    std::vector<EnsPhase> ph[2];
+    hipGraph_t graph[2];
+    hipGraphExec_t exec[2];
+};
+
+extern "C" int gam_ens_destroy(gam_ens *e)
+{
+    if (!e) return GAM_OK;
+    if (e->stream) hipStreamSynchronize(e->stream);
+    for (int q = 0; q < 2; q++) {
+        if (e->exec[q]) hipGraphExecDestroy(e->exec[q]);
+        if (e->graph[q]) hipGraphDestroy(e->graph[q]);
+    }
+    if (e->dev) hipFree(e->dev);
+    if (e->stream) hipStreamDestroy(e->stream);
+    free(e->m);
+    free(e->cur0);
+    e->ph[0].~vector<EnsPhase>();
+    e->ph[1].~vector<EnsPhase>();
+    free(e);
+    return GAM_OK;
+}
+
+static bool same_variant(const gam_ctx *a, const gam_ctx *b)
+{
+    const gam_model &x = a->model, &y = b->model;
+    if (x.rmeth != y.rmeth || x.doMHD != y.doMHD || x.doBoris != y.doBoris ||
+        x.doBackground != y.doBackground || x.doLFMLim != y.doLFMLim)
+        return false;
+    for (int s = 0; s < 6; s++) if (x.bc[s] != y.bc[s]) return false;
+    return true;
+}
+
+extern "C" int gam_ens_create(gam_ens **out, int n, gam_ctx *const *members)
+{
+    if (!out || !members) return fail(GAM_E_ARG, "gam_ens_create: NULL argument");
+    *out = nullptr;
+    if (n < 1 || n > GAM_ENS_MAX) return fail(GAM_E_ARG, "gam_ens_create: n = %d outside 1..%d", n, GAM_ENS_MAX);
+    int rc;
+    for (int q = 0; q < n; q++) {
+        if ((rc = check_ready(members[q], "gam_ens_create"))) return rc;
+        if (!same_variant(members[0], members[q]))
+            return fail(GAM_E_ARG, "gam_ens_create: member %d differs from member 0 in rmeth, doMHD, doBoris, "
+                                   "doBackground, doLFMLim or BC kinds", q);
+        for (int r = 0; r < q; r++)
+            if (members[r] == members[q]) return fail(GAM_E_ARG, "gam_ens_create: member %d repeats member %d", q, r);
+    }
+    gam_ens *e = (gam_ens *)calloc(1, sizeof(gam_ens));
+    if (!e) return fail(GAM_E_NOMEM, "gam_ens_create: host allocation failed");
+    new (&e->ph[0]) std::vector<EnsPhase>();
+    new (&e->ph[1]) std::vector<EnsPhase>();
+    e->n = n;
+    e->m = (gam_ctx **)calloc(n, sizeof(gam_ctx *));
+    e->cur0 = (int *)calloc(n, sizeof(int));
+    if (!e->m || !e->cur0) { gam_ens_destroy(e); return fail(GAM_E_NOMEM, "gam_ens_create: host allocation failed"); }
+
+    // record two steps of every member (enq_step swaps its buffers twice)
+    std::vector<std::vector<EnsLaunch>> rec[2];
+    rec[0].resize(n);
+    rec[1].resize(n);
+    for (int q = 0; q < n; q++) {
+        gam_ctx *c = members[q];
+        e->m[q] = c;
+        e->cur0[q] = c->cur;
+        const int prof = c->prof;
+        c->prof = 0;
+        for (int r = 0; r < 2; r++) {
+            set_launch_recorder(&rec[r][q]);
+            enq_step(c);
+        }
+        set_launch_recorder(nullptr);
+        c->prof = prof;
+    }
+    // phases must line up across members
+    for (int r = 0; r < 2; r++)
+        for (int q = 1; q < n; q++) {
+            bool ok = rec[r][q].size() == rec[r][0].size();
+            for (size_t p = 0; ok && p < rec[r][0].size(); p++) {
+                const EnsLaunch &a = rec[r][0][p], &b = rec[r][q][p];
+                ok = a.launch == b.launch && a.block.x == b.block.x && a.block.y == b.block.y &&
+                     a.block.z == b.block.z && a.rec.size() == b.rec.size();
+            }
+            if (!ok) {
+                gam_ens_destroy(e);
+                return fail(GAM_E_ARG, "gam_ens_create: member %d issues a different kernel sequence", q);
+            }
+        }
+    // device layout: per role and phase an EnsHdr and n records, 256-aligned
+    std::vector<unsigned char> host;
+    auto put = [&](const void *p, size_t bytes) -> size_t {
+        const size_t off = (host.size() + 255) / 256 * 256;
+        host.resize(off + bytes);
+        memcpy(host.data() + off, p, bytes);
+        return off;
+    };
+    for (int r = 0; r < 2; r++) {
+        for (size_t p = 0; p < rec[r][0].size(); p++) {
+            EnsHdr h;
+            memset(&h, 0, sizeof(h));
+            h.n = (unsigned)n;
+            unsigned long tot = 0;
+            for (int q = 0; q < n; q++) {
+                const EnsLaunch &L = rec[r][q][p];
+                h.wg0[q] = (unsigned)tot;
+                h.gx[q] = L.grid.x;
+                h.gy[q] = L.grid.y;
+                h.gz[q] = L.grid.z;
+                tot += ((unsigned long)L.grid.x * L.grid.y * L.grid.z + 7) / 8 * 8;
+            }
+            if (tot > 0x7fffffffUL) {
+                gam_ens_destroy(e);
+                return fail(GAM_E_ARG, "gam_ens_create: batched grid too large");
+            }
+            h.wg0[n] = (unsigned)tot;
+            EnsPhase ph;
+            ph.launch = rec[r][0][p].launch;
+            ph.block = rec[r][0][p].block;
+            ph.nwg = (unsigned)tot;
+            ph.hdr_off = put(&h, sizeof(h));
+            const size_t rs = rec[r][0][p].rec.size();
+            ph.rec_off = put(rec[r][0][p].rec.data(), rs);
+            for (int q = 1; q < n; q++) {
+                const size_t at = host.size();
+                host.resize(at + rs);
+                memcpy(host.data() + at, rec[r][q][p].rec.data(), rs);
+            }
+            e->ph[r].push_back(ph);
+        }
+    }
+    hipError_t err = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+    if (err == hipSuccess) err = hipMalloc((void **)&e->dev, host.size());
+    if (err == hipSuccess) err = hipMemcpy(e->dev, host.data(), host.size(), hipMemcpyHostToDevice);
+    for (int r = 0; err == hipSuccess && r < 2; r++) {
+        err = hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal);
+        if (err != hipSuccess) break;
+        for (const EnsPhase &ph : e->ph[r])
+            ph.launch(e->stream, e->dev + ph.hdr_off, e->dev + ph.rec_off, ph.nwg, ph.block);
+        err = hipStreamEndCapture(e->stream, &e->graph[r]);
+        if (err == hipSuccess) err = hipGraphInstantiate(&e->exec[r], e->graph[r], nullptr, nullptr, 0);
+    }
+    if (err != hipSuccess) {
+        rc = fail(GAM_E_HIP, "gam_ens_create: %s", hipGetErrorString(err));
+        gam_ens_destroy(e);
+        return rc;
+    }
+    *out = e;
+    return GAM_OK;
+}
+
+// Issue nsteps graph launches; members' buffers and parity follow.
+static int ens_enqueue(gam_ens *e, int nsteps)
+{
+    int q0 = -1;
+    for (int q = 0; q < e->n; q++) {
+        gam_ctx *c = e->m[q];
+        const int rel = c->cur ^ e->cur0[q];
+        if (q0 < 0) q0 = rel;
+        if (rel != q0)
+            return fail(GAM_E_STATE, "gam_ens_run: member %d was stepped an odd number of times on its own", q);
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    e->parity = q0;
+    for (int n = 0; n < nsteps; n++) {
+        HIPCHK(hipGraphLaunch(e->exec[e->parity], e->stream));
+        e->parity ^= 1;
+    }
+    for (int q = 0; q < e->n; q++) e->m[q]->cur = e->cur0[q] ^ e->parity;
+    return GAM_OK;
+}
+
+static int ens_status(gam_ens *e)
+{
+    HIPCHK(hipStreamSynchronize(e->stream));
+    for (int q = 0; q < e->n; q++) {
+        Clock ck;
+        HIPCHK(hipMemcpy(&ck, e->m[q]->clk, sizeof(Clock), hipMemcpyDeviceToHost));
+        if (ck.status) {
+            status_code(ck.status);
+            char msg[512];
+            snprintf(msg, sizeof(msg), "%s", g_err);
+            return fail(ck.status >= 2 ? GAM_E_DT_DISASTER : GAM_E_DT_BAD, "ensemble member %d: %s", q, msg);
+        }
+    }
+    return GAM_OK;
+}
+
+extern "C" int gam_ens_run(gam_ens *e, int nsteps)
+{
+    if (!e) return fail(GAM_E_ARG, "gam_ens_run: NULL ensemble");
+    if (nsteps < 0) return fail(GAM_E_ARG, "gam_ens_run: nsteps < 0");
+    int rc = ens_enqueue(e, nsteps);
+    if (rc) return rc;
+    return ens_status(e);
+}
+
+extern "C" int gam_ens_time_steps(gam_ens *e, int nsteps, double *ms)
+{
+    if (!e) return fail(GAM_E_ARG, "gam_ens_time_steps: NULL ensemble");
+    if (nsteps < 1 || !ms) return fail(GAM_E_ARG, "gam_ens_time_steps: need nsteps >= 1 and an output");
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    int rc = GAM_OK;
+    for (int q = 0; q < e->n; q++) HIPCHK(hipStreamSynchronize(e->m[q]->stream));
+    HIPCHK(hipEventRecord(a, e->stream));
+    rc = ens_enqueue(e, nsteps);
+    HIPCHK(hipEventRecord(b, e->stream));
+    HIPCHK(hipEventSynchronize(b));
+    float f = 0.0f;
+    HIPCHK(hipEventElapsedTime(&f, a, b));
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    *ms = f;
+    if (rc) return rc;
+    return ens_status(e);
+}
+
+extern "C" int gam_ens_launches_per_step(gam_ens *e, int *n)
+{
+    if (!e || !n) return fail(GAM_E_ARG, "gam_ens_launches_per_step: NULL argument");
+    *n = (int)e->ph[0].size();
+    return GAM_OK;
+}

@@ -1,6 +1,47 @@
 // gam_kernels.h -- host-side launchers of the gfx950 kernels.
 #pragma once
 #include "gam_common.h"
+#include <string.h>
+#include <vector>
+
+// Argument record of one kernel launch: the arguments in order, as a
+// nested aggregate (a0, then the record of the rest).  The host writes it
+// to device memory for batched launches; layout is the same on both sides.
+template <class... A>
+struct Pack {};
+template <class A0, class... A>
+struct Pack<A0, A...> {
+    A0 a0;
+    Pack<A...> r;
+};
+__host__ __device__ inline Pack<> make_pack() { return Pack<>{}; }
+template <class A0, class... A>
+__host__ __device__ inline Pack<A0, A...> make_pack(A0 a0, A... a)
+{
+    return Pack<A0, A...>{a0, make_pack(a...)};
+}
+
+// Batched launch over the members of an ensemble: member m owns workgroups
+// wg0[m] .. wg0[m+1]-1 (wg0[m] a multiple of 8) of which the first
+// gx*gy*gz run its own grid.
+#define ENS_MAX 256
+struct EnsHdr {
+    unsigned n;
+    unsigned wg0[ENS_MAX + 1];
+    unsigned gx[ENS_MAX], gy[ENS_MAX], gz[ENS_MAX];
+};
+
+// One recorded launch of a step kernel: the function that issues the
+// batched form for this kernel and argument types, the member's own grid
+// and block, and the argument record bytes.
+struct EnsLaunch {
+    void (*launch)(hipStream_t s, const void *hdr, const void *rec, unsigned nwg, dim3 block);
+    dim3 grid, block;
+    std::vector<unsigned char> rec;
+};
+// While r is set, the step kernels that this thread would launch are
+// appended to *r instead (nothing is issued); nullptr restores launching.
+void set_launch_recorder(std::vector<EnsLaunch> *r);
 
 // k_finalize modes
 enum {

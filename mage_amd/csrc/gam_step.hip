@@ -52,9 +52,8 @@ struct Tile {
     int nb;           // chunks in all
 };
 
-__device__ __forceinline__ bool tile_point(const Tile &t, int &i, int &j, int &k)
+__device__ __forceinline__ bool tile_point(const Tile &t, int w, int &i, int &j, int &k)
 {
-    const int w = blockIdx.x;
     const int per = (t.nb + 7) >> 3;
     const int L = (w & 7) * per + (w >> 3);
     i = j = k = 0x40000000;
@@ -90,7 +89,7 @@ __device__ __forceinline__ bool tile_point(const Tile &t, int &i, int &j, int &k
 
 #define TILE_IJK(t)                  \
     int i, j, k;                     \
-    if (!tile_point((t), i, j, k)) return;
+    if (!tile_point((t), (int)bi.x, i, j, k)) return;
 
 static inline Tile tile_of(const Box &b, int ax1)
 {
@@ -112,6 +111,81 @@ static inline dim3 grid_of(const Box &b)
                 (unsigned)(b.k1 - b.k0 + 1));
 }
 
+// ====================================================================
+// Launch forms.  Every kernel of the step is a functor whose body takes its
+// workgroup's block index bi.  k_one<F> runs F over its own grid (one
+// context); k_ens<F> runs F for many contexts in one launch: the
+// concatenated grid of the members, each member's workgroups starting at a
+// multiple of 8 (so a member's chunk-to-XCD mapping is the same as in its
+// own launch) and each with its own argument record.
+// ====================================================================
+template <class F, class... A>
+__global__ __launch_bounds__(256) void k_one(A... a)
+{
+    F{}(make_uint3(blockIdx.x, blockIdx.y, blockIdx.z), a...);
+}
+
+template <class F, class... B>
+__device__ __forceinline__ void pcall(const Pack<> &, uint3 bi, B... b)
+{
+    F{}(bi, b...);
+}
+template <class F, class A0, class... A, class... B>
+__device__ __forceinline__ void pcall(const Pack<A0, A...> &p, uint3 bi, B... b)
+{
+    pcall<F>(p.r, bi, b..., p.a0);
+}
+
+template <class F, class... A>
+__global__ __launch_bounds__(256) void k_ens(const EnsHdr *__restrict__ h,
+                                             const Pack<A...> *__restrict__ rec)
+{
+    const unsigned w = blockIdx.x;
+    unsigned lo = 0, hi = h->n;   // wg0[lo] <= w < wg0[hi]
+    while (hi - lo > 1) {
+        const unsigned mid = (lo + hi) >> 1;
+        if (h->wg0[mid] <= w) lo = mid;
+        else hi = mid;
+    }
+    const unsigned gx = h->gx[lo], gy = h->gy[lo], gz = h->gz[lo];
+    unsigned l = w - h->wg0[lo];
+    if (l >= gx * gy * gz) return;   // padding of the member's range
+    uint3 bi;
+    bi.x = l % gx;
+    l /= gx;
+    bi.y = l % gy;
+    bi.z = l / gy;
+    pcall<F>(rec[lo], bi);
+}
+
+template <class F, class... A>
+static void launch_ens(hipStream_t s, const void *hdr, const void *rec, unsigned nwg, dim3 block)
+{
+    hipLaunchKernelGGL((k_ens<F, A...>), dim3(nwg), block, 0, s, (const EnsHdr *)hdr,
+                       (const Pack<A...> *)rec);
+}
+
+// Batchable launch: issued at once, or appended to this thread's recorder.
+static thread_local std::vector<EnsLaunch> *g_rec = nullptr;
+void set_launch_recorder(std::vector<EnsLaunch> *r) { g_rec = r; }
+
+template <class F, class... A>
+static void launch_f(dim3 grid, dim3 block, hipStream_t s, A... a)
+{
+    if (g_rec) {
+        const Pack<A...> p = make_pack(a...);
+        EnsLaunch e;
+        e.launch = &launch_ens<F, A...>;
+        e.grid = grid;
+        e.block = block;
+        e.rec.resize(sizeof(p));
+        memcpy(e.rec.data(), &p, sizeof(p));
+        g_rec->push_back(std::move(e));
+        return;
+    }
+    hipLaunchKernelGGL((k_one<F, A...>), grid, block, 0, s, a...);
+}
+
 // AB2 extrapolation (ExtrapAB2, mhdgroup.F90:709-717): q + pdt*(q-oq)/odt
 // with w = pdt/odt formed once per thread.
 __device__ __forceinline__ double ab2(double q, double oq, double w)
@@ -126,76 +200,79 @@ __device__ __forceinline__ double ab2(double q, double oq, double w)
 // the limiter state n.
 // Domain: all face points isg..ieg+1 (x) jsg..jeg+1 (x) ksg..keg+1.
 // ====================================================================
-__global__ __launch_bounds__(256) void k_predict(Geo g, MParm m, Tile fb, const Clock *clk,
-                                                 const double *__restrict__ S,
-                                                 const double *__restrict__ O,
-                                                 const double *__restrict__ met,
-                                                 double *__restrict__ H)
-{
-    TILE_IJK(fb);
-    const long N3 = g.N3;
-    const long c = IX(g, i, j, k);
-    const unsigned oc = (unsigned)(c * 8);    // byte offset of the point
-    const double st = clk->st, dtM = clk->dt;
-    double ost = clk->ost;
-    if (fabs(ost - st) < GAM_TINY) ost = st - dtM;   // mhdgroup.F90:26-28
-    const double odt = st - ost;
-    const double pdt = 0.5 * clk->dtA;
-    const double w = pdt / odt;
+struct k_predict {
+    __device__ __forceinline__ void operator()(const uint3 bi, Geo g, MParm m,
+                                               Tile fb, const Clock *clk,
+                                               const double *__restrict__ S,
+                                               const double *__restrict__ O,
+                                               const double *__restrict__ met,
+                                               double *__restrict__ H) const
+    {
+        TILE_IJK(fb);
+        const long N3 = g.N3;
+        const long c = IX(g, i, j, k);
+        const unsigned oc = (unsigned)(c * 8);    // byte offset of the point
+        const double st = clk->st, dtM = clk->dt;
+        double ost = clk->ost;
+        if (fabs(ost - st) < GAM_TINY) ost = st - dtM;   // mhdgroup.F90:26-28
+        const double odt = st - ost;
+        const double pdt = 0.5 * clk->dtA;
+        const double w = pdt / odt;
 
-    double f[3];
+        double f[3];
 #pragma unroll
-    for (int d = 0; d < 3; d++) {
-        f[d] = m.doMHD ? ab2(ldp(S + (S_MF + d) * N3, oc), ldp(O + (S_MF + d) * N3, oc), w)
-                       : ldp(H + (H_MF + d) * N3, oc);
-        stp(H + (H_MF + d) * N3, oc, f[d]);
+        for (int d = 0; d < 3; d++) {
+            f[d] = m.doMHD ? ab2(ldp(S + (S_MF + d) * N3, oc), ldp(O + (S_MF + d) * N3, oc), w)
+                           : ldp(H + (H_MF + d) * N3, oc);
+            stp(H + (H_MF + d) * N3, oc, f[d]);
+        }
+        if (i > g.ieg || j > g.jeg || k > g.keg) return;
+
+        // half-state cell B from the six extrapolated face fluxes
+        double fIp, fJp, fKp;
+        const unsigned oI = oc + 8, oJ = oc + 8 * g.P, oK = oc + (unsigned)(8 * g.PP);
+        if (m.doMHD) {
+            fIp = ab2(ldp(S + (S_MF + 0) * N3, oI), ldp(O + (S_MF + 0) * N3, oI), w);
+            fJp = ab2(ldp(S + (S_MF + 1) * N3, oJ), ldp(O + (S_MF + 1) * N3, oJ), w);
+            fKp = ab2(ldp(S + (S_MF + 2) * N3, oK), ldp(O + (S_MF + 2) * N3, oK), w);
+        } else {
+            fIp = ldp(H + (H_MF + 0) * N3, oI);
+            fJp = ldp(H + (H_MF + 1) * N3, oJ);
+            fKp = ldp(H + (H_MF + 2) * N3, oK);
+        }
+        double B[3];
+        cell_b(g, met, c, f[0], f[1], f[2], fIp, fJp, fKp, B);
+#pragma unroll
+        for (int x = 0; x < 3; x++) stp(H + (H_B + x) * N3, oc, B[x]);
+
+        // PredictCell (mhdgroup.F90:528-561)
+        double Q[5], oQ[5], W[5], oW[5], pW[5], pQ[5], Wb[5];
+#pragma unroll
+        for (int v = 0; v < 5; v++) { Q[v] = ldp(S + (S_GAS + v) * N3, oc); oQ[v] = ldp(O + (S_GAS + v) * N3, oc); }
+        c2p_cell(m, oQ, oW);
+        c2p_cell(m, Q, W);
+#pragma unroll
+        for (int v = 0; v < 5; v++) pW[v] = ab2(W[v], oW[v], w);
+        pW[DEN] = dmax(pW[DEN], m.dFloor);
+        pW[PRS] = dmax(pW[PRS], m.pFloor);
+        p2c_cell(m, pW, pQ);
+#pragma unroll
+        for (int v = 0; v < 5; v++) stp(H + (H_GAS + v) * N3, oc, pQ[v]);
+        // primitives of the reconstruction stencils (half state), stored
+        // multiplied by the cell volume: every use is the volume-weighted
+        // interpolation <V q>/<V> (recon.F90:125-198, fields.F90:262-270), and
+        // the product is the one those formulas would form.  The limiter
+        // stencils take plain primitives: of state n (doLFMLim), else of the
+        // half state.
+        c2p_block(m, pQ, Wb);
+        const double vol = ldp(met + M_VOL * N3, oc);
+#pragma unroll
+        for (int v = 0; v < 5; v++) stp(H + (H_WH + v) * N3, oc, vol * Wb[v]);
+        if (m.doLFM) c2p_block(m, Q, Wb);
+#pragma unroll
+        for (int v = 0; v < 5; v++) stp(H + (H_WN + v) * N3, oc, Wb[v]);
     }
-    if (i > g.ieg || j > g.jeg || k > g.keg) return;
-
-    // half-state cell B from the six extrapolated face fluxes
-    double fIp, fJp, fKp;
-    const unsigned oI = oc + 8, oJ = oc + 8 * g.P, oK = oc + (unsigned)(8 * g.PP);
-    if (m.doMHD) {
-        fIp = ab2(ldp(S + (S_MF + 0) * N3, oI), ldp(O + (S_MF + 0) * N3, oI), w);
-        fJp = ab2(ldp(S + (S_MF + 1) * N3, oJ), ldp(O + (S_MF + 1) * N3, oJ), w);
-        fKp = ab2(ldp(S + (S_MF + 2) * N3, oK), ldp(O + (S_MF + 2) * N3, oK), w);
-    } else {
-        fIp = ldp(H + (H_MF + 0) * N3, oI);
-        fJp = ldp(H + (H_MF + 1) * N3, oJ);
-        fKp = ldp(H + (H_MF + 2) * N3, oK);
-    }
-    double B[3];
-    cell_b(g, met, c, f[0], f[1], f[2], fIp, fJp, fKp, B);
-#pragma unroll
-    for (int x = 0; x < 3; x++) stp(H + (H_B + x) * N3, oc, B[x]);
-
-    // PredictCell (mhdgroup.F90:528-561)
-    double Q[5], oQ[5], W[5], oW[5], pW[5], pQ[5], Wb[5];
-#pragma unroll
-    for (int v = 0; v < 5; v++) { Q[v] = ldp(S + (S_GAS + v) * N3, oc); oQ[v] = ldp(O + (S_GAS + v) * N3, oc); }
-    c2p_cell(m, oQ, oW);
-    c2p_cell(m, Q, W);
-#pragma unroll
-    for (int v = 0; v < 5; v++) pW[v] = ab2(W[v], oW[v], w);
-    pW[DEN] = dmax(pW[DEN], m.dFloor);
-    pW[PRS] = dmax(pW[PRS], m.pFloor);
-    p2c_cell(m, pW, pQ);
-#pragma unroll
-    for (int v = 0; v < 5; v++) stp(H + (H_GAS + v) * N3, oc, pQ[v]);
-    // primitives of the reconstruction stencils (half state), stored
-    // multiplied by the cell volume: every use is the volume-weighted
-    // interpolation <V q>/<V> (recon.F90:125-198, fields.F90:262-270), and
-    // the product is the one those formulas would form.  The limiter
-    // stencils take plain primitives: of state n (doLFMLim), else of the
-    // half state.
-    c2p_block(m, pQ, Wb);
-    const double vol = ldp(met + M_VOL * N3, oc);
-#pragma unroll
-    for (int v = 0; v < 5; v++) stp(H + (H_WH + v) * N3, oc, vol * Wb[v]);
-    if (m.doLFM) c2p_block(m, Q, Wb);
-#pragma unroll
-    for (int v = 0; v < 5; v++) stp(H + (H_WN + v) * N3, oc, Wb[v]);
-}
+};
 
 // ====================================================================
 // Column tiles for kernels that march along k: a workgroup owns 256
@@ -224,9 +301,8 @@ static inline ColTile coltile_of(const Box &b)
 static inline dim3 cgrid(const ColTile &t) { return dim3((unsigned)(((t.nb + 7) / 8) * 8)); }
 
 // Column of this thread: (i, j) and the k range [k0, k1]; false if none.
-__device__ __forceinline__ bool col_point(const ColTile &t, int &i, int &j, int &k0, int &k1)
+__device__ __forceinline__ bool col_point(const ColTile &t, int w, int &i, int &j, int &k0, int &k1)
 {
-    const int w = blockIdx.x;
     const int per = (t.nb + 7) >> 3;
     const int L = (w & 7) * per + (w >> 3);
     if (L >= t.nb) return false;
@@ -250,80 +326,82 @@ __device__ __forceinline__ bool col_point(const ColTile &t, int &i, int &j, int 
 // all cells).  The velocities rho*v/max(rho,dFloor) are the half-state
 // primitives the predictor stored.  The k stencil slides along the column.
 // ====================================================================
-__global__ __launch_bounds__(256) void k_vface(Geo g, MParm m, ColTile t,
-                                               const double *__restrict__ H,
+struct k_vface {
+    __device__ __forceinline__ void operator()(const uint3 bi, Geo g, MParm m,
+                                               ColTile t, const double *__restrict__ H,
                                                const double *__restrict__ met,
-                                               double *__restrict__ Vf)
-{
-    int i, j, k0, k1;
-    if (!col_point(t, i, j, k0, k1)) return;
-    const long N3 = g.N3;
-    const double *vol = met + (long)M_VOL * N3;
-    const double *wv = H + (long)(H_WH + 1) * N3;   // V * velocity
-    // k window: slot n holds cell k-4+n (zero outside the cell range)
-    double zv[8], zx[8], zy[8], zz[8];
+                                               double *__restrict__ Vf) const
+    {
+        int i, j, k0, k1;
+        if (!col_point(t, (int)bi.x, i, j, k0, k1)) return;
+        const long N3 = g.N3;
+        const double *vol = met + (long)M_VOL * N3;
+        const double *wv = H + (long)(H_WH + 1) * N3;   // V * velocity
+        // k window: slot n holds cell k-4+n (zero outside the cell range)
+        double zv[8], zx[8], zy[8], zz[8];
 #pragma unroll
-    for (int n = 0; n < 7; n++) {
-        const int kk = k0 - 4 + n;
-        const bool in = kk >= g.ksg && kk <= g.keg;
-        const long q = IX(g, i, j, in ? kk : g.ksg);
-        zv[n] = in ? vol[q] : 0.0;
-        zx[n] = in ? wv[q] : 0.0;
-        zy[n] = in ? wv[N3 + q] : 0.0;
-        zz[n] = in ? wv[2 * N3 + q] : 0.0;
-    }
-    const bool iI = i >= g.is && i <= g.ie + 1;
-    const bool iJ = j >= g.js && j <= g.je + 1;
+        for (int n = 0; n < 7; n++) {
+            const int kk = k0 - 4 + n;
+            const bool in = kk >= g.ksg && kk <= g.keg;
+            const long q = IX(g, i, j, in ? kk : g.ksg);
+            zv[n] = in ? vol[q] : 0.0;
+            zx[n] = in ? wv[q] : 0.0;
+            zy[n] = in ? wv[N3 + q] : 0.0;
+            zz[n] = in ? wv[2 * N3 + q] : 0.0;
+        }
+        const bool iI = i >= g.is && i <= g.ie + 1;
+        const bool iJ = j >= g.js && j <= g.je + 1;
 #pragma unroll 1
-    for (int kl = k0; kl <= k1; kl++) {
-        int k = kl;
-        asm volatile("" : "+v"(k));   // one address base per iteration
-        {
-            const int kk = k + 3;
-            const bool in = kk <= g.keg;
-            long q = IX(g, i, j, in ? kk : g.keg);
-            asm volatile("" : "+v"(q));
-            zv[7] = in ? vol[q] : 0.0;
-            zx[7] = in ? wv[q] : 0.0;
-            zy[7] = in ? wv[N3 + q] : 0.0;
-            zz[7] = in ? wv[2 * N3 + q] : 0.0;
-        }
-        long c = IX(g, i, j, k);
-        asm volatile("" : "+v"(c));
-        if (k >= g.ks && k <= g.ke + 1) {
-            const double r = 1.0 / dmax(wsum8([&](int n) { return zv[n]; }), GAM_TINY);
-            Vf[(6 + 0) * N3 + c] = wsum8([&](int n) { return zx[n]; }) * r;
-            Vf[(6 + 1) * N3 + c] = wsum8([&](int n) { return zy[n]; }) * r;
-            Vf[(6 + 2) * N3 + c] = wsum8([&](int n) { return zz[n]; }) * r;
-        }
-        if (iI) {
-            double v8[8], x8[8], y8[8], z8[8];
-#pragma unroll
-            for (int n = 0; n < 8; n++) {
-                const long q = c + (n - 4);
-                v8[n] = vol[q]; x8[n] = wv[q]; y8[n] = wv[N3 + q]; z8[n] = wv[2 * N3 + q];
+        for (int kl = k0; kl <= k1; kl++) {
+            int k = kl;
+            asm volatile("" : "+v"(k));   // one address base per iteration
+            {
+                const int kk = k + 3;
+                const bool in = kk <= g.keg;
+                long q = IX(g, i, j, in ? kk : g.keg);
+                asm volatile("" : "+v"(q));
+                zv[7] = in ? vol[q] : 0.0;
+                zx[7] = in ? wv[q] : 0.0;
+                zy[7] = in ? wv[N3 + q] : 0.0;
+                zz[7] = in ? wv[2 * N3 + q] : 0.0;
             }
-            const double r = 1.0 / dmax(wsum8([&](int n) { return v8[n]; }), GAM_TINY);
-            Vf[0 * N3 + c] = wsum8([&](int n) { return x8[n]; }) * r;
-            Vf[1 * N3 + c] = wsum8([&](int n) { return y8[n]; }) * r;
-            Vf[2 * N3 + c] = wsum8([&](int n) { return z8[n]; }) * r;
-        }
-        if (iJ) {
-            double v8[8], x8[8], y8[8], z8[8];
-#pragma unroll
-            for (int n = 0; n < 8; n++) {
-                const long q = c + (long)(n - 4) * g.P;
-                v8[n] = vol[q]; x8[n] = wv[q]; y8[n] = wv[N3 + q]; z8[n] = wv[2 * N3 + q];
+            long c = IX(g, i, j, k);
+            asm volatile("" : "+v"(c));
+            if (k >= g.ks && k <= g.ke + 1) {
+                const double r = 1.0 / dmax(wsum8([&](int n) { return zv[n]; }), GAM_TINY);
+                Vf[(6 + 0) * N3 + c] = wsum8([&](int n) { return zx[n]; }) * r;
+                Vf[(6 + 1) * N3 + c] = wsum8([&](int n) { return zy[n]; }) * r;
+                Vf[(6 + 2) * N3 + c] = wsum8([&](int n) { return zz[n]; }) * r;
             }
-            const double r = 1.0 / dmax(wsum8([&](int n) { return v8[n]; }), GAM_TINY);
-            Vf[3 * N3 + c] = wsum8([&](int n) { return x8[n]; }) * r;
-            Vf[4 * N3 + c] = wsum8([&](int n) { return y8[n]; }) * r;
-            Vf[5 * N3 + c] = wsum8([&](int n) { return z8[n]; }) * r;
-        }
+            if (iI) {
+                double v8[8], x8[8], y8[8], z8[8];
 #pragma unroll
-        for (int n = 0; n < 7; n++) { zv[n] = zv[n + 1]; zx[n] = zx[n + 1]; zy[n] = zy[n + 1]; zz[n] = zz[n + 1]; }
+                for (int n = 0; n < 8; n++) {
+                    const long q = c + (n - 4);
+                    v8[n] = vol[q]; x8[n] = wv[q]; y8[n] = wv[N3 + q]; z8[n] = wv[2 * N3 + q];
+                }
+                const double r = 1.0 / dmax(wsum8([&](int n) { return v8[n]; }), GAM_TINY);
+                Vf[0 * N3 + c] = wsum8([&](int n) { return x8[n]; }) * r;
+                Vf[1 * N3 + c] = wsum8([&](int n) { return y8[n]; }) * r;
+                Vf[2 * N3 + c] = wsum8([&](int n) { return z8[n]; }) * r;
+            }
+            if (iJ) {
+                double v8[8], x8[8], y8[8], z8[8];
+#pragma unroll
+                for (int n = 0; n < 8; n++) {
+                    const long q = c + (long)(n - 4) * g.P;
+                    v8[n] = vol[q]; x8[n] = wv[q]; y8[n] = wv[N3 + q]; z8[n] = wv[2 * N3 + q];
+                }
+                const double r = 1.0 / dmax(wsum8([&](int n) { return v8[n]; }), GAM_TINY);
+                Vf[3 * N3 + c] = wsum8([&](int n) { return x8[n]; }) * r;
+                Vf[4 * N3 + c] = wsum8([&](int n) { return y8[n]; }) * r;
+                Vf[5 * N3 + c] = wsum8([&](int n) { return z8[n]; }) * r;
+            }
+#pragma unroll
+            for (int n = 0; n < 7; n++) { zv[n] = zv[n + 1]; zx[n] = zx[n + 1]; zy[n] = zy[n + 1]; zz[n] = zz[n + 1]; }
+        }
     }
-}
+};
 
 // ====================================================================
 // Edge EMF: GetCornerV/GetCornerD/GetCornerB/EdgeLRs + VxB
@@ -357,72 +435,75 @@ __device__ __forceinline__ void edge_lr(const Geo &g, const MParm &m, long c,
 }
 
 template <int ED, int RM, bool BG, bool BORIS>
-__global__ __launch_bounds__(256) void k_emf(Geo g, MParm m, Tile b, const Clock *clk,
-                                             const double *__restrict__ H,
-                                             const double *__restrict__ S,
-                                             const double *__restrict__ Vf,
-                                             const double *__restrict__ met,
-                                             const double *__restrict__ bg,
-                                             double *__restrict__ E)
-{
-    TILE_IJK(b);
-    constexpr int T1 = (ED == 0) ? 1 : (ED == 1) ? 2 : 0;
-    constexpr int T2 = (ED == 0) ? 2 : (ED == 1) ? 0 : 1;
-    // getNormalPlane (metric.F90:151-167)
-    constexpr int E1 = (ED == 0) ? 1 : 0;
-    constexpr int E2 = (ED == 2) ? 1 : 2;
-    const long N3 = g.N3;
-    const long c = IX(g, i, j, k);
-    const long s2 = STRIDE<T2>(g);
+struct k_emf {
+    __device__ __forceinline__ void operator()(const uint3 bi, Geo g, MParm m,
+                                               Tile b, const Clock *clk,
+                                               const double *__restrict__ H,
+                                               const double *__restrict__ S,
+                                               const double *__restrict__ Vf,
+                                               const double *__restrict__ met,
+                                               const double *__restrict__ bg,
+                                               double *__restrict__ E) const
+    {
+        TILE_IJK(b);
+        constexpr int T1 = (ED == 0) ? 1 : (ED == 1) ? 2 : 0;
+        constexpr int T2 = (ED == 0) ? 2 : (ED == 1) ? 0 : 1;
+        // getNormalPlane (metric.F90:151-167)
+        constexpr int E1 = (ED == 0) ? 1 : 0;
+        constexpr int E2 = (ED == 2) ? 1 : 2;
+        const long N3 = g.N3;
+        const long c = IX(g, i, j, k);
+        const long s2 = STRIDE<T2>(g);
 
-    // GetCornerV (fields.F90:457-517), doVdA
-    double A[8];
+        // GetCornerV (fields.F90:457-517), doVdA
+        double A[8];
 #pragma unroll
-    for (int n = 0; n < 8; n++) A[n] = ldx<T2>(met + (M_FACE + T1) * N3, c, s2, n - 4);
-    const double rdA = 1.0 / dmax(wsum8([&](int n) { return A[n]; }), GAM_TINY);
-    double vc[3];
+        for (int n = 0; n < 8; n++) A[n] = ldx<T2>(met + (M_FACE + T1) * N3, c, s2, n - 4);
+        const double rdA = 1.0 / dmax(wsum8([&](int n) { return A[n]; }), GAM_TINY);
+        double vc[3];
 #pragma unroll
-    for (int d = 0; d < 3; d++)
-        vc[d] = wsum8([&](int n) { return A[n] * ldx<T2>(Vf + d * N3, c, s2, n - 4); }) * rdA;
-    const double *Te = met + (long)(M_TE + 9 * ED) * N3 + c;
-    const double v1 = vc[0] * Te[3 * N3] + vc[1] * Te[4 * N3] + vc[2] * Te[5 * N3];
-    const double v2 = vc[0] * Te[6 * N3] + vc[1] * Te[7 * N3] + vc[2] * Te[8 * N3];
-    double vDiff = sqrt(v1 * v1 + v2 * v2);
+        for (int d = 0; d < 3; d++)
+            vc[d] = wsum8([&](int n) { return A[n] * ldx<T2>(Vf + d * N3, c, s2, n - 4); }) * rdA;
+        const double *Te = met + (long)(M_TE + 9 * ED) * N3 + c;
+        const double v1 = vc[0] * Te[3 * N3] + vc[1] * Te[4 * N3] + vc[2] * Te[5 * N3];
+        const double v2 = vc[0] * Te[6 * N3] + vc[1] * Te[7 * N3] + vc[2] * Te[8 * N3];
+        double vDiff = sqrt(v1 * v1 + v2 * v2);
 
-    // GetCornerD (fields.F90:427-454)
-    const long se1 = STRIDE<E1>(g), se2 = STRIDE<E2>(g);
-    const double *Dh = H + (H_GAS + DEN) * N3;
-    const double Dc = 0.25 * (Dh[c] + Dh[c - se1] + Dh[c - se2] + Dh[c - se1 - se2]);
+        // GetCornerD (fields.F90:427-454)
+        const long se1 = STRIDE<E1>(g), se2 = STRIDE<E2>(g);
+        const double *Dh = H + (H_GAS + DEN) * N3;
+        const double Dc = 0.25 * (Dh[c] + Dh[c - se1] + Dh[c - se2] + Dh[c - se1 - se2]);
 
-    // GetCornerB (fields.F90:365-424)
-    const double *mfH = H + H_MF * N3;
-    const double *mfL = m.doLFM ? S + S_MF * N3 : mfH;
-    double bT1L, bT1R, fA1, bT2L, bT2R, fA2;
-    edge_lr<T1, T2, RM>(g, m, c, met, mfH, mfL, bT1L, bT1R, fA1);
-    edge_lr<T2, T1, RM>(g, m, c, met, mfH, mfL, bT2L, bT2R, fA2);
-    const double *Tb = met + (long)(M_TEB + 4 * ED) * N3 + c;
-    const double xni = Tb[0], yni = Tb[N3], xnj = Tb[2 * N3], ynj = Tb[3 * N3];
-    const double detT = 1.0 / (xni * ynj - xnj * yni);
-    const double rA1 = 1.0 / fA1, rA2 = 1.0 / fA2;
-    const double bT1 = 0.5 * (bT1L + bT1R) * rA1;
-    const double bT2 = 0.5 * (bT2L + bT2R) * rA2;
-    const double db1 = (bT1R - bT1L) * rA1;
-    const double db2 = (bT2R - bT2L) * rA2;
-    const double Jd = db2 - db1;
-    double b1 = detT * (ynj * bT1 - yni * bT2);
-    double b2 = detT * (-(xnj * bT1) + xni * bT2);
-    if (BG) {
-        b1 = b1 + bg[(B_EDGB0 + 0 + 2 * ED) * N3 + c];
-        b2 = b2 + bg[(B_EDGB0 + 1 + 2 * ED) * N3 + c];
+        // GetCornerB (fields.F90:365-424)
+        const double *mfH = H + H_MF * N3;
+        const double *mfL = m.doLFM ? S + S_MF * N3 : mfH;
+        double bT1L, bT1R, fA1, bT2L, bT2R, fA2;
+        edge_lr<T1, T2, RM>(g, m, c, met, mfH, mfL, bT1L, bT1R, fA1);
+        edge_lr<T2, T1, RM>(g, m, c, met, mfH, mfL, bT2L, bT2R, fA2);
+        const double *Tb = met + (long)(M_TEB + 4 * ED) * N3 + c;
+        const double xni = Tb[0], yni = Tb[N3], xnj = Tb[2 * N3], ynj = Tb[3 * N3];
+        const double detT = 1.0 / (xni * ynj - xnj * yni);
+        const double rA1 = 1.0 / fA1, rA2 = 1.0 / fA2;
+        const double bT1 = 0.5 * (bT1L + bT1R) * rA1;
+        const double bT2 = 0.5 * (bT2L + bT2R) * rA2;
+        const double db1 = (bT1R - bT1L) * rA1;
+        const double db2 = (bT2R - bT2L) * rA2;
+        const double Jd = db2 - db1;
+        double b1 = detT * (ynj * bT1 - yni * bT2);
+        double b2 = detT * (-(xnj * bT1) + xni * bT2);
+        if (BG) {
+            b1 = b1 + bg[(B_EDGB0 + 0 + 2 * ED) * N3 + c];
+            b2 = b2 + bg[(B_EDGB0 + 1 + 2 * ED) * N3 + c];
+        }
+        const double edge = met[(M_EDGE + ED) * N3 + c];
+        double vA = sqrt(b1 * b1 + b2 * b2) / sqrt(Dc);
+        if (BORIS) vA = m.Ca * vA / sqrt(m.Ca * m.Ca + vA * vA);
+        vDiff = vDiff + vA;
+        vDiff = dmin(vDiff, m.CFL * edge / clk->dt);
+        const double e = -(v1 * b2 - v2 * b1) + m.Vd0 * vDiff * Jd;
+        E[ED * N3 + c] = e * edge;
     }
-    const double edge = met[(M_EDGE + ED) * N3 + c];
-    double vA = sqrt(b1 * b1 + b2 * b2) / sqrt(Dc);
-    if (BORIS) vA = m.Ca * vA / sqrt(m.Ca * m.Ca + vA * vA);
-    vDiff = vDiff + vA;
-    vDiff = dmin(vDiff, m.CFL * edge / clk->dt);
-    const double e = -(v1 * b2 - v2 * b1) + m.Vd0 * vDiff * Jd;
-    E[ED * N3 + c] = e * edge;
-}
+};
 
 // ====================================================================
 // The fluxes of one face from its reconstructed L/R states (the part of
@@ -504,66 +585,68 @@ __device__ __forceinline__ void face_flux_lr(const Geo &g, const MParm &m,
 // are contiguous and merge into wide loads)
 // ====================================================================
 template <int D, int RM, bool BG, bool BORIS>
-__global__ __launch_bounds__(256) void k_flux(Geo g, MParm m, Tile b,
-                                              const double *__restrict__ H,
-                                              const double *__restrict__ S,
-                                              const double *__restrict__ met,
-                                              const double *__restrict__ bg,
-                                              double *__restrict__ gflx,
-                                              double *__restrict__ mflx)
-{
-    TILE_IJK(b);
-    const long N3 = g.N3;
-    const long c = IX(g, i, j, k);
-    const long s = STRIDE<D>(g);
-    // byte offsets of the 8 stencil points (cells c-4s .. c+3s), shared by
-    // every plane; o[4] is the face point c itself.  Along i the points are
-    // contiguous and the compiler merges them into wide loads from one
-    // address, so that direction keeps plain indexing.
-    unsigned o[8];
+struct k_flux {
+    __device__ __forceinline__ void operator()(const uint3 bi, Geo g, MParm m,
+                                               Tile b, const double *__restrict__ H,
+                                               const double *__restrict__ S,
+                                               const double *__restrict__ met,
+                                               const double *__restrict__ bg,
+                                               double *__restrict__ gflx,
+                                               double *__restrict__ mflx) const
+    {
+        TILE_IJK(b);
+        const long N3 = g.N3;
+        const long c = IX(g, i, j, k);
+        const long s = STRIDE<D>(g);
+        // byte offsets of the 8 stencil points (cells c-4s .. c+3s), shared by
+        // every plane; o[4] is the face point c itself.  Along i the points are
+        // contiguous and the compiler merges them into wide loads from one
+        // address, so that direction keeps plain indexing.
+        unsigned o[8];
 #pragma unroll
-    for (int n = 0; n < 8; n++) o[n] = (unsigned)((c + (n - 4) * s) * 8);
-    auto LD = [&](const double *__restrict__ plane, int n) -> double {
-        if (D == 0) return plane[c + (n - 4)];
-        return ldp(plane, o[n]);
-    };
+        for (int n = 0; n < 8; n++) o[n] = (unsigned)((c + (n - 4) * s) * 8);
+        auto LD = [&](const double *__restrict__ plane, int n) -> double {
+            if (D == 0) return plane[c + (n - 4)];
+            return ldp(plane, o[n]);
+        };
 
-    double vol[8];
+        double vol[8];
 #pragma unroll
-    for (int n = 0; n < 8; n++) vol[n] = LD(met + M_VOL * N3, n);
-    const double rVi = 1.0 / wsum8([&](int n) { return vol[n]; });
+        for (int n = 0; n < 8; n++) vol[n] = LD(met + M_VOL * N3, n);
+        const double rVi = 1.0 / wsum8([&](int n) { return vol[n]; });
 
-    // L/R primitives (BlockStateLRs, recon.F90:39-93); Wv holds V*W
-    const double *Wv = H + H_WH * N3;
-    const double *Wl = H + H_WN * N3;
-    double WL[5], WR[5];
+        // L/R primitives (BlockStateLRs, recon.F90:39-93); Wv holds V*W
+        const double *Wv = H + H_WH * N3;
+        const double *Wl = H + H_WN * N3;
+        double WL[5], WR[5];
 #pragma unroll
-    for (int v = 0; v < 5; v++) {
-        double qv[8], nq[4];
+        for (int v = 0; v < 5; v++) {
+            double qv[8], nq[4];
 #pragma unroll
-        for (int n = 0; n < 8; n++) qv[n] = LD(Wv + v * N3, n);
+            for (int n = 0; n < 8; n++) qv[n] = LD(Wv + v * N3, n);
 #pragma unroll
-        for (int n = 0; n < 4; n++) nq[n] = LD(Wl + v * N3, n + 2);
-        recon_lr_w<RM>(m.pdmb, rVi, nq, qv, WL[v], WR[v]);
-    }
-    auto recon_b = [&](double *BL, double *BR) {
-        const double *Bh = H + H_B * N3;
-        const double *Bl = m.doLFM ? S + S_B * N3 : Bh;
-#pragma unroll
-        for (int x = 0; x < 3; x++) {
-            double q[8], nq[4];
-#pragma unroll
-            for (int n = 0; n < 8; n++) q[n] = LD(Bh + x * N3, n);
-#pragma unroll
-            for (int n = 0; n < 4; n++) nq[n] = LD(Bl + x * N3, n + 2);
-            recon_lr<RM>(m.pdmb, vol, rVi, nq, q, BL[x], BR[x]);
+            for (int n = 0; n < 4; n++) nq[n] = LD(Wl + v * N3, n + 2);
+            recon_lr_w<RM>(m.pdmb, rVi, nq, qv, WL[v], WR[v]);
         }
-    };
-    double BL[3], BR[3];
-    face_flux_lr<D, BG, BORIS>(
-        g, m, H, met, bg, [&](const double *p) { return LD(p, 4); },
-        [&](const double *p) { return LD(p, 3); }, recon_b, WL, WR, BL, BR, o[4], gflx, mflx);
-}
+        auto recon_b = [&](double *BL, double *BR) {
+            const double *Bh = H + H_B * N3;
+            const double *Bl = m.doLFM ? S + S_B * N3 : Bh;
+#pragma unroll
+            for (int x = 0; x < 3; x++) {
+                double q[8], nq[4];
+#pragma unroll
+                for (int n = 0; n < 8; n++) q[n] = LD(Bh + x * N3, n);
+#pragma unroll
+                for (int n = 0; n < 4; n++) nq[n] = LD(Bl + x * N3, n + 2);
+                recon_lr<RM>(m.pdmb, vol, rVi, nq, q, BL[x], BR[x]);
+            }
+        };
+        double BL[3], BR[3];
+        face_flux_lr<D, BG, BORIS>(
+            g, m, H, met, bg, [&](const double *p) { return LD(p, 4); },
+            [&](const double *p) { return LD(p, 3); }, recon_b, WL, WR, BL, BR, o[4], gflx, mflx);
+    }
+};
 
 // ====================================================================
 // Interface fluxes along j or k (D = 1, 2) with the stencil planes staged
@@ -585,121 +668,123 @@ struct FluxTile {
 };
 
 template <int D, int RM, bool BG, bool BORIS>
-__global__ __launch_bounds__(256) void k_flux_s(Geo g, MParm m, FluxTile t,
-                                                const double *__restrict__ H,
-                                                const double *__restrict__ S,
-                                                const double *__restrict__ met,
-                                                const double *__restrict__ bg,
-                                                double *__restrict__ gflx,
-                                                double *__restrict__ mflx)
-{
-    __shared__ double vs[SR8][BX];
-    __shared__ double buf[2][SRW][BX];
-    const int w = blockIdx.x;
-    const int per = (t.nb + 7) >> 3;
-    const int L = (w & 7) * per + (w >> 3);
-    if (L >= t.nb) return;                   // whole workgroup
-    const int ti = L % t.nti;
-    const int r = L / t.nti;
-    const int ta = r % t.nta;
-    const int bb = t.b0 + r / t.nta;
-    const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + BX * ty;
-    const int i0 = t.i0 + ti * BX, a0 = t.a0 + ta * BY;
-    const int i = i0 + tx, a = a0 + ty;
-    const bool act = i <= t.i1 && a <= t.a1;
-    const long N3 = g.N3;
-    const long s = STRIDE<D>(g);
-    // point (i', a', bb) of the box's index space
-    auto at = [&](int ii, int aa) -> long {
-        return D == 1 ? IX(g, ii, aa, bb) : IX(g, ii, bb, aa);
-    };
-    // rows that exist: the stencils of the active faces reach a1 + 3
-    const int aTop = t.a1 + 3;
-    // stage loaders: element e of a stage -> (row, column)
-    auto fetch = [&](const double *p8, const double *p4, double *reg) {
+struct k_flux_s {
+    __device__ __forceinline__ void operator()(const uint3 bi, Geo g, MParm m,
+                                               FluxTile t, const double *__restrict__ H,
+                                               const double *__restrict__ S,
+                                               const double *__restrict__ met,
+                                               const double *__restrict__ bg,
+                                               double *__restrict__ gflx,
+                                               double *__restrict__ mflx) const
+    {
+        __shared__ double vs[SR8][BX];
+        __shared__ double buf[2][SRW][BX];
+        const int w = (int)bi.x;
+        const int per = (t.nb + 7) >> 3;
+        const int L = (w & 7) * per + (w >> 3);
+        if (L >= t.nb) return;                   // whole workgroup
+        const int ti = L % t.nti;
+        const int r = L / t.nti;
+        const int ta = r % t.nta;
+        const int bb = t.b0 + r / t.nta;
+        const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + BX * ty;
+        const int i0 = t.i0 + ti * BX, a0 = t.a0 + ta * BY;
+        const int i = i0 + tx, a = a0 + ty;
+        const bool act = i <= t.i1 && a <= t.a1;
+        const long N3 = g.N3;
+        const long s = STRIDE<D>(g);
+        // point (i', a', bb) of the box's index space
+        auto at = [&](int ii, int aa) -> long {
+            return D == 1 ? IX(g, ii, aa, bb) : IX(g, ii, bb, aa);
+        };
+        // rows that exist: the stencils of the active faces reach a1 + 3
+        const int aTop = t.a1 + 3;
+        // stage loaders: element e of a stage -> (row, column)
+        auto fetch = [&](const double *p8, const double *p4, double *reg) {
 #pragma unroll
-        for (int q = 0; q < SLD; q++) {
-            const int e = tid + q * BX * BY;
-            const int row = e / BX, col = e % BX;
-            const int ii = i0 + col;
-            double v = 0.0;
-            if (row < SRW && ii <= t.i1) {
-                const int aa = row < SR8 ? a0 - 4 + row : a0 - 2 + (row - SR8);
-                const double *p = row < SR8 ? p8 : p4;
-                if (aa <= aTop) v = p[at(ii, aa)];
+            for (int q = 0; q < SLD; q++) {
+                const int e = tid + q * BX * BY;
+                const int row = e / BX, col = e % BX;
+                const int ii = i0 + col;
+                double v = 0.0;
+                if (row < SRW && ii <= t.i1) {
+                    const int aa = row < SR8 ? a0 - 4 + row : a0 - 2 + (row - SR8);
+                    const double *p = row < SR8 ? p8 : p4;
+                    if (aa <= aTop) v = p[at(ii, aa)];
+                }
+                reg[q] = v;
             }
-            reg[q] = v;
-        }
-    };
-    auto store = [&](int bsel, const double *reg) {
+        };
+        auto store = [&](int bsel, const double *reg) {
 #pragma unroll
-        for (int q = 0; q < SLD; q++) {
-            const int e = tid + q * BX * BY;
-            if (e < SRW * BX) buf[bsel][e / BX][e % BX] = reg[q];
+            for (int q = 0; q < SLD; q++) {
+                const int e = tid + q * BX * BY;
+                if (e < SRW * BX) buf[bsel][e / BX][e % BX] = reg[q];
+            }
+        };
+        // volumes (8-point rows) and the first stage
+        {
+            const double *pv = met + M_VOL * N3;
+            for (int e = tid; e < SR8 * BX; e += BX * BY) {
+                const int row = e / BX, col = e % BX;
+                const int ii = i0 + col, aa = a0 - 4 + row;
+                vs[row][col] = (ii <= t.i1 && aa <= aTop) ? pv[at(ii, aa)] : 0.0;
+            }
         }
-    };
-    // volumes (8-point rows) and the first stage
-    {
-        const double *pv = met + M_VOL * N3;
-        for (int e = tid; e < SR8 * BX; e += BX * BY) {
-            const int row = e / BX, col = e % BX;
-            const int ii = i0 + col, aa = a0 - 4 + row;
-            vs[row][col] = (ii <= t.i1 && aa <= aTop) ? pv[at(ii, aa)] : 0.0;
-        }
-    }
-    const int nst = m.doMHD ? 8 : 5;
-    const double *Wv = H + H_WH * N3, *Wl = H + H_WN * N3;
-    const double *Bh = H + H_B * N3, *Bl = m.doLFM ? S + S_B * N3 : Bh;
-    auto planes = [&](int st, const double *&p8, const double *&p4) {
-        if (st < 5) { p8 = Wv + st * N3; p4 = Wl + st * N3; }
-        else { p8 = Bh + (st - 5) * N3; p4 = Bl + (st - 5) * N3; }
-    };
-    double reg[SLD];
-    {
-        const double *p8, *p4;
-        planes(0, p8, p4);
-        fetch(p8, p4, reg);
-        store(0, reg);
-    }
-    __syncthreads();
-    const double rVi = 1.0 / wsum8([&](int n) { return vs[ty + n][tx]; });
-    double WL[5], WR[5], BL[3] = {0.0, 0.0, 0.0}, BR[3] = {0.0, 0.0, 0.0};
-    // kept a loop: unrolled, the compiler hoists later stages' loads and
-    // the register count doubles
-#pragma unroll 1
-    for (int st = 0; st < 8; st++) {
-        if (st >= nst) break;
-        const int cur = st & 1;
-        if (st + 1 < nst) {
+        const int nst = m.doMHD ? 8 : 5;
+        const double *Wv = H + H_WH * N3, *Wl = H + H_WN * N3;
+        const double *Bh = H + H_B * N3, *Bl = m.doLFM ? S + S_B * N3 : Bh;
+        auto planes = [&](int st, const double *&p8, const double *&p4) {
+            if (st < 5) { p8 = Wv + st * N3; p4 = Wl + st * N3; }
+            else { p8 = Bh + (st - 5) * N3; p4 = Bl + (st - 5) * N3; }
+        };
+        double reg[SLD];
+        {
             const double *p8, *p4;
-            planes(st + 1, p8, p4);
+            planes(0, p8, p4);
             fetch(p8, p4, reg);
+            store(0, reg);
         }
-        double q[8], nq[4];
-#pragma unroll
-        for (int n = 0; n < 8; n++) q[n] = buf[cur][ty + n][tx];
-#pragma unroll
-        for (int n = 0; n < 4; n++) nq[n] = buf[cur][SR8 + ty + n][tx];
-        if (st < 5) {
-            recon_lr_w<RM>(m.pdmb, rVi, nq, q, WL[st], WR[st]);
-        } else {
-            // cell B is not volume-weighted in H: weight it here (LDS volumes)
-#pragma unroll
-            for (int n = 0; n < 8; n++) q[n] = vs[ty + n][tx] * q[n];
-            recon_lr_w<RM>(m.pdmb, rVi, nq, q, BL[st - 5], BR[st - 5]);
-        }
-        if (st + 1 < nst) store(cur ^ 1, reg);
         __syncthreads();
+        const double rVi = 1.0 / wsum8([&](int n) { return vs[ty + n][tx]; });
+        double WL[5], WR[5], BL[3] = {0.0, 0.0, 0.0}, BR[3] = {0.0, 0.0, 0.0};
+        // kept a loop: unrolled, the compiler hoists later stages' loads and
+        // the register count doubles
+#pragma unroll 1
+        for (int st = 0; st < 8; st++) {
+            if (st >= nst) break;
+            const int cur = st & 1;
+            if (st + 1 < nst) {
+                const double *p8, *p4;
+                planes(st + 1, p8, p4);
+                fetch(p8, p4, reg);
+            }
+            double q[8], nq[4];
+#pragma unroll
+            for (int n = 0; n < 8; n++) q[n] = buf[cur][ty + n][tx];
+#pragma unroll
+            for (int n = 0; n < 4; n++) nq[n] = buf[cur][SR8 + ty + n][tx];
+            if (st < 5) {
+                recon_lr_w<RM>(m.pdmb, rVi, nq, q, WL[st], WR[st]);
+            } else {
+                // cell B is not volume-weighted in H: weight it here (LDS volumes)
+#pragma unroll
+                for (int n = 0; n < 8; n++) q[n] = vs[ty + n][tx] * q[n];
+                recon_lr_w<RM>(m.pdmb, rVi, nq, q, BL[st - 5], BR[st - 5]);
+            }
+            if (st + 1 < nst) store(cur ^ 1, reg);
+            __syncthreads();
+        }
+        if (act) {
+            const long c = at(i, a);
+            const unsigned oc = (unsigned)(c * 8);
+            face_flux_lr<D, BG, BORIS>(
+                g, m, H, met, bg, [&](const double *p) { return ldp(p, oc); },
+                [&](const double *p) { return p[c - s]; },
+                [](double *, double *) {}, WL, WR, BL, BR, oc, gflx, mflx);
+        }
     }
-    if (act) {
-        const long c = at(i, a);
-        const unsigned oc = (unsigned)(c * 8);
-        face_flux_lr<D, BG, BORIS>(
-            g, m, H, met, bg, [&](const double *p) { return ldp(p, oc); },
-            [&](const double *p) { return p[c - s]; },
-            [](double *, double *) {}, WL, WR, BL, BR, oc, gflx, mflx);
-    }
-}
+};
 
 // ====================================================================
 // CT update (E2Flux, fields.F90:18-51) + bFlux2Fld of the new state.
@@ -768,227 +853,236 @@ __device__ __forceinline__ void block_min_dt(double v, Clock *clk)
 // Every thread reaches the block reduction.
 // ====================================================================
 template <bool BG, bool BORIS>
-__global__ __launch_bounds__(256) void k_update(Geo g, MParm m, Tile ft, Box ctb, EdgeBoxes eb,
-                                                Box ab, Box dtb, Clock *clk,
-                                                const double *__restrict__ So,
-                                                double *__restrict__ Sn,
-                                                const double *__restrict__ gflx,
-                                                const double *__restrict__ mflx,
-                                                const double *__restrict__ met,
-                                                const double *__restrict__ bg)
-{
-    int i, j, k;
-    const bool live = tile_point(ft, i, j, k);
-    double dtc = GAM_HUGE;
-    const long N3 = g.N3;
-    const long c = IX(g, i, j, k);
-    const unsigned oc = (unsigned)(c * 8);    // byte offset of the point
-    const bool cell = live && i <= g.ieg && j <= g.jeg && k <= g.keg;
-    double Bn[3] = {0.0, 0.0, 0.0};
-    if (live) {
-        const double dt = clk->dtA;
-        const double *E = Sn + S_E * N3;
-        const double *mfo = So + S_MF * N3;
-        double f[3];
+struct k_update {
+    __device__ __forceinline__ void operator()(const uint3 bi, Geo g, MParm m,
+                                               Tile ft, Box ctb, EdgeBoxes eb, Box ab, Box dtb,
+                                               Clock *clk, const double *__restrict__ So,
+                                               double *__restrict__ Sn,
+                                               const double *__restrict__ gflx,
+                                               const double *__restrict__ mflx,
+                                               const double *__restrict__ met,
+                                               const double *__restrict__ bg) const
+    {
+        int i, j, k;
+        const bool live = tile_point(ft, (int)bi.x, i, j, k);
+        double dtc = GAM_HUGE;
+        const long N3 = g.N3;
+        const long c = IX(g, i, j, k);
+        const unsigned oc = (unsigned)(c * 8);    // byte offset of the point
+        const bool cell = live && i <= g.ieg && j <= g.jeg && k <= g.keg;
+        double Bn[3] = {0.0, 0.0, 0.0};
+        if (live) {
+            const double dt = clk->dtA;
+            const double *E = Sn + S_E * N3;
+            const double *mfo = So + S_MF * N3;
+            double f[3];
 #pragma unroll
-        for (int d = 0; d < 3; d++) {
-            f[d] = m.doMHD ? ct_face(g, eb, ctb, E, mfo, dt, d, i, j, k, oc) : ldp(mfo + d * N3, oc);
-            stp(Sn + (S_MF + d) * N3, oc, f[d]);
+            for (int d = 0; d < 3; d++) {
+                f[d] = m.doMHD ? ct_face(g, eb, ctb, E, mfo, dt, d, i, j, k, oc) : ldp(mfo + d * N3, oc);
+                stp(Sn + (S_MF + d) * N3, oc, f[d]);
+            }
+            if (cell) {
+                double fIp, fJp, fKp;
+                if (m.doMHD) {
+                    fIp = ct_face(g, eb, ctb, E, mfo, dt, 0, i + 1, j, k, oc + 8);
+                    fJp = ct_face(g, eb, ctb, E, mfo, dt, 1, i, j + 1, k, oc + 8 * g.P);
+                    fKp = ct_face(g, eb, ctb, E, mfo, dt, 2, i, j, k + 1, oc + (unsigned)(8 * g.PP));
+                } else {
+                    fIp = mfo[0 * N3 + c + 1]; fJp = mfo[1 * N3 + c + g.P]; fKp = mfo[2 * N3 + c + g.PP];
+                }
+                cell_b(g, met, c, f[0], f[1], f[2], fIp, fJp, fKp, Bn);
+#pragma unroll
+                for (int x = 0; x < 3; x++) stp(Sn + (S_B + x) * N3, oc, Bn[x]);
+            }
         }
         if (cell) {
-            double fIp, fJp, fKp;
-            if (m.doMHD) {
-                fIp = ct_face(g, eb, ctb, E, mfo, dt, 0, i + 1, j, k, oc + 8);
-                fJp = ct_face(g, eb, ctb, E, mfo, dt, 1, i, j + 1, k, oc + 8 * g.P);
-                fKp = ct_face(g, eb, ctb, E, mfo, dt, 2, i, j, k + 1, oc + (unsigned)(8 * g.PP));
-            } else {
-                fIp = mfo[0 * N3 + c + 1]; fJp = mfo[1 * N3 + c + g.P]; fKp = mfo[2 * N3 + c + g.PP];
-            }
-            cell_b(g, met, c, f[0], f[1], f[2], fIp, fJp, fKp, Bn);
+            double U[5];
 #pragma unroll
-            for (int x = 0; x < 3; x++) stp(Sn + (S_B + x) * N3, oc, Bn[x]);
-        }
-    }
-    if (cell) {
-        double U[5];
+            for (int v = 0; v < 5; v++) U[v] = ldp(So + (S_GAS + v) * N3, oc);
+            if (in_box(ab, i, j, k)) {
+                const double idV = 1.0 / ldp(met + M_VOL * N3, oc);
+                const unsigned oI = oc + 8, oJ = oc + 8 * g.P, oK = oc + (unsigned)(8 * g.PP);
+                const double dtM = clk->dt, dt = clk->dtA;
+                double dH[5], oU[5];
 #pragma unroll
-        for (int v = 0; v < 5; v++) U[v] = ldp(So + (S_GAS + v) * N3, oc);
-        if (in_box(ab, i, j, k)) {
-            const double idV = 1.0 / ldp(met + M_VOL * N3, oc);
-            const unsigned oI = oc + 8, oJ = oc + 8 * g.P, oK = oc + (unsigned)(8 * g.PP);
-            const double dtM = clk->dt, dt = clk->dtA;
-            double dH[5], oU[5];
-#pragma unroll
-            for (int v = 0; v < 5; v++) {
-                const double *gv = gflx + (long)v * N3;
-                dH[v] = (ldp(gv, oc) - ldp(gv, oI) + ldp(gv + 5 * N3, oc) - ldp(gv + 5 * N3, oJ)
-                         + ldp(gv + 10 * N3, oc) - ldp(gv + 10 * N3, oK)) * idV;
-                oU[v] = U[v];
-                U[v] = U[v] + dtM * dH[v];
-            }
-            if (m.doMHD) {
-                double dM[3], B[3], oB[3];
-#pragma unroll
-                for (int x = 0; x < 3; x++) {
-                    const double *mv = mflx + (long)x * N3;
-                    double r = (ldp(mv, oc) - ldp(mv, oI) + ldp(mv + 3 * N3, oc) - ldp(mv + 3 * N3, oJ)
-                                + ldp(mv + 6 * N3, oc) - ldp(mv + 6 * N3, oK)) * idV;
-                    if (BG) r = r + ldp(bg + (B_DPB0 + x) * N3, oc);
-                    dM[x] = r;
-                    B[x] = Bn[x];
-                    oB[x] = ldp(So + (S_B + x) * N3, oc);
-                    if (BG) {
-                        const double b0 = ldp(bg + (B_B0 + x) * N3, oc);
-                        B[x] = B[x] + b0;
-                        oB[x] = oB[x] + b0;
-                    }
+                for (int v = 0; v < 5; v++) {
+                    const double *gv = gflx + (long)v * N3;
+                    dH[v] = (ldp(gv, oc) - ldp(gv, oI) + ldp(gv + 5 * N3, oc) - ldp(gv + 5 * N3, oJ)
+                             + ldp(gv + 10 * N3, oc) - ldp(gv + 10 * N3, oK)) * idV;
+                    oU[v] = U[v];
+                    U[v] = U[v] + dtM * dH[v];
                 }
-                if (BORIS) {
-                    // CellBoris (mhdgroup.F90:336-441), single fluid
-                    double W2[5];
-                    c2p_cell(m, U, W2);
-                    const double D2 = dmax(W2[DEN], m.dFloor);
-                    const double P2 = dmax(W2[PRS], m.pFloor);
-                    const double D0 = dmax(oU[DEN], m.dFloor);
-                    const double iD0 = 1.0 / D0, iD2 = 1.0 / D2;
-                    double V0[3] = {oU[MOMX] * iD0, oU[MOMY] * iD0, oU[MOMZ] * iD0};
-                    double B1[3] = {0.5 * (B[0] + oB[0]), 0.5 * (B[1] + oB[1]), 0.5 * (B[2] + oB[2])};
-                    const double alpha = (B1[0] * B1[0] + B1[1] * B1[1] + B1[2] * B1[2]) * (iD2 * m.iCa2);
-                    const double dRho = dt * dH[DEN];
-                    double dPG[3] = {dt * dH[MOMX], dt * dH[MOMY], dt * dH[MOMZ]};
-                    double dPM[3] = {dt * dM[0], dt * dM[1], dt * dM[2]};
-                    const double pCon = 1.0 / (1.0 + alpha);
-                    double dMom[3];
-                    if (alpha > GAM_TINY) {
-                        const double nb = sqrt(B1[0] * B1[0] + B1[1] * B1[1] + B1[2] * B1[2]);
-                        double bh[3];
-                        if (nb <= GAM_TINY) { bh[0] = bh[1] = bh[2] = 0.0; }
-                        else { const double inb = 1.0 / nb; bh[0] = B1[0] * inb; bh[1] = B1[1] * inb; bh[2] = B1[2] * inb; }
-                        double sGM[3] = {dPG[0] + dPM[0], dPG[1] + dPM[1], dPG[2] + dPM[2]};
-                        const double pG = dPG[0] * bh[0] + dPG[1] * bh[1] + dPG[2] * bh[2];
-                        const double pS = sGM[0] * bh[0] + sGM[1] * bh[1] + sGM[2] * bh[2];
-                        const double pV = V0[0] * bh[0] + V0[1] * bh[1] + V0[2] * bh[2];
+                if (m.doMHD) {
+                    double dM[3], B[3], oB[3];
 #pragma unroll
-                        for (int x = 0; x < 3; x++)
-                            dMom[x] = pG * bh[x] + pCon * (sGM[x] - pS * bh[x]) + alpha * pCon * dRho * (V0[x] - pV * bh[x]);
+                    for (int x = 0; x < 3; x++) {
+                        const double *mv = mflx + (long)x * N3;
+                        double r = (ldp(mv, oc) - ldp(mv, oI) + ldp(mv + 3 * N3, oc) - ldp(mv + 3 * N3, oJ)
+                                    + ldp(mv + 6 * N3, oc) - ldp(mv + 6 * N3, oK)) * idV;
+                        if (BG) r = r + ldp(bg + (B_DPB0 + x) * N3, oc);
+                        dM[x] = r;
+                        B[x] = Bn[x];
+                        oB[x] = ldp(So + (S_B + x) * N3, oc);
+                        if (BG) {
+                            const double b0 = ldp(bg + (B_B0 + x) * N3, oc);
+                            B[x] = B[x] + b0;
+                            oB[x] = oB[x] + b0;
+                        }
+                    }
+                    if (BORIS) {
+                        // CellBoris (mhdgroup.F90:336-441), single fluid
+                        double W2[5];
+                        c2p_cell(m, U, W2);
+                        const double D2 = dmax(W2[DEN], m.dFloor);
+                        const double P2 = dmax(W2[PRS], m.pFloor);
+                        const double D0 = dmax(oU[DEN], m.dFloor);
+                        const double iD0 = 1.0 / D0, iD2 = 1.0 / D2;
+                        double V0[3] = {oU[MOMX] * iD0, oU[MOMY] * iD0, oU[MOMZ] * iD0};
+                        double B1[3] = {0.5 * (B[0] + oB[0]), 0.5 * (B[1] + oB[1]), 0.5 * (B[2] + oB[2])};
+                        const double alpha = (B1[0] * B1[0] + B1[1] * B1[1] + B1[2] * B1[2]) * (iD2 * m.iCa2);
+                        const double dRho = dt * dH[DEN];
+                        double dPG[3] = {dt * dH[MOMX], dt * dH[MOMY], dt * dH[MOMZ]};
+                        double dPM[3] = {dt * dM[0], dt * dM[1], dt * dM[2]};
+                        const double pCon = 1.0 / (1.0 + alpha);
+                        double dMom[3];
+                        if (alpha > GAM_TINY) {
+                            const double nb = sqrt(B1[0] * B1[0] + B1[1] * B1[1] + B1[2] * B1[2]);
+                            double bh[3];
+                            if (nb <= GAM_TINY) { bh[0] = bh[1] = bh[2] = 0.0; }
+                            else { const double inb = 1.0 / nb; bh[0] = B1[0] * inb; bh[1] = B1[1] * inb; bh[2] = B1[2] * inb; }
+                            double sGM[3] = {dPG[0] + dPM[0], dPG[1] + dPM[1], dPG[2] + dPM[2]};
+                            const double pG = dPG[0] * bh[0] + dPG[1] * bh[1] + dPG[2] * bh[2];
+                            const double pS = sGM[0] * bh[0] + sGM[1] * bh[1] + sGM[2] * bh[2];
+                            const double pV = V0[0] * bh[0] + V0[1] * bh[1] + V0[2] * bh[2];
+#pragma unroll
+                            for (int x = 0; x < 3; x++)
+                                dMom[x] = pG * bh[x] + pCon * (sGM[x] - pS * bh[x]) + alpha * pCon * dRho * (V0[x] - pV * bh[x]);
+                        } else {
+#pragma unroll
+                            for (int x = 0; x < 3; x++) dMom[x] = dPG[x] + dPM[x];
+                        }
+                        double Wn[5];
+                        Wn[DEN] = D2;
+#pragma unroll
+                        for (int x = 0; x < 3; x++) Wn[1 + x] = (D0 * V0[x] + dMom[x]) * iD2;
+                        Wn[PRS] = P2;
+                        p2c_cell(m, Wn, U);
                     } else {
-#pragma unroll
-                        for (int x = 0; x < 3; x++) dMom[x] = dPG[x] + dPM[x];
+                        // CellMaxwell (mhdgroup.F90:299-329)
+                        const double d = dmax(U[DEN], m.dFloor);
+                        double mx = U[MOMX], my = U[MOMY], mz = U[MOMZ];
+                        const double id = 1.0 / d;
+                        double ke = 0.5 * (mx * mx + my * my + mz * mz) * id;
+                        const double p = dmax((m.gamma - 1) * (U[ENER] - ke), m.pFloor);
+                        mx = mx + dt * dM[0];
+                        my = my + dt * dM[1];
+                        mz = mz + dt * dM[2];
+                        ke = 0.5 * (mx * mx + my * my + mz * mz) * id;
+                        U[DEN] = d; U[MOMX] = mx; U[MOMY] = my; U[MOMZ] = mz;
+                        U[ENER] = ke + p * m.igm1;
                     }
-                    double Wn[5];
-                    Wn[DEN] = D2;
-#pragma unroll
-                    for (int x = 0; x < 3; x++) Wn[1 + x] = (D0 * V0[x] + dMom[x]) * iD2;
-                    Wn[PRS] = P2;
-                    p2c_cell(m, Wn, U);
-                } else {
-                    // CellMaxwell (mhdgroup.F90:299-329)
-                    const double d = dmax(U[DEN], m.dFloor);
-                    double mx = U[MOMX], my = U[MOMY], mz = U[MOMZ];
-                    const double id = 1.0 / d;
-                    double ke = 0.5 * (mx * mx + my * my + mz * mz) * id;
-                    const double p = dmax((m.gamma - 1) * (U[ENER] - ke), m.pFloor);
-                    mx = mx + dt * dM[0];
-                    my = my + dt * dM[1];
-                    mz = mz + dt * dM[2];
-                    ke = 0.5 * (mx * mx + my * my + mz * mz) * id;
-                    U[DEN] = d; U[MOMX] = mx; U[MOMY] = my; U[MOMZ] = mz;
-                    U[ENER] = ke + p * m.igm1;
                 }
             }
-        }
 #pragma unroll
-        for (int v = 0; v < 5; v++) stp(Sn + (S_GAS + v) * N3, oc, U[v]);
-        if (in_box(dtb, i, j, k)) {
-            double bx = Bn[0], by = Bn[1], bz = Bn[2];
-            if (BG) {
-                bx = bx + ldp(bg + (B_B0 + 0) * N3, oc);
-                by = by + ldp(bg + (B_B0 + 1) * N3, oc);
-                bz = bz + ldp(bg + (B_B0 + 2) * N3, oc);
+            for (int v = 0; v < 5; v++) stp(Sn + (S_GAS + v) * N3, oc, U[v]);
+            if (in_box(dtb, i, j, k)) {
+                double bx = Bn[0], by = Bn[1], bz = Bn[2];
+                if (BG) {
+                    bx = bx + ldp(bg + (B_B0 + 0) * N3, oc);
+                    by = by + ldp(bg + (B_B0 + 1) * N3, oc);
+                    bz = bz + ldp(bg + (B_B0 + 2) * N3, oc);
+                }
+                const double dl = dmin(dmin(ldp(met + M_DI * N3, oc), ldp(met + M_DJ * N3, oc)), ldp(met + M_DK * N3, oc));
+                dtc = cell_dt<BORIS>(m, U, bx, by, bz, dl);
+                if (!(dtc == dtc)) dtc = GAM_HUGE;
             }
-            const double dl = dmin(dmin(ldp(met + M_DI * N3, oc), ldp(met + M_DJ * N3, oc)), ldp(met + M_DK * N3, oc));
-            dtc = cell_dt<BORIS>(m, U, bx, by, bz, dl);
-            if (!(dtc == dtc)) dtc = GAM_HUGE;
         }
+        block_min_dt(dtc, clk);
     }
-    block_min_dt(dtc, clk);
-}
+};
 
 // CalcDT over the DT box on a state (standalone CalcDT, step.F90:58-101)
 template <bool BG, bool BORIS>
-__global__ __launch_bounds__(256) void k_calc_dt(Geo g, MParm m, Tile dtb, Clock *clk,
-                                                 const double *__restrict__ S,
-                                                 const double *__restrict__ met,
-                                                 const double *__restrict__ bg)
-{
-    int i, j, k;
-    const bool live = tile_point(dtb, i, j, k);
-    double dtc = GAM_HUGE;
-    if (live) {
-        const long N3 = g.N3, c = IX(g, i, j, k);
-        double U[5];
-        for (int v = 0; v < 5; v++) U[v] = S[(S_GAS + v) * N3 + c];
-        double bx = S[(S_B + 0) * N3 + c], by = S[(S_B + 1) * N3 + c], bz = S[(S_B + 2) * N3 + c];
-        if (BG) { bx = bx + bg[(B_B0 + 0) * N3 + c]; by = by + bg[(B_B0 + 1) * N3 + c]; bz = bz + bg[(B_B0 + 2) * N3 + c]; }
-        const double dl = dmin(dmin(met[M_DI * N3 + c], met[M_DJ * N3 + c]), met[M_DK * N3 + c]);
-        dtc = cell_dt<BORIS>(m, U, bx, by, bz, dl);
-        if (!(dtc == dtc)) dtc = GAM_HUGE;
+struct k_calc_dt {
+    __device__ __forceinline__ void operator()(const uint3 bi, Geo g, MParm m,
+                                               Tile dtb, Clock *clk, const double *__restrict__ S,
+                                               const double *__restrict__ met,
+                                               const double *__restrict__ bg) const
+    {
+        int i, j, k;
+        const bool live = tile_point(dtb, (int)bi.x, i, j, k);
+        double dtc = GAM_HUGE;
+        if (live) {
+            const long N3 = g.N3, c = IX(g, i, j, k);
+            double U[5];
+            for (int v = 0; v < 5; v++) U[v] = S[(S_GAS + v) * N3 + c];
+            double bx = S[(S_B + 0) * N3 + c], by = S[(S_B + 1) * N3 + c], bz = S[(S_B + 2) * N3 + c];
+            if (BG) { bx = bx + bg[(B_B0 + 0) * N3 + c]; by = by + bg[(B_B0 + 1) * N3 + c]; bz = bz + bg[(B_B0 + 2) * N3 + c]; }
+            const double dl = dmin(dmin(met[M_DI * N3 + c], met[M_DJ * N3 + c]), met[M_DK * N3 + c]);
+            dtc = cell_dt<BORIS>(m, U, bx, by, bz, dl);
+            if (!(dtc == dtc)) dtc = GAM_HUGE;
+        }
+        block_min_dt(dtc, clk);
     }
-    block_min_dt(dtc, clk);
-}
+};
 
 // ====================================================================
 // Time bookkeeping and CalcDT checks (step.F90:103-166); one thread.
 // ====================================================================
-__global__ void k_finalize(MParm m, Clock *clk, int mode)
-{
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    Clock c = *clk;
-    if (mode & FIN_ADVANCE) {          // applyUpdate: oState%time=State%time; State%time+=dt
-        c.ost = c.st;
-        c.st = c.st + c.dtA;
-    }
-    if (mode & FIN_MODEL) {            // UpdateStateData (gamapp.F90:108-110)
-        c.ts = c.ts + 1;
-        c.t = c.t + c.dt;
-    }
-    if (mode & FIN_CALCDT) {
-        const double dtMin = __longlong_as_double((long long)c.dtBits);
-        int st = 0;
-        if (dtMin == GAM_HUGE) st = 2;
-        double dt = dtMin;
-        if (c.ts > 0) {
-            if (c.dt / dtMin >= 10) st = st > 1 ? st : 1;
-            if ((m.dt0 > GAM_TINY) && (dtMin / m.dt0 <= m.limDT0)) st = st > 1 ? st : 1;
-            if (dtMin <= GAM_TINY) st = 2;
+struct k_finalize {
+    __device__ __forceinline__ void operator()(const uint3 bi, MParm m, Clock *clk, int mode) const
+    {
+        if (threadIdx.x != 0 || bi.x != 0) return;
+        Clock c = *clk;
+        if (mode & FIN_ADVANCE) {          // applyUpdate: oState%time=State%time; State%time+=dt
+            c.ost = c.st;
+            c.st = c.st + c.dtA;
         }
-        if ((c.t + dt) > m.tFin) dt = dmax(m.tFin - c.t, GAM_TINY);
-        if (m.dt0 > GAM_TINY && dt > (10.0 * m.dt0)) dt = 10.0 * m.dt0;
-        c.dtRet = dt;
-        c.lastStatus = st;
-        c.status = c.status > st ? c.status : st;
-        if (mode & FIN_SETDT) { c.dt = dt; c.dtA = dt; }
-        if (mode & FIN_INIT) c.ost = c.st - c.dt;
+        if (mode & FIN_MODEL) {            // UpdateStateData (gamapp.F90:108-110)
+            c.ts = c.ts + 1;
+            c.t = c.t + c.dt;
+        }
+        if (mode & FIN_CALCDT) {
+            const double dtMin = __longlong_as_double((long long)c.dtBits);
+            int st = 0;
+            if (dtMin == GAM_HUGE) st = 2;
+            double dt = dtMin;
+            if (c.ts > 0) {
+                if (c.dt / dtMin >= 10) st = st > 1 ? st : 1;
+                if ((m.dt0 > GAM_TINY) && (dtMin / m.dt0 <= m.limDT0)) st = st > 1 ? st : 1;
+                if (dtMin <= GAM_TINY) st = 2;
+            }
+            if ((c.t + dt) > m.tFin) dt = dmax(m.tFin - c.t, GAM_TINY);
+            if (m.dt0 > GAM_TINY && dt > (10.0 * m.dt0)) dt = 10.0 * m.dt0;
+            c.dtRet = dt;
+            c.lastStatus = st;
+            c.status = c.status > st ? c.status : st;
+            if (mode & FIN_SETDT) { c.dt = dt; c.dtA = dt; }
+            if (mode & FIN_INIT) c.ost = c.st - c.dt;
+        }
+        c.dtBits = (unsigned long long)__double_as_longlong(GAM_HUGE);
+        *clk = c;
     }
-    c.dtBits = (unsigned long long)__double_as_longlong(GAM_HUGE);
-    *clk = c;
-}
+};
 
 // ====================================================================
 // bFlux2Fld (gridutils.F90:281-301) over all cells
 // ====================================================================
-__global__ __launch_bounds__(256) void k_bflux2fld(Geo g, Tile cb, Box skip, const double *__restrict__ mf,
-                                                   double *__restrict__ B,
-                                                   const double *__restrict__ met)
-{
-    TILE_IJK(cb);
-    if (in_box(skip, i, j, k)) return;
-    const long N3 = g.N3, c = IX(g, i, j, k);
-    double b[3];
-    cell_b(g, met, c, mf[c], mf[N3 + c], mf[2 * N3 + c], mf[c + 1], mf[N3 + c + g.P],
-           mf[2 * N3 + c + g.PP], b);
-    for (int x = 0; x < 3; x++) B[x * N3 + c] = b[x];
-}
+struct k_bflux2fld {
+    __device__ __forceinline__ void operator()(const uint3 bi, Geo g,
+                                               Tile cb, Box skip, const double *__restrict__ mf,
+                                               double *__restrict__ B,
+                                               const double *__restrict__ met) const
+    {
+        TILE_IJK(cb);
+        if (in_box(skip, i, j, k)) return;
+        const long N3 = g.N3, c = IX(g, i, j, k);
+        double b[3];
+        cell_b(g, met, c, mf[c], mf[N3 + c], mf[2 * N3 + c], mf[c + 1], mf[N3 + c + g.P],
+               mf[2 * N3 + c + g.PP], b);
+        for (int x = 0; x < 3; x++) B[x * N3 + c] = b[x];
+    }
+};
 
 // bFlux2Fld over the ghost shell only: up to six boxes (the i, j and k
 // ghost slabs, disjoint), each cut into 64 x 4 (i, j) tiles; workgroup w
@@ -1000,27 +1094,30 @@ struct GhostBoxes {
     int n;
 };
 
-__global__ __launch_bounds__(256) void k_bflux2fld_ghosts(Geo g, GhostBoxes gb, const double *__restrict__ mf,
-                                                          double *__restrict__ B,
-                                                          const double *__restrict__ met)
-{
-    const int w = blockIdx.x;
-    int q = 0;
-    while (q + 1 < gb.n && w >= gb.start[q + 1]) q++;
-    const int L = w - gb.start[q];
-    const Box &b = gb.b[q];
-    const int t0 = L % gb.n0[q];
-    const int r = L / gb.n0[q];
-    const int i = b.i0 + t0 * BX + threadIdx.x;
-    const int j = b.j0 + (r % gb.n1[q]) * BY + threadIdx.y;
-    const int k = b.k0 + r / gb.n1[q];
-    if (i > b.i1 || j > b.j1 || k > b.k1) return;
-    const long N3 = g.N3, c = IX(g, i, j, k);
-    double bb[3];
-    cell_b(g, met, c, mf[c], mf[N3 + c], mf[2 * N3 + c], mf[c + 1], mf[N3 + c + g.P],
-           mf[2 * N3 + c + g.PP], bb);
-    for (int x = 0; x < 3; x++) B[x * N3 + c] = bb[x];
-}
+struct k_bflux2fld_ghosts {
+    __device__ __forceinline__ void operator()(const uint3 bi, Geo g,
+                                               GhostBoxes gb, const double *__restrict__ mf,
+                                               double *__restrict__ B,
+                                               const double *__restrict__ met) const
+    {
+        const int w = (int)bi.x;
+        int q = 0;
+        while (q + 1 < gb.n && w >= gb.start[q + 1]) q++;
+        const int L = w - gb.start[q];
+        const Box &b = gb.b[q];
+        const int t0 = L % gb.n0[q];
+        const int r = L / gb.n0[q];
+        const int i = b.i0 + t0 * BX + threadIdx.x;
+        const int j = b.j0 + (r % gb.n1[q]) * BY + threadIdx.y;
+        const int k = b.k0 + r / gb.n1[q];
+        if (i > b.i1 || j > b.j1 || k > b.k1) return;
+        const long N3 = g.N3, c = IX(g, i, j, k);
+        double bb[3];
+        cell_b(g, met, c, mf[c], mf[N3 + c], mf[2 * N3 + c], mf[c + 1], mf[N3 + c + g.P],
+               mf[2 * N3 + c + g.PP], bb);
+        for (int x = 0; x < 3; x++) B[x * N3 + c] = bb[x];
+    }
+};
 
 // ====================================================================
 // Boundary conditions.  One thread per transverse ghost column; the n loop
@@ -1041,141 +1138,148 @@ __device__ __forceinline__ bool is_cell(const Geo &g, int i, int j, int k)
 }
 
 // Ghost layers n = 1..4: the i-side kernels (one thread per (j, k) row of
-// 4-wide ghost slabs, uncoalesced along i) take n from blockIdx.z, one
+// 4-wide ghost slabs, uncoalesced along i) take n from the block's z index, one
 // layer per thread, for four times the parallelism; the j- and k-side
 // kernels (threads along i, coalesced) loop over n.
-#define NL0(side) (((side) <= 1) ? 1 + (int)blockIdx.z : 1)
-#define NL1(side) (((side) <= 1) ? 1 + (int)blockIdx.z : 4)
+#define NL0(side) (((side) <= 1) ? 1 + (int)bi.z : 1)
+#define NL1(side) (((side) <= 1) ? 1 + (int)bi.z : 4)
 
 // periodic (bcs.F90:279-450).  side: 0..5.  (a,b) = transverse indices
 template <int SIDE>
-__global__ void k_bc_periodic(Geo g, Box t, double *__restrict__ S)
-{
-    const int a = t.i0 + blockIdx.x * blockDim.x + threadIdx.x;
-    const int b = t.j0 + blockIdx.y;
-    if (a > t.i1 || b > t.j1) return;
-    const long N3 = g.N3;
-    double *mf = S + S_MF * N3;
-    for (int n = NL0(SIDE); n <= NL1(SIDE); n++) {
-        if (SIDE == 0) {        // inner I: a=j, b=k
-            const int ig = g.is - n, sr = g.ie - n + 1;
-            const long d = IX(g, ig, a, b), s = IX(g, sr, a, b);
-            if (is_cell(g, ig, a, b)) { copy_cell(S, N3, d, s); copy_b(S, N3, d, s); }
-            for (int x = 0; x < 3; x++) mf[x * N3 + d] = mf[x * N3 + s];
-        } else if (SIDE == 1) { // outer I
-            const int ig = g.ie + n, sr = g.is + n - 1;
-            const long d = IX(g, ig, a, b), s = IX(g, sr, a, b);
-            if (is_cell(g, ig, a, b)) { copy_cell(S, N3, d, s); copy_b(S, N3, d, s); }
-            mf[1 * N3 + d] = mf[1 * N3 + s];
-            mf[2 * N3 + d] = mf[2 * N3 + s];
-            mf[IX(g, ig + 1, a, b)] = mf[IX(g, g.is + n, a, b)];
-        } else if (SIDE == 2) { // inner J: a=i, b=k
-            const int jg = g.js - n, sr = g.je - n + 1;
-            const long d = IX(g, a, jg, b), s = IX(g, a, sr, b);
-            if (is_cell(g, a, jg, b)) { copy_cell(S, N3, d, s); copy_b(S, N3, d, s); }
-            for (int x = 0; x < 3; x++) mf[x * N3 + d] = mf[x * N3 + s];
-        } else if (SIDE == 3) { // outer J
-            const int jg = g.je + n, sr = g.js + n - 1;
-            const long d = IX(g, a, jg, b), s = IX(g, a, sr, b);
-            if (is_cell(g, a, jg, b)) { copy_cell(S, N3, d, s); copy_b(S, N3, d, s); }
-            mf[0 * N3 + d] = mf[0 * N3 + s];
-            mf[2 * N3 + d] = mf[2 * N3 + s];
-            mf[1 * N3 + IX(g, a, jg + 1, b)] = mf[1 * N3 + IX(g, a, g.js + n, b)];
-        } else if (SIDE == 4) { // inner K: a=i, b=j
-            const int kg = g.ks - n, sr = g.ke - n + 1;
-            const long d = IX(g, a, b, kg), s = IX(g, a, b, sr);
-            if (is_cell(g, a, b, kg)) { copy_cell(S, N3, d, s); copy_b(S, N3, d, s); }
-            for (int x = 0; x < 3; x++) mf[x * N3 + d] = mf[x * N3 + s];
-        } else {                // outer K
-            const int kg = g.ke + n, sr = g.ks + n - 1;
-            const long d = IX(g, a, b, kg), s = IX(g, a, b, sr);
-            if (is_cell(g, a, b, kg)) { copy_cell(S, N3, d, s); copy_b(S, N3, d, s); }
-            mf[0 * N3 + d] = mf[0 * N3 + s];
-            mf[1 * N3 + d] = mf[1 * N3 + s];
-            mf[2 * N3 + IX(g, a, b, kg + 1)] = mf[2 * N3 + IX(g, a, b, g.ks + n)];
+struct k_bc_periodic {
+    __device__ __forceinline__ void operator()(const uint3 bi, Geo g, Box t,
+                                               double *__restrict__ S) const
+    {
+        const int a = t.i0 + bi.x * blockDim.x + threadIdx.x;
+        const int b = t.j0 + bi.y;
+        if (a > t.i1 || b > t.j1) return;
+        const long N3 = g.N3;
+        double *mf = S + S_MF * N3;
+        for (int n = NL0(SIDE); n <= NL1(SIDE); n++) {
+            if (SIDE == 0) {        // inner I: a=j, b=k
+                const int ig = g.is - n, sr = g.ie - n + 1;
+                const long d = IX(g, ig, a, b), s = IX(g, sr, a, b);
+                if (is_cell(g, ig, a, b)) { copy_cell(S, N3, d, s); copy_b(S, N3, d, s); }
+                for (int x = 0; x < 3; x++) mf[x * N3 + d] = mf[x * N3 + s];
+            } else if (SIDE == 1) { // outer I
+                const int ig = g.ie + n, sr = g.is + n - 1;
+                const long d = IX(g, ig, a, b), s = IX(g, sr, a, b);
+                if (is_cell(g, ig, a, b)) { copy_cell(S, N3, d, s); copy_b(S, N3, d, s); }
+                mf[1 * N3 + d] = mf[1 * N3 + s];
+                mf[2 * N3 + d] = mf[2 * N3 + s];
+                mf[IX(g, ig + 1, a, b)] = mf[IX(g, g.is + n, a, b)];
+            } else if (SIDE == 2) { // inner J: a=i, b=k
+                const int jg = g.js - n, sr = g.je - n + 1;
+                const long d = IX(g, a, jg, b), s = IX(g, a, sr, b);
+                if (is_cell(g, a, jg, b)) { copy_cell(S, N3, d, s); copy_b(S, N3, d, s); }
+                for (int x = 0; x < 3; x++) mf[x * N3 + d] = mf[x * N3 + s];
+            } else if (SIDE == 3) { // outer J
+                const int jg = g.je + n, sr = g.js + n - 1;
+                const long d = IX(g, a, jg, b), s = IX(g, a, sr, b);
+                if (is_cell(g, a, jg, b)) { copy_cell(S, N3, d, s); copy_b(S, N3, d, s); }
+                mf[0 * N3 + d] = mf[0 * N3 + s];
+                mf[2 * N3 + d] = mf[2 * N3 + s];
+                mf[1 * N3 + IX(g, a, jg + 1, b)] = mf[1 * N3 + IX(g, a, g.js + n, b)];
+            } else if (SIDE == 4) { // inner K: a=i, b=j
+                const int kg = g.ks - n, sr = g.ke - n + 1;
+                const long d = IX(g, a, b, kg), s = IX(g, a, b, sr);
+                if (is_cell(g, a, b, kg)) { copy_cell(S, N3, d, s); copy_b(S, N3, d, s); }
+                for (int x = 0; x < 3; x++) mf[x * N3 + d] = mf[x * N3 + s];
+            } else {                // outer K
+                const int kg = g.ke + n, sr = g.ks + n - 1;
+                const long d = IX(g, a, b, kg), s = IX(g, a, b, sr);
+                if (is_cell(g, a, b, kg)) { copy_cell(S, N3, d, s); copy_b(S, N3, d, s); }
+                mf[0 * N3 + d] = mf[0 * N3 + s];
+                mf[1 * N3 + d] = mf[1 * N3 + s];
+                mf[2 * N3 + IX(g, a, b, kg + 1)] = mf[2 * N3 + IX(g, a, b, g.ks + n)];
+            }
         }
     }
-}
+};
 
 // zero-gradient (bcs.F90:799-930), sphere in/out (934-982)
 template <int SIDE, int KIND>
-__global__ void k_bc_copy(Geo g, Box t, double *__restrict__ S, const double *__restrict__ met)
-{
-    const int a = t.i0 + blockIdx.x * blockDim.x + threadIdx.x;
-    const int b = t.j0 + blockIdx.y;
-    if (a > t.i1 || b > t.j1) return;
-    const long N3 = g.N3;
-    double *mf = S + S_MF * N3;
-    if (SIDE == 0) {            // inner I: a=j, b=k
-        const long s = IX(g, g.is, a, b);
-        for (int n = NL0(SIDE); n <= NL1(SIDE); n++) {
-            const long d = IX(g, g.is - n, a, b);
-            copy_cell(S, N3, d, s);
-            for (int x = 0; x < 3; x++) mf[x * N3 + d] = mf[x * N3 + s];
-            if (KIND == 2) copy_b(S, N3, d, s);      // SphIn also copies Bxyz
-        }
-    } else if (SIDE == 1) {     // outer I
-        const long s = IX(g, g.ie, a, b);
-        if (KIND == 3) {        // SphOut
+struct k_bc_copy {
+    __device__ __forceinline__ void operator()(const uint3 bi, Geo g, Box t,
+                                               double *__restrict__ S,
+                                               const double *__restrict__ met) const
+    {
+        const int a = t.i0 + bi.x * blockDim.x + threadIdx.x;
+        const int b = t.j0 + bi.y;
+        if (a > t.i1 || b > t.j1) return;
+        const long N3 = g.N3;
+        double *mf = S + S_MF * N3;
+        if (SIDE == 0) {            // inner I: a=j, b=k
+            const long s = IX(g, g.is, a, b);
             for (int n = NL0(SIDE); n <= NL1(SIDE); n++) {
-                const int ig = g.ie + n;
-                const long d = IX(g, ig, a, b);
+                const long d = IX(g, g.is - n, a, b);
                 copy_cell(S, N3, d, s);
-                mf[IX(g, ig + 1, a, b)] = mf[s];
-                mf[N3 + d] = mf[N3 + s];
-                mf[2 * N3 + d] = mf[2 * N3 + s];
-                copy_b(S, N3, d, s);
+                for (int x = 0; x < 3; x++) mf[x * N3 + d] = mf[x * N3 + s];
+                if (KIND == 2) copy_b(S, N3, d, s);      // SphIn also copies Bxyz
             }
-        } else {                // zeroGrad_obcI: fluxes of the last physical cell's B
-            double B[3];
-            cell_b(g, met, s, mf[s], mf[N3 + s], mf[2 * N3 + s], mf[s + 1], mf[N3 + s + g.P],
-                   mf[2 * N3 + s + g.PP], B);
-            for (int n = NL0(SIDE); n <= NL1(SIDE); n++) {
-                const int ig = g.ie + n;
-                const long d = IX(g, ig, a, b);
-                copy_cell(S, N3, d, s);
-                const long fi[3] = {IX(g, ig + 1, a, b), d, d};
-                for (int x = 0; x < 3; x++) {
-                    const long f = fi[x];
-                    const double *nrm = met + (long)(M_TF + 9 * x) * N3 + f;
-                    mf[x * N3 + f] = met[(M_FACE + x) * N3 + f] * (B[0] * nrm[0] + B[1] * nrm[N3] + B[2] * nrm[2 * N3]);
+        } else if (SIDE == 1) {     // outer I
+            const long s = IX(g, g.ie, a, b);
+            if (KIND == 3) {        // SphOut
+                for (int n = NL0(SIDE); n <= NL1(SIDE); n++) {
+                    const int ig = g.ie + n;
+                    const long d = IX(g, ig, a, b);
+                    copy_cell(S, N3, d, s);
+                    mf[IX(g, ig + 1, a, b)] = mf[s];
+                    mf[N3 + d] = mf[N3 + s];
+                    mf[2 * N3 + d] = mf[2 * N3 + s];
+                    copy_b(S, N3, d, s);
+                }
+            } else {                // zeroGrad_obcI: fluxes of the last physical cell's B
+                double B[3];
+                cell_b(g, met, s, mf[s], mf[N3 + s], mf[2 * N3 + s], mf[s + 1], mf[N3 + s + g.P],
+                       mf[2 * N3 + s + g.PP], B);
+                for (int n = NL0(SIDE); n <= NL1(SIDE); n++) {
+                    const int ig = g.ie + n;
+                    const long d = IX(g, ig, a, b);
+                    copy_cell(S, N3, d, s);
+                    const long fi[3] = {IX(g, ig + 1, a, b), d, d};
+                    for (int x = 0; x < 3; x++) {
+                        const long f = fi[x];
+                        const double *nrm = met + (long)(M_TF + 9 * x) * N3 + f;
+                        mf[x * N3 + f] = met[(M_FACE + x) * N3 + f] * (B[0] * nrm[0] + B[1] * nrm[N3] + B[2] * nrm[2 * N3]);
+                    }
                 }
             }
-        }
-    } else if (SIDE == 2) {     // inner J: a=i, b=k
-        const long s = IX(g, a, g.js, b);
-        for (int n = NL0(SIDE); n <= NL1(SIDE); n++) {
-            const long d = IX(g, a, g.js - n, b);
-            copy_cell(S, N3, d, s);
-            for (int x = 0; x < 3; x++) mf[x * N3 + d] = mf[x * N3 + s];
-        }
-    } else if (SIDE == 3) {     // outer J
-        const long s = IX(g, a, g.je, b);
-        for (int n = NL0(SIDE); n <= NL1(SIDE); n++) {
-            const long d = IX(g, a, g.je + n, b);
-            copy_cell(S, N3, d, s);
-            mf[d] = mf[s];
-            mf[2 * N3 + d] = mf[2 * N3 + s];
-            mf[N3 + IX(g, a, g.je + n + 1, b)] = mf[N3 + IX(g, a, g.je + 1, b)];
-        }
-    } else if (SIDE == 4) {     // inner K: a=i, b=j
-        const long s = IX(g, a, b, g.ks);
-        for (int n = NL0(SIDE); n <= NL1(SIDE); n++) {
-            const long d = IX(g, a, b, g.ks - n);
-            copy_cell(S, N3, d, s);
-            for (int x = 0; x < 3; x++) mf[x * N3 + d] = mf[x * N3 + s];
-        }
-    } else {                    // outer K; the k+1 face takes the J flux (bcs.F90:925)
-        const long s = IX(g, a, b, g.ke);
-        for (int n = NL0(SIDE); n <= NL1(SIDE); n++) {
-            const long d = IX(g, a, b, g.ke + n);
-            copy_cell(S, N3, d, s);
-            mf[d] = mf[s];
-            mf[N3 + d] = mf[N3 + s];
-            mf[2 * N3 + IX(g, a, b, g.ke + n + 1)] = mf[N3 + IX(g, a, b, g.ke + 1)];
+        } else if (SIDE == 2) {     // inner J: a=i, b=k
+            const long s = IX(g, a, g.js, b);
+            for (int n = NL0(SIDE); n <= NL1(SIDE); n++) {
+                const long d = IX(g, a, g.js - n, b);
+                copy_cell(S, N3, d, s);
+                for (int x = 0; x < 3; x++) mf[x * N3 + d] = mf[x * N3 + s];
+            }
+        } else if (SIDE == 3) {     // outer J
+            const long s = IX(g, a, g.je, b);
+            for (int n = NL0(SIDE); n <= NL1(SIDE); n++) {
+                const long d = IX(g, a, g.je + n, b);
+                copy_cell(S, N3, d, s);
+                mf[d] = mf[s];
+                mf[2 * N3 + d] = mf[2 * N3 + s];
+                mf[N3 + IX(g, a, g.je + n + 1, b)] = mf[N3 + IX(g, a, g.je + 1, b)];
+            }
+        } else if (SIDE == 4) {     // inner K: a=i, b=j
+            const long s = IX(g, a, b, g.ks);
+            for (int n = NL0(SIDE); n <= NL1(SIDE); n++) {
+                const long d = IX(g, a, b, g.ks - n);
+                copy_cell(S, N3, d, s);
+                for (int x = 0; x < 3; x++) mf[x * N3 + d] = mf[x * N3 + s];
+            }
+        } else {                    // outer K; the k+1 face takes the J flux (bcs.F90:925)
+            const long s = IX(g, a, b, g.ke);
+            for (int n = NL0(SIDE); n <= NL1(SIDE); n++) {
+                const long d = IX(g, a, b, g.ke + n);
+                copy_cell(S, N3, d, s);
+                mf[d] = mf[s];
+                mf[N3 + d] = mf[N3 + s];
+                mf[2 * N3 + IX(g, a, b, g.ke + n + 1)] = mf[N3 + IX(g, a, b, g.ke + 1)];
+            }
         }
     }
-}
+};
 
 // ---------------------------------------------------------------- wind
 
@@ -1237,95 +1341,102 @@ __device__ double wind_wgt(const WindDev &w, const double *vF, const double *x, 
 
 // WindBC part 1 (wind.F90:226-335): ghost gas and blended face fluxes.
 // Transverse: a = j in jsg..jeg+1, b = k in ksg..keg+1.
-__global__ void k_bc_wind(Geo g, MParm m, Box t, const Clock *clk, WindDev w,
-                          double *__restrict__ S, const double *__restrict__ met)
-{
-    const int j = t.i0 + blockIdx.x * blockDim.x + threadIdx.x;
-    const int k = t.j0 + blockIdx.y;
-    if (j > t.i1 || k > t.j1) return;
-    const long N3 = g.N3;
-    double *mf = S + S_MF * N3;
-    const double tm = clk->t;
-    double vF[3];
-    wind_front(w, tm, vF);
-    const int ip = g.ie;
-    for (int n = NL0(1); n <= NL1(1); n++) {
-        const int ig = g.ie + n;
-        if (k <= g.keg && j <= g.jeg) {
-            const long cg = IX(g, ig, j, k), cp = IX(g, ip, j, k);
-            const double xc[3] = {met[(M_XYZCC + 0) * N3 + cg], met[(M_XYZCC + 1) * N3 + cg], met[(M_XYZCC + 2) * N3 + cg]};
-            const double wSW = wind_wgt(w, vF, xc, tm), wMHD = 1.0 - wSW;
-            double gs[5] = {0, 0, 0, 0, 0}, gi[5] = {0, 0, 0, 0, 0};
-            if (wSW > GAM_TINY) {
-                double r, p, V[3], B[3];
-                wind_at(w, vF, xc, tm, &r, &p, V, B);
-                gs[0] = r; gs[1] = V[0]; gs[2] = V[1]; gs[3] = V[2]; gs[4] = p;
-            }
-            if (wMHD > GAM_TINY) {
-                double U[5], Bi[3];
-                for (int v = 0; v < 5; v++) U[v] = S[(S_GAS + v) * N3 + cp];
-                for (int x = 0; x < 3; x++) Bi[x] = S[(S_B + x) * N3 + cp];
-                c2p_cell(m, U, gi);
-                // Con2Fast (gamutils.F90:338-360)
-                const double rho = dmax(U[DEN], m.dFloor);
-                const double vx = U[MOMX] / rho, vy = U[MOMY] / rho, vz = U[MOMZ] / rho;
-                const double ke = 0.5 * rho * (vx * vx + vy * vy + vz * vz);
-                const double P = dmax((m.gamma - 1) * (U[ENER] - ke), m.pFloor);
-                const double Cs = sqrt(m.gamma * P / rho);
-                double Va = sqrt(Bi[0] * Bi[0] + Bi[1] * Bi[1] + Bi[2] * Bi[2]) / sqrt(U[DEN]);
-                if (m.doBoris) Va = m.Ca * Va / sqrt(m.Ca * m.Ca + Va * Va);
-                const double Cx = sqrt(Cs * Cs + Va * Va);
-                const long fp = IX(g, ip + 1, j, k);
-                const double nH[3] = {met[(M_TF + 0) * N3 + fp], met[(M_TF + 1) * N3 + fp], met[(M_TF + 2) * N3 + fp]};
-                double V[3] = {gi[1], gi[2], gi[3]};
-                const double vp = V[0] * nH[0] + V[1] * nH[1] + V[2] * nH[2];
-                if (vp < 1.0 * Cx) {
-                    for (int x = 0; x < 3; x++) V[x] = V[x] - vp * nH[x] + 1.0 * Cx * nH[x];
+struct k_bc_wind {
+    __device__ __forceinline__ void operator()(const uint3 bi, Geo g, MParm m, Box t,
+                                               const Clock *clk, WindDev w, double *__restrict__ S,
+                                               const double *__restrict__ met) const
+    {
+        const int j = t.i0 + bi.x * blockDim.x + threadIdx.x;
+        const int k = t.j0 + bi.y;
+        if (j > t.i1 || k > t.j1) return;
+        const long N3 = g.N3;
+        double *mf = S + S_MF * N3;
+        const double tm = clk->t;
+        double vF[3];
+        wind_front(w, tm, vF);
+        const int ip = g.ie;
+        for (int n = NL0(1); n <= NL1(1); n++) {
+            const int ig = g.ie + n;
+            if (k <= g.keg && j <= g.jeg) {
+                const long cg = IX(g, ig, j, k), cp = IX(g, ip, j, k);
+                const double xc[3] = {met[(M_XYZCC + 0) * N3 + cg], met[(M_XYZCC + 1) * N3 + cg], met[(M_XYZCC + 2) * N3 + cg]};
+                const double wSW = wind_wgt(w, vF, xc, tm), wMHD = 1.0 - wSW;
+                double gs[5] = {0, 0, 0, 0, 0}, gi[5] = {0, 0, 0, 0, 0};
+                if (wSW > GAM_TINY) {
+                    double r, p, V[3], B[3];
+                    wind_at(w, vF, xc, tm, &r, &p, V, B);
+                    gs[0] = r; gs[1] = V[0]; gs[2] = V[1]; gs[3] = V[2]; gs[4] = p;
                 }
-                gi[1] = V[0]; gi[2] = V[1]; gi[3] = V[2];
+                if (wMHD > GAM_TINY) {
+                    double U[5], Bi[3];
+                    for (int v = 0; v < 5; v++) U[v] = S[(S_GAS + v) * N3 + cp];
+                    for (int x = 0; x < 3; x++) Bi[x] = S[(S_B + x) * N3 + cp];
+                    c2p_cell(m, U, gi);
+                    // Con2Fast (gamutils.F90:338-360)
+                    const double rho = dmax(U[DEN], m.dFloor);
+                    const double vx = U[MOMX] / rho, vy = U[MOMY] / rho, vz = U[MOMZ] / rho;
+                    const double ke = 0.5 * rho * (vx * vx + vy * vy + vz * vz);
+                    const double P = dmax((m.gamma - 1) * (U[ENER] - ke), m.pFloor);
+                    const double Cs = sqrt(m.gamma * P / rho);
+                    double Va = sqrt(Bi[0] * Bi[0] + Bi[1] * Bi[1] + Bi[2] * Bi[2]) / sqrt(U[DEN]);
+                    if (m.doBoris) Va = m.Ca * Va / sqrt(m.Ca * m.Ca + Va * Va);
+                    const double Cx = sqrt(Cs * Cs + Va * Va);
+                    const long fp = IX(g, ip + 1, j, k);
+                    const double nH[3] = {met[(M_TF + 0) * N3 + fp], met[(M_TF + 1) * N3 + fp], met[(M_TF + 2) * N3 + fp]};
+                    double V[3] = {gi[1], gi[2], gi[3]};
+                    const double vp = V[0] * nH[0] + V[1] * nH[1] + V[2] * nH[2];
+                    if (vp < 1.0 * Cx) {
+                        for (int x = 0; x < 3; x++) V[x] = V[x] - vp * nH[x] + 1.0 * Cx * nH[x];
+                    }
+                    gi[1] = V[0]; gi[2] = V[1]; gi[3] = V[2];
+                }
+                double gw[5], U[5];
+                for (int v = 0; v < 5; v++) gw[v] = wSW * gs[v] + wMHD * gi[v];
+                p2c_cell(m, gw, U);
+                for (int v = 0; v < 5; v++) S[(S_GAS + v) * N3 + cg] = U[v];
             }
-            double gw[5], U[5];
-            for (int v = 0; v < 5; v++) gw[v] = wSW * gs[v] + wMHD * gi[v];
-            p2c_cell(m, gw, U);
-            for (int v = 0; v < 5; v++) S[(S_GAS + v) * N3 + cg] = U[v];
-        }
-        const int fi[3] = {ig + 1, ig, ig};
-        const int ii[3] = {ip + 1, ip, ip};
-        for (int d = 0; d < 3; d++) {
-            const long f = IX(g, fi[d], j, k);
-            const double xf[3] = {met[(M_XFC + 0 + 3 * d) * N3 + f], met[(M_XFC + 1 + 3 * d) * N3 + f], met[(M_XFC + 2 + 3 * d) * N3 + f]};
-            const double wSW = wind_wgt(w, vF, xf, tm);
-            double r, p, V[3], B[3];
-            wind_at(w, vF, xf, tm, &r, &p, V, B);
-            const double *nrm = met + (long)(M_TF + 9 * d) * N3 + f;
-            const double sw = met[(M_FACE + d) * N3 + f] * (nrm[0] * B[0] + nrm[N3] * B[1] + nrm[2 * N3] * B[2]);
-            const double in = mf[d * N3 + IX(g, ii[d], j, k)];
-            mf[d * N3 + f] = wSW * sw + (1 - wSW) * in;
+            const int fi[3] = {ig + 1, ig, ig};
+            const int ii[3] = {ip + 1, ip, ip};
+            for (int d = 0; d < 3; d++) {
+                const long f = IX(g, fi[d], j, k);
+                const double xf[3] = {met[(M_XFC + 0 + 3 * d) * N3 + f], met[(M_XFC + 1 + 3 * d) * N3 + f], met[(M_XFC + 2 + 3 * d) * N3 + f]};
+                const double wSW = wind_wgt(w, vF, xf, tm);
+                double r, p, V[3], B[3];
+                wind_at(w, vF, xf, tm, &r, &p, V, B);
+                const double *nrm = met + (long)(M_TF + 9 * d) * N3 + f;
+                const double sw = met[(M_FACE + d) * N3 + f] * (nrm[0] * B[0] + nrm[N3] * B[1] + nrm[2 * N3] * B[2]);
+                const double in = mf[d * N3 + IX(g, ii[d], j, k)];
+                mf[d * N3 + f] = wSW * sw + (1 - wSW) * in;
+            }
         }
     }
-}
+};
 
 // WindBC part 2 (wind.F90:338-348): ghost Bxyz from the blended fluxes with
 // the periodic k mapping of lfmIJKcc (ringutils.F90:482-518).
-__global__ void k_bc_wind_b(Geo g, Box t, double *__restrict__ S, const double *__restrict__ met)
-{
-    const int j = t.i0 + blockIdx.x * blockDim.x + threadIdx.x;
-    const int k = t.j0 + blockIdx.y;
-    if (j > t.i1 || k > t.j1) return;
-    const long N3 = g.N3;
-    const double *mf = S + S_MF * N3;
-    int kp = k;
-    if (k < g.ks) kp = k + g.Nkp;
-    else if (k > g.ke) kp = k - g.Nkp;
-    for (int n = NL0(1); n <= NL1(1); n++) {
-        const int ig = g.ie + n;
-        const long s = IX(g, ig, j, kp);
-        double B[3];
-        cell_b(g, met, s, mf[s], mf[N3 + s], mf[2 * N3 + s], mf[s + 1], mf[N3 + s + g.P], mf[2 * N3 + s + g.PP], B);
-        const long d = IX(g, ig, j, k);
-        for (int x = 0; x < 3; x++) S[(S_B + x) * N3 + d] = B[x];
+struct k_bc_wind_b {
+    __device__ __forceinline__ void operator()(const uint3 bi, Geo g, Box t,
+                                               double *__restrict__ S,
+                                               const double *__restrict__ met) const
+    {
+        const int j = t.i0 + bi.x * blockDim.x + threadIdx.x;
+        const int k = t.j0 + bi.y;
+        if (j > t.i1 || k > t.j1) return;
+        const long N3 = g.N3;
+        const double *mf = S + S_MF * N3;
+        int kp = k;
+        if (k < g.ks) kp = k + g.Nkp;
+        else if (k > g.ke) kp = k - g.Nkp;
+        for (int n = NL0(1); n <= NL1(1); n++) {
+            const int ig = g.ie + n;
+            const long s = IX(g, ig, j, kp);
+            double B[3];
+            cell_b(g, met, s, mf[s], mf[N3 + s], mf[2 * N3 + s], mf[s + 1], mf[N3 + s + g.P], mf[2 * N3 + s + g.PP], B);
+            const long d = IX(g, ig, j, k);
+            for (int x = 0; x < 3; x++) S[(S_B + x) * N3 + d] = B[x];
+        }
     }
-}
+};
 
 // DivB partial sums (gridutils.F90:717-770)
 __global__ void k_divb(Geo g, Box ab, const double *__restrict__ mf, const double *__restrict__ bg,
@@ -1403,7 +1514,7 @@ void launch_predict(hipStream_t s, const Geo &g, const MParm &m, const Box &fb, 
                     const double *S, const double *O, const double *met, double *H)
 {
     const Tile t = tile_of(fb, 3);
-    LAUNCH(k_predict, tgrid(t), dim3(BX, BY), s, g, m, t, clk, S, O, met, H);
+    launch_f<k_predict>(tgrid(t), dim3(BX, BY), s, g, m, t, clk, S, O, met, H);
 }
 
 void launch_vface(hipStream_t s, const Geo &g, const MParm &m, const double *H, const double *met,
@@ -1411,7 +1522,7 @@ void launch_vface(hipStream_t s, const Geo &g, const MParm &m, const double *H, 
 {
     const Box b = {g.isg, g.ieg, g.jsg, g.jeg, g.ksg, g.keg};
     const ColTile t = coltile_of(b);
-    LAUNCH(k_vface, cgrid(t), dim3(BX, BY), s, g, m, t, H, met, Vf);
+    launch_f<k_vface>(cgrid(t), dim3(BX, BY), s, g, m, t, H, met, Vf);
 }
 
 template <int ED, int RM, bool BG, bool BO>
@@ -1422,7 +1533,7 @@ static void emf_t(hipStream_t s, const Geo &g, const MParm &m, const Box &b, con
     // stencils run along T2 and T1: rows along j for K edges, along k for J
     // edges (both stencils along i or k), j-slabs for I edges (j and k)
     const Tile t = tile_of(b, ED == 2 ? 1 : (ED == 0 ? 3 : 2));
-    LAUNCH((k_emf<ED, RM, BG, BO>), tgrid(t), dim3(BX, BY), s, g, m, t, clk, H, S, Vf, met, bg, E);
+    launch_f<k_emf<ED, RM, BG, BO>>(tgrid(t), dim3(BX, BY), s, g, m, t, clk, H, S, Vf, met, bg, E);
 }
 
 template <int ED>
@@ -1458,7 +1569,7 @@ static void flux_t(hipStream_t s, const Geo &g, const MParm &m, const Box &b, co
 {
     if constexpr (D == 0) {
         const Tile t = tile_of(b, 1);
-        LAUNCH((k_flux<D, RM, BG, BO>), tgrid(t), dim3(BX, BY), s, g, m, t, H, S, met, bg, gf, mf);
+        launch_f<k_flux<D, RM, BG, BO>>(tgrid(t), dim3(BX, BY), s, g, m, t, H, S, met, bg, gf, mf);
         return;
     } else {
     FluxTile t;
@@ -1468,7 +1579,7 @@ static void flux_t(hipStream_t s, const Geo &g, const MParm &m, const Box &b, co
     t.nti = (t.i1 - t.i0 + BX) / BX;
     t.nta = (t.a1 - t.a0 + BY) / BY;
     t.nb = t.nti * t.nta * (t.b1 - t.b0 + 1);
-    LAUNCH((k_flux_s<D, RM, BG, BO>), dim3((unsigned)(((t.nb + 7) / 8) * 8)), dim3(BX, BY), s, g, m, t,
+    launch_f<k_flux_s<D, RM, BG, BO>>(dim3((unsigned)(((t.nb + 7) / 8) * 8)), dim3(BX, BY), s, g, m, t,
            H, S, met, bg, gf, mf);
     }
 }
@@ -1507,10 +1618,10 @@ void launch_update(hipStream_t s, const Geo &g, const MParm &m, const Box &fb, c
     const int sel = (m.doBG ? 2 : 0) + (m.doBoris ? 1 : 0);
     const Tile t = tile_of(fb, 3);
     switch (sel) {
-    case 0: LAUNCH((k_update<false, false>), tgrid(t), dim3(BX, BY), s, g, m, t, ctb, eb, ab, dtb, clk, So, Sn, gf, mf, met, bg); break;
-    case 1: LAUNCH((k_update<false, true>), tgrid(t), dim3(BX, BY), s, g, m, t, ctb, eb, ab, dtb, clk, So, Sn, gf, mf, met, bg); break;
-    case 2: LAUNCH((k_update<true, false>), tgrid(t), dim3(BX, BY), s, g, m, t, ctb, eb, ab, dtb, clk, So, Sn, gf, mf, met, bg); break;
-    default: LAUNCH((k_update<true, true>), tgrid(t), dim3(BX, BY), s, g, m, t, ctb, eb, ab, dtb, clk, So, Sn, gf, mf, met, bg); break;
+    case 0: launch_f<k_update<false, false>>(tgrid(t), dim3(BX, BY), s, g, m, t, ctb, eb, ab, dtb, clk, So, Sn, gf, mf, met, bg); break;
+    case 1: launch_f<k_update<false, true>>(tgrid(t), dim3(BX, BY), s, g, m, t, ctb, eb, ab, dtb, clk, So, Sn, gf, mf, met, bg); break;
+    case 2: launch_f<k_update<true, false>>(tgrid(t), dim3(BX, BY), s, g, m, t, ctb, eb, ab, dtb, clk, So, Sn, gf, mf, met, bg); break;
+    default: launch_f<k_update<true, true>>(tgrid(t), dim3(BX, BY), s, g, m, t, ctb, eb, ab, dtb, clk, So, Sn, gf, mf, met, bg); break;
     }
 }
 
@@ -1520,23 +1631,23 @@ void launch_calc_dt(hipStream_t s, const Geo &g, const MParm &m, const Box &dtb,
     const int sel = (m.doBG ? 2 : 0) + (m.doBoris ? 1 : 0);
     const Tile t = tile_of(dtb, 1);
     switch (sel) {
-    case 0: LAUNCH((k_calc_dt<false, false>), tgrid(t), dim3(BX, BY), s, g, m, t, clk, S, met, bg); break;
-    case 1: LAUNCH((k_calc_dt<false, true>), tgrid(t), dim3(BX, BY), s, g, m, t, clk, S, met, bg); break;
-    case 2: LAUNCH((k_calc_dt<true, false>), tgrid(t), dim3(BX, BY), s, g, m, t, clk, S, met, bg); break;
-    default: LAUNCH((k_calc_dt<true, true>), tgrid(t), dim3(BX, BY), s, g, m, t, clk, S, met, bg); break;
+    case 0: launch_f<k_calc_dt<false, false>>(tgrid(t), dim3(BX, BY), s, g, m, t, clk, S, met, bg); break;
+    case 1: launch_f<k_calc_dt<false, true>>(tgrid(t), dim3(BX, BY), s, g, m, t, clk, S, met, bg); break;
+    case 2: launch_f<k_calc_dt<true, false>>(tgrid(t), dim3(BX, BY), s, g, m, t, clk, S, met, bg); break;
+    default: launch_f<k_calc_dt<true, true>>(tgrid(t), dim3(BX, BY), s, g, m, t, clk, S, met, bg); break;
     }
 }
 
 void launch_finalize(hipStream_t s, const MParm &m, Clock *clk, int mode)
 {
-    LAUNCH(k_finalize, dim3(1), dim3(64), s, m, clk, mode);
+    launch_f<k_finalize>(dim3(1), dim3(64), s, m, clk, mode);
 }
 
 void launch_bflux2fld(hipStream_t s, const Geo &g, const Box &cb, const Box &skip, const double *mf,
                       double *B, const double *met)
 {
     const Tile t = tile_of(cb, 1);
-    LAUNCH(k_bflux2fld, tgrid(t), dim3(BX, BY), s, g, t, skip, mf, B, met);
+    launch_f<k_bflux2fld>(tgrid(t), dim3(BX, BY), s, g, t, skip, mf, B, met);
 }
 
 void launch_bflux2fld_ghosts(hipStream_t s, const Geo &g, const double *mf, double *B, const double *met)
@@ -1559,7 +1670,7 @@ void launch_bflux2fld_ghosts(hipStream_t s, const Geo &g, const double *mf, doub
     }
     gb.start[6] = tot;
     gb.n = 6;
-    LAUNCH(k_bflux2fld_ghosts, dim3((unsigned)tot), dim3(BX, BY), s, g, gb, mf, B, met);
+    launch_f<k_bflux2fld_ghosts>(dim3((unsigned)tot), dim3(BX, BY), s, g, gb, mf, B, met);
 }
 
 static inline dim3 grid2(const Box &t) { return dim3((unsigned)((t.i1 - t.i0 + 64) / 64), (unsigned)(t.j1 - t.j0 + 1)); }
@@ -1575,38 +1686,38 @@ void launch_bc(hipStream_t s, int side, int kind, const Geo &g, const MParm &m, 
     else { t.i0 = g.isg; t.i1 = g.ieg + (pi ? 1 : 0); t.j0 = g.jsg; t.j1 = g.jeg + (pi ? 1 : 0); }
     t.k0 = t.k1 = 0;
     dim3 gr = grid2(t), bl(64);
-    if (side <= 1) gr.z = 4;    // one ghost layer per blockIdx.z
+    if (side <= 1) gr.z = 4;    // one ghost layer per block z index
     if (kind == 0) {
         switch (side) {
-        case 0: LAUNCH(k_bc_periodic<0>, gr, bl, s, g, t, S); break;
-        case 1: LAUNCH(k_bc_periodic<1>, gr, bl, s, g, t, S); break;
-        case 2: LAUNCH(k_bc_periodic<2>, gr, bl, s, g, t, S); break;
-        case 3: LAUNCH(k_bc_periodic<3>, gr, bl, s, g, t, S); break;
-        case 4: LAUNCH(k_bc_periodic<4>, gr, bl, s, g, t, S); break;
-        default: LAUNCH(k_bc_periodic<5>, gr, bl, s, g, t, S); break;
+        case 0: launch_f<k_bc_periodic<0>>(gr, bl, s, g, t, S); break;
+        case 1: launch_f<k_bc_periodic<1>>(gr, bl, s, g, t, S); break;
+        case 2: launch_f<k_bc_periodic<2>>(gr, bl, s, g, t, S); break;
+        case 3: launch_f<k_bc_periodic<3>>(gr, bl, s, g, t, S); break;
+        case 4: launch_f<k_bc_periodic<4>>(gr, bl, s, g, t, S); break;
+        default: launch_f<k_bc_periodic<5>>(gr, bl, s, g, t, S); break;
         }
     } else if (kind == 1) {
         switch (side) {
-        case 0: LAUNCH((k_bc_copy<0, 1>), gr, bl, s, g, t, S, met); break;
-        case 1: LAUNCH((k_bc_copy<1, 1>), gr, bl, s, g, t, S, met); break;
-        case 2: LAUNCH((k_bc_copy<2, 1>), gr, bl, s, g, t, S, met); break;
-        case 3: LAUNCH((k_bc_copy<3, 1>), gr, bl, s, g, t, S, met); break;
-        case 4: LAUNCH((k_bc_copy<4, 1>), gr, bl, s, g, t, S, met); break;
-        default: LAUNCH((k_bc_copy<5, 1>), gr, bl, s, g, t, S, met); break;
+        case 0: launch_f<k_bc_copy<0, 1>>(gr, bl, s, g, t, S, met); break;
+        case 1: launch_f<k_bc_copy<1, 1>>(gr, bl, s, g, t, S, met); break;
+        case 2: launch_f<k_bc_copy<2, 1>>(gr, bl, s, g, t, S, met); break;
+        case 3: launch_f<k_bc_copy<3, 1>>(gr, bl, s, g, t, S, met); break;
+        case 4: launch_f<k_bc_copy<4, 1>>(gr, bl, s, g, t, S, met); break;
+        default: launch_f<k_bc_copy<5, 1>>(gr, bl, s, g, t, S, met); break;
         }
     } else if (kind == 2) {
-        LAUNCH((k_bc_copy<0, 2>), gr, bl, s, g, t, S, met);
+        launch_f<k_bc_copy<0, 2>>(gr, bl, s, g, t, S, met);
     } else if (kind == 3) {
-        LAUNCH((k_bc_copy<1, 3>), gr, bl, s, g, t, S, met);
+        launch_f<k_bc_copy<1, 3>>(gr, bl, s, g, t, S, met);
     } else if (kind == 4) {
         Box tw = t;
         tw.i1 = g.jeg + 1; tw.j1 = g.keg + 1;
         dim3 gw = grid2(tw);
         gw.z = 4;
-        LAUNCH(k_bc_wind, gw, bl, s, g, m, tw, clk, *w, S, met);
+        launch_f<k_bc_wind>(gw, bl, s, g, m, tw, clk, *w, S, met);
         // ghost-cell B of the wind BC (wind.F90:338-348); skipped when a
         // bFlux2Fld over every ghost cell follows, which overwrites it
-        if (ghostB) LAUNCH(k_bc_wind_b, gr, bl, s, g, t, S, met);
+        if (ghostB) launch_f<k_bc_wind_b>(gr, bl, s, g, t, S, met);
     }
 }
 

@@ -32,6 +32,7 @@ EXPORTS = (
     "gam_init_run", "gam_advance_mhd", "gam_calc_dt", "gam_enforce_bcs", "gam_bflux2fld",
     "gam_step", "gam_run", "gam_advance_model", "gam_sync", "gam_divb", "gam_time_steps", "gam_set_profiling",
     "gam_kernel_times", "gam_kernel_group_name", "gam_get_solver",
+    "gam_ens_create", "gam_ens_run", "gam_ens_time_steps", "gam_ens_launches_per_step", "gam_ens_destroy",
 )
 
 DP = C.POINTER(C.c_double)
@@ -111,6 +112,11 @@ def load():
     L.gam_set_profiling.argtypes = [vp, C.c_int]
     L.gam_kernel_times.argtypes = [vp, DP, C.POINTER(C.c_int)]
     L.gam_get_solver.argtypes = [vp, C.c_int, DP]
+    L.gam_ens_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(vp)]
+    L.gam_ens_run.argtypes = [vp, C.c_int]
+    L.gam_ens_time_steps.argtypes = [vp, C.c_int, DP]
+    L.gam_ens_launches_per_step.argtypes = [vp, C.POINTER(C.c_int)]
+    L.gam_ens_destroy.argtypes = [vp]
     _lib = L
     return L
 
@@ -325,6 +331,49 @@ class GamHip:
         n = (C.c_int * 8)()
         self._chk(self.L.gam_kernel_times(self.ctx, ms, n))
         return {self.L.gam_kernel_group_name(q).decode(): (ms[q], n[q]) for q in range(8)}
+
+
+class GamEnsemble:
+    """Independent contexts stepped together: one launch per step kernel
+    over all members, one captured hipGraph per step.  The members stay
+    usable as GamHip objects (state transfers show the ensemble's
+    progress); they must outlive the ensemble."""
+
+    def __init__(self, members):
+        self.L = load()
+        self.members = list(members)
+        arr = (C.c_void_p * len(self.members))(*[m.ctx for m in self.members])
+        self.ens = C.c_void_p()
+        self._chk(self.L.gam_ens_create(C.byref(self.ens), len(self.members), arr))
+
+    def _chk(self, rc):
+        if rc != GAM_OK:
+            raise GamError(rc, self.L.gam_last_error().decode())
+        return rc
+
+    def run(self, nsteps):
+        self._chk(self.L.gam_ens_run(self.ens, int(nsteps)))
+
+    def time_steps(self, nsteps):
+        ms = C.c_double()
+        self._chk(self.L.gam_ens_time_steps(self.ens, int(nsteps), C.byref(ms)))
+        return ms.value
+
+    def launches_per_step(self):
+        n = C.c_int()
+        self._chk(self.L.gam_ens_launches_per_step(self.ens, C.byref(n)))
+        return n.value
+
+    def close(self):
+        if getattr(self, "ens", None):
+            self.L.gam_ens_destroy(self.ens)
+            self.ens = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------- restart files

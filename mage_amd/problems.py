@@ -255,6 +255,26 @@ def constant_wind(rho=1.0, vx=-0.4, p=1.0e-3, bz=-0.05):
 SHELL_M0 = -(2.5 ** 3)
 
 
+def ensemble_members(n=32, seed=SEED0, lo=(48, 48, 64), hi=(96, 96, 128)):
+    """The ensemble workload: n shell runs of uneven size.
+
+    Grid sizes come from SplitMix64(seed) as multiples of 8 in
+    [lo, hi] per axis (i, j, k drawn in that order, member by member).
+    Member m uses seed + m for its grid warp and initial state, and draws
+    its constant inflow from SplitMix64(seed + m): vx in [-0.8, -0.3],
+    Bz in [-0.1, 0.1].  Returns dicts with N, seed, vx, bz.
+    """
+    rng = SplitMix64(seed)
+    out = []
+    for m in range(n):
+        N = tuple(8 * rng.randint(lo[a] // 8, hi[a] // 8) for a in range(3))
+        r = SplitMix64(seed + m)
+        vx = -0.8 + 0.5 * r.uniform()
+        bz = -0.1 + 0.2 * r.uniform()
+        out.append({"N": N, "seed": seed + m, "vx": vx, "bz": bz})
+    return out
+
+
 def box_params(rmeth="7UP"):
     return model_params(rmeth={"8CENT": 0, "7UP": 1}[rmeth], bc=["periodic"] * 6)
 

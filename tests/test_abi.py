@@ -62,3 +62,19 @@ def test_bad_bc_and_recon_rejected():
     assert rc == -1 and "outer I" in err
     rc, err = _create(16, 16, 16, rmeth=3)
     assert rc == -1 and "rmeth" in err
+
+
+def test_ensemble_argument_checks():
+    """Ensemble entry points reject bad arguments before touching a GPU."""
+    L = gamhip.load()
+    out = C.c_void_p()
+    arr = (C.c_void_p * 1)()
+    assert L.gam_ens_create(None, 1, arr) == -1
+    assert L.gam_ens_create(C.byref(out), 0, arr) == -1
+    assert "outside" in L.gam_last_error().decode()
+    assert L.gam_ens_create(C.byref(out), 257, arr) == -1
+    assert L.gam_ens_create(C.byref(out), 1, arr) == -1      # NULL member
+    assert L.gam_ens_run(None, 1) == -1
+    ms = C.c_double()
+    assert L.gam_ens_time_steps(None, 1, C.byref(ms)) == -1
+    assert L.gam_ens_destroy(None) == 0
