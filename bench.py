@@ -91,11 +91,14 @@ def setup_member(gamhip, pb, spec, rmeth="7UP"):
     return g
 
 
-def ensemble_gain(gamhip, pb, n=32, steps=100, warmup=2):
+def ensemble_gain(gamhip, pb, n=32, steps=100, warmup=2, tune=True):
     """Batched ensemble vs. the same members stepped one after another with
-    plain launches: aggregate cell-updates/s of each, HIP-event timed."""
+    plain launches: aggregate cell-updates/s of each, HIP-event timed.  With
+    tune, every member's launch plan is autotuned for its grid first (both
+    timings use the tuned plans)."""
     specs = pb.ensemble_members(n=n)
     members = [setup_member(gamhip, pb, s) for s in specs]
+    plans = [g.autotune(reps=3) for g in members] if tune else None
     cells = sum(s["N"][0] * s["N"][1] * s["N"][2] for s in specs)
     for g in members:
         g.run(warmup)
@@ -115,7 +118,8 @@ def ensemble_gain(gamhip, pb, n=32, steps=100, warmup=2):
             "sizes": [list(s["N"]) for s in specs],
             "sequential_ms": seq_ms, "batched_ms": ens_ms,
             "sequential_cell_updates_per_s": seq, "batched_cell_updates_per_s": bat,
-            "gain": seq_ms / ens_ms, "launches_per_step": launches}
+            "gain": seq_ms / ens_ms, "launches_per_step": launches, "tuned": bool(tune),
+            "distinct_plans": len({tuple(sorted(p.items())) for p in plans}) if plans else 1}
 
 
 def cpu_baseline(gamhip, pb, min_seconds=12.0, max_steps=400):
@@ -155,6 +159,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-box", action="store_true")
     ap.add_argument("--no-ens", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--ens-members", type=int, default=32)
     ap.add_argument("--ens-steps", type=int, default=100)
     args = ap.parse_args()
@@ -170,6 +175,11 @@ def main():
     t0 = time.time()
     g, p, grid, st, wind = setup_shell(gamhip, pb, N)
     log("setup %.1f s" % (time.time() - t0))
+    plan = g.get_plan()
+    if not args.no_tune:
+        t0 = time.time()
+        plan = g.autotune(reps=3)
+        log("autotuned plan %s (%.1f s)" % (plan, time.time() - t0))
 
     g.run(args.warmup)
     ms = g.time_steps(args.steps)
@@ -218,7 +228,7 @@ def main():
     ens = None
     if not args.no_ens:
         t1 = time.time()
-        ens = ensemble_gain(gamhip, pb, n=args.ens_members, steps=args.ens_steps)
+        ens = ensemble_gain(gamhip, pb, n=args.ens_members, steps=args.ens_steps, tune=not args.no_tune)
         log("ensemble %d members: sequential %.1f ms, batched %.1f ms, gain %.2f (%.1f s)"
             % (ens["members"], ens["sequential_ms"], ens["batched_ms"], ens["gain"], time.time() - t1))
 
@@ -250,6 +260,7 @@ def main():
             "cells": cells,
             "reconstruction": "7UP",
         },
+        "launch_plan": plan,
         "roofline": {
             "bound": "hbm",
             "kernel": "k_flux (one interface direction per launch)",

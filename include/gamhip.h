@@ -189,6 +189,27 @@ const char *gam_kernel_group_name(int group);
 int gam_get_solver(gam_ctx *ctx, int which, double *out);
 
 /* ---------------------------------------------------------------------
+ * Launch plans.  No counterpart in the reference (its loop blocking is
+ * fixed at compile time, vecLen in kdefs.F90).  The step's tunable
+ * launches ("slots": predictor, the EMF of each edge direction, the
+ * interface flux of each face direction, the update) each have a few
+ * launch shapes that give the same results: the row order of the
+ * 256-point chunk tiling (1 = j fastest, 2 = k fastest, 3 = slabs of 8
+ * j-rows) and, for the j and k fluxes, 4 = the LDS-staged 64 x 4 face tile.
+ * gam_autotune times every candidate of every slot on the context's own
+ * grid (reps steps per candidate, hipEvents around that slot's launch),
+ * keeps the fastest, and restores State, oState, time and dt as they were.
+ * ------------------------------------------------------------------- */
+#define GAM_TUNE_SLOTS 8
+#define GAM_TUNE_MAXREPS 32
+int gam_autotune(gam_ctx *ctx, int reps);             /* 1 <= reps <= GAM_TUNE_MAXREPS */
+int gam_get_plan(gam_ctx *ctx, int *plan);            /* GAM_TUNE_SLOTS values */
+int gam_set_plan(gam_ctx *ctx, const int *plan);      /* GAM_E_ARG if a value is not a candidate */
+const char *gam_tune_slot_name(int slot);              /* "" outside 0..GAM_TUNE_SLOTS-1 */
+/* Candidate values of a slot (up to 4); returns their number, 0 for a bad slot. */
+int gam_tune_candidates(int slot, int *values);
+
+/* ---------------------------------------------------------------------
  * Ensembles: many independent runs stepped together.  The reference runs
  * one gamApp_T per process and has no batched form; an ensemble here is a
  * set of contexts (each one stepGamera instance, gamapp.F90:73-113, with

@@ -60,6 +60,11 @@ struct gam_ctx {
     double kg_ms[GAM_KG_COUNT];
     int kg_n[GAM_KG_COUNT];
     std::vector<int> pend;   // group id of each pending event pair
+    // launch plan and its tuning
+    Plan plan;
+    int tune_slot;           // slot whose launches get events, or -1
+    int ntev;
+    hipEvent_t tev[2 * GAM_TUNE_MAXREPS];
 };
 
 // ------------------------------------------------------------ geometry
@@ -195,7 +200,14 @@ extern "C" int gam_create(gam_ctx **out, const gam_dims *d, const gam_model *m)
         gam_destroy(c);
         return rc;
     }
-    // Grid defaults (face=1, edge=1, Teb=1) until metrics arrive
+    for (int q = 0; e == hipSuccess && q < 2 * GAM_TUNE_MAXREPS; q++) e = hipEventCreate(&c->tev[q]);
+    if (e != hipSuccess) {
+        rc = fail(GAM_E_HIP, "gam_create: %s", hipGetErrorString(e));
+        gam_destroy(c);
+        return rc;
+    }
+    c->plan = default_plan();
+    c->tune_slot = -1;
     c->cur = 0;
     *out = c;
     return GAM_OK;
@@ -209,6 +221,7 @@ extern "C" int gam_destroy(gam_ctx *c)
     for (double *b : bufs) if (b) hipFree(b);
     if (c->clk) hipFree(c->clk);
     for (int q = 0; q < (int)(sizeof(c->ev) / sizeof(c->ev[0])); q++) if (c->ev[q]) hipEventDestroy(c->ev[q]);
+    for (int q = 0; q < 2 * GAM_TUNE_MAXREPS; q++) if (c->tev[q]) hipEventDestroy(c->tev[q]);
     if (c->stream) hipStreamDestroy(c->stream);
     c->pend.~vector<int>();
     free(c);
@@ -515,6 +528,13 @@ static void enq_bcs(gam_ctx *c, double *S, bool ghostB = true)
         launch_bc(c->stream, side, c->model.bc[side], c->g, c->mp, c->clk, &c->wd, S, c->met, ghostB);
 }
 
+// While a slot is being tuned, its launches are bracketed by events.
+static void tune_mark(gam_ctx *c, int slot)
+{
+    if (c->tune_slot != slot || c->ntev >= 2 * GAM_TUNE_MAXREPS) return;
+    hipEventRecord(c->tev[c->ntev++], c->stream);
+}
+
 // AdvanceMHD on the device (mhdgroup.F90:16-76); swaps State/oState.
 static void enq_advance(gam_ctx *c, int finmode)
 {
@@ -525,7 +545,9 @@ static void enq_advance(gam_ctx *c, int finmode)
     double *Sn = O;   // becomes State
 
     prof_begin(c, KG_PREDICT);
-    launch_predict(s, g, c->mp, face_box(g), c->clk, S, O, c->met, c->H);
+    tune_mark(c, TS_PREDICT);
+    launch_predict(s, c->plan.v[TS_PREDICT], g, c->mp, face_box(g), c->clk, S, O, c->met, c->H);
+    tune_mark(c, TS_PREDICT);
     prof_end(c, KG_PREDICT);
 
     Box eb[3];
@@ -539,8 +561,10 @@ static void enq_advance(gam_ctx *c, int finmode)
             // edge direction eD takes the face velocities along dT1
             const int dT1 = (eD == 0) ? 1 : (eD == 1) ? 2 : 0;
             prof_begin(c, KG_EMF);
-            launch_emf(s, eD, g, c->mp, eb[eD], c->clk, c->H, S, c->vf + (long)3 * dT1 * N3, c->met,
-                       c->bg, Sn + S_E * N3);
+            tune_mark(c, TS_EMF_I + eD);
+            launch_emf(s, eD, c->plan.v[TS_EMF_I + eD], g, c->mp, eb[eD], c->clk, c->H, S,
+                       c->vf + (long)3 * dT1 * N3, c->met, c->bg, Sn + S_E * N3);
+            tune_mark(c, TS_EMF_I + eD);
             prof_end(c, KG_EMF);
         }
     }
@@ -548,14 +572,18 @@ static void enq_advance(gam_ctx *c, int finmode)
         Box fb = act_box(g);
         if (d == 0) fb.i1 += 1; else if (d == 1) fb.j1 += 1; else fb.k1 += 1;
         prof_begin(c, KG_FLUX);
-        launch_flux(s, d, g, c->mp, fb, c->H, S, c->met, c->bg, c->gflx, c->mflx);
+        tune_mark(c, TS_FLUX_I + d);
+        launch_flux(s, d, c->plan.v[TS_FLUX_I + d], g, c->mp, fb, c->H, S, c->met, c->bg, c->gflx, c->mflx);
+        tune_mark(c, TS_FLUX_I + d);
         prof_end(c, KG_FLUX);
     }
 
     prof_begin(c, KG_UPDATE);
     Box ctb = mkbox(g.is, g.ie + 1, g.js, g.je + 1, g.ks, g.ke + 1);
-    launch_update(s, g, c->mp, face_box(g), ctb, eb, act_box(g), dt_box(g), c->clk, S, Sn, c->gflx,
-                  c->mflx, c->met, c->bg);
+    tune_mark(c, TS_UPDATE);
+    launch_update(s, c->plan.v[TS_UPDATE], g, c->mp, face_box(g), ctb, eb, act_box(g), dt_box(g), c->clk,
+                  S, Sn, c->gflx, c->mflx, c->met, c->bg);
+    tune_mark(c, TS_UPDATE);
     prof_end(c, KG_UPDATE);
 
     launch_finalize(s, c->mp, c->clk, finmode);
@@ -878,21 +906,17 @@ extern "C" int gam_ens_create(gam_ens **out, int n, gam_ctx *const *members)
         set_launch_recorder(nullptr);
         c->prof = prof;
     }
-    // phases must line up across members
+    // every member issues the same number of launches per step; at each
+    // position the members are grouped by kernel (their launch plans may
+    // pick different shapes), one batched launch per group
     for (int r = 0; r < 2; r++)
-        for (int q = 1; q < n; q++) {
-            bool ok = rec[r][q].size() == rec[r][0].size();
-            for (size_t p = 0; ok && p < rec[r][0].size(); p++) {
-                const EnsLaunch &a = rec[r][0][p], &b = rec[r][q][p];
-                ok = a.launch == b.launch && a.block.x == b.block.x && a.block.y == b.block.y &&
-                     a.block.z == b.block.z && a.rec.size() == b.rec.size();
-            }
-            if (!ok) {
+        for (int q = 1; q < n; q++)
+            if (rec[r][q].size() != rec[r][0].size()) {
                 gam_ens_destroy(e);
                 return fail(GAM_E_ARG, "gam_ens_create: member %d issues a different kernel sequence", q);
             }
-        }
-    // device layout: per role and phase an EnsHdr and n records, 256-aligned
+    // device layout: per batched launch an EnsHdr, its owner table and the
+    // group's argument records, each 256-aligned
     std::vector<unsigned char> host;
     auto put = [&](const void *p, size_t bytes) -> size_t {
         const size_t off = (host.size() + 255) / 256 * 256;
@@ -902,36 +926,58 @@ extern "C" int gam_ens_create(gam_ens **out, int n, gam_ctx *const *members)
     };
     for (int r = 0; r < 2; r++) {
         for (size_t p = 0; p < rec[r][0].size(); p++) {
-            EnsHdr h;
-            memset(&h, 0, sizeof(h));
-            h.n = (unsigned)n;
-            unsigned long tot = 0;
+            std::vector<std::vector<int>> groups;
             for (int q = 0; q < n; q++) {
                 const EnsLaunch &L = rec[r][q][p];
-                h.wg0[q] = (unsigned)tot;
-                h.gx[q] = L.grid.x;
-                h.gy[q] = L.grid.y;
-                h.gz[q] = L.grid.z;
-                tot += ((unsigned long)L.grid.x * L.grid.y * L.grid.z + 7) / 8 * 8;
+                size_t gi = 0;
+                for (; gi < groups.size(); gi++) {
+                    const EnsLaunch &F = rec[r][groups[gi][0]][p];
+                    if (F.launch == L.launch && F.block.x == L.block.x && F.block.y == L.block.y &&
+                        F.block.z == L.block.z && F.rec.size() == L.rec.size())
+                        break;
+                }
+                if (gi == groups.size()) groups.emplace_back();
+                groups[gi].push_back(q);
             }
-            if (tot > 0x7fffffffUL) {
-                gam_ens_destroy(e);
-                return fail(GAM_E_ARG, "gam_ens_create: batched grid too large");
+            for (const std::vector<int> &grp : groups) {
+                const int gn = (int)grp.size();
+                EnsHdr h;
+                memset(&h, 0, sizeof(h));
+                h.n = (unsigned)gn;
+                unsigned long tot = 0;
+                for (int q = 0; q < gn; q++) {
+                    const EnsLaunch &L = rec[r][grp[q]][p];
+                    h.wg0[q] = (unsigned)tot;
+                    h.gx[q] = L.grid.x;
+                    h.gy[q] = L.grid.y;
+                    h.gz[q] = L.grid.z;
+                    tot += ((unsigned long)L.grid.x * L.grid.y * L.grid.z + 7) / 8 * 8;
+                }
+                if (tot > 0x7fffffffUL) {
+                    gam_ens_destroy(e);
+                    return fail(GAM_E_ARG, "gam_ens_create: batched grid too large");
+                }
+                h.wg0[gn] = (unsigned)tot;
+                std::vector<unsigned short> own(tot / 8);
+                for (int q = 0; q < gn; q++)
+                    for (unsigned w = h.wg0[q]; w < h.wg0[q + 1]; w += 8) own[w / 8] = (unsigned short)q;
+                const EnsLaunch &F = rec[r][grp[0]][p];
+                EnsPhase ph;
+                ph.launch = F.launch;
+                ph.block = F.block;
+                ph.nwg = (unsigned)tot;
+                ph.hdr_off = put(&h, sizeof(h));
+                host.resize(ph.hdr_off + sizeof(h) + own.size() * sizeof(unsigned short));
+                memcpy(host.data() + ph.hdr_off + sizeof(h), own.data(), own.size() * sizeof(unsigned short));
+                const size_t rs = F.rec.size();
+                ph.rec_off = put(F.rec.data(), rs);
+                for (int q = 1; q < gn; q++) {
+                    const size_t at = host.size();
+                    host.resize(at + rs);
+                    memcpy(host.data() + at, rec[r][grp[q]][p].rec.data(), rs);
+                }
+                e->ph[r].push_back(ph);
             }
-            h.wg0[n] = (unsigned)tot;
-            EnsPhase ph;
-            ph.launch = rec[r][0][p].launch;
-            ph.block = rec[r][0][p].block;
-            ph.nwg = (unsigned)tot;
-            ph.hdr_off = put(&h, sizeof(h));
-            const size_t rs = rec[r][0][p].rec.size();
-            ph.rec_off = put(rec[r][0][p].rec.data(), rs);
-            for (int q = 1; q < n; q++) {
-                const size_t at = host.size();
-                host.resize(at + rs);
-                memcpy(host.data() + at, rec[r][q][p].rec.data(), rs);
-            }
-            e->ph[r].push_back(ph);
         }
     }
     hipError_t err = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
@@ -1026,5 +1072,117 @@ extern "C" int gam_ens_launches_per_step(gam_ens *e, int *n)
 {
     if (!e || !n) return fail(GAM_E_ARG, "gam_ens_launches_per_step: NULL argument");
     *n = (int)e->ph[0].size();
+    return GAM_OK;
+}
+
+// ------------------------------------------------------------ launch plans
+static const char *slot_names[GAM_TUNE_SLOTS] = {"predict", "emf_i", "emf_j", "emf_k",
+                                                 "flux_i", "flux_j", "flux_k", "update"};
+static_assert(GAM_TUNE_SLOTS == TS_COUNT, "slot count");
+
+extern "C" const char *gam_tune_slot_name(int slot)
+{
+    if (slot < 0 || slot >= GAM_TUNE_SLOTS) return "";
+    return slot_names[slot];
+}
+
+extern "C" int gam_tune_candidates(int slot, int *values)
+{
+    int v[8];
+    const int n = plan_candidates(slot, v);
+    if (values) for (int q = 0; q < n; q++) values[q] = v[q];
+    return n;
+}
+
+extern "C" int gam_get_plan(gam_ctx *c, int *plan)
+{
+    if (!c || !plan) return fail(GAM_E_ARG, "gam_get_plan: NULL argument");
+    for (int q = 0; q < TS_COUNT; q++) plan[q] = c->plan.v[q];
+    return GAM_OK;
+}
+
+extern "C" int gam_set_plan(gam_ctx *c, const int *plan)
+{
+    if (!c || !plan) return fail(GAM_E_ARG, "gam_set_plan: NULL argument");
+    for (int q = 0; q < TS_COUNT; q++) {
+        int v[8];
+        const int n = plan_candidates(q, v);
+        bool ok = false;
+        for (int r = 0; r < n; r++) ok = ok || v[r] == plan[q];
+        if (!ok) return fail(GAM_E_ARG, "gam_set_plan: %d is not a shape of slot %s", plan[q], slot_names[q]);
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int q = 0; q < TS_COUNT; q++) c->plan.v[q] = plan[q];
+    return GAM_OK;
+}
+
+// Time one slot's launches over reps steps (after one untimed step).
+static int tune_time(gam_ctx *c, int slot, int reps, double *ms)
+{
+    c->tune_slot = -1;
+    enq_step(c);
+    c->tune_slot = slot;
+    c->ntev = 0;
+    for (int r = 0; r < reps; r++) enq_step(c);
+    c->tune_slot = -1;
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double tot = 0.0;
+    for (int q = 0; q + 1 < c->ntev; q += 2) {
+        float f = 0.0f;
+        HIPCHK(hipEventElapsedTime(&f, c->tev[q], c->tev[q + 1]));
+        tot += f;
+    }
+    *ms = tot;
+    return GAM_OK;
+}
+
+extern "C" int gam_autotune(gam_ctx *c, int reps)
+{
+    int rc = check_ready(c, "gam_autotune");
+    if (rc) return rc;
+    if (reps < 1 || reps > GAM_TUNE_MAXREPS) return fail(GAM_E_ARG, "gam_autotune: reps must be in 1..%d", GAM_TUNE_MAXREPS);
+    // snapshot State, oState and the clock; tuning takes real steps
+    const size_t bytes = (size_t)S_NPL * c->g.N3 * sizeof(double);
+    double *save = nullptr;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMalloc((void **)&save, 2 * bytes));
+    Clock ck;
+    const int cur0 = c->cur;
+    const int prof = c->prof;
+    hipError_t e = hipMemcpyAsync(save, c->st[0], bytes, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(save + bytes / sizeof(double), c->st[1], bytes, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(&ck, c->clk, sizeof(Clock), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+        hipFree(save);
+        return fail(GAM_E_HIP, "gam_autotune: %s", hipGetErrorString(e));
+    }
+    c->prof = 0;
+    for (int slot = 0; slot < TS_COUNT && rc == GAM_OK; slot++) {
+        int v[8];
+        const int n = plan_candidates(slot, v);
+        double best = 1e300;
+        int bestv = c->plan.v[slot];
+        for (int q = 0; q < n && rc == GAM_OK; q++) {
+            c->plan.v[slot] = v[q];
+            double ms = 0.0;
+            rc = tune_time(c, slot, reps, &ms);
+            if (rc == GAM_OK && ms < best) { best = ms; bestv = v[q]; }
+        }
+        c->plan.v[slot] = bestv;
+    }
+    c->prof = prof;
+    c->tune_slot = -1;
+    // put the run back where it was
+    hipStreamSynchronize(c->stream);
+    e = hipMemcpyAsync(c->st[0], save, bytes, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->st[1], save + bytes / sizeof(double), bytes, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->clk, &ck, sizeof(Clock), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    c->cur = cur0;
+    hipFree(save);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(GAM_E_HIP, "gam_autotune: %s", hipGetErrorString(e));
     return GAM_OK;
 }

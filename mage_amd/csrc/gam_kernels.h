@@ -23,7 +23,8 @@ __host__ __device__ inline Pack<A0, A...> make_pack(A0 a0, A... a)
 
 // Batched launch over the members of an ensemble: member m owns workgroups
 // wg0[m] .. wg0[m+1]-1 (wg0[m] a multiple of 8) of which the first
-// gx*gy*gz run its own grid.
+// gx*gy*gz run its own grid.  In device memory the header is followed by
+// an unsigned short per run of 8 workgroups: the member that owns it.
 #define ENS_MAX 256
 struct EnsHdr {
     unsigned n;
@@ -43,6 +44,20 @@ struct EnsLaunch {
 // appended to *r instead (nothing is issued); nullptr restores launching.
 void set_launch_recorder(std::vector<EnsLaunch> *r);
 
+// Launch plan: the shape of every tunable launch of the step.  Values
+// 1, 2, 3 are the Tile row orders (ax1: j fastest, k fastest, j-slabs);
+// PLAN_LDS selects the LDS-staged form of the j/k interface fluxes (a
+// 64 x 4 face tile with its stencil halo rows in LDS).
+enum { TS_PREDICT = 0, TS_EMF_I, TS_EMF_J, TS_EMF_K, TS_FLUX_I, TS_FLUX_J, TS_FLUX_K, TS_UPDATE,
+       TS_COUNT };
+#define PLAN_LDS 4
+struct Plan {
+    int v[TS_COUNT];
+};
+Plan default_plan();
+// candidate values of a slot; returns their number
+int plan_candidates(int slot, int *vals);
+
 // k_finalize modes
 enum {
     FIN_ADVANCE = 1,   // State/oState times after applyUpdate
@@ -60,17 +75,17 @@ struct WindDev {
     double xW, yW, zW;
 };
 
-void launch_predict(hipStream_t s, const Geo &g, const MParm &m, const Box &fb, const Clock *clk,
+void launch_predict(hipStream_t s, int ax, const Geo &g, const MParm &m, const Box &fb, const Clock *clk,
                     const double *S, const double *O, const double *met, double *H);
 // Mom2Face of the three face directions: Vf planes [3*dT1 + comp]
 void launch_vface(hipStream_t s, const Geo &g, const MParm &m, const double *H, const double *met,
                   double *Vf);
-void launch_emf(hipStream_t s, int eD, const Geo &g, const MParm &m, const Box &b, const Clock *clk,
+void launch_emf(hipStream_t s, int eD, int ax, const Geo &g, const MParm &m, const Box &b, const Clock *clk,
                 const double *H, const double *S, const double *Vf, const double *met,
                 const double *bg, double *E);
-void launch_flux(hipStream_t s, int d, const Geo &g, const MParm &m, const Box &b, const double *H,
+void launch_flux(hipStream_t s, int d, int form, const Geo &g, const MParm &m, const Box &b, const double *H,
                  const double *S, const double *met, const double *bg, double *gf, double *mf);
-void launch_update(hipStream_t s, const Geo &g, const MParm &m, const Box &fb, const Box &ctb,
+void launch_update(hipStream_t s, int ax, const Geo &g, const MParm &m, const Box &fb, const Box &ctb,
                    const Box *eboxes, const Box &ab, const Box &dtb, Clock *clk, const double *So,
                    double *Sn, const double *gf, const double *mf, const double *met, const double *bg);
 void launch_calc_dt(hipStream_t s, const Geo &g, const MParm &m, const Box &dtb, Clock *clk,

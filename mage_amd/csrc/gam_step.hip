@@ -117,7 +117,8 @@ static inline dim3 grid_of(const Box &b)
 // context); k_ens<F> runs F for many contexts in one launch: the
 // concatenated grid of the members, each member's workgroups starting at a
 // multiple of 8 (so a member's chunk-to-XCD mapping is the same as in its
-// own launch) and each with its own argument record.
+// own launch) and each with its own argument record.  A table after the
+// header names the member of every run of 8 workgroups.
 // ====================================================================
 template <class F, class... A>
 __global__ __launch_bounds__(256) void k_one(A... a)
@@ -125,13 +126,31 @@ __global__ __launch_bounds__(256) void k_one(A... a)
     F{}(make_uint3(blockIdx.x, blockIdx.y, blockIdx.z), a...);
 }
 
-template <class F, class... B>
-__device__ __forceinline__ void pcall(const Pack<> &, uint3 bi, B... b)
+// Pointers read from an argument record are generic pointers to the
+// compiler; every one of them addresses device (global) memory, and saying
+// so lets it use global instead of flat loads and stores.
+template <class T>
+__device__ __forceinline__ const T &in_global(const T &v)
 {
-    F{}(bi, b...);
+    return v;
+}
+template <class T>
+__device__ __forceinline__ T *in_global(T *const &p)
+{
+    __attribute__((address_space(1))) T *q = (__attribute__((address_space(1))) T *)p;
+    asm("" : "+s"(q));   // keeps the cast from folding back to a generic pointer
+    return (T *)q;
+}
+
+// Call F with the record's fields, in order (passed by reference into the
+// record, so that arrays in them are read where they lie).
+template <class F, class... B>
+__device__ __forceinline__ void pcall(const Pack<> &, uint3 bi, const B &...b)
+{
+    F{}(bi, in_global(b)...);
 }
 template <class F, class A0, class... A, class... B>
-__device__ __forceinline__ void pcall(const Pack<A0, A...> &p, uint3 bi, B... b)
+__device__ __forceinline__ void pcall(const Pack<A0, A...> &p, uint3 bi, const B &...b)
 {
     pcall<F>(p.r, bi, b..., p.a0);
 }
@@ -141,12 +160,8 @@ __global__ __launch_bounds__(256) void k_ens(const EnsHdr *__restrict__ h,
                                              const Pack<A...> *__restrict__ rec)
 {
     const unsigned w = blockIdx.x;
-    unsigned lo = 0, hi = h->n;   // wg0[lo] <= w < wg0[hi]
-    while (hi - lo > 1) {
-        const unsigned mid = (lo + hi) >> 1;
-        if (h->wg0[mid] <= w) lo = mid;
-        else hi = mid;
-    }
+    // owner of each run of 8 workgroups, stored right after the header
+    const unsigned lo = reinterpret_cast<const unsigned short *>(h + 1)[w >> 3];
     const unsigned gx = h->gx[lo], gy = h->gy[lo], gz = h->gz[lo];
     unsigned l = w - h->wg0[lo];
     if (l >= gx * gy * gz) return;   // padding of the member's range
@@ -1096,7 +1111,7 @@ struct GhostBoxes {
 
 struct k_bflux2fld_ghosts {
     __device__ __forceinline__ void operator()(const uint3 bi, Geo g,
-                                               GhostBoxes gb, const double *__restrict__ mf,
+                                               const GhostBoxes &gb, const double *__restrict__ mf,
                                                double *__restrict__ B,
                                                const double *__restrict__ met) const
     {
@@ -1510,10 +1525,34 @@ __global__ void k_mask_e(Geo g, Box all, EdgeBoxes eb, double *__restrict__ E)
 // ====================================================================
 #define LAUNCH(kern, grid, block, stream, ...) hipLaunchKernelGGL(kern, grid, block, 0, stream, __VA_ARGS__)
 
-void launch_predict(hipStream_t s, const Geo &g, const MParm &m, const Box &fb, const Clock *clk,
+Plan default_plan()
+{
+    Plan p;
+    p.v[TS_PREDICT] = 3;
+    p.v[TS_EMF_I] = 3;    // stencils along j and k
+    p.v[TS_EMF_J] = 2;    // along i and k
+    p.v[TS_EMF_K] = 1;    // along i and j
+    p.v[TS_FLUX_I] = 1;
+    p.v[TS_FLUX_J] = PLAN_LDS;
+    p.v[TS_FLUX_K] = PLAN_LDS;
+    p.v[TS_UPDATE] = 3;
+    return p;
+}
+
+int plan_candidates(int slot, int *vals)
+{
+    int n = 0;
+    if (slot < 0 || slot >= TS_COUNT) return 0;
+    const int maxax = (slot == TS_FLUX_I) ? 2 : 3;
+    for (int a = 1; a <= maxax; a++) vals[n++] = a;
+    if (slot == TS_FLUX_J || slot == TS_FLUX_K) vals[n++] = PLAN_LDS;
+    return n;
+}
+
+void launch_predict(hipStream_t s, int ax, const Geo &g, const MParm &m, const Box &fb, const Clock *clk,
                     const double *S, const double *O, const double *met, double *H)
 {
-    const Tile t = tile_of(fb, 3);
+    const Tile t = tile_of(fb, ax);
     launch_f<k_predict>(tgrid(t), dim3(BX, BY), s, g, m, t, clk, S, O, met, H);
 }
 
@@ -1526,97 +1565,97 @@ void launch_vface(hipStream_t s, const Geo &g, const MParm &m, const double *H, 
 }
 
 template <int ED, int RM, bool BG, bool BO>
-static void emf_t(hipStream_t s, const Geo &g, const MParm &m, const Box &b, const Clock *clk,
+static void emf_t(hipStream_t s, int ax, const Geo &g, const MParm &m, const Box &b, const Clock *clk,
                   const double *H, const double *S, const double *Vf, const double *met,
                   const double *bg, double *E)
 {
-    // stencils run along T2 and T1: rows along j for K edges, along k for J
-    // edges (both stencils along i or k), j-slabs for I edges (j and k)
-    const Tile t = tile_of(b, ED == 2 ? 1 : (ED == 0 ? 3 : 2));
+    const Tile t = tile_of(b, ax);
     launch_f<k_emf<ED, RM, BG, BO>>(tgrid(t), dim3(BX, BY), s, g, m, t, clk, H, S, Vf, met, bg, E);
 }
 
 template <int ED>
-static void emf_d(hipStream_t s, const Geo &g, const MParm &m, const Box &b, const Clock *clk,
+static void emf_d(hipStream_t s, int ax, const Geo &g, const MParm &m, const Box &b, const Clock *clk,
                   const double *H, const double *S, const double *Vf, const double *met,
                   const double *bg, double *E)
 {
     const int sel = (m.rmeth ? 4 : 0) + (m.doBG ? 2 : 0) + (m.doBoris ? 1 : 0);
     switch (sel) {
-    case 0: emf_t<ED, 0, false, false>(s, g, m, b, clk, H, S, Vf, met, bg, E); break;
-    case 1: emf_t<ED, 0, false, true>(s, g, m, b, clk, H, S, Vf, met, bg, E); break;
-    case 2: emf_t<ED, 0, true, false>(s, g, m, b, clk, H, S, Vf, met, bg, E); break;
-    case 3: emf_t<ED, 0, true, true>(s, g, m, b, clk, H, S, Vf, met, bg, E); break;
-    case 4: emf_t<ED, 1, false, false>(s, g, m, b, clk, H, S, Vf, met, bg, E); break;
-    case 5: emf_t<ED, 1, false, true>(s, g, m, b, clk, H, S, Vf, met, bg, E); break;
-    case 6: emf_t<ED, 1, true, false>(s, g, m, b, clk, H, S, Vf, met, bg, E); break;
-    default: emf_t<ED, 1, true, true>(s, g, m, b, clk, H, S, Vf, met, bg, E); break;
+    case 0: emf_t<ED, 0, false, false>(s, ax, g, m, b, clk, H, S, Vf, met, bg, E); break;
+    case 1: emf_t<ED, 0, false, true>(s, ax, g, m, b, clk, H, S, Vf, met, bg, E); break;
+    case 2: emf_t<ED, 0, true, false>(s, ax, g, m, b, clk, H, S, Vf, met, bg, E); break;
+    case 3: emf_t<ED, 0, true, true>(s, ax, g, m, b, clk, H, S, Vf, met, bg, E); break;
+    case 4: emf_t<ED, 1, false, false>(s, ax, g, m, b, clk, H, S, Vf, met, bg, E); break;
+    case 5: emf_t<ED, 1, false, true>(s, ax, g, m, b, clk, H, S, Vf, met, bg, E); break;
+    case 6: emf_t<ED, 1, true, false>(s, ax, g, m, b, clk, H, S, Vf, met, bg, E); break;
+    default: emf_t<ED, 1, true, true>(s, ax, g, m, b, clk, H, S, Vf, met, bg, E); break;
     }
 }
 
-void launch_emf(hipStream_t s, int eD, const Geo &g, const MParm &m, const Box &b, const Clock *clk,
+void launch_emf(hipStream_t s, int eD, int ax, const Geo &g, const MParm &m, const Box &b, const Clock *clk,
                 const double *H, const double *S, const double *Vf, const double *met,
                 const double *bg, double *E)
 {
-    if (eD == 0) emf_d<0>(s, g, m, b, clk, H, S, Vf, met, bg, E);
-    else if (eD == 1) emf_d<1>(s, g, m, b, clk, H, S, Vf, met, bg, E);
-    else emf_d<2>(s, g, m, b, clk, H, S, Vf, met, bg, E);
+    if (eD == 0) emf_d<0>(s, ax, g, m, b, clk, H, S, Vf, met, bg, E);
+    else if (eD == 1) emf_d<1>(s, ax, g, m, b, clk, H, S, Vf, met, bg, E);
+    else emf_d<2>(s, ax, g, m, b, clk, H, S, Vf, met, bg, E);
 }
 
 template <int D, int RM, bool BG, bool BO>
-static void flux_t(hipStream_t s, const Geo &g, const MParm &m, const Box &b, const double *H,
+static void flux_t(hipStream_t s, int form, const Geo &g, const MParm &m, const Box &b, const double *H,
                    const double *S, const double *met, const double *bg, double *gf, double *mf)
 {
-    if constexpr (D == 0) {
-        const Tile t = tile_of(b, 1);
-        launch_f<k_flux<D, RM, BG, BO>>(tgrid(t), dim3(BX, BY), s, g, m, t, H, S, met, bg, gf, mf);
-        return;
-    } else {
-    FluxTile t;
-    t.i0 = b.i0; t.i1 = b.i1;
-    if (D == 1) { t.a0 = b.j0; t.a1 = b.j1; t.b0 = b.k0; t.b1 = b.k1; }
-    else { t.a0 = b.k0; t.a1 = b.k1; t.b0 = b.j0; t.b1 = b.j1; }
-    t.nti = (t.i1 - t.i0 + BX) / BX;
-    t.nta = (t.a1 - t.a0 + BY) / BY;
-    t.nb = t.nti * t.nta * (t.b1 - t.b0 + 1);
-    launch_f<k_flux_s<D, RM, BG, BO>>(dim3((unsigned)(((t.nb + 7) / 8) * 8)), dim3(BX, BY), s, g, m, t,
-           H, S, met, bg, gf, mf);
+    if constexpr (D != 0) {
+        if (form == PLAN_LDS) {
+            FluxTile t;
+            t.i0 = b.i0;
+            t.i1 = b.i1;
+            if (D == 1) { t.a0 = b.j0; t.a1 = b.j1; t.b0 = b.k0; t.b1 = b.k1; }
+            else { t.a0 = b.k0; t.a1 = b.k1; t.b0 = b.j0; t.b1 = b.j1; }
+            t.nti = (t.i1 - t.i0 + BX) / BX;
+            t.nta = (t.a1 - t.a0 + BY) / BY;
+            t.nb = t.nti * t.nta * (t.b1 - t.b0 + 1);
+            launch_f<k_flux_s<D, RM, BG, BO>>(dim3((unsigned)(((t.nb + 7) / 8) * 8)), dim3(BX, BY), s, g, m,
+                                              t, H, S, met, bg, gf, mf);
+            return;
+        }
     }
+    const Tile t = tile_of(b, form == PLAN_LDS ? 1 : form);
+    launch_f<k_flux<D, RM, BG, BO>>(tgrid(t), dim3(BX, BY), s, g, m, t, H, S, met, bg, gf, mf);
 }
 
 template <int D>
-static void flux_d(hipStream_t s, const Geo &g, const MParm &m, const Box &b, const double *H,
+static void flux_d(hipStream_t s, int form, const Geo &g, const MParm &m, const Box &b, const double *H,
                    const double *S, const double *met, const double *bg, double *gf, double *mf)
 {
     const int sel = (m.rmeth ? 4 : 0) + (m.doBG ? 2 : 0) + (m.doBoris ? 1 : 0);
     switch (sel) {
-    case 0: flux_t<D, 0, false, false>(s, g, m, b, H, S, met, bg, gf, mf); break;
-    case 1: flux_t<D, 0, false, true>(s, g, m, b, H, S, met, bg, gf, mf); break;
-    case 2: flux_t<D, 0, true, false>(s, g, m, b, H, S, met, bg, gf, mf); break;
-    case 3: flux_t<D, 0, true, true>(s, g, m, b, H, S, met, bg, gf, mf); break;
-    case 4: flux_t<D, 1, false, false>(s, g, m, b, H, S, met, bg, gf, mf); break;
-    case 5: flux_t<D, 1, false, true>(s, g, m, b, H, S, met, bg, gf, mf); break;
-    case 6: flux_t<D, 1, true, false>(s, g, m, b, H, S, met, bg, gf, mf); break;
-    default: flux_t<D, 1, true, true>(s, g, m, b, H, S, met, bg, gf, mf); break;
+    case 0: flux_t<D, 0, false, false>(s, form, g, m, b, H, S, met, bg, gf, mf); break;
+    case 1: flux_t<D, 0, false, true>(s, form, g, m, b, H, S, met, bg, gf, mf); break;
+    case 2: flux_t<D, 0, true, false>(s, form, g, m, b, H, S, met, bg, gf, mf); break;
+    case 3: flux_t<D, 0, true, true>(s, form, g, m, b, H, S, met, bg, gf, mf); break;
+    case 4: flux_t<D, 1, false, false>(s, form, g, m, b, H, S, met, bg, gf, mf); break;
+    case 5: flux_t<D, 1, false, true>(s, form, g, m, b, H, S, met, bg, gf, mf); break;
+    case 6: flux_t<D, 1, true, false>(s, form, g, m, b, H, S, met, bg, gf, mf); break;
+    default: flux_t<D, 1, true, true>(s, form, g, m, b, H, S, met, bg, gf, mf); break;
     }
 }
 
-void launch_flux(hipStream_t s, int d, const Geo &g, const MParm &m, const Box &b, const double *H,
+void launch_flux(hipStream_t s, int d, int form, const Geo &g, const MParm &m, const Box &b, const double *H,
                  const double *S, const double *met, const double *bg, double *gf, double *mf)
 {
-    if (d == 0) flux_d<0>(s, g, m, b, H, S, met, bg, gf, mf);
-    else if (d == 1) flux_d<1>(s, g, m, b, H, S, met, bg, gf, mf);
-    else flux_d<2>(s, g, m, b, H, S, met, bg, gf, mf);
+    if (d == 0) flux_d<0>(s, form, g, m, b, H, S, met, bg, gf, mf);
+    else if (d == 1) flux_d<1>(s, form, g, m, b, H, S, met, bg, gf, mf);
+    else flux_d<2>(s, form, g, m, b, H, S, met, bg, gf, mf);
 }
 
-void launch_update(hipStream_t s, const Geo &g, const MParm &m, const Box &fb, const Box &ctb,
+void launch_update(hipStream_t s, int ax, const Geo &g, const MParm &m, const Box &fb, const Box &ctb,
                    const Box *eboxes, const Box &ab, const Box &dtb, Clock *clk, const double *So,
                    double *Sn, const double *gf, const double *mf, const double *met, const double *bg)
 {
     EdgeBoxes eb;
     for (int d = 0; d < 3; d++) eb.e[d] = eboxes[d];
     const int sel = (m.doBG ? 2 : 0) + (m.doBoris ? 1 : 0);
-    const Tile t = tile_of(fb, 3);
+    const Tile t = tile_of(fb, ax);
     switch (sel) {
     case 0: launch_f<k_update<false, false>>(tgrid(t), dim3(BX, BY), s, g, m, t, ctb, eb, ab, dtb, clk, So, Sn, gf, mf, met, bg); break;
     case 1: launch_f<k_update<false, true>>(tgrid(t), dim3(BX, BY), s, g, m, t, ctb, eb, ab, dtb, clk, So, Sn, gf, mf, met, bg); break;

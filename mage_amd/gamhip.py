@@ -32,10 +32,12 @@ EXPORTS = (
     "gam_init_run", "gam_advance_mhd", "gam_calc_dt", "gam_enforce_bcs", "gam_bflux2fld",
     "gam_step", "gam_run", "gam_advance_model", "gam_sync", "gam_divb", "gam_time_steps", "gam_set_profiling",
     "gam_kernel_times", "gam_kernel_group_name", "gam_get_solver",
+    "gam_autotune", "gam_get_plan", "gam_set_plan", "gam_tune_slot_name", "gam_tune_candidates",
     "gam_ens_create", "gam_ens_run", "gam_ens_time_steps", "gam_ens_launches_per_step", "gam_ens_destroy",
 )
 
 DP = C.POINTER(C.c_double)
+TUNE_SLOTS = 8
 
 
 class GamError(RuntimeError):
@@ -112,6 +114,12 @@ def load():
     L.gam_set_profiling.argtypes = [vp, C.c_int]
     L.gam_kernel_times.argtypes = [vp, DP, C.POINTER(C.c_int)]
     L.gam_get_solver.argtypes = [vp, C.c_int, DP]
+    L.gam_autotune.argtypes = [vp, C.c_int]
+    L.gam_get_plan.argtypes = [vp, C.POINTER(C.c_int)]
+    L.gam_set_plan.argtypes = [vp, C.POINTER(C.c_int)]
+    L.gam_tune_slot_name.restype = C.c_char_p
+    L.gam_tune_slot_name.argtypes = [C.c_int]
+    L.gam_tune_candidates.argtypes = [C.c_int, C.POINTER(C.c_int)]
     L.gam_ens_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(vp)]
     L.gam_ens_run.argtypes = [vp, C.c_int]
     L.gam_ens_time_steps.argtypes = [vp, C.c_int, DP]
@@ -325,6 +333,23 @@ class GamHip:
 
     def set_profiling(self, on):
         self._chk(self.L.gam_set_profiling(self.ctx, 1 if on else 0))
+
+    # ---- launch plans
+    def autotune(self, reps=3):
+        """Time every launch shape of every tunable slot on this grid and keep
+        the fastest (the run's state is left as it was); returns the plan."""
+        self._chk(self.L.gam_autotune(self.ctx, int(reps)))
+        return self.get_plan()
+
+    def get_plan(self):
+        v = (C.c_int * TUNE_SLOTS)()
+        self._chk(self.L.gam_get_plan(self.ctx, v))
+        return {self.L.gam_tune_slot_name(q).decode(): v[q] for q in range(TUNE_SLOTS)}
+
+    def set_plan(self, plan):
+        names = [self.L.gam_tune_slot_name(q).decode() for q in range(TUNE_SLOTS)]
+        v = (C.c_int * TUNE_SLOTS)(*[int(plan[n]) for n in names])
+        self._chk(self.L.gam_set_plan(self.ctx, v))
 
     def kernel_times(self):
         ms = (C.c_double * 8)()

@@ -78,3 +78,20 @@ def test_ensemble_argument_checks():
     ms = C.c_double()
     assert L.gam_ens_time_steps(None, 1, C.byref(ms)) == -1
     assert L.gam_ens_destroy(None) == 0
+
+
+def test_plan_entry_points_without_gpu():
+    """Slot names and candidates are host tables; plan calls check arguments."""
+    L = gamhip.load()
+    names = [L.gam_tune_slot_name(q).decode() for q in range(gamhip.TUNE_SLOTS)]
+    assert names == ["predict", "emf_i", "emf_j", "emf_k", "flux_i", "flux_j", "flux_k", "update"]
+    assert L.gam_tune_slot_name(gamhip.TUNE_SLOTS).decode() == ""
+    v = (C.c_int * 8)()
+    assert L.gam_tune_candidates(0, v) == 3 and list(v[:3]) == [1, 2, 3]
+    assert L.gam_tune_candidates(4, v) == 2
+    assert L.gam_tune_candidates(5, v) == 4 and v[3] == 4
+    assert L.gam_tune_candidates(8, v) == 0
+    plan = (C.c_int * 8)()
+    assert L.gam_get_plan(None, plan) == -1
+    assert L.gam_set_plan(None, plan) == -1
+    assert L.gam_autotune(None, 2) == -1
