@@ -62,8 +62,10 @@ def setup_shell(gamhip, pb, N, rmeth="7UP", full_grid=False):
 
 
 def flux_traffic(path=PMC_SUMMARY):
-    """Calibrated HBM bytes per k_flux dispatch from the committed PMC
-    summary (mean over the three directions), or None without one."""
+    """Calibrated HBM bytes per interface-flux dispatch from the committed
+    PMC summary (mean over the three directions of the launch shapes the
+    step ran; shapes launched only while autotuning have few calls), or
+    None without one."""
     if not os.path.exists(path):
         return None
     vals = []
@@ -71,7 +73,8 @@ def flux_traffic(path=PMC_SUMMARY):
         cols = [c.strip() for c in line.strip().strip("|").split("|")]
         if len(cols) >= 7 and cols[0].startswith("`k_flux"):
             try:
-                vals.append(float(cols[5]))
+                if int(cols[1]) >= 50:
+                    vals.append(float(cols[5]))
             except ValueError:
                 pass
     return sum(vals) / len(vals) if vals else None
@@ -122,6 +125,24 @@ def ensemble_gain(gamhip, pb, n=32, steps=100, warmup=2, tune=True):
             "distinct_plans": len({tuple(sorted(p.items())) for p in plans}) if plans else 1}
 
 
+def full_size(gamhip, pb, N=(256, 256, 384), steps=500, warmup=5, tune=True):
+    """Forward steps of the full-size shell (same physics as the 192x192x256
+    workload), HIP-event timed, against the step's algorithmic bytes."""
+    g, p, grid, st, wind = setup_shell(gamhip, pb, N)
+    del st, grid
+    plan = g.autotune(reps=2) if tune else g.get_plan()
+    g.run(warmup)
+    ms = g.time_steps(steps)
+    g.close()
+    cells = N[0] * N[1] * N[2]
+    value = cells * steps / (ms * 1e-3)
+    return {"workload": "%dx%dx%d warped shell (25.2M cells), dipole B0, inflow, 7UP, Boris, %d forward steps"
+                        % (N + (steps,)),
+            "cells": cells, "steps": steps, "ms_per_step": ms / steps, "value": value,
+            "unit": "cell-updates/s", "launch_plan": plan,
+            "step_frac_of_hbm_roofline": STEP_BYTES_PER_CELL * value / 1e9 / HBM_PEAK_GBS}
+
+
 def cpu_baseline(gamhip, pb, min_seconds=12.0, max_steps=400):
     """Time the oracle (C/OpenMP restatement of the reference step) on a
     96x96x128 sample of the same shell workload."""
@@ -160,6 +181,8 @@ def main():
     ap.add_argument("--no-box", action="store_true")
     ap.add_argument("--no-ens", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-full", action="store_true")
+    ap.add_argument("--full-steps", type=int, default=500)
     ap.add_argument("--ens-members", type=int, default=32)
     ap.add_argument("--ens-steps", type=int, default=100)
     args = ap.parse_args()
@@ -225,6 +248,13 @@ def main():
                "step_frac_of_hbm_roofline": 8 * (19 + 14 + 79) * 64 ** 3 * 100 / (msb * 1e-3) / 1e9 / HBM_PEAK_GBS}
         log("box 64^3: %.3f ms/step, %.4e cell-updates/s" % (box["ms_per_step"], box["value"]))
 
+    full = None
+    if not args.no_full:
+        t1 = time.time()
+        full = full_size(gamhip, pb, steps=args.full_steps, tune=not args.no_tune)
+        log("full-size shell: %.3f ms/step, %.4e cell-updates/s (%.1f s)"
+            % (full["ms_per_step"], full["value"], time.time() - t1))
+
     ens = None
     if not args.no_ens:
         t1 = time.time()
@@ -282,6 +312,7 @@ def main():
         },
         "mhd_cell_updates_per_s": mhd_value,
         "box_64cubed": box,
+        "shell_256x256x384": full,
         "ensemble": ens,
         "kernels": kernels,
         "final": {"t": t, "dt": dt, "ts": ts},

@@ -198,7 +198,9 @@ int gam_get_solver(gam_ctx *ctx, int which, double *out);
  * j-rows) and, for the j and k fluxes, 4 = the LDS-staged 64 x 4 face tile.
  * gam_autotune times every candidate of every slot on the context's own
  * grid (reps steps per candidate, hipEvents around that slot's launch),
- * keeps the fastest, and restores State, oState, time and dt as they were.
+ * keeps the fastest (the current shape unless another is more than 1 %
+ * faster), and restores State, oState, time and dt as they were; the
+ * solver intermediates (gam_get_solver) are then those of a tuning step.
  * ------------------------------------------------------------------- */
 #define GAM_TUNE_SLOTS 8
 #define GAM_TUNE_MAXREPS 32

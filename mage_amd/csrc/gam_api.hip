@@ -1159,16 +1159,20 @@ extern "C" int gam_autotune(gam_ctx *c, int reps)
         return fail(GAM_E_HIP, "gam_autotune: %s", hipGetErrorString(e));
     }
     c->prof = 0;
+    // the current shape is timed first and another replaces it only when
+    // more than 1 % faster, so timing noise does not flip near-equal shapes
     for (int slot = 0; slot < TS_COUNT && rc == GAM_OK; slot++) {
         int v[8];
         const int n = plan_candidates(slot, v);
-        double best = 1e300;
         int bestv = c->plan.v[slot];
+        double best = 0.0;
+        rc = tune_time(c, slot, reps, &best);
         for (int q = 0; q < n && rc == GAM_OK; q++) {
+            if (v[q] == bestv) continue;
             c->plan.v[slot] = v[q];
             double ms = 0.0;
             rc = tune_time(c, slot, reps, &ms);
-            if (rc == GAM_OK && ms < best) { best = ms; bestv = v[q]; }
+            if (rc == GAM_OK && ms < 0.99 * best) { best = ms; bestv = v[q]; }
         }
         c->plan.v[slot] = bestv;
     }

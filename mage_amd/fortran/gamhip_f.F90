@@ -21,6 +21,7 @@ module gamhip
                                  GAM_BC_SPHIN = 2, GAM_BC_SPHOUT = 3, GAM_BC_WIND = 4
     integer(c_int), parameter :: GAM_RECON_8CENT = 0, GAM_RECON_7UP = 1
     integer(c_int), parameter :: GAM_STATE = 0, GAM_OSTATE = 1
+    integer(c_int), parameter :: GAM_TUNE_SLOTS = 8
 
     !gam_dims (Grid_T%Nip.., Model_T%nG)
     type, bind(C) :: gam_dims_t
@@ -172,6 +173,44 @@ module gamhip
             type(c_ptr), value :: ctx
             integer(c_int), value :: nsteps
             real(c_double) :: ms
+        end function
+        ! launch plans (tile shapes per grid size)
+        integer(c_int) function gam_autotune(ctx, reps) bind(C, name="gam_autotune")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: reps
+        end function
+        integer(c_int) function gam_get_plan(ctx, plan) bind(C, name="gam_get_plan")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int) :: plan(GAM_TUNE_SLOTS)
+        end function
+        integer(c_int) function gam_set_plan(ctx, plan) bind(C, name="gam_set_plan")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int) :: plan(GAM_TUNE_SLOTS)
+        end function
+        ! ensembles: several contexts stepped as one batched launch per kernel
+        integer(c_int) function gam_ens_create(ens, n, members) bind(C, name="gam_ens_create")
+            import :: c_int, c_ptr
+            type(c_ptr) :: ens
+            integer(c_int), value :: n
+            type(c_ptr) :: members(*)
+        end function
+        integer(c_int) function gam_ens_run(ens, nsteps) bind(C, name="gam_ens_run")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ens
+            integer(c_int), value :: nsteps
+        end function
+        integer(c_int) function gam_ens_time_steps(ens, nsteps, ms) bind(C, name="gam_ens_time_steps")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ens
+            integer(c_int), value :: nsteps
+            real(c_double) :: ms
+        end function
+        integer(c_int) function gam_ens_destroy(ens) bind(C, name="gam_ens_destroy")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ens
         end function
         function c_strlen(s) bind(C, name="strlen")
             import :: c_ptr, c_size_t
