@@ -1,0 +1,70 @@
+/*
+ * gamadj.h -- discrete adjoint of the GAMERA step on the MI355X (libgamhip).
+ *
+ * No counterpart in the reference (it has no adjoint).  The adjoint is of
+ * the step this library runs (stepGamera, gamapp.F90:73-113), built by
+ * hand kernel by kernel in reverse order: boundary copies, the Reynolds
+ * update with Flux2Deltas (mhdgroup.F90:79-121, stress.F90:152-194), the
+ * interface fluxes (reconstruction with the PDM limiter's branch taken in
+ * the forward run, GasKinFlux; stress.F90:204-511, recon.F90:125-301) and
+ * the AB2 predictor (mhdgroup.F90:444-561).
+ *
+ * Scope: hydrodynamic runs (doMHD = 0, doBoris = 0) with periodic,
+ * zero-gradient or sphere boundaries; GAM_E_ARG otherwise.  The magnetic
+ * terms (EMF, CT, magnetic fluxes, Maxwell/Boris updates) and the inflow
+ * boundary are not differentiated yet.
+ *
+ * Objective: J = sum over cells of w(c) * P(c) of the State after nsteps
+ * (P the CellC2P pressure), w a weight array (Ni,Nj,Nk) given by the
+ * caller (e.g. cell volume times a region mask).
+ *
+ * dt: each step keeps the dt of the recorded forward run (the CFL minimum
+ * is not differentiated); gam_adj_replay runs the forward with those dt
+ * values, which is what finite-difference checks of the gradient need.
+ *
+ * Memory: checkpoints of State and oState every ckpt_every steps, and the
+ * states of one checkpoint interval, recomputed forward before its
+ * adjoint steps run backward.
+ */
+#ifndef GAMADJ_H
+#define GAMADJ_H
+
+#include "gamhip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gam_adj gam_adj;
+
+/* An adjoint run of nsteps steps of ctx with checkpoints every ckpt_every
+ * (1 <= ckpt_every <= nsteps). */
+int gam_adj_create(gam_adj **out, gam_ctx *ctx, int nsteps, int ckpt_every);
+int gam_adj_destroy(gam_adj *a);
+/* Objective weights w(Ni,Nj,Nk) (Fortran layout, ghosts included). */
+int gam_adj_set_weights(gam_adj *a, const double *w);
+/* nsteps x stepGamera from the context's current State, oState and clock,
+ * recording checkpoints and every step's clock; *J = objective at the
+ * end.  The context is left at the final state. */
+int gam_adj_forward(gam_adj *a, double *J);
+/* Reverse sweep of the recorded run: gGas = dJ/d(State Gas) and gGasO =
+ * dJ/d(oState Gas) at the start, both (Ni,Nj,Nk,5) (gGasO may be NULL),
+ * with respect to the active cells: each start state's ghost cells are
+ * taken to follow from its interior through the boundary conditions, so
+ * their adjoints are folded into the interior and the ghost entries are 0.
+ * The context is left at the final state of the forward run. */
+int gam_adj_backward(gam_adj *a, double *gGas, double *gGasO);
+/* Forward from the recorded start with the given State Gas (and oState
+ * Gas, or NULL to keep the recorded one), their ghost cells refilled by
+ * the boundary conditions, each step's dt held at the recorded value;
+ * *J = objective at the end.  The context is then put back at the final
+ * state of the recorded run. */
+int gam_adj_replay(gam_adj *a, const double *Gas, const double *oGas, double *J);
+/* Milliseconds of the last gam_adj_forward and gam_adj_backward calls
+ * (HIP events on the context's stream). */
+int gam_adj_times(gam_adj *a, double *fwd_ms, double *bwd_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

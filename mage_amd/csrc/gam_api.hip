@@ -2,6 +2,7 @@
 // and the step driver.  Host code only; kernels live in gam_step.hip and
 // gam_grid.hip.
 #include "gamhip.h"
+#include "gamadj.h"
 #include "gam_kernels.h"
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -1188,5 +1189,263 @@ extern "C" int gam_autotune(gam_ctx *c, int reps)
     hipFree(save);
     if (rc) return rc;
     if (e != hipSuccess) return fail(GAM_E_HIP, "gam_autotune: %s", hipGetErrorString(e));
+    return GAM_OK;
+}
+
+// ------------------------------------------------------------ adjoint
+// Discrete adjoint of the hydrodynamic step (include/gamadj.h).  The
+// forward run keeps State/oState checkpoints every K steps and the clock
+// of every step; the reverse sweep recomputes each checkpoint interval
+// forward (keeping its states on a tape) and then runs the adjoint
+// kernels of its steps from last to first.
+struct gam_adj {
+    gam_ctx *c;
+    int nsteps, K, nck;
+    int cur0;                // State buffer at the start of the run
+    int haveW, recorded;
+    double *ck;              // nck x (State, oState), S_NPL planes each
+    double *fin;             // State, oState at the end
+    Clock *clkh;             // clock before step n, n = 0..nsteps
+    double *tape;            // K + 1 State buffers of one interval
+    double *lam;             // three sets of gas adjoints (5 planes each)
+    double *ag;              // adjoint of gFlx (15 planes)
+    double *awh, *awn;       // adjoints of H_WH and H_WN (5 planes each)
+    double *wgt;             // objective weights (1 plane)
+    double *part;            // objective partial sums (1 plane)
+    hipEvent_t ev[4];
+    float fwd_ms, bwd_ms;
+};
+
+extern "C" int gam_adj_destroy(gam_adj *a)
+{
+    if (!a) return GAM_OK;
+    if (a->c && a->c->stream) hipStreamSynchronize(a->c->stream);
+    double *bufs[] = {a->ck, a->fin, a->tape, a->lam, a->ag, a->awh, a->awn, a->wgt, a->part};
+    for (double *b : bufs) if (b) hipFree(b);
+    if (a->clkh) hipFree(a->clkh);
+    for (int q = 0; q < 4; q++) if (a->ev[q]) hipEventDestroy(a->ev[q]);
+    free(a);
+    return GAM_OK;
+}
+
+extern "C" int gam_adj_create(gam_adj **out, gam_ctx *c, int nsteps, int K)
+{
+    if (!out) return fail(GAM_E_ARG, "gam_adj_create: NULL argument");
+    *out = nullptr;
+    int rc = check_ready(c, "gam_adj_create");
+    if (rc) return rc;
+    if (c->mp.doMHD || c->mp.doBoris)
+        return fail(GAM_E_ARG, "gam_adj_create: the adjoint covers hydrodynamic runs (doMHD = 0, doBoris = 0)");
+    for (int sd = 0; sd < 6; sd++)
+        if (c->model.bc[sd] == GAM_BC_WIND) return fail(GAM_E_ARG, "gam_adj_create: the inflow BC has no adjoint yet");
+    if (nsteps < 1 || K < 1 || K > nsteps) return fail(GAM_E_ARG, "gam_adj_create: need 1 <= ckpt_every <= nsteps");
+    gam_adj *a = (gam_adj *)calloc(1, sizeof(gam_adj));
+    if (!a) return fail(GAM_E_NOMEM, "gam_adj_create: host allocation failed");
+    a->c = c;
+    a->nsteps = nsteps;
+    a->K = K;
+    a->nck = (nsteps + K - 1) / K;
+    const size_t pl = (size_t)c->g.N3 * sizeof(double);
+    hipError_t e = hipSuccess;
+#define AALLOC(ptr, bytes) if (e == hipSuccess) e = hipMalloc((void **)&(ptr), (bytes))
+    AALLOC(a->ck, (size_t)a->nck * 2 * S_NPL * pl);
+    AALLOC(a->fin, 2 * S_NPL * pl);
+    AALLOC(a->clkh, (size_t)(nsteps + 1) * sizeof(Clock));
+    AALLOC(a->tape, (size_t)(K + 1) * S_NPL * pl);
+    AALLOC(a->lam, 15 * pl);
+    AALLOC(a->ag, 15 * pl);
+    AALLOC(a->awh, 5 * pl);
+    AALLOC(a->awn, 5 * pl);
+    AALLOC(a->wgt, pl);
+    AALLOC(a->part, pl);
+#undef AALLOC
+    if (e == hipSuccess) e = hipMemset(a->ag, 0, 15 * pl);
+    for (int q = 0; e == hipSuccess && q < 4; q++) e = hipEventCreate(&a->ev[q]);
+    if (e != hipSuccess) {
+        rc = fail(GAM_E_HIP, "gam_adj_create: %s", hipGetErrorString(e));
+        gam_adj_destroy(a);
+        return rc;
+    }
+    *out = a;
+    return GAM_OK;
+}
+
+extern "C" int gam_adj_set_weights(gam_adj *a, const double *w)
+{
+    if (!a || !w) return fail(GAM_E_ARG, "gam_adj_set_weights: NULL argument");
+    int rc = upload(a->c, w, 0, 1, a->wgt);
+    if (rc) return rc;
+    a->haveW = 1;
+    return GAM_OK;
+}
+
+static int adj_objective(gam_adj *a, const double *S, double *lamS, double *J)
+{
+    gam_ctx *c = a->c;
+    int np = 0;
+    launch_adj_objective(c->stream, c->g, c->mp, S, a->wgt, lamS, a->part, &np);
+    HIPCHK(hipGetLastError());
+    std::vector<double> h(np);
+    HIPCHK(hipMemcpyAsync(h.data(), a->part, sizeof(double) * np, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double tot = 0.0;
+    for (int q = 0; q < np; q++) tot += h[q];
+    if (J) *J = tot;
+    return GAM_OK;
+}
+
+// Put the context at step n's recorded buffers: State/oState from src
+// (two consecutive S_NPL blocks), clock from the record.
+static int adj_put(gam_adj *a, int n, const double *src)
+{
+    gam_ctx *c = a->c;
+    const size_t bytes = (size_t)S_NPL * c->g.N3 * sizeof(double);
+    c->cur = a->cur0 ^ (n & 1);
+    HIPCHK(hipMemcpyAsync(S_cur(c), src, bytes, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(S_old(c), src + (size_t)S_NPL * c->g.N3, bytes, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->clk, a->clkh + n, sizeof(Clock), hipMemcpyDeviceToDevice, c->stream));
+    return GAM_OK;
+}
+
+extern "C" int gam_adj_forward(gam_adj *a, double *J)
+{
+    if (!a) return fail(GAM_E_ARG, "gam_adj_forward: NULL argument");
+    if (!a->haveW) return fail(GAM_E_STATE, "gam_adj_forward: objective weights not set");
+    gam_ctx *c = a->c;
+    const long N3 = c->g.N3;
+    const size_t bytes = (size_t)S_NPL * N3 * sizeof(double);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    a->cur0 = c->cur;
+    const int prof = c->prof;
+    c->prof = 0;
+    HIPCHK(hipEventRecord(a->ev[0], c->stream));
+    HIPCHK(hipMemcpyAsync(a->clkh, c->clk, sizeof(Clock), hipMemcpyDeviceToDevice, c->stream));
+    for (int n = 0; n < a->nsteps; n++) {
+        if (n % a->K == 0) {
+            double *ck = a->ck + (size_t)(n / a->K) * 2 * S_NPL * N3;
+            HIPCHK(hipMemcpyAsync(ck, S_cur(c), bytes, hipMemcpyDeviceToDevice, c->stream));
+            HIPCHK(hipMemcpyAsync(ck + (size_t)S_NPL * N3, S_old(c), bytes, hipMemcpyDeviceToDevice, c->stream));
+        }
+        enq_step(c);
+        HIPCHK(hipMemcpyAsync(a->clkh + n + 1, c->clk, sizeof(Clock), hipMemcpyDeviceToDevice, c->stream));
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(a->ev[1], c->stream));
+    c->prof = prof;
+    HIPCHK(hipMemcpyAsync(a->fin, S_cur(c), bytes, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(a->fin + (size_t)S_NPL * N3, S_old(c), bytes, hipMemcpyDeviceToDevice, c->stream));
+    int rc = adj_objective(a, S_cur(c), a->lam, J);
+    if (rc) return rc;
+    HIPCHK(hipEventElapsedTime(&a->fwd_ms, a->ev[0], a->ev[1]));
+    Clock ck;
+    if ((rc = read_clock(c, &ck))) return rc;
+    a->recorded = 1;
+    return status_code(ck.status);
+}
+
+static int adj_restore_final(gam_adj *a)
+{
+    gam_ctx *c = a->c;
+    int rc = adj_put(a, a->nsteps, a->fin);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return GAM_OK;
+}
+
+extern "C" int gam_adj_backward(gam_adj *a, double *gGas, double *gGasO)
+{
+    if (!a || !gGas) return fail(GAM_E_ARG, "gam_adj_backward: NULL argument");
+    if (!a->recorded) return fail(GAM_E_STATE, "gam_adj_backward: no recorded forward run");
+    gam_ctx *c = a->c;
+    const Geo &g = c->g;
+    const long N3 = g.N3;
+    const size_t bytes = (size_t)S_NPL * N3 * sizeof(double);
+    hipStream_t s = c->stream;
+    double *lA = a->lam, *lB = a->lam + 5 * N3, *lC = a->lam + 10 * N3;
+    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipEventRecord(a->ev[2], s));
+    int rc = adj_objective(a, a->fin, lA, nullptr);
+    if (rc) return rc;
+    HIPCHK(hipMemsetAsync(lB, 0, 5 * N3 * sizeof(double), s));
+    const int prof = c->prof;
+    c->prof = 0;
+    for (int seg = a->nck - 1; seg >= 0; seg--) {
+        const int s0 = seg * a->K, s1 = (s0 + a->K < a->nsteps) ? s0 + a->K : a->nsteps;
+        const double *ck = a->ck + (size_t)seg * 2 * S_NPL * N3;
+        if ((rc = adj_put(a, s0, ck))) return rc;
+        // tape[0] = oState at s0, tape[1 + q] = State at step s0 + q
+        HIPCHK(hipMemcpyAsync(a->tape, ck + (size_t)S_NPL * N3, bytes, hipMemcpyDeviceToDevice, s));
+        for (int q = 0; q < s1 - s0; q++) {
+            HIPCHK(hipMemcpyAsync(a->tape + (size_t)(1 + q) * S_NPL * N3, S_cur(c), bytes,
+                                  hipMemcpyDeviceToDevice, s));
+            if (q + 1 < s1 - s0) enq_step(c);
+        }
+        for (int n = s1 - 1; n >= s0; n--) {
+            const int q = n - s0;
+            const double *Sn = a->tape + (size_t)(1 + q) * S_NPL * N3;
+            const double *On = a->tape + (size_t)q * S_NPL * N3;
+            const Clock *clk = a->clkh + n;
+            for (int sd = 5; sd >= 0; sd--)
+                launch_bc_adj(s, sd, c->model.bc[sd] == GAM_BC_PERIODIC ? 0 : 1, g, lA);
+            launch_update_adj(s, g, act_box(g), clk, lA, lB, a->ag, c->met);
+            launch_predict(s, c->plan.v[TS_PREDICT], g, c->mp, face_box(g), clk, Sn, On, c->met, c->H);
+            HIPCHK(hipMemsetAsync(a->awh, 0, 5 * N3 * sizeof(double), s));
+            HIPCHK(hipMemsetAsync(a->awn, 0, 5 * N3 * sizeof(double), s));
+            for (int d = 0; d < 3; d++) {
+                Box fb = act_box(g);
+                if (d == 0) fb.i1 += 1; else if (d == 1) fb.j1 += 1; else fb.k1 += 1;
+                launch_flux_adj(s, d, g, c->mp, fb, c->H, c->met, a->ag, a->awh, a->awn);
+            }
+            launch_predict_adj(s, g, c->mp, clk, Sn, On, c->met, a->awh, a->awn, lB, lC);
+            double *t = lA;
+            lA = lB;
+            lB = lC;
+            lC = t;
+        }
+        HIPCHK(hipGetLastError());
+    }
+    c->prof = prof;
+    // the start states' ghost cells follow from their interiors through the
+    // boundary conditions: fold their adjoints into the interior
+    for (int sd = 5; sd >= 0; sd--) {
+        launch_bc_adj(s, sd, c->model.bc[sd] == GAM_BC_PERIODIC ? 0 : 1, g, lA);
+        launch_bc_adj(s, sd, c->model.bc[sd] == GAM_BC_PERIODIC ? 0 : 1, g, lB);
+    }
+    HIPCHK(hipEventRecord(a->ev[3], s));
+    HIPCHK(hipEventSynchronize(a->ev[3]));
+    HIPCHK(hipEventElapsedTime(&a->bwd_ms, a->ev[2], a->ev[3]));
+    if ((rc = download(c, lA, 0, 5, gGas))) return rc;
+    if (gGasO && (rc = download(c, lB, 0, 5, gGasO))) return rc;
+    return adj_restore_final(a);
+}
+
+extern "C" int gam_adj_replay(gam_adj *a, const double *Gas, const double *oGas, double *J)
+{
+    if (!a || !Gas) return fail(GAM_E_ARG, "gam_adj_replay: NULL argument");
+    if (!a->recorded) return fail(GAM_E_STATE, "gam_adj_replay: no recorded forward run");
+    gam_ctx *c = a->c;
+    int rc = adj_put(a, 0, a->ck);
+    if (rc) return rc;
+    if ((rc = upload(c, Gas, 0, 5, S_cur(c) + S_GAS * c->g.N3))) return rc;
+    if (oGas && (rc = upload(c, oGas, 0, 5, S_old(c) + S_GAS * c->g.N3))) return rc;
+    enq_bcs(c, S_cur(c));    // ghosts from the given interiors
+    enq_bcs(c, S_old(c));
+    const int prof = c->prof;
+    c->prof = 0;
+    for (int n = 0; n < a->nsteps; n++) {
+        enq_step(c);
+        launch_set_dt(c->stream, c->clk, a->clkh + n + 1);
+    }
+    c->prof = prof;
+    HIPCHK(hipGetLastError());
+    if ((rc = adj_objective(a, S_cur(c), a->lam + 10 * c->g.N3, J))) return rc;
+    return adj_restore_final(a);
+}
+
+extern "C" int gam_adj_times(gam_adj *a, double *fwd_ms, double *bwd_ms)
+{
+    if (!a) return fail(GAM_E_ARG, "gam_adj_times: NULL argument");
+    if (fwd_ms) *fwd_ms = a->fwd_ms;
+    if (bwd_ms) *bwd_ms = a->bwd_ms;
     return GAM_OK;
 }

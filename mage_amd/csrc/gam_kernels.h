@@ -107,3 +107,16 @@ void launch_mask_e(hipStream_t s, const Geo &g, const Box &all, const Box *eboxe
 void launch_grid_from_corners(hipStream_t s, const Geo &g, double *met);
 void launch_dipole_b0(hipStream_t s, const Geo &g, double M0, const double *met, double *bg,
                       double *scratch9);
+
+// Adjoint of the hydrodynamic step (gam_adj.hip)
+void launch_adj_objective(hipStream_t s, const Geo &g, const MParm &m, const double *S,
+                          const double *wgt, double *aS, double *part, int *nparts);
+void launch_bc_adj(hipStream_t s, int side, int kind, const Geo &g, double *aS);
+void launch_update_adj(hipStream_t s, const Geo &g, const Box &ab, const Clock *clk, const double *aSn,
+                       double *aSo, double *ag, const double *met);
+void launch_flux_adj(hipStream_t s, int d, const Geo &g, const MParm &m, const Box &fb, const double *H,
+                     const double *met, const double *ag, double *aWH, double *aWN);
+void launch_predict_adj(hipStream_t s, const Geo &g, const MParm &m, const Clock *clk, const double *S,
+                        const double *O, const double *met, const double *aWH, const double *aWN,
+                        double *aS, double *aO);
+void launch_set_dt(hipStream_t s, Clock *clk, const Clock *rec);

@@ -51,6 +51,13 @@ __device__ __forceinline__ void c2p_block(const MParm &m, const double *U, doubl
     W[PRS] = dmax((m.gamma - 1) * (U[ENER] - ke), m.pFloor);
 }
 
+// AB2 extrapolation (ExtrapAB2, mhdgroup.F90:709-717): q + pdt*(q-oq)/odt
+// with w = pdt/odt formed once per thread.
+__device__ __forceinline__ double ab2(double q, double oq, double w)
+{
+    return q + w * (q - oq);
+}
+
 // ---------------------------------------------------------- limiter
 // PDM (recon.F90:275-301)
 __device__ __forceinline__ double pdm_one(double b, double q0, double q1, double q2, double qI)

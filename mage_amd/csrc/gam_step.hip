@@ -201,12 +201,6 @@ static void launch_f(dim3 grid, dim3 block, hipStream_t s, A... a)
     hipLaunchKernelGGL((k_one<F, A...>), grid, block, 0, s, a...);
 }
 
-// AB2 extrapolation (ExtrapAB2, mhdgroup.F90:709-717): q + pdt*(q-oq)/odt
-// with w = pdt/odt formed once per thread.
-__device__ __forceinline__ double ab2(double q, double oq, double w)
-{
-    return q + w * (q - oq);
-}
 
 // ====================================================================
 // Predictor (mhdgroup.F90:444-561) fused with bFlux2Fld of the half state

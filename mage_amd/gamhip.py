@@ -401,6 +401,83 @@ class GamEnsemble:
             pass
 
 
+# ---------------------------------------------------------------- adjoint
+# Every symbol include/gamadj.h declares (exported by libgamhip)
+ADJ_EXPORTS = ("gam_adj_create", "gam_adj_destroy", "gam_adj_set_weights", "gam_adj_forward",
+               "gam_adj_backward", "gam_adj_replay", "gam_adj_times")
+
+
+def _load_adj(L):
+    vp = C.c_void_p
+    L.gam_adj_create.argtypes = [C.POINTER(vp), vp, C.c_int, C.c_int]
+    L.gam_adj_destroy.argtypes = [vp]
+    L.gam_adj_set_weights.argtypes = [vp, DP]
+    L.gam_adj_forward.argtypes = [vp, DP]
+    L.gam_adj_backward.argtypes = [vp, DP, DP]
+    L.gam_adj_replay.argtypes = [vp, DP, DP, DP]
+    L.gam_adj_times.argtypes = [vp, DP, DP]
+    return L
+
+
+class GamAdjoint:
+    """Discrete adjoint of nsteps steps of a (hydrodynamic) context for the
+    objective J = sum_c w_c P_c of the final State, with State/oState
+    checkpoints every ckpt_every steps."""
+
+    def __init__(self, g, nsteps, ckpt_every):
+        self.L = _load_adj(load())
+        self.g = g
+        self.adj = C.c_void_p()
+        self._chk(self.L.gam_adj_create(C.byref(self.adj), g.ctx, int(nsteps), int(ckpt_every)))
+
+    def _chk(self, rc):
+        if rc != GAM_OK:
+            raise GamError(rc, self.L.gam_last_error().decode())
+        return rc
+
+    def set_weights(self, w):
+        w = np.asfortranarray(w, dtype=np.float64)
+        if w.shape != self.g.cshape[:3]:
+            raise ValueError("weights must be (Ni, Nj, Nk)")
+        self._chk(self.L.gam_adj_set_weights(self.adj, _p(w)))
+
+    def forward(self):
+        J = C.c_double()
+        self._chk(self.L.gam_adj_forward(self.adj, C.byref(J)))
+        return J.value
+
+    def backward(self):
+        shp = self.g.cshape[:3] + (5,)
+        gS = np.zeros(shp, order="F")
+        gO = np.zeros(shp, order="F")
+        self._chk(self.L.gam_adj_backward(self.adj, _p(gS), _p(gO)))
+        return gS, gO
+
+    def replay(self, Gas, oGas=None):
+        J = C.c_double()
+        Gas = np.asfortranarray(Gas, dtype=np.float64)
+        oGas = None if oGas is None else np.asfortranarray(oGas, dtype=np.float64)
+        self._chk(self.L.gam_adj_replay(self.adj, _p(Gas), _p(oGas), C.byref(J)))
+        return J.value
+
+    def times(self):
+        f = C.c_double()
+        b = C.c_double()
+        self._chk(self.L.gam_adj_times(self.adj, C.byref(f), C.byref(b)))
+        return f.value, b.value
+
+    def close(self):
+        if getattr(self, "adj", None):
+            self.L.gam_adj_destroy(self.adj)
+            self.adj = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 # ---------------------------------------------------------------- restart files
 RES_LIBPATH = os.path.join(HERE, "libgamres.so")
 NG = 4

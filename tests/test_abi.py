@@ -95,3 +95,23 @@ def test_plan_entry_points_without_gpu():
     assert L.gam_get_plan(None, plan) == -1
     assert L.gam_set_plan(None, plan) == -1
     assert L.gam_autotune(None, 2) == -1
+
+
+def test_adjoint_header_symbols_exported():
+    """libgamhip exports every symbol include/gamadj.h declares, and the
+    adjoint entry points reject bad arguments before touching a GPU."""
+    src = open(os.path.join(ROOT, "include", "gamadj.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = sorted(set(re.findall(r"\b(gam_adj_[a-z0-9_]+)\s*\(", src)))
+    assert names == sorted(gamhip.ADJ_EXPORTS)
+    L = gamhip._load_adj(gamhip.load())
+    for n in names:
+        assert hasattr(L, n)
+    out = C.c_void_p()
+    assert L.gam_adj_create(None, None, 4, 2) == -1
+    assert L.gam_adj_create(C.byref(out), None, 4, 2) == -1
+    J = C.c_double()
+    assert L.gam_adj_forward(None, C.byref(J)) == -1
+    assert L.gam_adj_backward(None, None, None) == -1
+    assert L.gam_adj_replay(None, None, None, C.byref(J)) == -1
+    assert L.gam_adj_destroy(None) == 0

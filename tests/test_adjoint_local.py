@@ -1,0 +1,212 @@
+"""Reverse-mode products of the hydrodynamic per-point functions
+(mage_amd/csrc/gam_adj.h) against central differences of the forward
+functions they differentiate (gam_phys.h), both compiled for the host from
+the same headers.
+
+For a function y = f(x), random x, direction dx and output weights a, the
+adjoint must satisfy a . (f(x + e dx) - f(x - e dx)) / (2 e) = (J^T a) . dx.
+Smooth functions: agreement to 1e-7 of the sum of the absolute terms of
+either product, with e = 1e-6 (central-difference truncation ~e^2,
+rounding ~1e-16/e).  Piecewise-linear limiter functions:
+exact within a branch, so e = 1e-9 and at most 1 % of samples may straddle
+a branch switch; every other sample must agree to 1e-6.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SRC = r'''
+#include "gam_adj.h"
+static MParm mp(double gamma, double dF, double pF, int lfm)
+{
+    MParm m = {};
+    m.gamma = gamma; m.igm1 = 1.0 / (gamma - 1.0); m.dFloor = dF; m.pFloor = pF; m.doLFM = lfm;
+    return m;
+}
+extern "C" {
+void f_gas(double g, double s, const double *W, const double *T, double *y)
+{ double dW[5] = {0, 0, 0, 0, 0}; for (int v = 0; v < 5; v++) y[v] = 0; gas_kin(1.0 / (g - 1.0), s, W, T, y, dW); }
+void a_gas(double g, double s, const double *W, const double *T, const double *a, double *ax)
+{ for (int v = 0; v < 5; v++) ax[v] = 0; gas_kin_adj(1.0 / (g - 1.0), s, W, T, a, ax); }
+void f_c2p(double g, double dF, double pF, const double *U, double *y) { MParm m = mp(g, dF, pF, 1); c2p_cell(m, U, y); }
+void a_c2p(double g, double dF, double pF, const double *U, const double *a, double *ax)
+{ MParm m = mp(g, dF, pF, 1); for (int v = 0; v < 5; v++) ax[v] = 0; c2p_cell_adj(m, U, a, ax); }
+void f_p2c(double g, double dF, double pF, const double *W, double *y) { MParm m = mp(g, dF, pF, 1); p2c_cell(m, W, y); }
+void a_p2c(double g, double dF, double pF, const double *W, const double *a, double *ax)
+{ MParm m = mp(g, dF, pF, 1); for (int v = 0; v < 5; v++) ax[v] = 0; p2c_cell_adj(m, W, a, ax); }
+void f_blk(double g, double dF, double pF, const double *U, double *y) { MParm m = mp(g, dF, pF, 1); c2p_block(m, U, y); }
+void a_blk(double g, double dF, double pF, const double *U, const double *a, double *ax)
+{ MParm m = mp(g, dF, pF, 1); for (int v = 0; v < 5; v++) ax[v] = 0; c2p_block_adj(m, U, a, ax); }
+void f_rec(int rm, double b, double rVi, const double *x, double *y)
+{   // x = qv[8], nq[4]
+    if (rm) recon_lr_w<1>(b, rVi, x + 8, x, y[0], y[1]); else recon_lr_w<0>(b, rVi, x + 8, x, y[0], y[1]);
+}
+void a_rec(int rm, double b, double rVi, const double *x, const double *a, double *ax)
+{
+    for (int n = 0; n < 12; n++) ax[n] = 0;
+    if (rm) recon_lr_w_adj<1>(b, rVi, x + 8, x, a[0], a[1], ax + 8, ax);
+    else recon_lr_w_adj<0>(b, rVi, x + 8, x, a[0], a[1], ax + 8, ax);
+}
+void f_pred(double g, int lfm, double w, double vol, const double *x, double *y)
+{   // x = Q[5], oQ[5]; y = H_GAS 5, H_WH 5, H_WN 5 (as k_predict writes them)
+    MParm m = mp(g, 1e-12, 1e-12, lfm);
+    double W[5], oW[5], pW[5], pQ[5], Wb[5];
+    c2p_cell(m, x + 5, oW); c2p_cell(m, x, W);
+    for (int v = 0; v < 5; v++) pW[v] = ab2(W[v], oW[v], w);
+    pW[DEN] = dmax(pW[DEN], m.dFloor); pW[PRS] = dmax(pW[PRS], m.pFloor);
+    p2c_cell(m, pW, pQ);
+    for (int v = 0; v < 5; v++) y[v] = pQ[v];
+    c2p_block(m, pQ, Wb);
+    for (int v = 0; v < 5; v++) y[5 + v] = vol * Wb[v];
+    if (lfm) c2p_block(m, x, Wb);
+    for (int v = 0; v < 5; v++) y[10 + v] = Wb[v];
+}
+void a_pred(double g, int lfm, double w, double vol, const double *x, const double *a, double *ax)
+{
+    MParm m = mp(g, 1e-12, 1e-12, lfm);
+    for (int n = 0; n < 10; n++) ax[n] = 0;
+    predict_cell_adj(m, x, x + 5, w, vol, a, a + 5, a + 10, ax, ax + 5);
+}
+}
+'''
+
+
+@pytest.fixture(scope="module")
+def lib(tmp_path_factory):
+    d = tmp_path_factory.mktemp("adjloc")
+    src = d / "adj_host.cpp"
+    src.write_text(SRC)
+    so = d / "libadj_host.so"
+    subprocess.run(["g++", "-O2", "-ffp-contract=off", "-shared", "-fPIC", "-std=c++17",
+                    "-I", os.path.join(ROOT, "tests", "host_shim"),
+                    "-I", os.path.join(ROOT, "mage_amd", "csrc"), str(src), "-o", str(so)], check=True)
+    return C.CDLL(str(so))
+
+
+DP = C.POINTER(C.c_double)
+
+
+def _p(a):
+    return a.ctypes.data_as(DP)
+
+
+def _check(fwd, adj, x0, nout, rng, eps, smooth, nsamp=400, perturb=None):
+    bad = 0
+    worst = 0.0
+    for _ in range(nsamp):
+        x = perturb(rng) if perturb else x0(rng)
+        dx = rng.standard_normal(x.size) * np.maximum(np.abs(x), 1e-3)
+        a = rng.standard_normal(nout)
+        yp = np.zeros(nout); ym = np.zeros(nout); ax = np.zeros(x.size)
+        fwd(x + eps * dx, yp)
+        fwd(x - eps * dx, ym)
+        lhs = a @ (yp - ym) / (2 * eps)
+        adj(x, a, ax)
+        rhs = ax @ dx
+        # scale free of cancellation between the terms of the two dot products
+        scale = max(np.sum(np.abs(a * (yp - ym))) / (2 * eps), np.sum(np.abs(ax * dx)), 1e-300)
+        err = abs(lhs - rhs) / scale
+        if smooth:
+            worst = max(worst, err)
+        elif err > 1e-6:
+            bad += 1
+    if smooth:
+        assert worst <= 1e-7, worst
+    else:
+        assert bad <= nsamp // 100, bad
+
+
+def _triad(rng):
+    n = rng.standard_normal(3); n /= np.linalg.norm(n)
+    t1 = np.cross(n, rng.standard_normal(3)); t1 /= np.linalg.norm(t1)
+    t2 = np.cross(n, t1)
+    return np.concatenate([n, t1, t2])
+
+
+@pytest.mark.parametrize("s", [-1.0, 1.0])
+def test_gas_kinetic_flux_adjoint(lib, s):
+    rng = np.random.default_rng(11)
+    T = _triad(rng)
+    g = 5.0 / 3.0
+
+    def x0(r):
+        return np.array([r.uniform(0.2, 2.0), r.standard_normal(), r.standard_normal(),
+                         r.standard_normal(), r.uniform(0.05, 2.0)])
+    W = None
+    fwd = lambda x, y: lib.f_gas(C.c_double(g), C.c_double(s), _p(np.array([x[0], x[1], x[2], x[3], x[4]])), _p(T), _p(y))
+    adj = lambda x, a, ax: lib.a_gas(C.c_double(g), C.c_double(s), _p(x), _p(T), _p(a), _p(ax))
+    _check(fwd, adj, x0, 5, rng, 1e-6, True)
+
+
+@pytest.mark.parametrize("name", ["c2p", "p2c", "blk"])
+def test_conversion_adjoints(lib, name):
+    rng = np.random.default_rng(12)
+    g = 5.0 / 3.0
+    f = getattr(lib, "f_" + name)
+    a_ = getattr(lib, "a_" + name)
+
+    def x0(r):
+        d = r.uniform(0.3, 2.0)
+        v = r.standard_normal(3) * 0.5
+        if name == "p2c":
+            return np.array([d, v[0], v[1], v[2], r.uniform(0.1, 2.0)])
+        return np.array([d, d * v[0], d * v[1], d * v[2], 0.5 * d * v @ v + r.uniform(0.1, 2.0) * 1.5])
+    fwd = lambda x, y: f(C.c_double(g), C.c_double(1e-12), C.c_double(1e-12), _p(x), _p(y))
+    adj = lambda x, a, ax: a_(C.c_double(g), C.c_double(1e-12), C.c_double(1e-12), _p(x), _p(a), _p(ax))
+    _check(fwd, adj, x0, 5, rng, 1e-6, True)
+
+
+@pytest.mark.parametrize("rm", [0, 1])
+def test_reconstruction_and_pdm_adjoint(lib, rm):
+    rng = np.random.default_rng(13 + rm)
+    vol = rng.uniform(0.8, 1.2, 8)
+    Vi = sum(w * v for w, v in zip([-3, 29, -139, 533, 533, -139, 29, -3], vol)) / 840.0
+
+    def x0(r):
+        q = np.cumsum(r.standard_normal(8)) * 0.3 + r.standard_normal() * 0.2
+        nq = q[2:6] + 0.05 * r.standard_normal(4)
+        return np.concatenate([vol * q, nq])
+    fwd = lambda x, y: lib.f_rec(rm, C.c_double(0.75), C.c_double(1.0 / Vi), _p(x), _p(y))
+    adj = lambda x, a, ax: lib.a_rec(rm, C.c_double(0.75), C.c_double(1.0 / Vi), _p(x), _p(a), _p(ax))
+    _check(fwd, adj, x0, 2, rng, 1e-9, False, nsamp=2000)
+
+
+@pytest.mark.parametrize("lfm", [0, 1])
+def test_predictor_cell_adjoint(lib, lfm):
+    rng = np.random.default_rng(17)
+    g = 5.0 / 3.0
+
+    def cons(r):
+        d = r.uniform(0.3, 2.0)
+        v = r.standard_normal(3) * 0.5
+        return np.array([d, d * v[0], d * v[1], d * v[2], 0.5 * d * v @ v + r.uniform(0.2, 2.0) * 1.5])
+
+    def x0(r):
+        Q = cons(r)
+        return np.concatenate([Q, Q * (1 + 0.05 * r.standard_normal(5))])
+    fwd = lambda x, y: lib.f_pred(C.c_double(g), lfm, C.c_double(0.5), C.c_double(1.3), _p(x), _p(y))
+    adj = lambda x, a, ax: lib.a_pred(C.c_double(g), lfm, C.c_double(0.5), C.c_double(1.3), _p(x), _p(a), _p(ax))
+    _check(fwd, adj, x0, 15, rng, 1e-6, True)
+
+
+def test_adjoint_catches_a_planted_sign_error(lib):
+    """The check fails for a wrong product: flip the sign of one adjoint
+    component of the gas flux and the comparison must reject it."""
+    rng = np.random.default_rng(19)
+    T = _triad(rng)
+    g = 5.0 / 3.0
+    x = np.array([1.1, 0.3, -0.2, 0.4, 0.7])
+    dx = rng.standard_normal(5)
+    a = rng.standard_normal(5)
+    e = 1e-6
+    yp = np.zeros(5); ym = np.zeros(5); ax = np.zeros(5)
+    lib.f_gas(C.c_double(g), C.c_double(1.0), _p(x + e * dx), _p(T), _p(yp))
+    lib.f_gas(C.c_double(g), C.c_double(1.0), _p(x - e * dx), _p(T), _p(ym))
+    lib.a_gas(C.c_double(g), C.c_double(1.0), _p(x), _p(T), _p(a), _p(ax))
+    ax[4] = -ax[4]
+    lhs = a @ (yp - ym) / (2 * e)
+    assert abs(lhs - ax @ dx) > 1e-4 * abs(lhs)

@@ -1,0 +1,155 @@
+"""Discrete adjoint of the hydrodynamic step against central finite
+differences of the same device step (dt held at the recorded values in
+both), on the periodic box (8CENT and 7UP) and on a warped shell with
+zero-gradient and periodic boundaries.
+
+Gradients are with respect to the interiors of the starting State and
+oState gas (ghost cells follow through the boundary conditions, in the
+adjoint and in the replays alike).  For seeded smooth directions d the
+adjoint's <g, d> must match (J(Gas + e d) - J(Gas - e d)) / (2 e) at the
+best of four steps, e = 1e-2 ... 1e-5 of 1 % of each field's scale
+(relative perturbations 1e-4 to 1e-7).  The difference quotient is
+squeezed from both sides: at small steps by the round-off of the forward
+step (the error grows like 1/e, about 1e-16 J / (e |d|)), at large ones by
+the PDM limiter's branch switches that the perturbation crosses (the
+adjoint differentiates the branch taken, as intended away from switches).
+A wrong adjoint misses at every step; the best one must agree, for
+State directions, to 1e-4 relative to the derivative and to 1e-6 of
+sum_c |g_c d_c| (the size of the terms the directional derivative sums,
+free of their cancellation).  oState directions move J 10-1000x less
+(oState enters only through the AB2 slope), so no step escapes both the
+round-off floor and the limiter switches as cleanly: measured best
+agreement 1e-6 to 2e-4; they are held to 1e-3 relative.
+The states get smooth variation in every variable first: where a variable
+is exactly uniform every limiter difference is exactly 0, a branch switch,
+and a central difference there averages two one-sided slopes instead of
+measuring the derivative.
+"""
+import numpy as np
+import pytest
+
+from helpers import box_case
+from mage_amd import gamhip, problems as pb
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-6
+
+
+def _directions(shape, n, seed):
+    rng = np.random.default_rng(seed)
+    Ni, Nj, Nk, _ = shape
+    x = np.linspace(0, 1, Ni)[:, None, None]
+    y = np.linspace(0, 1, Nj)[None, :, None]
+    z = np.linspace(0, 1, Nk)[None, None, :]
+    out = []
+    for _ in range(n):
+        d = np.zeros(shape, order="F")
+        for v in range(5):
+            k = rng.integers(1, 4, 3)
+            ph = rng.uniform(0, 2 * np.pi, 3)
+            d[..., v] = (np.sin(2 * np.pi * k[0] * x + ph[0]) * np.sin(2 * np.pi * k[1] * y + ph[1])
+                         * np.sin(2 * np.pi * k[2] * z + ph[2]))
+        out.append(d)
+    return out
+
+
+def _generic(g, seed):
+    """Add smooth variation to every gas variable of State and oState: with
+    an exactly uniform variable (rho = 1 or p = 1 in the case states) every
+    limiter difference is exactly 0, i.e. on a branch switch, where a
+    central difference averages the two one-sided slopes."""
+    S = g.get_state(gamhip.GAM_STATE)
+    O = g.get_state(gamhip.GAM_OSTATE)
+    t = g.get_time()[0]
+    scale = np.max(np.abs(S["Gas"]), axis=(0, 1, 2))
+    d = _directions(S["Gas"].shape, 1, seed)[0]
+    for v in range(5):
+        S["Gas"][..., v] += 0.03 * scale[v] * d[..., v]
+        O["Gas"][..., v] += 0.03 * scale[v] * d[..., v]
+    g.put_state(S, gamhip.GAM_STATE)
+    g.put_state(O, gamhip.GAM_OSTATE)
+    return t
+
+
+def _weights(g):
+    vol = g.get_grid(fields=("volume",))["volume"]
+    Ni, Nj, Nk = vol.shape
+    x = np.linspace(0, 1, Ni)[:, None, None]
+    y = np.linspace(0, 1, Nj)[None, :, None]
+    return np.asfortranarray(vol * (1.0 + 0.5 * np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y)) * np.ones((1, 1, Nk)))
+
+
+def _fd_check(g, nsteps, ckpt, ndir, seed, tag):
+    s0 = g.get_state(gamhip.GAM_STATE)["Gas"].copy(order="F")
+    o0 = g.get_state(gamhip.GAM_OSTATE)["Gas"].copy(order="F")
+    adj = gamhip.GamAdjoint(g, nsteps, ckpt)
+    adj.set_weights(_weights(g))
+    J = adj.forward()
+    gS, gO = adj.backward()
+    assert adj.replay(s0, o0) == pytest.approx(J, rel=1e-14, abs=0.0)
+    scale = np.max(np.abs(s0), axis=(0, 1, 2))
+    worst = 0.0
+    for q, d in enumerate(_directions(s0.shape, ndir, seed)):
+        d = d * scale * 1e-2
+        which = q % 2   # alternate: State gas, oState gas
+        ad = np.sum(gS * d) if which == 0 else np.sum(gO * d)
+        gd = np.sum(np.abs(gS * d if which == 0 else gO * d))
+        best = None
+        for e in (1e-2, 1e-3, 1e-4, 1e-5):
+            if which == 0:
+                fd = (adj.replay(s0 + e * d, o0) - adj.replay(s0 - e * d, o0)) / (2 * e)
+            else:
+                fd = (adj.replay(s0, o0 + e * d) - adj.replay(s0, o0 - e * d)) / (2 * e)
+            rel = abs(fd - ad) / max(abs(fd), abs(ad))
+            print("%s dir %d (%s) e %.0e: fd %.12e adj %.12e rel %.2e of sum|g d| %.2e" % (
+                tag, q, "State" if which == 0 else "oState", e, fd, ad, rel, abs(fd - ad) / gd), flush=True)
+            if best is None or rel < best[0]:
+                best = (rel, abs(fd - ad) / gd)
+        if which == 0:
+            assert best[0] <= 1e-4 and best[1] <= 1e-6, (tag, q, best)
+            worst = max(worst, best[1])
+        else:
+            assert best[0] <= 1e-3, (tag, q, best)
+    f_ms, b_ms = adj.times()
+    print("%s: worst State-direction error %.2e of sum|g d|; forward %.2f ms, reverse %.2f ms for %d steps" % (tag, worst, f_ms, b_ms, nsteps), flush=True)
+    adj.close()
+    return worst
+
+
+@pytest.mark.parametrize("rmeth", ["8CENT", "7UP"])
+def test_adjoint_box_matches_finite_differences(rmeth):
+    g = box_case((16, 16, 16), rmeth, doMHD=0)[0]
+    g.init_run()
+    _generic(g, 7)
+    g.run(2)              # a non-trivial oState (AB2 slope), both states BC-consistent
+    assert _fd_check(g, 6, 4, 6, 101, "box " + rmeth) <= TOL
+    g.close()
+
+
+def test_adjoint_shell_zero_gradient_matches_finite_differences():
+    N = (24, 16, 24)
+    p = pb.shell_params("7UP")
+    p.update(doMHD=0, doBoris=0, doBackground=0,
+             bc=["zerograd", "zerograd", "zerograd", "zerograd", "periodic", "periodic"])
+    g = gamhip.GamHip(*N, p)
+    g.grid_from_corners(pb.shell_corners(*N))
+    xyzcc = g.get_grid(fields=("xyzcc",))["xyzcc"]
+    st = pb.shell_state(*N, xyzcc, p0=0.1)
+    rng = np.random.default_rng(5)
+    st["Gas"][..., 1] = 0.05 * np.sin(xyzcc[..., 1])   # some momentum
+    st["Gas"][..., 4] += 0.5 * st["Gas"][..., 1] ** 2
+    g.put_state(st)
+    g.init_run()
+    _generic(g, 9)
+    g.run(2)
+    assert _fd_check(g, 5, 2, 6, 202, "shell") <= TOL
+    g.close()
+
+
+def test_adjoint_refuses_mhd_contexts():
+    g = box_case((16, 16, 16), "7UP")[0]
+    with pytest.raises(gamhip.GamError) as ei:
+        gamhip.GamAdjoint(g, 4, 2)
+    assert ei.value.code == -1
+    g.close()
