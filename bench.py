@@ -143,6 +143,40 @@ def full_size(gamhip, pb, N=(256, 256, 384), steps=500, warmup=5, tune=True):
             "step_frac_of_hbm_roofline": STEP_BYTES_PER_CELL * value / 1e9 / HBM_PEAK_GBS}
 
 
+def adjoint_cost(gamhip, pb, N=(96, 96, 128), steps=200, ckpt=20):
+    """Adjoint / forward cost on the adjoint workload's grid, step count and
+    checkpoint interval, for the step the adjoint covers (hydrodynamic:
+    doMHD = 0; zero-gradient r and theta walls, periodic phi).  Objective:
+    the pressure integral over 3 <= r <= 6 at the final step."""
+    import numpy as np
+    p = pb.shell_params("7UP")
+    p.update(doMHD=0, doBoris=0, doBackground=0,
+             bc=["zerograd", "zerograd", "zerograd", "zerograd", "periodic", "periodic"])
+    g = gamhip.GamHip(*N, p)
+    g.grid_from_corners(pb.shell_corners(*N))
+    grid = g.get_grid(fields=("xyzcc", "volume"))
+    st = pb.shell_state(*N, grid["xyzcc"], p0=0.1)
+    g.put_state(st)
+    g.init_run()
+    g.run(2)
+    r = np.sqrt(np.sum(grid["xyzcc"] ** 2, axis=-1))
+    w = np.asfortranarray(grid["volume"] * ((r >= 3.0) & (r <= 6.0)))
+    plain_ms = g.time_steps(steps)
+    adj = gamhip.GamAdjoint(g, steps, ckpt)
+    adj.set_weights(w)
+    J = adj.forward()
+    adj.backward()
+    f_ms, b_ms = adj.times()
+    adj.close()
+    g.close()
+    return {"workload": "%dx%dx%d shell, hydrodynamic step (the part with an adjoint), %d steps, "
+                        "checkpoint every %d, J = pressure integral over 3 <= r <= 6" % (N + (steps, ckpt)),
+            "J": J, "forward_plain_ms": plain_ms, "forward_recorded_ms": f_ms, "reverse_ms": b_ms,
+            "adjoint_over_forward": b_ms / plain_ms,
+            "note": "reverse = checkpoint-interval recompute + adjoint kernels; "
+                    "MHD terms and the inflow boundary have no adjoint yet"}
+
+
 def cpu_baseline(gamhip, pb, min_seconds=12.0, max_steps=400):
     """Time the oracle (C/OpenMP restatement of the reference step) on a
     96x96x128 sample of the same shell workload."""
@@ -182,6 +216,7 @@ def main():
     ap.add_argument("--no-ens", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-full", action="store_true")
+    ap.add_argument("--no-adj", action="store_true")
     ap.add_argument("--full-steps", type=int, default=500)
     ap.add_argument("--ens-members", type=int, default=32)
     ap.add_argument("--ens-steps", type=int, default=100)
@@ -262,6 +297,14 @@ def main():
         log("ensemble %d members: sequential %.1f ms, batched %.1f ms, gain %.2f (%.1f s)"
             % (ens["members"], ens["sequential_ms"], ens["batched_ms"], ens["gain"], time.time() - t1))
 
+    adjc = None
+    if not args.no_adj:
+        t1 = time.time()
+        adjc = adjoint_cost(gamhip, pb)
+        log("adjoint: forward %.1f ms (recorded %.1f), reverse %.1f ms, ratio %.2f (%.1f s)"
+            % (adjc["forward_plain_ms"], adjc["forward_recorded_ms"], adjc["reverse_ms"],
+               adjc["adjoint_over_forward"], time.time() - t1))
+
     cpu = None
     if not args.no_cpu:
         t1 = time.time()
@@ -314,6 +357,7 @@ def main():
         "box_64cubed": box,
         "shell_256x256x384": full,
         "ensemble": ens,
+        "adjoint": adjc,
         "kernels": kernels,
         "final": {"t": t, "dt": dt, "ts": ts},
         "cpu_baseline": cpu,
