@@ -9,10 +9,12 @@
  * the forward run, GasKinFlux; stress.F90:204-511, recon.F90:125-301) and
  * the AB2 predictor (mhdgroup.F90:444-561).
  *
- * Scope: hydrodynamic runs (doMHD = 0, doBoris = 0) with periodic,
- * zero-gradient or sphere boundaries; GAM_E_ARG otherwise.  The magnetic
- * terms (EMF, CT, magnetic fluxes, Maxwell/Boris updates) and the inflow
- * boundary are not differentiated yet.
+ * Scope: runs without Boris correction and background field
+ * (doBoris = 0, doBackground = 0) and without the inflow boundary;
+ * hydrodynamic runs (doMHD = 0) with periodic, zero-gradient or sphere
+ * boundaries, MHD runs with periodic boundaries (GAM_E_ARG otherwise).
+ * With MHD the adjoint runs through the edge EMFs, constrained transport,
+ * the magnetic kinetic fluxes with HOGS and the Maxwell update.
  *
  * Objective: J = sum over cells of w(c) * P(c) of the State after nsteps
  * (P the CellC2P pressure), w a weight array (Ni,Nj,Nk) given by the
@@ -48,18 +50,22 @@ int gam_adj_set_weights(gam_adj *a, const double *w);
  * end.  The context is left at the final state. */
 int gam_adj_forward(gam_adj *a, double *J);
 /* Reverse sweep of the recorded run: gGas = dJ/d(State Gas) and gGasO =
- * dJ/d(oState Gas) at the start, both (Ni,Nj,Nk,5) (gGasO may be NULL),
- * with respect to the active cells: each start state's ghost cells are
- * taken to follow from its interior through the boundary conditions, so
- * their adjoints are folded into the interior and the ghost entries are 0.
+ * dJ/d(oState Gas) at the start, (Ni,Nj,Nk,5); with MHD gMF and gMFO =
+ * dJ/d(magFlux) of State and oState, (Ni+1,Nj+1,Nk+1,3).  Any output but
+ * gGas may be NULL.  Gradients are with respect to the active cells and
+ * the face fluxes: each start state's ghost cells follow from its interior
+ * through the boundary conditions and its cell B from its face fluxes
+ * (bFlux2Fld), so those adjoints are folded in (the ghost entries are 0).
  * The context is left at the final state of the forward run. */
-int gam_adj_backward(gam_adj *a, double *gGas, double *gGasO);
+int gam_adj_backward(gam_adj *a, double *gGas, double *gGasO, double *gMF, double *gMFO);
 /* Forward from the recorded start with the given State Gas (and oState
- * Gas, or NULL to keep the recorded one), their ghost cells refilled by
- * the boundary conditions, each step's dt held at the recorded value;
- * *J = objective at the end.  The context is then put back at the final
- * state of the recorded run. */
-int gam_adj_replay(gam_adj *a, const double *Gas, const double *oGas, double *J);
+ * Gas, State and oState magFlux; NULL keeps the recorded array), ghost
+ * cells refilled by the boundary conditions and cell B recomputed from the
+ * face fluxes, each step's dt held at the recorded value; *J = objective
+ * at the end.  The context is then put back at the final state of the
+ * recorded run. */
+int gam_adj_replay(gam_adj *a, const double *Gas, const double *oGas, const double *MF,
+                   const double *oMF, double *J);
 /* Milliseconds of the last gam_adj_forward and gam_adj_backward calls
  * (HIP events on the context's stream). */
 int gam_adj_times(gam_adj *a, double *fwd_ms, double *bwd_ms);

@@ -184,10 +184,11 @@ __device__ __forceinline__ void recon_lr_w_adj(double pdmb, double rVi, const do
 }
 
 // ---------------------------------------------------------- gas flux
-// One side of GasKinFlux (gas_kin) without the HOGS difference dW:
-// aF = adjoint of the flux contribution gF (5), adds into aW (5).
+// One side of GasKinFlux (gas_kin): aF = adjoint of the flux contribution
+// gF (5), adW (may be null) that of the HOGS difference dW (5); adds into
+// aW (5).
 __device__ __forceinline__ void gas_kin_adj(double igm1, double s, const double *W, const double *T,
-                                            const double *aF, double *aW)
+                                            const double *aF, double *aW, const double *adW = nullptr)
 {
     const double d = W[DEN], p = W[PRS], vx = W[1], vy = W[2], vz = W[3];
     const double e = 0.5 * d * (vx * vx + vy * vy + vz * vz) + p * igm1;
@@ -209,10 +210,18 @@ __device__ __forceinline__ void gas_kin_adj(double igm1, double s, const double 
     double avn1 = d * aF[DEN] + (e + 0.5 * p) * aE + d * vn * afn + d * vt1 * aft1 + d * vt2 * aft2;
     double avn0 = 0.5 * p * vn * aE + p * afn;
     double avn = 0.5 * p * vn0 * aE + d * vn1 * afn;
-    const double ae = vn1 * aE;
+    double ae = vn1 * aE;
     double ap = 0.5 * vn1 * aE + 0.5 * vn0 * vn * aE + vn0 * afn;
     double ad = vn1 * aF[DEN] + vn * vn1 * afn + vt1 * vn1 * aft1 + vt2 * vn1 * aft2;
     const double avt1 = d * vn1 * aft1, avt2 = d * vn1 * aft2;
+    double ae2 = 0.0, avx = 0.0, avy = 0.0, avz = 0.0;
+    if (adW) {   // dW += s (d, d v, e)
+        ad += s * (adW[DEN] + vx * adW[MOMX] + vy * adW[MOMY] + vz * adW[MOMZ]);
+        avx = s * d * adW[MOMX];
+        avy = s * d * adW[MOMY];
+        avz = s * d * adW[MOMZ];
+        ae2 = s * adW[ENER];
+    }
     // vn1 = vn vn0 - s ez rl / (2 sqrt(pi))
     avn += vn0 * avn1;
     avn0 += vn * avn1;
@@ -228,13 +237,121 @@ __device__ __forceinline__ void gas_kin_adj(double igm1, double s, const double 
     ad += alam / (2 * p);
     ap += -lam / p * alam;
     // e = d |v|^2 / 2 + p igm1
+    ae += ae2;
     ad += 0.5 * (vx * vx + vy * vy + vz * vz) * ae;
     ap += igm1 * ae;
     aW[DEN] += ad;
     aW[PRS] += ap;
-    aW[1] += d * vx * ae + T[0] * avn + T[3] * avt1 + T[6] * avt2;
-    aW[2] += d * vy * ae + T[1] * avn + T[4] * avt1 + T[7] * avt2;
-    aW[3] += d * vz * ae + T[2] * avn + T[5] * avt1 + T[8] * avt2;
+    aW[1] += d * vx * ae + T[0] * avn + T[3] * avt1 + T[6] * avt2 + avx;
+    aW[2] += d * vy * ae + T[1] * avn + T[4] * avt1 + T[7] * avt2 + avy;
+    aW[3] += d * vz * ae + T[2] * avn + T[5] * avt1 + T[8] * avt2 + avz;
+}
+
+// ---------------------------------------------------------- magnetic flux
+// One side of MagKinFlux (mag_kin) without background field and Boris
+// factor: aF = adjoint of the flux contribution mF (3), aVaD that of the
+// side's VaD term.  Adds into aW (rho, P, v), aB (3) and aBn.
+__device__ __forceinline__ void mag_kin_adj(double s, const double *W, const double *B, double Bn,
+                                            const double *T, const double *aF, double aVaD,
+                                            double *aW, double *aB, double &aBn)
+{
+    const double d = W[DEN], p = W[PRS];
+    const double id = 1.0 / d;
+    const double bx = B[0], by = B[1], bz = B[2];
+    const double dPb = 0.5 * (bx * bx + by * by + bz * bz);
+    const double vac2 = 2 * dPb * id;
+    const double nx = T[0], ny = T[1], nz = T[2];
+    const double vn = nx * W[1] + ny * W[2] + nz * W[3];
+    const double lam = 1 / (2 * p * id + vac2);
+    const double sl = sqrt(lam);
+    const double z = s * sl * vn;
+    double ez;
+    const double vn0 = 0.5 * gam_erfc_exp(z, ez);
+    const double b[3] = {bx, by, bz}, n[3] = {nx, ny, nz};
+    double avn0 = 0.0, adPb = 0.0, ndot = 0.0, bdot = 0.0;
+    for (int x = 0; x < 3; x++) {
+        avn0 += aF[x] * (dPb * n[x] - b[x] * Bn);
+        ndot += aF[x] * n[x];
+        bdot += aF[x] * b[x];
+    }
+    adPb += vn0 * ndot;
+    aBn += -vn0 * bdot;
+    double ab[3];
+    for (int x = 0; x < 3; x++) ab[x] = -vn0 * Bn * aF[x];
+    // VaD += sqrt(vac2) / 2
+    double avac2 = (vac2 > 0.0) ? 0.25 / sqrt(vac2) * aVaD : 0.0;
+    // vn0 = erfc(z)/2, z = s sqrt(lam) vn
+    const double az = -GAM_RSQRT_PI * ez * avn0;
+    const double avn = s * sl * az;
+    const double alam = (s * vn * az) * 0.5 / sl;
+    // lam = 1 / (2 p / d + vac2)
+    const double aX = -lam * lam * alam;
+    double ap = 2 * id * aX, aid = 2 * p * aX;
+    avac2 += aX;
+    // vac2 = 2 dPb / d
+    adPb += 2 * id * avac2;
+    aid += 2 * dPb * avac2;
+    for (int x = 0; x < 3; x++) aB[x] += ab[x] + b[x] * adPb;
+    aW[DEN] += -id * id * aid;
+    aW[PRS] += ap;
+    aW[1] += nx * avn;
+    aW[2] += ny * avn;
+    aW[3] += nz * avn;
+}
+
+// ---------------------------------------------------------- cell B
+// Coefficients of gamCellBxyz (cell_b): dB_x / d(face flux) for the six
+// faces of cell c in the order I, J, K (low), I+1, J+1, K+1 (high);
+// af[f] += sum_x aB[x] dB_x/df.
+__device__ __forceinline__ void cell_b_adj(const Geo &g, const double *met, long c, const double *aB,
+                                           double *af)
+{
+    const long N3 = g.N3;
+    const long fI = c, fJ = c, fK = c, fIp = c + 1, fJp = c + g.P, fKp = c + g.PP;
+    const double idV = 1.0 / met[M_VOL * N3 + c];
+    for (int x = 0; x < 3; x++) {
+        const double xc = met[(M_XYZCC + x) * N3 + c];
+        const double a = aB[x] * idV;
+        af[0] += a * (-met[(M_XFC + x + 0) * N3 + fI] + xc);
+        af[1] += a * (-met[(M_XFC + x + 3) * N3 + fJ] + xc);
+        af[2] += a * (-met[(M_XFC + x + 6) * N3 + fK] + xc);
+        af[3] += a * (met[(M_XFC + x + 0) * N3 + fIp] - xc);
+        af[4] += a * (met[(M_XFC + x + 3) * N3 + fJp] - xc);
+        af[5] += a * (met[(M_XFC + x + 6) * N3 + fKp] - xc);
+    }
+}
+
+// ---------------------------------------------------------- update
+// CellMaxwell (the Maxwell branch of k_update): U' = Maxwell(U, dM), with
+// dt the step's dt.  aUp = adjoint of U'; adds into aU (5) and adM (3).
+__device__ __forceinline__ void maxwell_adj(const MParm &m, const double *U, const double *dM, double dt,
+                                            const double *aUp, double *aU, double *adM)
+{
+    const double d = dmax(U[DEN], m.dFloor);
+    const double id = 1.0 / d;
+    const double mo[3] = {U[MOMX], U[MOMY], U[MOMZ]};
+    const double mp[3] = {mo[0] + dt * dM[0], mo[1] + dt * dM[1], mo[2] + dt * dM[2]};
+    const double m2 = mo[0] * mo[0] + mo[1] * mo[1] + mo[2] * mo[2];
+    const double mp2 = mp[0] * mp[0] + mp[1] * mp[1] + mp[2] * mp[2];
+    const double pr = (m.gamma - 1) * (U[ENER] - 0.5 * m2 * id);
+    double ad = aUp[DEN];
+    double aid = 0.5 * mp2 * aUp[ENER];
+    double amp[3], am[3];
+    for (int x = 0; x < 3; x++) {
+        amp[x] = aUp[1 + x] + mp[x] * id * aUp[ENER];
+        am[x] = amp[x];
+        adM[x] += dt * amp[x];
+    }
+    if (pr > m.pFloor) {
+        const double apr = m.igm1 * aUp[ENER];
+        aU[ENER] += (m.gamma - 1) * apr;
+        const double ake = -(m.gamma - 1) * apr;
+        for (int x = 0; x < 3; x++) am[x] += mo[x] * id * ake;
+        aid += 0.5 * m2 * ake;
+    }
+    ad += -id * id * aid;
+    if (U[DEN] > m.dFloor) aU[DEN] += ad;
+    for (int x = 0; x < 3; x++) aU[1 + x] += am[x];
 }
 
 // ---------------------------------------------------------- predictor
@@ -271,4 +388,126 @@ __device__ __forceinline__ void predict_cell_adj(const MParm &m, const double *Q
     c2p_cell_adj(m, Q, aW, aQ);
     c2p_cell_adj(m, oQ, aoW, aoQ);
     if (m.doLFM) c2p_block_adj(m, Q, aWN, aQ);
+}
+
+// ---------------------------------------------------------- edge EMF
+// Inputs of one edge EMF (k_emf) gathered into local arrays:
+//   A1[8], V1[3][8]: areas and face velocities of the T1 faces along T2
+//   mH1[8], mL1[4]: half-state and limiter-state fluxes of those faces
+//   A2[8], mH2[8], mL2[4]: the same for the T2 faces along T1
+//   te1[3], te2[3]: edge triad rows; tb[4] = xni, yni, xnj, ynj (Teb);
+//   D4[4]: half-state densities of the four cells around the edge;
+//   b01, b02: background edge field (0 without); edge length; cap = the
+//   vDiff bound CFL * edge / dt.
+struct EdgeIn {
+    double A1[8], V1[3][8], mH1[8], mL1[4];
+    double A2[8], mH2[8], mL2[4];
+    double te1[3], te2[3], tb[4], D4[4];
+    double b01, b02, edge, cap;
+};
+struct EdgeAdj {
+    double V1[3][8], mH1[8], mL1[4], mH2[8], mL2[4], D4[4];
+};
+
+// forward (k_emf's arithmetic), also returning the pieces the adjoint needs
+template <int RM, bool BORIS>
+__device__ __forceinline__ double edge_emf(const MParm &m, const EdgeIn &in, double *l1, double *r1,
+                                           double *l2, double *r2)
+{
+    const double rdA = 1.0 / dmax(wsum8([&](int n) { return in.A1[n]; }), GAM_TINY);
+    double vc[3];
+    for (int d = 0; d < 3; d++) vc[d] = wsum8([&](int n) { return in.A1[n] * in.V1[d][n]; }) * rdA;
+    const double v1 = vc[0] * in.te1[0] + vc[1] * in.te1[1] + vc[2] * in.te1[2];
+    const double v2 = vc[0] * in.te2[0] + vc[1] * in.te2[1] + vc[2] * in.te2[2];
+    double vDiff = sqrt(v1 * v1 + v2 * v2);
+    const double Dc = 0.25 * (in.D4[0] + in.D4[1] + in.D4[2] + in.D4[3]);
+    double q[8], nq[4];
+    for (int n = 0; n < 8; n++) q[n] = in.mH1[n] / dmax(in.A1[n], GAM_TINY);
+    for (int n = 0; n < 4; n++) nq[n] = in.mL1[n] / dmax(in.A1[n + 2], GAM_TINY);
+    const double Vi1 = wsum8([&](int n) { return in.A1[n]; });
+    recon_lr<RM>(m.pdmb, in.A1, 1.0 / Vi1, nq, q, *l1, *r1);
+    for (int n = 0; n < 8; n++) q[n] = in.mH2[n] / dmax(in.A2[n], GAM_TINY);
+    for (int n = 0; n < 4; n++) nq[n] = in.mL2[n] / dmax(in.A2[n + 2], GAM_TINY);
+    const double Vi2 = wsum8([&](int n) { return in.A2[n]; });
+    recon_lr<RM>(m.pdmb, in.A2, 1.0 / Vi2, nq, q, *l2, *r2);
+    const double bT1 = 0.5 * (*l1 + *r1), bT2 = 0.5 * (*l2 + *r2);
+    const double Jd = (*r2 - *l2) - (*r1 - *l1);
+    const double xni = in.tb[0], yni = in.tb[1], xnj = in.tb[2], ynj = in.tb[3];
+    const double detT = 1.0 / (xni * ynj - xnj * yni);
+    const double b1 = detT * (ynj * bT1 - yni * bT2) + in.b01;
+    const double b2 = detT * (-(xnj * bT1) + xni * bT2) + in.b02;
+    double vA = sqrt(b1 * b1 + b2 * b2) / sqrt(Dc);
+    if (BORIS) vA = m.Ca * vA / sqrt(m.Ca * m.Ca + vA * vA);
+    vDiff = dmin(vDiff + vA, in.cap);
+    return (-(v1 * b2 - v2 * b1) + m.Vd0 * vDiff * Jd) * in.edge;
+}
+
+template <int RM, bool BORIS>
+__device__ __forceinline__ void edge_emf_adj(const MParm &m, const EdgeIn &in, double aE, EdgeAdj &out)
+{
+    double l1, r1, l2, r2;
+    edge_emf<RM, BORIS>(m, in, &l1, &r1, &l2, &r2);
+    // forward pieces again
+    const double rdA = 1.0 / dmax(wsum8([&](int n) { return in.A1[n]; }), GAM_TINY);
+    double vc[3];
+    for (int d = 0; d < 3; d++) vc[d] = wsum8([&](int n) { return in.A1[n] * in.V1[d][n]; }) * rdA;
+    const double v1 = vc[0] * in.te1[0] + vc[1] * in.te1[1] + vc[2] * in.te1[2];
+    const double v2 = vc[0] * in.te2[0] + vc[1] * in.te2[1] + vc[2] * in.te2[2];
+    const double vD = sqrt(v1 * v1 + v2 * v2);
+    const double Dc = 0.25 * (in.D4[0] + in.D4[1] + in.D4[2] + in.D4[3]);
+    const double bT1 = 0.5 * (l1 + r1), bT2 = 0.5 * (l2 + r2);
+    const double Jd = (r2 - l2) - (r1 - l1);
+    const double xni = in.tb[0], yni = in.tb[1], xnj = in.tb[2], ynj = in.tb[3];
+    const double detT = 1.0 / (xni * ynj - xnj * yni);
+    const double b1 = detT * (ynj * bT1 - yni * bT2) + in.b01;
+    const double b2 = detT * (-(xnj * bT1) + xni * bT2) + in.b02;
+    const double nb = sqrt(b1 * b1 + b2 * b2);
+    const double u = nb / sqrt(Dc);
+    double vA = u;
+    if (BORIS) vA = m.Ca * u / sqrt(m.Ca * m.Ca + u * u);
+    const double vsum = vD + vA;
+    const double vDiff = dmin(vsum, in.cap);
+    // adjoint
+    const double ae = in.edge * aE;
+    double av1 = -b2 * ae, av2 = b1 * ae, ab1 = v2 * ae, ab2 = -v1 * ae;
+    const double avDiff = m.Vd0 * Jd * ae, aJd = m.Vd0 * vDiff * ae;
+    const double avs = (vsum < in.cap) ? avDiff : 0.0;
+    double au = avs;
+    if (BORIS) {
+        const double c2 = m.Ca * m.Ca + u * u;
+        au = avs * m.Ca * m.Ca * m.Ca / (c2 * sqrt(c2));
+    }
+    const double anb = au / sqrt(Dc);
+    const double aDc = -0.5 * u / Dc * au;
+    if (nb > 0.0) { ab1 += b1 / nb * anb; ab2 += b2 / nb * anb; }
+    const double abT1 = detT * (ynj * ab1 - xnj * ab2);
+    const double abT2 = detT * (-yni * ab1 + xni * ab2);
+    double al1 = aJd + 0.5 * abT1, ar1 = -aJd + 0.5 * abT1;
+    double al2 = -aJd + 0.5 * abT2, ar2 = aJd + 0.5 * abT2;
+    if (vD > 0.0) { av1 += v1 / vD * avs; av2 += v2 / vD * avs; }
+    // edge B reconstructions
+    {
+        double q[8], qv[8], nq[4], aqv[8] = {0, 0, 0, 0, 0, 0, 0, 0}, anq[4] = {0, 0, 0, 0};
+        for (int n = 0; n < 8; n++) { q[n] = in.mH1[n] / dmax(in.A1[n], GAM_TINY); qv[n] = in.A1[n] * q[n]; }
+        for (int n = 0; n < 4; n++) nq[n] = in.mL1[n] / dmax(in.A1[n + 2], GAM_TINY);
+        const double Vi = wsum8([&](int n) { return in.A1[n]; });
+        recon_lr_w_adj<RM>(m.pdmb, 1.0 / Vi, nq, qv, al1, ar1, anq, aqv);
+        for (int n = 0; n < 8; n++) out.mH1[n] += in.A1[n] * aqv[n] / dmax(in.A1[n], GAM_TINY);
+        for (int n = 0; n < 4; n++) out.mL1[n] += anq[n] / dmax(in.A1[n + 2], GAM_TINY);
+    }
+    {
+        double q[8], qv[8], nq[4], aqv[8] = {0, 0, 0, 0, 0, 0, 0, 0}, anq[4] = {0, 0, 0, 0};
+        for (int n = 0; n < 8; n++) { q[n] = in.mH2[n] / dmax(in.A2[n], GAM_TINY); qv[n] = in.A2[n] * q[n]; }
+        for (int n = 0; n < 4; n++) nq[n] = in.mL2[n] / dmax(in.A2[n + 2], GAM_TINY);
+        const double Vi = wsum8([&](int n) { return in.A2[n]; });
+        recon_lr_w_adj<RM>(m.pdmb, 1.0 / Vi, nq, qv, al2, ar2, anq, aqv);
+        for (int n = 0; n < 8; n++) out.mH2[n] += in.A2[n] * aqv[n] / dmax(in.A2[n], GAM_TINY);
+        for (int n = 0; n < 4; n++) out.mL2[n] += anq[n] / dmax(in.A2[n + 2], GAM_TINY);
+    }
+    // corner velocity and density
+    double avc[3];
+    for (int d = 0; d < 3; d++) avc[d] = in.te1[d] * av1 + in.te2[d] * av2;
+    for (int d = 0; d < 3; d++)
+        for (int n = 0; n < 8; n++) out.V1[d][n] += W8(n) * in.A1[n] * rdA * avc[d];
+    for (int n = 0; n < 4; n++) out.D4[n] += 0.25 * aDc;
 }

@@ -1,14 +1,17 @@
-// gam_adj.hip -- gfx950 kernels of the discrete adjoint of the
-// hydrodynamic GAMERA step (doMHD = 0): the transposes of the boundary
-// copies, of the Reynolds update (Flux2Deltas + applyUpdate,
-// mhdgroup.F90:79-121, stress.F90:152-194), of the interface fluxes
-// (Fluxes, stress.F90:204-429: reconstruction, PDM, GasKinFlux) and of the
-// predictor (mhdgroup.F90:444-561).
+// gam_adj.hip -- gfx950 kernels of the discrete adjoint of the GAMERA
+// step: the transposes of the boundary copies and of the ghost bFlux2Fld,
+// of the update (Flux2Deltas, Reynolds, E2Flux/CT, cell B, CellMaxwell;
+// mhdgroup.F90:79-329, stress.F90:152-194, fields.F90:18-51), of the
+// interface fluxes (Fluxes, stress.F90:204-599: reconstruction, PDM,
+// GasKinFlux, MagKinFlux, HOGS), of the edge EMF (CalcElecField,
+// fields.F90:70-517), of Mom2Face and of the predictor
+// (mhdgroup.F90:444-561).
 //
 // Adjoint planes hold the derivative of the objective with respect to the
-// same-named forward plane.  The flux adjoint scatters into the stencil
-// cells with FP64 atomic adds (hardware global_atomic_add_f64); every other
-// kernel gathers.  dt is the forward run's value at each step (the CFL
+// same-named forward plane; an adjoint state set has the State layout's
+// first 11 planes (gas 5, face flux 3, cell B 3).  Kernels whose forward
+// gathers over a stencil (fluxes, EMF, Mom2Face, cell B) scatter with FP64
+// atomic adds (hardware global_atomic_add_f64); the others gather.  dt is the forward run's value at each step (the CFL
 // minimum is held fixed, as the finite-difference checks also do).
 #include "gam_adj.h"
 #include "gam_kernels.h"
@@ -229,6 +232,419 @@ __global__ void k_set_dt(Clock *clk, const Clock *rec)
     *clk = c;
 }
 
+// ============================================================ MHD
+#define AADD(ptr, v) unsafeAtomicAdd((ptr), (v))
+
+// Transpose of bFlux2Fld over cells (ghost cells, or every cell with
+// all = true): each cell's B adjoint goes to its six face fluxes.
+__global__ __launch_bounds__(256) void k_cellb_adj(Geo g, Box cb, int all, double *__restrict__ aS,
+                                                   const double *__restrict__ met)
+{
+    POINT_IN(cb);
+    if (!all && i >= g.is && i <= g.ie && j >= g.js && j <= g.je && k >= g.ks && k <= g.ke) return;
+    const long N3 = g.N3;
+    const long c = IX(g, i, j, k);
+    double aB[3], af[6] = {0, 0, 0, 0, 0, 0};
+    for (int x = 0; x < 3; x++) {
+        aB[x] = aS[(S_B + x) * N3 + c];
+        aS[(S_B + x) * N3 + c] = 0.0;
+    }
+    cell_b_adj(g, met, c, aB, af);
+    double *mf = aS + S_MF * N3;
+    AADD(mf + c, af[0]);
+    AADD(mf + N3 + c, af[1]);
+    AADD(mf + 2 * N3 + c, af[2]);
+    AADD(mf + c + 1, af[3]);
+    AADD(mf + N3 + c + g.P, af[4]);
+    AADD(mf + 2 * N3 + c + g.PP, af[5]);
+}
+
+__device__ __forceinline__ void move_adj(double *p, long dst, long src)
+{
+    p[src] += p[dst];
+    p[dst] = 0.0;
+}
+
+// Transpose of the periodic BC's copies (k_bc_periodic): gas and cell B of
+// ghost cells, and the face fluxes with the outer sides' shifted faces.
+template <int SIDE>
+__global__ void k_bc_periodic_adj(Geo g, Box t, double *__restrict__ aS)
+{
+    const int a = t.i0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int b = t.j0 + (int)blockIdx.y;
+    if (a > t.i1 || b > t.j1) return;
+    const long N3 = g.N3;
+    double *mf = aS + S_MF * N3;
+    for (int n = 4; n >= 1; n--) {
+        int gi, gj, gk, si, sj, sk;
+        if (SIDE == 0) { gi = g.is - n; si = g.ie - n + 1; gj = sj = a; gk = sk = b; }
+        else if (SIDE == 1) { gi = g.ie + n; si = g.is + n - 1; gj = sj = a; gk = sk = b; }
+        else if (SIDE == 2) { gj = g.js - n; sj = g.je - n + 1; gi = si = a; gk = sk = b; }
+        else if (SIDE == 3) { gj = g.je + n; sj = g.js + n - 1; gi = si = a; gk = sk = b; }
+        else if (SIDE == 4) { gk = g.ks - n; sk = g.ke - n + 1; gi = si = a; gj = sj = b; }
+        else { gk = g.ke + n; sk = g.ks + n - 1; gi = si = a; gj = sj = b; }
+        const long d = IX(g, gi, gj, gk), s = IX(g, si, sj, sk);
+        // face fluxes (reverse of the forward statement order)
+        if (SIDE == 1) {
+            move_adj(mf, IX(g, gi + 1, gj, gk), IX(g, g.is + n, gj, gk));
+            move_adj(mf + 2 * N3, d, s);
+            move_adj(mf + N3, d, s);
+        } else if (SIDE == 3) {
+            move_adj(mf + N3, IX(g, gi, gj + 1, gk), IX(g, gi, g.js + n, gk));
+            move_adj(mf + 2 * N3, d, s);
+            move_adj(mf, d, s);
+        } else if (SIDE == 5) {
+            move_adj(mf + 2 * N3, IX(g, gi, gj, gk + 1), IX(g, gi, gj, g.ks + n));
+            move_adj(mf + N3, d, s);
+            move_adj(mf, d, s);
+        } else {
+            for (int x = 2; x >= 0; x--) move_adj(mf + x * N3, d, s);
+        }
+        if (gi <= g.ieg && gj <= g.jeg && gk <= g.keg) {
+            for (int x = 2; x >= 0; x--) move_adj(aS + (S_B + x) * N3, d, s);
+            for (int v = 4; v >= 0; v--) move_adj(aS + v * N3, d, s);
+        }
+    }
+}
+
+// Update transpose, part 1 (cells): gas of the Reynolds + Maxwell update
+// (active cells) or the copy (other cells).  Writes aUr (the adjoint of the
+// Reynolds-updated gas) and adM (of the magnetic stress divergence).
+__global__ __launch_bounds__(256) void k_update_adj_cells(Geo g, MParm m, Box cb, Box ab,
+                                                          const Clock *clk,
+                                                          const double *__restrict__ So,
+                                                          const double *__restrict__ gflx,
+                                                          const double *__restrict__ mflx,
+                                                          const double *__restrict__ met,
+                                                          const double *__restrict__ bg, int BG,
+                                                          const double *__restrict__ aSn,
+                                                          double *__restrict__ aSo,
+                                                          double *__restrict__ aUr,
+                                                          double *__restrict__ adM)
+{
+    POINT_IN(cb);
+    const long N3 = g.N3;
+    const long c = IX(g, i, j, k);
+    if (!inside(ab, i, j, k)) {
+        for (int v = 0; v < 5; v++) aSo[v * N3 + c] += aSn[v * N3 + c];
+        return;
+    }
+    const double idV = 1.0 / met[M_VOL * N3 + c];
+    const double dtM = clk->dt, dt = clk->dtA;
+    const long oI = c + 1, oJ = c + g.P, oK = c + g.PP;
+    double U[5], dM[3], aUp[5], aU[5] = {0, 0, 0, 0, 0}, adm[3] = {0, 0, 0};
+    for (int v = 0; v < 5; v++) {
+        const double *gv = gflx + (long)v * N3;
+        const double dH = (gv[c] - gv[oI] + gv[5 * N3 + c] - gv[5 * N3 + oJ] + gv[10 * N3 + c] - gv[10 * N3 + oK]) * idV;
+        U[v] = So[v * N3 + c] + dtM * dH;
+        aUp[v] = aSn[v * N3 + c];
+    }
+    for (int x = 0; x < 3; x++) {
+        const double *mv = mflx + (long)x * N3;
+        double r = (mv[c] - mv[oI] + mv[3 * N3 + c] - mv[3 * N3 + oJ] + mv[6 * N3 + c] - mv[6 * N3 + oK]) * idV;
+        if (BG) r = r + bg[(B_DPB0 + x) * N3 + c];
+        dM[x] = r;
+    }
+    maxwell_adj(m, U, dM, dt, aUp, aU, adm);
+    for (int v = 0; v < 5; v++) {
+        aSo[v * N3 + c] += aU[v];
+        aUr[v * N3 + c] = aU[v];
+    }
+    for (int x = 0; x < 3; x++) adM[x * N3 + c] = adm[x];
+}
+
+// Update transpose, part 2 (faces): adjoints of the active faces' gas and
+// magnetic fluxes, of the old face fluxes (copy + CT) and of the edge EMFs.
+__global__ __launch_bounds__(256) void k_update_adj_faces(Geo g, Box fb, Box ab, Box ctb, EdgeBoxes eb,
+                                                          const Clock *clk,
+                                                          const double *__restrict__ met,
+                                                          const double *__restrict__ aUr,
+                                                          const double *__restrict__ adM,
+                                                          const double *__restrict__ aSn,
+                                                          double *__restrict__ aSo,
+                                                          double *__restrict__ ag,
+                                                          double *__restrict__ amf,
+                                                          double *__restrict__ aE)
+{
+    POINT_IN(fb);
+    const long N3 = g.N3;
+    const long c = IX(g, i, j, k);
+    const double dtM = clk->dt, dtA = clk->dtA;
+    const long st[3] = {1, g.P, g.PP};
+    for (int d = 0; d < 3; d++) {
+        const bool face = i >= g.is && j >= g.js && k >= g.ks && i <= g.ie + (d == 0) &&
+                          j <= g.je + (d == 1) && k <= g.ke + (d == 2);
+        if (!face) continue;
+        const long cl = c - st[d];
+        const bool up = inside(ab, i, j, k);
+        const bool lo = inside(ab, i - (d == 0), j - (d == 1), k - (d == 2));
+        const double wu = up ? 1.0 / met[M_VOL * N3 + c] : 0.0;
+        const double wl = lo ? 1.0 / met[M_VOL * N3 + cl] : 0.0;
+        // aUr, adM hold values at active cells only
+        for (int v = 0; v < 5; v++)
+            ag[(v + 5 * d) * N3 + c] = dtM * ((up ? wu * aUr[v * N3 + c] : 0.0) - (lo ? wl * aUr[v * N3 + cl] : 0.0));
+        for (int x = 0; x < 3; x++)
+            amf[(x + 3 * d) * N3 + c] = (up ? wu * adM[x * N3 + c] : 0.0) - (lo ? wl * adM[x * N3 + cl] : 0.0);
+    }
+    const double *aF = aSn + S_MF * N3;   // adjoint of the new face fluxes
+    for (int d = 0; d < 3; d++) aSo[(S_MF + d) * N3 + c] += aF[d * N3 + c];
+    // CT: new flux(d, f) = old - dtA * (curl of E around f); gather per edge
+    auto fa = [&](int d, int ii, int jj, int kk) -> double {
+        return inside(ctb, ii, jj, kk) ? aF[d * N3 + IX(g, ii, jj, kk)] : 0.0;
+    };
+    aE[c] = inside(eb.e[0], i, j, k)
+                ? dtA * (fa(1, i, j, k) - fa(1, i, j, k - 1) + fa(2, i, j - 1, k) - fa(2, i, j, k)) : 0.0;
+    aE[N3 + c] = inside(eb.e[1], i, j, k)
+                ? dtA * (fa(0, i, j, k - 1) - fa(0, i, j, k) + fa(2, i, j, k) - fa(2, i - 1, j, k)) : 0.0;
+    aE[2 * N3 + c] = inside(eb.e[2], i, j, k)
+                ? dtA * (fa(0, i, j, k) - fa(0, i, j - 1, k) + fa(1, i - 1, j, k) - fa(1, i, j, k)) : 0.0;
+}
+
+// Interface fluxes' transpose with the magnetic terms (no background
+// field, no Boris): gas and magnetic kinetic fluxes, HOGS, L/R states of
+// gas (H_WH, H_WN) and cell B (H_B weighted by volume, limiter S_B), and
+// the normal flux Bn = H_MF / A.
+template <int D, int RM>
+__global__ __launch_bounds__(256) void k_flux_adj_mhd(Geo g, MParm m, Box fb,
+                                                      const double *__restrict__ H,
+                                                      const double *__restrict__ S,
+                                                      const double *__restrict__ met,
+                                                      const double *__restrict__ ag,
+                                                      const double *__restrict__ amf,
+                                                      double *__restrict__ aWH,
+                                                      double *__restrict__ aWN,
+                                                      double *__restrict__ aHB,
+                                                      double *__restrict__ aHMF,
+                                                      double *__restrict__ aSS)
+{
+    POINT_IN(fb);
+    const long N3 = g.N3;
+    const long c = IX(g, i, j, k);
+    const long s = STRIDE<D>(g);
+    double vol[8];
+    for (int n = 0; n < 8; n++) vol[n] = met[M_VOL * N3 + c + (n - 4) * s];
+    const double rVi = 1.0 / wsum8([&](int n) { return vol[n]; });
+    double WL[5], WR[5], BL[3], BR[3];
+    for (int v = 0; v < 5; v++) {
+        double qv[8], nq[4];
+        for (int n = 0; n < 8; n++) qv[n] = H[(H_WH + v) * N3 + c + (n - 4) * s];
+        for (int n = 0; n < 4; n++) nq[n] = H[(H_WN + v) * N3 + c + (n - 2) * s];
+        recon_lr_w<RM>(m.pdmb, rVi, nq, qv, WL[v], WR[v]);
+    }
+    const double *Bl = m.doLFM ? S + S_B * N3 : H + H_B * N3;
+    for (int x = 0; x < 3; x++) {
+        double q[8], nq[4];
+        for (int n = 0; n < 8; n++) q[n] = H[(H_B + x) * N3 + c + (n - 4) * s];
+        for (int n = 0; n < 4; n++) nq[n] = Bl[x * N3 + c + (n - 2) * s];
+        recon_lr<RM>(m.pdmb, vol, rVi, nq, q, BL[x], BR[x]);
+    }
+    double T[9];
+    for (int q = 0; q < 9; q++) T[q] = met[(M_TF + q + 9 * D) * N3 + c];
+    const double faB = met[(M_FACE + D) * N3 + c];
+    const double Bn = H[(H_MF + D) * N3 + c] / faB;
+    // forward pieces the adjoint needs: VaD and the HOGS differences dW
+    double mF[3] = {0, 0, 0}, bbD = 0.0, VaD = 0.0, B0[3] = {0, 0, 0};
+    mag_kin<false, false>(m, -1.0, WL, BL, Bn, T, B0, 0.0, mF, bbD, VaD);
+    mag_kin<false, false>(m, +1.0, WR, BR, Bn, T, B0, 0.0, mF, bbD, VaD);
+    double gF[5] = {0, 0, 0, 0, 0}, dW[5] = {0, 0, 0, 0, 0};
+    gas_kin(m.igm1, -1.0, WL, T, gF, dW);
+    gas_kin(m.igm1, +1.0, WR, T, gF, dW);
+    // adjoint
+    double aF[5], adW[5], aM[3], aVaD = 0.0;
+    for (int v = 0; v < 5; v++) {
+        aF[v] = faB * ag[(v + 5 * D) * N3 + c];
+        adW[v] = -m.cHogH * VaD * aF[v];
+        aVaD += -m.cHogH * dW[v] * aF[v];
+    }
+    for (int x = 0; x < 3; x++) aM[x] = faB * amf[(x + 3 * D) * N3 + c];
+    double aL[5] = {0, 0, 0, 0, 0}, aR[5] = {0, 0, 0, 0, 0}, abL[3] = {0, 0, 0}, abR[3] = {0, 0, 0}, aBn = 0.0;
+    gas_kin_adj(m.igm1, -1.0, WL, T, aF, aL, adW);
+    gas_kin_adj(m.igm1, +1.0, WR, T, aF, aR, adW);
+    mag_kin_adj(-1.0, WL, BL, Bn, T, aM, aVaD, aL, abL, aBn);
+    mag_kin_adj(+1.0, WR, BR, Bn, T, aM, aVaD, aR, abR, aBn);
+    AADD(aHMF + D * N3 + c, aBn / faB);
+    for (int v = 0; v < 5; v++) {
+        double qv[8], nq[4], aqv[8] = {0, 0, 0, 0, 0, 0, 0, 0}, anq[4] = {0, 0, 0, 0};
+        for (int n = 0; n < 8; n++) qv[n] = H[(H_WH + v) * N3 + c + (n - 4) * s];
+        for (int n = 0; n < 4; n++) nq[n] = H[(H_WN + v) * N3 + c + (n - 2) * s];
+        recon_lr_w_adj<RM>(m.pdmb, rVi, nq, qv, aL[v], aR[v], anq, aqv);
+        for (int n = 0; n < 8; n++) AADD(aWH + v * N3 + c + (n - 4) * s, aqv[n]);
+        for (int n = 0; n < 4; n++) AADD(aWN + v * N3 + c + (n - 2) * s, anq[n]);
+    }
+    double *aBl = m.doLFM ? aSS + S_B * N3 : aHB;
+    for (int x = 0; x < 3; x++) {
+        double qv[8], nq[4], aqv[8] = {0, 0, 0, 0, 0, 0, 0, 0}, anq[4] = {0, 0, 0, 0};
+        for (int n = 0; n < 8; n++) qv[n] = vol[n] * H[(H_B + x) * N3 + c + (n - 4) * s];
+        for (int n = 0; n < 4; n++) nq[n] = Bl[x * N3 + c + (n - 2) * s];
+        recon_lr_w_adj<RM>(m.pdmb, rVi, nq, qv, abL[x], abR[x], anq, aqv);
+        for (int n = 0; n < 8; n++) AADD(aHB + x * N3 + c + (n - 4) * s, vol[n] * aqv[n]);
+        for (int n = 0; n < 4; n++) AADD(aBl + x * N3 + c + (n - 2) * s, anq[n]);
+    }
+}
+
+// Edge EMF transpose (k_emf): scatters into the face velocities, the
+// half-state density, the half-state and limiter-state face fluxes.
+template <int ED, int RM, bool BG, bool BORIS>
+__global__ __launch_bounds__(256) void k_emf_adj(Geo g, MParm m, Box eb, const Clock *clk,
+                                                 const double *__restrict__ H,
+                                                 const double *__restrict__ S,
+                                                 const double *__restrict__ Vf,
+                                                 const double *__restrict__ met,
+                                                 const double *__restrict__ bg,
+                                                 const double *__restrict__ aE,
+                                                 double *__restrict__ aVf,
+                                                 double *__restrict__ aHG,
+                                                 double *__restrict__ aHMF,
+                                                 double *__restrict__ aSS)
+{
+    POINT_IN(eb);
+    constexpr int T1 = (ED == 0) ? 1 : (ED == 1) ? 2 : 0;
+    constexpr int T2 = (ED == 0) ? 2 : (ED == 1) ? 0 : 1;
+    constexpr int E1 = (ED == 0) ? 1 : 0;
+    constexpr int E2 = (ED == 2) ? 1 : 2;
+    const long N3 = g.N3;
+    const long c = IX(g, i, j, k);
+    const long s1 = STRIDE<T1>(g), s2 = STRIDE<T2>(g);
+    const long se1 = STRIDE<E1>(g), se2 = STRIDE<E2>(g);
+    const double *mfH = H + H_MF * N3;
+    const double *mfL = m.doLFM ? S + S_MF * N3 : mfH;
+    EdgeIn in;
+    for (int n = 0; n < 8; n++) {
+        const long q1 = c + (n - 4) * s2, q2 = c + (n - 4) * s1;
+        in.A1[n] = met[(M_FACE + T1) * N3 + q1];
+        in.mH1[n] = mfH[T1 * N3 + q1];
+        for (int d = 0; d < 3; d++) in.V1[d][n] = Vf[(3 * T1 + d) * N3 + q1];
+        in.A2[n] = met[(M_FACE + T2) * N3 + q2];
+        in.mH2[n] = mfH[T2 * N3 + q2];
+    }
+    for (int n = 0; n < 4; n++) {
+        in.mL1[n] = mfL[T1 * N3 + c + (n - 2) * s2];
+        in.mL2[n] = mfL[T2 * N3 + c + (n - 2) * s1];
+    }
+    const double *Te = met + (long)(M_TE + 9 * ED) * N3 + c;
+    for (int d = 0; d < 3; d++) { in.te1[d] = Te[(3 + d) * N3]; in.te2[d] = Te[(6 + d) * N3]; }
+    const double *Tb = met + (long)(M_TEB + 4 * ED) * N3 + c;
+    for (int q = 0; q < 4; q++) in.tb[q] = Tb[q * N3];
+    const long dc[4] = {c, c - se1, c - se2, c - se1 - se2};
+    for (int q = 0; q < 4; q++) in.D4[q] = H[(H_GAS + DEN) * N3 + dc[q]];
+    in.b01 = BG ? bg[(B_EDGB0 + 0 + 2 * ED) * N3 + c] : 0.0;
+    in.b02 = BG ? bg[(B_EDGB0 + 1 + 2 * ED) * N3 + c] : 0.0;
+    in.edge = met[(M_EDGE + ED) * N3 + c];
+    in.cap = m.CFL * in.edge / clk->dt;
+    EdgeAdj o = {};
+    edge_emf_adj<RM, BORIS>(m, in, aE[ED * N3 + c], o);
+    double *aL = m.doLFM ? aSS + S_MF * N3 : aHMF;
+    for (int n = 0; n < 8; n++) {
+        const long q1 = c + (n - 4) * s2, q2 = c + (n - 4) * s1;
+        for (int d = 0; d < 3; d++) AADD(aVf + (3 * T1 + d) * N3 + q1, o.V1[d][n]);
+        AADD(aHMF + T1 * N3 + q1, o.mH1[n]);
+        AADD(aHMF + T2 * N3 + q2, o.mH2[n]);
+    }
+    for (int n = 0; n < 4; n++) {
+        AADD(aL + T1 * N3 + c + (n - 2) * s2, o.mL1[n]);
+        AADD(aL + T2 * N3 + c + (n - 2) * s1, o.mL2[n]);
+        AADD(aHG + DEN * N3 + dc[n], o.D4[n]);
+    }
+}
+
+// Mom2Face transpose (k_vface): face velocities of direction d from the
+// volume-weighted half-state velocities of the 8 cells along d.
+__global__ __launch_bounds__(256) void k_vface_adj(Geo g, Box cb, const double *__restrict__ met,
+                                                   const double *__restrict__ aVf,
+                                                   double *__restrict__ aWH)
+{
+    POINT_IN(cb);
+    const long N3 = g.N3;
+    const long c = IX(g, i, j, k);
+    const double *vol = met + (long)M_VOL * N3;
+    for (int d = 0; d < 3; d++) {
+        const bool on = d == 0 ? (i >= g.is && i <= g.ie + 1)
+                      : d == 1 ? (j >= g.js && j <= g.je + 1) : (k >= g.ks && k <= g.ke + 1);
+        if (!on) continue;
+        const long s = d == 0 ? 1 : d == 1 ? (long)g.P : g.PP;
+        double v8[8];
+        bool in8[8];
+        for (int n = 0; n < 8; n++) {
+            const int kk = k + (n - 4);
+            in8[n] = d != 2 || (kk >= g.ksg && kk <= g.keg);   // the k stencil stops at the grid
+            v8[n] = in8[n] ? vol[c + (n - 4) * s] : 0.0;
+        }
+        const double r = 1.0 / dmax(wsum8([&](int n) { return v8[n]; }), GAM_TINY);
+        for (int x = 0; x < 3; x++) {
+            const double a = aVf[(3 * d + x) * N3 + c] * r;
+            for (int n = 0; n < 8; n++)
+                if (in8[n]) AADD(aWH + (1 + x) * N3 + c + (n - 4) * s, W8(n) * a);
+        }
+    }
+}
+
+// Predictor transpose, magnetic part: half-state cell B from the
+// extrapolated face fluxes (scatter into aHMF), then the extrapolation
+// H_MF = S_MF + w (S_MF - O_MF) at every face point.
+__global__ __launch_bounds__(256) void k_predict_b_adj(Geo g, Box cb, const double *__restrict__ met,
+                                                       const double *__restrict__ aHB,
+                                                       double *__restrict__ aHMF)
+{
+    POINT_IN(cb);
+    const long N3 = g.N3;
+    const long c = IX(g, i, j, k);
+    double aB[3] = {aHB[c], aHB[N3 + c], aHB[2 * N3 + c]}, af[6] = {0, 0, 0, 0, 0, 0};
+    cell_b_adj(g, met, c, aB, af);
+    AADD(aHMF + c, af[0]);
+    AADD(aHMF + N3 + c, af[1]);
+    AADD(aHMF + 2 * N3 + c, af[2]);
+    AADD(aHMF + c + 1, af[3]);
+    AADD(aHMF + N3 + c + g.P, af[4]);
+    AADD(aHMF + 2 * N3 + c + g.PP, af[5]);
+}
+
+__global__ __launch_bounds__(256) void k_predict_mf_adj(Geo g, Box fb, const Clock *clk,
+                                                        const double *__restrict__ aHMF,
+                                                        double *__restrict__ aS,
+                                                        double *__restrict__ aO)
+{
+    POINT_IN(fb);
+    const long N3 = g.N3;
+    const long c = IX(g, i, j, k);
+    const double w = ab2_weight(clk);
+    for (int d = 0; d < 3; d++) {
+        const double a = aHMF[d * N3 + c];
+        aS[(S_MF + d) * N3 + c] += (1.0 + w) * a;
+        aO[(S_MF + d) * N3 + c] = -w * a;
+    }
+}
+
+// predictor gas transpose with the half-state gas adjoint (the EMF's
+// corner density) included
+__global__ __launch_bounds__(256) void k_predict_adj_mhd(Geo g, MParm m, Box cb, const Clock *clk,
+                                                         const double *__restrict__ S,
+                                                         const double *__restrict__ O,
+                                                         const double *__restrict__ met,
+                                                         const double *__restrict__ aHG,
+                                                         const double *__restrict__ aWH,
+                                                         const double *__restrict__ aWN,
+                                                         double *__restrict__ aS,
+                                                         double *__restrict__ aO)
+{
+    POINT_IN(cb);
+    const long N3 = g.N3;
+    const long c = IX(g, i, j, k);
+    const double w = ab2_weight(clk);
+    double Q[5], oQ[5], ahg[5], awh[5], awn[5], aQ[5] = {0, 0, 0, 0, 0}, aoQ[5] = {0, 0, 0, 0, 0};
+    for (int v = 0; v < 5; v++) {
+        Q[v] = S[(S_GAS + v) * N3 + c];
+        oQ[v] = O[(S_GAS + v) * N3 + c];
+        ahg[v] = aHG[v * N3 + c];
+        awh[v] = aWH[v * N3 + c];
+        awn[v] = aWN[v * N3 + c];
+    }
+    predict_cell_adj(m, Q, oQ, w, met[M_VOL * N3 + c], ahg, awh, awn, aQ, aoQ);
+    for (int v = 0; v < 5; v++) {
+        aS[v * N3 + c] += aQ[v];
+        aO[v * N3 + c] = aoQ[v];
+    }
+}
+
 // ------------------------------------------------------------ launchers
 void launch_adj_objective(hipStream_t s, const Geo &g, const MParm &m, const double *S,
                           const double *wgt, double *aS, double *part, int *nparts)
@@ -300,4 +716,118 @@ void launch_predict_adj(hipStream_t s, const Geo &g, const MParm &m, const Clock
 void launch_set_dt(hipStream_t s, Clock *clk, const Clock *rec)
 {
     hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(64), 0, s, clk, rec);
+}
+
+// ------------------------------------------------------------ MHD launchers
+static inline Box cells_of(const Geo &g) { return Box{g.isg, g.ieg, g.jsg, g.jeg, g.ksg, g.keg}; }
+static inline Box faces_of(const Geo &g) { return Box{g.isg, g.ieg + 1, g.jsg, g.jeg + 1, g.ksg, g.keg + 1}; }
+
+void launch_cellb_adj(hipStream_t s, const Geo &g, int all, double *aS, const double *met)
+{
+    const Box cb = cells_of(g);
+    hipLaunchKernelGGL(k_cellb_adj, agrid(cb), dim3(ABX, ABY), 0, s, g, cb, all, aS, met);
+}
+
+template <int SIDE>
+static void bcp_adj_side(hipStream_t s, const Geo &g, double *aS)
+{
+    Box t;
+    if (SIDE <= 1) { t.i0 = g.jsg; t.i1 = g.jeg + 1; t.j0 = g.ksg; t.j1 = g.keg + 1; }
+    else if (SIDE <= 3) { t.i0 = g.isg; t.i1 = g.ieg + 1; t.j0 = g.ksg; t.j1 = g.keg + 1; }
+    else { t.i0 = g.isg; t.i1 = g.ieg + 1; t.j0 = g.jsg; t.j1 = g.jeg + 1; }
+    t.k0 = t.k1 = 0;
+    const dim3 gr((unsigned)((t.i1 - t.i0 + 64) / 64), (unsigned)(t.j1 - t.j0 + 1));
+    hipLaunchKernelGGL(k_bc_periodic_adj<SIDE>, gr, dim3(64), 0, s, g, t, aS);
+}
+
+void launch_bc_periodic_adj(hipStream_t s, int side, const Geo &g, double *aS)
+{
+    switch (side) {
+    case 0: bcp_adj_side<0>(s, g, aS); break;
+    case 1: bcp_adj_side<1>(s, g, aS); break;
+    case 2: bcp_adj_side<2>(s, g, aS); break;
+    case 3: bcp_adj_side<3>(s, g, aS); break;
+    case 4: bcp_adj_side<4>(s, g, aS); break;
+    default: bcp_adj_side<5>(s, g, aS); break;
+    }
+}
+
+void launch_update_adj_mhd(hipStream_t s, const Geo &g, const MParm &m, const Box &ab, const Box &ctb,
+                           const Box *eboxes, const Clock *clk, const double *So, const double *gflx,
+                           const double *mflx, const double *met, const double *bg, const double *aSn,
+                           double *aSo, double *aUr, double *adM, double *ag, double *amf, double *aE)
+{
+    const Box cb = cells_of(g), fb = faces_of(g);
+    hipLaunchKernelGGL(k_update_adj_cells, agrid(cb), dim3(ABX, ABY), 0, s, g, m, cb, ab, clk, So, gflx, mflx,
+                       met, bg, m.doBG, aSn, aSo, aUr, adM);
+    EdgeBoxes eb;
+    for (int d = 0; d < 3; d++) eb.e[d] = eboxes[d];
+    hipLaunchKernelGGL(k_update_adj_faces, agrid(fb), dim3(ABX, ABY), 0, s, g, fb, ab, ctb, eb, clk, met, aUr,
+                       adM, aSn, aSo, ag, amf, aE);
+}
+
+template <int D>
+static void flux_adj_mhd_d(hipStream_t s, const Geo &g, const MParm &m, const Box &fb, const double *H,
+                           const double *S, const double *met, const double *ag, const double *amf,
+                           double *aWH, double *aWN, double *aHB, double *aHMF, double *aSS)
+{
+    if (m.rmeth)
+        hipLaunchKernelGGL((k_flux_adj_mhd<D, 1>), agrid(fb), dim3(ABX, ABY), 0, s, g, m, fb, H, S, met, ag, amf,
+                           aWH, aWN, aHB, aHMF, aSS);
+    else
+        hipLaunchKernelGGL((k_flux_adj_mhd<D, 0>), agrid(fb), dim3(ABX, ABY), 0, s, g, m, fb, H, S, met, ag, amf,
+                           aWH, aWN, aHB, aHMF, aSS);
+}
+
+void launch_flux_adj_mhd(hipStream_t s, int d, const Geo &g, const MParm &m, const Box &fb, const double *H,
+                         const double *S, const double *met, const double *ag, const double *amf, double *aWH,
+                         double *aWN, double *aHB, double *aHMF, double *aSS)
+{
+    if (d == 0) flux_adj_mhd_d<0>(s, g, m, fb, H, S, met, ag, amf, aWH, aWN, aHB, aHMF, aSS);
+    else if (d == 1) flux_adj_mhd_d<1>(s, g, m, fb, H, S, met, ag, amf, aWH, aWN, aHB, aHMF, aSS);
+    else flux_adj_mhd_d<2>(s, g, m, fb, H, S, met, ag, amf, aWH, aWN, aHB, aHMF, aSS);
+}
+
+template <int ED, int RM, bool BG, bool BO>
+static void emf_adj_t(hipStream_t s, const Geo &g, const MParm &m, const Box &eb, const Clock *clk,
+                      const double *H, const double *S, const double *Vf, const double *met, const double *bg,
+                      const double *aE, double *aVf, double *aHG, double *aHMF, double *aSS)
+{
+    hipLaunchKernelGGL((k_emf_adj<ED, RM, BG, BO>), agrid(eb), dim3(ABX, ABY), 0, s, g, m, eb, clk, H, S, Vf, met,
+                       bg, aE, aVf, aHG, aHMF, aSS);
+}
+
+template <int ED>
+static void emf_adj_d(hipStream_t s, const Geo &g, const MParm &m, const Box &eb, const Clock *clk,
+                      const double *H, const double *S, const double *Vf, const double *met, const double *bg,
+                      const double *aE, double *aVf, double *aHG, double *aHMF, double *aSS)
+{
+    if (m.rmeth) emf_adj_t<ED, 1, false, false>(s, g, m, eb, clk, H, S, Vf, met, bg, aE, aVf, aHG, aHMF, aSS);
+    else emf_adj_t<ED, 0, false, false>(s, g, m, eb, clk, H, S, Vf, met, bg, aE, aVf, aHG, aHMF, aSS);
+}
+
+void launch_emf_adj(hipStream_t s, int ed, const Geo &g, const MParm &m, const Box &eb, const Clock *clk,
+                    const double *H, const double *S, const double *Vf, const double *met, const double *bg,
+                    const double *aE, double *aVf, double *aHG, double *aHMF, double *aSS)
+{
+    if (ed == 0) emf_adj_d<0>(s, g, m, eb, clk, H, S, Vf, met, bg, aE, aVf, aHG, aHMF, aSS);
+    else if (ed == 1) emf_adj_d<1>(s, g, m, eb, clk, H, S, Vf, met, bg, aE, aVf, aHG, aHMF, aSS);
+    else emf_adj_d<2>(s, g, m, eb, clk, H, S, Vf, met, bg, aE, aVf, aHG, aHMF, aSS);
+}
+
+void launch_vface_adj(hipStream_t s, const Geo &g, const double *met, const double *aVf, double *aWH)
+{
+    const Box cb = cells_of(g);
+    hipLaunchKernelGGL(k_vface_adj, agrid(cb), dim3(ABX, ABY), 0, s, g, cb, met, aVf, aWH);
+}
+
+void launch_predict_adj_mhd(hipStream_t s, const Geo &g, const MParm &m, const Clock *clk, const double *S,
+                            const double *O, const double *met, const double *aHG, const double *aWH,
+                            const double *aWN, double *aHB, double *aHMF, double *aS, double *aO)
+{
+    const Box cb = cells_of(g), fb = faces_of(g);
+    hipLaunchKernelGGL(k_predict_b_adj, agrid(cb), dim3(ABX, ABY), 0, s, g, cb, met, aHB, aHMF);
+    hipLaunchKernelGGL(k_predict_adj_mhd, agrid(cb), dim3(ABX, ABY), 0, s, g, m, cb, clk, S, O, met, aHG, aWH,
+                       aWN, aS, aO);
+    hipLaunchKernelGGL(k_predict_mf_adj, agrid(fb), dim3(ABX, ABY), 0, s, g, fb, clk, aHMF, aS, aO);
 }

@@ -1193,11 +1193,13 @@ extern "C" int gam_autotune(gam_ctx *c, int reps)
 }
 
 // ------------------------------------------------------------ adjoint
-// Discrete adjoint of the hydrodynamic step (include/gamadj.h).  The
-// forward run keeps State/oState checkpoints every K steps and the clock
-// of every step; the reverse sweep recomputes each checkpoint interval
-// forward (keeping its states on a tape) and then runs the adjoint
-// kernels of its steps from last to first.
+// Discrete adjoint of the step (include/gamadj.h).  The forward run keeps
+// State/oState checkpoints every K steps and the clock of every step; the
+// reverse sweep recomputes each checkpoint interval forward (keeping its
+// states on a tape) and then runs the adjoint kernels of its steps from
+// last to first.
+enum { ADJ_SET = 11 };   // planes of an adjoint state set: gas 5, face flux 3, cell B 3
+
 struct gam_adj {
     gam_ctx *c;
     int nsteps, K, nck;
@@ -1207,9 +1209,11 @@ struct gam_adj {
     double *fin;             // State, oState at the end
     Clock *clkh;             // clock before step n, n = 0..nsteps
     double *tape;            // K + 1 State buffers of one interval
-    double *lam;             // three sets of gas adjoints (5 planes each)
-    double *ag;              // adjoint of gFlx (15 planes)
-    double *awh, *awn;       // adjoints of H_WH and H_WN (5 planes each)
+    double *lam;             // three adjoint state sets (ADJ_SET planes each)
+    double *ag, *amf;        // adjoints of gFlx (15) and mFlx (9)
+    double *awh, *awn, *ahg; // adjoints of H_WH, H_WN, H_GAS (5 each)
+    double *ahmf, *ahb;      // adjoints of H_MF, H_B (3 each)
+    double *avf, *aur, *adm, *ae;   // Vf (9), Reynolds gas (5), dM (3), E (3)
     double *wgt;             // objective weights (1 plane)
     double *part;            // objective partial sums (1 plane)
     hipEvent_t ev[4];
@@ -1220,7 +1224,8 @@ extern "C" int gam_adj_destroy(gam_adj *a)
 {
     if (!a) return GAM_OK;
     if (a->c && a->c->stream) hipStreamSynchronize(a->c->stream);
-    double *bufs[] = {a->ck, a->fin, a->tape, a->lam, a->ag, a->awh, a->awn, a->wgt, a->part};
+    double *bufs[] = {a->ck, a->fin, a->tape, a->lam, a->ag, a->amf, a->awh, a->awn, a->ahg, a->ahmf,
+                      a->ahb, a->avf, a->aur, a->adm, a->ae, a->wgt, a->part};
     for (double *b : bufs) if (b) hipFree(b);
     if (a->clkh) hipFree(a->clkh);
     for (int q = 0; q < 4; q++) if (a->ev[q]) hipEventDestroy(a->ev[q]);
@@ -1234,10 +1239,13 @@ extern "C" int gam_adj_create(gam_adj **out, gam_ctx *c, int nsteps, int K)
     *out = nullptr;
     int rc = check_ready(c, "gam_adj_create");
     if (rc) return rc;
-    if (c->mp.doMHD || c->mp.doBoris)
-        return fail(GAM_E_ARG, "gam_adj_create: the adjoint covers hydrodynamic runs (doMHD = 0, doBoris = 0)");
-    for (int sd = 0; sd < 6; sd++)
+    if (c->mp.doBoris || c->mp.doBG)
+        return fail(GAM_E_ARG, "gam_adj_create: the adjoint covers runs without Boris correction and background field");
+    for (int sd = 0; sd < 6; sd++) {
         if (c->model.bc[sd] == GAM_BC_WIND) return fail(GAM_E_ARG, "gam_adj_create: the inflow BC has no adjoint yet");
+        if (c->mp.doMHD && c->model.bc[sd] != GAM_BC_PERIODIC)
+            return fail(GAM_E_ARG, "gam_adj_create: with MHD the adjoint covers periodic boundaries only");
+    }
     if (nsteps < 1 || K < 1 || K > nsteps) return fail(GAM_E_ARG, "gam_adj_create: need 1 <= ckpt_every <= nsteps");
     gam_adj *a = (gam_adj *)calloc(1, sizeof(gam_adj));
     if (!a) return fail(GAM_E_NOMEM, "gam_adj_create: host allocation failed");
@@ -1252,14 +1260,26 @@ extern "C" int gam_adj_create(gam_adj **out, gam_ctx *c, int nsteps, int K)
     AALLOC(a->fin, 2 * S_NPL * pl);
     AALLOC(a->clkh, (size_t)(nsteps + 1) * sizeof(Clock));
     AALLOC(a->tape, (size_t)(K + 1) * S_NPL * pl);
-    AALLOC(a->lam, 15 * pl);
+    AALLOC(a->lam, 3 * ADJ_SET * pl);
     AALLOC(a->ag, 15 * pl);
+    AALLOC(a->amf, 9 * pl);
     AALLOC(a->awh, 5 * pl);
     AALLOC(a->awn, 5 * pl);
+    AALLOC(a->ahg, 5 * pl);
+    AALLOC(a->ahmf, 3 * pl);
+    AALLOC(a->ahb, 3 * pl);
+    AALLOC(a->avf, 9 * pl);
+    AALLOC(a->aur, 5 * pl);
+    AALLOC(a->adm, 3 * pl);
+    AALLOC(a->ae, 3 * pl);
     AALLOC(a->wgt, pl);
     AALLOC(a->part, pl);
 #undef AALLOC
+    if (e == hipSuccess) e = hipMemset(a->lam, 0, 3 * ADJ_SET * pl);
     if (e == hipSuccess) e = hipMemset(a->ag, 0, 15 * pl);
+    if (e == hipSuccess) e = hipMemset(a->amf, 0, 9 * pl);
+    if (e == hipSuccess) e = hipMemset(a->aur, 0, 5 * pl);
+    if (e == hipSuccess) e = hipMemset(a->adm, 0, 3 * pl);
     for (int q = 0; e == hipSuccess && q < 4; q++) e = hipEventCreate(&a->ev[q]);
     if (e != hipSuccess) {
         rc = fail(GAM_E_HIP, "gam_adj_create: %s", hipGetErrorString(e));
@@ -1352,7 +1372,78 @@ static int adj_restore_final(gam_adj *a)
     return GAM_OK;
 }
 
-extern "C" int gam_adj_backward(gam_adj *a, double *gGas, double *gGasO)
+// Transposes of the boundary pass on one adjoint set, sides OUTK to INI.
+static void adj_bcs(gam_adj *a, double *l)
+{
+    gam_ctx *c = a->c;
+    for (int sd = 5; sd >= 0; sd--) {
+        if (c->mp.doMHD) launch_bc_periodic_adj(c->stream, sd, c->g, l);
+        else launch_bc_adj(c->stream, sd, c->model.bc[sd] == GAM_BC_PERIODIC ? 0 : 1, c->g, l);
+    }
+}
+
+// One step's adjoint: lA (wrt the step's output State) and lB (wrt its
+// output oState = input State) in, lB (input State) and lC (input oState)
+// out.  Sn, On = the step's input State and oState; clk its clock.
+static void adj_step(gam_adj *a, const double *Sn, const double *On, const Clock *clk, double *lA,
+                     double *lB, double *lC)
+{
+    gam_ctx *c = a->c;
+    const Geo &g = c->g;
+    const long N3 = g.N3;
+    hipStream_t s = c->stream;
+    const size_t pl = (size_t)N3 * sizeof(double);
+    Box eb[3];
+    edge_boxes(g, eb);
+    // the forward intermediates the transposes read
+    launch_predict(s, c->plan.v[TS_PREDICT], g, c->mp, face_box(g), clk, Sn, On, c->met, c->H);
+    if (c->mp.doMHD) {
+        launch_vface(s, g, c->mp, c->H, c->met, c->vf);
+        for (int d = 0; d < 3; d++) {
+            Box fb = act_box(g);
+            if (d == 0) fb.i1 += 1; else if (d == 1) fb.j1 += 1; else fb.k1 += 1;
+            launch_flux(s, d, c->plan.v[TS_FLUX_I + d], g, c->mp, fb, c->H, Sn, c->met, c->bg, c->gflx, c->mflx);
+        }
+        // ghost bFlux2Fld, boundary pass, cell B of the update, update
+        launch_cellb_adj(s, g, 0, lA, c->met);
+        adj_bcs(a, lA);
+        launch_cellb_adj(s, g, 1, lA, c->met);
+        const Box ctb = mkbox(g.is, g.ie + 1, g.js, g.je + 1, g.ks, g.ke + 1);
+        launch_update_adj_mhd(s, g, c->mp, act_box(g), ctb, eb, clk, Sn, c->gflx, c->mflx, c->met, c->bg, lA,
+                              lB, a->aur, a->adm, a->ag, a->amf, a->ae);
+        hipMemsetAsync(a->awh, 0, 5 * pl, s);
+        hipMemsetAsync(a->awn, 0, 5 * pl, s);
+        hipMemsetAsync(a->ahg, 0, 5 * pl, s);
+        hipMemsetAsync(a->ahmf, 0, 3 * pl, s);
+        hipMemsetAsync(a->ahb, 0, 3 * pl, s);
+        hipMemsetAsync(a->avf, 0, 9 * pl, s);
+        for (int d = 0; d < 3; d++) {
+            Box fb = act_box(g);
+            if (d == 0) fb.i1 += 1; else if (d == 1) fb.j1 += 1; else fb.k1 += 1;
+            launch_flux_adj_mhd(s, d, g, c->mp, fb, c->H, Sn, c->met, a->ag, a->amf, a->awh, a->awn, a->ahb,
+                                a->ahmf, lB);
+        }
+        for (int ed = 0; ed < 3; ed++)
+            launch_emf_adj(s, ed, g, c->mp, eb[ed], clk, c->H, Sn, c->vf, c->met, c->bg, a->ae, a->avf, a->ahg,
+                           a->ahmf, lB);
+        launch_vface_adj(s, g, c->met, a->avf, a->awh);
+        hipMemsetAsync(lC + S_B * N3, 0, 3 * pl, s);
+        launch_predict_adj_mhd(s, g, c->mp, clk, Sn, On, c->met, a->ahg, a->awh, a->awn, a->ahb, a->ahmf, lB, lC);
+    } else {
+        adj_bcs(a, lA);
+        launch_update_adj(s, g, act_box(g), clk, lA, lB, a->ag, c->met);
+        hipMemsetAsync(a->awh, 0, 5 * pl, s);
+        hipMemsetAsync(a->awn, 0, 5 * pl, s);
+        for (int d = 0; d < 3; d++) {
+            Box fb = act_box(g);
+            if (d == 0) fb.i1 += 1; else if (d == 1) fb.j1 += 1; else fb.k1 += 1;
+            launch_flux_adj(s, d, g, c->mp, fb, c->H, c->met, a->ag, a->awh, a->awn);
+        }
+        launch_predict_adj(s, g, c->mp, clk, Sn, On, c->met, a->awh, a->awn, lB, lC);
+    }
+}
+
+extern "C" int gam_adj_backward(gam_adj *a, double *gGas, double *gGasO, double *gMF, double *gMFO)
 {
     if (!a || !gGas) return fail(GAM_E_ARG, "gam_adj_backward: NULL argument");
     if (!a->recorded) return fail(GAM_E_STATE, "gam_adj_backward: no recorded forward run");
@@ -1361,12 +1452,12 @@ extern "C" int gam_adj_backward(gam_adj *a, double *gGas, double *gGasO)
     const long N3 = g.N3;
     const size_t bytes = (size_t)S_NPL * N3 * sizeof(double);
     hipStream_t s = c->stream;
-    double *lA = a->lam, *lB = a->lam + 5 * N3, *lC = a->lam + 10 * N3;
+    double *lA = a->lam, *lB = a->lam + ADJ_SET * N3, *lC = a->lam + 2 * ADJ_SET * N3;
     HIPCHK(hipStreamSynchronize(s));
     HIPCHK(hipEventRecord(a->ev[2], s));
+    HIPCHK(hipMemsetAsync(a->lam, 0, 3 * ADJ_SET * N3 * sizeof(double), s));
     int rc = adj_objective(a, a->fin, lA, nullptr);
     if (rc) return rc;
-    HIPCHK(hipMemsetAsync(lB, 0, 5 * N3 * sizeof(double), s));
     const int prof = c->prof;
     c->prof = 0;
     for (int seg = a->nck - 1; seg >= 0; seg--) {
@@ -1382,21 +1473,8 @@ extern "C" int gam_adj_backward(gam_adj *a, double *gGas, double *gGasO)
         }
         for (int n = s1 - 1; n >= s0; n--) {
             const int q = n - s0;
-            const double *Sn = a->tape + (size_t)(1 + q) * S_NPL * N3;
-            const double *On = a->tape + (size_t)q * S_NPL * N3;
-            const Clock *clk = a->clkh + n;
-            for (int sd = 5; sd >= 0; sd--)
-                launch_bc_adj(s, sd, c->model.bc[sd] == GAM_BC_PERIODIC ? 0 : 1, g, lA);
-            launch_update_adj(s, g, act_box(g), clk, lA, lB, a->ag, c->met);
-            launch_predict(s, c->plan.v[TS_PREDICT], g, c->mp, face_box(g), clk, Sn, On, c->met, c->H);
-            HIPCHK(hipMemsetAsync(a->awh, 0, 5 * N3 * sizeof(double), s));
-            HIPCHK(hipMemsetAsync(a->awn, 0, 5 * N3 * sizeof(double), s));
-            for (int d = 0; d < 3; d++) {
-                Box fb = act_box(g);
-                if (d == 0) fb.i1 += 1; else if (d == 1) fb.j1 += 1; else fb.k1 += 1;
-                launch_flux_adj(s, d, g, c->mp, fb, c->H, c->met, a->ag, a->awh, a->awn);
-            }
-            launch_predict_adj(s, g, c->mp, clk, Sn, On, c->met, a->awh, a->awn, lB, lC);
+            adj_step(a, a->tape + (size_t)(1 + q) * S_NPL * N3, a->tape + (size_t)q * S_NPL * N3, a->clkh + n,
+                     lA, lB, lC);
             double *t = lA;
             lA = lB;
             lB = lC;
@@ -1406,30 +1484,43 @@ extern "C" int gam_adj_backward(gam_adj *a, double *gGas, double *gGasO)
     }
     c->prof = prof;
     // the start states' ghost cells follow from their interiors through the
-    // boundary conditions: fold their adjoints into the interior
-    for (int sd = 5; sd >= 0; sd--) {
-        launch_bc_adj(s, sd, c->model.bc[sd] == GAM_BC_PERIODIC ? 0 : 1, g, lA);
-        launch_bc_adj(s, sd, c->model.bc[sd] == GAM_BC_PERIODIC ? 0 : 1, g, lB);
+    // boundary conditions, and their cell B from the face fluxes: fold those
+    // adjoints into the interior gas and face fluxes
+    if (c->mp.doMHD) {
+        launch_cellb_adj(s, g, 1, lA, c->met);
+        launch_cellb_adj(s, g, 1, lB, c->met);
     }
+    adj_bcs(a, lA);
+    adj_bcs(a, lB);
     HIPCHK(hipEventRecord(a->ev[3], s));
     HIPCHK(hipEventSynchronize(a->ev[3]));
     HIPCHK(hipEventElapsedTime(&a->bwd_ms, a->ev[2], a->ev[3]));
     if ((rc = download(c, lA, 0, 5, gGas))) return rc;
     if (gGasO && (rc = download(c, lB, 0, 5, gGasO))) return rc;
+    if (gMF && (rc = download(c, lA + S_MF * N3, 1, 3, gMF))) return rc;
+    if (gMFO && (rc = download(c, lB + S_MF * N3, 1, 3, gMFO))) return rc;
     return adj_restore_final(a);
 }
 
-extern "C" int gam_adj_replay(gam_adj *a, const double *Gas, const double *oGas, double *J)
+extern "C" int gam_adj_replay(gam_adj *a, const double *Gas, const double *oGas, const double *MF,
+                              const double *oMF, double *J)
 {
     if (!a || !Gas) return fail(GAM_E_ARG, "gam_adj_replay: NULL argument");
     if (!a->recorded) return fail(GAM_E_STATE, "gam_adj_replay: no recorded forward run");
     gam_ctx *c = a->c;
+    const long N3 = c->g.N3;
     int rc = adj_put(a, 0, a->ck);
     if (rc) return rc;
-    if ((rc = upload(c, Gas, 0, 5, S_cur(c) + S_GAS * c->g.N3))) return rc;
-    if (oGas && (rc = upload(c, oGas, 0, 5, S_old(c) + S_GAS * c->g.N3))) return rc;
+    if ((rc = upload(c, Gas, 0, 5, S_cur(c) + S_GAS * N3))) return rc;
+    if (oGas && (rc = upload(c, oGas, 0, 5, S_old(c) + S_GAS * N3))) return rc;
+    if (MF && (rc = upload(c, MF, 1, 3, S_cur(c) + S_MF * N3))) return rc;
+    if (oMF && (rc = upload(c, oMF, 1, 3, S_old(c) + S_MF * N3))) return rc;
     enq_bcs(c, S_cur(c));    // ghosts from the given interiors
     enq_bcs(c, S_old(c));
+    if (c->mp.doMHD) {       // cell B from the face fluxes
+        enq_bflux2fld(c, S_cur(c));
+        enq_bflux2fld(c, S_old(c));
+    }
     const int prof = c->prof;
     c->prof = 0;
     for (int n = 0; n < a->nsteps; n++) {
@@ -1438,7 +1529,7 @@ extern "C" int gam_adj_replay(gam_adj *a, const double *Gas, const double *oGas,
     }
     c->prof = prof;
     HIPCHK(hipGetLastError());
-    if ((rc = adj_objective(a, S_cur(c), a->lam + 10 * c->g.N3, J))) return rc;
+    if ((rc = adj_objective(a, S_cur(c), a->lam + 2 * ADJ_SET * N3, J))) return rc;
     return adj_restore_final(a);
 }
 

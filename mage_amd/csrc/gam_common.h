@@ -27,6 +27,8 @@ struct Geo {
 struct Box {
     int i0, i1, j0, j1, k0, k1;
 };
+// the active EMF edges of each edge direction
+struct EdgeBoxes { Box e[3]; };
 
 struct MParm {
     double gamma, CFL, Vd0, Ca, cHogH, cHogM, pdmb, dFloor, pFloor;

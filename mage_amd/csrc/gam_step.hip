@@ -801,8 +801,6 @@ struct k_flux_s {
 // keep their old flux; edges outside the active EMF boxes read as zero,
 // as CalcElecField zeroes E before filling active edges (fields.F90:89).
 // ====================================================================
-struct EdgeBoxes { Box e[3]; };
-
 __device__ __forceinline__ double e_at(const Geo &g, const EdgeBoxes &eb, const double *E, int d,
                                        int i, int j, int k, unsigned off)
 {

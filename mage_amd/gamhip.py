@@ -413,8 +413,8 @@ def _load_adj(L):
     L.gam_adj_destroy.argtypes = [vp]
     L.gam_adj_set_weights.argtypes = [vp, DP]
     L.gam_adj_forward.argtypes = [vp, DP]
-    L.gam_adj_backward.argtypes = [vp, DP, DP]
-    L.gam_adj_replay.argtypes = [vp, DP, DP, DP]
+    L.gam_adj_backward.argtypes = [vp, DP, DP, DP, DP]
+    L.gam_adj_replay.argtypes = [vp, DP, DP, DP, DP, DP]
     L.gam_adj_times.argtypes = [vp, DP, DP]
     return L
 
@@ -446,18 +446,25 @@ class GamAdjoint:
         self._chk(self.L.gam_adj_forward(self.adj, C.byref(J)))
         return J.value
 
-    def backward(self):
+    def backward(self, fluxes=False):
+        """Gradients with respect to the starting State and oState gas, and
+        with fluxes=True also their magFlux: (gS, gO) or (gS, gO, gMF, gMFO)."""
         shp = self.g.cshape[:3] + (5,)
         gS = np.zeros(shp, order="F")
         gO = np.zeros(shp, order="F")
-        self._chk(self.L.gam_adj_backward(self.adj, _p(gS), _p(gO)))
-        return gS, gO
+        if not fluxes:
+            self._chk(self.L.gam_adj_backward(self.adj, _p(gS), _p(gO), None, None))
+            return gS, gO
+        fshp = self.g.fshape + (3,)
+        gM = np.zeros(fshp, order="F")
+        gMO = np.zeros(fshp, order="F")
+        self._chk(self.L.gam_adj_backward(self.adj, _p(gS), _p(gO), _p(gM), _p(gMO)))
+        return gS, gO, gM, gMO
 
-    def replay(self, Gas, oGas=None):
+    def replay(self, Gas, oGas=None, MF=None, oMF=None):
         J = C.c_double()
-        Gas = np.asfortranarray(Gas, dtype=np.float64)
-        oGas = None if oGas is None else np.asfortranarray(oGas, dtype=np.float64)
-        self._chk(self.L.gam_adj_replay(self.adj, _p(Gas), _p(oGas), C.byref(J)))
+        arr = [None if x is None else np.asfortranarray(x, dtype=np.float64) for x in (Gas, oGas, MF, oMF)]
+        self._chk(self.L.gam_adj_replay(self.adj, *[_p(x) for x in arr], C.byref(J)))
         return J.value
 
     def times(self):

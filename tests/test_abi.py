@@ -112,6 +112,6 @@ def test_adjoint_header_symbols_exported():
     assert L.gam_adj_create(C.byref(out), None, 4, 2) == -1
     J = C.c_double()
     assert L.gam_adj_forward(None, C.byref(J)) == -1
-    assert L.gam_adj_backward(None, None, None) == -1
-    assert L.gam_adj_replay(None, None, None, C.byref(J)) == -1
+    assert L.gam_adj_backward(None, None, None, None, None) == -1
+    assert L.gam_adj_replay(None, None, None, None, None, C.byref(J)) == -1
     assert L.gam_adj_destroy(None) == 0

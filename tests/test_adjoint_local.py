@@ -51,6 +51,86 @@ void a_rec(int rm, double b, double rVi, const double *x, const double *a, doubl
     if (rm) recon_lr_w_adj<1>(b, rVi, x + 8, x, a[0], a[1], ax + 8, ax);
     else recon_lr_w_adj<0>(b, rVi, x + 8, x, a[0], a[1], ax + 8, ax);
 }
+void f_gas2(double g, double s, const double *W, const double *T, double *y)
+{ for (int v = 0; v < 10; v++) y[v] = 0; gas_kin(1.0 / (g - 1.0), s, W, T, y, y + 5); }
+void a_gas2(double g, double s, const double *W, const double *T, const double *a, double *ax)
+{ for (int v = 0; v < 5; v++) ax[v] = 0; gas_kin_adj(1.0 / (g - 1.0), s, W, T, a, ax, a + 5); }
+void f_mag(double s, const double *x, const double *T, double *y)
+{   // x = W[5], B[3], Bn; y = mF[3], VaD
+    MParm m = mp(5.0 / 3.0, 1e-12, 1e-12, 1);
+    double B0[3] = {0, 0, 0}, bbD = 0.0, VaD = 0.0;
+    for (int v = 0; v < 3; v++) y[v] = 0;
+    mag_kin<false, false>(m, s, x, x + 5, x[8], T, B0, 0.0, y, bbD, VaD);
+    y[3] = VaD;
+}
+void a_mag(double s, const double *x, const double *T, const double *a, double *ax)
+{
+    for (int v = 0; v < 9; v++) ax[v] = 0;
+    double aBn = 0.0;
+    mag_kin_adj(s, x, x + 5, x[8], T, a, a[3], ax, ax + 5, aBn);
+    ax[8] = aBn;
+}
+static Geo cb_geo() { Geo g = {}; g.P = 4; g.PP = 16; g.N3 = 64; return g; }
+void f_cellb(const double *met, const double *f, double *y)
+{
+    Geo g = cb_geo();
+    cell_b(g, met, 21, f[0], f[1], f[2], f[3], f[4], f[5], y);
+}
+void a_cellb(const double *met, const double *f, const double *a, double *ax)
+{
+    Geo g = cb_geo();
+    for (int v = 0; v < 6; v++) ax[v] = 0;
+    cell_b_adj(g, met, 21, a, ax);
+}
+void f_maxw(double g, double dt, const double *x, double *y)
+{   // x = U[5], dM[3]: the Maxwell branch of k_update
+    MParm m = mp(g, 1e-12, 1e-12, 1);
+    const double *U = x, *dM = x + 5;
+    const double d = dmax(U[DEN], m.dFloor);
+    double mx = U[MOMX], my = U[MOMY], mz = U[MOMZ];
+    const double id = 1.0 / d;
+    double ke = 0.5 * (mx * mx + my * my + mz * mz) * id;
+    const double p = dmax((m.gamma - 1) * (U[ENER] - ke), m.pFloor);
+    mx = mx + dt * dM[0]; my = my + dt * dM[1]; mz = mz + dt * dM[2];
+    ke = 0.5 * (mx * mx + my * my + mz * mz) * id;
+    y[0] = d; y[1] = mx; y[2] = my; y[3] = mz; y[4] = ke + p * m.igm1;
+}
+void a_maxw(double g, double dt, const double *x, const double *a, double *ax)
+{
+    MParm m = mp(g, 1e-12, 1e-12, 1);
+    for (int v = 0; v < 8; v++) ax[v] = 0;
+    maxwell_adj(m, x, x + 5, dt, a, ax, ax + 5);
+}
+// edge EMF: x = V1[24], mH1[8], mL1[4], mH2[8], mL2[4], D4[4] (52); geometry fixed by geo[]
+static void edge_fill(EdgeIn &in, const double *geo, const double *x)
+{
+    for (int n = 0; n < 8; n++) { in.A1[n] = geo[n]; in.A2[n] = geo[8 + n]; }
+    for (int d = 0; d < 3; d++) { in.te1[d] = geo[16 + d]; in.te2[d] = geo[19 + d]; }
+    for (int q = 0; q < 4; q++) in.tb[q] = geo[22 + q];
+    in.b01 = geo[26]; in.b02 = geo[27]; in.edge = geo[28]; in.cap = geo[29];
+    for (int d = 0; d < 3; d++) for (int n = 0; n < 8; n++) in.V1[d][n] = x[8 * d + n];
+    for (int n = 0; n < 8; n++) { in.mH1[n] = x[24 + n]; in.mH2[n] = x[36 + n]; }
+    for (int n = 0; n < 4; n++) { in.mL1[n] = x[32 + n]; in.mL2[n] = x[44 + n]; in.D4[n] = x[48 + n]; }
+}
+void f_emf(int rm, int boris, const double *geo, const double *x, double *y)
+{
+    MParm m = mp(5.0 / 3.0, 1e-12, 1e-12, 1); m.pdmb = 0.75; m.Vd0 = 0.5; m.Ca = 2.0;
+    EdgeIn in; edge_fill(in, geo, x);
+    double l1, r1, l2, r2;
+    if (rm) y[0] = boris ? edge_emf<1, true>(m, in, &l1, &r1, &l2, &r2) : edge_emf<1, false>(m, in, &l1, &r1, &l2, &r2);
+    else y[0] = boris ? edge_emf<0, true>(m, in, &l1, &r1, &l2, &r2) : edge_emf<0, false>(m, in, &l1, &r1, &l2, &r2);
+}
+void a_emf(int rm, int boris, const double *geo, const double *x, const double *a, double *ax)
+{
+    MParm m = mp(5.0 / 3.0, 1e-12, 1e-12, 1); m.pdmb = 0.75; m.Vd0 = 0.5; m.Ca = 2.0;
+    EdgeIn in; edge_fill(in, geo, x);
+    EdgeAdj o = {};
+    if (rm) { if (boris) edge_emf_adj<1, true>(m, in, a[0], o); else edge_emf_adj<1, false>(m, in, a[0], o); }
+    else { if (boris) edge_emf_adj<0, true>(m, in, a[0], o); else edge_emf_adj<0, false>(m, in, a[0], o); }
+    for (int d = 0; d < 3; d++) for (int n = 0; n < 8; n++) ax[8 * d + n] = o.V1[d][n];
+    for (int n = 0; n < 8; n++) { ax[24 + n] = o.mH1[n]; ax[36 + n] = o.mH2[n]; }
+    for (int n = 0; n < 4; n++) { ax[32 + n] = o.mL1[n]; ax[44 + n] = o.mL2[n]; ax[48 + n] = o.D4[n]; }
+}
 void f_pred(double g, int lfm, double w, double vol, const double *x, double *y)
 {   // x = Q[5], oQ[5]; y = H_GAS 5, H_WH 5, H_WN 5 (as k_predict writes them)
     MParm m = mp(g, 1e-12, 1e-12, lfm);
@@ -191,6 +271,76 @@ def test_predictor_cell_adjoint(lib, lfm):
     fwd = lambda x, y: lib.f_pred(C.c_double(g), lfm, C.c_double(0.5), C.c_double(1.3), _p(x), _p(y))
     adj = lambda x, a, ax: lib.a_pred(C.c_double(g), lfm, C.c_double(0.5), C.c_double(1.3), _p(x), _p(a), _p(ax))
     _check(fwd, adj, x0, 15, rng, 1e-6, True)
+
+
+@pytest.mark.parametrize("s", [-1.0, 1.0])
+def test_gas_flux_with_hogs_difference_adjoint(lib, s):
+    rng = np.random.default_rng(21)
+    T = _triad(rng)
+    g = 5.0 / 3.0
+
+    def x0(r):
+        return np.array([r.uniform(0.2, 2.0), r.standard_normal(), r.standard_normal(),
+                         r.standard_normal(), r.uniform(0.05, 2.0)])
+    fwd = lambda x, y: lib.f_gas2(C.c_double(g), C.c_double(s), _p(x), _p(T), _p(y))
+    adj = lambda x, a, ax: lib.a_gas2(C.c_double(g), C.c_double(s), _p(x), _p(T), _p(a), _p(ax))
+    _check(fwd, adj, x0, 10, rng, 1e-6, True)
+
+
+@pytest.mark.parametrize("s", [-1.0, 1.0])
+def test_magnetic_flux_adjoint(lib, s):
+    rng = np.random.default_rng(23)
+    T = _triad(rng)
+
+    def x0(r):
+        return np.concatenate([[r.uniform(0.3, 2.0)], r.standard_normal(3) * 0.5, [r.uniform(0.1, 2.0)],
+                               r.standard_normal(3), [r.standard_normal()]])[[0, 1, 2, 3, 4, 5, 6, 7, 8]]
+    fwd = lambda x, y: lib.f_mag(C.c_double(s), _p(x), _p(T), _p(y))
+    adj = lambda x, a, ax: lib.a_mag(C.c_double(s), _p(x), _p(T), _p(a), _p(ax))
+    _check(fwd, adj, x0, 4, rng, 1e-6, True)
+
+
+def test_cell_b_adjoint(lib):
+    rng = np.random.default_rng(25)
+    met = rng.uniform(0.5, 1.5, 91 * 64)
+    fwd = lambda x, y: lib.f_cellb(_p(met), _p(x), _p(y))
+    adj = lambda x, a, ax: lib.a_cellb(_p(met), _p(x), _p(a), _p(ax))
+    _check(fwd, adj, lambda r: r.standard_normal(6), 3, rng, 1e-6, True)
+
+
+def test_maxwell_update_adjoint(lib):
+    rng = np.random.default_rng(27)
+    g = 5.0 / 3.0
+
+    def x0(r):
+        d = r.uniform(0.3, 2.0)
+        v = r.standard_normal(3) * 0.5
+        U = [d, d * v[0], d * v[1], d * v[2], 0.5 * d * v @ v + r.uniform(0.2, 2.0) * 1.5]
+        return np.concatenate([U, r.standard_normal(3)])
+    fwd = lambda x, y: lib.f_maxw(C.c_double(g), C.c_double(0.07), _p(x), _p(y))
+    adj = lambda x, a, ax: lib.a_maxw(C.c_double(g), C.c_double(0.07), _p(x), _p(a), _p(ax))
+    _check(fwd, adj, x0, 5, rng, 1e-6, True)
+
+
+@pytest.mark.parametrize("rm,boris,cap", [(1, 0, 1e9), (0, 1, 1e9), (1, 1, 0.05)])
+def test_edge_emf_adjoint(lib, rm, boris, cap):
+    """Edge EMF (corner velocity, limited edge B, vDiff with Boris vA and
+    the CFL cap) -- piecewise smooth through the limiter, like the
+    reconstruction test."""
+    rng = np.random.default_rng(29 + rm + 2 * boris)
+    t1 = _triad(rng)
+    geo = np.concatenate([rng.uniform(0.8, 1.2, 16), t1[3:6], t1[6:9], [1.0, 0.1, 0.05, 0.9],
+                          [0.05, -0.02, 0.3, cap]])
+
+    def x0(r):
+        V = r.standard_normal(24) * 0.3
+        b1 = np.cumsum(r.standard_normal(8)) * 0.2 + 1.0
+        b2 = np.cumsum(r.standard_normal(8)) * 0.2 - 0.5
+        return np.concatenate([V, b1, b1[2:6] + 0.02 * r.standard_normal(4), b2,
+                               b2[2:6] + 0.02 * r.standard_normal(4), r.uniform(0.5, 1.5, 4)])
+    fwd = lambda x, y: lib.f_emf(rm, boris, _p(geo), _p(x), _p(y))
+    adj = lambda x, a, ax: lib.a_emf(rm, boris, _p(geo), _p(x), _p(a), _p(ax))
+    _check(fwd, adj, x0, 1, rng, 1e-9, False, nsamp=2000)
 
 
 def test_adjoint_catches_a_planted_sign_error(lib):

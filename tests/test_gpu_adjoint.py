@@ -147,11 +147,67 @@ def test_adjoint_shell_zero_gradient_matches_finite_differences():
     g.close()
 
 
-def test_adjoint_refuses_mhd_contexts():
-    g = box_case((16, 16, 16), "7UP")[0]
+def test_adjoint_refuses_unsupported_contexts():
+    """Boris correction, background field and the inflow BC have no adjoint
+    yet; MHD with non-periodic walls neither."""
+    N = (16, 16, 16)
+    g = gamhip.GamHip(*N, pb.shell_params("7UP"))
+    g.grid_from_corners(pb.shell_corners(*N))
+    g.add_dipole_b0(pb.SHELL_M0)
+    g.set_wind(pb.constant_wind())
     with pytest.raises(gamhip.GamError) as ei:
         gamhip.GamAdjoint(g, 4, 2)
     assert ei.value.code == -1
+    g.close()
+    p = pb.shell_params("7UP")
+    p.update(doBoris=0, doBackground=0, bc=["zerograd"] * 4 + ["periodic"] * 2)
+    g = gamhip.GamHip(*N, p)
+    g.grid_from_corners(pb.shell_corners(*N))
+    with pytest.raises(gamhip.GamError) as ei:
+        gamhip.GamAdjoint(g, 4, 2)
+    assert ei.value.code == -1
+    g.close()
+
+
+def _fd_check_flux(g, nsteps, ckpt, ndir, seed, tag):
+    """Face-flux directions of an MHD run (State magFlux, then oState)."""
+    S = g.get_state(gamhip.GAM_STATE)
+    O = g.get_state(gamhip.GAM_OSTATE)
+    s0, o0 = S["Gas"].copy(order="F"), O["Gas"].copy(order="F")
+    m0, mo0 = S["magFlux"].copy(order="F"), O["magFlux"].copy(order="F")
+    adj = gamhip.GamAdjoint(g, nsteps, ckpt)
+    adj.set_weights(_weights(g))
+    adj.forward()
+    gS, gO, gM, gMO = adj.backward(fluxes=True)
+    scale = np.max(np.abs(m0))
+    for q, d in enumerate(_directions(m0.shape[:3] + (5,), ndir, seed)):
+        d = np.asfortranarray(d[..., :3]) * scale * 1e-2
+        which = q % 2
+        ad = np.sum((gM if which == 0 else gMO) * d)
+        best = None
+        for e in (1e-2, 1e-3, 1e-4, 1e-5):
+            if which == 0:
+                fd = (adj.replay(s0, o0, m0 + e * d, mo0) - adj.replay(s0, o0, m0 - e * d, mo0)) / (2 * e)
+            else:
+                fd = (adj.replay(s0, o0, m0, mo0 + e * d) - adj.replay(s0, o0, m0, mo0 - e * d)) / (2 * e)
+            rel = abs(fd - ad) / max(abs(fd), abs(ad))
+            print("%s flux dir %d (%s) e %.0e: fd %.12e adj %.12e rel %.2e" % (
+                tag, q, "State" if which == 0 else "oState", e, fd, ad, rel), flush=True)
+            best = rel if best is None else min(best, rel)
+        assert best <= (1e-4 if which == 0 else 1e-3), (tag, q, best)
+    adj.close()
+
+
+@pytest.mark.parametrize("rmeth", ["8CENT", "7UP"])
+def test_adjoint_mhd_box_matches_finite_differences(rmeth):
+    """MHD on the periodic box: EMF, constrained transport, magnetic
+    kinetic flux with HOGS, Maxwell update, all in the adjoint."""
+    g = box_case((16, 16, 16), rmeth)[0]
+    g.init_run()
+    _generic(g, 11)
+    g.run(2)
+    assert _fd_check(g, 6, 4, 6, 303, "mhd box " + rmeth) <= TOL
+    _fd_check_flux(g, 6, 4, 4, 404, "mhd box " + rmeth)
     g.close()
 
 
