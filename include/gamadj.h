@@ -9,12 +9,13 @@
  * the forward run, GasKinFlux; stress.F90:204-511, recon.F90:125-301) and
  * the AB2 predictor (mhdgroup.F90:444-561).
  *
- * Scope: runs without Boris correction and background field
- * (doBoris = 0, doBackground = 0) and without the inflow boundary;
- * hydrodynamic runs (doMHD = 0) with periodic, zero-gradient or sphere
- * boundaries, MHD runs with periodic boundaries (GAM_E_ARG otherwise).
+ * Scope: every step option except the inflow boundary: hydrodynamic runs
+ * (doMHD = 0, no Boris, no background) with periodic, zero-gradient or
+ * sphere boundaries; MHD runs (Maxwell or Boris update, with or without
+ * the background field) with periodic boundaries (GAM_E_ARG otherwise).
  * With MHD the adjoint runs through the edge EMFs, constrained transport,
- * the magnetic kinetic fluxes with HOGS and the Maxwell update.
+ * the magnetic kinetic fluxes with HOGS (and the Boris mag-HOGS and nuke
+ * terms), the background cross terms and CellMaxwell / CellBoris.
  *
  * Objective: J = sum over cells of w(c) * P(c) of the State after nsteps
  * (P the CellC2P pressure), w a weight array (Ni,Nj,Nk) given by the

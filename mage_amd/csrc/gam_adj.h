@@ -248,55 +248,78 @@ __device__ __forceinline__ void gas_kin_adj(double igm1, double s, const double 
 }
 
 // ---------------------------------------------------------- magnetic flux
-// One side of MagKinFlux (mag_kin) without background field and Boris
-// factor: aF = adjoint of the flux contribution mF (3), aVaD that of the
-// side's VaD term.  Adds into aW (rho, P, v), aB (3) and aBn.
-__device__ __forceinline__ void mag_kin_adj(double s, const double *W, const double *B, double Bn,
-                                            const double *T, const double *aF, double aVaD,
-                                            double *aW, double *aB, double &aBn)
+// One side of MagKinFlux (mag_kin<BG, BORIS>): aF = adjoint of the flux
+// contribution mF (3), aVaD and abbD those of the side's VaD and bbD
+// terms.  Adds into aW (rho, P, v), aB (3) and aBn; B0, B0n are constants.
+template <bool BG, bool BORIS>
+__device__ __forceinline__ void mag_kin_adj(const MParm &m, double s, const double *W, const double *B,
+                                            double Bn, const double *T, const double *B0, double B0n,
+                                            const double *aF, double aVaD, double abbD, double *aW,
+                                            double *aB, double &aBn)
 {
     const double d = W[DEN], p = W[PRS];
     const double id = 1.0 / d;
-    const double bx = B[0], by = B[1], bz = B[2];
-    const double dPb = 0.5 * (bx * bx + by * by + bz * bz);
-    const double vac2 = 2 * dPb * id;
-    const double nx = T[0], ny = T[1], nz = T[2];
-    const double vn = nx * W[1] + ny * W[2] + nz * W[3];
+    const double b[3] = {B[0], B[1], B[2]};
+    const double dPb = 0.5 * (b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+    double t[3] = {b[0], b[1], b[2]};
+    if (BG)
+        for (int x = 0; x < 3; x++) t[x] += B0[x];
+    const double tt = t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
+    const double va2 = BG ? tt * id : 2 * dPb * id;
+    const double den = 1 + va2 * m.iCa2;
+    const double vac2 = BORIS ? va2 / den : va2;
+    const double n[3] = {T[0], T[1], T[2]};
+    const double vn = n[0] * W[1] + n[1] * W[2] + n[2] * W[3];
     const double lam = 1 / (2 * p * id + vac2);
     const double sl = sqrt(lam);
     const double z = s * sl * vn;
     double ez;
     const double vn0 = 0.5 * gam_erfc_exp(z, ez);
-    const double b[3] = {bx, by, bz}, n[3] = {nx, ny, nz};
-    double avn0 = 0.0, adPb = 0.0, ndot = 0.0, bdot = 0.0;
+    const double bdb0 = BG ? B0[0] * b[0] + B0[1] * b[1] + B0[2] * b[2] : 0.0;
+    double avn0 = 0.0, ndot = 0.0, bdot = 0.0;
     for (int x = 0; x < 3; x++) {
         avn0 += aF[x] * (dPb * n[x] - b[x] * Bn);
+        if (BG) avn0 += aF[x] * (-(B0[x] * Bn) - b[x] * B0n + bdb0 * n[x]);
         ndot += aF[x] * n[x];
         bdot += aF[x] * b[x];
     }
-    adPb += vn0 * ndot;
+    double adPb = vn0 * ndot;
     aBn += -vn0 * bdot;
     double ab[3];
     for (int x = 0; x < 3; x++) ab[x] = -vn0 * Bn * aF[x];
+    if (BG) {
+        double b0dot = 0.0;
+        for (int x = 0; x < 3; x++) b0dot += B0[x] * aF[x];
+        aBn += -vn0 * b0dot;
+        for (int x = 0; x < 3; x++) ab[x] += -vn0 * B0n * aF[x] + vn0 * B0[x] * ndot;
+    }
+    // bbD += tt / 2 (with the background) or |b|^2 / 2
+    double at[3] = {0.0, 0.0, 0.0};
+    for (int x = 0; x < 3; x++) at[x] = abbD * t[x];
     // VaD += sqrt(vac2) / 2
     double avac2 = (vac2 > 0.0) ? 0.25 / sqrt(vac2) * aVaD : 0.0;
-    // vn0 = erfc(z)/2, z = s sqrt(lam) vn
     const double az = -GAM_RSQRT_PI * ez * avn0;
     const double avn = s * sl * az;
     const double alam = (s * vn * az) * 0.5 / sl;
-    // lam = 1 / (2 p / d + vac2)
     const double aX = -lam * lam * alam;
-    double ap = 2 * id * aX, aid = 2 * p * aX;
+    const double ap = 2 * id * aX;
+    double aid = 2 * p * aX;
     avac2 += aX;
-    // vac2 = 2 dPb / d
-    adPb += 2 * id * avac2;
-    aid += 2 * dPb * avac2;
-    for (int x = 0; x < 3; x++) aB[x] += ab[x] + b[x] * adPb;
+    const double ava2 = BORIS ? avac2 / (den * den) : avac2;
+    if (BG) {
+        const double att = id * ava2;
+        aid += tt * ava2;
+        for (int x = 0; x < 3; x++) at[x] += 2 * t[x] * att;
+    } else {
+        adPb += 2 * id * ava2;
+        aid += 2 * dPb * ava2;
+    }
+    for (int x = 0; x < 3; x++) aB[x] += ab[x] + b[x] * adPb + at[x];
     aW[DEN] += -id * id * aid;
     aW[PRS] += ap;
-    aW[1] += nx * avn;
-    aW[2] += ny * avn;
-    aW[3] += nz * avn;
+    aW[1] += n[0] * avn;
+    aW[2] += n[1] * avn;
+    aW[3] += n[2] * avn;
 }
 
 // ---------------------------------------------------------- cell B
@@ -510,4 +533,124 @@ __device__ __forceinline__ void edge_emf_adj(const MParm &m, const EdgeIn &in, d
     for (int d = 0; d < 3; d++)
         for (int n = 0; n < 8; n++) out.V1[d][n] += W8(n) * in.A1[n] * rdA * avc[d];
     for (int n = 0; n < 4; n++) out.D4[n] += 0.25 * aDc;
+}
+
+// CellBoris (the Boris branch of k_update), single fluid.  Inputs: oU =
+// State(n) gas, dH = gas flux divergence, dM = magnetic stress divergence
+// (with dpB0), B = total cell B of the new state, oB = that of State(n),
+// dtM = Model%dt (Reynolds), dt = the advance dt.  aUp = adjoint of the
+// updated gas; adds into aoU, adH, adM, aB, aoB.
+__device__ __forceinline__ void boris_adj(const MParm &m, const double *oU, const double *dH,
+                                          const double *dM, const double *B, const double *oB, double dtM,
+                                          double dt, const double *aUp, double *aoU, double *adH,
+                                          double *adM, double *aB, double *aoB)
+{
+    double U[5], W2[5];
+    for (int v = 0; v < 5; v++) U[v] = oU[v] + dtM * dH[v];
+    c2p_cell(m, U, W2);
+    const double D2 = dmax(W2[DEN], m.dFloor), P2 = dmax(W2[PRS], m.pFloor);
+    const double D0 = dmax(oU[DEN], m.dFloor);
+    const double iD0 = 1.0 / D0, iD2 = 1.0 / D2;
+    double V0[3], B1[3], dPG[3], dPM[3];
+    for (int x = 0; x < 3; x++) {
+        V0[x] = oU[1 + x] * iD0;
+        B1[x] = 0.5 * (B[x] + oB[x]);
+        dPG[x] = dt * dH[1 + x];
+        dPM[x] = dt * dM[x];
+    }
+    const double b2 = B1[0] * B1[0] + B1[1] * B1[1] + B1[2] * B1[2];
+    const double alpha = b2 * (iD2 * m.iCa2);
+    const double dRho = dt * dH[DEN];
+    const double pCon = 1.0 / (1.0 + alpha);
+    const bool bor = alpha > GAM_TINY;
+    double bh[3] = {0.0, 0.0, 0.0}, sGM[3], dMom[3];
+    const double nb = sqrt(b2);
+    const bool nbOk = nb > GAM_TINY;
+    if (nbOk)
+        for (int x = 0; x < 3; x++) bh[x] = B1[x] / nb;
+    for (int x = 0; x < 3; x++) sGM[x] = dPG[x] + dPM[x];
+    const double pG = dPG[0] * bh[0] + dPG[1] * bh[1] + dPG[2] * bh[2];
+    const double pS = sGM[0] * bh[0] + sGM[1] * bh[1] + sGM[2] * bh[2];
+    const double pV = V0[0] * bh[0] + V0[1] * bh[1] + V0[2] * bh[2];
+    for (int x = 0; x < 3; x++)
+        dMom[x] = bor ? pG * bh[x] + pCon * (sGM[x] - pS * bh[x]) + alpha * pCon * dRho * (V0[x] - pV * bh[x])
+                      : dPG[x] + dPM[x];
+    double Wn[5];
+    Wn[DEN] = D2;
+    for (int x = 0; x < 3; x++) Wn[1 + x] = (D0 * V0[x] + dMom[x]) * iD2;
+    Wn[PRS] = P2;
+    // reverse
+    double aWn[5] = {0, 0, 0, 0, 0};
+    p2c_cell_adj(m, Wn, aUp, aWn);
+    double aD2 = aWn[DEN], aP2 = aWn[PRS], aiD2 = 0.0, aD0 = 0.0;
+    double aV0[3] = {0, 0, 0}, adMom[3], aDV[3];
+    for (int x = 0; x < 3; x++) {
+        aDV[x] = iD2 * aWn[1 + x];          // adjoint of D0 * V0[x]
+        adMom[x] = iD2 * aWn[1 + x];
+        aiD2 += (D0 * V0[x] + dMom[x]) * aWn[1 + x];
+        aD0 += V0[x] * aDV[x];
+        aV0[x] += D0 * aDV[x];
+    }
+    double adPG[3] = {0, 0, 0}, adPM[3] = {0, 0, 0}, aB1[3] = {0, 0, 0}, aalpha = 0.0, adRho = 0.0;
+    if (bor) {
+        const double K = alpha * pCon * dRho;
+        double apG = 0.0, apS = 0.0, apV = 0.0, apCon = 0.0, abh[3] = {0, 0, 0}, asGM[3];
+        for (int x = 0; x < 3; x++) {
+            const double a = adMom[x];
+            apG += bh[x] * a;
+            abh[x] += (pG - pCon * pS - K * pV) * a;
+            apCon += (sGM[x] - pS * bh[x]) * a + alpha * dRho * (V0[x] - pV * bh[x]) * a;
+            asGM[x] = pCon * a;
+            apS += -pCon * bh[x] * a;
+            aalpha += pCon * dRho * (V0[x] - pV * bh[x]) * a;
+            adRho += alpha * pCon * (V0[x] - pV * bh[x]) * a;
+            aV0[x] += K * a;
+            apV += -K * bh[x] * a;
+        }
+        for (int x = 0; x < 3; x++) {
+            adPG[x] += bh[x] * apG;
+            abh[x] += dPG[x] * apG;
+            asGM[x] += bh[x] * apS;
+            abh[x] += sGM[x] * apS;
+            aV0[x] += bh[x] * apV;
+            abh[x] += V0[x] * apV;
+            adPG[x] += asGM[x];
+            adPM[x] += asGM[x];
+        }
+        if (nbOk) {
+            const double bda = bh[0] * abh[0] + bh[1] * abh[1] + bh[2] * abh[2];
+            for (int x = 0; x < 3; x++) aB1[x] += (abh[x] - bh[x] * bda) / nb;
+        }
+        aalpha += -pCon * pCon * apCon;
+    } else {
+        for (int x = 0; x < 3; x++) { adPG[x] += adMom[x]; adPM[x] += adMom[x]; }
+    }
+    // alpha = |B1|^2 / (D2 Ca^2)
+    const double ab2 = iD2 * m.iCa2 * aalpha;
+    aiD2 += b2 * m.iCa2 * aalpha;
+    for (int x = 0; x < 3; x++) {
+        aB1[x] += 2 * B1[x] * ab2;
+        aB[x] += 0.5 * aB1[x];
+        aoB[x] += 0.5 * aB1[x];
+        adM[x] += dt * adPM[x];
+        adH[1 + x] += dt * adPG[x];
+    }
+    adH[DEN] += dt * adRho;
+    // V0 = oU_mom / D0, D0 = max(oU_rho, dFloor)
+    double aiD0 = 0.0;
+    for (int x = 0; x < 3; x++) {
+        aoU[1 + x] += iD0 * aV0[x];
+        aiD0 += oU[1 + x] * aV0[x];
+    }
+    aD0 += -iD0 * iD0 * aiD0;
+    if (oU[DEN] > m.dFloor) aoU[DEN] += aD0;
+    aD2 += -iD2 * iD2 * aiD2;
+    double aW2[5] = {0, 0, 0, 0, 0}, aU[5] = {0, 0, 0, 0, 0};
+    if (W2[DEN] > m.dFloor) aW2[DEN] = aD2;
+    if (W2[PRS] > m.pFloor) aW2[PRS] = aP2;
+    c2p_cell_adj(m, U, aW2, aU);
+    for (int v = 0; v < 5; v++) {
+        aoU[v] += aU[v];
+        adH[v] += dtM * aU[v];
+    }
 }

@@ -307,20 +307,25 @@ __global__ void k_bc_periodic_adj(Geo g, Box t, double *__restrict__ aS)
     }
 }
 
-// Update transpose, part 1 (cells): gas of the Reynolds + Maxwell update
-// (active cells) or the copy (other cells).  Writes aUr (the adjoint of the
-// Reynolds-updated gas) and adM (of the magnetic stress divergence).
+// Update transpose, part 1 (cells): gas of the Reynolds + Maxwell/Boris
+// update (active cells) or the copy (other cells).  Writes adH (adjoint of
+// the gas flux divergence) and adM (of the magnetic stress divergence);
+// with Boris also adds the adjoints of the new and old cell B (total B
+// minus the constant background) into aSn and aSo.  Sn1 = the step's
+// output State (its cell B is the new B of the update).
+template <bool BORIS>
 __global__ __launch_bounds__(256) void k_update_adj_cells(Geo g, MParm m, Box cb, Box ab,
                                                           const Clock *clk,
                                                           const double *__restrict__ So,
+                                                          const double *__restrict__ Sn1,
                                                           const double *__restrict__ gflx,
                                                           const double *__restrict__ mflx,
                                                           const double *__restrict__ met,
                                                           const double *__restrict__ bg, int BG,
-                                                          const double *__restrict__ aSn,
+                                                          double *__restrict__ aSn,
                                                           double *__restrict__ aSo,
-                                                          double *__restrict__ aUr,
-                                                          double *__restrict__ adM)
+                                                          double *__restrict__ adHo,
+                                                          double *__restrict__ adMo)
 {
     POINT_IN(cb);
     const long N3 = g.N3;
@@ -332,11 +337,12 @@ __global__ __launch_bounds__(256) void k_update_adj_cells(Geo g, MParm m, Box cb
     const double idV = 1.0 / met[M_VOL * N3 + c];
     const double dtM = clk->dt, dt = clk->dtA;
     const long oI = c + 1, oJ = c + g.P, oK = c + g.PP;
-    double U[5], dM[3], aUp[5], aU[5] = {0, 0, 0, 0, 0}, adm[3] = {0, 0, 0};
+    double oU[5], dH[5], dM[3], aUp[5], aoU[5] = {0, 0, 0, 0, 0}, adH[5] = {0, 0, 0, 0, 0},
+           adm[3] = {0, 0, 0};
     for (int v = 0; v < 5; v++) {
         const double *gv = gflx + (long)v * N3;
-        const double dH = (gv[c] - gv[oI] + gv[5 * N3 + c] - gv[5 * N3 + oJ] + gv[10 * N3 + c] - gv[10 * N3 + oK]) * idV;
-        U[v] = So[v * N3 + c] + dtM * dH;
+        dH[v] = (gv[c] - gv[oI] + gv[5 * N3 + c] - gv[5 * N3 + oJ] + gv[10 * N3 + c] - gv[10 * N3 + oK]) * idV;
+        oU[v] = So[v * N3 + c];
         aUp[v] = aSn[v * N3 + c];
     }
     for (int x = 0; x < 3; x++) {
@@ -345,12 +351,32 @@ __global__ __launch_bounds__(256) void k_update_adj_cells(Geo g, MParm m, Box cb
         if (BG) r = r + bg[(B_DPB0 + x) * N3 + c];
         dM[x] = r;
     }
-    maxwell_adj(m, U, dM, dt, aUp, aU, adm);
-    for (int v = 0; v < 5; v++) {
-        aSo[v * N3 + c] += aU[v];
-        aUr[v * N3 + c] = aU[v];
+    if (BORIS) {
+        double B[3], oB[3], aB[3] = {0, 0, 0}, aoB[3] = {0, 0, 0};
+        for (int x = 0; x < 3; x++) {
+            const double b0 = BG ? bg[(B_B0 + x) * N3 + c] : 0.0;
+            B[x] = Sn1[(S_B + x) * N3 + c] + b0;
+            oB[x] = So[(S_B + x) * N3 + c] + b0;
+        }
+        boris_adj(m, oU, dH, dM, B, oB, dtM, dt, aUp, aoU, adH, adm, aB, aoB);
+        for (int x = 0; x < 3; x++) {
+            aSn[(S_B + x) * N3 + c] += aB[x];
+            aSo[(S_B + x) * N3 + c] += aoB[x];
+        }
+    } else {
+        double U[5], aU[5] = {0, 0, 0, 0, 0};
+        for (int v = 0; v < 5; v++) U[v] = oU[v] + dtM * dH[v];
+        maxwell_adj(m, U, dM, dt, aUp, aU, adm);
+        for (int v = 0; v < 5; v++) {
+            aoU[v] = aU[v];
+            adH[v] = dtM * aU[v];
+        }
     }
-    for (int x = 0; x < 3; x++) adM[x * N3 + c] = adm[x];
+    for (int v = 0; v < 5; v++) {
+        aSo[v * N3 + c] += aoU[v];
+        adHo[v * N3 + c] = adH[v];
+    }
+    for (int x = 0; x < 3; x++) adMo[x * N3 + c] = adm[x];
 }
 
 // Update transpose, part 2 (faces): adjoints of the active faces' gas and
@@ -369,7 +395,7 @@ __global__ __launch_bounds__(256) void k_update_adj_faces(Geo g, Box fb, Box ab,
     POINT_IN(fb);
     const long N3 = g.N3;
     const long c = IX(g, i, j, k);
-    const double dtM = clk->dt, dtA = clk->dtA;
+    const double dtA = clk->dtA;
     const long st[3] = {1, g.P, g.PP};
     for (int d = 0; d < 3; d++) {
         const bool face = i >= g.is && j >= g.js && k >= g.ks && i <= g.ie + (d == 0) &&
@@ -380,9 +406,9 @@ __global__ __launch_bounds__(256) void k_update_adj_faces(Geo g, Box fb, Box ab,
         const bool lo = inside(ab, i - (d == 0), j - (d == 1), k - (d == 2));
         const double wu = up ? 1.0 / met[M_VOL * N3 + c] : 0.0;
         const double wl = lo ? 1.0 / met[M_VOL * N3 + cl] : 0.0;
-        // aUr, adM hold values at active cells only
+        // adH (in aUr), adM hold values at active cells only
         for (int v = 0; v < 5; v++)
-            ag[(v + 5 * d) * N3 + c] = dtM * ((up ? wu * aUr[v * N3 + c] : 0.0) - (lo ? wl * aUr[v * N3 + cl] : 0.0));
+            ag[(v + 5 * d) * N3 + c] = (up ? wu * aUr[v * N3 + c] : 0.0) - (lo ? wl * aUr[v * N3 + cl] : 0.0);
         for (int x = 0; x < 3; x++)
             amf[(x + 3 * d) * N3 + c] = (up ? wu * adM[x * N3 + c] : 0.0) - (lo ? wl * adM[x * N3 + cl] : 0.0);
     }
@@ -400,21 +426,23 @@ __global__ __launch_bounds__(256) void k_update_adj_faces(Geo g, Box fb, Box ab,
                 ? dtA * (fa(0, i, j, k) - fa(0, i, j - 1, k) + fa(1, i - 1, j, k) - fa(1, i, j, k)) : 0.0;
 }
 
-// Interface fluxes' transpose with the magnetic terms (no background
-// field, no Boris): gas and magnetic kinetic fluxes, HOGS, L/R states of
-// gas (H_WH, H_WN) and cell B (H_B weighted by volume, limiter S_B), and
-// the normal flux Bn = H_MF / A.
-template <int D, int RM>
+// Interface fluxes' transpose with the magnetic terms: gas and magnetic
+// kinetic fluxes (background-field cross terms, Boris factor), HOGS, the
+// Boris mag-HOGS and nuke terms, L/R states of gas (H_WH, H_WN) and cell B
+// (H_B weighted by volume, limiter S_B), the normal flux Bn = H_MF / A.
+template <int D, int RM, bool BG, bool BORIS>
 __global__ __launch_bounds__(256) void k_flux_adj_mhd(Geo g, MParm m, Box fb,
                                                       const double *__restrict__ H,
                                                       const double *__restrict__ S,
                                                       const double *__restrict__ met,
+                                                      const double *__restrict__ bg,
                                                       const double *__restrict__ ag,
                                                       const double *__restrict__ amf,
                                                       double *__restrict__ aWH,
                                                       double *__restrict__ aWN,
                                                       double *__restrict__ aHB,
                                                       double *__restrict__ aHMF,
+                                                      double *__restrict__ aHG,
                                                       double *__restrict__ aSS)
 {
     POINT_IN(fb);
@@ -441,28 +469,55 @@ __global__ __launch_bounds__(256) void k_flux_adj_mhd(Geo g, MParm m, Box fb,
     double T[9];
     for (int q = 0; q < 9; q++) T[q] = met[(M_TF + q + 9 * D) * N3 + c];
     const double faB = met[(M_FACE + D) * N3 + c];
-    const double Bn = H[(H_MF + D) * N3 + c] / faB;
-    // forward pieces the adjoint needs: VaD and the HOGS differences dW
-    double mF[3] = {0, 0, 0}, bbD = 0.0, VaD = 0.0, B0[3] = {0, 0, 0};
-    mag_kin<false, false>(m, -1.0, WL, BL, Bn, T, B0, 0.0, mF, bbD, VaD);
-    mag_kin<false, false>(m, +1.0, WR, BR, Bn, T, B0, 0.0, mF, bbD, VaD);
+    const double iFaB = 1.0 / faB;
+    const double Bn = H[(H_MF + D) * N3 + c] * iFaB;
+    double B0[3] = {0, 0, 0}, B0n = 0.0;
+    if (BG) {
+        for (int x = 0; x < 3; x++) B0[x] = bg[(B_FCB0 + x + 3 * D) * N3 + c];
+        B0n = bg[(B_BFLUX0 + D) * N3 + c] * iFaB;
+    }
+    // forward pieces the adjoint needs: VaD, bbD and the HOGS differences dW
+    double mF[3] = {0, 0, 0}, bbD = 0.0, VaD = 0.0;
+    mag_kin<BG, BORIS>(m, -1.0, WL, BL, Bn, T, B0, B0n, mF, bbD, VaD);
+    mag_kin<BG, BORIS>(m, +1.0, WR, BR, Bn, T, B0, B0n, mF, bbD, VaD);
     double gF[5] = {0, 0, 0, 0, 0}, dW[5] = {0, 0, 0, 0, 0};
     gas_kin(m.igm1, -1.0, WL, T, gF, dW);
     gas_kin(m.igm1, +1.0, WR, T, gF, dW);
     // adjoint
-    double aF[5], adW[5], aM[3], aVaD = 0.0;
+    double aF[5], adW[5], aM[3], aVaD = 0.0, abbD = 0.0;
+    double aL[5] = {0, 0, 0, 0, 0}, aR[5] = {0, 0, 0, 0, 0}, abL[3] = {0, 0, 0}, abR[3] = {0, 0, 0}, aBn = 0.0;
     for (int v = 0; v < 5; v++) {
         aF[v] = faB * ag[(v + 5 * D) * N3 + c];
         adW[v] = -m.cHogH * VaD * aF[v];
         aVaD += -m.cHogH * dW[v] * aF[v];
     }
     for (int x = 0; x < 3; x++) aM[x] = faB * amf[(x + 3 * D) * N3 + c];
-    double aL[5] = {0, 0, 0, 0, 0}, aR[5] = {0, 0, 0, 0, 0}, abL[3] = {0, 0, 0}, abR[3] = {0, 0, 0}, aBn = 0.0;
+    if (BORIS) {
+        // mag-HOGS: mF -= cHogM VaD bbD (WR - WL)_v / Ca^2
+        for (int x = 0; x < 3; x++) {
+            const double dv = WR[1 + x] - WL[1 + x];
+            const double deeP = bbD * dv * m.iCa2;
+            aVaD += -m.cHogM * deeP * aM[x];
+            const double adeeP = -m.cHogM * VaD * aM[x];
+            abbD += dv * m.iCa2 * adeeP;
+            aR[1 + x] += bbD * m.iCa2 * adeeP;
+            aL[1 + x] -= bbD * m.iCa2 * adeeP;
+        }
+        // nuke: gF -= Ca (U(c) - U(c - s)) where the mean speed is large
+        const double vL = sqrt(WL[1] * WL[1] + WL[2] * WL[2] + WL[3] * WL[3]);
+        const double vR = sqrt(WR[1] * WR[1] + WR[2] * WR[2] + WR[3] * WR[3]);
+        const double vn = sqrt(vL * vL + vR * vR) * 0.70710678118654752440084436210484903928483593768847;
+        if (vn >= 1.5 * m.Ca)
+            for (int v = 0; v < 5; v++) {
+                AADD(aHG + v * N3 + c, -m.Ca * aF[v]);
+                AADD(aHG + v * N3 + c - s, m.Ca * aF[v]);
+            }
+    }
     gas_kin_adj(m.igm1, -1.0, WL, T, aF, aL, adW);
     gas_kin_adj(m.igm1, +1.0, WR, T, aF, aR, adW);
-    mag_kin_adj(-1.0, WL, BL, Bn, T, aM, aVaD, aL, abL, aBn);
-    mag_kin_adj(+1.0, WR, BR, Bn, T, aM, aVaD, aR, abR, aBn);
-    AADD(aHMF + D * N3 + c, aBn / faB);
+    mag_kin_adj<BG, BORIS>(m, -1.0, WL, BL, Bn, T, B0, B0n, aM, aVaD, abbD, aL, abL, aBn);
+    mag_kin_adj<BG, BORIS>(m, +1.0, WR, BR, Bn, T, B0, B0n, aM, aVaD, abbD, aR, abR, aBn);
+    AADD(aHMF + D * N3 + c, aBn * iFaB);
     for (int v = 0; v < 5; v++) {
         double qv[8], nq[4], aqv[8] = {0, 0, 0, 0, 0, 0, 0, 0}, anq[4] = {0, 0, 0, 0};
         for (int n = 0; n < 8; n++) qv[n] = H[(H_WH + v) * N3 + c + (n - 4) * s];
@@ -753,39 +808,60 @@ void launch_bc_periodic_adj(hipStream_t s, int side, const Geo &g, double *aS)
 }
 
 void launch_update_adj_mhd(hipStream_t s, const Geo &g, const MParm &m, const Box &ab, const Box &ctb,
-                           const Box *eboxes, const Clock *clk, const double *So, const double *gflx,
-                           const double *mflx, const double *met, const double *bg, const double *aSn,
-                           double *aSo, double *aUr, double *adM, double *ag, double *amf, double *aE)
+                           const Box *eboxes, const Clock *clk, const double *So, const double *Sn1,
+                           const double *gflx, const double *mflx, const double *met, const double *bg,
+                           double *aSn, double *aSo, double *aUr, double *adM, double *ag, double *amf,
+                           double *aE)
 {
     const Box cb = cells_of(g), fb = faces_of(g);
-    hipLaunchKernelGGL(k_update_adj_cells, agrid(cb), dim3(ABX, ABY), 0, s, g, m, cb, ab, clk, So, gflx, mflx,
-                       met, bg, m.doBG, aSn, aSo, aUr, adM);
+    if (m.doBoris)
+        hipLaunchKernelGGL(k_update_adj_cells<true>, agrid(cb), dim3(ABX, ABY), 0, s, g, m, cb, ab, clk, So, Sn1,
+                           gflx, mflx, met, bg, m.doBG, aSn, aSo, aUr, adM);
+    else
+        hipLaunchKernelGGL(k_update_adj_cells<false>, agrid(cb), dim3(ABX, ABY), 0, s, g, m, cb, ab, clk, So, Sn1,
+                           gflx, mflx, met, bg, m.doBG, aSn, aSo, aUr, adM);
+    // the update's new cell B from its new face fluxes (every cell)
+    launch_cellb_adj(s, g, 1, aSn, met);
     EdgeBoxes eb;
     for (int d = 0; d < 3; d++) eb.e[d] = eboxes[d];
     hipLaunchKernelGGL(k_update_adj_faces, agrid(fb), dim3(ABX, ABY), 0, s, g, fb, ab, ctb, eb, clk, met, aUr,
                        adM, aSn, aSo, ag, amf, aE);
 }
 
+template <int D, int RM>
+static void flux_adj_mhd_r(hipStream_t s, const Geo &g, const MParm &m, const Box &fb, const double *H,
+                           const double *S, const double *met, const double *bg, const double *ag,
+                           const double *amf, double *aWH, double *aWN, double *aHB, double *aHMF, double *aHG,
+                           double *aSS)
+{
+#define FLUXADJ(BG, BO)                                                                                      \
+    hipLaunchKernelGGL((k_flux_adj_mhd<D, RM, BG, BO>), agrid(fb), dim3(ABX, ABY), 0, s, g, m, fb, H, S, met, \
+                       bg, ag, amf, aWH, aWN, aHB, aHMF, aHG, aSS)
+    if (m.doBG && m.doBoris) FLUXADJ(true, true);
+    else if (m.doBG) FLUXADJ(true, false);
+    else if (m.doBoris) FLUXADJ(false, true);
+    else FLUXADJ(false, false);
+#undef FLUXADJ
+}
+
 template <int D>
 static void flux_adj_mhd_d(hipStream_t s, const Geo &g, const MParm &m, const Box &fb, const double *H,
-                           const double *S, const double *met, const double *ag, const double *amf,
-                           double *aWH, double *aWN, double *aHB, double *aHMF, double *aSS)
+                           const double *S, const double *met, const double *bg, const double *ag,
+                           const double *amf, double *aWH, double *aWN, double *aHB, double *aHMF, double *aHG,
+                           double *aSS)
 {
-    if (m.rmeth)
-        hipLaunchKernelGGL((k_flux_adj_mhd<D, 1>), agrid(fb), dim3(ABX, ABY), 0, s, g, m, fb, H, S, met, ag, amf,
-                           aWH, aWN, aHB, aHMF, aSS);
-    else
-        hipLaunchKernelGGL((k_flux_adj_mhd<D, 0>), agrid(fb), dim3(ABX, ABY), 0, s, g, m, fb, H, S, met, ag, amf,
-                           aWH, aWN, aHB, aHMF, aSS);
+    if (m.rmeth) flux_adj_mhd_r<D, 1>(s, g, m, fb, H, S, met, bg, ag, amf, aWH, aWN, aHB, aHMF, aHG, aSS);
+    else flux_adj_mhd_r<D, 0>(s, g, m, fb, H, S, met, bg, ag, amf, aWH, aWN, aHB, aHMF, aHG, aSS);
 }
 
 void launch_flux_adj_mhd(hipStream_t s, int d, const Geo &g, const MParm &m, const Box &fb, const double *H,
-                         const double *S, const double *met, const double *ag, const double *amf, double *aWH,
-                         double *aWN, double *aHB, double *aHMF, double *aSS)
+                         const double *S, const double *met, const double *bg, const double *ag,
+                         const double *amf, double *aWH, double *aWN, double *aHB, double *aHMF, double *aHG,
+                         double *aSS)
 {
-    if (d == 0) flux_adj_mhd_d<0>(s, g, m, fb, H, S, met, ag, amf, aWH, aWN, aHB, aHMF, aSS);
-    else if (d == 1) flux_adj_mhd_d<1>(s, g, m, fb, H, S, met, ag, amf, aWH, aWN, aHB, aHMF, aSS);
-    else flux_adj_mhd_d<2>(s, g, m, fb, H, S, met, ag, amf, aWH, aWN, aHB, aHMF, aSS);
+    if (d == 0) flux_adj_mhd_d<0>(s, g, m, fb, H, S, met, bg, ag, amf, aWH, aWN, aHB, aHMF, aHG, aSS);
+    else if (d == 1) flux_adj_mhd_d<1>(s, g, m, fb, H, S, met, bg, ag, amf, aWH, aWN, aHB, aHMF, aHG, aSS);
+    else flux_adj_mhd_d<2>(s, g, m, fb, H, S, met, bg, ag, amf, aWH, aWN, aHB, aHMF, aHG, aSS);
 }
 
 template <int ED, int RM, bool BG, bool BO>
@@ -802,8 +878,19 @@ static void emf_adj_d(hipStream_t s, const Geo &g, const MParm &m, const Box &eb
                       const double *H, const double *S, const double *Vf, const double *met, const double *bg,
                       const double *aE, double *aVf, double *aHG, double *aHMF, double *aSS)
 {
-    if (m.rmeth) emf_adj_t<ED, 1, false, false>(s, g, m, eb, clk, H, S, Vf, met, bg, aE, aVf, aHG, aHMF, aSS);
-    else emf_adj_t<ED, 0, false, false>(s, g, m, eb, clk, H, S, Vf, met, bg, aE, aVf, aHG, aHMF, aSS);
+    const int sel = (m.rmeth ? 4 : 0) + (m.doBG ? 2 : 0) + (m.doBoris ? 1 : 0);
+#define EMFADJ(RM, BG, BO) emf_adj_t<ED, RM, BG, BO>(s, g, m, eb, clk, H, S, Vf, met, bg, aE, aVf, aHG, aHMF, aSS)
+    switch (sel) {
+    case 0: EMFADJ(0, false, false); break;
+    case 1: EMFADJ(0, false, true); break;
+    case 2: EMFADJ(0, true, false); break;
+    case 3: EMFADJ(0, true, true); break;
+    case 4: EMFADJ(1, false, false); break;
+    case 5: EMFADJ(1, false, true); break;
+    case 6: EMFADJ(1, true, false); break;
+    default: EMFADJ(1, true, true); break;
+    }
+#undef EMFADJ
 }
 
 void launch_emf_adj(hipStream_t s, int ed, const Geo &g, const MParm &m, const Box &eb, const Clock *clk,

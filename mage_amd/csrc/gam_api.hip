@@ -1208,7 +1208,7 @@ struct gam_adj {
     double *ck;              // nck x (State, oState), S_NPL planes each
     double *fin;             // State, oState at the end
     Clock *clkh;             // clock before step n, n = 0..nsteps
-    double *tape;            // K + 1 State buffers of one interval
+    double *tape;            // K + 2 State buffers of one interval
     double *lam;             // three adjoint state sets (ADJ_SET planes each)
     double *ag, *amf;        // adjoints of gFlx (15) and mFlx (9)
     double *awh, *awn, *ahg; // adjoints of H_WH, H_WN, H_GAS (5 each)
@@ -1239,8 +1239,8 @@ extern "C" int gam_adj_create(gam_adj **out, gam_ctx *c, int nsteps, int K)
     *out = nullptr;
     int rc = check_ready(c, "gam_adj_create");
     if (rc) return rc;
-    if (c->mp.doBoris || c->mp.doBG)
-        return fail(GAM_E_ARG, "gam_adj_create: the adjoint covers runs without Boris correction and background field");
+    if (!c->mp.doMHD && (c->mp.doBoris || c->mp.doBG))
+        return fail(GAM_E_ARG, "gam_adj_create: Boris correction and background field need MHD");
     for (int sd = 0; sd < 6; sd++) {
         if (c->model.bc[sd] == GAM_BC_WIND) return fail(GAM_E_ARG, "gam_adj_create: the inflow BC has no adjoint yet");
         if (c->mp.doMHD && c->model.bc[sd] != GAM_BC_PERIODIC)
@@ -1259,7 +1259,7 @@ extern "C" int gam_adj_create(gam_adj **out, gam_ctx *c, int nsteps, int K)
     AALLOC(a->ck, (size_t)a->nck * 2 * S_NPL * pl);
     AALLOC(a->fin, 2 * S_NPL * pl);
     AALLOC(a->clkh, (size_t)(nsteps + 1) * sizeof(Clock));
-    AALLOC(a->tape, (size_t)(K + 1) * S_NPL * pl);
+    AALLOC(a->tape, (size_t)(K + 2) * S_NPL * pl);
     AALLOC(a->lam, 3 * ADJ_SET * pl);
     AALLOC(a->ag, 15 * pl);
     AALLOC(a->amf, 9 * pl);
@@ -1384,9 +1384,10 @@ static void adj_bcs(gam_adj *a, double *l)
 
 // One step's adjoint: lA (wrt the step's output State) and lB (wrt its
 // output oState = input State) in, lB (input State) and lC (input oState)
-// out.  Sn, On = the step's input State and oState; clk its clock.
-static void adj_step(gam_adj *a, const double *Sn, const double *On, const Clock *clk, double *lA,
-                     double *lB, double *lC)
+// out.  Sn, On = the step's input State and oState, Sn1 its output State;
+// clk its clock.
+static void adj_step(gam_adj *a, const double *Sn, const double *On, const double *Sn1, const Clock *clk,
+                     double *lA, double *lB, double *lC)
 {
     gam_ctx *c = a->c;
     const Geo &g = c->g;
@@ -1404,13 +1405,12 @@ static void adj_step(gam_adj *a, const double *Sn, const double *On, const Clock
             if (d == 0) fb.i1 += 1; else if (d == 1) fb.j1 += 1; else fb.k1 += 1;
             launch_flux(s, d, c->plan.v[TS_FLUX_I + d], g, c->mp, fb, c->H, Sn, c->met, c->bg, c->gflx, c->mflx);
         }
-        // ghost bFlux2Fld, boundary pass, cell B of the update, update
+        // ghost bFlux2Fld, boundary pass, update (with its cell B)
         launch_cellb_adj(s, g, 0, lA, c->met);
         adj_bcs(a, lA);
-        launch_cellb_adj(s, g, 1, lA, c->met);
         const Box ctb = mkbox(g.is, g.ie + 1, g.js, g.je + 1, g.ks, g.ke + 1);
-        launch_update_adj_mhd(s, g, c->mp, act_box(g), ctb, eb, clk, Sn, c->gflx, c->mflx, c->met, c->bg, lA,
-                              lB, a->aur, a->adm, a->ag, a->amf, a->ae);
+        launch_update_adj_mhd(s, g, c->mp, act_box(g), ctb, eb, clk, Sn, Sn1, c->gflx, c->mflx, c->met, c->bg,
+                              lA, lB, a->aur, a->adm, a->ag, a->amf, a->ae);
         hipMemsetAsync(a->awh, 0, 5 * pl, s);
         hipMemsetAsync(a->awn, 0, 5 * pl, s);
         hipMemsetAsync(a->ahg, 0, 5 * pl, s);
@@ -1420,8 +1420,8 @@ static void adj_step(gam_adj *a, const double *Sn, const double *On, const Clock
         for (int d = 0; d < 3; d++) {
             Box fb = act_box(g);
             if (d == 0) fb.i1 += 1; else if (d == 1) fb.j1 += 1; else fb.k1 += 1;
-            launch_flux_adj_mhd(s, d, g, c->mp, fb, c->H, Sn, c->met, a->ag, a->amf, a->awh, a->awn, a->ahb,
-                                a->ahmf, lB);
+            launch_flux_adj_mhd(s, d, g, c->mp, fb, c->H, Sn, c->met, c->bg, a->ag, a->amf, a->awh, a->awn,
+                                a->ahb, a->ahmf, a->ahg, lB);
         }
         for (int ed = 0; ed < 3; ed++)
             launch_emf_adj(s, ed, g, c->mp, eb[ed], clk, c->H, Sn, c->vf, c->met, c->bg, a->ae, a->avf, a->ahg,
@@ -1464,17 +1464,17 @@ extern "C" int gam_adj_backward(gam_adj *a, double *gGas, double *gGasO, double 
         const int s0 = seg * a->K, s1 = (s0 + a->K < a->nsteps) ? s0 + a->K : a->nsteps;
         const double *ck = a->ck + (size_t)seg * 2 * S_NPL * N3;
         if ((rc = adj_put(a, s0, ck))) return rc;
-        // tape[0] = oState at s0, tape[1 + q] = State at step s0 + q
+        // tape[0] = oState at s0, tape[1 + q] = State at step s0 + q, q = 0..s1-s0
         HIPCHK(hipMemcpyAsync(a->tape, ck + (size_t)S_NPL * N3, bytes, hipMemcpyDeviceToDevice, s));
-        for (int q = 0; q < s1 - s0; q++) {
+        for (int q = 0; q <= s1 - s0; q++) {
             HIPCHK(hipMemcpyAsync(a->tape + (size_t)(1 + q) * S_NPL * N3, S_cur(c), bytes,
                                   hipMemcpyDeviceToDevice, s));
-            if (q + 1 < s1 - s0) enq_step(c);
+            if (q < s1 - s0) enq_step(c);
         }
         for (int n = s1 - 1; n >= s0; n--) {
             const int q = n - s0;
-            adj_step(a, a->tape + (size_t)(1 + q) * S_NPL * N3, a->tape + (size_t)q * S_NPL * N3, a->clkh + n,
-                     lA, lB, lC);
+            adj_step(a, a->tape + (size_t)(1 + q) * S_NPL * N3, a->tape + (size_t)q * S_NPL * N3,
+                     a->tape + (size_t)(2 + q) * S_NPL * N3, a->clkh + n, lA, lB, lC);
             double *t = lA;
             lA = lB;
             lB = lC;

@@ -123,12 +123,14 @@ void launch_set_dt(hipStream_t s, Clock *clk, const Clock *rec);
 void launch_cellb_adj(hipStream_t s, const Geo &g, int all, double *aS, const double *met);
 void launch_bc_periodic_adj(hipStream_t s, int side, const Geo &g, double *aS);
 void launch_update_adj_mhd(hipStream_t s, const Geo &g, const MParm &m, const Box &ab, const Box &ctb,
-                           const Box *eboxes, const Clock *clk, const double *So, const double *gflx,
-                           const double *mflx, const double *met, const double *bg, const double *aSn,
-                           double *aSo, double *aUr, double *adM, double *ag, double *amf, double *aE);
+                           const Box *eboxes, const Clock *clk, const double *So, const double *Sn1,
+                           const double *gflx, const double *mflx, const double *met, const double *bg,
+                           double *aSn, double *aSo, double *aUr, double *adM, double *ag, double *amf,
+                           double *aE);
 void launch_flux_adj_mhd(hipStream_t s, int d, const Geo &g, const MParm &m, const Box &fb, const double *H,
-                         const double *S, const double *met, const double *ag, const double *amf, double *aWH,
-                         double *aWN, double *aHB, double *aHMF, double *aSS);
+                         const double *S, const double *met, const double *bg, const double *ag,
+                         const double *amf, double *aWH, double *aWN, double *aHB, double *aHMF, double *aHG,
+                         double *aSS);
 void launch_emf_adj(hipStream_t s, int ed, const Geo &g, const MParm &m, const Box &eb, const Clock *clk,
                     const double *H, const double *S, const double *Vf, const double *met, const double *bg,
                     const double *aE, double *aVf, double *aHG, double *aHMF, double *aSS);

@@ -55,20 +55,61 @@ void f_gas2(double g, double s, const double *W, const double *T, double *y)
 { for (int v = 0; v < 10; v++) y[v] = 0; gas_kin(1.0 / (g - 1.0), s, W, T, y, y + 5); }
 void a_gas2(double g, double s, const double *W, const double *T, const double *a, double *ax)
 { for (int v = 0; v < 5; v++) ax[v] = 0; gas_kin_adj(1.0 / (g - 1.0), s, W, T, a, ax, a + 5); }
-void f_mag(double s, const double *x, const double *T, double *y)
-{   // x = W[5], B[3], Bn; y = mF[3], VaD
-    MParm m = mp(5.0 / 3.0, 1e-12, 1e-12, 1);
-    double B0[3] = {0, 0, 0}, bbD = 0.0, VaD = 0.0;
+void f_mag(int bg, int bo, double s, const double *x, const double *T, double *y)
+{   // x = W[5], B[3], Bn; y = mF[3], VaD, bbD
+    MParm m = mp(5.0 / 3.0, 1e-12, 1e-12, 1); m.iCa2 = 1.0 / 4.0;
+    double B0[3] = {0.3, -0.2, 0.5}, bbD = 0.0, VaD = 0.0;
     for (int v = 0; v < 3; v++) y[v] = 0;
-    mag_kin<false, false>(m, s, x, x + 5, x[8], T, B0, 0.0, y, bbD, VaD);
-    y[3] = VaD;
+    if (bg && bo) mag_kin<true, true>(m, s, x, x + 5, x[8], T, B0, 0.4, y, bbD, VaD);
+    else if (bg) mag_kin<true, false>(m, s, x, x + 5, x[8], T, B0, 0.4, y, bbD, VaD);
+    else if (bo) mag_kin<false, true>(m, s, x, x + 5, x[8], T, B0, 0.4, y, bbD, VaD);
+    else mag_kin<false, false>(m, s, x, x + 5, x[8], T, B0, 0.4, y, bbD, VaD);
+    y[3] = VaD; y[4] = bbD;
 }
-void a_mag(double s, const double *x, const double *T, const double *a, double *ax)
+void a_mag(int bg, int bo, double s, const double *x, const double *T, const double *a, double *ax)
 {
+    MParm m = mp(5.0 / 3.0, 1e-12, 1e-12, 1); m.iCa2 = 1.0 / 4.0;
+    double B0[3] = {0.3, -0.2, 0.5};
     for (int v = 0; v < 9; v++) ax[v] = 0;
     double aBn = 0.0;
-    mag_kin_adj(s, x, x + 5, x[8], T, a, a[3], ax, ax + 5, aBn);
+    if (bg && bo) mag_kin_adj<true, true>(m, s, x, x + 5, x[8], T, B0, 0.4, a, a[3], a[4], ax, ax + 5, aBn);
+    else if (bg) mag_kin_adj<true, false>(m, s, x, x + 5, x[8], T, B0, 0.4, a, a[3], a[4], ax, ax + 5, aBn);
+    else if (bo) mag_kin_adj<false, true>(m, s, x, x + 5, x[8], T, B0, 0.4, a, a[3], a[4], ax, ax + 5, aBn);
+    else mag_kin_adj<false, false>(m, s, x, x + 5, x[8], T, B0, 0.4, a, a[3], a[4], ax, ax + 5, aBn);
     ax[8] = aBn;
+}
+void f_boris(double dtM, double dt, const double *x, double *y)
+{   // x = oU[5], dH[5], dM[3], B[3], oB[3] (19): the Boris branch of k_update
+    MParm m = mp(5.0 / 3.0, 1e-12, 1e-12, 1); m.iCa2 = 1.0 / 4.0;
+    const double *oU = x, *dH = x + 5, *dM = x + 10, *B = x + 13, *oB = x + 16;
+    double U[5], W2[5];
+    for (int v = 0; v < 5; v++) U[v] = oU[v] + dtM * dH[v];
+    c2p_cell(m, U, W2);
+    const double D2 = dmax(W2[DEN], m.dFloor), P2 = dmax(W2[PRS], m.pFloor), D0 = dmax(oU[DEN], m.dFloor);
+    const double iD0 = 1.0 / D0, iD2 = 1.0 / D2;
+    double V0[3] = {oU[1] * iD0, oU[2] * iD0, oU[3] * iD0};
+    double B1[3] = {0.5 * (B[0] + oB[0]), 0.5 * (B[1] + oB[1]), 0.5 * (B[2] + oB[2])};
+    const double alpha = (B1[0] * B1[0] + B1[1] * B1[1] + B1[2] * B1[2]) * (iD2 * m.iCa2);
+    const double dRho = dt * dH[DEN];
+    double dPG[3] = {dt * dH[1], dt * dH[2], dt * dH[3]}, dPM[3] = {dt * dM[0], dt * dM[1], dt * dM[2]};
+    const double pCon = 1.0 / (1.0 + alpha);
+    double dMom[3];
+    const double nb = sqrt(B1[0] * B1[0] + B1[1] * B1[1] + B1[2] * B1[2]);
+    double bh[3] = {B1[0] / nb, B1[1] / nb, B1[2] / nb};
+    double sGM[3] = {dPG[0] + dPM[0], dPG[1] + dPM[1], dPG[2] + dPM[2]};
+    const double pG = dPG[0] * bh[0] + dPG[1] * bh[1] + dPG[2] * bh[2];
+    const double pS = sGM[0] * bh[0] + sGM[1] * bh[1] + sGM[2] * bh[2];
+    const double pV = V0[0] * bh[0] + V0[1] * bh[1] + V0[2] * bh[2];
+    for (int x = 0; x < 3; x++)
+        dMom[x] = pG * bh[x] + pCon * (sGM[x] - pS * bh[x]) + alpha * pCon * dRho * (V0[x] - pV * bh[x]);
+    double Wn[5] = {D2, (D0 * V0[0] + dMom[0]) * iD2, (D0 * V0[1] + dMom[1]) * iD2, (D0 * V0[2] + dMom[2]) * iD2, P2};
+    p2c_cell(m, Wn, y);
+}
+void a_boris(double dtM, double dt, const double *x, const double *a, double *ax)
+{
+    MParm m = mp(5.0 / 3.0, 1e-12, 1e-12, 1); m.iCa2 = 1.0 / 4.0;
+    for (int v = 0; v < 19; v++) ax[v] = 0;
+    boris_adj(m, x, x + 5, x + 10, x + 13, x + 16, dtM, dt, a, ax, ax + 5, ax + 10, ax + 13, ax + 16);
 }
 static Geo cb_geo() { Geo g = {}; g.P = 4; g.PP = 16; g.N3 = 64; return g; }
 void f_cellb(const double *met, const double *f, double *y)
@@ -288,16 +329,31 @@ def test_gas_flux_with_hogs_difference_adjoint(lib, s):
 
 
 @pytest.mark.parametrize("s", [-1.0, 1.0])
-def test_magnetic_flux_adjoint(lib, s):
-    rng = np.random.default_rng(23)
+@pytest.mark.parametrize("bg,bo", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_magnetic_flux_adjoint(lib, s, bg, bo):
+    rng = np.random.default_rng(23 + bg + 2 * bo)
     T = _triad(rng)
 
     def x0(r):
         return np.concatenate([[r.uniform(0.3, 2.0)], r.standard_normal(3) * 0.5, [r.uniform(0.1, 2.0)],
-                               r.standard_normal(3), [r.standard_normal()]])[[0, 1, 2, 3, 4, 5, 6, 7, 8]]
-    fwd = lambda x, y: lib.f_mag(C.c_double(s), _p(x), _p(T), _p(y))
-    adj = lambda x, a, ax: lib.a_mag(C.c_double(s), _p(x), _p(T), _p(a), _p(ax))
-    _check(fwd, adj, x0, 4, rng, 1e-6, True)
+                               r.standard_normal(3), [r.standard_normal()]])
+    fwd = lambda x, y: lib.f_mag(bg, bo, C.c_double(s), _p(x), _p(T), _p(y))
+    adj = lambda x, a, ax: lib.a_mag(bg, bo, C.c_double(s), _p(x), _p(T), _p(a), _p(ax))
+    _check(fwd, adj, x0, 5, rng, 1e-6, True)
+
+
+def test_boris_update_adjoint(lib):
+    rng = np.random.default_rng(31)
+
+    def x0(r):
+        d = r.uniform(0.5, 2.0)
+        v = r.standard_normal(3) * 0.3
+        oU = [d, d * v[0], d * v[1], d * v[2], 0.5 * d * v @ v + r.uniform(0.5, 2.0) * 1.5]
+        return np.concatenate([oU, r.standard_normal(5) * 0.1, r.standard_normal(3) * 0.2,
+                               r.standard_normal(3), r.standard_normal(3)])
+    fwd = lambda x, y: lib.f_boris(C.c_double(0.05), C.c_double(0.05), _p(x), _p(y))
+    adj = lambda x, a, ax: lib.a_boris(C.c_double(0.05), C.c_double(0.05), _p(x), _p(a), _p(ax))
+    _check(fwd, adj, x0, 5, rng, 1e-6, True)
 
 
 def test_cell_b_adjoint(lib):

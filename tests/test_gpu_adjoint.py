@@ -211,6 +211,27 @@ def test_adjoint_mhd_box_matches_finite_differences(rmeth):
     g.close()
 
 
+@pytest.mark.parametrize("rmeth", ["7UP"])
+def test_adjoint_mhd_boris_background_matches_finite_differences(rmeth):
+    """MHD with Boris correction (Ca = 2, so the Boris factor matters) and a
+    dipole background field on a periodic box off the origin: the B0 cross
+    terms, the Boris vA, mag-HOGS and the CellBoris update in the adjoint."""
+    N = (16, 16, 16)
+    p = pb.box_params(rmeth)
+    p.update(doBoris=1, Ca=2.0, doBackground=1)
+    g = gamhip.GamHip(*N, p)
+    xyz = pb.cartesian_corners(*N, lo=(1.0, 1.0, 1.0), hi=(2.0, 2.0, 2.0))
+    g.grid_from_corners(xyz)
+    g.add_dipole_b0(-1.0)
+    g.put_state(pb.periodic_box_state(*N, g.get_grid(fields=("xyzcc",))["xyzcc"], xyz))
+    g.init_run()
+    _generic(g, 13)
+    g.run(2)
+    assert _fd_check(g, 6, 4, 6, 505, "boris+B0 " + rmeth) <= TOL
+    _fd_check_flux(g, 6, 4, 4, 606, "boris+B0 " + rmeth)
+    g.close()
+
+
 def test_torch_autograd_bridge():
     """StepObjective (torch.autograd.Function over the C ABI): the tensor
     gradient is the adjoint of State plus oState at the start (both begin
