@@ -307,6 +307,108 @@ __global__ void k_bc_periodic_adj(Geo g, Box t, double *__restrict__ aS)
     }
 }
 
+// Transpose of the zero-gradient / sphere BC copies with the magnetic
+// face fluxes (k_bc_copy<SIDE, KIND>, KIND 1 zero gradient, 2 sphere in,
+// 3 sphere out).  One thread per transverse point, all four layers.
+template <int SIDE, int KIND>
+__global__ void k_bc_copy_adj(Geo g, Box t, double *__restrict__ aS, const double *__restrict__ met)
+{
+    const int a = t.i0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int b = t.j0 + (int)blockIdx.y;
+    if (a > t.i1 || b > t.j1) return;
+    const long N3 = g.N3;
+    double *mf = aS + S_MF * N3;
+    auto cells = [&](long d, long s, bool withB) {
+        for (int v = 4; v >= 0; v--) move_adj(aS + v * N3, d, s);
+        if (withB)
+            for (int x = 2; x >= 0; x--) move_adj(aS + (S_B + x) * N3, d, s);
+    };
+    if (SIDE == 0) {
+        const long s = IX(g, g.is, a, b);
+        for (int n = 4; n >= 1; n--) {
+            const long d = IX(g, g.is - n, a, b);
+            if (KIND == 2) for (int x = 2; x >= 0; x--) move_adj(aS + (S_B + x) * N3, d, s);
+            for (int x = 2; x >= 0; x--) move_adj(mf + x * N3, d, s);
+            for (int v = 4; v >= 0; v--) move_adj(aS + v * N3, d, s);
+        }
+    } else if (SIDE == 1) {
+        const long s = IX(g, g.ie, a, b);
+        if (KIND == 3) {
+            for (int n = 4; n >= 1; n--) {
+                const int ig = g.ie + n;
+                const long d = IX(g, ig, a, b);
+                for (int x = 2; x >= 0; x--) move_adj(aS + (S_B + x) * N3, d, s);
+                move_adj(mf + 2 * N3, d, s);
+                move_adj(mf + N3, d, s);
+                move_adj(mf, IX(g, ig + 1, a, b), s);
+                for (int v = 4; v >= 0; v--) move_adj(aS + v * N3, d, s);
+            }
+        } else {
+            // ghost fluxes = face area x (B . normal), B = cell B of the last
+            // active cell from its own six face fluxes
+            double aB[3] = {0.0, 0.0, 0.0};
+            for (int n = 4; n >= 1; n--) {
+                const int ig = g.ie + n;
+                const long d = IX(g, ig, a, b);
+                const long fi[3] = {IX(g, ig + 1, a, b), d, d};
+                for (int x = 2; x >= 0; x--) {
+                    const long f = fi[x];
+                    const double am = mf[x * N3 + f];
+                    mf[x * N3 + f] = 0.0;
+                    const double A = met[(M_FACE + x) * N3 + f];
+                    for (int y = 0; y < 3; y++) aB[y] += A * met[(M_TF + 9 * x + y) * N3 + f] * am;
+                }
+                for (int v = 4; v >= 0; v--) move_adj(aS + v * N3, d, s);
+            }
+            // the J and K faces of the last cell are shared with the
+            // neighbouring transverse threads: atomic adds
+            double af[6] = {0, 0, 0, 0, 0, 0};
+            cell_b_adj(g, met, s, aB, af);
+            AADD(mf + s, af[0]);
+            AADD(mf + N3 + s, af[1]);
+            AADD(mf + 2 * N3 + s, af[2]);
+            AADD(mf + s + 1, af[3]);
+            AADD(mf + N3 + s + g.P, af[4]);
+            AADD(mf + 2 * N3 + s + g.PP, af[5]);
+        }
+    } else if (SIDE == 2) {
+        const long s = IX(g, a, g.js, b);
+        for (int n = 4; n >= 1; n--) {
+            const long d = IX(g, a, g.js - n, b);
+            for (int x = 2; x >= 0; x--) move_adj(mf + x * N3, d, s);
+            cells(d, s, false);
+        }
+    } else if (SIDE == 3) {
+        const long s = IX(g, a, g.je, b);
+        for (int n = 4; n >= 1; n--) {
+            const long d = IX(g, a, g.je + n, b);
+            move_adj(mf + N3, IX(g, a, g.je + n + 1, b), IX(g, a, g.je + 1, b));
+            move_adj(mf + 2 * N3, d, s);
+            move_adj(mf, d, s);
+            cells(d, s, false);
+        }
+    } else if (SIDE == 4) {
+        const long s = IX(g, a, b, g.ks);
+        for (int n = 4; n >= 1; n--) {
+            const long d = IX(g, a, b, g.ks - n);
+            for (int x = 2; x >= 0; x--) move_adj(mf + x * N3, d, s);
+            cells(d, s, false);
+        }
+    } else {
+        const long s = IX(g, a, b, g.ke);
+        for (int n = 4; n >= 1; n--) {
+            const long d = IX(g, a, b, g.ke + n);
+            // the k+1 ghost face takes the J flux (bcs.F90:925)
+            const long fk = IX(g, a, b, g.ke + n + 1), fj = IX(g, a, b, g.ke + 1);
+            mf[N3 + fj] += mf[2 * N3 + fk];
+            mf[2 * N3 + fk] = 0.0;
+            move_adj(mf + N3, d, s);
+            move_adj(mf, d, s);
+            cells(d, s, false);
+        }
+    }
+}
+
 // Update transpose, part 1 (cells): gas of the Reynolds + Maxwell/Boris
 // update (active cells) or the copy (other cells).  Writes adH (adjoint of
 // the gas flux divergence) and adM (of the magnetic stress divergence);
@@ -793,6 +895,32 @@ static void bcp_adj_side(hipStream_t s, const Geo &g, double *aS)
     t.k0 = t.k1 = 0;
     const dim3 gr((unsigned)((t.i1 - t.i0 + 64) / 64), (unsigned)(t.j1 - t.j0 + 1));
     hipLaunchKernelGGL(k_bc_periodic_adj<SIDE>, gr, dim3(64), 0, s, g, t, aS);
+}
+
+template <int SIDE, int KIND>
+static void bcc_adj_side(hipStream_t s, const Geo &g, double *aS, const double *met)
+{
+    Box t;
+    if (SIDE <= 1) { t.i0 = g.jsg; t.i1 = g.jeg; t.j0 = g.ksg; t.j1 = g.keg; }
+    else if (SIDE <= 3) { t.i0 = g.isg; t.i1 = g.ieg; t.j0 = g.ksg; t.j1 = g.keg; }
+    else { t.i0 = g.isg; t.i1 = g.ieg; t.j0 = g.jsg; t.j1 = g.jeg; }
+    t.k0 = t.k1 = 0;
+    const dim3 gr((unsigned)((t.i1 - t.i0 + 64) / 64), (unsigned)(t.j1 - t.j0 + 1));
+    hipLaunchKernelGGL((k_bc_copy_adj<SIDE, KIND>), gr, dim3(64), 0, s, g, t, aS, met);
+}
+
+// transpose of one side's BC with magnetic fields; kind as gam_model.bc
+void launch_bc_mhd_adj(hipStream_t s, int side, int kind, const Geo &g, double *aS, const double *met)
+{
+    if (kind == 0) { launch_bc_periodic_adj(s, side, g, aS); return; }
+    switch (side) {
+    case 0: if (kind == 2) bcc_adj_side<0, 2>(s, g, aS, met); else bcc_adj_side<0, 1>(s, g, aS, met); break;
+    case 1: if (kind == 3) bcc_adj_side<1, 3>(s, g, aS, met); else bcc_adj_side<1, 1>(s, g, aS, met); break;
+    case 2: bcc_adj_side<2, 1>(s, g, aS, met); break;
+    case 3: bcc_adj_side<3, 1>(s, g, aS, met); break;
+    case 4: bcc_adj_side<4, 1>(s, g, aS, met); break;
+    default: bcc_adj_side<5, 1>(s, g, aS, met); break;
+    }
 }
 
 void launch_bc_periodic_adj(hipStream_t s, int side, const Geo &g, double *aS)

@@ -1241,11 +1241,8 @@ extern "C" int gam_adj_create(gam_adj **out, gam_ctx *c, int nsteps, int K)
     if (rc) return rc;
     if (!c->mp.doMHD && (c->mp.doBoris || c->mp.doBG))
         return fail(GAM_E_ARG, "gam_adj_create: Boris correction and background field need MHD");
-    for (int sd = 0; sd < 6; sd++) {
+    for (int sd = 0; sd < 6; sd++)
         if (c->model.bc[sd] == GAM_BC_WIND) return fail(GAM_E_ARG, "gam_adj_create: the inflow BC has no adjoint yet");
-        if (c->mp.doMHD && c->model.bc[sd] != GAM_BC_PERIODIC)
-            return fail(GAM_E_ARG, "gam_adj_create: with MHD the adjoint covers periodic boundaries only");
-    }
     if (nsteps < 1 || K < 1 || K > nsteps) return fail(GAM_E_ARG, "gam_adj_create: need 1 <= ckpt_every <= nsteps");
     gam_adj *a = (gam_adj *)calloc(1, sizeof(gam_adj));
     if (!a) return fail(GAM_E_NOMEM, "gam_adj_create: host allocation failed");
@@ -1377,7 +1374,7 @@ static void adj_bcs(gam_adj *a, double *l)
 {
     gam_ctx *c = a->c;
     for (int sd = 5; sd >= 0; sd--) {
-        if (c->mp.doMHD) launch_bc_periodic_adj(c->stream, sd, c->g, l);
+        if (c->mp.doMHD) launch_bc_mhd_adj(c->stream, sd, c->model.bc[sd], c->g, l, c->met);
         else launch_bc_adj(c->stream, sd, c->model.bc[sd] == GAM_BC_PERIODIC ? 0 : 1, c->g, l);
     }
 }

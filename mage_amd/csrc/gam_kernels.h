@@ -122,6 +122,7 @@ void launch_predict_adj(hipStream_t s, const Geo &g, const MParm &m, const Clock
 void launch_set_dt(hipStream_t s, Clock *clk, const Clock *rec);
 void launch_cellb_adj(hipStream_t s, const Geo &g, int all, double *aS, const double *met);
 void launch_bc_periodic_adj(hipStream_t s, int side, const Geo &g, double *aS);
+void launch_bc_mhd_adj(hipStream_t s, int side, int kind, const Geo &g, double *aS, const double *met);
 void launch_update_adj_mhd(hipStream_t s, const Geo &g, const MParm &m, const Box &ab, const Box &ctb,
                            const Box *eboxes, const Clock *clk, const double *So, const double *Sn1,
                            const double *gflx, const double *mflx, const double *met, const double *bg,
