@@ -9,13 +9,16 @@
  * the forward run, GasKinFlux; stress.F90:204-511, recon.F90:125-301) and
  * the AB2 predictor (mhdgroup.F90:444-561).
  *
- * Scope: every step option except the inflow boundary: hydrodynamic runs
- * (doMHD = 0, no Boris, no background) with periodic, zero-gradient or
- * sphere boundaries; MHD runs (Maxwell or Boris update, with or without
- * the background field) with periodic boundaries (GAM_E_ARG otherwise).
- * With MHD the adjoint runs through the edge EMFs, constrained transport,
- * the magnetic kinetic fluxes with HOGS (and the Boris mag-HOGS and nuke
- * terms), the background cross terms and CellMaxwell / CellBoris.
+ * Scope: every step option: hydrodynamic and MHD runs (Maxwell or Boris
+ * update, with or without the background field) with periodic,
+ * zero-gradient, sphere and inflow (wind) boundaries.  With MHD the adjoint
+ * runs through the edge EMFs, constrained transport, the magnetic kinetic
+ * fluxes with HOGS (and the Boris mag-HOGS and nuke terms), the background
+ * cross terms and CellMaxwell / CellBoris.  The inflow boundary's adjoint
+ * (wind.F90:226-335) covers the ghost-gas blend with the fast-speed
+ * outflow fix, the blended ghost face fluxes and the blend weight's
+ * dependence on the wind direction, and yields the gradient with respect
+ * to the inflow time series (gam_adj_wind_grad).
  *
  * Objective: J = sum over cells of w(c) * P(c) of the State after nsteps
  * (P the CellC2P pressure), w a weight array (Ni,Nj,Nk) given by the
@@ -61,12 +64,20 @@ int gam_adj_forward(gam_adj *a, double *J);
 int gam_adj_backward(gam_adj *a, double *gGas, double *gGasO, double *gMF, double *gMFO);
 /* Forward from the recorded start with the given State Gas (and oState
  * Gas, State and oState magFlux; NULL keeps the recorded array), ghost
- * cells refilled by the boundary conditions and cell B recomputed from the
- * face fluxes, each step's dt held at the recorded value; *J = objective
+ * cells refilled by the boundary conditions (the oState's at oState%time,
+ * which matters for the time-dependent inflow) and cell B recomputed from
+ * the face fluxes, each step's dt held at the recorded value; *J = objective
  * at the end.  The context is then put back at the final state of the
  * recorded run. */
 int gam_adj_replay(gam_adj *a, const double *Gas, const double *oGas, const double *MF,
                    const double *oMF, double *J);
+/* After gam_adj_backward with an inflow BC: gQ(n,5) = dJ/dQ and gB(n,3) =
+ * dJ/dB of the series given to gam_set_wind (column-major, the same
+ * layout; the Bx column is 0: Bx = Bx0 + ByC*By + BzC*Bz).  Each sample
+ * the boundary reads at a time counts; the sample index itself is a step
+ * function of time and is not differentiated.  The series must be the one
+ * the forward run used.  GAM_E_STATE without an inflow BC. */
+int gam_adj_wind_grad(gam_adj *a, double *gQ, double *gB);
 /* Milliseconds of the last gam_adj_forward and gam_adj_backward calls
  * (HIP events on the context's stream). */
 int gam_adj_times(gam_adj *a, double *fwd_ms, double *bwd_ms);

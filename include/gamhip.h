@@ -230,7 +230,10 @@ typedef struct gam_ens gam_ens;
  * owned by the caller; state transfers (gam_get_state, ...) on a member
  * show the ensemble's progress.  A member may also be stepped on its own
  * between ensemble runs only an even number of times (the recorded graphs
- * fix which of its two state buffers is current; GAM_E_STATE otherwise). */
+ * fix which of its two state buffers is current; GAM_E_STATE otherwise).
+ * The recorded launches keep each member's grid, background, inflow series
+ * and launch plans as they were at creation: create the ensemble again
+ * after changing any of them. */
 int gam_ens_create(gam_ens **out, int n, gam_ctx *const *members);
 /* nsteps x stepGamera of every member; synchronises at the end and returns
  * the first member's CalcDT failure code, if any (gam_last_error names the

@@ -57,6 +57,58 @@ __device__ __forceinline__ void p2c_cell_adj(const MParm &m, const double *W, co
     if (W[PRS] > m.pFloor) aW[PRS] += m.igm1 * aU[ENER];
 }
 
+// wind_inner: (U, Bi) -> gi, the face normal nH held fixed
+__device__ __forceinline__ void wind_inner_adj(const MParm &m, const double *U, const double *Bi,
+                                               const double *nH, const double *agi, double *aU, double *aB)
+{
+    double W[5];
+    c2p_cell(m, U, W);
+    const double rho = dmax(U[DEN], m.dFloor);
+    const double vv[3] = {U[MOMX] / rho, U[MOMY] / rho, U[MOMZ] / rho};
+    const double v2 = vv[0] * vv[0] + vv[1] * vv[1] + vv[2] * vv[2];
+    const double ke = 0.5 * rho * v2;
+    const double pr = (m.gamma - 1) * (U[ENER] - ke);
+    const double P = dmax(pr, m.pFloor);
+    const double Cs = sqrt(m.gamma * P / rho);
+    const double bm = sqrt(Bi[0] * Bi[0] + Bi[1] * Bi[1] + Bi[2] * Bi[2]);
+    const double Va0 = bm / sqrt(U[DEN]);
+    const double Va = m.doBoris ? m.Ca * Va0 / sqrt(m.Ca * m.Ca + Va0 * Va0) : Va0;
+    const double Cx = sqrt(Cs * Cs + Va * Va);
+    const double vp = W[1] * nH[0] + W[2] * nH[1] + W[3] * nH[2];
+    double aW[5] = {agi[0], agi[1], agi[2], agi[3], agi[4]};
+    if (vp < 1.0 * Cx) {
+        // V = V0 + (Cx - V0 . nH) nH
+        const double aCx = agi[1] * nH[0] + agi[2] * nH[1] + agi[3] * nH[2];
+        for (int x = 0; x < 3; x++) aW[1 + x] -= aCx * nH[x];
+        const double aCs = aCx * Cs / Cx, aVa = aCx * Va / Cx;
+        // Cs = sqrt(gamma P / rho) with the Con2Fast rho, velocities, P
+        const double aP = aCs * 0.5 * Cs / P;
+        double arho = -aCs * 0.5 * Cs / rho;
+        if (pr > m.pFloor) {
+            aU[ENER] += (m.gamma - 1) * aP;
+            const double ake = -(m.gamma - 1) * aP;
+            arho += 0.5 * v2 * ake;
+            double air = 0.0;
+            for (int x = 0; x < 3; x++) {
+                const double av = rho * vv[x] * ake;
+                aU[1 + x] += av / rho;
+                air += U[1 + x] * av;
+            }
+            arho -= air / (rho * rho);
+        }
+        if (U[DEN] > m.dFloor) aU[DEN] += arho;
+        // Va = |B| / sqrt(rho), Boris-limited: Ca Va0 / sqrt(Ca^2 + Va0^2)
+        const double q2 = m.Ca * m.Ca + Va0 * Va0;
+        const double aVa0 = m.doBoris ? aVa * m.Ca * m.Ca * m.Ca / (q2 * sqrt(q2)) : aVa;
+        if (bm > 0.0) {
+            const double sr = sqrt(U[DEN]);
+            for (int x = 0; x < 3; x++) aB[x] += aVa0 * Bi[x] / (bm * sr);
+            aU[DEN] += -0.5 * aVa0 * Va0 / U[DEN];
+        }
+    }
+    c2p_cell_adj(m, U, aW, aU);
+}
+
 // BlockCon2Prim (c2p_block): U -> W
 __device__ __forceinline__ void c2p_block_adj(const MParm &m, const double *U, const double *aW,
                                               double *aU)

@@ -1046,3 +1046,202 @@ void launch_predict_adj_mhd(hipStream_t s, const Geo &g, const MParm &m, const C
                        aWN, aS, aO);
     hipLaunchKernelGGL(k_predict_mf_adj, agrid(fb), dim3(ABX, ABY), 0, s, g, fb, clk, aHMF, aS, aO);
 }
+
+// ============================================================ inflow BC
+#include "gam_wind.h"
+
+// Per-lane accumulators of the inflow-series adjoint: up to two sample
+// indices with seven columns each (rho, vx, vy, vz, P, By, Bz).
+struct WindAcc {
+    int ia, ib;
+    double a[7], b[7];
+};
+
+__device__ __forceinline__ void wacc_store(const WindDev &w, double *gQ, double *gB, int i0, const double *v)
+{
+    for (int q = 0; q < 5; q++)
+        if (v[q] != 0.0) AADD(gQ + i0 + (long)q * w.n, v[q]);
+    if (v[5] != 0.0) AADD(gB + i0 + (long)w.n, v[5]);
+    if (v[6] != 0.0) AADD(gB + i0 + 2L * w.n, v[6]);
+}
+
+__device__ __forceinline__ void wacc_add(WindAcc &A, const WindDev &w, double *gQ, double *gB, int i0,
+                                         const double *v)
+{
+    if (A.ia == i0 || A.ia < 0) {
+        A.ia = i0;
+        for (int q = 0; q < 7; q++) A.a[q] += v[q];
+    } else if (A.ib == i0 || A.ib < 0) {
+        A.ib = i0;
+        for (int q = 0; q < 7; q++) A.b[q] += v[q];
+    } else {
+        wacc_store(w, gQ, gB, A.ia, A.a);
+        A.ia = i0;
+        for (int q = 0; q < 7; q++) A.a[q] = v[q];
+    }
+}
+
+// Flush one accumulator slot of every lane of the (64-lane) block: one
+// reduction and one set of atomics when all lanes hold the same sample,
+// per-lane atomics otherwise.  Every lane of the wave must call it.
+__device__ __forceinline__ void wacc_flush(const WindDev &w, double *gQ, double *gB, int key, double *v)
+{
+    int lo = key < 0 ? 0x7fffffff : key, hi = key;
+    for (int o = 32; o >= 1; o >>= 1) {
+        lo = min(lo, __shfl_xor(lo, o, 64));
+        hi = max(hi, __shfl_xor(hi, o, 64));
+    }
+    if (hi < 0) return;
+    if (lo == hi) {
+        for (int q = 0; q < 7; q++) {
+            double s = key < 0 ? 0.0 : v[q];
+            for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
+            v[q] = s;
+        }
+        if (threadIdx.x == 0) wacc_store(w, gQ, gB, hi, v);
+    } else if (key >= 0) {
+        wacc_store(w, gQ, gB, key, v);
+    }
+}
+
+// Transpose of the inflow BC (k_bc_wind, wind.F90:226-335).  Ghost gas =
+// P2C(wSW * wind sample + (1 - wSW) * last active cell's primitives, the
+// latter with the outflow velocity raised to the fast speed); ghost face
+// flux = wSW * A (n . B_wind) + (1 - wSW) * the first inner face's flux.
+// Adds the adjoints of the last active cell (gas, and its cell B through
+// the cell's face fluxes), of the inner faces, and of the inflow series
+// samples (gQ (n,5), gB (n,3), column-major; the sample index is a step
+// function of time and carries no derivative).  One thread per transverse
+// point (j, k), all four layers; blocks of 64.
+__global__ __launch_bounds__(64) void k_bc_wind_adj(Geo g, MParm m, Box t, const Clock *clk, WindDev w,
+                                                    int mhd, const double *__restrict__ S,
+                                                    double *__restrict__ aS, const double *__restrict__ met,
+                                                    double *__restrict__ gQ, double *__restrict__ gB)
+{
+    const int j = t.i0 + (int)(blockIdx.x * 64 + threadIdx.x);
+    const int k = t.j0 + (int)blockIdx.y;
+    WindAcc A;
+    A.ia = A.ib = -1;
+    for (int q = 0; q < 7; q++) A.a[q] = A.b[q] = 0.0;
+    if (j <= t.i1 && k <= t.j1) {
+        const long N3 = g.N3;
+        double *amf = aS + S_MF * N3;
+        const double *mf = S + S_MF * N3;
+        const double tm = clk->t;
+        double vF[3];
+        wind_front(w, tm, vF);
+        const int ip = g.ie;
+        const long cp = IX(g, ip, j, k);
+        const bool cell = (k <= g.keg && j <= g.jeg);
+        double U[5], Bi[3], aUp[5] = {0, 0, 0, 0, 0}, aBp[3] = {0, 0, 0};
+        for (int v = 0; v < 5; v++) U[v] = cell ? S[(S_GAS + v) * N3 + cp] : 0.0;
+        for (int x = 0; x < 3; x++) Bi[x] = cell ? S[(S_B + x) * N3 + cp] : 0.0;
+        for (int n = 4; n >= 1; n--) {
+            const int ig = g.ie + n;
+            // ghost face fluxes
+            const int fi[3] = {ig + 1, ig, ig};
+            for (int d = 2; mhd && d >= 0; d--) {
+                const long f = IX(g, fi[d], j, k);
+                const double ao = amf[d * N3 + f];
+                amf[d * N3 + f] = 0.0;
+                if (ao == 0.0) continue;
+                const double xf[3] = {met[(M_XFC + 0 + 3 * d) * N3 + f], met[(M_XFC + 1 + 3 * d) * N3 + f],
+                                      met[(M_XFC + 2 + 3 * d) * N3 + f]};
+                const double wSW = wind_wgt(w, vF, xf, tm);
+                const double tl = wind_lag_time(w, vF, xf, tm);
+                const int i0 = wind_index(w, tl);
+                double r, p, V[3], B[3];
+                wind_at(w, vF, xf, tm, &r, &p, V, B);
+                const double *nrm = met + (long)(M_TF + 9 * d) * N3 + f;
+                const double Af = met[(M_FACE + d) * N3 + f];
+                const double sw = Af * (nrm[0] * B[0] + nrm[N3] * B[1] + nrm[2 * N3] * B[2]);
+                const long fin = (d == 0) ? IX(g, ip + 1, j, k) : cp;
+                const double in = mf[d * N3 + fin];
+                AADD(amf + d * N3 + fin, (1.0 - wSW) * ao);
+                double v7[7] = {0, 0, 0, 0, 0, 0, 0};
+                if (tl > w.tMin) {
+                    const double aBn = wSW * ao * Af;
+                    const double aB[3] = {aBn * nrm[0], aBn * nrm[N3], aBn * nrm[2 * N3]};
+                    v7[5] = aB[0] * w.ByC + aB[1];
+                    v7[6] = aB[0] * w.BzC + aB[2];
+                }
+                double dw[3];
+                wind_wgt_dv(w, vF, xf, tm, dw);
+                const double awt = (sw - in) * ao;
+                for (int x = 0; x < 3; x++) v7[1 + x] += awt * dw[x];
+                wacc_add(A, w, gQ, gB, i0, v7);
+            }
+            if (!cell) continue;
+            // ghost gas
+            const long cg = IX(g, ig, j, k);
+            double aU[5];
+            bool any = false;
+            for (int v = 0; v < 5; v++) {
+                aU[v] = aS[(S_GAS + v) * N3 + cg];
+                aS[(S_GAS + v) * N3 + cg] = 0.0;
+                any |= aU[v] != 0.0;
+            }
+            if (!any) continue;
+            const double xc[3] = {met[(M_XYZCC + 0) * N3 + cg], met[(M_XYZCC + 1) * N3 + cg],
+                                  met[(M_XYZCC + 2) * N3 + cg]};
+            const double wSW = wind_wgt(w, vF, xc, tm), wMHD = 1.0 - wSW;
+            const int i0 = wind_index(w, wind_lag_time(w, vF, xc, tm));
+            double gs[5] = {0, 0, 0, 0, 0}, gi[5] = {0, 0, 0, 0, 0};
+            if (wSW > GAM_TINY) {
+                double r, p, V[3], B[3];
+                wind_at(w, vF, xc, tm, &r, &p, V, B);
+                gs[0] = r; gs[1] = V[0]; gs[2] = V[1]; gs[3] = V[2]; gs[4] = p;
+            }
+            double nH[3] = {0, 0, 0};
+            if (wMHD > GAM_TINY) {
+                const long fp = IX(g, ip + 1, j, k);
+                for (int x = 0; x < 3; x++) nH[x] = met[(M_TF + x) * N3 + fp];
+                wind_inner(m, U, Bi, nH, gi);
+            }
+            double gw[5], agw[5] = {0, 0, 0, 0, 0};
+            for (int v = 0; v < 5; v++) gw[v] = wSW * gs[v] + wMHD * gi[v];
+            p2c_cell_adj(m, gw, aU, agw);
+            // the series: the sample (when used) and the blend weight
+            double v7[7] = {0, 0, 0, 0, 0, 0, 0};
+            if (wSW > GAM_TINY)
+                for (int v = 0; v < 5; v++) v7[v] = wSW * agw[v];
+            double awt = 0.0;
+            for (int v = 0; v < 5; v++) awt += (gs[v] - gi[v]) * agw[v];
+            double dw[3];
+            wind_wgt_dv(w, vF, xc, tm, dw);
+            for (int x = 0; x < 3; x++) v7[1 + x] += awt * dw[x];
+            wacc_add(A, w, gQ, gB, i0, v7);
+            if (!(wMHD > GAM_TINY)) continue;
+            // the interior cell
+            double agi[5];
+            for (int v = 0; v < 5; v++) agi[v] = wMHD * agw[v];
+            wind_inner_adj(m, U, Bi, nH, agi, aUp, aBp);
+        }
+        if (cell) {
+            for (int v = 0; v < 5; v++) aS[(S_GAS + v) * N3 + cp] += aUp[v];
+            if (mhd && (aBp[0] != 0.0 || aBp[1] != 0.0 || aBp[2] != 0.0)) {
+                // the last active cell's B = its bFlux2Fld from its face fluxes
+                double af[6] = {0, 0, 0, 0, 0, 0};
+                cell_b_adj(g, met, cp, aBp, af);
+                AADD(amf + cp, af[0]);
+                AADD(amf + N3 + cp, af[1]);
+                AADD(amf + 2 * N3 + cp, af[2]);
+                AADD(amf + cp + 1, af[3]);
+                AADD(amf + N3 + cp + g.P, af[4]);
+                AADD(amf + 2 * N3 + cp + g.PP, af[5]);
+            }
+        }
+    }
+    wacc_flush(w, gQ, gB, A.ia, A.a);
+    wacc_flush(w, gQ, gB, A.ib, A.b);
+}
+
+void launch_bc_wind_adj(hipStream_t s, const Geo &g, const MParm &m, const Clock *clk, const WindDev &w,
+                        const double *S, double *aS, const double *met, double *gQ, double *gB)
+{
+    // transverse ranges of the forward (launch_bc, kind 4)
+    Box t;
+    t.i0 = g.jsg; t.i1 = g.jeg + 1; t.j0 = g.ksg; t.j1 = g.keg + 1; t.k0 = t.k1 = 0;
+    const dim3 gr((unsigned)((t.i1 - t.i0 + 64) / 64), (unsigned)(t.j1 - t.j0 + 1));
+    hipLaunchKernelGGL(k_bc_wind_adj, gr, dim3(64), 0, s, g, m, t, clk, w, m.doMHD ? 1 : 0, S, aS, met, gQ, gB);
+}

@@ -366,8 +366,10 @@ extern "C" int gam_set_wind(gam_ctx *c, int n, const double *tW, const double *Q
     if (fabs(dtMin - dtMax) > 1.0e-6 * sqrt(dtMin * dtMin + dtMax * dtMax))
         return fail(GAM_E_ARG, "gam_set_wind: wind time series must be uniformly spaced");
     HIPCHK(hipStreamSynchronize(c->stream));   // no pending step may still read the old series
-    if (c->wind) { hipFree(c->wind); c->wind = nullptr; }
-    HIPCHK(hipMalloc((void **)&c->wind, sizeof(double) * (size_t)n * 9));
+    // a series of the same length reuses the device buffer, so launches
+    // recorded earlier (ensembles) never hold a freed address
+    if (c->wind && c->wd.n != n) { hipFree(c->wind); c->wind = nullptr; }
+    if (!c->wind) HIPCHK(hipMalloc((void **)&c->wind, sizeof(double) * (size_t)n * 9));
     std::vector<double> h((size_t)n * 9);
     memcpy(h.data(), tW, sizeof(double) * n);
     memcpy(h.data() + n, Q, sizeof(double) * n * 5);
@@ -522,11 +524,13 @@ static void enq_bflux2fld(gam_ctx *c, double *S, bool ghosts_only = false)
 }
 
 // ghostB: whether BCs also set ghost-cell B (EnforceBCs on its own does;
-// inside stepGamera the bFlux2Fld that follows rewrites every ghost cell)
-static void enq_bcs(gam_ctx *c, double *S, bool ghostB = true)
+// inside stepGamera the bFlux2Fld that follows rewrites every ghost cell).
+// clk: the clock whose time the inflow BC reads (the context's by default).
+static void enq_bcs(gam_ctx *c, double *S, bool ghostB = true, const Clock *clk = nullptr)
 {
     for (int side = 0; side < 6; side++)
-        launch_bc(c->stream, side, c->model.bc[side], c->g, c->mp, c->clk, &c->wd, S, c->met, ghostB);
+        launch_bc(c->stream, side, c->model.bc[side], c->g, c->mp, clk ? clk : c->clk, &c->wd, S, c->met,
+                  ghostB);
 }
 
 // While a slot is being tuned, its launches are bracketed by events.
@@ -1208,6 +1212,7 @@ struct gam_adj {
     double *ck;              // nck x (State, oState), S_NPL planes each
     double *fin;             // State, oState at the end
     Clock *clkh;             // clock before step n, n = 0..nsteps
+    Clock *oclk;             // clock at the start with t = oState%time
     double *tape;            // K + 2 State buffers of one interval
     double *lam;             // three adjoint state sets (ADJ_SET planes each)
     double *ag, *amf;        // adjoints of gFlx (15) and mFlx (9)
@@ -1216,6 +1221,8 @@ struct gam_adj {
     double *avf, *aur, *adm, *ae;   // Vf (9), Reynolds gas (5), dM (3), E (3)
     double *wgt;             // objective weights (1 plane)
     double *part;            // objective partial sums (1 plane)
+    int nw;                  // inflow series length (0: no inflow BC)
+    double *gwq, *gwb;       // adjoints of the inflow series Q (nw,5), B (nw,3)
     hipEvent_t ev[4];
     float fwd_ms, bwd_ms;
 };
@@ -1225,9 +1232,10 @@ extern "C" int gam_adj_destroy(gam_adj *a)
     if (!a) return GAM_OK;
     if (a->c && a->c->stream) hipStreamSynchronize(a->c->stream);
     double *bufs[] = {a->ck, a->fin, a->tape, a->lam, a->ag, a->amf, a->awh, a->awn, a->ahg, a->ahmf,
-                      a->ahb, a->avf, a->aur, a->adm, a->ae, a->wgt, a->part};
+                      a->ahb, a->avf, a->aur, a->adm, a->ae, a->wgt, a->part, a->gwq, a->gwb};
     for (double *b : bufs) if (b) hipFree(b);
     if (a->clkh) hipFree(a->clkh);
+    if (a->oclk) hipFree(a->oclk);
     for (int q = 0; q < 4; q++) if (a->ev[q]) hipEventDestroy(a->ev[q]);
     free(a);
     return GAM_OK;
@@ -1241,8 +1249,6 @@ extern "C" int gam_adj_create(gam_adj **out, gam_ctx *c, int nsteps, int K)
     if (rc) return rc;
     if (!c->mp.doMHD && (c->mp.doBoris || c->mp.doBG))
         return fail(GAM_E_ARG, "gam_adj_create: Boris correction and background field need MHD");
-    for (int sd = 0; sd < 6; sd++)
-        if (c->model.bc[sd] == GAM_BC_WIND) return fail(GAM_E_ARG, "gam_adj_create: the inflow BC has no adjoint yet");
     if (nsteps < 1 || K < 1 || K > nsteps) return fail(GAM_E_ARG, "gam_adj_create: need 1 <= ckpt_every <= nsteps");
     gam_adj *a = (gam_adj *)calloc(1, sizeof(gam_adj));
     if (!a) return fail(GAM_E_NOMEM, "gam_adj_create: host allocation failed");
@@ -1256,6 +1262,7 @@ extern "C" int gam_adj_create(gam_adj **out, gam_ctx *c, int nsteps, int K)
     AALLOC(a->ck, (size_t)a->nck * 2 * S_NPL * pl);
     AALLOC(a->fin, 2 * S_NPL * pl);
     AALLOC(a->clkh, (size_t)(nsteps + 1) * sizeof(Clock));
+    AALLOC(a->oclk, sizeof(Clock));
     AALLOC(a->tape, (size_t)(K + 2) * S_NPL * pl);
     AALLOC(a->lam, 3 * ADJ_SET * pl);
     AALLOC(a->ag, 15 * pl);
@@ -1271,6 +1278,12 @@ extern "C" int gam_adj_create(gam_adj **out, gam_ctx *c, int nsteps, int K)
     AALLOC(a->ae, 3 * pl);
     AALLOC(a->wgt, pl);
     AALLOC(a->part, pl);
+    for (int sd = 0; sd < 6; sd++)
+        if (c->model.bc[sd] == GAM_BC_WIND) a->nw = c->wd.n;
+    if (a->nw) {
+        AALLOC(a->gwq, (size_t)a->nw * 5 * sizeof(double));
+        AALLOC(a->gwb, (size_t)a->nw * 3 * sizeof(double));
+    }
 #undef AALLOC
     if (e == hipSuccess) e = hipMemset(a->lam, 0, 3 * ADJ_SET * pl);
     if (e == hipSuccess) e = hipMemset(a->ag, 0, 15 * pl);
@@ -1354,7 +1367,11 @@ extern "C" int gam_adj_forward(gam_adj *a, double *J)
     int rc = adj_objective(a, S_cur(c), a->lam, J);
     if (rc) return rc;
     HIPCHK(hipEventElapsedTime(&a->fwd_ms, a->ev[0], a->ev[1]));
+    // the start oState's boundary values are those of its own time
     Clock ck;
+    HIPCHK(hipMemcpy(&ck, a->clkh, sizeof(Clock), hipMemcpyDeviceToHost));
+    ck.t = ck.ost;
+    HIPCHK(hipMemcpy(a->oclk, &ck, sizeof(Clock), hipMemcpyHostToDevice));
     if ((rc = read_clock(c, &ck))) return rc;
     a->recorded = 1;
     return status_code(ck.status);
@@ -1369,12 +1386,16 @@ static int adj_restore_final(gam_adj *a)
     return GAM_OK;
 }
 
-// Transposes of the boundary pass on one adjoint set, sides OUTK to INI.
-static void adj_bcs(gam_adj *a, double *l)
+// Transposes of the boundary pass on one adjoint set, sides OUTK to INI;
+// S = the state the pass ran on (its active cells and faces), bclk = the
+// clock it ran at.
+static void adj_bcs(gam_adj *a, double *l, const double *S, const Clock *bclk)
 {
     gam_ctx *c = a->c;
     for (int sd = 5; sd >= 0; sd--) {
-        if (c->mp.doMHD) launch_bc_mhd_adj(c->stream, sd, c->model.bc[sd], c->g, l, c->met);
+        if (c->model.bc[sd] == GAM_BC_WIND)
+            launch_bc_wind_adj(c->stream, c->g, c->mp, bclk, c->wd, S, l, c->met, a->gwq, a->gwb);
+        else if (c->mp.doMHD) launch_bc_mhd_adj(c->stream, sd, c->model.bc[sd], c->g, l, c->met);
         else launch_bc_adj(c->stream, sd, c->model.bc[sd] == GAM_BC_PERIODIC ? 0 : 1, c->g, l);
     }
 }
@@ -1382,9 +1403,9 @@ static void adj_bcs(gam_adj *a, double *l)
 // One step's adjoint: lA (wrt the step's output State) and lB (wrt its
 // output oState = input State) in, lB (input State) and lC (input oState)
 // out.  Sn, On = the step's input State and oState, Sn1 its output State;
-// clk its clock.
+// clk its clock, bclk the clock after it (its boundary pass's time).
 static void adj_step(gam_adj *a, const double *Sn, const double *On, const double *Sn1, const Clock *clk,
-                     double *lA, double *lB, double *lC)
+                     const Clock *bclk, double *lA, double *lB, double *lC)
 {
     gam_ctx *c = a->c;
     const Geo &g = c->g;
@@ -1404,7 +1425,7 @@ static void adj_step(gam_adj *a, const double *Sn, const double *On, const doubl
         }
         // ghost bFlux2Fld, boundary pass, update (with its cell B)
         launch_cellb_adj(s, g, 0, lA, c->met);
-        adj_bcs(a, lA);
+        adj_bcs(a, lA, Sn1, bclk);
         const Box ctb = mkbox(g.is, g.ie + 1, g.js, g.je + 1, g.ks, g.ke + 1);
         launch_update_adj_mhd(s, g, c->mp, act_box(g), ctb, eb, clk, Sn, Sn1, c->gflx, c->mflx, c->met, c->bg,
                               lA, lB, a->aur, a->adm, a->ag, a->amf, a->ae);
@@ -1427,7 +1448,7 @@ static void adj_step(gam_adj *a, const double *Sn, const double *On, const doubl
         hipMemsetAsync(lC + S_B * N3, 0, 3 * pl, s);
         launch_predict_adj_mhd(s, g, c->mp, clk, Sn, On, c->met, a->ahg, a->awh, a->awn, a->ahb, a->ahmf, lB, lC);
     } else {
-        adj_bcs(a, lA);
+        adj_bcs(a, lA, Sn1, bclk);
         launch_update_adj(s, g, act_box(g), clk, lA, lB, a->ag, c->met);
         hipMemsetAsync(a->awh, 0, 5 * pl, s);
         hipMemsetAsync(a->awn, 0, 5 * pl, s);
@@ -1450,9 +1471,15 @@ extern "C" int gam_adj_backward(gam_adj *a, double *gGas, double *gGasO, double 
     const size_t bytes = (size_t)S_NPL * N3 * sizeof(double);
     hipStream_t s = c->stream;
     double *lA = a->lam, *lB = a->lam + ADJ_SET * N3, *lC = a->lam + 2 * ADJ_SET * N3;
+    if (a->nw && a->nw != c->wd.n)
+        return fail(GAM_E_STATE, "gam_adj_backward: the inflow series changed length since gam_adj_create");
     HIPCHK(hipStreamSynchronize(s));
     HIPCHK(hipEventRecord(a->ev[2], s));
     HIPCHK(hipMemsetAsync(a->lam, 0, 3 * ADJ_SET * N3 * sizeof(double), s));
+    if (a->nw) {
+        HIPCHK(hipMemsetAsync(a->gwq, 0, (size_t)a->nw * 5 * sizeof(double), s));
+        HIPCHK(hipMemsetAsync(a->gwb, 0, (size_t)a->nw * 3 * sizeof(double), s));
+    }
     int rc = adj_objective(a, a->fin, lA, nullptr);
     if (rc) return rc;
     const int prof = c->prof;
@@ -1471,7 +1498,7 @@ extern "C" int gam_adj_backward(gam_adj *a, double *gGas, double *gGasO, double 
         for (int n = s1 - 1; n >= s0; n--) {
             const int q = n - s0;
             adj_step(a, a->tape + (size_t)(1 + q) * S_NPL * N3, a->tape + (size_t)q * S_NPL * N3,
-                     a->tape + (size_t)(2 + q) * S_NPL * N3, a->clkh + n, lA, lB, lC);
+                     a->tape + (size_t)(2 + q) * S_NPL * N3, a->clkh + n, a->clkh + n + 1, lA, lB, lC);
             double *t = lA;
             lA = lB;
             lB = lC;
@@ -1487,8 +1514,8 @@ extern "C" int gam_adj_backward(gam_adj *a, double *gGas, double *gGasO, double 
         launch_cellb_adj(s, g, 1, lA, c->met);
         launch_cellb_adj(s, g, 1, lB, c->met);
     }
-    adj_bcs(a, lA);
-    adj_bcs(a, lB);
+    adj_bcs(a, lA, a->ck, a->clkh);
+    adj_bcs(a, lB, a->ck + (size_t)S_NPL * N3, a->oclk);
     HIPCHK(hipEventRecord(a->ev[3], s));
     HIPCHK(hipEventSynchronize(a->ev[3]));
     HIPCHK(hipEventElapsedTime(&a->bwd_ms, a->ev[2], a->ev[3]));
@@ -1512,8 +1539,12 @@ extern "C" int gam_adj_replay(gam_adj *a, const double *Gas, const double *oGas,
     if (oGas && (rc = upload(c, oGas, 0, 5, S_old(c) + S_GAS * N3))) return rc;
     if (MF && (rc = upload(c, MF, 1, 3, S_cur(c) + S_MF * N3))) return rc;
     if (oMF && (rc = upload(c, oMF, 1, 3, S_old(c) + S_MF * N3))) return rc;
-    enq_bcs(c, S_cur(c));    // ghosts from the given interiors
-    enq_bcs(c, S_old(c));
+    if (c->mp.doMHD) {       // active cell B from the given face fluxes
+        enq_bflux2fld(c, S_cur(c));
+        enq_bflux2fld(c, S_old(c));
+    }
+    enq_bcs(c, S_cur(c));    // ghosts from the given interiors, oState's at its time
+    enq_bcs(c, S_old(c), true, a->oclk);
     if (c->mp.doMHD) {       // cell B from the face fluxes
         enq_bflux2fld(c, S_cur(c));
         enq_bflux2fld(c, S_old(c));
@@ -1528,6 +1559,17 @@ extern "C" int gam_adj_replay(gam_adj *a, const double *Gas, const double *oGas,
     HIPCHK(hipGetLastError());
     if ((rc = adj_objective(a, S_cur(c), a->lam + 2 * ADJ_SET * N3, J))) return rc;
     return adj_restore_final(a);
+}
+
+extern "C" int gam_adj_wind_grad(gam_adj *a, double *gQ, double *gB)
+{
+    if (!a || !gQ || !gB) return fail(GAM_E_ARG, "gam_adj_wind_grad: NULL argument");
+    if (!a->nw) return fail(GAM_E_STATE, "gam_adj_wind_grad: the model has no inflow BC");
+    gam_ctx *c = a->c;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(gQ, a->gwq, (size_t)a->nw * 5 * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(gB, a->gwb, (size_t)a->nw * 3 * sizeof(double), hipMemcpyDeviceToHost));
+    return GAM_OK;
 }
 
 extern "C" int gam_adj_times(gam_adj *a, double *fwd_ms, double *bwd_ms)

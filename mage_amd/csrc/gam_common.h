@@ -39,6 +39,15 @@ struct MParm {
 };
 
 // Device-resident time/dt bookkeeping (Model_T%t/dt/ts, State%time, oState%time)
+// Inflow series of the wind BC on the device (readWind, wind.F90:685-699)
+struct WindDev {
+    const double *tW, *Q, *B;  // n, (n,5), (n,3) column-major
+    int n;
+    double tMin, tMax, dtW;
+    double Bx0, ByC, BzC;
+    double xW, yW, zW;
+};
+
 struct Clock {
     double t;       // Model%t
     double dt;      // Model%dt

@@ -67,14 +67,6 @@ enum {
     FIN_INIT = 16      // oState%time = State%time - dt (init.F90:238-244)
 };
 
-struct WindDev {
-    const double *tW, *Q, *B;  // n, (n,5), (n,3) column-major
-    int n;
-    double tMin, tMax, dtW;
-    double Bx0, ByC, BzC;
-    double xW, yW, zW;
-};
-
 void launch_predict(hipStream_t s, int ax, const Geo &g, const MParm &m, const Box &fb, const Clock *clk,
                     const double *S, const double *O, const double *met, double *H);
 // Mom2Face of the three face directions: Vf planes [3*dT1 + comp]
@@ -123,6 +115,9 @@ void launch_set_dt(hipStream_t s, Clock *clk, const Clock *rec);
 void launch_cellb_adj(hipStream_t s, const Geo &g, int all, double *aS, const double *met);
 void launch_bc_periodic_adj(hipStream_t s, int side, const Geo &g, double *aS);
 void launch_bc_mhd_adj(hipStream_t s, int side, int kind, const Geo &g, double *aS, const double *met);
+// transpose of the inflow BC; adds the series adjoint into gQ (n,5), gB (n,3)
+void launch_bc_wind_adj(hipStream_t s, const Geo &g, const MParm &m, const Clock *clk, const WindDev &w,
+                        const double *S, double *aS, const double *met, double *gQ, double *gB);
 void launch_update_adj_mhd(hipStream_t s, const Geo &g, const MParm &m, const Box &ab, const Box &ctb,
                            const Box *eboxes, const Clock *clk, const double *So, const double *Sn1,
                            const double *gflx, const double *mflx, const double *met, const double *bg,

@@ -40,6 +40,31 @@ __device__ __forceinline__ void p2c_cell(const MParm &m, const double *W, double
     U[ENER] = ie + ke;
 }
 
+// The MHD side of the inflow BC's ghost gas (WindBC, wind.F90:226-335):
+// CellC2P of the last active cell U (cell B Bi) with the velocity's
+// component along the boundary face normal nH raised to the fast speed
+// Cx = sqrt(Cs^2 + Va^2) where it is below it (Con2Fast,
+// gamutils.F90:338-360; Va Boris-limited with doBoris).
+__device__ __forceinline__ void wind_inner(const MParm &m, const double *U, const double *Bi, const double *nH,
+                                           double *gi)
+{
+    c2p_cell(m, U, gi);
+    const double rho = dmax(U[DEN], m.dFloor);
+    const double vx = U[MOMX] / rho, vy = U[MOMY] / rho, vz = U[MOMZ] / rho;
+    const double ke = 0.5 * rho * (vx * vx + vy * vy + vz * vz);
+    const double P = dmax((m.gamma - 1) * (U[ENER] - ke), m.pFloor);
+    const double Cs = sqrt(m.gamma * P / rho);
+    double Va = sqrt(Bi[0] * Bi[0] + Bi[1] * Bi[1] + Bi[2] * Bi[2]) / sqrt(U[DEN]);
+    if (m.doBoris) Va = m.Ca * Va / sqrt(m.Ca * m.Ca + Va * Va);
+    const double Cx = sqrt(Cs * Cs + Va * Va);
+    double V[3] = {gi[1], gi[2], gi[3]};
+    const double vp = V[0] * nH[0] + V[1] * nH[1] + V[2] * nH[2];
+    if (vp < 1.0 * Cx) {
+        for (int x = 0; x < 3; x++) V[x] = V[x] - vp * nH[x] + 1.0 * Cx * nH[x];
+    }
+    gi[1] = V[0]; gi[2] = V[1]; gi[3] = V[2];
+}
+
 // BlockCon2Prim (recon.F90:75-90): density floored before dividing.
 __device__ __forceinline__ void c2p_block(const MParm &m, const double *U, double *W)
 {

@@ -1289,62 +1289,7 @@ struct k_bc_copy {
 };
 
 // ---------------------------------------------------------------- wind
-
-__device__ void wind_interp(const WindDev &w, double t, double *rho, double *pr, double *V, double *B)
-{   // InterpWind/InterpWgts with discrete walls (wind.F90:766-868)
-    int i0;
-    if (t >= w.tMax) i0 = w.n - 1;
-    else if (t <= w.tMin) i0 = 0;
-    else {
-        i0 = (int)floor((t - w.tMin) / w.dtW);
-        if (i0 + 1 > w.n - 1) i0 = w.n - 2;
-    }
-    *rho = w.Q[i0];
-    *pr = w.Q[i0 + 4 * w.n];
-    V[0] = w.Q[i0 + 1 * w.n]; V[1] = w.Q[i0 + 2 * w.n]; V[2] = w.Q[i0 + 3 * w.n];
-    B[1] = w.B[i0 + w.n]; B[2] = w.B[i0 + 2 * w.n];
-    B[0] = w.Bx0 + w.ByC * B[1] + w.BzC * B[2];
-    if (t <= w.tMin) { B[0] = 0.0; B[1] = 0.0; B[2] = 0.0; }
-}
-
-__device__ void wind_front(const WindDev &w, double t, double *vF)
-{   // RefreshWind (wind.F90:114-155)
-    double r, p, V[3], B[3];
-    wind_interp(w, t, &r, &p, V, B);
-    const double vs = -sqrt(V[0] * V[0] + V[1] * V[1] + V[2] * V[2]);
-    const double bcc = sqrt(w.ByC * w.ByC + w.BzC * w.BzC);
-    const double btt = sqrt(1 + bcc * bcc);
-    if (bcc > GAM_TINY) {
-        const double cB = w.ByC / bcc, sB = w.BzC / bcc, cF = 1.0 / btt, sF = bcc / btt;
-        vF[0] = vs * (cF * cF);
-        vF[1] = -vs * cF * sF * cB;
-        vF[2] = -vs * cF * sF * sB;
-    } else { vF[0] = vs; vF[1] = 0.0; vF[2] = 0.0; }
-}
-
-__device__ void wind_at(const WindDev &w, const double *vF, const double *x, double t,
-                        double *rho, double *pr, double *V, double *B)
-{   // GetWindAt (wind.F90:157-175)
-    const double vm = sqrt(vF[0] * vF[0] + vF[1] * vF[1] + vF[2] * vF[2]);
-    const double dl = ((x[0] - w.xW) * vF[0] + (x[1] - w.yW) * vF[1] + (x[2] - w.zW) * vF[2]) / (vm * vm);
-    wind_interp(w, t - dl, rho, pr, V, B);
-}
-
-__device__ double wind_wgt(const WindDev &w, const double *vF, const double *x, double t)
-{   // wgtWind (wind.F90:180-203) with RampDown(dTh,90,90) (math.F90:97-111)
-    double nh[3], vh[3], r, p, V[3], B[3];
-    double na = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
-    if (na <= GAM_TINY) { nh[0] = nh[1] = nh[2] = 0.0; } else { nh[0] = x[0] / na; nh[1] = x[1] / na; nh[2] = x[2] / na; }
-    wind_at(w, vF, x, t, &r, &p, V, B);
-    double nv = sqrt(V[0] * V[0] + V[1] * V[1] + V[2] * V[2]);
-    if (nv <= GAM_TINY) { vh[0] = vh[1] = vh[2] = 0.0; } else { vh[0] = V[0] / nv; vh[1] = V[1] / nv; vh[2] = V[2] / nv; }
-    const double dTh = acos(-vh[0] * nh[0] + -vh[1] * nh[1] + -vh[2] * nh[2]) * 180.0 / GAM_PI;
-    if (dTh <= 90.0) return 1.0;
-    if (dTh >= 180.0) return 0.0;
-    const double rs = (dTh - 90.0) / 90.0;
-    const double ea = -1.0 / (1.0 - rs * rs);
-    return exp(ea + 1.0);
-}
+#include "gam_wind.h"
 
 // WindBC part 1 (wind.F90:226-335): ghost gas and blended face fluxes.
 // Transverse: a = j in jsg..jeg+1, b = k in ksg..keg+1.
@@ -1378,24 +1323,9 @@ struct k_bc_wind {
                     double U[5], Bi[3];
                     for (int v = 0; v < 5; v++) U[v] = S[(S_GAS + v) * N3 + cp];
                     for (int x = 0; x < 3; x++) Bi[x] = S[(S_B + x) * N3 + cp];
-                    c2p_cell(m, U, gi);
-                    // Con2Fast (gamutils.F90:338-360)
-                    const double rho = dmax(U[DEN], m.dFloor);
-                    const double vx = U[MOMX] / rho, vy = U[MOMY] / rho, vz = U[MOMZ] / rho;
-                    const double ke = 0.5 * rho * (vx * vx + vy * vy + vz * vz);
-                    const double P = dmax((m.gamma - 1) * (U[ENER] - ke), m.pFloor);
-                    const double Cs = sqrt(m.gamma * P / rho);
-                    double Va = sqrt(Bi[0] * Bi[0] + Bi[1] * Bi[1] + Bi[2] * Bi[2]) / sqrt(U[DEN]);
-                    if (m.doBoris) Va = m.Ca * Va / sqrt(m.Ca * m.Ca + Va * Va);
-                    const double Cx = sqrt(Cs * Cs + Va * Va);
                     const long fp = IX(g, ip + 1, j, k);
                     const double nH[3] = {met[(M_TF + 0) * N3 + fp], met[(M_TF + 1) * N3 + fp], met[(M_TF + 2) * N3 + fp]};
-                    double V[3] = {gi[1], gi[2], gi[3]};
-                    const double vp = V[0] * nH[0] + V[1] * nH[1] + V[2] * nH[2];
-                    if (vp < 1.0 * Cx) {
-                        for (int x = 0; x < 3; x++) V[x] = V[x] - vp * nH[x] + 1.0 * Cx * nH[x];
-                    }
-                    gi[1] = V[0]; gi[2] = V[1]; gi[3] = V[2];
+                    wind_inner(m, U, Bi, nH, gi);
                 }
                 double gw[5], U[5];
                 for (int v = 0; v < 5; v++) gw[v] = wSW * gs[v] + wMHD * gi[v];

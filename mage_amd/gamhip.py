@@ -404,7 +404,7 @@ class GamEnsemble:
 # ---------------------------------------------------------------- adjoint
 # Every symbol include/gamadj.h declares (exported by libgamhip)
 ADJ_EXPORTS = ("gam_adj_create", "gam_adj_destroy", "gam_adj_set_weights", "gam_adj_forward",
-               "gam_adj_backward", "gam_adj_replay", "gam_adj_times")
+               "gam_adj_backward", "gam_adj_replay", "gam_adj_wind_grad", "gam_adj_times")
 
 
 def _load_adj(L):
@@ -415,6 +415,7 @@ def _load_adj(L):
     L.gam_adj_forward.argtypes = [vp, DP]
     L.gam_adj_backward.argtypes = [vp, DP, DP, DP, DP]
     L.gam_adj_replay.argtypes = [vp, DP, DP, DP, DP, DP]
+    L.gam_adj_wind_grad.argtypes = [vp, DP, DP]
     L.gam_adj_times.argtypes = [vp, DP, DP]
     return L
 
@@ -466,6 +467,14 @@ class GamAdjoint:
         arr = [None if x is None else np.asfortranarray(x, dtype=np.float64) for x in (Gas, oGas, MF, oMF)]
         self._chk(self.L.gam_adj_replay(self.adj, *[_p(x) for x in arr], C.byref(J)))
         return J.value
+
+    def wind_grad(self, n):
+        """Gradient with respect to the inflow series of the last backward:
+        (gQ (n,5), gB (n,3)) in the layout set_wind takes."""
+        gQ = np.zeros((n, 5), order="F")
+        gB = np.zeros((n, 3), order="F")
+        self._chk(self.L.gam_adj_wind_grad(self.adj, _p(gQ), _p(gB)))
+        return gQ, gB
 
     def times(self):
         f = C.c_double()

@@ -21,6 +21,7 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SRC = r'''
 #include "gam_adj.h"
+#include "gam_wind.h"
 static MParm mp(double gamma, double dF, double pF, int lfm)
 {
     MParm m = {};
@@ -171,6 +172,40 @@ void a_emf(int rm, int boris, const double *geo, const double *x, const double *
     for (int d = 0; d < 3; d++) for (int n = 0; n < 8; n++) ax[8 * d + n] = o.V1[d][n];
     for (int n = 0; n < 8; n++) { ax[24 + n] = o.mH1[n]; ax[36 + n] = o.mH2[n]; }
     for (int n = 0; n < 4; n++) { ax[32 + n] = o.mL1[n]; ax[44 + n] = o.mL2[n]; ax[48 + n] = o.D4[n]; }
+}
+void f_win(int boris, const double *nH, const double *x, double *y)
+{   // x = U[5], B[3] of the last active cell; nH the boundary face normal
+    MParm m = mp(5.0 / 3.0, 1e-12, 1e-12, 1); m.doBoris = boris; m.Ca = 0.8;
+    wind_inner(m, x, x + 5, nH, y);
+}
+void a_win(int boris, const double *nH, const double *x, const double *a, double *ax)
+{
+    MParm m = mp(5.0 / 3.0, 1e-12, 1e-12, 1); m.doBoris = boris; m.Ca = 0.8;
+    for (int v = 0; v < 8; v++) ax[v] = 0;
+    wind_inner_adj(m, x, x + 5, nH, a, ax, ax + 5);
+}
+static WindDev wind2(double *tW, double *Q, double *B, const double *V)
+{   // two equal samples with velocity V
+    tW[0] = -1e3; tW[1] = 1e3;
+    for (int i = 0; i < 2; i++) { Q[i] = 1.0; Q[2 + i] = V[0]; Q[4 + i] = V[1]; Q[6 + i] = V[2]; Q[8 + i] = 1e-3; }
+    for (int i = 0; i < 6; i++) B[i] = 0.0;
+    WindDev w = {tW, Q, B, 2, -1e3, 1e3, 2e3, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    return w;
+}
+void f_wgt(const double *pos, const double *x, double *y)
+{   // x = the sample's velocity
+    double tW[2], Q[10], B[6];
+    WindDev w = wind2(tW, Q, B, x);
+    const double vF[3] = {-0.5, 0.0, 0.0};
+    y[0] = wind_wgt(w, vF, pos, 0.0);
+}
+void a_wgt(const double *pos, const double *x, const double *a, double *ax)
+{
+    double tW[2], Q[10], B[6], dw[3];
+    WindDev w = wind2(tW, Q, B, x);
+    const double vF[3] = {-0.5, 0.0, 0.0};
+    wind_wgt_dv(w, vF, pos, 0.0, dw);
+    for (int q = 0; q < 3; q++) ax[q] = a[0] * dw[q];
 }
 void f_pred(double g, int lfm, double w, double vol, const double *x, double *y)
 {   // x = Q[5], oQ[5]; y = H_GAS 5, H_WH 5, H_WN 5 (as k_predict writes them)
@@ -397,6 +432,45 @@ def test_edge_emf_adjoint(lib, rm, boris, cap):
     fwd = lambda x, y: lib.f_emf(rm, boris, _p(geo), _p(x), _p(y))
     adj = lambda x, a, ax: lib.a_emf(rm, boris, _p(geo), _p(x), _p(a), _p(ax))
     _check(fwd, adj, x0, 1, rng, 1e-9, False, nsamp=2000)
+
+
+@pytest.mark.parametrize("boris", [0, 1])
+@pytest.mark.parametrize("fix", [True, False])
+def test_inflow_interior_cell_adjoint(lib, boris, fix):
+    """The inflow BC's interior primitives: CellC2P of the last active cell
+    with the outflow velocity raised to the fast speed (Con2Fast sound and
+    Boris-limited Alfven speeds) where it is below it (fix), or left as it
+    is (a supersonic outflow, no fix)."""
+    rng = np.random.default_rng(41 + boris + 2 * fix)
+    nH = _triad(rng)[:3].copy()
+
+    def x0(r):
+        d = r.uniform(0.3, 2.0)
+        v = r.standard_normal(3) * 0.3
+        v += ((0.0 if fix else 8.0) - v @ nH) * nH + r.uniform(-0.2, 0.2) * nH
+        U = [d, d * v[0], d * v[1], d * v[2], 0.5 * d * v @ v + r.uniform(0.1, 2.0) * 1.5]
+        return np.concatenate([U, r.standard_normal(3)])
+    fwd = lambda x, y: lib.f_win(boris, _p(nH), _p(x), _p(y))
+    adj = lambda x, a, ax: lib.a_win(boris, _p(nH), _p(x), _p(a), _p(ax))
+    _check(fwd, adj, x0, 5, rng, 1e-6, True)
+
+
+def test_inflow_weight_direction_adjoint(lib):
+    """The inflow blend weight (RampDown of the angle between the position
+    and the upstream direction) against the sample's velocity, at positions
+    on the downstream side where the weight falls from 1 to 0."""
+    rng = np.random.default_rng(43)
+    for _ in range(20):
+        ph = rng.uniform(0.55, 0.95) * np.pi
+        th = rng.uniform(0.2, 0.8) * np.pi
+        pos = 20.0 * np.array([np.cos(ph) * np.sin(th), np.sin(ph) * np.sin(th), np.cos(th)])
+        fwd = lambda x, y: lib.f_wgt(_p(pos), _p(x), _p(y))
+        adj = lambda x, a, ax: lib.a_wgt(_p(pos), _p(x), _p(a), _p(ax))
+        _check(fwd, adj, lambda r: np.array([-0.5, 0.0, 0.0]) + r.standard_normal(3) * 0.1, 1, rng, 1e-6,
+               True, nsamp=20)
+        y = np.zeros(1)
+        fwd(np.array([-0.5, 0.0, 0.0]), y)
+        assert 0.0 <= y[0] <= 1.0
 
 
 def test_adjoint_catches_a_planted_sign_error(lib):

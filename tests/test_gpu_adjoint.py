@@ -1,7 +1,9 @@
 """Discrete adjoint of the hydrodynamic step against central finite
 differences of the same device step (dt held at the recorded values in
 both), on the periodic box (8CENT and 7UP) and on a warped shell with
-zero-gradient and periodic boundaries.
+zero-gradient, inflow and periodic boundaries (MHD with the dipole
+background and Boris), including the gradient with respect to the inflow
+time series.
 
 Gradients are with respect to the interiors of the starting State and
 oState gas (ghost cells follow through the boundary conditions, in the
@@ -88,6 +90,14 @@ def _shell_weights(g):
     return np.asfortranarray(grid["volume"] * ((r >= 3.0) & (r <= 6.0)))
 
 
+def _outer_weights(g, r0=20.0):
+    """Pressure integral over r >= r0: the cells the inflow reaches within
+    a few steps."""
+    grid = g.get_grid(fields=("xyzcc", "volume"))
+    r = np.sqrt(np.sum(grid["xyzcc"] ** 2, axis=-1))
+    return np.asfortranarray(grid["volume"] * (r >= r0))
+
+
 def _fd_check(g, nsteps, ckpt, ndir, seed, tag, weights=None, ostate=True):
     s0 = g.get_state(gamhip.GAM_STATE)["Gas"].copy(order="F")
     o0 = g.get_state(gamhip.GAM_OSTATE)["Gas"].copy(order="F")
@@ -156,16 +166,19 @@ def test_adjoint_shell_zero_gradient_matches_finite_differences():
 
 
 def test_adjoint_refuses_unsupported_contexts():
-    """The inflow BC has no adjoint yet; Boris or a background without MHD
-    is refused too."""
+    """Boris or a background without MHD is refused; the inflow-series
+    gradient needs an inflow BC."""
     N = (16, 16, 16)
-    g = gamhip.GamHip(*N, pb.shell_params("7UP"))
-    g.grid_from_corners(pb.shell_corners(*N))
-    g.add_dipole_b0(pb.SHELL_M0)
-    g.set_wind(pb.constant_wind())
+    g = box_case(N, "7UP", doMHD=0)[0]
+    g.init_run()
+    adj = gamhip.GamAdjoint(g, 2, 1)
+    adj.set_weights(_weights(g))
+    adj.forward()
+    adj.backward()
     with pytest.raises(gamhip.GamError) as ei:
-        gamhip.GamAdjoint(g, 4, 2)
-    assert ei.value.code == -1
+        adj.wind_grad(2)
+    assert ei.value.code == -3
+    adj.close()
     g.close()
     p = pb.box_params("7UP")
     p.update(doMHD=0, doBoris=1)
@@ -276,6 +289,104 @@ def test_adjoint_mhd_shell_walls_matches_finite_differences():
     # (the same with all-periodic walls and without B0 or Boris, while the
     # Cartesian boxes above reach 1e-6): held to 5e-3 here
     assert max(_fd_check_flux(g, 5, 3, 3, 808, "shell walls", w, ostate=False)) <= 5e-3
+    g.close()
+
+
+def _wind_series(n=16, seed=0):
+    """A time-varying inflow whose lag across the r = 30 boundary spans
+    about ten samples; every column varies, the flow is off the x axis and
+    the IMF has a Bx from ByC, BzC."""
+    rng = np.random.default_rng(seed)
+    ph = rng.uniform(0, 2 * np.pi, 7)
+    s = 2 * np.pi * np.arange(n) / n
+    Q = np.zeros((n, 5), order="F")
+    Q[:, 0] = 1.0 + 0.2 * np.sin(s + ph[0])
+    Q[:, 1] = -0.4 + 0.05 * np.sin(2 * s + ph[1])
+    Q[:, 2] = 0.04 * np.sin(s + ph[2])
+    Q[:, 3] = 0.03 * np.cos(s + ph[3])
+    Q[:, 4] = 1e-3 * (1.0 + 0.3 * np.sin(s + ph[4]))
+    B = np.zeros((n, 3), order="F")
+    B[:, 1] = 0.02 * np.sin(s + ph[5])
+    B[:, 2] = -0.05 + 0.02 * np.cos(s + ph[6])
+    return {"tW": np.linspace(-120.0, 120.0, n), "Q": Q, "B": B, "Bx0": 0.01, "ByC": 0.1, "BzC": -0.05,
+            "xyzW": (0.0, 0.0, 0.0)}
+
+
+def _fd_check_wind(g, wind, nsteps, ckpt, seed, tag, weights):
+    """Directions in the inflow series: all columns; density, pressure and
+    field only (these leave the lag, a step function of the velocity, in
+    place); velocity only.  Best relative agreement over e = 1e-2 .. 1e-6 of
+    1 % of each column's scale."""
+    S = g.get_state(gamhip.GAM_STATE)
+    O = g.get_state(gamhip.GAM_OSTATE)
+    s0, o0 = S["Gas"].copy(order="F"), O["Gas"].copy(order="F")
+    adj = gamhip.GamAdjoint(g, nsteps, ckpt)
+    adj.set_weights(weights)
+    adj.forward()
+    adj.backward()
+    n = len(wind["tW"])
+    gQ, gB = adj.wind_grad(n)
+    assert np.all(gB[:, 0] == 0.0)
+    assert np.count_nonzero(gQ[:, 0]) >= 3 and np.count_nonzero(gB[:, 2]) >= 3, (gQ, gB)
+    rng = np.random.default_rng(seed)
+    qs = np.max(np.abs(wind["Q"]), axis=0)
+    qs[2:4] = np.max(np.abs(wind["Q"][:, 1]))
+    bs = np.max(np.abs(wind["B"]))
+    bests = []
+    for kind in ("all", "rho-p-B", "velocity"):
+        dQ = np.asfortranarray(rng.standard_normal((n, 5)) * qs * 1e-2)
+        dB = np.zeros((n, 3), order="F")
+        dB[:, 1:] = rng.standard_normal((n, 2)) * bs * 1e-2
+        if kind == "rho-p-B":
+            dQ[:, 1:4] = 0.0
+        elif kind == "velocity":
+            dQ[:, [0, 4]] = 0.0
+            dB[:] = 0.0
+        ad = np.sum(gQ * dQ) + np.sum(gB * dB)
+        best = None
+        for e in (1e-2, 1e-3, 1e-4, 1e-5, 1e-6):
+            J = []
+            for sg in (1.0, -1.0):
+                w = dict(wind)
+                w["Q"] = wind["Q"] + sg * e * dQ
+                w["B"] = wind["B"] + sg * e * dB
+                g.set_wind(w)
+                J.append(adj.replay(s0, o0))
+            fd = (J[0] - J[1]) / (2 * e)
+            rel = abs(fd - ad) / max(abs(fd), abs(ad))
+            print("%s series dir %s e %.0e: fd %.12e adj %.12e rel %.2e" % (tag, kind, e, fd, ad, rel), flush=True)
+            best = rel if best is None else min(best, rel)
+        g.set_wind(wind)
+        bests.append(best)
+    adj.close()
+    print("%s series directions, best relative agreement: %s" % (tag, ["%.1e" % b for b in bests]), flush=True)
+    return bests
+
+
+def test_adjoint_mhd_shell_inflow_matches_finite_differences():
+    """The adjoint workload's physics: warped shell, dipole background,
+    Boris (Ca = 10), 7UP, a time-varying inflow on the outer r face,
+    zero-gradient inner r and theta walls, periodic phi.  Gas and face-flux
+    directions of the start state and directions in the inflow series."""
+    N = (24, 16, 24)
+    g = gamhip.GamHip(*N, pb.shell_params("7UP"))
+    xyz = pb.shell_corners(*N)
+    g.grid_from_corners(xyz)
+    g.add_dipole_b0(pb.SHELL_M0)
+    wind = _wind_series()
+    g.set_wind(wind)
+    xyzcc = g.get_grid(fields=("xyzcc",))["xyzcc"]
+    st = pb.shell_state(*N, xyzcc, p0=0.1)
+    st["magFlux"] = pb.vecpot_fluxes(xyz, lambda x, y, z: (0.2 * np.sin(0.3 * y), 0.2 * np.cos(0.2 * z),
+                                                           0.2 * np.sin(0.25 * x)))
+    g.put_state(st)
+    g.init_run()
+    _generic(g, 17)
+    g.run(2)
+    w = _outer_weights(g)
+    assert _fd_check(g, 5, 3, 4, 909, "shell inflow", w, ostate=False) <= TOL
+    assert max(_fd_check_flux(g, 5, 3, 2, 910, "shell inflow", w, ostate=False)) <= 5e-3
+    assert max(_fd_check_wind(g, wind, 5, 3, 911, "shell inflow", w)) <= 1e-5
     g.close()
 
 
