@@ -143,20 +143,21 @@ def full_size(gamhip, pb, N=(256, 256, 384), steps=500, warmup=5, tune=True):
             "step_frac_of_hbm_roofline": STEP_BYTES_PER_CELL * value / 1e9 / HBM_PEAK_GBS}
 
 
-def adjoint_cost(gamhip, pb, N=(96, 96, 128), steps=200, ckpt=20):
-    """Adjoint / forward cost on the adjoint workload's grid, step count and
-    checkpoint interval, for the step the adjoint covers (hydrodynamic:
-    doMHD = 0; zero-gradient r and theta walls, periodic phi).  Objective:
-    the pressure integral over 3 <= r <= 6 at the final step."""
+def adjoint_cost(gamhip, pb, N=(96, 96, 128), steps=200, ckpt=20, nwind=200):
+    """Adjoint / forward cost of the adjoint workload: the shell with the
+    dipole background, Boris, 7UP, a time-varying inflow series of nwind
+    samples on the outer r face, zero-gradient inner walls, periodic phi;
+    steps steps with checkpoints every ckpt; objective the pressure integral
+    over 3 <= r <= 6 at the final step; gradients with respect to the start
+    state and to the inflow series (its rho, vx and Bz columns reported)."""
     import numpy as np
     p = pb.shell_params("7UP")
-    p.update(doMHD=0, doBoris=0, doBackground=0,
-             bc=["zerograd", "zerograd", "zerograd", "zerograd", "periodic", "periodic"])
     g = gamhip.GamHip(*N, p)
     g.grid_from_corners(pb.shell_corners(*N))
+    g.add_dipole_b0(pb.SHELL_M0)
+    g.set_wind(pb.varying_wind(nwind))
     grid = g.get_grid(fields=("xyzcc", "volume"))
-    st = pb.shell_state(*N, grid["xyzcc"], p0=0.1)
-    g.put_state(st)
+    g.put_state(pb.shell_state(*N, grid["xyzcc"], p0=0.1))
     g.init_run()
     g.run(2)
     r = np.sqrt(np.sum(grid["xyzcc"] ** 2, axis=-1))
@@ -165,16 +166,25 @@ def adjoint_cost(gamhip, pb, N=(96, 96, 128), steps=200, ckpt=20):
     adj = gamhip.GamAdjoint(g, steps, ckpt)
     adj.set_weights(w)
     J = adj.forward()
-    adj.backward()
+    t0 = time.perf_counter()
+    gS, gO, gM, gMO = adj.backward(fluxes=True)
+    call_ms = 1e3 * (time.perf_counter() - t0)
+    gQ, gB = adj.wind_grad(nwind)
     f_ms, b_ms = adj.times()
     adj.close()
     g.close()
-    return {"workload": "%dx%dx%d shell, hydrodynamic step (the part with an adjoint), %d steps, "
-                        "checkpoint every %d, J = pressure integral over 3 <= r <= 6" % (N + (steps, ckpt)),
+    return {"workload": "%dx%dx%d shell, MHD with dipole background, Boris, 7UP, %d-sample inflow series, "
+                        "%d steps, checkpoint every %d, J = pressure integral over 3 <= r <= 6"
+                        % (N + (nwind, steps, ckpt)),
             "J": J, "forward_plain_ms": plain_ms, "forward_recorded_ms": f_ms, "reverse_ms": b_ms,
-            "adjoint_over_forward": b_ms / plain_ms,
-            "note": "reverse = checkpoint-interval recompute + adjoint kernels; "
-                    "MHD terms and the inflow boundary have no adjoint yet"}
+            "backward_call_ms": call_ms, "adjoint_over_forward": b_ms / plain_ms,
+            "grad_norms": {"state_gas": float(np.linalg.norm(gS)), "state_flux": float(np.linalg.norm(gM)),
+                           "inflow_rho": float(np.linalg.norm(gQ[:, 0])),
+                           "inflow_vx": float(np.linalg.norm(gQ[:, 1])),
+                           "inflow_bz": float(np.linalg.norm(gB[:, 2])),
+                           "inflow_samples_reached": int(np.count_nonzero(gQ[:, 0]))},
+            "note": "reverse = checkpoint-interval recompute + adjoint kernels (HIP events); "
+                    "backward_call_ms adds the host side and the gradient downloads"}
 
 
 def cpu_baseline(gamhip, pb, min_seconds=12.0, max_steps=400):

@@ -206,6 +206,37 @@ __device__ __forceinline__ void pdm_pair_adj(double b, double v0, double v1, dou
     clip_adj(v1, v2, qi, avN, av1, av2, aqi);
 }
 
+// recon_lr_w's transpose up to its linear stencil: sl, sr = the adjoints of
+// the left and right high-order values times rVi, so that the stencil
+// adjoint is aqv[n] += UP7(n) sl (n <= 6) + UP7(7 - n) sr (n >= 1) for
+// RM == 1, aqv[n] += W8(n) sl (sr = 0) for RM == 0; adds into anq.
+template <int RM>
+__device__ __forceinline__ void recon_lr_w_adj_s(double pdmb, double rVi, const double *nq,
+                                                 const double *qv, double aql, double aqr,
+                                                 double *anq, double &sl, double &sr)
+{
+    if (RM == 1) {
+        double l = qv[0] * UP7(0) + qv[1] * UP7(1) + qv[2] * UP7(2) + qv[3] * UP7(3)
+                 + qv[4] * UP7(4) + qv[5] * UP7(5) + qv[6] * UP7(6);
+        double r = qv[7] * UP7(0) + qv[6] * UP7(1) + qv[5] * UP7(2) + qv[4] * UP7(3)
+                 + qv[3] * UP7(4) + qv[2] * UP7(5) + qv[1] * UP7(6);
+        double al = 0.0, ar = 0.0;
+        pdm_one_adj(pdmb, nq[0], nq[1], nq[2], l * rVi, aql, anq[0], anq[1], anq[2], al);
+        pdm_one_adj(pdmb, nq[3], nq[2], nq[1], r * rVi, aqr, anq[3], anq[2], anq[1], ar);
+        sl = rVi * al;
+        sr = rVi * ar;
+    } else {
+        double t = 0.0;
+        for (int n = 0; n < 8; n++) t += W8(n) * qv[n];
+        const double qi = t * rVi;
+        double ai = 0.0;
+        pdm_pair_adj(pdmb, nq[0], nq[1], nq[2], nq[3], qi, aql, aqr, anq[0], anq[1], anq[2],
+                     anq[3], ai);
+        sl = rVi * ai;
+        sr = 0.0;
+    }
+}
+
 // recon_lr_w: qv[8] volume-weighted stencil values, nq[4] limiter values
 template <int RM>
 __device__ __forceinline__ void recon_lr_w_adj(double pdmb, double rVi, const double *nq,
@@ -517,8 +548,18 @@ __device__ __forceinline__ double edge_emf(const MParm &m, const EdgeIn &in, dou
     return (-(v1 * b2 - v2 * b1) + m.Vd0 * vDiff * Jd) * in.edge;
 }
 
+// edge_emf's transpose up to its linear stencils (gathered by the cell and
+// face side, k_emf_adj_gather): sv = the corner-velocity adjoint over the
+// area sum (V1[d][n] gets W8(n) A1[n] sv[d]); sl, sr and anq the edge-B
+// reconstructions' stencil scalars and limiter adjoints (recon_lr_w_adj_s)
+// for the mH/A and mL/A values; aDc the corner density's adjoint (each D4
+// gets aDc / 4).
+struct EdgeAdjS {
+    double sv[3], sl1, sr1, anq1[4], sl2, sr2, anq2[4], aDc;
+};
+
 template <int RM, bool BORIS>
-__device__ __forceinline__ void edge_emf_adj(const MParm &m, const EdgeIn &in, double aE, EdgeAdj &out)
+__device__ __forceinline__ void edge_emf_adj_s(const MParm &m, const EdgeIn &in, double aE, EdgeAdjS &o)
 {
     double l1, r1, l2, r2;
     edge_emf<RM, BORIS>(m, in, &l1, &r1, &l2, &r2);
@@ -562,29 +603,49 @@ __device__ __forceinline__ void edge_emf_adj(const MParm &m, const EdgeIn &in, d
     if (vD > 0.0) { av1 += v1 / vD * avs; av2 += v2 / vD * avs; }
     // edge B reconstructions
     {
-        double q[8], qv[8], nq[4], aqv[8] = {0, 0, 0, 0, 0, 0, 0, 0}, anq[4] = {0, 0, 0, 0};
-        for (int n = 0; n < 8; n++) { q[n] = in.mH1[n] / dmax(in.A1[n], GAM_TINY); qv[n] = in.A1[n] * q[n]; }
+        double qv[8], nq[4];
+        for (int n = 0; n < 8; n++) qv[n] = in.A1[n] * (in.mH1[n] / dmax(in.A1[n], GAM_TINY));
         for (int n = 0; n < 4; n++) nq[n] = in.mL1[n] / dmax(in.A1[n + 2], GAM_TINY);
         const double Vi = wsum8([&](int n) { return in.A1[n]; });
-        recon_lr_w_adj<RM>(m.pdmb, 1.0 / Vi, nq, qv, al1, ar1, anq, aqv);
-        for (int n = 0; n < 8; n++) out.mH1[n] += in.A1[n] * aqv[n] / dmax(in.A1[n], GAM_TINY);
-        for (int n = 0; n < 4; n++) out.mL1[n] += anq[n] / dmax(in.A1[n + 2], GAM_TINY);
+        for (int n = 0; n < 4; n++) o.anq1[n] = 0.0;
+        recon_lr_w_adj_s<RM>(m.pdmb, 1.0 / Vi, nq, qv, al1, ar1, o.anq1, o.sl1, o.sr1);
     }
     {
-        double q[8], qv[8], nq[4], aqv[8] = {0, 0, 0, 0, 0, 0, 0, 0}, anq[4] = {0, 0, 0, 0};
-        for (int n = 0; n < 8; n++) { q[n] = in.mH2[n] / dmax(in.A2[n], GAM_TINY); qv[n] = in.A2[n] * q[n]; }
+        double qv[8], nq[4];
+        for (int n = 0; n < 8; n++) qv[n] = in.A2[n] * (in.mH2[n] / dmax(in.A2[n], GAM_TINY));
         for (int n = 0; n < 4; n++) nq[n] = in.mL2[n] / dmax(in.A2[n + 2], GAM_TINY);
         const double Vi = wsum8([&](int n) { return in.A2[n]; });
-        recon_lr_w_adj<RM>(m.pdmb, 1.0 / Vi, nq, qv, al2, ar2, anq, aqv);
-        for (int n = 0; n < 8; n++) out.mH2[n] += in.A2[n] * aqv[n] / dmax(in.A2[n], GAM_TINY);
-        for (int n = 0; n < 4; n++) out.mL2[n] += anq[n] / dmax(in.A2[n + 2], GAM_TINY);
+        for (int n = 0; n < 4; n++) o.anq2[n] = 0.0;
+        recon_lr_w_adj_s<RM>(m.pdmb, 1.0 / Vi, nq, qv, al2, ar2, o.anq2, o.sl2, o.sr2);
     }
     // corner velocity and density
-    double avc[3];
-    for (int d = 0; d < 3; d++) avc[d] = in.te1[d] * av1 + in.te2[d] * av2;
-    for (int d = 0; d < 3; d++)
-        for (int n = 0; n < 8; n++) out.V1[d][n] += W8(n) * in.A1[n] * rdA * avc[d];
-    for (int n = 0; n < 4; n++) out.D4[n] += 0.25 * aDc;
+    for (int d = 0; d < 3; d++) o.sv[d] = rdA * (in.te1[d] * av1 + in.te2[d] * av2);
+    o.aDc = aDc;
+}
+
+// the stencil weight of point n of the transposes split above
+template <int RM>
+__device__ __forceinline__ double stencil_adj(int n, double sl, double sr)
+{
+    if (RM == 1) return (n <= 6 ? UP7(n) * sl : 0.0) + (n >= 1 ? UP7(7 - n) * sr : 0.0);
+    return W8(n) * sl;
+}
+
+template <int RM, bool BORIS>
+__device__ __forceinline__ void edge_emf_adj(const MParm &m, const EdgeIn &in, double aE, EdgeAdj &out)
+{
+    EdgeAdjS t;
+    edge_emf_adj_s<RM, BORIS>(m, in, aE, t);
+    for (int n = 0; n < 8; n++) {
+        out.mH1[n] += in.A1[n] * stencil_adj<RM>(n, t.sl1, t.sr1) / dmax(in.A1[n], GAM_TINY);
+        out.mH2[n] += in.A2[n] * stencil_adj<RM>(n, t.sl2, t.sr2) / dmax(in.A2[n], GAM_TINY);
+        for (int d = 0; d < 3; d++) out.V1[d][n] += W8(n) * in.A1[n] * t.sv[d];
+    }
+    for (int n = 0; n < 4; n++) {
+        out.mL1[n] += t.anq1[n] / dmax(in.A1[n + 2], GAM_TINY);
+        out.mL2[n] += t.anq2[n] / dmax(in.A2[n + 2], GAM_TINY);
+        out.D4[n] += 0.25 * t.aDc;
+    }
 }
 
 // CellBoris (the Boris branch of k_update), single fluid.  Inputs: oU =

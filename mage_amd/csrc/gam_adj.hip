@@ -533,19 +533,16 @@ __global__ __launch_bounds__(256) void k_update_adj_faces(Geo g, Box fb, Box ab,
 // Boris mag-HOGS and nuke terms, L/R states of gas (H_WH, H_WN) and cell B
 // (H_B weighted by volume, limiter S_B), the normal flux Bn = H_MF / A.
 template <int D, int RM, bool BG, bool BORIS>
-__global__ __launch_bounds__(256) void k_flux_adj_mhd(Geo g, MParm m, Box fb,
+__global__ __launch_bounds__(256, (D == 0) ? 2 : 1) void k_flux_adj_mhd(Geo g, MParm m, Box fb,
                                                       const double *__restrict__ H,
                                                       const double *__restrict__ S,
                                                       const double *__restrict__ met,
                                                       const double *__restrict__ bg,
                                                       const double *__restrict__ ag,
                                                       const double *__restrict__ amf,
-                                                      double *__restrict__ aWH,
-                                                      double *__restrict__ aWN,
-                                                      double *__restrict__ aHB,
+                                                      double *__restrict__ sc,
                                                       double *__restrict__ aHMF,
-                                                      double *__restrict__ aHG,
-                                                      double *__restrict__ aSS)
+                                                      double *__restrict__ aHG)
 {
     POINT_IN(fb);
     const long N3 = g.N3;
@@ -620,39 +617,116 @@ __global__ __launch_bounds__(256) void k_flux_adj_mhd(Geo g, MParm m, Box fb,
     mag_kin_adj<BG, BORIS>(m, -1.0, WL, BL, Bn, T, B0, B0n, aM, aVaD, abbD, aL, abL, aBn);
     mag_kin_adj<BG, BORIS>(m, +1.0, WR, BR, Bn, T, B0, B0n, aM, aVaD, abbD, aR, abR, aBn);
     AADD(aHMF + D * N3 + c, aBn * iFaB);
+    // the reconstructions' transposes up to their linear stencils: per
+    // variable (gas 5, then B 3) the stencil scalars sl, sr and the four
+    // limiter adjoints, gathered into the cells by k_recon_adj_gather
     for (int v = 0; v < 5; v++) {
-        double qv[8], nq[4], aqv[8] = {0, 0, 0, 0, 0, 0, 0, 0}, anq[4] = {0, 0, 0, 0};
+        double qv[8], nq[4], anq[4] = {0, 0, 0, 0}, sl, sr;
         for (int n = 0; n < 8; n++) qv[n] = H[(H_WH + v) * N3 + c + (n - 4) * s];
         for (int n = 0; n < 4; n++) nq[n] = H[(H_WN + v) * N3 + c + (n - 2) * s];
-        recon_lr_w_adj<RM>(m.pdmb, rVi, nq, qv, aL[v], aR[v], anq, aqv);
-        for (int n = 0; n < 8; n++) AADD(aWH + v * N3 + c + (n - 4) * s, aqv[n]);
-        for (int n = 0; n < 4; n++) AADD(aWN + v * N3 + c + (n - 2) * s, anq[n]);
+        recon_lr_w_adj_s<RM>(m.pdmb, rVi, nq, qv, aL[v], aR[v], anq, sl, sr);
+        sc[(6 * v + 0) * N3 + c] = sl;
+        sc[(6 * v + 1) * N3 + c] = sr;
+        for (int n = 0; n < 4; n++) sc[(6 * v + 2 + n) * N3 + c] = anq[n];
     }
-    double *aBl = m.doLFM ? aSS + S_B * N3 : aHB;
     for (int x = 0; x < 3; x++) {
-        double qv[8], nq[4], aqv[8] = {0, 0, 0, 0, 0, 0, 0, 0}, anq[4] = {0, 0, 0, 0};
+        double qv[8], nq[4], anq[4] = {0, 0, 0, 0}, sl, sr;
         for (int n = 0; n < 8; n++) qv[n] = vol[n] * H[(H_B + x) * N3 + c + (n - 4) * s];
         for (int n = 0; n < 4; n++) nq[n] = Bl[x * N3 + c + (n - 2) * s];
-        recon_lr_w_adj<RM>(m.pdmb, rVi, nq, qv, abL[x], abR[x], anq, aqv);
-        for (int n = 0; n < 8; n++) AADD(aHB + x * N3 + c + (n - 4) * s, vol[n] * aqv[n]);
-        for (int n = 0; n < 4; n++) AADD(aBl + x * N3 + c + (n - 2) * s, anq[n]);
+        recon_lr_w_adj_s<RM>(m.pdmb, rVi, nq, qv, abL[x], abR[x], anq, sl, sr);
+        const int v = 5 + x;
+        sc[(6 * v + 0) * N3 + c] = sl;
+        sc[(6 * v + 1) * N3 + c] = sr;
+        for (int n = 0; n < 4; n++) sc[(6 * v + 2 + n) * N3 + c] = anq[n];
     }
 }
 
-// Edge EMF transpose (k_emf): scatters into the face velocities, the
-// half-state density, the half-state and limiter-state face fluxes.
+// The cell side of the reconstruction transposes of the three face
+// directions (k_flux_adj_mhd's scratch, 48 planes per direction) and of
+// Mom2Face (rv, aVf): one thread per cell sums what the faces whose
+// stencils hold it give it, with no atomics, and WRITES the half-state
+// adjoints aWH (5), aWN (5) and aHB (3) of every cell; with the LFM
+// limiter the B limiter adjoints go into aSSB (3, added).  Faces of
+// direction D exist on the active box extended by one along D.
+template <int RM>
+__global__ __launch_bounds__(256) void k_recon_adj_gather(Geo g, Box cb, const double *__restrict__ sc,
+                                                          const double *__restrict__ rv,
+                                                          const double *__restrict__ aVf,
+                                                          const double *__restrict__ met, int lfm,
+                                                          double *__restrict__ aWH, double *__restrict__ aWN,
+                                                          double *__restrict__ aHB, double *__restrict__ aSSB)
+{
+    POINT_IN(cb);
+    const long N3 = g.N3;
+    const long c = IX(g, i, j, k);
+    double gh[8], gn[8], av[3] = {0.0, 0.0, 0.0};
+    for (int v = 0; v < 8; v++) gh[v] = gn[v] = 0.0;
+    for (int D = 0; D < 3; D++) {
+        const long s = D == 0 ? 1 : D == 1 ? (long)g.P : g.PP;
+        const int at = D == 0 ? i : D == 1 ? j : k;
+        const int lo = D == 0 ? g.is : D == 1 ? g.js : g.ks;
+        const int hi = (D == 0 ? g.ie : D == 1 ? g.je : g.ke) + 1;
+        // Mom2Face faces span every transverse cell
+        for (int n = 0; n < 8; n++) {
+            const int fi = at - (n - 4);
+            if (fi < lo || fi > hi) continue;
+            const long f = c - (n - 4) * s;
+            const double w = W8(n) * rv[D * N3 + f];
+            for (int x = 0; x < 3; x++) av[x] += w * aVf[(3 * D + x) * N3 + f];
+        }
+        // interface-flux faces: active transverse cells only
+        const bool tr = D == 0 ? (j >= g.js && j <= g.je && k >= g.ks && k <= g.ke)
+                      : D == 1 ? (i >= g.is && i <= g.ie && k >= g.ks && k <= g.ke)
+                               : (i >= g.is && i <= g.ie && j >= g.js && j <= g.je);
+        if (!tr) continue;
+        const double *sd = sc + (long)48 * D * N3;
+#pragma unroll
+        for (int n = 0; n < 8; n++) {
+            const int fi = at - (n - 4);   // this cell is stencil point n of face fi
+            if (fi < lo || fi > hi) continue;
+            const long f = c - (n - 4) * s;
+            for (int v = 0; v < 8; v++) {
+                if (RM == 1) {
+                    if (n <= 6) gh[v] += UP7(n) * sd[(6 * v + 0) * N3 + f];
+                    if (n >= 1) gh[v] += UP7(7 - n) * sd[(6 * v + 1) * N3 + f];
+                } else {
+                    gh[v] += W8(n) * sd[(6 * v + 0) * N3 + f];
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int fi = at - (q - 2);   // this cell is limiter point q of face fi
+            if (fi < lo || fi > hi) continue;
+            const long f = c - (q - 2) * s;
+            for (int v = 0; v < 8; v++) gn[v] += sd[(6 * v + 2 + q) * N3 + f];
+        }
+    }
+    for (int v = 0; v < 5; v++) {
+        aWH[v * N3 + c] = gh[v] + ((v >= 1 && v <= 3) ? av[v - 1] : 0.0);
+        aWN[v * N3 + c] = gn[v];
+    }
+    const double V = met[M_VOL * N3 + c];
+    for (int x = 0; x < 3; x++) {
+        aHB[x * N3 + c] = V * gh[5 + x] + (lfm ? 0.0 : gn[5 + x]);
+        if (lfm) aSSB[x * N3 + c] += gn[5 + x];
+    }
+}
+
+// Edge EMF transpose (k_emf), pass 1: per edge, the pieces of
+// edge_emf_adj_s into sc (16 planes for this edge direction: sv 3, sl1,
+// sr1, anq1 4, sl2, sr2, anq2 4, aDc), gathered into the face velocities,
+// the half-state and limiter face fluxes and the half-state density by
+// k_emf_adj_gather.
 template <int ED, int RM, bool BG, bool BORIS>
-__global__ __launch_bounds__(256) void k_emf_adj(Geo g, MParm m, Box eb, const Clock *clk,
+__global__ __launch_bounds__(256, 2) void k_emf_adj(Geo g, MParm m, Box eb, const Clock *clk,
                                                  const double *__restrict__ H,
                                                  const double *__restrict__ S,
                                                  const double *__restrict__ Vf,
                                                  const double *__restrict__ met,
                                                  const double *__restrict__ bg,
                                                  const double *__restrict__ aE,
-                                                 double *__restrict__ aVf,
-                                                 double *__restrict__ aHG,
-                                                 double *__restrict__ aHMF,
-                                                 double *__restrict__ aSS)
+                                                 double *__restrict__ sc)
 {
     POINT_IN(eb);
     constexpr int T1 = (ED == 0) ? 1 : (ED == 1) ? 2 : 0;
@@ -688,27 +762,96 @@ __global__ __launch_bounds__(256) void k_emf_adj(Geo g, MParm m, Box eb, const C
     in.b02 = BG ? bg[(B_EDGB0 + 1 + 2 * ED) * N3 + c] : 0.0;
     in.edge = met[(M_EDGE + ED) * N3 + c];
     in.cap = m.CFL * in.edge / clk->dt;
-    EdgeAdj o = {};
-    edge_emf_adj<RM, BORIS>(m, in, aE[ED * N3 + c], o);
-    double *aL = m.doLFM ? aSS + S_MF * N3 : aHMF;
-    for (int n = 0; n < 8; n++) {
-        const long q1 = c + (n - 4) * s2, q2 = c + (n - 4) * s1;
-        for (int d = 0; d < 3; d++) AADD(aVf + (3 * T1 + d) * N3 + q1, o.V1[d][n]);
-        AADD(aHMF + T1 * N3 + q1, o.mH1[n]);
-        AADD(aHMF + T2 * N3 + q2, o.mH2[n]);
+    EdgeAdjS o;
+    edge_emf_adj_s<RM, BORIS>(m, in, aE[ED * N3 + c], o);
+    const double v[16] = {o.sv[0], o.sv[1], o.sv[2], o.sl1, o.sr1, o.anq1[0], o.anq1[1], o.anq1[2], o.anq1[3],
+                          o.sl2, o.sr2, o.anq2[0], o.anq2[1], o.anq2[2], o.anq2[3], o.aDc};
+    for (int q = 0; q < 16; q++) sc[q * N3 + c] = v[q];
+}
+
+// Edge EMF transpose, pass 2: one thread per face position x (every
+// direction T) sums what the edges whose stencils hold it give it:
+// the corner velocities of the edges of direction T-1 along T+1 (aVf,
+// WRITTEN for every face), their edge-B reconstructions of the T fluxes
+// and those of the edges of direction T+1 along T+2 (added to aHMF, the
+// limiter parts to aSSMF with the LFM limiter); and, as a cell, the corner
+// density of the four edges of each direction around it (added to aHG).
+// sc: 16 planes per edge direction from k_emf_adj; edges exist in eb.
+__device__ __forceinline__ bool in_box(const Box &b, int i, int j, int k)
+{
+    return i >= b.i0 && i <= b.i1 && j >= b.j0 && j <= b.j1 && k >= b.k0 && k <= b.k1;
+}
+
+template <int RM>
+__global__ __launch_bounds__(256) void k_emf_adj_gather(Geo g, Box fb, EdgeBoxes eb, const double *__restrict__ sc,
+                                                        const double *__restrict__ met, int lfm,
+                                                        double *__restrict__ aVf, double *__restrict__ aHMF,
+                                                        double *__restrict__ aHG, double *__restrict__ aSSMF)
+{
+    POINT_IN(fb);
+    const long N3 = g.N3;
+    const long c = IX(g, i, j, k);
+    const long st[3] = {1, (long)g.P, g.PP};
+#pragma unroll
+    for (int T = 0; T < 3; T++) {
+        const int EDa = (T + 2) % 3, Da = (T + 1) % 3;   // edges with T1 = T, stencil along T2
+        const int EDb = (T + 1) % 3, Db = (T + 2) % 3;   // edges with T2 = T, stencil along T1
+        const double *sa = sc + (long)16 * EDa * N3, *sb = sc + (long)16 * EDb * N3;
+        double sv[3] = {0.0, 0.0, 0.0}, sh = 0.0, sn = 0.0;
+#pragma unroll
+        for (int n = 0; n < 8; n++) {
+            int e3[3] = {i, j, k};
+            e3[Da] -= n - 4;
+            if (in_box(eb.e[EDa], e3[0], e3[1], e3[2])) {
+                const long e = c - (n - 4) * st[Da];
+                for (int d = 0; d < 3; d++) sv[d] += W8(n) * sa[d * N3 + e];
+                sh += stencil_adj<RM>(n, sa[3 * N3 + e], sa[4 * N3 + e]);
+            }
+            int e4[3] = {i, j, k};
+            e4[Db] -= n - 4;
+            if (in_box(eb.e[EDb], e4[0], e4[1], e4[2])) {
+                const long e = c - (n - 4) * st[Db];
+                sh += stencil_adj<RM>(n, sb[9 * N3 + e], sb[10 * N3 + e]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            int e3[3] = {i, j, k};
+            e3[Da] -= q - 2;
+            if (in_box(eb.e[EDa], e3[0], e3[1], e3[2])) sn += sa[(5 + q) * N3 + c - (q - 2) * st[Da]];
+            int e4[3] = {i, j, k};
+            e4[Db] -= q - 2;
+            if (in_box(eb.e[EDb], e4[0], e4[1], e4[2])) sn += sb[(11 + q) * N3 + c - (q - 2) * st[Db]];
+        }
+        const double A = met[(M_FACE + T) * N3 + c], Am = dmax(A, GAM_TINY);
+        for (int d = 0; d < 3; d++) aVf[(3 * T + d) * N3 + c] = A * sv[d];
+        aHMF[T * N3 + c] += (A / Am) * sh + (lfm ? 0.0 : sn / Am);
+        if (lfm) aSSMF[T * N3 + c] += sn / Am;
     }
-    for (int n = 0; n < 4; n++) {
-        AADD(aL + T1 * N3 + c + (n - 2) * s2, o.mL1[n]);
-        AADD(aL + T2 * N3 + c + (n - 2) * s1, o.mL2[n]);
-        AADD(aHG + DEN * N3 + dc[n], o.D4[n]);
+    if (i <= g.ieg && j <= g.jeg && k <= g.keg) {
+        double sd = 0.0;
+#pragma unroll
+        for (int ED = 0; ED < 3; ED++) {
+            const int E1 = (ED == 0) ? 1 : 0, E2 = (ED == 2) ? 1 : 2;
+            for (int ab = 0; ab < 4; ab++) {
+                int e3[3] = {i, j, k};
+                e3[E1] += ab & 1;
+                e3[E2] += ab >> 1;
+                if (in_box(eb.e[ED], e3[0], e3[1], e3[2]))
+                    sd += sc[(16 * ED + 15) * N3 + c + (ab & 1) * st[E1] + (ab >> 1) * st[E2]];
+            }
+        }
+        aHG[DEN * N3 + c] += 0.25 * sd;
     }
 }
 
-// Mom2Face transpose (k_vface): face velocities of direction d from the
-// volume-weighted half-state velocities of the 8 cells along d.
-__global__ __launch_bounds__(256) void k_vface_adj(Geo g, Box cb, const double *__restrict__ met,
-                                                   const double *__restrict__ aVf,
-                                                   double *__restrict__ aWH)
+// Mom2Face (k_vface): face velocity of direction d at face f =
+// sum_n W8(n) V_n v_n / sum_n W8(n) V_n over the 8 cells along d (cells
+// past the k ends of the grid left out).  rv[d] = that normalisation,
+// 1 / max(sum_n W8(n) V_n, tiny), at each face k_vface writes (0
+// elsewhere); geometry only, formed once per adjoint run.
+__global__ __launch_bounds__(256) void k_vface_rv(Geo g, Box cb, const double *__restrict__ met,
+                                                  double *__restrict__ rv)
 {
     POINT_IN(cb);
     const long N3 = g.N3;
@@ -717,21 +860,17 @@ __global__ __launch_bounds__(256) void k_vface_adj(Geo g, Box cb, const double *
     for (int d = 0; d < 3; d++) {
         const bool on = d == 0 ? (i >= g.is && i <= g.ie + 1)
                       : d == 1 ? (j >= g.js && j <= g.je + 1) : (k >= g.ks && k <= g.ke + 1);
-        if (!on) continue;
-        const long s = d == 0 ? 1 : d == 1 ? (long)g.P : g.PP;
-        double v8[8];
-        bool in8[8];
-        for (int n = 0; n < 8; n++) {
-            const int kk = k + (n - 4);
-            in8[n] = d != 2 || (kk >= g.ksg && kk <= g.keg);   // the k stencil stops at the grid
-            v8[n] = in8[n] ? vol[c + (n - 4) * s] : 0.0;
+        double r = 0.0;
+        if (on) {
+            const long s = d == 0 ? 1 : d == 1 ? (long)g.P : g.PP;
+            double v8[8];
+            for (int n = 0; n < 8; n++) {
+                const int kk = k + (n - 4);
+                v8[n] = (d != 2 || (kk >= g.ksg && kk <= g.keg)) ? vol[c + (n - 4) * s] : 0.0;
+            }
+            r = 1.0 / dmax(wsum8([&](int n) { return v8[n]; }), GAM_TINY);
         }
-        const double r = 1.0 / dmax(wsum8([&](int n) { return v8[n]; }), GAM_TINY);
-        for (int x = 0; x < 3; x++) {
-            const double a = aVf[(3 * d + x) * N3 + c] * r;
-            for (int n = 0; n < 8; n++)
-                if (in8[n]) AADD(aWH + (1 + x) * N3 + c + (n - 4) * s, W8(n) * a);
-        }
+        rv[d * N3 + c] = r;
     }
 }
 
@@ -959,12 +1098,11 @@ void launch_update_adj_mhd(hipStream_t s, const Geo &g, const MParm &m, const Bo
 template <int D, int RM>
 static void flux_adj_mhd_r(hipStream_t s, const Geo &g, const MParm &m, const Box &fb, const double *H,
                            const double *S, const double *met, const double *bg, const double *ag,
-                           const double *amf, double *aWH, double *aWN, double *aHB, double *aHMF, double *aHG,
-                           double *aSS)
+                           const double *amf, double *sc, double *aHMF, double *aHG)
 {
 #define FLUXADJ(BG, BO)                                                                                      \
     hipLaunchKernelGGL((k_flux_adj_mhd<D, RM, BG, BO>), agrid(fb), dim3(ABX, ABY), 0, s, g, m, fb, H, S, met, \
-                       bg, ag, amf, aWH, aWN, aHB, aHMF, aHG, aSS)
+                       bg, ag, amf, sc, aHMF, aHG)
     if (m.doBG && m.doBoris) FLUXADJ(true, true);
     else if (m.doBG) FLUXADJ(true, false);
     else if (m.doBoris) FLUXADJ(false, true);
@@ -975,39 +1113,50 @@ static void flux_adj_mhd_r(hipStream_t s, const Geo &g, const MParm &m, const Bo
 template <int D>
 static void flux_adj_mhd_d(hipStream_t s, const Geo &g, const MParm &m, const Box &fb, const double *H,
                            const double *S, const double *met, const double *bg, const double *ag,
-                           const double *amf, double *aWH, double *aWN, double *aHB, double *aHMF, double *aHG,
-                           double *aSS)
+                           const double *amf, double *sc, double *aHMF, double *aHG)
 {
-    if (m.rmeth) flux_adj_mhd_r<D, 1>(s, g, m, fb, H, S, met, bg, ag, amf, aWH, aWN, aHB, aHMF, aHG, aSS);
-    else flux_adj_mhd_r<D, 0>(s, g, m, fb, H, S, met, bg, ag, amf, aWH, aWN, aHB, aHMF, aHG, aSS);
+    if (m.rmeth) flux_adj_mhd_r<D, 1>(s, g, m, fb, H, S, met, bg, ag, amf, sc, aHMF, aHG);
+    else flux_adj_mhd_r<D, 0>(s, g, m, fb, H, S, met, bg, ag, amf, sc, aHMF, aHG);
 }
 
 void launch_flux_adj_mhd(hipStream_t s, int d, const Geo &g, const MParm &m, const Box &fb, const double *H,
                          const double *S, const double *met, const double *bg, const double *ag,
-                         const double *amf, double *aWH, double *aWN, double *aHB, double *aHMF, double *aHG,
-                         double *aSS)
+                         const double *amf, double *sc, double *aHMF, double *aHG)
 {
-    if (d == 0) flux_adj_mhd_d<0>(s, g, m, fb, H, S, met, bg, ag, amf, aWH, aWN, aHB, aHMF, aHG, aSS);
-    else if (d == 1) flux_adj_mhd_d<1>(s, g, m, fb, H, S, met, bg, ag, amf, aWH, aWN, aHB, aHMF, aHG, aSS);
-    else flux_adj_mhd_d<2>(s, g, m, fb, H, S, met, bg, ag, amf, aWH, aWN, aHB, aHMF, aHG, aSS);
+    if (d == 0) flux_adj_mhd_d<0>(s, g, m, fb, H, S, met, bg, ag, amf, sc, aHMF, aHG);
+    else if (d == 1) flux_adj_mhd_d<1>(s, g, m, fb, H, S, met, bg, ag, amf, sc, aHMF, aHG);
+    else flux_adj_mhd_d<2>(s, g, m, fb, H, S, met, bg, ag, amf, sc, aHMF, aHG);
+}
+
+void launch_recon_adj_gather(hipStream_t s, const Geo &g, const MParm &m, const double *sc, const double *rv,
+                             const double *aVf, const double *met, double *aWH, double *aWN, double *aHB,
+                             double *aSSB)
+{
+    const Box cb = cells_of(g);
+    if (m.rmeth)
+        hipLaunchKernelGGL(k_recon_adj_gather<1>, agrid(cb), dim3(ABX, ABY), 0, s, g, cb, sc, rv, aVf, met,
+                           m.doLFM ? 1 : 0, aWH, aWN, aHB, aSSB);
+    else
+        hipLaunchKernelGGL(k_recon_adj_gather<0>, agrid(cb), dim3(ABX, ABY), 0, s, g, cb, sc, rv, aVf, met,
+                           m.doLFM ? 1 : 0, aWH, aWN, aHB, aSSB);
 }
 
 template <int ED, int RM, bool BG, bool BO>
 static void emf_adj_t(hipStream_t s, const Geo &g, const MParm &m, const Box &eb, const Clock *clk,
                       const double *H, const double *S, const double *Vf, const double *met, const double *bg,
-                      const double *aE, double *aVf, double *aHG, double *aHMF, double *aSS)
+                      const double *aE, double *sc)
 {
     hipLaunchKernelGGL((k_emf_adj<ED, RM, BG, BO>), agrid(eb), dim3(ABX, ABY), 0, s, g, m, eb, clk, H, S, Vf, met,
-                       bg, aE, aVf, aHG, aHMF, aSS);
+                       bg, aE, sc + (size_t)16 * ED * g.N3);
 }
 
 template <int ED>
 static void emf_adj_d(hipStream_t s, const Geo &g, const MParm &m, const Box &eb, const Clock *clk,
                       const double *H, const double *S, const double *Vf, const double *met, const double *bg,
-                      const double *aE, double *aVf, double *aHG, double *aHMF, double *aSS)
+                      const double *aE, double *sc)
 {
     const int sel = (m.rmeth ? 4 : 0) + (m.doBG ? 2 : 0) + (m.doBoris ? 1 : 0);
-#define EMFADJ(RM, BG, BO) emf_adj_t<ED, RM, BG, BO>(s, g, m, eb, clk, H, S, Vf, met, bg, aE, aVf, aHG, aHMF, aSS)
+#define EMFADJ(RM, BG, BO) emf_adj_t<ED, RM, BG, BO>(s, g, m, eb, clk, H, S, Vf, met, bg, aE, sc)
     switch (sel) {
     case 0: EMFADJ(0, false, false); break;
     case 1: EMFADJ(0, false, true); break;
@@ -1023,17 +1172,31 @@ static void emf_adj_d(hipStream_t s, const Geo &g, const MParm &m, const Box &eb
 
 void launch_emf_adj(hipStream_t s, int ed, const Geo &g, const MParm &m, const Box &eb, const Clock *clk,
                     const double *H, const double *S, const double *Vf, const double *met, const double *bg,
-                    const double *aE, double *aVf, double *aHG, double *aHMF, double *aSS)
+                    const double *aE, double *sc)
 {
-    if (ed == 0) emf_adj_d<0>(s, g, m, eb, clk, H, S, Vf, met, bg, aE, aVf, aHG, aHMF, aSS);
-    else if (ed == 1) emf_adj_d<1>(s, g, m, eb, clk, H, S, Vf, met, bg, aE, aVf, aHG, aHMF, aSS);
-    else emf_adj_d<2>(s, g, m, eb, clk, H, S, Vf, met, bg, aE, aVf, aHG, aHMF, aSS);
+    if (ed == 0) emf_adj_d<0>(s, g, m, eb, clk, H, S, Vf, met, bg, aE, sc);
+    else if (ed == 1) emf_adj_d<1>(s, g, m, eb, clk, H, S, Vf, met, bg, aE, sc);
+    else emf_adj_d<2>(s, g, m, eb, clk, H, S, Vf, met, bg, aE, sc);
 }
 
-void launch_vface_adj(hipStream_t s, const Geo &g, const double *met, const double *aVf, double *aWH)
+void launch_emf_adj_gather(hipStream_t s, const Geo &g, const MParm &m, const Box *eboxes, const double *sc,
+                           const double *met, double *aVf, double *aHMF, double *aHG, double *aSSMF)
+{
+    const Box fb = faces_of(g);
+    EdgeBoxes eb;
+    for (int d = 0; d < 3; d++) eb.e[d] = eboxes[d];
+    if (m.rmeth)
+        hipLaunchKernelGGL(k_emf_adj_gather<1>, agrid(fb), dim3(ABX, ABY), 0, s, g, fb, eb, sc, met,
+                           m.doLFM ? 1 : 0, aVf, aHMF, aHG, aSSMF);
+    else
+        hipLaunchKernelGGL(k_emf_adj_gather<0>, agrid(fb), dim3(ABX, ABY), 0, s, g, fb, eb, sc, met,
+                           m.doLFM ? 1 : 0, aVf, aHMF, aHG, aSSMF);
+}
+
+void launch_vface_rv(hipStream_t s, const Geo &g, const double *met, double *rv)
 {
     const Box cb = cells_of(g);
-    hipLaunchKernelGGL(k_vface_adj, agrid(cb), dim3(ABX, ABY), 0, s, g, cb, met, aVf, aWH);
+    hipLaunchKernelGGL(k_vface_rv, agrid(cb), dim3(ABX, ABY), 0, s, g, cb, met, rv);
 }
 
 void launch_predict_adj_mhd(hipStream_t s, const Geo &g, const MParm &m, const Clock *clk, const double *S,

@@ -1203,6 +1203,15 @@ extern "C" int gam_autotune(gam_ctx *c, int reps)
 // states on a tape) and then runs the adjoint kernels of its steps from
 // last to first.
 enum { ADJ_SET = 11 };   // planes of an adjoint state set: gas 5, face flux 3, cell B 3
+// planes of one step's forward intermediates: half state (H_NPL), face
+// velocities 9, gas fluxes 15, magnetic fluxes 9
+enum { IT_VF = H_NPL, IT_GF = IT_VF + 9, IT_MF = IT_GF + 15, IT_NPL = IT_MF + 9 };
+
+// One step's forward intermediates (the context's own buffers, or a slot
+// of the interval tape).
+struct StepIT {
+    double *H, *vf, *gflx, *mflx;
+};
 
 struct gam_adj {
     gam_ctx *c;
@@ -1214,11 +1223,15 @@ struct gam_adj {
     Clock *clkh;             // clock before step n, n = 0..nsteps
     Clock *oclk;             // clock at the start with t = oState%time
     double *tape;            // K + 2 State buffers of one interval
+    double *itape;           // K steps' forward intermediates (IT_NPL planes each), or null
     double *lam;             // three adjoint state sets (ADJ_SET planes each)
     double *ag, *amf;        // adjoints of gFlx (15) and mFlx (9)
     double *awh, *awn, *ahg; // adjoints of H_WH, H_WN, H_GAS (5 each)
     double *ahmf, *ahb;      // adjoints of H_MF, H_B (3 each)
     double *avf, *aur, *adm, *ae;   // Vf (9), Reynolds gas (5), dM (3), E (3)
+    double *rv;              // Mom2Face normalisation per face direction (3)
+    double *fsc;             // flux-transpose scratch, 48 planes per face direction (MHD)
+    double *esc;             // EMF-transpose scratch, 16 planes per edge direction (MHD)
     double *wgt;             // objective weights (1 plane)
     double *part;            // objective partial sums (1 plane)
     int nw;                  // inflow series length (0: no inflow BC)
@@ -1231,8 +1244,9 @@ extern "C" int gam_adj_destroy(gam_adj *a)
 {
     if (!a) return GAM_OK;
     if (a->c && a->c->stream) hipStreamSynchronize(a->c->stream);
-    double *bufs[] = {a->ck, a->fin, a->tape, a->lam, a->ag, a->amf, a->awh, a->awn, a->ahg, a->ahmf,
-                      a->ahb, a->avf, a->aur, a->adm, a->ae, a->wgt, a->part, a->gwq, a->gwb};
+    double *bufs[] = {a->ck, a->fin, a->tape, a->itape, a->lam, a->ag, a->amf, a->awh, a->awn, a->ahg, a->ahmf,
+                      a->ahb, a->avf, a->aur, a->adm, a->ae, a->wgt, a->part, a->gwq, a->gwb, a->rv,
+                      a->fsc, a->esc};
     for (double *b : bufs) if (b) hipFree(b);
     if (a->clkh) hipFree(a->clkh);
     if (a->oclk) hipFree(a->oclk);
@@ -1278,6 +1292,9 @@ extern "C" int gam_adj_create(gam_adj **out, gam_ctx *c, int nsteps, int K)
     AALLOC(a->ae, 3 * pl);
     AALLOC(a->wgt, pl);
     AALLOC(a->part, pl);
+    AALLOC(a->rv, 3 * pl);
+    if (c->mp.doMHD) AALLOC(a->fsc, 3 * 48 * pl);
+    if (c->mp.doMHD) AALLOC(a->esc, 3 * 16 * pl);
     for (int sd = 0; sd < 6; sd++)
         if (c->model.bc[sd] == GAM_BC_WIND) a->nw = c->wd.n;
     if (a->nw) {
@@ -1285,11 +1302,25 @@ extern "C" int gam_adj_create(gam_adj **out, gam_ctx *c, int nsteps, int K)
         AALLOC(a->gwb, (size_t)a->nw * 3 * sizeof(double));
     }
 #undef AALLOC
+    // the interval's forward intermediates, kept where HBM allows (otherwise
+    // each adjoint step recomputes its predictor, face velocities and fluxes)
+    size_t mfree = 0, mtot = 0;
+    if (e == hipSuccess) e = hipMemGetInfo(&mfree, &mtot);
+    const size_t itb = (size_t)K * IT_NPL * pl;
+    if (e == hipSuccess && itb + ((size_t)4 << 30) < mfree) {
+        e = hipMalloc((void **)&a->itape, itb);
+        if (e == hipSuccess) e = hipMemset(a->itape, 0, itb);
+    }
     if (e == hipSuccess) e = hipMemset(a->lam, 0, 3 * ADJ_SET * pl);
     if (e == hipSuccess) e = hipMemset(a->ag, 0, 15 * pl);
     if (e == hipSuccess) e = hipMemset(a->amf, 0, 9 * pl);
     if (e == hipSuccess) e = hipMemset(a->aur, 0, 5 * pl);
     if (e == hipSuccess) e = hipMemset(a->adm, 0, 3 * pl);
+    if (e == hipSuccess) {
+        launch_vface_rv(c->stream, c->g, c->met, a->rv);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    }
     for (int q = 0; e == hipSuccess && q < 4; q++) e = hipEventCreate(&a->ev[q]);
     if (e != hipSuccess) {
         rc = fail(GAM_E_HIP, "gam_adj_create: %s", hipGetErrorString(e));
@@ -1386,6 +1417,13 @@ static int adj_restore_final(gam_adj *a)
     return GAM_OK;
 }
 
+static StepIT it_slot(gam_adj *a, int q)
+{
+    const long N3 = a->c->g.N3;
+    double *b = a->itape + (size_t)q * IT_NPL * N3;
+    return StepIT{b, b + (size_t)IT_VF * N3, b + (size_t)IT_GF * N3, b + (size_t)IT_MF * N3};
+}
+
 // Transposes of the boundary pass on one adjoint set, sides OUTK to INI;
 // S = the state the pass ran on (its active cells and faces), bclk = the
 // clock it ran at.
@@ -1404,8 +1442,9 @@ static void adj_bcs(gam_adj *a, double *l, const double *S, const Clock *bclk)
 // output oState = input State) in, lB (input State) and lC (input oState)
 // out.  Sn, On = the step's input State and oState, Sn1 its output State;
 // clk its clock, bclk the clock after it (its boundary pass's time).
-static void adj_step(gam_adj *a, const double *Sn, const double *On, const double *Sn1, const Clock *clk,
-                     const Clock *bclk, double *lA, double *lB, double *lC)
+// it: the step's taped forward intermediates, or null to recompute them.
+static void adj_step(gam_adj *a, const StepIT *it, const double *Sn, const double *On, const double *Sn1,
+                     const Clock *clk, const Clock *bclk, double *lA, double *lB, double *lC)
 {
     gam_ctx *c = a->c;
     const Geo &g = c->g;
@@ -1414,37 +1453,42 @@ static void adj_step(gam_adj *a, const double *Sn, const double *On, const doubl
     const size_t pl = (size_t)N3 * sizeof(double);
     Box eb[3];
     edge_boxes(g, eb);
+    const double *H = it ? it->H : c->H, *vf = it ? it->vf : c->vf;
+    const double *gflx = it ? it->gflx : c->gflx, *mflx = it ? it->mflx : c->mflx;
     // the forward intermediates the transposes read
-    launch_predict(s, c->plan.v[TS_PREDICT], g, c->mp, face_box(g), clk, Sn, On, c->met, c->H);
-    if (c->mp.doMHD) {
-        launch_vface(s, g, c->mp, c->H, c->met, c->vf);
-        for (int d = 0; d < 3; d++) {
-            Box fb = act_box(g);
-            if (d == 0) fb.i1 += 1; else if (d == 1) fb.j1 += 1; else fb.k1 += 1;
-            launch_flux(s, d, c->plan.v[TS_FLUX_I + d], g, c->mp, fb, c->H, Sn, c->met, c->bg, c->gflx, c->mflx);
+    if (!it) {
+        launch_predict(s, c->plan.v[TS_PREDICT], g, c->mp, face_box(g), clk, Sn, On, c->met, c->H);
+        if (c->mp.doMHD) {
+            launch_vface(s, g, c->mp, c->H, c->met, c->vf);
+            for (int d = 0; d < 3; d++) {
+                Box fb = act_box(g);
+                if (d == 0) fb.i1 += 1; else if (d == 1) fb.j1 += 1; else fb.k1 += 1;
+                launch_flux(s, d, c->plan.v[TS_FLUX_I + d], g, c->mp, fb, c->H, Sn, c->met, c->bg, c->gflx,
+                            c->mflx);
+            }
         }
+    }
+    if (c->mp.doMHD) {
         // ghost bFlux2Fld, boundary pass, update (with its cell B)
         launch_cellb_adj(s, g, 0, lA, c->met);
         adj_bcs(a, lA, Sn1, bclk);
         const Box ctb = mkbox(g.is, g.ie + 1, g.js, g.je + 1, g.ks, g.ke + 1);
-        launch_update_adj_mhd(s, g, c->mp, act_box(g), ctb, eb, clk, Sn, Sn1, c->gflx, c->mflx, c->met, c->bg,
+        launch_update_adj_mhd(s, g, c->mp, act_box(g), ctb, eb, clk, Sn, Sn1, gflx, mflx, c->met, c->bg,
                               lA, lB, a->aur, a->adm, a->ag, a->amf, a->ae);
-        hipMemsetAsync(a->awh, 0, 5 * pl, s);
-        hipMemsetAsync(a->awn, 0, 5 * pl, s);
+        // aVf, aWH, aWN and aHB are written whole by the gathers below
         hipMemsetAsync(a->ahg, 0, 5 * pl, s);
         hipMemsetAsync(a->ahmf, 0, 3 * pl, s);
-        hipMemsetAsync(a->ahb, 0, 3 * pl, s);
-        hipMemsetAsync(a->avf, 0, 9 * pl, s);
         for (int d = 0; d < 3; d++) {
             Box fb = act_box(g);
             if (d == 0) fb.i1 += 1; else if (d == 1) fb.j1 += 1; else fb.k1 += 1;
-            launch_flux_adj_mhd(s, d, g, c->mp, fb, c->H, Sn, c->met, c->bg, a->ag, a->amf, a->awh, a->awn,
-                                a->ahb, a->ahmf, a->ahg, lB);
+            launch_flux_adj_mhd(s, d, g, c->mp, fb, H, Sn, c->met, c->bg, a->ag, a->amf,
+                                a->fsc + (size_t)48 * d * N3, a->ahmf, a->ahg);
         }
         for (int ed = 0; ed < 3; ed++)
-            launch_emf_adj(s, ed, g, c->mp, eb[ed], clk, c->H, Sn, c->vf, c->met, c->bg, a->ae, a->avf, a->ahg,
-                           a->ahmf, lB);
-        launch_vface_adj(s, g, c->met, a->avf, a->awh);
+            launch_emf_adj(s, ed, g, c->mp, eb[ed], clk, H, Sn, vf, c->met, c->bg, a->ae, a->esc);
+        launch_emf_adj_gather(s, g, c->mp, eb, a->esc, c->met, a->avf, a->ahmf, a->ahg, lB + S_MF * N3);
+        launch_recon_adj_gather(s, g, c->mp, a->fsc, a->rv, a->avf, c->met, a->awh, a->awn, a->ahb,
+                                lB + S_B * N3);
         hipMemsetAsync(lC + S_B * N3, 0, 3 * pl, s);
         launch_predict_adj_mhd(s, g, c->mp, clk, Sn, On, c->met, a->ahg, a->awh, a->awn, a->ahb, a->ahmf, lB, lC);
     } else {
@@ -1455,7 +1499,7 @@ static void adj_step(gam_adj *a, const double *Sn, const double *On, const doubl
         for (int d = 0; d < 3; d++) {
             Box fb = act_box(g);
             if (d == 0) fb.i1 += 1; else if (d == 1) fb.j1 += 1; else fb.k1 += 1;
-            launch_flux_adj(s, d, g, c->mp, fb, c->H, c->met, a->ag, a->awh, a->awn);
+            launch_flux_adj(s, d, g, c->mp, fb, H, c->met, a->ag, a->awh, a->awn);
         }
         launch_predict_adj(s, g, c->mp, clk, Sn, On, c->met, a->awh, a->awn, lB, lC);
     }
@@ -1490,14 +1534,23 @@ extern "C" int gam_adj_backward(gam_adj *a, double *gGas, double *gGasO, double 
         if ((rc = adj_put(a, s0, ck))) return rc;
         // tape[0] = oState at s0, tape[1 + q] = State at step s0 + q, q = 0..s1-s0
         HIPCHK(hipMemcpyAsync(a->tape, ck + (size_t)S_NPL * N3, bytes, hipMemcpyDeviceToDevice, s));
+        const StepIT own = {c->H, c->vf, c->gflx, c->mflx};
         for (int q = 0; q <= s1 - s0; q++) {
             HIPCHK(hipMemcpyAsync(a->tape + (size_t)(1 + q) * S_NPL * N3, S_cur(c), bytes,
                                   hipMemcpyDeviceToDevice, s));
-            if (q < s1 - s0) enq_step(c);
+            if (q < s1 - s0) {
+                if (a->itape) {   // the step writes its intermediates into the tape slot
+                    const StepIT t = it_slot(a, q);
+                    c->H = t.H; c->vf = t.vf; c->gflx = t.gflx; c->mflx = t.mflx;
+                }
+                enq_step(c);
+            }
         }
+        c->H = own.H; c->vf = own.vf; c->gflx = own.gflx; c->mflx = own.mflx;
         for (int n = s1 - 1; n >= s0; n--) {
             const int q = n - s0;
-            adj_step(a, a->tape + (size_t)(1 + q) * S_NPL * N3, a->tape + (size_t)q * S_NPL * N3,
+            const StepIT its = a->itape ? it_slot(a, q) : own;
+            adj_step(a, a->itape ? &its : nullptr, a->tape + (size_t)(1 + q) * S_NPL * N3, a->tape + (size_t)q * S_NPL * N3,
                      a->tape + (size_t)(2 + q) * S_NPL * N3, a->clkh + n, a->clkh + n + 1, lA, lB, lC);
             double *t = lA;
             lA = lB;

@@ -123,14 +123,25 @@ void launch_update_adj_mhd(hipStream_t s, const Geo &g, const MParm &m, const Bo
                            const double *gflx, const double *mflx, const double *met, const double *bg,
                            double *aSn, double *aSo, double *aUr, double *adM, double *ag, double *amf,
                            double *aE);
+// pass 1 of the flux transpose: per face, the stencil scalars and limiter
+// adjoints of its 8 reconstructions into sc (48 planes), aBn into aHMF, the
+// Boris nuke term into aHG
 void launch_flux_adj_mhd(hipStream_t s, int d, const Geo &g, const MParm &m, const Box &fb, const double *H,
                          const double *S, const double *met, const double *bg, const double *ag,
-                         const double *amf, double *aWH, double *aWN, double *aHB, double *aHMF, double *aHG,
-                         double *aSS);
+                         const double *amf, double *sc, double *aHMF, double *aHG);
+// pass 2 (all three directions' sc, 48 planes each) fused with the Mom2Face
+// transpose: writes aWH, aWN, aHB of every cell
+void launch_recon_adj_gather(hipStream_t s, const Geo &g, const MParm &m, const double *sc, const double *rv,
+                             const double *aVf, const double *met, double *aWH, double *aWN, double *aHB,
+                             double *aSSB);
+// pass 1 of the edge EMF transpose: 16 planes per edge direction into sc
 void launch_emf_adj(hipStream_t s, int ed, const Geo &g, const MParm &m, const Box &eb, const Clock *clk,
                     const double *H, const double *S, const double *Vf, const double *met, const double *bg,
-                    const double *aE, double *aVf, double *aHG, double *aHMF, double *aSS);
-void launch_vface_adj(hipStream_t s, const Geo &g, const double *met, const double *aVf, double *aWH);
+                    const double *aE, double *sc);
+// pass 2: writes aVf of every face; adds to aHMF (or aSSMF), aHG's density
+void launch_emf_adj_gather(hipStream_t s, const Geo &g, const MParm &m, const Box *eboxes, const double *sc,
+                           const double *met, double *aVf, double *aHMF, double *aHG, double *aSSMF);
+void launch_vface_rv(hipStream_t s, const Geo &g, const double *met, double *rv);
 void launch_predict_adj_mhd(hipStream_t s, const Geo &g, const MParm &m, const Clock *clk, const double *S,
                             const double *O, const double *met, const double *aHG, const double *aWH,
                             const double *aWN, double *aHB, double *aHMF, double *aS, double *aO);

@@ -250,6 +250,30 @@ def constant_wind(rho=1.0, vx=-0.4, p=1.0e-3, bz=-0.05):
     return {"tW": tW, "Q": Q, "B": B, "Bx0": 0.0, "ByC": 0.0, "BzC": 0.0, "xyzW": (0.0, 0.0, 0.0)}
 
 
+def varying_wind(n=200, t0=-100.0, t1=100.0, seed=SEED0):
+    """An n-sample inflow series on [t0, t1] whose density, velocity,
+    pressure and IMF vary smoothly (a few harmonics each) around a
+    southward-IMF solar wind like constant_wind's; the flow is a little off
+    the x axis and ByC, BzC give the IMF a Bx."""
+    rng = np.random.default_rng(seed)
+    s = np.linspace(0.0, 2.0 * np.pi, n)
+
+    def wave(amp):
+        ph = rng.uniform(0, 2 * np.pi, 3)
+        return amp * (np.sin(s + ph[0]) + 0.5 * np.sin(2 * s + ph[1]) + 0.25 * np.sin(3 * s + ph[2])) / 1.75
+    Q = np.zeros((n, 5), order="F")
+    Q[:, 0] = 1.0 + wave(0.2)
+    Q[:, 1] = -0.4 + wave(0.05)
+    Q[:, 2] = wave(0.04)
+    Q[:, 3] = wave(0.03)
+    Q[:, 4] = 1.0e-3 * (1.0 + wave(0.3))
+    B = np.zeros((n, 3), order="F")
+    B[:, 1] = wave(0.02)
+    B[:, 2] = -0.05 + wave(0.02)
+    return {"tW": np.linspace(t0, t1, n), "Q": Q, "B": B, "Bx0": 0.01, "ByC": 0.1, "BzC": -0.05,
+            "xyzW": (0.0, 0.0, 0.0)}
+
+
 # Dipole moment giving |B0| = 1 at r = 2.5 on the equator, oriented like the
 # Earth's (equatorial B0 along +z): M0 = -(2.5)^3.
 SHELL_M0 = -(2.5 ** 3)
