@@ -31,6 +31,18 @@ static inline dim3 agrid(const Box &b)
     const int k = (b).k0 + (int)blockIdx.z;                  \
     if (i > (b).i1 || j > (b).j1) return;
 
+// i x k blocks (for stencils along k: the block's rows share stencil planes)
+static inline dim3 agrid_k(const Box &b)
+{
+    return dim3((unsigned)((b.i1 - b.i0 + ABX) / ABX), (unsigned)(b.j1 - b.j0 + 1),
+                (unsigned)((b.k1 - b.k0 + ABY) / ABY));
+}
+#define POINT_IN_K(b)                                        \
+    const int i = (b).i0 + (int)(blockIdx.x * ABX + threadIdx.x); \
+    const int j = (b).j0 + (int)blockIdx.y;                  \
+    const int k = (b).k0 + (int)(blockIdx.z * ABY + threadIdx.y); \
+    if (i > (b).i1 || k > (b).k1) return;
+
 __device__ __forceinline__ bool inside(const Box &b, int i, int j, int k)
 {
     return i >= b.i0 && i <= b.i1 && j >= b.j0 && j <= b.j1 && k >= b.k0 && k <= b.k1;
@@ -533,7 +545,7 @@ __global__ __launch_bounds__(256) void k_update_adj_faces(Geo g, Box fb, Box ab,
 // Boris mag-HOGS and nuke terms, L/R states of gas (H_WH, H_WN) and cell B
 // (H_B weighted by volume, limiter S_B), the normal flux Bn = H_MF / A.
 template <int D, int RM, bool BG, bool BORIS>
-__global__ __launch_bounds__(256, (D == 0) ? 2 : 1) void k_flux_adj_mhd(Geo g, MParm m, Box fb,
+__global__ __launch_bounds__(256, 2) void k_flux_adj_mhd(Geo g, MParm m, Box fb,
                                                       const double *__restrict__ H,
                                                       const double *__restrict__ S,
                                                       const double *__restrict__ met,
@@ -544,36 +556,48 @@ __global__ __launch_bounds__(256, (D == 0) ? 2 : 1) void k_flux_adj_mhd(Geo g, M
                                                       double *__restrict__ aHMF,
                                                       double *__restrict__ aHG)
 {
-    POINT_IN(fb);
+    const int i = fb.i0 + (int)(blockIdx.x * ABX + threadIdx.x);
+    const int j = fb.j0 + (int)(D == 2 ? blockIdx.y : blockIdx.y * ABY + threadIdx.y);
+    const int k = fb.k0 + (int)(D == 2 ? blockIdx.z * ABY + threadIdx.y : blockIdx.z);
+    if (i > fb.i1 || j > fb.j1 || k > fb.k1) return;
     const long N3 = g.N3;
     const long c = IX(g, i, j, k);
     const long s = STRIDE<D>(g);
+    // byte offsets of the 8 stencil points shared by every plane (as k_flux);
+    // along i plain indexing lets the compiler merge neighbouring points
+    unsigned o[8];
+#pragma unroll
+    for (int n = 0; n < 8; n++) o[n] = (unsigned)((c + (n - 4) * s) * 8);
+    auto LD = [&](const double *__restrict__ plane, int n) -> double {
+        if (D == 0) return plane[c + (n - 4)];
+        return ldp(plane, o[n]);
+    };
     double vol[8];
-    for (int n = 0; n < 8; n++) vol[n] = met[M_VOL * N3 + c + (n - 4) * s];
+    for (int n = 0; n < 8; n++) vol[n] = LD(met + M_VOL * N3, n);
     const double rVi = 1.0 / wsum8([&](int n) { return vol[n]; });
     double WL[5], WR[5], BL[3], BR[3];
     for (int v = 0; v < 5; v++) {
         double qv[8], nq[4];
-        for (int n = 0; n < 8; n++) qv[n] = H[(H_WH + v) * N3 + c + (n - 4) * s];
-        for (int n = 0; n < 4; n++) nq[n] = H[(H_WN + v) * N3 + c + (n - 2) * s];
+        for (int n = 0; n < 8; n++) qv[n] = LD(H + (H_WH + v) * N3, n);
+        for (int n = 0; n < 4; n++) nq[n] = LD(H + (H_WN + v) * N3, n + 2);
         recon_lr_w<RM>(m.pdmb, rVi, nq, qv, WL[v], WR[v]);
     }
     const double *Bl = m.doLFM ? S + S_B * N3 : H + H_B * N3;
     for (int x = 0; x < 3; x++) {
         double q[8], nq[4];
-        for (int n = 0; n < 8; n++) q[n] = H[(H_B + x) * N3 + c + (n - 4) * s];
-        for (int n = 0; n < 4; n++) nq[n] = Bl[x * N3 + c + (n - 2) * s];
+        for (int n = 0; n < 8; n++) q[n] = LD(H + (H_B + x) * N3, n);
+        for (int n = 0; n < 4; n++) nq[n] = LD(Bl + x * N3, n + 2);
         recon_lr<RM>(m.pdmb, vol, rVi, nq, q, BL[x], BR[x]);
     }
     double T[9];
-    for (int q = 0; q < 9; q++) T[q] = met[(M_TF + q + 9 * D) * N3 + c];
-    const double faB = met[(M_FACE + D) * N3 + c];
+    for (int q = 0; q < 9; q++) T[q] = ldp(met + (M_TF + q + 9 * D) * N3, o[4]);
+    const double faB = ldp(met + (M_FACE + D) * N3, o[4]);
     const double iFaB = 1.0 / faB;
-    const double Bn = H[(H_MF + D) * N3 + c] * iFaB;
+    const double Bn = ldp(H + (H_MF + D) * N3, o[4]) * iFaB;
     double B0[3] = {0, 0, 0}, B0n = 0.0;
     if (BG) {
-        for (int x = 0; x < 3; x++) B0[x] = bg[(B_FCB0 + x + 3 * D) * N3 + c];
-        B0n = bg[(B_BFLUX0 + D) * N3 + c] * iFaB;
+        for (int x = 0; x < 3; x++) B0[x] = ldp(bg + (B_FCB0 + x + 3 * D) * N3, o[4]);
+        B0n = ldp(bg + (B_BFLUX0 + D) * N3, o[4]) * iFaB;
     }
     // forward pieces the adjoint needs: VaD, bbD and the HOGS differences dW
     double mF[3] = {0, 0, 0}, bbD = 0.0, VaD = 0.0;
@@ -586,11 +610,11 @@ __global__ __launch_bounds__(256, (D == 0) ? 2 : 1) void k_flux_adj_mhd(Geo g, M
     double aF[5], adW[5], aM[3], aVaD = 0.0, abbD = 0.0;
     double aL[5] = {0, 0, 0, 0, 0}, aR[5] = {0, 0, 0, 0, 0}, abL[3] = {0, 0, 0}, abR[3] = {0, 0, 0}, aBn = 0.0;
     for (int v = 0; v < 5; v++) {
-        aF[v] = faB * ag[(v + 5 * D) * N3 + c];
+        aF[v] = faB * ldp(ag + (v + 5 * D) * N3, o[4]);
         adW[v] = -m.cHogH * VaD * aF[v];
         aVaD += -m.cHogH * dW[v] * aF[v];
     }
-    for (int x = 0; x < 3; x++) aM[x] = faB * amf[(x + 3 * D) * N3 + c];
+    for (int x = 0; x < 3; x++) aM[x] = faB * ldp(amf + (x + 3 * D) * N3, o[4]);
     if (BORIS) {
         // mag-HOGS: mF -= cHogM VaD bbD (WR - WL)_v / Ca^2
         for (int x = 0; x < 3; x++) {
@@ -622,22 +646,22 @@ __global__ __launch_bounds__(256, (D == 0) ? 2 : 1) void k_flux_adj_mhd(Geo g, M
     // limiter adjoints, gathered into the cells by k_recon_adj_gather
     for (int v = 0; v < 5; v++) {
         double qv[8], nq[4], anq[4] = {0, 0, 0, 0}, sl, sr;
-        for (int n = 0; n < 8; n++) qv[n] = H[(H_WH + v) * N3 + c + (n - 4) * s];
-        for (int n = 0; n < 4; n++) nq[n] = H[(H_WN + v) * N3 + c + (n - 2) * s];
+        for (int n = 0; n < 8; n++) qv[n] = LD(H + (H_WH + v) * N3, n);
+        for (int n = 0; n < 4; n++) nq[n] = LD(H + (H_WN + v) * N3, n + 2);
         recon_lr_w_adj_s<RM>(m.pdmb, rVi, nq, qv, aL[v], aR[v], anq, sl, sr);
-        sc[(6 * v + 0) * N3 + c] = sl;
-        sc[(6 * v + 1) * N3 + c] = sr;
-        for (int n = 0; n < 4; n++) sc[(6 * v + 2 + n) * N3 + c] = anq[n];
+        stp(sc + (6 * v + 0) * N3, o[4], sl);
+        stp(sc + (6 * v + 1) * N3, o[4], sr);
+        for (int n = 0; n < 4; n++) stp(sc + (6 * v + 2 + n) * N3, o[4], anq[n]);
     }
     for (int x = 0; x < 3; x++) {
         double qv[8], nq[4], anq[4] = {0, 0, 0, 0}, sl, sr;
-        for (int n = 0; n < 8; n++) qv[n] = vol[n] * H[(H_B + x) * N3 + c + (n - 4) * s];
-        for (int n = 0; n < 4; n++) nq[n] = Bl[x * N3 + c + (n - 2) * s];
+        for (int n = 0; n < 8; n++) qv[n] = vol[n] * LD(H + (H_B + x) * N3, n);
+        for (int n = 0; n < 4; n++) nq[n] = LD(Bl + x * N3, n + 2);
         recon_lr_w_adj_s<RM>(m.pdmb, rVi, nq, qv, abL[x], abR[x], anq, sl, sr);
         const int v = 5 + x;
-        sc[(6 * v + 0) * N3 + c] = sl;
-        sc[(6 * v + 1) * N3 + c] = sr;
-        for (int n = 0; n < 4; n++) sc[(6 * v + 2 + n) * N3 + c] = anq[n];
+        stp(sc + (6 * v + 0) * N3, o[4], sl);
+        stp(sc + (6 * v + 1) * N3, o[4], sr);
+        for (int n = 0; n < 4; n++) stp(sc + (6 * v + 2 + n) * N3, o[4], anq[n]);
     }
 }
 
@@ -777,11 +801,6 @@ __global__ __launch_bounds__(256, 2) void k_emf_adj(Geo g, MParm m, Box eb, cons
 // limiter parts to aSSMF with the LFM limiter); and, as a cell, the corner
 // density of the four edges of each direction around it (added to aHG).
 // sc: 16 planes per edge direction from k_emf_adj; edges exist in eb.
-__device__ __forceinline__ bool in_box(const Box &b, int i, int j, int k)
-{
-    return i >= b.i0 && i <= b.i1 && j >= b.j0 && j <= b.j1 && k >= b.k0 && k <= b.k1;
-}
-
 template <int RM>
 __global__ __launch_bounds__(256) void k_emf_adj_gather(Geo g, Box fb, EdgeBoxes eb, const double *__restrict__ sc,
                                                         const double *__restrict__ met, int lfm,
@@ -802,14 +821,14 @@ __global__ __launch_bounds__(256) void k_emf_adj_gather(Geo g, Box fb, EdgeBoxes
         for (int n = 0; n < 8; n++) {
             int e3[3] = {i, j, k};
             e3[Da] -= n - 4;
-            if (in_box(eb.e[EDa], e3[0], e3[1], e3[2])) {
+            if (inside(eb.e[EDa], e3[0], e3[1], e3[2])) {
                 const long e = c - (n - 4) * st[Da];
                 for (int d = 0; d < 3; d++) sv[d] += W8(n) * sa[d * N3 + e];
                 sh += stencil_adj<RM>(n, sa[3 * N3 + e], sa[4 * N3 + e]);
             }
             int e4[3] = {i, j, k};
             e4[Db] -= n - 4;
-            if (in_box(eb.e[EDb], e4[0], e4[1], e4[2])) {
+            if (inside(eb.e[EDb], e4[0], e4[1], e4[2])) {
                 const long e = c - (n - 4) * st[Db];
                 sh += stencil_adj<RM>(n, sb[9 * N3 + e], sb[10 * N3 + e]);
             }
@@ -818,10 +837,10 @@ __global__ __launch_bounds__(256) void k_emf_adj_gather(Geo g, Box fb, EdgeBoxes
         for (int q = 0; q < 4; q++) {
             int e3[3] = {i, j, k};
             e3[Da] -= q - 2;
-            if (in_box(eb.e[EDa], e3[0], e3[1], e3[2])) sn += sa[(5 + q) * N3 + c - (q - 2) * st[Da]];
+            if (inside(eb.e[EDa], e3[0], e3[1], e3[2])) sn += sa[(5 + q) * N3 + c - (q - 2) * st[Da]];
             int e4[3] = {i, j, k};
             e4[Db] -= q - 2;
-            if (in_box(eb.e[EDb], e4[0], e4[1], e4[2])) sn += sb[(11 + q) * N3 + c - (q - 2) * st[Db]];
+            if (inside(eb.e[EDb], e4[0], e4[1], e4[2])) sn += sb[(11 + q) * N3 + c - (q - 2) * st[Db]];
         }
         const double A = met[(M_FACE + T) * N3 + c], Am = dmax(A, GAM_TINY);
         for (int d = 0; d < 3; d++) aVf[(3 * T + d) * N3 + c] = A * sv[d];
@@ -837,7 +856,7 @@ __global__ __launch_bounds__(256) void k_emf_adj_gather(Geo g, Box fb, EdgeBoxes
                 int e3[3] = {i, j, k};
                 e3[E1] += ab & 1;
                 e3[E2] += ab >> 1;
-                if (in_box(eb.e[ED], e3[0], e3[1], e3[2]))
+                if (inside(eb.e[ED], e3[0], e3[1], e3[2]))
                     sd += sc[(16 * ED + 15) * N3 + c + (ab & 1) * st[E1] + (ab >> 1) * st[E2]];
             }
         }
@@ -1101,7 +1120,8 @@ static void flux_adj_mhd_r(hipStream_t s, const Geo &g, const MParm &m, const Bo
                            const double *amf, double *sc, double *aHMF, double *aHG)
 {
 #define FLUXADJ(BG, BO)                                                                                      \
-    hipLaunchKernelGGL((k_flux_adj_mhd<D, RM, BG, BO>), agrid(fb), dim3(ABX, ABY), 0, s, g, m, fb, H, S, met, \
+    hipLaunchKernelGGL((k_flux_adj_mhd<D, RM, BG, BO>), D == 2 ? agrid_k(fb) : agrid(fb), dim3(ABX, ABY), 0, s, \
+                       g, m, fb, H, S, met,                                                                  \
                        bg, ag, amf, sc, aHMF, aHG)
     if (m.doBG && m.doBoris) FLUXADJ(true, true);
     else if (m.doBG) FLUXADJ(true, false);

@@ -1218,12 +1218,19 @@ struct gam_adj {
     int nsteps, K, nck;
     int cur0;                // State buffer at the start of the run
     int haveW, recorded;
+    int lastTaped;           // the tapes hold the last interval from gam_adj_forward
     double *ck;              // nck x (State, oState), S_NPL planes each
     double *fin;             // State, oState at the end
     Clock *clkh;             // clock before step n, n = 0..nsteps
     Clock *oclk;             // clock at the start with t = oState%time
     double *tape;            // K + 2 State buffers of one interval
     double *itape;           // K steps' forward intermediates (IT_NPL planes each), or null
+    // a second pair of interval tapes and a stream of its own: the reverse
+    // sweep recomputes the next interval into one pair while the adjoint
+    // steps of the current interval read the other (null: no overlap)
+    double *tape2, *itape2;
+    hipStream_t rs;
+    hipEvent_t evR[2], evA[2];   // recompute of a pair done / adjoint done reading it
     double *lam;             // three adjoint state sets (ADJ_SET planes each)
     double *ag, *amf;        // adjoints of gFlx (15) and mFlx (9)
     double *awh, *awn, *ahg; // adjoints of H_WH, H_WN, H_GAS (5 each)
@@ -1240,17 +1247,29 @@ struct gam_adj {
     float fwd_ms, bwd_ms;
 };
 
+static StepIT it_slot(gam_adj *a, int q, int buf = 0)
+{
+    const long N3 = a->c->g.N3;
+    double *b = (buf ? a->itape2 : a->itape) + (size_t)q * IT_NPL * N3;
+    return StepIT{b, b + (size_t)IT_VF * N3, b + (size_t)IT_GF * N3, b + (size_t)IT_MF * N3};
+}
+
 extern "C" int gam_adj_destroy(gam_adj *a)
 {
     if (!a) return GAM_OK;
     if (a->c && a->c->stream) hipStreamSynchronize(a->c->stream);
-    double *bufs[] = {a->ck, a->fin, a->tape, a->itape, a->lam, a->ag, a->amf, a->awh, a->awn, a->ahg, a->ahmf,
+    double *bufs[] = {a->ck, a->fin, a->tape, a->itape, a->tape2, a->itape2, a->lam, a->ag, a->amf, a->awh, a->awn, a->ahg, a->ahmf,
                       a->ahb, a->avf, a->aur, a->adm, a->ae, a->wgt, a->part, a->gwq, a->gwb, a->rv,
                       a->fsc, a->esc};
     for (double *b : bufs) if (b) hipFree(b);
     if (a->clkh) hipFree(a->clkh);
     if (a->oclk) hipFree(a->oclk);
     for (int q = 0; q < 4; q++) if (a->ev[q]) hipEventDestroy(a->ev[q]);
+    for (int q = 0; q < 2; q++) {
+        if (a->evR[q]) hipEventDestroy(a->evR[q]);
+        if (a->evA[q]) hipEventDestroy(a->evA[q]);
+    }
+    if (a->rs) hipStreamDestroy(a->rs);
     free(a);
     return GAM_OK;
 }
@@ -1310,6 +1329,18 @@ extern "C" int gam_adj_create(gam_adj **out, gam_ctx *c, int nsteps, int K)
     if (e == hipSuccess && itb + ((size_t)4 << 30) < mfree) {
         e = hipMalloc((void **)&a->itape, itb);
         if (e == hipSuccess) e = hipMemset(a->itape, 0, itb);
+        // and, where HBM allows, the second pair for the overlapped sweep
+        const size_t t2 = (size_t)(K + 2) * S_NPL * pl;
+        if (e == hipSuccess && a->nck > 1 && itb + t2 + ((size_t)4 << 30) < mfree - itb) {
+            e = hipMalloc((void **)&a->itape2, itb);
+            if (e == hipSuccess) e = hipMemset(a->itape2, 0, itb);
+            if (e == hipSuccess) e = hipMalloc((void **)&a->tape2, t2);
+            if (e == hipSuccess) e = hipStreamCreateWithFlags(&a->rs, hipStreamNonBlocking);
+            for (int q = 0; e == hipSuccess && q < 2; q++) {
+                e = hipEventCreateWithFlags(&a->evR[q], hipEventDisableTiming);
+                if (e == hipSuccess) e = hipEventCreateWithFlags(&a->evA[q], hipEventDisableTiming);
+            }
+        }
     }
     if (e == hipSuccess) e = hipMemset(a->lam, 0, 3 * ADJ_SET * pl);
     if (e == hipSuccess) e = hipMemset(a->ag, 0, 15 * pl);
@@ -1381,15 +1412,33 @@ extern "C" int gam_adj_forward(gam_adj *a, double *J)
     c->prof = 0;
     HIPCHK(hipEventRecord(a->ev[0], c->stream));
     HIPCHK(hipMemcpyAsync(a->clkh, c->clk, sizeof(Clock), hipMemcpyDeviceToDevice, c->stream));
+    // the last interval's states (and intermediates) go straight onto the
+    // tapes, so the reverse sweep starts without recomputing it
+    const int sL = (a->nck - 1) * a->K;
+    const StepIT own = {c->H, c->vf, c->gflx, c->mflx};
     for (int n = 0; n < a->nsteps; n++) {
         if (n % a->K == 0) {
             double *ck = a->ck + (size_t)(n / a->K) * 2 * S_NPL * N3;
             HIPCHK(hipMemcpyAsync(ck, S_cur(c), bytes, hipMemcpyDeviceToDevice, c->stream));
             HIPCHK(hipMemcpyAsync(ck + (size_t)S_NPL * N3, S_old(c), bytes, hipMemcpyDeviceToDevice, c->stream));
         }
+        if (n >= sL) {
+            const int q = n - sL;
+            if (q == 0) HIPCHK(hipMemcpyAsync(a->tape, S_old(c), bytes, hipMemcpyDeviceToDevice, c->stream));
+            HIPCHK(hipMemcpyAsync(a->tape + (size_t)(1 + q) * S_NPL * N3, S_cur(c), bytes,
+                                  hipMemcpyDeviceToDevice, c->stream));
+            if (a->itape) {
+                const StepIT t = it_slot(a, q);
+                c->H = t.H; c->vf = t.vf; c->gflx = t.gflx; c->mflx = t.mflx;
+            }
+        }
         enq_step(c);
         HIPCHK(hipMemcpyAsync(a->clkh + n + 1, c->clk, sizeof(Clock), hipMemcpyDeviceToDevice, c->stream));
     }
+    c->H = own.H; c->vf = own.vf; c->gflx = own.gflx; c->mflx = own.mflx;
+    HIPCHK(hipMemcpyAsync(a->tape + (size_t)(1 + a->nsteps - sL) * S_NPL * N3, S_cur(c), bytes,
+                          hipMemcpyDeviceToDevice, c->stream));
+    a->lastTaped = 1;
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(a->ev[1], c->stream));
     c->prof = prof;
@@ -1415,13 +1464,6 @@ static int adj_restore_final(gam_adj *a)
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(c->stream));
     return GAM_OK;
-}
-
-static StepIT it_slot(gam_adj *a, int q)
-{
-    const long N3 = a->c->g.N3;
-    double *b = a->itape + (size_t)q * IT_NPL * N3;
-    return StepIT{b, b + (size_t)IT_VF * N3, b + (size_t)IT_GF * N3, b + (size_t)IT_MF * N3};
 }
 
 // Transposes of the boundary pass on one adjoint set, sides OUTK to INI;
@@ -1528,37 +1570,69 @@ extern "C" int gam_adj_backward(gam_adj *a, double *gGas, double *gGasO, double 
     if (rc) return rc;
     const int prof = c->prof;
     c->prof = 0;
-    for (int seg = a->nck - 1; seg >= 0; seg--) {
+    // Interval seg recomputed from its checkpoint onto tape pair buf: on the
+    // second stream when the sweep overlaps, else on the context's stream.
+    const bool ovl = a->tape2 != nullptr;
+    auto recompute = [&](int seg, int buf) -> int {
         const int s0 = seg * a->K, s1 = (s0 + a->K < a->nsteps) ? s0 + a->K : a->nsteps;
         const double *ck = a->ck + (size_t)seg * 2 * S_NPL * N3;
-        if ((rc = adj_put(a, s0, ck))) return rc;
-        // tape[0] = oState at s0, tape[1 + q] = State at step s0 + q, q = 0..s1-s0
-        HIPCHK(hipMemcpyAsync(a->tape, ck + (size_t)S_NPL * N3, bytes, hipMemcpyDeviceToDevice, s));
+        double *tp = buf ? a->tape2 : a->tape;
         const StepIT own = {c->H, c->vf, c->gflx, c->mflx};
-        for (int q = 0; q <= s1 - s0; q++) {
-            HIPCHK(hipMemcpyAsync(a->tape + (size_t)(1 + q) * S_NPL * N3, S_cur(c), bytes,
-                                  hipMemcpyDeviceToDevice, s));
-            if (q < s1 - s0) {
+        c->stream = ovl ? a->rs : s;
+        int r = adj_put(a, s0, ck);
+        // tape[0] = oState at s0, tape[1 + q] = State at step s0 + q, q = 0..s1-s0
+        if (!r && hipMemcpyAsync(tp, ck + (size_t)S_NPL * N3, bytes, hipMemcpyDeviceToDevice, c->stream))
+            r = fail(GAM_E_HIP, "gam_adj_backward: tape copy failed");
+        for (int q = 0; !r && q <= s1 - s0; q++) {
+            if (hipMemcpyAsync(tp + (size_t)(1 + q) * S_NPL * N3, S_cur(c), bytes, hipMemcpyDeviceToDevice,
+                               c->stream))
+                r = fail(GAM_E_HIP, "gam_adj_backward: tape copy failed");
+            if (!r && q < s1 - s0) {
                 if (a->itape) {   // the step writes its intermediates into the tape slot
-                    const StepIT t = it_slot(a, q);
+                    const StepIT t = it_slot(a, q, buf);
                     c->H = t.H; c->vf = t.vf; c->gflx = t.gflx; c->mflx = t.mflx;
                 }
                 enq_step(c);
             }
         }
+        if (!r && ovl && hipEventRecord(a->evR[buf], c->stream)) r = fail(GAM_E_HIP, "gam_adj_backward: event");
         c->H = own.H; c->vf = own.vf; c->gflx = own.gflx; c->mflx = own.mflx;
+        c->stream = s;
+        return r;
+    };
+    // the last interval: from the recorded forward run's tapes, or recomputed
+    const bool reuse = a->lastTaped;
+    if (!reuse && (rc = recompute(a->nck - 1, 0))) return rc;
+    for (int seg = a->nck - 1; seg >= 0; seg--) {
+        const int s0 = seg * a->K, s1 = (s0 + a->K < a->nsteps) ? s0 + a->K : a->nsteps;
+        const int buf = ovl ? (a->nck - 1 - seg) & 1 : 0;
+        if (ovl) {
+            // the next interval goes onto the other pair once the adjoint
+            // steps that last read it are done
+            if (seg > 0) {
+                if (seg + 1 < a->nck) HIPCHK(hipStreamWaitEvent(a->rs, a->evA[buf ^ 1], 0));
+                if ((rc = recompute(seg - 1, buf ^ 1))) return rc;
+            }
+            if (!(seg == a->nck - 1 && reuse)) HIPCHK(hipStreamWaitEvent(s, a->evR[buf], 0));
+        } else if (seg < a->nck - 1 && (rc = recompute(seg, 0))) {
+            return rc;
+        }
+        const double *tp = buf ? a->tape2 : a->tape;
         for (int n = s1 - 1; n >= s0; n--) {
             const int q = n - s0;
-            const StepIT its = a->itape ? it_slot(a, q) : own;
-            adj_step(a, a->itape ? &its : nullptr, a->tape + (size_t)(1 + q) * S_NPL * N3, a->tape + (size_t)q * S_NPL * N3,
-                     a->tape + (size_t)(2 + q) * S_NPL * N3, a->clkh + n, a->clkh + n + 1, lA, lB, lC);
+            const StepIT its = a->itape ? it_slot(a, q, buf) : StepIT{c->H, c->vf, c->gflx, c->mflx};
+            adj_step(a, a->itape ? &its : nullptr, tp + (size_t)(1 + q) * S_NPL * N3, tp + (size_t)q * S_NPL * N3,
+                     tp + (size_t)(2 + q) * S_NPL * N3, a->clkh + n, a->clkh + n + 1, lA, lB, lC);
             double *t = lA;
             lA = lB;
             lB = lC;
             lC = t;
         }
+        if (ovl) HIPCHK(hipEventRecord(a->evA[buf], s));
         HIPCHK(hipGetLastError());
     }
+    // pair 0 still holds the last interval only if no other went onto it
+    a->lastTaped = (a->nck == 1);
     c->prof = prof;
     // the start states' ghost cells follow from their interiors through the
     // boundary conditions, and their cell B from the face fluxes: fold those

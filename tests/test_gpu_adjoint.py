@@ -390,6 +390,28 @@ def test_adjoint_mhd_shell_inflow_matches_finite_differences():
     g.close()
 
 
+def test_adjoint_backward_repeats():
+    """The reverse sweep takes the last interval from the recorded forward
+    run's tapes; a second sweep recomputes every interval from its
+    checkpoint and must give the same gradients (to the order in which
+    atomic adds land)."""
+    g = box_case((16, 16, 16), "7UP")[0]
+    g.init_run()
+    _generic(g, 19)
+    g.run(1)
+    adj = gamhip.GamAdjoint(g, 7, 3)
+    adj.set_weights(_weights(g))
+    adj.forward()
+    first = adj.backward(fluxes=True)
+    second = adj.backward(fluxes=True)
+    for a, b in zip(first, second):
+        scale = np.max(np.abs(a))
+        assert scale > 0.0
+        assert np.max(np.abs(a - b)) <= 1e-12 * scale
+    adj.close()
+    g.close()
+
+
 def test_torch_autograd_bridge():
     """StepObjective (torch.autograd.Function over the C ABI): the tensor
     gradient is the adjoint of State plus oState at the start (both begin
