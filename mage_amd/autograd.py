@@ -1,7 +1,7 @@
 """torch.autograd bridge to the device adjoint (include/gamadj.h).
 
 ``StepObjective(g, nsteps, ckpt_every, weights)`` turns a prepared
-hydrodynamic context into a differentiable function of the initial gas
+context into a differentiable function of the initial gas
 state: calling it with a float64 tensor ``gas0`` of shape (Ni, Nj, Nk, 5)
 puts ``gas0`` into the context's State, runs the set-up tail
 (``gam_init_run``: boundary conditions, oState = State, dt = CalcDT), steps

@@ -421,9 +421,10 @@ def _load_adj(L):
 
 
 class GamAdjoint:
-    """Discrete adjoint of nsteps steps of a (hydrodynamic) context for the
-    objective J = sum_c w_c P_c of the final State, with State/oState
-    checkpoints every ckpt_every steps."""
+    """Discrete adjoint of nsteps steps of a context (any step option and
+    BC kind) for the objective J = sum_c w_c P_c of the final State, with
+    State/oState checkpoints every ckpt_every steps; wind_grad gives the
+    gradient with respect to the inflow series."""
 
     def __init__(self, g, nsteps, ckpt_every):
         self.L = _load_adj(load())
