@@ -125,22 +125,45 @@ def ensemble_gain(gamhip, pb, n=32, steps=100, warmup=2, tune=True):
             "distinct_plans": len({tuple(sorted(p.items())) for p in plans}) if plans else 1}
 
 
-def full_size(gamhip, pb, N=(256, 256, 384), steps=500, warmup=5, tune=True):
+def full_size(gamhip, pb, N=(256, 256, 384), steps=500, warmup=5, tune=True, adj_steps=100, adj_ckpt=10):
     """Forward steps of the full-size shell (same physics as the 192x192x256
-    workload), HIP-event timed, against the step's algorithmic bytes."""
+    workload), HIP-event timed, against the step's algorithmic bytes; then
+    the adjoint over adj_steps more steps with checkpoints every adj_ckpt
+    (sqrt(adj_steps) checkpoints), objective the pressure integral over
+    3 <= r <= 6."""
+    import numpy as np
     g, p, grid, st, wind = setup_shell(gamhip, pb, N)
     del st, grid
     plan = g.autotune(reps=2) if tune else g.get_plan()
     g.run(warmup)
     ms = g.time_steps(steps)
-    g.close()
     cells = N[0] * N[1] * N[2]
     value = cells * steps / (ms * 1e-3)
+    adj = None
+    if adj_steps:
+        gr = g.get_grid(fields=("xyzcc", "volume"))
+        r = np.sqrt(np.sum(gr["xyzcc"] ** 2, axis=-1))
+        w = np.asfortranarray(gr["volume"] * ((r >= 3.0) & (r <= 6.0)))
+        del gr, r
+        try:
+            a = gamhip.GamAdjoint(g, adj_steps, adj_ckpt)
+            a.set_weights(w)
+            J = a.forward()
+            a.backward()
+            f_ms, b_ms = a.times()
+            a.close()
+            fwd = ms / steps * adj_steps
+            adj = {"steps": adj_steps, "ckpt_every": adj_ckpt, "J": J, "forward_plain_ms": fwd,
+                   "forward_recorded_ms": f_ms, "reverse_ms": b_ms, "adjoint_over_forward": b_ms / fwd}
+        except gamhip.GamError as e:
+            adj = {"steps": adj_steps, "ckpt_every": adj_ckpt, "error": str(e)}
+    g.close()
     return {"workload": "%dx%dx%d warped shell (25.2M cells), dipole B0, inflow, 7UP, Boris, %d forward steps"
                         % (N + (steps,)),
             "cells": cells, "steps": steps, "ms_per_step": ms / steps, "value": value,
             "unit": "cell-updates/s", "launch_plan": plan,
-            "step_frac_of_hbm_roofline": STEP_BYTES_PER_CELL * value / 1e9 / HBM_PEAK_GBS}
+            "step_frac_of_hbm_roofline": STEP_BYTES_PER_CELL * value / 1e9 / HBM_PEAK_GBS,
+            "adjoint": adj}
 
 
 def adjoint_cost(gamhip, pb, N=(96, 96, 128), steps=200, ckpt=20, nwind=200):
@@ -197,6 +220,108 @@ def adjoint_cost(gamhip, pb, N=(96, 96, 128), steps=200, ckpt=20, nwind=200):
                     "backward_call_ms adds the host side and the gradient downloads"}
 
 
+def _smooth_direction(shape, rng):
+    """A seeded smooth field (a product of sines per variable) of shape
+    (Ni, Nj, Nk, nv)."""
+    import numpy as np
+    Ni, Nj, Nk, nv = shape
+    x = np.linspace(0, 1, Ni)[:, None, None]
+    y = np.linspace(0, 1, Nj)[None, :, None]
+    z = np.linspace(0, 1, Nk)[None, None, :]
+    d = np.zeros(shape, order="F")
+    for v in range(nv):
+        k = rng.integers(1, 4, 3)
+        ph = rng.uniform(0, 2 * np.pi, 3)
+        d[..., v] = (np.sin(2 * np.pi * k[0] * x + ph[0]) * np.sin(2 * np.pi * k[1] * y + ph[1])
+                     * np.sin(2 * np.pi * k[2] * z + ph[2]))
+    return d
+
+
+def adjoint_fd_check(gamhip, pb, ndir=64, N=(96, 96, 128), steps=200, ckpt=20, nwind=200, seed=31337):
+    """Central-difference check of the adjoint on the adjoint workload
+    itself: ndir/2 seeded smooth directions in the starting State gas with
+    the stated objective (pressure over 3 <= r <= 6), and ndir/2 seeded
+    directions in the inflow series with the pressure over r >= 20 (the
+    stated objective does not see the inflow within the run): 3/4 of them in
+    the rho and Bz columns, 1/4 in vx alone.  A vx change moves the lag
+    GetWindAt samples at, and the sample index is a step function of that
+    time: over the run hundreds of boundary points cross a sample wall for
+    any step size, so the difference quotient of a vx direction carries
+    those jumps and is reported apart ("inflow_vx").  Each direction takes
+    the better of two steps (1e-4 and 1e-5 of 1 % of each field's scale);
+    reports the worst relative error and the worst error relative to
+    sum |g d| (free of cancellation)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    p = pb.shell_params("7UP")
+    g = gamhip.GamHip(*N, p)
+    g.grid_from_corners(pb.shell_corners(*N))
+    g.add_dipole_b0(pb.SHELL_M0)
+    wind = pb.varying_wind(nwind)
+    g.set_wind(wind)
+    grid = g.get_grid(fields=("xyzcc", "volume"))
+    g.put_state(pb.shell_state(*N, grid["xyzcc"], p0=0.1))
+    g.init_run()
+    g.run(2)
+    r = np.sqrt(np.sum(grid["xyzcc"] ** 2, axis=-1))
+    out = {"directions": ndir, "steps": steps, "ckpt_every": ckpt}
+    adj = gamhip.GamAdjoint(g, steps, ckpt)
+    for part in ("state", "inflow"):
+        w = grid["volume"] * (((r >= 3.0) & (r <= 6.0)) if part == "state" else (r >= 20.0))
+        worst = {}
+        adj.set_weights(np.asfortranarray(w))
+        S = g.get_state(gamhip.GAM_STATE)
+        O = g.get_state(gamhip.GAM_OSTATE)
+        s0, o0 = S["Gas"].copy(order="F"), O["Gas"].copy(order="F")
+        adj.forward()
+        gS, gO = adj.backward()
+        gQ, gB = adj.wind_grad(nwind)
+        for q in range(ndir // 2):
+            kind = part if part == "state" else ("inflow_vx" if q % 4 == 3 else "inflow_rho_bz")
+            if part == "state":
+                d = _smooth_direction(s0.shape, rng) * np.max(np.abs(s0), axis=(0, 1, 2)) * 1e-2
+                ad = float(np.sum(gS * d))
+                gd = float(np.sum(np.abs(gS * d)))
+            else:
+                dQ = np.zeros((nwind, 5), order="F")
+                dB = np.zeros((nwind, 3), order="F")
+                sm = np.cumsum(rng.standard_normal((nwind, 3)), axis=0)
+                sm -= sm.mean(axis=0)
+                sm /= np.max(np.abs(sm), axis=0)
+                if kind == "inflow_vx":
+                    dQ[:, 1] = sm[:, 1] * np.max(np.abs(wind["Q"][:, 1])) * 1e-2
+                else:
+                    dQ[:, 0] = sm[:, 0] * np.max(np.abs(wind["Q"][:, 0])) * 1e-2
+                    dB[:, 2] = sm[:, 2] * np.max(np.abs(wind["B"][:, 2])) * 1e-2
+                ad = float(np.sum(gQ * dQ) + np.sum(gB * dB))
+                gd = float(np.sum(np.abs(gQ * dQ)) + np.sum(np.abs(gB * dB)))
+            best = None
+            for e in (1e-4, 1e-5):
+                if part == "state":
+                    fd = (adj.replay(s0 + e * d, o0) - adj.replay(s0 - e * d, o0)) / (2 * e)
+                else:
+                    Jp = []
+                    for sg in (1.0, -1.0):
+                        ww = dict(wind)
+                        ww["Q"] = wind["Q"] + sg * e * dQ
+                        ww["B"] = wind["B"] + sg * e * dB
+                        g.set_wind(ww)
+                        Jp.append(adj.replay(s0, o0))
+                    g.set_wind(wind)
+                    fd = (Jp[0] - Jp[1]) / (2 * e)
+                rel = abs(fd - ad) / max(abs(fd), abs(ad), 1e-300)
+                if best is None or rel < best[0]:
+                    best = (rel, abs(fd - ad) / max(gd, 1e-300))
+            log("adjoint fd %s dir %d: adj %.10e rel %.2e of sum|g d| %.2e" % (kind, q, ad, best[0], best[1]))
+            wr, wa, nd = worst.get(kind, (0.0, 0.0, 0))
+            worst[kind] = (max(wr, best[0]), max(wa, best[1]), nd + 1)
+        for kind, (wr, wa, nd) in worst.items():
+            out[kind] = {"directions": nd, "worst_rel": wr, "worst_of_sum_abs": wa}
+    adj.close()
+    g.close()
+    return out
+
+
 def cpu_baseline(gamhip, pb, min_seconds=12.0, max_steps=400):
     """Time the oracle (C/OpenMP restatement of the reference step) on a
     96x96x128 sample of the same shell workload."""
@@ -237,6 +362,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-full", action="store_true")
     ap.add_argument("--no-adj", action="store_true")
+    ap.add_argument("--adj-fd", type=int, default=0,
+                    help="central-difference check of the adjoint workload with this many directions")
     ap.add_argument("--full-steps", type=int, default=500)
     ap.add_argument("--ens-members", type=int, default=32)
     ap.add_argument("--ens-steps", type=int, default=100)
@@ -309,6 +436,9 @@ def main():
         full = full_size(gamhip, pb, steps=args.full_steps, tune=not args.no_tune)
         log("full-size shell: %.3f ms/step, %.4e cell-updates/s (%.1f s)"
             % (full["ms_per_step"], full["value"], time.time() - t1))
+        if full["adjoint"] and "reverse_ms" in full["adjoint"]:
+            log("full-size adjoint over %d steps: reverse %.1f ms, ratio %.2f"
+                % (full["adjoint"]["steps"], full["adjoint"]["reverse_ms"], full["adjoint"]["adjoint_over_forward"]))
 
     ens = None
     if not args.no_ens:
@@ -324,6 +454,10 @@ def main():
         log("adjoint: forward %.1f ms (recorded %.1f), reverse %.1f ms, ratio %.2f (%.1f s)"
             % (adjc["forward_plain_ms"], adjc["forward_recorded_ms"], adjc["reverse_ms"],
                adjc["adjoint_over_forward"], time.time() - t1))
+        if args.adj_fd:
+            t1 = time.time()
+            adjc["fd_check"] = adjoint_fd_check(gamhip, pb, ndir=args.adj_fd)
+            log("adjoint fd check: %s (%.1f s)" % (adjc["fd_check"], time.time() - t1))
 
     cpu = None
     if not args.no_cpu:

@@ -1224,7 +1224,8 @@ struct gam_adj {
     Clock *clkh;             // clock before step n, n = 0..nsteps
     Clock *oclk;             // clock at the start with t = oState%time
     double *tape;            // K + 2 State buffers of one interval
-    double *itape;           // K steps' forward intermediates (IT_NPL planes each), or null
+    double *itape;           // Kt steps' forward intermediates (IT_NPL planes each), or null
+    int Kt;                  // steps of an interval whose intermediates are taped (<= K)
     // a second pair of interval tapes and a stream of its own: the reverse
     // sweep recomputes the next interval into one pair while the adjoint
     // steps of the current interval read the other (null: no overlap)
@@ -1252,6 +1253,17 @@ static StepIT it_slot(gam_adj *a, int q, int buf = 0)
     const long N3 = a->c->g.N3;
     double *b = (buf ? a->itape2 : a->itape) + (size_t)q * IT_NPL * N3;
     return StepIT{b, b + (size_t)IT_VF * N3, b + (size_t)IT_GF * N3, b + (size_t)IT_MF * N3};
+}
+
+static bool it_taped(const gam_adj *a, int q) { return a->itape && q < a->Kt; }
+
+// Point the context's forward intermediates at step q's tape slot (pair
+// buf) when that step is taped, else at its own buffers.
+static void it_use(gam_adj *a, int q, int buf, const StepIT &own)
+{
+    gam_ctx *c = a->c;
+    const StepIT t = it_taped(a, q) ? it_slot(a, q, buf) : own;
+    c->H = t.H; c->vf = t.vf; c->gflx = t.gflx; c->mflx = t.mflx;
 }
 
 extern "C" int gam_adj_destroy(gam_adj *a)
@@ -1325,13 +1337,16 @@ extern "C" int gam_adj_create(gam_adj **out, gam_ctx *c, int nsteps, int K)
     // each adjoint step recomputes its predictor, face velocities and fluxes)
     size_t mfree = 0, mtot = 0;
     if (e == hipSuccess) e = hipMemGetInfo(&mfree, &mtot);
-    const size_t itb = (size_t)K * IT_NPL * pl;
-    if (e == hipSuccess && itb + ((size_t)4 << 30) < mfree) {
+    // (as many steps of an interval as fit, 4 GiB kept free)
+    const size_t slot = (size_t)IT_NPL * pl, spare = (size_t)4 << 30;
+    a->Kt = (e == hipSuccess && mfree > spare) ? (int)std::min<size_t>((size_t)K, (mfree - spare) / slot) : 0;
+    const size_t itb = (size_t)a->Kt * slot;
+    if (a->Kt > 0) {
         e = hipMalloc((void **)&a->itape, itb);
         if (e == hipSuccess) e = hipMemset(a->itape, 0, itb);
         // and, where HBM allows, the second pair for the overlapped sweep
         const size_t t2 = (size_t)(K + 2) * S_NPL * pl;
-        if (e == hipSuccess && a->nck > 1 && itb + t2 + ((size_t)4 << 30) < mfree - itb) {
+        if (e == hipSuccess && a->Kt == K && a->nck > 1 && itb + t2 + spare < mfree - itb) {
             e = hipMalloc((void **)&a->itape2, itb);
             if (e == hipSuccess) e = hipMemset(a->itape2, 0, itb);
             if (e == hipSuccess) e = hipMalloc((void **)&a->tape2, t2);
@@ -1427,10 +1442,7 @@ extern "C" int gam_adj_forward(gam_adj *a, double *J)
             if (q == 0) HIPCHK(hipMemcpyAsync(a->tape, S_old(c), bytes, hipMemcpyDeviceToDevice, c->stream));
             HIPCHK(hipMemcpyAsync(a->tape + (size_t)(1 + q) * S_NPL * N3, S_cur(c), bytes,
                                   hipMemcpyDeviceToDevice, c->stream));
-            if (a->itape) {
-                const StepIT t = it_slot(a, q);
-                c->H = t.H; c->vf = t.vf; c->gflx = t.gflx; c->mflx = t.mflx;
-            }
+            it_use(a, q, 0, own);
         }
         enq_step(c);
         HIPCHK(hipMemcpyAsync(a->clkh + n + 1, c->clk, sizeof(Clock), hipMemcpyDeviceToDevice, c->stream));
@@ -1588,10 +1600,7 @@ extern "C" int gam_adj_backward(gam_adj *a, double *gGas, double *gGasO, double 
                                c->stream))
                 r = fail(GAM_E_HIP, "gam_adj_backward: tape copy failed");
             if (!r && q < s1 - s0) {
-                if (a->itape) {   // the step writes its intermediates into the tape slot
-                    const StepIT t = it_slot(a, q, buf);
-                    c->H = t.H; c->vf = t.vf; c->gflx = t.gflx; c->mflx = t.mflx;
-                }
+                it_use(a, q, buf, own);   // a taped step writes its intermediates into its slot
                 enq_step(c);
             }
         }
@@ -1620,8 +1629,9 @@ extern "C" int gam_adj_backward(gam_adj *a, double *gGas, double *gGasO, double 
         const double *tp = buf ? a->tape2 : a->tape;
         for (int n = s1 - 1; n >= s0; n--) {
             const int q = n - s0;
-            const StepIT its = a->itape ? it_slot(a, q, buf) : StepIT{c->H, c->vf, c->gflx, c->mflx};
-            adj_step(a, a->itape ? &its : nullptr, tp + (size_t)(1 + q) * S_NPL * N3, tp + (size_t)q * S_NPL * N3,
+            const bool tpd = it_taped(a, q);
+            const StepIT its = tpd ? it_slot(a, q, buf) : StepIT{c->H, c->vf, c->gflx, c->mflx};
+            adj_step(a, tpd ? &its : nullptr, tp + (size_t)(1 + q) * S_NPL * N3, tp + (size_t)q * S_NPL * N3,
                      tp + (size_t)(2 + q) * S_NPL * N3, a->clkh + n, a->clkh + n + 1, lA, lB, lC);
             double *t = lA;
             lA = lB;
