@@ -66,7 +66,21 @@ struct gam_ctx {
     int tune_slot;           // slot whose launches get events, or -1
     int ntev;
     hipEvent_t tev[2 * GAM_TUNE_MAXREPS];
+    // two steps captured as one graph, from State buffer 0 / 1 (null: not
+    // captured yet or dropped because a launch argument changed)
+    hipGraphExec_t gx[2];
 };
+
+static void graphs_drop(gam_ctx *c)
+{
+    if (!c->gx[0] && !c->gx[1]) return;
+    hipStreamSynchronize(c->stream);   // no replay may be in flight
+    for (int q = 0; q < 2; q++)
+        if (c->gx[q]) {
+            hipGraphExecDestroy(c->gx[q]);
+            c->gx[q] = nullptr;
+        }
+}
 
 // ------------------------------------------------------------ geometry
 static Box mkbox(int i0, int i1, int j0, int j1, int k0, int k1)
@@ -218,6 +232,7 @@ extern "C" int gam_destroy(gam_ctx *c)
 {
     if (!c) return GAM_OK;
     if (c->stream) hipStreamSynchronize(c->stream);
+    graphs_drop(c);
     double *bufs[] = {c->st[0], c->st[1], c->H, c->met, c->bg, c->gflx, c->mflx, c->vf, c->stage, c->part, c->wind};
     for (double *b : bufs) if (b) hipFree(b);
     if (c->clk) hipFree(c->clk);
@@ -262,6 +277,7 @@ static int download(gam_ctx *c, const double *src, int face, int ncomp, double *
 extern "C" int gam_set_grid(gam_ctx *c, const gam_grid_arrays *a)
 {
     if (!c || !a) return fail(GAM_E_ARG, "gam_set_grid: NULL argument");
+    graphs_drop(c);   // launch arguments change
     if (!a->xyzcc || !a->xfc || !a->di || !a->dj || !a->dk || !a->volume || !a->Tf || !a->Te ||
         !a->Teb || !a->face || !a->edge)
         return fail(GAM_E_ARG, "gam_set_grid: every metric array is required");
@@ -285,6 +301,7 @@ extern "C" int gam_set_grid(gam_ctx *c, const gam_grid_arrays *a)
 extern "C" int gam_grid_from_corners(gam_ctx *c, const double *xyz)
 {
     if (!c || !xyz) return fail(GAM_E_ARG, "gam_grid_from_corners: NULL argument");
+    graphs_drop(c);   // launch arguments change
     int rc = upload(c, xyz, 1, 3, c->met + (long)M_XYZ * c->g.N3);
     if (rc) return rc;
     launch_grid_from_corners(c->stream, c->g, c->met);
@@ -316,6 +333,7 @@ extern "C" int gam_get_grid(gam_ctx *c, const gam_grid_arrays *a)
 extern "C" int gam_set_background(gam_ctx *c, const gam_background_arrays *b)
 {
     if (!c || !b) return fail(GAM_E_ARG, "gam_set_background: NULL argument");
+    graphs_drop(c);   // launch arguments change
     if (!b->fcB0 || !b->edgB0 || !b->B0 || !b->bFlux0 || !b->dpB0)
         return fail(GAM_E_ARG, "gam_set_background: every background array is required");
     const long N3 = c->g.N3;
@@ -333,6 +351,7 @@ extern "C" int gam_add_dipole_b0(gam_ctx *c, double M0)
 {
     if (!c) return fail(GAM_E_ARG, "gam_add_dipole_b0: NULL context");
     if (!c->haveGrid) return fail(GAM_E_STATE, "gam_add_dipole_b0: grid metrics must be set first");
+    graphs_drop(c);   // launch arguments change
     launch_dipole_b0(c->stream, c->g, M0, c->met, c->bg, c->bg + (long)B_NPL * c->g.N3);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -357,6 +376,7 @@ extern "C" int gam_set_wind(gam_ctx *c, int n, const double *tW, const double *Q
                             double Bx0, double ByC, double BzC, const double *xyzW)
 {
     if (!c || !tW || !Q || !B) return fail(GAM_E_ARG, "gam_set_wind: NULL argument");
+    graphs_drop(c);   // launch arguments change
     if (n < 2) return fail(GAM_E_ARG, "gam_set_wind: need at least 2 samples (got %d)", n);
     // tMin/tMax/dtW as readWind (wind.F90:685-699)
     double tMin = tW[0], tMax = tW[0], dtMin = 1e300, dtMax = -1e300;
@@ -528,9 +548,16 @@ static void enq_bflux2fld(gam_ctx *c, double *S, bool ghosts_only = false)
 // clk: the clock whose time the inflow BC reads (the context's by default).
 static void enq_bcs(gam_ctx *c, double *S, bool ghostB = true, const Clock *clk = nullptr)
 {
-    for (int side = 0; side < 6; side++)
-        launch_bc(c->stream, side, c->model.bc[side], c->g, c->mp, clk ? clk : c->clk, &c->wd, S, c->met,
-                  ghostB);
+    for (int ax = 0; ax < 3; ax++) {
+        const int lo = 2 * ax, hi = lo + 1;
+        if (c->model.bc[lo] == GAM_BC_PERIODIC && c->model.bc[hi] == GAM_BC_PERIODIC) {
+            launch_bc_periodic_pair(c->stream, ax, c->g, S);
+            continue;
+        }
+        for (int side = lo; side <= hi; side++)
+            launch_bc(c->stream, side, c->model.bc[side], c->g, c->mp, clk ? clk : c->clk, &c->wd, S, c->met,
+                      ghostB);
+    }
 }
 
 // While a slot is being tuned, its launches are bracketed by events.
@@ -605,6 +632,36 @@ static void enq_step(gam_ctx *c)
     prof_begin(c, KG_BFLD);
     enq_bflux2fld(c, S, true);
     prof_end(c, KG_BFLD);
+}
+
+// n steps: pairs as one launch of a graph of two steps (captured on first
+// use for the current State buffer; a step is only kernels, so the graph
+// replays exactly what enq_step launches), the odd one plainly.  While
+// profiling or tuning every step is enqueued plainly (those add events).
+static int enq_steps(gam_ctx *c, int nsteps)
+{
+    int n = 0;
+    const bool graph = !c->prof && c->tune_slot < 0;
+    for (; graph && n + 2 <= nsteps; n += 2) {
+        const int p = c->cur;
+        if (!c->gx[p]) {
+            hipGraph_t gr = nullptr;
+            HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            enq_step(c);
+            enq_step(c);
+            hipError_t e = hipStreamEndCapture(c->stream, &gr);
+            c->cur = p;
+            if (e == hipSuccess) e = hipGraphInstantiate(&c->gx[p], gr, nullptr, nullptr, 0);
+            if (gr) hipGraphDestroy(gr);
+            if (e != hipSuccess) {
+                c->gx[p] = nullptr;
+                return fail(GAM_E_HIP, "step graph capture: %s", hipGetErrorString(e));
+            }
+        }
+        HIPCHK(hipGraphLaunch(c->gx[p], c->stream));   // two steps: the parity comes back to p
+    }
+    for (; n < nsteps; n++) enq_step(c);
+    return GAM_OK;
 }
 
 static int read_clock(gam_ctx *c, Clock *ck)
@@ -705,9 +762,7 @@ extern "C" int gam_run(gam_ctx *c, int nsteps)
     int rc = check_ready(c, "gam_run");
     if (rc) return rc;
     if (nsteps < 0) return fail(GAM_E_ARG, "gam_run: nsteps < 0");
-    for (int n = 0; n < nsteps; n++) {
-        enq_step(c);
-    }
+    if ((rc = enq_steps(c, nsteps))) return rc;
     HIPCHK(hipGetLastError());
     Clock ck;
     if ((rc = read_clock(c, &ck))) return rc;
@@ -774,7 +829,7 @@ extern "C" int gam_time_steps(gam_ctx *c, int nsteps, double *ms)
     HIPCHK(hipEventCreate(&b));
     HIPCHK(hipStreamSynchronize(c->stream));
     HIPCHK(hipEventRecord(a, c->stream));
-    for (int n = 0; n < nsteps; n++) enq_step(c);
+    if ((rc = enq_steps(c, nsteps))) return rc;
     HIPCHK(hipEventRecord(b, c->stream));
     HIPCHK(hipEventSynchronize(b));
     float f = 0.0f;
@@ -1109,6 +1164,7 @@ extern "C" int gam_get_plan(gam_ctx *c, int *plan)
 extern "C" int gam_set_plan(gam_ctx *c, const int *plan)
 {
     if (!c || !plan) return fail(GAM_E_ARG, "gam_set_plan: NULL argument");
+    graphs_drop(c);   // launch arguments change
     for (int q = 0; q < TS_COUNT; q++) {
         int v[8];
         const int n = plan_candidates(q, v);
@@ -1146,6 +1202,7 @@ extern "C" int gam_autotune(gam_ctx *c, int reps)
 {
     int rc = check_ready(c, "gam_autotune");
     if (rc) return rc;
+    graphs_drop(c);   // launch arguments change
     if (reps < 1 || reps > GAM_TUNE_MAXREPS) return fail(GAM_E_ARG, "gam_autotune: reps must be in 1..%d", GAM_TUNE_MAXREPS);
     // snapshot State, oState and the clock; tuning takes real steps
     const size_t bytes = (size_t)S_NPL * c->g.N3 * sizeof(double);

@@ -87,6 +87,8 @@ void launch_bflux2fld(hipStream_t s, const Geo &g, const Box &cb, const Box &ski
                       double *B, const double *met);
 void launch_bflux2fld_ghosts(hipStream_t s, const Geo &g, const double *mf, double *B, const double *met);
 // ghostB = false skips ghost-cell B that a following ghost bFlux2Fld rewrites
+// both sides of axis ax (0 = I, 1 = J, 2 = K) when both are periodic
+void launch_bc_periodic_pair(hipStream_t s, int ax, const Geo &g, double *S);
 void launch_bc(hipStream_t s, int side, int kind, const Geo &g, const MParm &m, const Clock *clk,
                const WindDev *w, double *S, const double *met, bool ghostB);
 void launch_divb(hipStream_t s, const Geo &g, const Box &ab, const double *mf, const double *bg,

@@ -285,15 +285,18 @@ struct k_predict {
 
 // ====================================================================
 // Column tiles for kernels that march along k: a workgroup owns 256
-// consecutive (i, j) columns of the plane (i fastest) over KC consecutive k,
+// consecutive (i, j) columns of the plane (i fastest) over kc consecutive k,
 // so k-direction stencils slide through registers instead of being re-read.
 // Same XCD-aware numbering as Tile: column chunk fastest, then k-chunk.
+// kc is 32 where that still gives ~1024 workgroups (4 per CU), shorter on
+// small grids so that every CU gets work (each chunk re-reads 7 planes of
+// its window, so short chunks cost reads, not results).
 // ====================================================================
-#define KC 32
 struct ColTile {
     Box b;
     int ni, nj;       // extents along i and j
     int nc, nk, nb;   // 256-column chunks of the (i, j) plane, k chunks, total
+    int kc;           // k extent of a chunk
 };
 
 static inline ColTile coltile_of(const Box &b)
@@ -303,7 +306,10 @@ static inline ColTile coltile_of(const Box &b)
     t.ni = b.i1 - b.i0 + 1;
     t.nj = b.j1 - b.j0 + 1;
     t.nc = (t.ni * t.nj + BX * BY - 1) / (BX * BY);
-    t.nk = (b.k1 - b.k0 + KC) / KC;
+    const int nkc = b.k1 - b.k0 + 1;
+    t.kc = 32;
+    while (t.kc > 4 && t.nc * ((nkc + t.kc - 1) / t.kc) < 1024) t.kc >>= 1;
+    t.nk = (nkc + t.kc - 1) / t.kc;
     t.nb = t.nc * t.nk;
     return t;
 }
@@ -322,8 +328,8 @@ __device__ __forceinline__ bool col_point(const ColTile &t, int w, int &i, int &
     const int a = (int)(p / (unsigned)t.ni);
     i = t.b.i0 + (int)(p - (unsigned)a * t.ni);
     j = t.b.j0 + a;
-    k0 = t.b.k0 + t2 * KC;
-    k1 = min(k0 + KC - 1, t.b.k1);
+    k0 = t.b.k0 + t2 * t.kc;
+    k1 = min(k0 + t.kc - 1, t.b.k1);
     return true;
 }
 
@@ -1635,6 +1641,34 @@ void launch_bflux2fld_ghosts(hipStream_t s, const Geo &g, const double *mf, doub
 }
 
 static inline dim3 grid2(const Box &t) { return dim3((unsigned)((t.i1 - t.i0 + 64) / 64), (unsigned)(t.j1 - t.j0 + 1)); }
+
+// Both periodic sides of one axis in one launch: each reads only interior
+// cells and faces and writes its own ghosts, so the two commute.  Blocks
+// with z below nz run the low side (with their own z), the rest the high
+// side.
+template <int AX>
+struct k_bc_periodic_pair {
+    __device__ __forceinline__ void operator()(const uint3 bi, Geo g, Box t, double *__restrict__ S) const
+    {
+        const unsigned nz = (AX == 0) ? 4u : 1u;
+        if (bi.z < nz) k_bc_periodic<2 * AX>{}(bi, g, t, S);
+        else k_bc_periodic<2 * AX + 1>{}(make_uint3(bi.x, bi.y, bi.z - nz), g, t, S);
+    }
+};
+
+void launch_bc_periodic_pair(hipStream_t s, int ax, const Geo &g, double *S)
+{
+    Box t;   // the transverse boxes of launch_bc's periodic sides
+    if (ax == 0) { t.i0 = g.jsg; t.i1 = g.jeg + 1; t.j0 = g.ksg; t.j1 = g.keg + 1; }
+    else if (ax == 1) { t.i0 = g.isg; t.i1 = g.ieg + 1; t.j0 = g.ksg; t.j1 = g.keg + 1; }
+    else { t.i0 = g.isg; t.i1 = g.ieg + 1; t.j0 = g.jsg; t.j1 = g.jeg + 1; }
+    t.k0 = t.k1 = 0;
+    dim3 gr = grid2(t), bl(64);
+    gr.z = (ax == 0) ? 8 : 2;
+    if (ax == 0) launch_f<k_bc_periodic_pair<0>>(gr, bl, s, g, t, S);
+    else if (ax == 1) launch_f<k_bc_periodic_pair<1>>(gr, bl, s, g, t, S);
+    else launch_f<k_bc_periodic_pair<2>>(gr, bl, s, g, t, S);
+}
 
 void launch_bc(hipStream_t s, int side, int kind, const Geo &g, const MParm &m, const Clock *clk,
                const WindDev *w, double *S, const double *met, bool ghostB)
