@@ -159,7 +159,12 @@ int gam_bflux2fld(gam_ctx *ctx, int which);        /* bFlux2Fld    gridutils.F90
  * copied back and the call does not synchronise. */
 int gam_step(gam_ctx *ctx, double *dt_next);
 /* nsteps x stepGamera without host synchronisation between steps; returns
- * the first CalcDT failure code seen, if any. */
+ * the first CalcDT failure code seen, if any.  Pairs of steps are replayed
+ * as one hipGraph of two steps, captured on first use for each State
+ * buffer (a step is kernels only, so a replay launches exactly what
+ * gam_step would: the results are bit-identical).  Changing the grid,
+ * background, inflow series or launch plan drops the captured graphs;
+ * with profiling on, steps are launched one by one. */
 int gam_run(gam_ctx *ctx, int nsteps);
 /* gamAdvanceModel (gamtypessub.F90:74-90): one stepGamera, then more while
  * Model%t < t0 + dt.  *nsteps (may be NULL) receives the step count.
@@ -171,8 +176,8 @@ int gam_sync(gam_ctx *ctx);
  * (DivB, gridutils.F90:717-770). */
 int gam_divb(gam_ctx *ctx, double *totDivB);
 
-/* hipEvent-timed loop of nsteps x stepGamera on the context's stream.
- * ms = elapsed milliseconds between the events. */
+/* hipEvent-timed loop of nsteps x stepGamera on the context's stream (as
+ * gam_run, graphs included).  ms = elapsed milliseconds between the events. */
 int gam_time_steps(gam_ctx *ctx, int nsteps, double *ms);
 
 /* Per-kernel-group timing: when enabled, every launch group of the step is
