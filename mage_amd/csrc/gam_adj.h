@@ -1,5 +1,5 @@
 // gam_adj.h -- hand-written reverse-mode (vector-Jacobian) products of the
-// per-point functions in gam_phys.h that the hydrodynamic step uses.
+// per-point functions in gam_phys.h that the step uses.
 //
 // Each *_adj function takes the forward inputs (it recomputes what it
 // needs), the adjoints of the forward outputs, and ADDS the adjoints of

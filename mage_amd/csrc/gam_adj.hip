@@ -9,10 +9,15 @@
 //
 // Adjoint planes hold the derivative of the objective with respect to the
 // same-named forward plane; an adjoint state set has the State layout's
-// first 11 planes (gas 5, face flux 3, cell B 3).  Kernels whose forward
-// gathers over a stencil (fluxes, EMF, Mom2Face, cell B) scatter with FP64
-// atomic adds (hardware global_atomic_add_f64); the others gather.  dt is the forward run's value at each step (the CFL
-// minimum is held fixed, as the finite-difference checks also do).
+// first 11 planes (gas 5, face flux 3, cell B 3).  The MHD flux and EMF
+// transposes are split at their linear stencils: a per-face (per-edge)
+// pass writes stencil scalars to scratch planes and a per-cell (per-face)
+// gather sums them, with no atomics.  The other kernels whose forward
+// gathers over a stencil (the hydrodynamic flux, cell B, the boundary
+// scatter of the inflow and outer-wall BCs) scatter with FP64 atomic adds
+// (hardware global_atomic_add_f64); the rest gather.  dt is the forward
+// run's value at each step (the CFL minimum is held fixed, as the
+// finite-difference checks also do).
 #include "gam_adj.h"
 #include "gam_kernels.h"
 
