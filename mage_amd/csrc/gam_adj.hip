@@ -1335,11 +1335,11 @@ __global__ __launch_bounds__(64) void k_bc_wind_adj(Geo g, MParm m, Box t, const
                 if (ao == 0.0) continue;
                 const double xf[3] = {met[(M_XFC + 0 + 3 * d) * N3 + f], met[(M_XFC + 1 + 3 * d) * N3 + f],
                                       met[(M_XFC + 2 + 3 * d) * N3 + f]};
-                const double wSW = wind_wgt(w, vF, xf, tm);
                 const double tl = wind_lag_time(w, vF, xf, tm);
                 const int i0 = wind_index(w, tl);
                 double r, p, V[3], B[3];
                 wind_at(w, vF, xf, tm, &r, &p, V, B);
+                const double wSW = wind_wgt_v(xf, V);
                 const double *nrm = met + (long)(M_TF + 9 * d) * N3 + f;
                 const double Af = met[(M_FACE + d) * N3 + f];
                 const double sw = Af * (nrm[0] * B[0] + nrm[N3] * B[1] + nrm[2 * N3] * B[2]);
@@ -1372,13 +1372,13 @@ __global__ __launch_bounds__(64) void k_bc_wind_adj(Geo g, MParm m, Box t, const
             if (!any) continue;
             const double xc[3] = {met[(M_XYZCC + 0) * N3 + cg], met[(M_XYZCC + 1) * N3 + cg],
                                   met[(M_XYZCC + 2) * N3 + cg]};
-            const double wSW = wind_wgt(w, vF, xc, tm), wMHD = 1.0 - wSW;
             const int i0 = wind_index(w, wind_lag_time(w, vF, xc, tm));
+            double rs, ps, Vs[3], Bs[3];
+            wind_at(w, vF, xc, tm, &rs, &ps, Vs, Bs);
+            const double wSW = wind_wgt_v(xc, Vs), wMHD = 1.0 - wSW;
             double gs[5] = {0, 0, 0, 0, 0}, gi[5] = {0, 0, 0, 0, 0};
             if (wSW > GAM_TINY) {
-                double r, p, V[3], B[3];
-                wind_at(w, vF, xc, tm, &r, &p, V, B);
-                gs[0] = r; gs[1] = V[0]; gs[2] = V[1]; gs[3] = V[2]; gs[4] = p;
+                gs[0] = rs; gs[1] = Vs[0]; gs[2] = Vs[1]; gs[3] = Vs[2]; gs[4] = ps;
             }
             double nH[3] = {0, 0, 0};
             if (wMHD > GAM_TINY) {

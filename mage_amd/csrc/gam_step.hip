@@ -1318,11 +1318,11 @@ struct k_bc_wind {
             if (k <= g.keg && j <= g.jeg) {
                 const long cg = IX(g, ig, j, k), cp = IX(g, ip, j, k);
                 const double xc[3] = {met[(M_XYZCC + 0) * N3 + cg], met[(M_XYZCC + 1) * N3 + cg], met[(M_XYZCC + 2) * N3 + cg]};
-                const double wSW = wind_wgt(w, vF, xc, tm), wMHD = 1.0 - wSW;
+                double r, p, V[3], B[3];
+                wind_at(w, vF, xc, tm, &r, &p, V, B);
+                const double wSW = wind_wgt_v(xc, V), wMHD = 1.0 - wSW;
                 double gs[5] = {0, 0, 0, 0, 0}, gi[5] = {0, 0, 0, 0, 0};
                 if (wSW > GAM_TINY) {
-                    double r, p, V[3], B[3];
-                    wind_at(w, vF, xc, tm, &r, &p, V, B);
                     gs[0] = r; gs[1] = V[0]; gs[2] = V[1]; gs[3] = V[2]; gs[4] = p;
                 }
                 if (wMHD > GAM_TINY) {
@@ -1343,9 +1343,9 @@ struct k_bc_wind {
             for (int d = 0; d < 3; d++) {
                 const long f = IX(g, fi[d], j, k);
                 const double xf[3] = {met[(M_XFC + 0 + 3 * d) * N3 + f], met[(M_XFC + 1 + 3 * d) * N3 + f], met[(M_XFC + 2 + 3 * d) * N3 + f]};
-                const double wSW = wind_wgt(w, vF, xf, tm);
                 double r, p, V[3], B[3];
                 wind_at(w, vF, xf, tm, &r, &p, V, B);
+                const double wSW = wind_wgt_v(xf, V);
                 const double *nrm = met + (long)(M_TF + 9 * d) * N3 + f;
                 const double sw = met[(M_FACE + d) * N3 + f] * (nrm[0] * B[0] + nrm[N3] * B[1] + nrm[2 * N3] * B[2]);
                 const double in = mf[d * N3 + IX(g, ii[d], j, k)];

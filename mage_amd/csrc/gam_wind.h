@@ -50,12 +50,13 @@ __device__ inline void wind_at(const WindDev &w, const double *vF, const double 
     wind_interp(w, t - dl, rho, pr, V, B);
 }
 
-__device__ inline double wind_wgt(const WindDev &w, const double *vF, const double *x, double t)
-{   // wgtWind (wind.F90:180-203) with RampDown(dTh,90,90) (math.F90:97-111)
-    double nh[3], vh[3], r, p, V[3], B[3];
+// wgtWind (wind.F90:180-203) with RampDown(dTh,90,90) (math.F90:97-111)
+// for the sample velocity V that wind_at gives at x
+__device__ inline double wind_wgt_v(const double *x, const double *V)
+{
+    double nh[3], vh[3];
     double na = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
     if (na <= GAM_TINY) { nh[0] = nh[1] = nh[2] = 0.0; } else { nh[0] = x[0] / na; nh[1] = x[1] / na; nh[2] = x[2] / na; }
-    wind_at(w, vF, x, t, &r, &p, V, B);
     double nv = sqrt(V[0] * V[0] + V[1] * V[1] + V[2] * V[2]);
     if (nv <= GAM_TINY) { vh[0] = vh[1] = vh[2] = 0.0; } else { vh[0] = V[0] / nv; vh[1] = V[1] / nv; vh[2] = V[2] / nv; }
     const double dTh = acos(-vh[0] * nh[0] + -vh[1] * nh[1] + -vh[2] * nh[2]) * 180.0 / GAM_PI;
@@ -64,6 +65,13 @@ __device__ inline double wind_wgt(const WindDev &w, const double *vF, const doub
     const double rs = (dTh - 90.0) / 90.0;
     const double ea = -1.0 / (1.0 - rs * rs);
     return exp(ea + 1.0);
+}
+
+__device__ inline double wind_wgt(const WindDev &w, const double *vF, const double *x, double t)
+{
+    double r, p, V[3], B[3];
+    wind_at(w, vF, x, t, &r, &p, V, B);
+    return wind_wgt_v(x, V);
 }
 
 
