@@ -356,6 +356,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--grid", type=str, default="192x192x256")
     ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--repeats", type=int, default=5, help="timed loops of --steps steps (value = the first)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-box", action="store_true")
     ap.add_argument("--no-ens", action="store_true")
@@ -391,6 +392,13 @@ def main():
     ms_step = ms / args.steps
     value = cells * args.steps / (ms / 1e3)
     log("timed %d steps: %.3f ms/step, %.4e cell-updates/s" % (args.steps, ms_step, value))
+    # the same timed loop again (value stays the first): median and spread
+    # over the runs
+    runs = [ms_step] + [g.time_steps(args.steps) / args.steps for _ in range(max(args.repeats - 1, 0))]
+    srt = sorted(runs)
+    repeat = {"runs": len(runs), "ms_per_step": runs, "median_ms_per_step": srt[len(srt) // 2],
+              "spread": (srt[-1] - srt[0]) / srt[len(srt) // 2]}
+    log("repeats: median %.3f ms/step, spread %.2f %%" % (repeat["median_ms_per_step"], 100 * repeat["spread"]))
 
     # Per-kernel timing pass (hipEvents around every launch on the stream)
     g.set_profiling(True)
@@ -473,6 +481,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": ms_step,
+        "repeats": repeat,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
