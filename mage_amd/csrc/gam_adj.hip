@@ -23,30 +23,37 @@
 
 #define ABX 64
 #define ABY 4
+#define ABT 256   // threads of a point-kernel workgroup (one 256-point chunk)
 
+// Point kernels: the points of each (i, j) plane of a box, i fastest, in
+// 256-point chunks (no idle lanes whatever the row length), k from
+// blockIdx.y.  Chunks are numbered so that workgroup w runs chunk
+// (w % 8) * per + w / 8: the hardware puts workgroup w on XCD w % 8, so
+// each XCD gets one contiguous run of chunks (which share stencil rows) in
+// its own L2.
 static inline dim3 agrid(const Box &b)
+{
+    const long nij = (long)(b.i1 - b.i0 + 1) * (b.j1 - b.j0 + 1);
+    const unsigned nc = (unsigned)((nij + ABT - 1) / ABT);
+    return dim3(((nc + 7) / 8) * 8, (unsigned)(b.k1 - b.k0 + 1));
+}
+
+#define POINT_IN(b)                                                                          \
+    const int ni_ = (b).i1 - (b).i0 + 1;                                                     \
+    const unsigned nij_ = (unsigned)ni_ * (unsigned)((b).j1 - (b).j0 + 1);                   \
+    const unsigned L_ = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);            \
+    const unsigned p_ = L_ * ABT + threadIdx.x;                                              \
+    if (p_ >= nij_) return;                                                                  \
+    const int j = (b).j0 + (int)(p_ / (unsigned)ni_);                                        \
+    const int i = (b).i0 + (int)(p_ - (unsigned)(j - (b).j0) * (unsigned)ni_);              \
+    const int k = (b).k0 + (int)blockIdx.y;
+
+// the objective's block-sum reduction keeps plain 64 x 4 blocks
+static inline dim3 agrid2(const Box &b)
 {
     return dim3((unsigned)((b.i1 - b.i0 + ABX) / ABX), (unsigned)((b.j1 - b.j0 + ABY) / ABY),
                 (unsigned)(b.k1 - b.k0 + 1));
 }
-
-#define POINT_IN(b)                                          \
-    const int i = (b).i0 + (int)(blockIdx.x * ABX + threadIdx.x); \
-    const int j = (b).j0 + (int)(blockIdx.y * ABY + threadIdx.y); \
-    const int k = (b).k0 + (int)blockIdx.z;                  \
-    if (i > (b).i1 || j > (b).j1) return;
-
-// i x k blocks (for stencils along k: the block's rows share stencil planes)
-static inline dim3 agrid_k(const Box &b)
-{
-    return dim3((unsigned)((b.i1 - b.i0 + ABX) / ABX), (unsigned)(b.j1 - b.j0 + 1),
-                (unsigned)((b.k1 - b.k0 + ABY) / ABY));
-}
-#define POINT_IN_K(b)                                        \
-    const int i = (b).i0 + (int)(blockIdx.x * ABX + threadIdx.x); \
-    const int j = (b).j0 + (int)blockIdx.y;                  \
-    const int k = (b).k0 + (int)(blockIdx.z * ABY + threadIdx.y); \
-    if (i > (b).i1 || k > (b).k1) return;
 
 __device__ __forceinline__ bool inside(const Box &b, int i, int j, int k)
 {
@@ -561,10 +568,7 @@ __global__ __launch_bounds__(256, 2) void k_flux_adj_mhd(Geo g, MParm m, Box fb,
                                                       double *__restrict__ aHMF,
                                                       double *__restrict__ aHG)
 {
-    const int i = fb.i0 + (int)(blockIdx.x * ABX + threadIdx.x);
-    const int j = fb.j0 + (int)(D == 2 ? blockIdx.y : blockIdx.y * ABY + threadIdx.y);
-    const int k = fb.k0 + (int)(D == 2 ? blockIdx.z * ABY + threadIdx.y : blockIdx.z);
-    if (i > fb.i1 || j > fb.j1 || k > fb.k1) return;
+    POINT_IN(fb);
     const long N3 = g.N3;
     const long c = IX(g, i, j, k);
     const long s = STRIDE<D>(g);
@@ -970,7 +974,7 @@ void launch_adj_objective(hipStream_t s, const Geo &g, const MParm &m, const dou
                           const double *wgt, double *aS, double *part, int *nparts)
 {
     const Box cb = {g.isg, g.ieg, g.jsg, g.jeg, g.ksg, g.keg};
-    const dim3 gr = agrid(cb);
+    const dim3 gr = agrid2(cb);
     *nparts = (int)(gr.x * gr.y * gr.z);
     hipLaunchKernelGGL(k_adj_objective, gr, dim3(ABX, ABY), 0, s, g, m, cb, S, wgt, aS, part);
 }
@@ -1005,15 +1009,15 @@ void launch_update_adj(hipStream_t s, const Geo &g, const Box &ab, const Clock *
                        double *aSo, double *ag, const double *met)
 {
     const Box fb = {g.isg, g.ieg + 1, g.jsg, g.jeg + 1, g.ksg, g.keg + 1};
-    hipLaunchKernelGGL(k_update_adj, agrid(fb), dim3(ABX, ABY), 0, s, g, fb, ab, clk, aSn, aSo, ag, met);
+    hipLaunchKernelGGL(k_update_adj, agrid(fb), dim3(ABT), 0, s, g, fb, ab, clk, aSn, aSo, ag, met);
 }
 
 template <int D>
 static void flux_adj_d(hipStream_t s, const Geo &g, const MParm &m, const Box &fb, const double *H,
                        const double *met, const double *ag, double *aWH, double *aWN)
 {
-    if (m.rmeth) hipLaunchKernelGGL((k_flux_adj<D, 1>), agrid(fb), dim3(ABX, ABY), 0, s, g, m, fb, H, met, ag, aWH, aWN);
-    else hipLaunchKernelGGL((k_flux_adj<D, 0>), agrid(fb), dim3(ABX, ABY), 0, s, g, m, fb, H, met, ag, aWH, aWN);
+    if (m.rmeth) hipLaunchKernelGGL((k_flux_adj<D, 1>), agrid(fb), dim3(ABT), 0, s, g, m, fb, H, met, ag, aWH, aWN);
+    else hipLaunchKernelGGL((k_flux_adj<D, 0>), agrid(fb), dim3(ABT), 0, s, g, m, fb, H, met, ag, aWH, aWN);
 }
 
 void launch_flux_adj(hipStream_t s, int d, const Geo &g, const MParm &m, const Box &fb, const double *H,
@@ -1029,7 +1033,7 @@ void launch_predict_adj(hipStream_t s, const Geo &g, const MParm &m, const Clock
                         double *aS, double *aO)
 {
     const Box cb = {g.isg, g.ieg, g.jsg, g.jeg, g.ksg, g.keg};
-    hipLaunchKernelGGL(k_predict_adj, agrid(cb), dim3(ABX, ABY), 0, s, g, m, cb, clk, S, O, met, aWH, aWN,
+    hipLaunchKernelGGL(k_predict_adj, agrid(cb), dim3(ABT), 0, s, g, m, cb, clk, S, O, met, aWH, aWN,
                        aS, aO);
 }
 
@@ -1045,7 +1049,7 @@ static inline Box faces_of(const Geo &g) { return Box{g.isg, g.ieg + 1, g.jsg, g
 void launch_cellb_adj(hipStream_t s, const Geo &g, int all, double *aS, const double *met)
 {
     const Box cb = cells_of(g);
-    hipLaunchKernelGGL(k_cellb_adj, agrid(cb), dim3(ABX, ABY), 0, s, g, cb, all, aS, met);
+    hipLaunchKernelGGL(k_cellb_adj, agrid(cb), dim3(ABT), 0, s, g, cb, all, aS, met);
 }
 
 template <int SIDE>
@@ -1106,16 +1110,16 @@ void launch_update_adj_mhd(hipStream_t s, const Geo &g, const MParm &m, const Bo
 {
     const Box cb = cells_of(g), fb = faces_of(g);
     if (m.doBoris)
-        hipLaunchKernelGGL(k_update_adj_cells<true>, agrid(cb), dim3(ABX, ABY), 0, s, g, m, cb, ab, clk, So, Sn1,
+        hipLaunchKernelGGL(k_update_adj_cells<true>, agrid(cb), dim3(ABT), 0, s, g, m, cb, ab, clk, So, Sn1,
                            gflx, mflx, met, bg, m.doBG, aSn, aSo, aUr, adM);
     else
-        hipLaunchKernelGGL(k_update_adj_cells<false>, agrid(cb), dim3(ABX, ABY), 0, s, g, m, cb, ab, clk, So, Sn1,
+        hipLaunchKernelGGL(k_update_adj_cells<false>, agrid(cb), dim3(ABT), 0, s, g, m, cb, ab, clk, So, Sn1,
                            gflx, mflx, met, bg, m.doBG, aSn, aSo, aUr, adM);
     // the update's new cell B from its new face fluxes (every cell)
     launch_cellb_adj(s, g, 1, aSn, met);
     EdgeBoxes eb;
     for (int d = 0; d < 3; d++) eb.e[d] = eboxes[d];
-    hipLaunchKernelGGL(k_update_adj_faces, agrid(fb), dim3(ABX, ABY), 0, s, g, fb, ab, ctb, eb, clk, met, aUr,
+    hipLaunchKernelGGL(k_update_adj_faces, agrid(fb), dim3(ABT), 0, s, g, fb, ab, ctb, eb, clk, met, aUr,
                        adM, aSn, aSo, ag, amf, aE);
 }
 
@@ -1125,8 +1129,7 @@ static void flux_adj_mhd_r(hipStream_t s, const Geo &g, const MParm &m, const Bo
                            const double *amf, double *sc, double *aHMF, double *aHG)
 {
 #define FLUXADJ(BG, BO)                                                                                      \
-    hipLaunchKernelGGL((k_flux_adj_mhd<D, RM, BG, BO>), D == 2 ? agrid_k(fb) : agrid(fb), dim3(ABX, ABY), 0, s, \
-                       g, m, fb, H, S, met,                                                                  \
+    hipLaunchKernelGGL((k_flux_adj_mhd<D, RM, BG, BO>), agrid(fb), dim3(ABT), 0, s, g, m, fb, H, S, met,      \
                        bg, ag, amf, sc, aHMF, aHG)
     if (m.doBG && m.doBoris) FLUXADJ(true, true);
     else if (m.doBG) FLUXADJ(true, false);
@@ -1159,10 +1162,10 @@ void launch_recon_adj_gather(hipStream_t s, const Geo &g, const MParm &m, const 
 {
     const Box cb = cells_of(g);
     if (m.rmeth)
-        hipLaunchKernelGGL(k_recon_adj_gather<1>, agrid(cb), dim3(ABX, ABY), 0, s, g, cb, sc, rv, aVf, met,
+        hipLaunchKernelGGL(k_recon_adj_gather<1>, agrid(cb), dim3(ABT), 0, s, g, cb, sc, rv, aVf, met,
                            m.doLFM ? 1 : 0, aWH, aWN, aHB, aSSB);
     else
-        hipLaunchKernelGGL(k_recon_adj_gather<0>, agrid(cb), dim3(ABX, ABY), 0, s, g, cb, sc, rv, aVf, met,
+        hipLaunchKernelGGL(k_recon_adj_gather<0>, agrid(cb), dim3(ABT), 0, s, g, cb, sc, rv, aVf, met,
                            m.doLFM ? 1 : 0, aWH, aWN, aHB, aSSB);
 }
 
@@ -1171,7 +1174,7 @@ static void emf_adj_t(hipStream_t s, const Geo &g, const MParm &m, const Box &eb
                       const double *H, const double *S, const double *Vf, const double *met, const double *bg,
                       const double *aE, double *sc)
 {
-    hipLaunchKernelGGL((k_emf_adj<ED, RM, BG, BO>), agrid(eb), dim3(ABX, ABY), 0, s, g, m, eb, clk, H, S, Vf, met,
+    hipLaunchKernelGGL((k_emf_adj<ED, RM, BG, BO>), agrid(eb), dim3(ABT), 0, s, g, m, eb, clk, H, S, Vf, met,
                        bg, aE, sc + (size_t)16 * ED * g.N3);
 }
 
@@ -1211,17 +1214,17 @@ void launch_emf_adj_gather(hipStream_t s, const Geo &g, const MParm &m, const Bo
     EdgeBoxes eb;
     for (int d = 0; d < 3; d++) eb.e[d] = eboxes[d];
     if (m.rmeth)
-        hipLaunchKernelGGL(k_emf_adj_gather<1>, agrid(fb), dim3(ABX, ABY), 0, s, g, fb, eb, sc, met,
+        hipLaunchKernelGGL(k_emf_adj_gather<1>, agrid(fb), dim3(ABT), 0, s, g, fb, eb, sc, met,
                            m.doLFM ? 1 : 0, aVf, aHMF, aHG, aSSMF);
     else
-        hipLaunchKernelGGL(k_emf_adj_gather<0>, agrid(fb), dim3(ABX, ABY), 0, s, g, fb, eb, sc, met,
+        hipLaunchKernelGGL(k_emf_adj_gather<0>, agrid(fb), dim3(ABT), 0, s, g, fb, eb, sc, met,
                            m.doLFM ? 1 : 0, aVf, aHMF, aHG, aSSMF);
 }
 
 void launch_vface_rv(hipStream_t s, const Geo &g, const double *met, double *rv)
 {
     const Box cb = cells_of(g);
-    hipLaunchKernelGGL(k_vface_rv, agrid(cb), dim3(ABX, ABY), 0, s, g, cb, met, rv);
+    hipLaunchKernelGGL(k_vface_rv, agrid(cb), dim3(ABT), 0, s, g, cb, met, rv);
 }
 
 void launch_predict_adj_mhd(hipStream_t s, const Geo &g, const MParm &m, const Clock *clk, const double *S,
@@ -1229,10 +1232,10 @@ void launch_predict_adj_mhd(hipStream_t s, const Geo &g, const MParm &m, const C
                             const double *aWN, double *aHB, double *aHMF, double *aS, double *aO)
 {
     const Box cb = cells_of(g), fb = faces_of(g);
-    hipLaunchKernelGGL(k_predict_b_adj, agrid(cb), dim3(ABX, ABY), 0, s, g, cb, met, aHB, aHMF);
-    hipLaunchKernelGGL(k_predict_adj_mhd, agrid(cb), dim3(ABX, ABY), 0, s, g, m, cb, clk, S, O, met, aHG, aWH,
+    hipLaunchKernelGGL(k_predict_b_adj, agrid(cb), dim3(ABT), 0, s, g, cb, met, aHB, aHMF);
+    hipLaunchKernelGGL(k_predict_adj_mhd, agrid(cb), dim3(ABT), 0, s, g, m, cb, clk, S, O, met, aHG, aWH,
                        aWN, aS, aO);
-    hipLaunchKernelGGL(k_predict_mf_adj, agrid(fb), dim3(ABX, ABY), 0, s, g, fb, clk, aHMF, aS, aO);
+    hipLaunchKernelGGL(k_predict_mf_adj, agrid(fb), dim3(ABT), 0, s, g, fb, clk, aHMF, aS, aO);
 }
 
 // ============================================================ inflow BC
