@@ -902,27 +902,13 @@ __global__ __launch_bounds__(256) void k_vface_rv(Geo g, Box cb, const double *_
     }
 }
 
-// Predictor transpose, magnetic part: half-state cell B from the
-// extrapolated face fluxes (scatter into aHMF), then the extrapolation
-// H_MF = S_MF + w (S_MF - O_MF) at every face point.
-__global__ __launch_bounds__(256) void k_predict_b_adj(Geo g, Box cb, const double *__restrict__ met,
-                                                       const double *__restrict__ aHB,
-                                                       double *__restrict__ aHMF)
-{
-    POINT_IN(cb);
-    const long N3 = g.N3;
-    const long c = IX(g, i, j, k);
-    double aB[3] = {aHB[c], aHB[N3 + c], aHB[2 * N3 + c]}, af[6] = {0, 0, 0, 0, 0, 0};
-    cell_b_adj(g, met, c, aB, af);
-    AADD(aHMF + c, af[0]);
-    AADD(aHMF + N3 + c, af[1]);
-    AADD(aHMF + 2 * N3 + c, af[2]);
-    AADD(aHMF + c + 1, af[3]);
-    AADD(aHMF + N3 + c + g.P, af[4]);
-    AADD(aHMF + 2 * N3 + c + g.PP, af[5]);
-}
-
+// The predictor's face fluxes (AB2 of State and oState) and, gathered per
+// face, the transpose of the half-state cell B it forms from them (cell_b
+// of the cells on either side of the face: this face is the low face of
+// cell c and the high face of cell c - s).
 __global__ __launch_bounds__(256) void k_predict_mf_adj(Geo g, Box fb, const Clock *clk,
+                                                        const double *__restrict__ met,
+                                                        const double *__restrict__ aHB,
                                                         const double *__restrict__ aHMF,
                                                         double *__restrict__ aS,
                                                         double *__restrict__ aO)
@@ -931,8 +917,24 @@ __global__ __launch_bounds__(256) void k_predict_mf_adj(Geo g, Box fb, const Clo
     const long N3 = g.N3;
     const long c = IX(g, i, j, k);
     const double w = ab2_weight(clk);
+    const bool cin = i <= g.ieg && j <= g.jeg && k <= g.keg;   // cell c exists
     for (int d = 0; d < 3; d++) {
-        const double a = aHMF[d * N3 + c];
+        double a = aHMF[d * N3 + c];
+        const double *xf = met + (long)(M_XFC + 3 * d) * N3;
+        if (cin) {
+            const double iv = 1.0 / met[M_VOL * N3 + c];
+            for (int x = 0; x < 3; x++)
+                a += aHB[x * N3 + c] * iv * (-xf[x * N3 + c] + met[(M_XYZCC + x) * N3 + c]);
+        }
+        const int at = d == 0 ? i : d == 1 ? j : k;
+        const int lo = d == 0 ? g.isg : d == 1 ? g.jsg : g.ksg;
+        const bool oin = at - 1 >= lo && (d == 0 || i <= g.ieg) && (d == 1 || j <= g.jeg) && (d == 2 || k <= g.keg);
+        if (oin) {
+            const long cm = c - (d == 0 ? 1 : d == 1 ? (long)g.P : g.PP);
+            const double iv = 1.0 / met[M_VOL * N3 + cm];
+            for (int x = 0; x < 3; x++)
+                a += aHB[x * N3 + cm] * iv * (xf[x * N3 + c] - met[(M_XYZCC + x) * N3 + cm]);
+        }
         aS[(S_MF + d) * N3 + c] += (1.0 + w) * a;
         aO[(S_MF + d) * N3 + c] = -w * a;
     }
@@ -967,6 +969,9 @@ __global__ __launch_bounds__(256) void k_predict_adj_mhd(Geo g, MParm m, Box cb,
         aS[v * N3 + c] += aQ[v];
         aO[v * N3 + c] = aoQ[v];
     }
+    // oState's cell B does not enter the step (the predictor forms the half
+    // state's B from the extrapolated face fluxes)
+    for (int x = 0; x < 3; x++) aO[(S_B + x) * N3 + c] = 0.0;
 }
 
 // ------------------------------------------------------------ launchers
@@ -1232,10 +1237,9 @@ void launch_predict_adj_mhd(hipStream_t s, const Geo &g, const MParm &m, const C
                             const double *aWN, double *aHB, double *aHMF, double *aS, double *aO)
 {
     const Box cb = cells_of(g), fb = faces_of(g);
-    hipLaunchKernelGGL(k_predict_b_adj, agrid(cb), dim3(ABT), 0, s, g, cb, met, aHB, aHMF);
     hipLaunchKernelGGL(k_predict_adj_mhd, agrid(cb), dim3(ABT), 0, s, g, m, cb, clk, S, O, met, aHG, aWH,
                        aWN, aS, aO);
-    hipLaunchKernelGGL(k_predict_mf_adj, agrid(fb), dim3(ABT), 0, s, g, fb, clk, aHMF, aS, aO);
+    hipLaunchKernelGGL(k_predict_mf_adj, agrid(fb), dim3(ABT), 0, s, g, fb, clk, met, aHB, aHMF, aS, aO);
 }
 
 // ============================================================ inflow BC
@@ -1303,7 +1307,7 @@ __device__ __forceinline__ void wacc_flush(const WindDev &w, double *gQ, double 
 // the cell's face fluxes), of the inner faces, and of the inflow series
 // samples (gQ (n,5), gB (n,3), column-major; the sample index is a step
 // function of time and carries no derivative).  One thread per transverse
-// point (j, k), all four layers; blocks of 64.
+// point (j, k) and ghost layer (block z); blocks of 64.
 __global__ __launch_bounds__(64) void k_bc_wind_adj(Geo g, MParm m, Box t, const Clock *clk, WindDev w,
                                                     int mhd, const double *__restrict__ S,
                                                     double *__restrict__ aS, const double *__restrict__ met,
@@ -1311,6 +1315,7 @@ __global__ __launch_bounds__(64) void k_bc_wind_adj(Geo g, MParm m, Box t, const
 {
     const int j = t.i0 + (int)(blockIdx.x * 64 + threadIdx.x);
     const int k = t.j0 + (int)blockIdx.y;
+    const int n = 1 + (int)blockIdx.z;   // ghost layer
     WindAcc A;
     A.ia = A.ib = -1;
     for (int q = 0; q < 7; q++) A.a[q] = A.b[q] = 0.0;
@@ -1327,7 +1332,7 @@ __global__ __launch_bounds__(64) void k_bc_wind_adj(Geo g, MParm m, Box t, const
         double U[5], Bi[3], aUp[5] = {0, 0, 0, 0, 0}, aBp[3] = {0, 0, 0};
         for (int v = 0; v < 5; v++) U[v] = cell ? S[(S_GAS + v) * N3 + cp] : 0.0;
         for (int x = 0; x < 3; x++) Bi[x] = cell ? S[(S_B + x) * N3 + cp] : 0.0;
-        for (int n = 4; n >= 1; n--) {
+        {
             const int ig = g.ie + n;
             // ghost face fluxes
             const int fi[3] = {ig + 1, ig, ig};
@@ -1362,8 +1367,8 @@ __global__ __launch_bounds__(64) void k_bc_wind_adj(Geo g, MParm m, Box t, const
                 for (int x = 0; x < 3; x++) v7[1 + x] += awt * dw[x];
                 wacc_add(A, w, gQ, gB, i0, v7);
             }
-            if (!cell) continue;
             // ghost gas
+            if (cell) {
             const long cg = IX(g, ig, j, k);
             double aU[5];
             bool any = false;
@@ -1372,7 +1377,7 @@ __global__ __launch_bounds__(64) void k_bc_wind_adj(Geo g, MParm m, Box t, const
                 aS[(S_GAS + v) * N3 + cg] = 0.0;
                 any |= aU[v] != 0.0;
             }
-            if (!any) continue;
+            if (any) {
             const double xc[3] = {met[(M_XYZCC + 0) * N3 + cg], met[(M_XYZCC + 1) * N3 + cg],
                                   met[(M_XYZCC + 2) * N3 + cg]};
             const int i0 = wind_index(w, wind_lag_time(w, vF, xc, tm));
@@ -1402,14 +1407,19 @@ __global__ __launch_bounds__(64) void k_bc_wind_adj(Geo g, MParm m, Box t, const
             wind_wgt_dv(w, vF, xc, tm, dw);
             for (int x = 0; x < 3; x++) v7[1 + x] += awt * dw[x];
             wacc_add(A, w, gQ, gB, i0, v7);
-            if (!(wMHD > GAM_TINY)) continue;
-            // the interior cell
-            double agi[5];
-            for (int v = 0; v < 5; v++) agi[v] = wMHD * agw[v];
-            wind_inner_adj(m, U, Bi, nH, agi, aUp, aBp);
+            if (wMHD > GAM_TINY) {
+                // the interior cell
+                double agi[5];
+                for (int v = 0; v < 5; v++) agi[v] = wMHD * agw[v];
+                wind_inner_adj(m, U, Bi, nH, agi, aUp, aBp);
+            }
+            }
+            }
         }
         if (cell) {
-            for (int v = 0; v < 5; v++) aS[(S_GAS + v) * N3 + cp] += aUp[v];
+            // the four layers' threads share the last active cell: atomic adds
+            for (int v = 0; v < 5; v++)
+                if (aUp[v] != 0.0) AADD(aS + (S_GAS + v) * N3 + cp, aUp[v]);
             if (mhd && (aBp[0] != 0.0 || aBp[1] != 0.0 || aBp[2] != 0.0)) {
                 // the last active cell's B = its bFlux2Fld from its face fluxes
                 double af[6] = {0, 0, 0, 0, 0, 0};
@@ -1433,6 +1443,6 @@ void launch_bc_wind_adj(hipStream_t s, const Geo &g, const MParm &m, const Clock
     // transverse ranges of the forward (launch_bc, kind 4)
     Box t;
     t.i0 = g.jsg; t.i1 = g.jeg + 1; t.j0 = g.ksg; t.j1 = g.keg + 1; t.k0 = t.k1 = 0;
-    const dim3 gr((unsigned)((t.i1 - t.i0 + 64) / 64), (unsigned)(t.j1 - t.j0 + 1));
+    const dim3 gr((unsigned)((t.i1 - t.i0 + 64) / 64), (unsigned)(t.j1 - t.j0 + 1), 4);
     hipLaunchKernelGGL(k_bc_wind_adj, gr, dim3(64), 0, s, g, m, t, clk, w, m.doMHD ? 1 : 0, S, aS, met, gQ, gB);
 }

@@ -1600,7 +1600,6 @@ static void adj_step(gam_adj *a, const StepIT *it, const double *Sn, const doubl
         launch_emf_adj_gather(s, g, c->mp, eb, a->esc, c->met, a->avf, a->ahmf, a->ahg, lB + S_MF * N3);
         launch_recon_adj_gather(s, g, c->mp, a->fsc, a->rv, a->avf, c->met, a->awh, a->awn, a->ahb,
                                 lB + S_B * N3);
-        hipMemsetAsync(lC + S_B * N3, 0, 3 * pl, s);
         launch_predict_adj_mhd(s, g, c->mp, clk, Sn, On, c->met, a->ahg, a->awh, a->awn, a->ahb, a->ahmf, lB, lC);
     } else {
         adj_bcs(a, lA, Sn1, bclk);
