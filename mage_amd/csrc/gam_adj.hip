@@ -1368,52 +1368,51 @@ __global__ __launch_bounds__(64) void k_bc_wind_adj(Geo g, MParm m, Box t, const
                 wacc_add(A, w, gQ, gB, i0, v7);
             }
             // ghost gas
-            if (cell) {
-            const long cg = IX(g, ig, j, k);
-            double aU[5];
+            double aU[5] = {0, 0, 0, 0, 0};
             bool any = false;
-            for (int v = 0; v < 5; v++) {
-                aU[v] = aS[(S_GAS + v) * N3 + cg];
-                aS[(S_GAS + v) * N3 + cg] = 0.0;
-                any |= aU[v] != 0.0;
-            }
+            const long cg = IX(g, ig, j, k);
+            if (cell)
+                for (int v = 0; v < 5; v++) {
+                    aU[v] = aS[(S_GAS + v) * N3 + cg];
+                    aS[(S_GAS + v) * N3 + cg] = 0.0;
+                    any |= aU[v] != 0.0;
+                }
             if (any) {
-            const double xc[3] = {met[(M_XYZCC + 0) * N3 + cg], met[(M_XYZCC + 1) * N3 + cg],
-                                  met[(M_XYZCC + 2) * N3 + cg]};
-            const int i0 = wind_index(w, wind_lag_time(w, vF, xc, tm));
-            double rs, ps, Vs[3], Bs[3];
-            wind_at(w, vF, xc, tm, &rs, &ps, Vs, Bs);
-            const double wSW = wind_wgt_v(xc, Vs), wMHD = 1.0 - wSW;
-            double gs[5] = {0, 0, 0, 0, 0}, gi[5] = {0, 0, 0, 0, 0};
-            if (wSW > GAM_TINY) {
-                gs[0] = rs; gs[1] = Vs[0]; gs[2] = Vs[1]; gs[3] = Vs[2]; gs[4] = ps;
-            }
-            double nH[3] = {0, 0, 0};
-            if (wMHD > GAM_TINY) {
-                const long fp = IX(g, ip + 1, j, k);
-                for (int x = 0; x < 3; x++) nH[x] = met[(M_TF + x) * N3 + fp];
-                wind_inner(m, U, Bi, nH, gi);
-            }
-            double gw[5], agw[5] = {0, 0, 0, 0, 0};
-            for (int v = 0; v < 5; v++) gw[v] = wSW * gs[v] + wMHD * gi[v];
-            p2c_cell_adj(m, gw, aU, agw);
-            // the series: the sample (when used) and the blend weight
-            double v7[7] = {0, 0, 0, 0, 0, 0, 0};
-            if (wSW > GAM_TINY)
-                for (int v = 0; v < 5; v++) v7[v] = wSW * agw[v];
-            double awt = 0.0;
-            for (int v = 0; v < 5; v++) awt += (gs[v] - gi[v]) * agw[v];
-            double dw[3];
-            wind_wgt_dv(w, vF, xc, tm, dw);
-            for (int x = 0; x < 3; x++) v7[1 + x] += awt * dw[x];
-            wacc_add(A, w, gQ, gB, i0, v7);
-            if (wMHD > GAM_TINY) {
-                // the interior cell
-                double agi[5];
-                for (int v = 0; v < 5; v++) agi[v] = wMHD * agw[v];
-                wind_inner_adj(m, U, Bi, nH, agi, aUp, aBp);
-            }
-            }
+                const double xc[3] = {met[(M_XYZCC + 0) * N3 + cg], met[(M_XYZCC + 1) * N3 + cg],
+                                      met[(M_XYZCC + 2) * N3 + cg]};
+                const int i0 = wind_index(w, wind_lag_time(w, vF, xc, tm));
+                double rs, ps, Vs[3], Bs[3];
+                wind_at(w, vF, xc, tm, &rs, &ps, Vs, Bs);
+                const double wSW = wind_wgt_v(xc, Vs), wMHD = 1.0 - wSW;
+                double gs[5] = {0, 0, 0, 0, 0}, gi[5] = {0, 0, 0, 0, 0};
+                if (wSW > GAM_TINY) {
+                    gs[0] = rs; gs[1] = Vs[0]; gs[2] = Vs[1]; gs[3] = Vs[2]; gs[4] = ps;
+                }
+                double nH[3] = {0, 0, 0};
+                if (wMHD > GAM_TINY) {
+                    const long fp = IX(g, ip + 1, j, k);
+                    for (int x = 0; x < 3; x++) nH[x] = met[(M_TF + x) * N3 + fp];
+                    wind_inner(m, U, Bi, nH, gi);
+                }
+                double gw[5], agw[5] = {0, 0, 0, 0, 0};
+                for (int v = 0; v < 5; v++) gw[v] = wSW * gs[v] + wMHD * gi[v];
+                p2c_cell_adj(m, gw, aU, agw);
+                // the series: the sample (when used) and the blend weight
+                double v7[7] = {0, 0, 0, 0, 0, 0, 0};
+                if (wSW > GAM_TINY)
+                    for (int v = 0; v < 5; v++) v7[v] = wSW * agw[v];
+                double awt = 0.0;
+                for (int v = 0; v < 5; v++) awt += (gs[v] - gi[v]) * agw[v];
+                double dw[3];
+                wind_wgt_dv(w, vF, xc, tm, dw);
+                for (int x = 0; x < 3; x++) v7[1 + x] += awt * dw[x];
+                wacc_add(A, w, gQ, gB, i0, v7);
+                if (wMHD > GAM_TINY) {
+                    // the interior cell
+                    double agi[5];
+                    for (int v = 0; v < 5; v++) agi[v] = wMHD * agw[v];
+                    wind_inner_adj(m, U, Bi, nH, agi, aUp, aBp);
+                }
             }
         }
         if (cell) {
