@@ -804,12 +804,14 @@ __global__ __launch_bounds__(256, 2) void k_emf_adj(Geo g, MParm m, Box eb, cons
 
 // Edge EMF transpose, pass 2: one thread per face position x (every
 // direction T) sums what the edges whose stencils hold it give it:
-// the corner velocities of the edges of direction T-1 along T+1 (aVf,
-// WRITTEN for every face), their edge-B reconstructions of the T fluxes
-// and those of the edges of direction T+1 along T+2 (added to aHMF, the
-// limiter parts to aSSMF with the LFM limiter); and, as a cell, the corner
-// density of the four edges of each direction around it (added to aHG).
-// sc: 16 planes per edge direction from k_emf_adj; edges exist in eb.
+// the corner velocities of the edges of direction T-1 along T+1 (aVf),
+// their edge-B reconstructions of the T fluxes and those of the edges of
+// direction T+1 along T+2 (aHMF; the limiter parts added to aSSMF with
+// the LFM limiter); and, as a cell, the corner density of the four edges
+// of each direction around it (aHG's density; its other planes get 0).
+// aVf, aHMF and aHG are WRITTEN at every point (the flux transpose adds
+// into aHMF and aHG afterwards).  sc: 16 planes per edge direction from
+// k_emf_adj; edges exist in eb.
 template <int RM>
 __global__ __launch_bounds__(256) void k_emf_adj_gather(Geo g, Box fb, EdgeBoxes eb, const double *__restrict__ sc,
                                                         const double *__restrict__ met, int lfm,
@@ -853,7 +855,7 @@ __global__ __launch_bounds__(256) void k_emf_adj_gather(Geo g, Box fb, EdgeBoxes
         }
         const double A = met[(M_FACE + T) * N3 + c], Am = dmax(A, GAM_TINY);
         for (int d = 0; d < 3; d++) aVf[(3 * T + d) * N3 + c] = A * sv[d];
-        aHMF[T * N3 + c] += (A / Am) * sh + (lfm ? 0.0 : sn / Am);
+        aHMF[T * N3 + c] = (A / Am) * sh + (lfm ? 0.0 : sn / Am);
         if (lfm) aSSMF[T * N3 + c] += sn / Am;
     }
     if (i <= g.ieg && j <= g.jeg && k <= g.keg) {
@@ -869,7 +871,8 @@ __global__ __launch_bounds__(256) void k_emf_adj_gather(Geo g, Box fb, EdgeBoxes
                     sd += sc[(16 * ED + 15) * N3 + c + (ab & 1) * st[E1] + (ab >> 1) * st[E2]];
             }
         }
-        aHG[DEN * N3 + c] += 0.25 * sd;
+        aHG[DEN * N3 + c] = 0.25 * sd;
+        for (int v = 1; v < 5; v++) aHG[v * N3 + c] = 0.0;
     }
 }
 

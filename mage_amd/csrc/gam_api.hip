@@ -1586,18 +1586,18 @@ static void adj_step(gam_adj *a, const StepIT *it, const double *Sn, const doubl
         const Box ctb = mkbox(g.is, g.ie + 1, g.js, g.je + 1, g.ks, g.ke + 1);
         launch_update_adj_mhd(s, g, c->mp, act_box(g), ctb, eb, clk, Sn, Sn1, gflx, mflx, c->met, c->bg,
                               lA, lB, a->aur, a->adm, a->ag, a->amf, a->ae);
-        // aVf, aWH, aWN and aHB are written whole by the gathers below
-        hipMemsetAsync(a->ahg, 0, 5 * pl, s);
-        hipMemsetAsync(a->ahmf, 0, 3 * pl, s);
+        // the EMF transpose first: its gather writes aVf, aHMF and aHG whole,
+        // the flux transpose adds into aHMF and aHG, and its gather writes
+        // aWH, aWN and aHB whole (no plane needs zeroing)
+        for (int ed = 0; ed < 3; ed++)
+            launch_emf_adj(s, ed, g, c->mp, eb[ed], clk, H, Sn, vf, c->met, c->bg, a->ae, a->esc);
+        launch_emf_adj_gather(s, g, c->mp, eb, a->esc, c->met, a->avf, a->ahmf, a->ahg, lB + S_MF * N3);
         for (int d = 0; d < 3; d++) {
             Box fb = act_box(g);
             if (d == 0) fb.i1 += 1; else if (d == 1) fb.j1 += 1; else fb.k1 += 1;
             launch_flux_adj_mhd(s, d, g, c->mp, fb, H, Sn, c->met, c->bg, a->ag, a->amf,
                                 a->fsc + (size_t)48 * d * N3, a->ahmf, a->ahg);
         }
-        for (int ed = 0; ed < 3; ed++)
-            launch_emf_adj(s, ed, g, c->mp, eb[ed], clk, H, Sn, vf, c->met, c->bg, a->ae, a->esc);
-        launch_emf_adj_gather(s, g, c->mp, eb, a->esc, c->met, a->avf, a->ahmf, a->ahg, lB + S_MF * N3);
         launch_recon_adj_gather(s, g, c->mp, a->fsc, a->rv, a->avf, c->met, a->awh, a->awn, a->ahb,
                                 lB + S_B * N3);
         launch_predict_adj_mhd(s, g, c->mp, clk, Sn, On, c->met, a->ahg, a->awh, a->awn, a->ahb, a->ahmf, lB, lC);

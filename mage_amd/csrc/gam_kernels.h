@@ -140,7 +140,8 @@ void launch_recon_adj_gather(hipStream_t s, const Geo &g, const MParm &m, const 
 void launch_emf_adj(hipStream_t s, int ed, const Geo &g, const MParm &m, const Box &eb, const Clock *clk,
                     const double *H, const double *S, const double *Vf, const double *met, const double *bg,
                     const double *aE, double *sc);
-// pass 2: writes aVf of every face; adds to aHMF (or aSSMF), aHG's density
+// pass 2: writes aVf, aHMF and aHG at every point (adds the limiter parts
+// to aSSMF with the LFM limiter)
 void launch_emf_adj_gather(hipStream_t s, const Geo &g, const MParm &m, const Box *eboxes, const double *sc,
                            const double *met, double *aVf, double *aHMF, double *aHG, double *aSSMF);
 void launch_vface_rv(hipStream_t s, const Geo &g, const double *met, double *rv);
