@@ -30,7 +30,11 @@
  *
  * Memory: checkpoints of State and oState every ckpt_every steps, and the
  * states of one checkpoint interval, recomputed forward before its
- * adjoint steps run backward.
+ * adjoint steps run backward, with the interval's forward intermediates
+ * (half state, face velocities, face fluxes) as far as HBM allows.  Where
+ * it allows a second set of interval tapes, the next interval is
+ * recomputed on a second stream (default priority) while the current one
+ * runs backward.  gam_adj_forward leaves the last interval on the tapes.
  */
 #ifndef GAMADJ_H
 #define GAMADJ_H
