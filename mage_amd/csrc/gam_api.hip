@@ -1260,6 +1260,9 @@ extern "C" int gam_autotune(gam_ctx *c, int reps)
 // states on a tape) and then runs the adjoint kernels of its steps from
 // last to first.
 enum { ADJ_SET = 11 };   // planes of an adjoint state set: gas 5, face flux 3, cell B 3
+// planes a checkpoint or tape state keeps: gas, face flux, cell B (a step
+// computes its edge EMF before reading it, so a state's E is never input)
+enum { CK_NPL = S_E };
 // planes of one step's forward intermediates: half state (H_NPL), face
 // velocities 9, gas fluxes 15, magnetic fluxes 9
 enum { IT_VF = H_NPL, IT_GF = IT_VF + 9, IT_MF = IT_GF + 15, IT_NPL = IT_MF + 9 };
@@ -1276,7 +1279,7 @@ struct gam_adj {
     int cur0;                // State buffer at the start of the run
     int haveW, recorded;
     int lastTaped;           // the tapes hold the last interval from gam_adj_forward
-    double *ck;              // nck x (State, oState), S_NPL planes each
+    double *ck;              // nck x (State, oState), CK_NPL planes each
     double *fin;             // State, oState at the end
     Clock *clkh;             // clock before step n, n = 0..nsteps
     Clock *oclk;             // clock at the start with t = oState%time
@@ -1361,11 +1364,11 @@ extern "C" int gam_adj_create(gam_adj **out, gam_ctx *c, int nsteps, int K)
     const size_t pl = (size_t)c->g.N3 * sizeof(double);
     hipError_t e = hipSuccess;
 #define AALLOC(ptr, bytes) if (e == hipSuccess) e = hipMalloc((void **)&(ptr), (bytes))
-    AALLOC(a->ck, (size_t)a->nck * 2 * S_NPL * pl);
+    AALLOC(a->ck, (size_t)a->nck * 2 * CK_NPL * pl);
     AALLOC(a->fin, 2 * S_NPL * pl);
     AALLOC(a->clkh, (size_t)(nsteps + 1) * sizeof(Clock));
     AALLOC(a->oclk, sizeof(Clock));
-    AALLOC(a->tape, (size_t)(K + 2) * S_NPL * pl);
+    AALLOC(a->tape, (size_t)(K + 2) * CK_NPL * pl);
     AALLOC(a->lam, 3 * ADJ_SET * pl);
     AALLOC(a->ag, 15 * pl);
     AALLOC(a->amf, 9 * pl);
@@ -1402,7 +1405,7 @@ extern "C" int gam_adj_create(gam_adj **out, gam_ctx *c, int nsteps, int K)
         e = hipMalloc((void **)&a->itape, itb);
         if (e == hipSuccess) e = hipMemset(a->itape, 0, itb);
         // and, where HBM allows, the second pair for the overlapped sweep
-        const size_t t2 = (size_t)(K + 2) * S_NPL * pl;
+        const size_t t2 = (size_t)(K + 2) * CK_NPL * pl;
         if (e == hipSuccess && a->Kt == K && a->nck > 1 && itb + t2 + spare < mfree - itb) {
             e = hipMalloc((void **)&a->itape2, itb);
             if (e == hipSuccess) e = hipMemset(a->itape2, 0, itb);
@@ -1459,14 +1462,14 @@ static int adj_objective(gam_adj *a, const double *S, double *lamS, double *J)
 }
 
 // Put the context at step n's recorded buffers: State/oState from src
-// (two consecutive S_NPL blocks), clock from the record.
-static int adj_put(gam_adj *a, int n, const double *src)
+// (two consecutive blocks of npl planes), clock from the record.
+static int adj_put(gam_adj *a, int n, const double *src, int npl)
 {
     gam_ctx *c = a->c;
-    const size_t bytes = (size_t)S_NPL * c->g.N3 * sizeof(double);
+    const size_t bytes = (size_t)npl * c->g.N3 * sizeof(double);
     c->cur = a->cur0 ^ (n & 1);
     HIPCHK(hipMemcpyAsync(S_cur(c), src, bytes, hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(S_old(c), src + (size_t)S_NPL * c->g.N3, bytes, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(S_old(c), src + (size_t)npl * c->g.N3, bytes, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->clk, a->clkh + n, sizeof(Clock), hipMemcpyDeviceToDevice, c->stream));
     return GAM_OK;
 }
@@ -1478,6 +1481,7 @@ extern "C" int gam_adj_forward(gam_adj *a, double *J)
     gam_ctx *c = a->c;
     const long N3 = c->g.N3;
     const size_t bytes = (size_t)S_NPL * N3 * sizeof(double);
+    const size_t ckb = (size_t)CK_NPL * N3 * sizeof(double);
     HIPCHK(hipStreamSynchronize(c->stream));
     a->cur0 = c->cur;
     const int prof = c->prof;
@@ -1490,14 +1494,14 @@ extern "C" int gam_adj_forward(gam_adj *a, double *J)
     const StepIT own = {c->H, c->vf, c->gflx, c->mflx};
     for (int n = 0; n < a->nsteps; n++) {
         if (n % a->K == 0) {
-            double *ck = a->ck + (size_t)(n / a->K) * 2 * S_NPL * N3;
-            HIPCHK(hipMemcpyAsync(ck, S_cur(c), bytes, hipMemcpyDeviceToDevice, c->stream));
-            HIPCHK(hipMemcpyAsync(ck + (size_t)S_NPL * N3, S_old(c), bytes, hipMemcpyDeviceToDevice, c->stream));
+            double *ck = a->ck + (size_t)(n / a->K) * 2 * CK_NPL * N3;
+            HIPCHK(hipMemcpyAsync(ck, S_cur(c), ckb, hipMemcpyDeviceToDevice, c->stream));
+            HIPCHK(hipMemcpyAsync(ck + (size_t)CK_NPL * N3, S_old(c), ckb, hipMemcpyDeviceToDevice, c->stream));
         }
         if (n >= sL) {
             const int q = n - sL;
-            if (q == 0) HIPCHK(hipMemcpyAsync(a->tape, S_old(c), bytes, hipMemcpyDeviceToDevice, c->stream));
-            HIPCHK(hipMemcpyAsync(a->tape + (size_t)(1 + q) * S_NPL * N3, S_cur(c), bytes,
+            if (q == 0) HIPCHK(hipMemcpyAsync(a->tape, S_old(c), ckb, hipMemcpyDeviceToDevice, c->stream));
+            HIPCHK(hipMemcpyAsync(a->tape + (size_t)(1 + q) * CK_NPL * N3, S_cur(c), ckb,
                                   hipMemcpyDeviceToDevice, c->stream));
             it_use(a, q, 0, own);
         }
@@ -1505,7 +1509,7 @@ extern "C" int gam_adj_forward(gam_adj *a, double *J)
         HIPCHK(hipMemcpyAsync(a->clkh + n + 1, c->clk, sizeof(Clock), hipMemcpyDeviceToDevice, c->stream));
     }
     c->H = own.H; c->vf = own.vf; c->gflx = own.gflx; c->mflx = own.mflx;
-    HIPCHK(hipMemcpyAsync(a->tape + (size_t)(1 + a->nsteps - sL) * S_NPL * N3, S_cur(c), bytes,
+    HIPCHK(hipMemcpyAsync(a->tape + (size_t)(1 + a->nsteps - sL) * CK_NPL * N3, S_cur(c), ckb,
                           hipMemcpyDeviceToDevice, c->stream));
     a->lastTaped = 1;
     HIPCHK(hipGetLastError());
@@ -1529,7 +1533,7 @@ extern "C" int gam_adj_forward(gam_adj *a, double *J)
 static int adj_restore_final(gam_adj *a)
 {
     gam_ctx *c = a->c;
-    int rc = adj_put(a, a->nsteps, a->fin);
+    int rc = adj_put(a, a->nsteps, a->fin, S_NPL);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(c->stream));
     return GAM_OK;
@@ -1622,7 +1626,7 @@ extern "C" int gam_adj_backward(gam_adj *a, double *gGas, double *gGasO, double 
     gam_ctx *c = a->c;
     const Geo &g = c->g;
     const long N3 = g.N3;
-    const size_t bytes = (size_t)S_NPL * N3 * sizeof(double);
+    const size_t bytes = (size_t)CK_NPL * N3 * sizeof(double);   // a tape state
     hipStream_t s = c->stream;
     double *lA = a->lam, *lB = a->lam + ADJ_SET * N3, *lC = a->lam + 2 * ADJ_SET * N3;
     if (a->nw && a->nw != c->wd.n)
@@ -1643,16 +1647,16 @@ extern "C" int gam_adj_backward(gam_adj *a, double *gGas, double *gGasO, double 
     const bool ovl = a->tape2 != nullptr;
     auto recompute = [&](int seg, int buf) -> int {
         const int s0 = seg * a->K, s1 = (s0 + a->K < a->nsteps) ? s0 + a->K : a->nsteps;
-        const double *ck = a->ck + (size_t)seg * 2 * S_NPL * N3;
+        const double *ck = a->ck + (size_t)seg * 2 * CK_NPL * N3;
         double *tp = buf ? a->tape2 : a->tape;
         const StepIT own = {c->H, c->vf, c->gflx, c->mflx};
         c->stream = ovl ? a->rs : s;
-        int r = adj_put(a, s0, ck);
+        int r = adj_put(a, s0, ck, CK_NPL);
         // tape[0] = oState at s0, tape[1 + q] = State at step s0 + q, q = 0..s1-s0
-        if (!r && hipMemcpyAsync(tp, ck + (size_t)S_NPL * N3, bytes, hipMemcpyDeviceToDevice, c->stream))
+        if (!r && hipMemcpyAsync(tp, ck + (size_t)CK_NPL * N3, bytes, hipMemcpyDeviceToDevice, c->stream))
             r = fail(GAM_E_HIP, "gam_adj_backward: tape copy failed");
         for (int q = 0; !r && q <= s1 - s0; q++) {
-            if (hipMemcpyAsync(tp + (size_t)(1 + q) * S_NPL * N3, S_cur(c), bytes, hipMemcpyDeviceToDevice,
+            if (hipMemcpyAsync(tp + (size_t)(1 + q) * CK_NPL * N3, S_cur(c), bytes, hipMemcpyDeviceToDevice,
                                c->stream))
                 r = fail(GAM_E_HIP, "gam_adj_backward: tape copy failed");
             if (!r && q < s1 - s0) {
@@ -1687,8 +1691,8 @@ extern "C" int gam_adj_backward(gam_adj *a, double *gGas, double *gGasO, double 
             const int q = n - s0;
             const bool tpd = it_taped(a, q);
             const StepIT its = tpd ? it_slot(a, q, buf) : StepIT{c->H, c->vf, c->gflx, c->mflx};
-            adj_step(a, tpd ? &its : nullptr, tp + (size_t)(1 + q) * S_NPL * N3, tp + (size_t)q * S_NPL * N3,
-                     tp + (size_t)(2 + q) * S_NPL * N3, a->clkh + n, a->clkh + n + 1, lA, lB, lC);
+            adj_step(a, tpd ? &its : nullptr, tp + (size_t)(1 + q) * CK_NPL * N3, tp + (size_t)q * CK_NPL * N3,
+                     tp + (size_t)(2 + q) * CK_NPL * N3, a->clkh + n, a->clkh + n + 1, lA, lB, lC);
             double *t = lA;
             lA = lB;
             lB = lC;
@@ -1708,7 +1712,7 @@ extern "C" int gam_adj_backward(gam_adj *a, double *gGas, double *gGasO, double 
         launch_cellb_adj(s, g, 1, lB, c->met);
     }
     adj_bcs(a, lA, a->ck, a->clkh);
-    adj_bcs(a, lB, a->ck + (size_t)S_NPL * N3, a->oclk);
+    adj_bcs(a, lB, a->ck + (size_t)CK_NPL * N3, a->oclk);
     HIPCHK(hipEventRecord(a->ev[3], s));
     HIPCHK(hipEventSynchronize(a->ev[3]));
     HIPCHK(hipEventElapsedTime(&a->bwd_ms, a->ev[2], a->ev[3]));
@@ -1726,7 +1730,7 @@ extern "C" int gam_adj_replay(gam_adj *a, const double *Gas, const double *oGas,
     if (!a->recorded) return fail(GAM_E_STATE, "gam_adj_replay: no recorded forward run");
     gam_ctx *c = a->c;
     const long N3 = c->g.N3;
-    int rc = adj_put(a, 0, a->ck);
+    int rc = adj_put(a, 0, a->ck, CK_NPL);
     if (rc) return rc;
     if ((rc = upload(c, Gas, 0, 5, S_cur(c) + S_GAS * N3))) return rc;
     if (oGas && (rc = upload(c, oGas, 0, 5, S_old(c) + S_GAS * N3))) return rc;
