@@ -212,6 +212,44 @@ module gamhip
             import :: c_int, c_ptr
             type(c_ptr), value :: ens
         end function
+        ! discrete adjoint (include/gamadj.h; no counterpart in the reference)
+        integer(c_int) function gam_adj_create(adj, ctx, nsteps, ckpt_every) bind(C, name="gam_adj_create")
+            import :: c_int, c_ptr
+            type(c_ptr) :: adj
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nsteps, ckpt_every
+        end function
+        integer(c_int) function gam_adj_destroy(adj) bind(C, name="gam_adj_destroy")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: adj
+        end function
+        integer(c_int) function gam_adj_set_weights(adj, w) bind(C, name="gam_adj_set_weights")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: adj, w
+        end function
+        integer(c_int) function gam_adj_forward(adj, J) bind(C, name="gam_adj_forward")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: adj
+            real(c_double) :: J
+        end function
+        integer(c_int) function gam_adj_backward(adj, gGas, gGasO, gMF, gMFO) bind(C, name="gam_adj_backward")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: adj, gGas, gGasO, gMF, gMFO
+        end function
+        integer(c_int) function gam_adj_replay(adj, Gas, oGas, MF, oMF, J) bind(C, name="gam_adj_replay")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: adj, Gas, oGas, MF, oMF
+            real(c_double) :: J
+        end function
+        integer(c_int) function gam_adj_wind_grad(adj, gQ, gB) bind(C, name="gam_adj_wind_grad")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: adj, gQ, gB
+        end function
+        integer(c_int) function gam_adj_times(adj, fwd_ms, bwd_ms) bind(C, name="gam_adj_times")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: adj
+            real(c_double) :: fwd_ms, bwd_ms
+        end function
         function c_strlen(s) bind(C, name="strlen")
             import :: c_ptr, c_size_t
             type(c_ptr), value :: s
