@@ -291,6 +291,9 @@ __device__ __forceinline__ void move_adj(double *p, long dst, long src)
 
 // Transpose of the periodic BC's copies (k_bc_periodic): gas and cell B of
 // ghost cells, and the face fluxes with the outer sides' shifted faces.
+// Block z selects the planes: 0 the face fluxes, 1..5 one gas plane, 6..8
+// one cell-B plane (each plane's chain of moves is independent of the
+// others', so they run in parallel instead of one after another).
 template <int SIDE>
 __global__ void k_bc_periodic_adj(Geo g, Box t, double *__restrict__ aS)
 {
@@ -298,6 +301,8 @@ __global__ void k_bc_periodic_adj(Geo g, Box t, double *__restrict__ aS)
     const int b = t.j0 + (int)blockIdx.y;
     if (a > t.i1 || b > t.j1) return;
     const long N3 = g.N3;
+    const int z = (int)blockIdx.z;
+    const int pl = z <= 5 ? S_GAS + z - 1 : S_B + z - 6;
     double *mf = aS + S_MF * N3;
     for (int n = 4; n >= 1; n--) {
         int gi, gj, gk, si, sj, sk;
@@ -309,7 +314,8 @@ __global__ void k_bc_periodic_adj(Geo g, Box t, double *__restrict__ aS)
         else { gk = g.ke + n; sk = g.ks + n - 1; gi = si = a; gj = sj = b; }
         const long d = IX(g, gi, gj, gk), s = IX(g, si, sj, sk);
         // face fluxes (reverse of the forward statement order)
-        if (SIDE == 1) {
+        if (z > 0) {
+        } else if (SIDE == 1) {
             move_adj(mf, IX(g, gi + 1, gj, gk), IX(g, g.is + n, gj, gk));
             move_adj(mf + 2 * N3, d, s);
             move_adj(mf + N3, d, s);
@@ -324,16 +330,15 @@ __global__ void k_bc_periodic_adj(Geo g, Box t, double *__restrict__ aS)
         } else {
             for (int x = 2; x >= 0; x--) move_adj(mf + x * N3, d, s);
         }
-        if (gi <= g.ieg && gj <= g.jeg && gk <= g.keg) {
-            for (int x = 2; x >= 0; x--) move_adj(aS + (S_B + x) * N3, d, s);
-            for (int v = 4; v >= 0; v--) move_adj(aS + v * N3, d, s);
-        }
+        if (z > 0 && gi <= g.ieg && gj <= g.jeg && gk <= g.keg) move_adj(aS + pl * N3, d, s);
     }
 }
 
 // Transpose of the zero-gradient / sphere BC copies with the magnetic
 // face fluxes (k_bc_copy<SIDE, KIND>, KIND 1 zero gradient, 2 sphere in,
-// 3 sphere out).  One thread per transverse point, all four layers.
+// 3 sphere out).  One thread per transverse point, all four layers; block
+// z selects the planes as in k_bc_periodic_adj (the cell-B planes only move
+// with the sphere copies).
 template <int SIDE, int KIND>
 __global__ void k_bc_copy_adj(Geo g, Box t, double *__restrict__ aS, const double *__restrict__ met)
 {
@@ -341,19 +346,32 @@ __global__ void k_bc_copy_adj(Geo g, Box t, double *__restrict__ aS, const doubl
     const int b = t.j0 + (int)blockIdx.y;
     if (a > t.i1 || b > t.j1) return;
     const long N3 = g.N3;
+    const int z = (int)blockIdx.z;
+    const bool movesB = (SIDE == 0 && KIND == 2) || (SIDE == 1 && KIND == 3);
+    if (z > 5 && !movesB) return;
+    if (z > 0) {
+        // the cell planes: plane pl of every ghost layer into the source cell
+        const int pl = z <= 5 ? S_GAS + z - 1 : S_B + z - 6;
+        double *p = aS + pl * N3;
+        for (int n = 4; n >= 1; n--) {
+            long d, sc;
+            if (SIDE == 0) { d = IX(g, g.is - n, a, b); sc = IX(g, g.is, a, b); }
+            else if (SIDE == 1) { d = IX(g, g.ie + n, a, b); sc = IX(g, g.ie, a, b); }
+            else if (SIDE == 2) { d = IX(g, a, g.js - n, b); sc = IX(g, a, g.js, b); }
+            else if (SIDE == 3) { d = IX(g, a, g.je + n, b); sc = IX(g, a, g.je, b); }
+            else if (SIDE == 4) { d = IX(g, a, b, g.ks - n); sc = IX(g, a, b, g.ks); }
+            else { d = IX(g, a, b, g.ke + n); sc = IX(g, a, b, g.ke); }
+            move_adj(p, d, sc);
+        }
+        return;
+    }
+    // z = 0: the face fluxes
     double *mf = aS + S_MF * N3;
-    auto cells = [&](long d, long s, bool withB) {
-        for (int v = 4; v >= 0; v--) move_adj(aS + v * N3, d, s);
-        if (withB)
-            for (int x = 2; x >= 0; x--) move_adj(aS + (S_B + x) * N3, d, s);
-    };
     if (SIDE == 0) {
         const long s = IX(g, g.is, a, b);
         for (int n = 4; n >= 1; n--) {
             const long d = IX(g, g.is - n, a, b);
-            if (KIND == 2) for (int x = 2; x >= 0; x--) move_adj(aS + (S_B + x) * N3, d, s);
             for (int x = 2; x >= 0; x--) move_adj(mf + x * N3, d, s);
-            for (int v = 4; v >= 0; v--) move_adj(aS + v * N3, d, s);
         }
     } else if (SIDE == 1) {
         const long s = IX(g, g.ie, a, b);
@@ -361,11 +379,9 @@ __global__ void k_bc_copy_adj(Geo g, Box t, double *__restrict__ aS, const doubl
             for (int n = 4; n >= 1; n--) {
                 const int ig = g.ie + n;
                 const long d = IX(g, ig, a, b);
-                for (int x = 2; x >= 0; x--) move_adj(aS + (S_B + x) * N3, d, s);
                 move_adj(mf + 2 * N3, d, s);
                 move_adj(mf + N3, d, s);
                 move_adj(mf, IX(g, ig + 1, a, b), s);
-                for (int v = 4; v >= 0; v--) move_adj(aS + v * N3, d, s);
             }
         } else {
             // ghost fluxes = face area x (B . normal), B = cell B of the last
@@ -382,7 +398,6 @@ __global__ void k_bc_copy_adj(Geo g, Box t, double *__restrict__ aS, const doubl
                     const double A = met[(M_FACE + x) * N3 + f];
                     for (int y = 0; y < 3; y++) aB[y] += A * met[(M_TF + 9 * x + y) * N3 + f] * am;
                 }
-                for (int v = 4; v >= 0; v--) move_adj(aS + v * N3, d, s);
             }
             // the J and K faces of the last cell are shared with the
             // neighbouring transverse threads: atomic adds
@@ -400,7 +415,6 @@ __global__ void k_bc_copy_adj(Geo g, Box t, double *__restrict__ aS, const doubl
         for (int n = 4; n >= 1; n--) {
             const long d = IX(g, a, g.js - n, b);
             for (int x = 2; x >= 0; x--) move_adj(mf + x * N3, d, s);
-            cells(d, s, false);
         }
     } else if (SIDE == 3) {
         const long s = IX(g, a, g.je, b);
@@ -409,14 +423,12 @@ __global__ void k_bc_copy_adj(Geo g, Box t, double *__restrict__ aS, const doubl
             move_adj(mf + N3, IX(g, a, g.je + n + 1, b), IX(g, a, g.je + 1, b));
             move_adj(mf + 2 * N3, d, s);
             move_adj(mf, d, s);
-            cells(d, s, false);
         }
     } else if (SIDE == 4) {
         const long s = IX(g, a, b, g.ks);
         for (int n = 4; n >= 1; n--) {
             const long d = IX(g, a, b, g.ks - n);
             for (int x = 2; x >= 0; x--) move_adj(mf + x * N3, d, s);
-            cells(d, s, false);
         }
     } else {
         const long s = IX(g, a, b, g.ke);
@@ -428,7 +440,6 @@ __global__ void k_bc_copy_adj(Geo g, Box t, double *__restrict__ aS, const doubl
             mf[2 * N3 + fk] = 0.0;
             move_adj(mf + N3, d, s);
             move_adj(mf, d, s);
-            cells(d, s, false);
         }
     }
 }
@@ -1068,7 +1079,7 @@ static void bcp_adj_side(hipStream_t s, const Geo &g, double *aS)
     else if (SIDE <= 3) { t.i0 = g.isg; t.i1 = g.ieg + 1; t.j0 = g.ksg; t.j1 = g.keg + 1; }
     else { t.i0 = g.isg; t.i1 = g.ieg + 1; t.j0 = g.jsg; t.j1 = g.jeg + 1; }
     t.k0 = t.k1 = 0;
-    const dim3 gr((unsigned)((t.i1 - t.i0 + 64) / 64), (unsigned)(t.j1 - t.j0 + 1));
+    const dim3 gr((unsigned)((t.i1 - t.i0 + 64) / 64), (unsigned)(t.j1 - t.j0 + 1), 9);
     hipLaunchKernelGGL(k_bc_periodic_adj<SIDE>, gr, dim3(64), 0, s, g, t, aS);
 }
 
@@ -1080,7 +1091,8 @@ static void bcc_adj_side(hipStream_t s, const Geo &g, double *aS, const double *
     else if (SIDE <= 3) { t.i0 = g.isg; t.i1 = g.ieg; t.j0 = g.ksg; t.j1 = g.keg; }
     else { t.i0 = g.isg; t.i1 = g.ieg; t.j0 = g.jsg; t.j1 = g.jeg; }
     t.k0 = t.k1 = 0;
-    const dim3 gr((unsigned)((t.i1 - t.i0 + 64) / 64), (unsigned)(t.j1 - t.j0 + 1));
+    const bool movesB = (SIDE == 0 && KIND == 2) || (SIDE == 1 && KIND == 3);
+    const dim3 gr((unsigned)((t.i1 - t.i0 + 64) / 64), (unsigned)(t.j1 - t.j0 + 1), movesB ? 9 : 6);
     hipLaunchKernelGGL((k_bc_copy_adj<SIDE, KIND>), gr, dim3(64), 0, s, g, t, aS, met);
 }
 
@@ -1279,26 +1291,28 @@ __device__ __forceinline__ void wacc_add(WindAcc &A, const WindDev &w, double *g
     }
 }
 
-// Flush one accumulator slot of every lane of the (64-lane) block: one
-// reduction and one set of atomics when all lanes hold the same sample,
-// per-lane atomics otherwise.  Every lane of the wave must call it.
+// Flush one accumulator slot of every lane of the (64-lane) block: the
+// lanes holding the same sample are summed across the wave and one lane
+// adds the sum, one set of atomics per distinct sample.  Every lane of the
+// wave must call it.
 __device__ __forceinline__ void wacc_flush(const WindDev &w, double *gQ, double *gB, int key, double *v)
 {
-    int lo = key < 0 ? 0x7fffffff : key, hi = key;
-    for (int o = 32; o >= 1; o >>= 1) {
-        lo = min(lo, __shfl_xor(lo, o, 64));
-        hi = max(hi, __shfl_xor(hi, o, 64));
-    }
-    if (hi < 0) return;
-    if (lo == hi) {
+    const int lane = (int)threadIdx.x;
+    bool pend = key >= 0;
+    for (;;) {
+        const unsigned long long mk = __ballot(pend);
+        if (mk == 0ull) break;
+        const int lead = __builtin_ctzll(mk);
+        const int k0 = __shfl(key, lead, 64);
+        const bool mine = pend && key == k0;
+        double sv[7];
         for (int q = 0; q < 7; q++) {
-            double s = key < 0 ? 0.0 : v[q];
-            for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
-            v[q] = s;
+            double x = mine ? v[q] : 0.0;
+            for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o, 64);
+            sv[q] = x;
         }
-        if (threadIdx.x == 0) wacc_store(w, gQ, gB, hi, v);
-    } else if (key >= 0) {
-        wacc_store(w, gQ, gB, key, v);
+        if (lane == lead) wacc_store(w, gQ, gB, k0, sv);
+        if (mine) pend = false;
     }
 }
 
@@ -1309,30 +1323,41 @@ __device__ __forceinline__ void wacc_flush(const WindDev &w, double *gQ, double 
 // Adds the adjoints of the last active cell (gas, and its cell B through
 // the cell's face fluxes), of the inner faces, and of the inflow series
 // samples (gQ (n,5), gB (n,3), column-major; the sample index is a step
-// function of time and carries no derivative).  One thread per transverse
-// point (j, k) and ghost layer (block z); blocks of 64.
+// function of time and carries no derivative) into copy b % WIND_COPIES of
+// the series adjoint for block b, which spreads the atomics of the many
+// points that read one sample.  One thread per transverse point (j, k) and
+// ghost layer; a 64-lane block holds 16 points along j with their four
+// layers in adjacent lanes, so that a load reads 32 contiguous bytes (the
+// layers along i) instead of 8.
 __global__ __launch_bounds__(64) void k_bc_wind_adj(Geo g, MParm m, Box t, const Clock *clk, WindDev w,
                                                     int mhd, const double *__restrict__ S,
                                                     double *__restrict__ aS, const double *__restrict__ met,
                                                     double *__restrict__ gQ, double *__restrict__ gB)
 {
-    const int j = t.i0 + (int)(blockIdx.x * 64 + threadIdx.x);
+    const int n = 1 + (int)(threadIdx.x & 3);   // ghost layer
+    const int j = t.i0 + (int)(blockIdx.x * 16 + (threadIdx.x >> 2));
     const int k = t.j0 + (int)blockIdx.y;
-    const int n = 1 + (int)blockIdx.z;   // ghost layer
+    {
+        const long cpy = (long)((blockIdx.x + blockIdx.y * gridDim.x) % WIND_COPIES);
+        gQ += cpy * 5 * w.n;
+        gB += cpy * 3 * w.n;
+    }
     WindAcc A;
     A.ia = A.ib = -1;
     for (int q = 0; q < 7; q++) A.a[q] = A.b[q] = 0.0;
-    if (j <= t.i1 && k <= t.j1) {
-        const long N3 = g.N3;
-        double *amf = aS + S_MF * N3;
+    const long N3 = g.N3;
+    double *amf = aS + S_MF * N3;
+    const int ip = g.ie;
+    const long cp = IX(g, ip, j, k);
+    const bool live = j <= t.i1 && k <= t.j1;
+    const bool cell = live && k <= g.keg && j <= g.jeg;
+    double aUp[5] = {0, 0, 0, 0, 0}, aBp[3] = {0, 0, 0};
+    if (live) {
         const double *mf = S + S_MF * N3;
         const double tm = clk->t;
         double vF[3];
         wind_front(w, tm, vF);
-        const int ip = g.ie;
-        const long cp = IX(g, ip, j, k);
-        const bool cell = (k <= g.keg && j <= g.jeg);
-        double U[5], Bi[3], aUp[5] = {0, 0, 0, 0, 0}, aBp[3] = {0, 0, 0};
+        double U[5], Bi[3];
         for (int v = 0; v < 5; v++) U[v] = cell ? S[(S_GAS + v) * N3 + cp] : 0.0;
         for (int x = 0; x < 3; x++) Bi[x] = cell ? S[(S_B + x) * N3 + cp] : 0.0;
         {
@@ -1418,21 +1443,31 @@ __global__ __launch_bounds__(64) void k_bc_wind_adj(Geo g, MParm m, Box t, const
                 }
             }
         }
-        if (cell) {
-            // the four layers' threads share the last active cell: atomic adds
-            for (int v = 0; v < 5; v++)
-                if (aUp[v] != 0.0) AADD(aS + (S_GAS + v) * N3 + cp, aUp[v]);
-            if (mhd && (aBp[0] != 0.0 || aBp[1] != 0.0 || aBp[2] != 0.0)) {
-                // the last active cell's B = its bFlux2Fld from its face fluxes
-                double af[6] = {0, 0, 0, 0, 0, 0};
-                cell_b_adj(g, met, cp, aBp, af);
-                AADD(amf + cp, af[0]);
-                AADD(amf + N3 + cp, af[1]);
-                AADD(amf + 2 * N3 + cp, af[2]);
-                AADD(amf + cp + 1, af[3]);
-                AADD(amf + N3 + cp + g.P, af[4]);
-                AADD(amf + 2 * N3 + cp + g.PP, af[5]);
-            }
+    }
+    // the four layers' lanes share the last active cell: summed across them,
+    // added by the first (no other thread writes that cell's gas adjoint)
+    for (int v = 0; v < 5; v++) {
+        aUp[v] += __shfl_xor(aUp[v], 1, 64);
+        aUp[v] += __shfl_xor(aUp[v], 2, 64);
+    }
+    for (int x = 0; x < 3; x++) {
+        aBp[x] += __shfl_xor(aBp[x], 1, 64);
+        aBp[x] += __shfl_xor(aBp[x], 2, 64);
+    }
+    if (cell && n == 1) {
+        for (int v = 0; v < 5; v++)
+            if (aUp[v] != 0.0) aS[(S_GAS + v) * N3 + cp] += aUp[v];
+        if (mhd && (aBp[0] != 0.0 || aBp[1] != 0.0 || aBp[2] != 0.0)) {
+            // the last active cell's B = its bFlux2Fld from its face fluxes
+            // (faces shared with the neighbouring points: atomic adds)
+            double af[6] = {0, 0, 0, 0, 0, 0};
+            cell_b_adj(g, met, cp, aBp, af);
+            AADD(amf + cp, af[0]);
+            AADD(amf + N3 + cp, af[1]);
+            AADD(amf + 2 * N3 + cp, af[2]);
+            AADD(amf + cp + 1, af[3]);
+            AADD(amf + N3 + cp + g.P, af[4]);
+            AADD(amf + 2 * N3 + cp + g.PP, af[5]);
         }
     }
     wacc_flush(w, gQ, gB, A.ia, A.a);
@@ -1445,6 +1480,6 @@ void launch_bc_wind_adj(hipStream_t s, const Geo &g, const MParm &m, const Clock
     // transverse ranges of the forward (launch_bc, kind 4)
     Box t;
     t.i0 = g.jsg; t.i1 = g.jeg + 1; t.j0 = g.ksg; t.j1 = g.keg + 1; t.k0 = t.k1 = 0;
-    const dim3 gr((unsigned)((t.i1 - t.i0 + 64) / 64), (unsigned)(t.j1 - t.j0 + 1), 4);
+    const dim3 gr((unsigned)((t.i1 - t.i0 + 16) / 16), (unsigned)(t.j1 - t.j0 + 1), 1);
     hipLaunchKernelGGL(k_bc_wind_adj, gr, dim3(64), 0, s, g, m, t, clk, w, m.doMHD ? 1 : 0, S, aS, met, gQ, gB);
 }

@@ -1303,7 +1303,7 @@ struct gam_adj {
     double *wgt;             // objective weights (1 plane)
     double *part;            // objective partial sums (1 plane)
     int nw;                  // inflow series length (0: no inflow BC)
-    double *gwq, *gwb;       // adjoints of the inflow series Q (nw,5), B (nw,3)
+    double *gwq, *gwb;       // adjoints of the inflow series Q (nw,5), B (nw,3), WIND_COPIES copies each
     hipEvent_t ev[4];
     float fwd_ms, bwd_ms;
 };
@@ -1389,8 +1389,8 @@ extern "C" int gam_adj_create(gam_adj **out, gam_ctx *c, int nsteps, int K)
     for (int sd = 0; sd < 6; sd++)
         if (c->model.bc[sd] == GAM_BC_WIND) a->nw = c->wd.n;
     if (a->nw) {
-        AALLOC(a->gwq, (size_t)a->nw * 5 * sizeof(double));
-        AALLOC(a->gwb, (size_t)a->nw * 3 * sizeof(double));
+        AALLOC(a->gwq, (size_t)WIND_COPIES * a->nw * 5 * sizeof(double));
+        AALLOC(a->gwb, (size_t)WIND_COPIES * a->nw * 3 * sizeof(double));
     }
 #undef AALLOC
     // the interval's forward intermediates, kept where HBM allows (otherwise
@@ -1635,8 +1635,8 @@ extern "C" int gam_adj_backward(gam_adj *a, double *gGas, double *gGasO, double 
     HIPCHK(hipEventRecord(a->ev[2], s));
     HIPCHK(hipMemsetAsync(a->lam, 0, 3 * ADJ_SET * N3 * sizeof(double), s));
     if (a->nw) {
-        HIPCHK(hipMemsetAsync(a->gwq, 0, (size_t)a->nw * 5 * sizeof(double), s));
-        HIPCHK(hipMemsetAsync(a->gwb, 0, (size_t)a->nw * 3 * sizeof(double), s));
+        HIPCHK(hipMemsetAsync(a->gwq, 0, (size_t)WIND_COPIES * a->nw * 5 * sizeof(double), s));
+        HIPCHK(hipMemsetAsync(a->gwb, 0, (size_t)WIND_COPIES * a->nw * 3 * sizeof(double), s));
     }
     int rc = adj_objective(a, a->fin, lA, nullptr);
     if (rc) return rc;
@@ -1764,8 +1764,21 @@ extern "C" int gam_adj_wind_grad(gam_adj *a, double *gQ, double *gB)
     if (!a->nw) return fail(GAM_E_STATE, "gam_adj_wind_grad: the model has no inflow BC");
     gam_ctx *c = a->c;
     HIPCHK(hipStreamSynchronize(c->stream));
-    HIPCHK(hipMemcpy(gQ, a->gwq, (size_t)a->nw * 5 * sizeof(double), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(gB, a->gwb, (size_t)a->nw * 3 * sizeof(double), hipMemcpyDeviceToHost));
+    // the kernels add into WIND_COPIES copies of each array: sum them in order
+    const size_t nq = (size_t)a->nw * 5, nb = (size_t)a->nw * 3;
+    std::vector<double> hq(WIND_COPIES * nq), hb(WIND_COPIES * nb);
+    HIPCHK(hipMemcpy(hq.data(), a->gwq, hq.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(hb.data(), a->gwb, hb.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t e = 0; e < nq; e++) {
+        double v = 0.0;
+        for (int q = 0; q < WIND_COPIES; q++) v += hq[q * nq + e];
+        gQ[e] = v;
+    }
+    for (size_t e = 0; e < nb; e++) {
+        double v = 0.0;
+        for (int q = 0; q < WIND_COPIES; q++) v += hb[q * nb + e];
+        gB[e] = v;
+    }
     return GAM_OK;
 }
 

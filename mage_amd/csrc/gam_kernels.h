@@ -117,7 +117,9 @@ void launch_set_dt(hipStream_t s, Clock *clk, const Clock *rec);
 void launch_cellb_adj(hipStream_t s, const Geo &g, int all, double *aS, const double *met);
 void launch_bc_periodic_adj(hipStream_t s, int side, const Geo &g, double *aS);
 void launch_bc_mhd_adj(hipStream_t s, int side, int kind, const Geo &g, double *aS, const double *met);
-// transpose of the inflow BC; adds the series adjoint into gQ (n,5), gB (n,3)
+// transpose of the inflow BC; adds the series adjoint into gQ (n,5), gB (n,3),
+// each WIND_COPIES consecutive copies of those arrays (the caller sums them)
+#define WIND_COPIES 32
 void launch_bc_wind_adj(hipStream_t s, const Geo &g, const MParm &m, const Clock *clk, const WindDev &w,
                         const double *S, double *aS, const double *met, double *gQ, double *gB);
 void launch_update_adj_mhd(hipStream_t s, const Geo &g, const MParm &m, const Box &ab, const Box &ctb,

@@ -1150,12 +1150,15 @@ __device__ __forceinline__ bool is_cell(const Geo &g, int i, int j, int k)
     return i >= g.isg && i <= g.ieg && j >= g.jsg && j <= g.jeg && k >= g.ksg && k <= g.keg;
 }
 
-// Ghost layers n = 1..4: the i-side kernels (one thread per (j, k) row of
-// 4-wide ghost slabs, uncoalesced along i) take n from the block's z index, one
-// layer per thread, for four times the parallelism; the j- and k-side
-// kernels (threads along i, coalesced) loop over n.
-#define NL0(side) (((side) <= 1) ? 1 + (int)bi.z : 1)
-#define NL1(side) (((side) <= 1) ? 1 + (int)bi.z : 4)
+// Ghost layers n = 1..4: the i-side kernels (points (j, k) with 4-wide
+// ghost slabs along i) run one layer per thread, the four layers of a point
+// in adjacent lanes (a 64-lane block covers 16 points along j), so that one
+// load reads 32 contiguous bytes along i rather than 8; the j- and k-side
+// kernels (threads along i, coalesced) loop over n.  TA = the thread's
+// transverse offset along the first transverse axis.
+#define NL0(side) (((side) <= 1) ? 1 + (int)(threadIdx.x & 3) : 1)
+#define NL1(side) (((side) <= 1) ? 1 + (int)(threadIdx.x & 3) : 4)
+#define TA(side) (((side) <= 1) ? (int)(bi.x * 16 + (threadIdx.x >> 2)) : (int)(bi.x * blockDim.x + threadIdx.x))
 
 // periodic (bcs.F90:279-450).  side: 0..5.  (a,b) = transverse indices
 template <int SIDE>
@@ -1163,7 +1166,7 @@ struct k_bc_periodic {
     __device__ __forceinline__ void operator()(const uint3 bi, Geo g, Box t,
                                                double *__restrict__ S) const
     {
-        const int a = t.i0 + bi.x * blockDim.x + threadIdx.x;
+        const int a = t.i0 + TA(SIDE);
         const int b = t.j0 + bi.y;
         if (a > t.i1 || b > t.j1) return;
         const long N3 = g.N3;
@@ -1217,7 +1220,7 @@ struct k_bc_copy {
                                                double *__restrict__ S,
                                                const double *__restrict__ met) const
     {
-        const int a = t.i0 + bi.x * blockDim.x + threadIdx.x;
+        const int a = t.i0 + TA(SIDE);
         const int b = t.j0 + bi.y;
         if (a > t.i1 || b > t.j1) return;
         const long N3 = g.N3;
@@ -1304,7 +1307,7 @@ struct k_bc_wind {
                                                const Clock *clk, WindDev w, double *__restrict__ S,
                                                const double *__restrict__ met) const
     {
-        const int j = t.i0 + bi.x * blockDim.x + threadIdx.x;
+        const int j = t.i0 + TA(1);
         const int k = t.j0 + bi.y;
         if (j > t.i1 || k > t.j1) return;
         const long N3 = g.N3;
@@ -1362,7 +1365,7 @@ struct k_bc_wind_b {
                                                double *__restrict__ S,
                                                const double *__restrict__ met) const
     {
-        const int j = t.i0 + bi.x * blockDim.x + threadIdx.x;
+        const int j = t.i0 + TA(1);
         const int k = t.j0 + bi.y;
         if (j > t.i1 || k > t.j1) return;
         const long N3 = g.N3;
@@ -1644,15 +1647,13 @@ static inline dim3 grid2(const Box &t) { return dim3((unsigned)((t.i1 - t.i0 + 6
 
 // Both periodic sides of one axis in one launch: each reads only interior
 // cells and faces and writes its own ghosts, so the two commute.  Blocks
-// with z below nz run the low side (with their own z), the rest the high
-// side.
+// with z = 0 run the low side, z = 1 the high side.
 template <int AX>
 struct k_bc_periodic_pair {
     __device__ __forceinline__ void operator()(const uint3 bi, Geo g, Box t, double *__restrict__ S) const
     {
-        const unsigned nz = (AX == 0) ? 4u : 1u;
-        if (bi.z < nz) k_bc_periodic<2 * AX>{}(bi, g, t, S);
-        else k_bc_periodic<2 * AX + 1>{}(make_uint3(bi.x, bi.y, bi.z - nz), g, t, S);
+        if (bi.z == 0) k_bc_periodic<2 * AX>{}(bi, g, t, S);
+        else k_bc_periodic<2 * AX + 1>{}(make_uint3(bi.x, bi.y, 0), g, t, S);
     }
 };
 
@@ -1664,7 +1665,8 @@ void launch_bc_periodic_pair(hipStream_t s, int ax, const Geo &g, double *S)
     else { t.i0 = g.isg; t.i1 = g.ieg + 1; t.j0 = g.jsg; t.j1 = g.jeg + 1; }
     t.k0 = t.k1 = 0;
     dim3 gr = grid2(t), bl(64);
-    gr.z = (ax == 0) ? 8 : 2;
+    if (ax == 0) gr.x = (unsigned)((t.i1 - t.i0 + 16) / 16);   // 16 points x 4 layers per block
+    gr.z = 2;
     if (ax == 0) launch_f<k_bc_periodic_pair<0>>(gr, bl, s, g, t, S);
     else if (ax == 1) launch_f<k_bc_periodic_pair<1>>(gr, bl, s, g, t, S);
     else launch_f<k_bc_periodic_pair<2>>(gr, bl, s, g, t, S);
@@ -1681,7 +1683,7 @@ void launch_bc(hipStream_t s, int side, int kind, const Geo &g, const MParm &m, 
     else { t.i0 = g.isg; t.i1 = g.ieg + (pi ? 1 : 0); t.j0 = g.jsg; t.j1 = g.jeg + (pi ? 1 : 0); }
     t.k0 = t.k1 = 0;
     dim3 gr = grid2(t), bl(64);
-    if (side <= 1) gr.z = 4;    // one ghost layer per block z index
+    if (side <= 1) gr.x = (unsigned)((t.i1 - t.i0 + 16) / 16);   // 16 points x 4 layers per block
     if (kind == 0) {
         switch (side) {
         case 0: launch_f<k_bc_periodic<0>>(gr, bl, s, g, t, S); break;
@@ -1708,7 +1710,7 @@ void launch_bc(hipStream_t s, int side, int kind, const Geo &g, const MParm &m, 
         Box tw = t;
         tw.i1 = g.jeg + 1; tw.j1 = g.keg + 1;
         dim3 gw = grid2(tw);
-        gw.z = 4;
+        gw.x = (unsigned)((tw.i1 - tw.i0 + 16) / 16);
         launch_f<k_bc_wind>(gw, bl, s, g, m, tw, clk, *w, S, met);
         // ghost-cell B of the wind BC (wind.F90:338-348); skipped when a
         // bFlux2Fld over every ghost cell follows, which overwrites it
