@@ -136,18 +136,21 @@ __device__ __forceinline__ void c2p_block_adj(const MParm &m, const double *U, c
 }
 
 // ---------------------------------------------------------- limiter
-// qN = dmax(dmin(a, b), dmin(qI, dmax(a, b))): adjoint of the clipped value
+// qN = dmax(dmin(a, b), dmin(qI, dmax(a, b))): adjoint of the clipped value.
+// Each output takes aq or 0 by a value select: with branches the compiler
+// merges the two adds into one add through a selected address, and the
+// caller's arrays (the limiter adjoints) then live in scratch memory.
 __device__ __forceinline__ void clip_adj(double a, double b, double qI, double aq, double &aa,
                                          double &ab, double &aqI)
 {
     const double m1 = dmin(a, b), M = dmax(a, b), m2 = dmin(qI, M);
-    if (m1 > m2) {
-        if (a < b) aa += aq; else ab += aq;
-    } else {
-        if (qI < M) aqI += aq;
-        else if (a > b) aa += aq;
-        else ab += aq;
-    }
+    const bool lo = m1 > m2;
+    const bool toQ = !lo && qI < M;
+    const bool toA = lo ? (a < b) : (!toQ && a > b);
+    const bool toB = !toQ && !toA;
+    aa += toA ? aq : 0.0;
+    ab += toB ? aq : 0.0;
+    aqI += toQ ? aq : 0.0;
 }
 
 // PDM (pdm_one)

@@ -1100,10 +1100,13 @@ struct k_bflux2fld {
 // bFlux2Fld over the ghost shell only: up to six boxes (the i, j and k
 // ghost slabs, disjoint), each cut into 64 x 4 (i, j) tiles; workgroup w
 // finds its box from the running tile counts.
+// A tile of BX*BY threads covers iw[q] points along i by BX*BY/iw[q] along
+// j: 64 x 4 in the wide boxes, 4 x 64 in the 4-wide i-ghost slabs (every
+// lane busy, 32 contiguous bytes per row).
 struct GhostBoxes {
     Box b[6];
     int start[7];     // first tile of each box; start[n] = total
-    int n0[6], n1[6];
+    int n0[6], n1[6], iw[6];
     int n;
 };
 
@@ -1120,8 +1123,9 @@ struct k_bflux2fld_ghosts {
         const Box &b = gb.b[q];
         const int t0 = L % gb.n0[q];
         const int r = L / gb.n0[q];
-        const int i = b.i0 + t0 * BX + threadIdx.x;
-        const int j = b.j0 + (r % gb.n1[q]) * BY + threadIdx.y;
+        const int tid = (int)(threadIdx.x + BX * threadIdx.y), iw = gb.iw[q];
+        const int i = b.i0 + t0 * iw + tid % iw;
+        const int j = b.j0 + (r % gb.n1[q]) * (BX * BY / iw) + tid / iw;
         const int k = b.k0 + r / gb.n1[q];
         if (i > b.i1 || j > b.j1 || k > b.k1) return;
         const long N3 = g.N3, c = IX(g, i, j, k);
@@ -1634,8 +1638,10 @@ void launch_bflux2fld_ghosts(hipStream_t s, const Geo &g, const double *mf, doub
     for (int q = 0; q < 6; q++) {
         gb.b[q] = bx[q];
         gb.start[q] = tot;
-        gb.n0[q] = (bx[q].i1 - bx[q].i0 + BX) / BX;
-        gb.n1[q] = (bx[q].j1 - bx[q].j0 + BY) / BY;
+        gb.iw[q] = (q >= 4) ? 4 : BX;
+        const int jw = BX * BY / gb.iw[q];
+        gb.n0[q] = (bx[q].i1 - bx[q].i0 + gb.iw[q]) / gb.iw[q];
+        gb.n1[q] = (bx[q].j1 - bx[q].j0 + jw) / jw;
         tot += gb.n0[q] * gb.n1[q] * (bx[q].k1 - bx[q].k0 + 1);
     }
     gb.start[6] = tot;
