@@ -583,14 +583,14 @@ __global__ __launch_bounds__(256, 2) void k_flux_adj_mhd(Geo g, MParm m, Box fb,
     const long N3 = g.N3;
     const long c = IX(g, i, j, k);
     const long s = STRIDE<D>(g);
-    // byte offsets of the 8 stencil points shared by every plane (as k_flux);
-    // along i plain indexing lets the compiler merge neighbouring points
-    unsigned o[8];
-#pragma unroll
-    for (int n = 0; n < 8; n++) o[n] = (unsigned)((c + (n - 4) * s) * 8);
+    // stencil point n of a plane: a 32-bit byte offset from the plane base
+    // (as k_flux), formed where it is used rather than kept in registers
+    // through the kernel; along i plain indexing lets the compiler merge
+    // neighbouring points
+    const unsigned oc = (unsigned)(c * 8), os = (unsigned)(s * 8);
     auto LD = [&](const double *__restrict__ plane, int n) -> double {
         if (D == 0) return plane[c + (n - 4)];
-        return ldp(plane, o[n]);
+        return ldp(plane, oc + (unsigned)(n - 4) * os);
     };
     double vol[8];
     for (int n = 0; n < 8; n++) vol[n] = LD(met + M_VOL * N3, n);
@@ -610,14 +610,14 @@ __global__ __launch_bounds__(256, 2) void k_flux_adj_mhd(Geo g, MParm m, Box fb,
         recon_lr<RM>(m.pdmb, vol, rVi, nq, q, BL[x], BR[x]);
     }
     double T[9];
-    for (int q = 0; q < 9; q++) T[q] = ldp(met + (M_TF + q + 9 * D) * N3, o[4]);
-    const double faB = ldp(met + (M_FACE + D) * N3, o[4]);
+    for (int q = 0; q < 9; q++) T[q] = ldp(met + (M_TF + q + 9 * D) * N3, oc);
+    const double faB = ldp(met + (M_FACE + D) * N3, oc);
     const double iFaB = 1.0 / faB;
-    const double Bn = ldp(H + (H_MF + D) * N3, o[4]) * iFaB;
+    const double Bn = ldp(H + (H_MF + D) * N3, oc) * iFaB;
     double B0[3] = {0, 0, 0}, B0n = 0.0;
     if (BG) {
-        for (int x = 0; x < 3; x++) B0[x] = ldp(bg + (B_FCB0 + x + 3 * D) * N3, o[4]);
-        B0n = ldp(bg + (B_BFLUX0 + D) * N3, o[4]) * iFaB;
+        for (int x = 0; x < 3; x++) B0[x] = ldp(bg + (B_FCB0 + x + 3 * D) * N3, oc);
+        B0n = ldp(bg + (B_BFLUX0 + D) * N3, oc) * iFaB;
     }
     // forward pieces the adjoint needs: VaD, bbD and the HOGS differences dW
     double mF[3] = {0, 0, 0}, bbD = 0.0, VaD = 0.0;
@@ -630,11 +630,11 @@ __global__ __launch_bounds__(256, 2) void k_flux_adj_mhd(Geo g, MParm m, Box fb,
     double aF[5], adW[5], aM[3], aVaD = 0.0, abbD = 0.0;
     double aL[5] = {0, 0, 0, 0, 0}, aR[5] = {0, 0, 0, 0, 0}, abL[3] = {0, 0, 0}, abR[3] = {0, 0, 0}, aBn = 0.0;
     for (int v = 0; v < 5; v++) {
-        aF[v] = faB * ldp(ag + (v + 5 * D) * N3, o[4]);
+        aF[v] = faB * ldp(ag + (v + 5 * D) * N3, oc);
         adW[v] = -m.cHogH * VaD * aF[v];
         aVaD += -m.cHogH * dW[v] * aF[v];
     }
-    for (int x = 0; x < 3; x++) aM[x] = faB * ldp(amf + (x + 3 * D) * N3, o[4]);
+    for (int x = 0; x < 3; x++) aM[x] = faB * ldp(amf + (x + 3 * D) * N3, oc);
     if (BORIS) {
         // mag-HOGS: mF -= cHogM VaD bbD (WR - WL)_v / Ca^2
         for (int x = 0; x < 3; x++) {
@@ -669,19 +669,21 @@ __global__ __launch_bounds__(256, 2) void k_flux_adj_mhd(Geo g, MParm m, Box fb,
         for (int n = 0; n < 8; n++) qv[n] = LD(H + (H_WH + v) * N3, n);
         for (int n = 0; n < 4; n++) nq[n] = LD(H + (H_WN + v) * N3, n + 2);
         recon_lr_w_adj_s<RM>(m.pdmb, rVi, nq, qv, aL[v], aR[v], anq, sl, sr);
-        stp(sc + (6 * v + 0) * N3, o[4], sl);
-        stp(sc + (6 * v + 1) * N3, o[4], sr);
-        for (int n = 0; n < 4; n++) stp(sc + (6 * v + 2 + n) * N3, o[4], anq[n]);
+        stp(sc + (6 * v + 0) * N3, oc, sl);
+        stp(sc + (6 * v + 1) * N3, oc, sr);
+        for (int n = 0; n < 4; n++) stp(sc + (6 * v + 2 + n) * N3, oc, anq[n]);
     }
     for (int x = 0; x < 3; x++) {
         double qv[8], nq[4], anq[4] = {0, 0, 0, 0}, sl, sr;
-        for (int n = 0; n < 8; n++) qv[n] = vol[n] * LD(H + (H_B + x) * N3, n);
+        // the volumes are loaded again (cached) rather than kept live through
+        // the flux adjoint, which holds the kernel's register peak
+        for (int n = 0; n < 8; n++) qv[n] = LD(met + M_VOL * N3, n) * LD(H + (H_B + x) * N3, n);
         for (int n = 0; n < 4; n++) nq[n] = LD(Bl + x * N3, n + 2);
         recon_lr_w_adj_s<RM>(m.pdmb, rVi, nq, qv, abL[x], abR[x], anq, sl, sr);
         const int v = 5 + x;
-        stp(sc + (6 * v + 0) * N3, o[4], sl);
-        stp(sc + (6 * v + 1) * N3, o[4], sr);
-        for (int n = 0; n < 4; n++) stp(sc + (6 * v + 2 + n) * N3, o[4], anq[n]);
+        stp(sc + (6 * v + 0) * N3, oc, sl);
+        stp(sc + (6 * v + 1) * N3, oc, sr);
+        for (int n = 0; n < 4; n++) stp(sc + (6 * v + 2 + n) * N3, oc, anq[n]);
     }
 }
 
