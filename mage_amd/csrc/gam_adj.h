@@ -508,8 +508,11 @@ __device__ __forceinline__ void predict_cell_adj(const MParm &m, const double *Q
 //   D4[4]: half-state densities of the four cells around the edge;
 //   b01, b02: background edge field (0 without); edge length; cap = the
 //   vDiff bound CFL * edge / dt.
+// AV[d] = wsum8 over n of A1[n] V1[d][n] (V1 the face velocities of the
+// Mom2Face average): the caller forms the sums as it loads V1, which never
+// has to be held whole
 struct EdgeIn {
-    double A1[8], V1[3][8], mH1[8], mL1[4];
+    double A1[8], AV[3], mH1[8], mL1[4];
     double A2[8], mH2[8], mL2[4];
     double te1[3], te2[3], tb[4], D4[4];
     double b01, b02, edge, cap;
@@ -525,7 +528,7 @@ __device__ __forceinline__ double edge_emf(const MParm &m, const EdgeIn &in, dou
 {
     const double rdA = 1.0 / dmax(wsum8([&](int n) { return in.A1[n]; }), GAM_TINY);
     double vc[3];
-    for (int d = 0; d < 3; d++) vc[d] = wsum8([&](int n) { return in.A1[n] * in.V1[d][n]; }) * rdA;
+    for (int d = 0; d < 3; d++) vc[d] = in.AV[d] * rdA;
     const double v1 = vc[0] * in.te1[0] + vc[1] * in.te1[1] + vc[2] * in.te1[2];
     const double v2 = vc[0] * in.te2[0] + vc[1] * in.te2[1] + vc[2] * in.te2[2];
     double vDiff = sqrt(v1 * v1 + v2 * v2);
@@ -569,7 +572,7 @@ __device__ __forceinline__ void edge_emf_adj_s(const MParm &m, const EdgeIn &in,
     // forward pieces again
     const double rdA = 1.0 / dmax(wsum8([&](int n) { return in.A1[n]; }), GAM_TINY);
     double vc[3];
-    for (int d = 0; d < 3; d++) vc[d] = wsum8([&](int n) { return in.A1[n] * in.V1[d][n]; }) * rdA;
+    for (int d = 0; d < 3; d++) vc[d] = in.AV[d] * rdA;
     const double v1 = vc[0] * in.te1[0] + vc[1] * in.te1[1] + vc[2] * in.te1[2];
     const double v2 = vc[0] * in.te2[0] + vc[1] * in.te2[1] + vc[2] * in.te2[2];
     const double vD = sqrt(v1 * v1 + v2 * v2);

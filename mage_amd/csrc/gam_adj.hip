@@ -193,11 +193,14 @@ __global__ __launch_bounds__(256) void k_flux_adj(Geo g, MParm m, Box fb,
     double vol[8];
     for (int n = 0; n < 8; n++) vol[n] = met[M_VOL * N3 + c + (n - 4) * s];
     const double rVi = 1.0 / wsum8([&](int n) { return vol[n]; });
-    double qv[5][8], nq[5][4], WL[5], WR[5];
+    // the stencil values are loaded again (cached) for the transpose below
+    // rather than kept live through the flux adjoint
+    double WL[5], WR[5];
     for (int v = 0; v < 5; v++) {
-        for (int n = 0; n < 8; n++) qv[v][n] = H[(H_WH + v) * N3 + c + (n - 4) * s];
-        for (int n = 0; n < 4; n++) nq[v][n] = H[(H_WN + v) * N3 + c + (n - 2) * s];
-        recon_lr_w<RM>(m.pdmb, rVi, nq[v], qv[v], WL[v], WR[v]);
+        double qv[8], nq[4];
+        for (int n = 0; n < 8; n++) qv[n] = H[(H_WH + v) * N3 + c + (n - 4) * s];
+        for (int n = 0; n < 4; n++) nq[n] = H[(H_WN + v) * N3 + c + (n - 2) * s];
+        recon_lr_w<RM>(m.pdmb, rVi, nq, qv, WL[v], WR[v]);
     }
     double T[9];
     for (int q = 0; q < 9; q++) T[q] = met[(M_TF + q + 9 * D) * N3 + c];
@@ -207,8 +210,10 @@ __global__ __launch_bounds__(256) void k_flux_adj(Geo g, MParm m, Box fb,
     gas_kin_adj(m.igm1, -1.0, WL, T, aF, aL);
     gas_kin_adj(m.igm1, +1.0, WR, T, aF, aR);
     for (int v = 0; v < 5; v++) {
-        double aqv[8] = {0, 0, 0, 0, 0, 0, 0, 0}, anq[4] = {0, 0, 0, 0};
-        recon_lr_w_adj<RM>(m.pdmb, rVi, nq[v], qv[v], aL[v], aR[v], anq, aqv);
+        double qv[8], nq[4], aqv[8] = {0, 0, 0, 0, 0, 0, 0, 0}, anq[4] = {0, 0, 0, 0};
+        for (int n = 0; n < 8; n++) qv[n] = H[(H_WH + v) * N3 + c + (n - 4) * s];
+        for (int n = 0; n < 4; n++) nq[n] = H[(H_WN + v) * N3 + c + (n - 2) * s];
+        recon_lr_w_adj<RM>(m.pdmb, rVi, nq, qv, aL[v], aR[v], anq, aqv);
         for (int n = 0; n < 8; n++) unsafeAtomicAdd(aWH + v * N3 + c + (n - 4) * s, aqv[n]);
         for (int n = 0; n < 4; n++) unsafeAtomicAdd(aWN + v * N3 + c + (n - 2) * s, anq[n]);
     }
@@ -790,10 +795,11 @@ __global__ __launch_bounds__(256, 2) void k_emf_adj(Geo g, MParm m, Box eb, cons
         const long q1 = c + (n - 4) * s2, q2 = c + (n - 4) * s1;
         in.A1[n] = met[(M_FACE + T1) * N3 + q1];
         in.mH1[n] = mfH[T1 * N3 + q1];
-        for (int d = 0; d < 3; d++) in.V1[d][n] = Vf[(3 * T1 + d) * N3 + q1];
         in.A2[n] = met[(M_FACE + T2) * N3 + q2];
         in.mH2[n] = mfH[T2 * N3 + q2];
     }
+    for (int d = 0; d < 3; d++)
+        in.AV[d] = wsum8([&](int n) { return in.A1[n] * Vf[(3 * T1 + d) * N3 + c + (n - 4) * s2]; });
     for (int n = 0; n < 4; n++) {
         in.mL1[n] = mfL[T1 * N3 + c + (n - 2) * s2];
         in.mL2[n] = mfL[T2 * N3 + c + (n - 2) * s1];

@@ -150,7 +150,7 @@ static void edge_fill(EdgeIn &in, const double *geo, const double *x)
     for (int d = 0; d < 3; d++) { in.te1[d] = geo[16 + d]; in.te2[d] = geo[19 + d]; }
     for (int q = 0; q < 4; q++) in.tb[q] = geo[22 + q];
     in.b01 = geo[26]; in.b02 = geo[27]; in.edge = geo[28]; in.cap = geo[29];
-    for (int d = 0; d < 3; d++) for (int n = 0; n < 8; n++) in.V1[d][n] = x[8 * d + n];
+    for (int d = 0; d < 3; d++) in.AV[d] = wsum8([&](int n) { return in.A1[n] * x[8 * d + n]; });
     for (int n = 0; n < 8; n++) { in.mH1[n] = x[24 + n]; in.mH2[n] = x[36 + n]; }
     for (int n = 0; n < 4; n++) { in.mL1[n] = x[32 + n]; in.mL2[n] = x[44 + n]; in.D4[n] = x[48 + n]; }
 }
