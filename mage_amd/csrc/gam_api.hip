@@ -1299,7 +1299,9 @@ struct gam_adj {
     double *avf, *aur, *adm, *ae;   // Vf (9), Reynolds gas (5), dM (3), E (3)
     double *rv;              // Mom2Face normalisation per face direction (3)
     double *fsc;             // flux-transpose scratch, 48 planes per face direction (MHD)
-    double *esc;             // EMF-transpose scratch, 16 planes per edge direction (MHD)
+    double *esc;             // EMF-transpose scratch, 16 planes per edge direction (MHD);
+                             // the first planes of fsc: the EMF gather has read it before
+                             // the flux transpose writes fsc
     double *wgt;             // objective weights (1 plane)
     double *part;            // objective partial sums (1 plane)
     int nw;                  // inflow series length (0: no inflow BC)
@@ -1332,7 +1334,7 @@ extern "C" int gam_adj_destroy(gam_adj *a)
     if (a->c && a->c->stream) hipStreamSynchronize(a->c->stream);
     double *bufs[] = {a->ck, a->fin, a->tape, a->itape, a->tape2, a->itape2, a->lam, a->ag, a->amf, a->awh, a->awn, a->ahg, a->ahmf,
                       a->ahb, a->avf, a->aur, a->adm, a->ae, a->wgt, a->part, a->gwq, a->gwb, a->rv,
-                      a->fsc, a->esc};
+                      a->fsc};
     for (double *b : bufs) if (b) hipFree(b);
     if (a->clkh) hipFree(a->clkh);
     if (a->oclk) hipFree(a->oclk);
@@ -1385,7 +1387,7 @@ extern "C" int gam_adj_create(gam_adj **out, gam_ctx *c, int nsteps, int K)
     AALLOC(a->part, pl);
     AALLOC(a->rv, 3 * pl);
     if (c->mp.doMHD) AALLOC(a->fsc, 3 * 48 * pl);
-    if (c->mp.doMHD) AALLOC(a->esc, 3 * 16 * pl);
+    a->esc = a->fsc;
     for (int sd = 0; sd < 6; sd++)
         if (c->model.bc[sd] == GAM_BC_WIND) a->nw = c->wd.n;
     if (a->nw) {
