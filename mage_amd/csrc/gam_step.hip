@@ -1137,8 +1137,9 @@ struct k_bflux2fld_ghosts {
 };
 
 // ====================================================================
-// Boundary conditions.  One thread per transverse ghost column; the n loop
-// runs inside as in the reference.  Kernels of the six faces run in the
+// Boundary conditions.  One thread per transverse ghost column with the n
+// loop inside as in the reference (j and k sides), or per column and
+// ghost layer (i sides, below).  Kernels of the six faces run in the
 // reference order INI, OUTI, INJ, OUTJ, INK, OUTK.
 // ====================================================================
 __device__ __forceinline__ void copy_cell(double *S, long N3, long dst, long src)
