@@ -674,8 +674,7 @@ __global__ __launch_bounds__(256, 2) void k_flux_adj_mhd(Geo g, MParm m, Box fb,
         for (int n = 0; n < 8; n++) qv[n] = LD(H + (H_WH + v) * N3, n);
         for (int n = 0; n < 4; n++) nq[n] = LD(H + (H_WN + v) * N3, n + 2);
         recon_lr_w_adj_s<RM>(m.pdmb, rVi, nq, qv, aL[v], aR[v], anq, sl, sr);
-        stp(sc + (6 * v + 0) * N3, oc, sl);
-        stp(sc + (6 * v + 1) * N3, oc, sr);
+        *(double2 *)(sc + 6 * v * N3 + 2 * c) = make_double2(sl, sr);
         for (int n = 0; n < 4; n++) stp(sc + (6 * v + 2 + n) * N3, oc, anq[n]);
     }
     for (int x = 0; x < 3; x++) {
@@ -686,8 +685,7 @@ __global__ __launch_bounds__(256, 2) void k_flux_adj_mhd(Geo g, MParm m, Box fb,
         for (int n = 0; n < 4; n++) nq[n] = LD(Bl + x * N3, n + 2);
         recon_lr_w_adj_s<RM>(m.pdmb, rVi, nq, qv, abL[x], abR[x], anq, sl, sr);
         const int v = 5 + x;
-        stp(sc + (6 * v + 0) * N3, oc, sl);
-        stp(sc + (6 * v + 1) * N3, oc, sr);
+        *(double2 *)(sc + 6 * v * N3 + 2 * c) = make_double2(sl, sr);
         for (int n = 0; n < 4; n++) stp(sc + (6 * v + 2 + n) * N3, oc, anq[n]);
     }
 }
@@ -737,11 +735,12 @@ __global__ __launch_bounds__(256) void k_recon_adj_gather(Geo g, Box cb, const d
             if (fi < lo || fi > hi) continue;
             const long f = c - (n - 4) * s;
             for (int v = 0; v < 8; v++) {
+                const double2 lr = *(const double2 *)(sd + 6 * v * N3 + 2 * f);
                 if (RM == 1) {
-                    if (n <= 6) gh[v] += UP7(n) * sd[(6 * v + 0) * N3 + f];
-                    if (n >= 1) gh[v] += UP7(7 - n) * sd[(6 * v + 1) * N3 + f];
+                    if (n <= 6) gh[v] += UP7(n) * lr.x;
+                    if (n >= 1) gh[v] += UP7(7 - n) * lr.y;
                 } else {
-                    gh[v] += W8(n) * sd[(6 * v + 0) * N3 + f];
+                    gh[v] += W8(n) * lr.x;
                 }
             }
         }

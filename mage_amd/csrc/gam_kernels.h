@@ -128,8 +128,9 @@ void launch_update_adj_mhd(hipStream_t s, const Geo &g, const MParm &m, const Bo
                            double *aSn, double *aSo, double *aUr, double *adM, double *ag, double *amf,
                            double *aE);
 // pass 1 of the flux transpose: per face, the stencil scalars and limiter
-// adjoints of its 8 reconstructions into sc (48 planes), aBn into aHMF, the
-// Boris nuke term into aHG
+// adjoints of its 8 reconstructions into sc (48 planes: per variable the
+// pair (sl, sr) interleaved over two planes, then the four limiter
+// adjoints), aBn into aHMF, the Boris nuke term into aHG
 void launch_flux_adj_mhd(hipStream_t s, int d, const Geo &g, const MParm &m, const Box &fb, const double *H,
                          const double *S, const double *met, const double *bg, const double *ag,
                          const double *amf, double *sc, double *aHMF, double *aHG);
