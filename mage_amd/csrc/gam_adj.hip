@@ -764,8 +764,9 @@ __global__ __launch_bounds__(256) void k_recon_adj_gather(Geo g, Box cb, const d
 }
 
 // Edge EMF transpose (k_emf), pass 1: per edge, the pieces of
-// edge_emf_adj_s into sc (16 planes for this edge direction: sv 3, sl1,
-// sr1, anq1 4, sl2, sr2, anq2 4, aDc), gathered into the face velocities,
+// edge_emf_adj_s into sc (16 planes for this edge direction: sv 3, the
+// pair (sl1, sr1) interleaved over two planes, anq1 4, the pair (sl2, sr2),
+// anq2 4, aDc), gathered into the face velocities,
 // the half-state and limiter face fluxes and the half-state density by
 // k_emf_adj_gather.
 template <int ED, int RM, bool BG, bool BORIS>
@@ -817,7 +818,11 @@ __global__ __launch_bounds__(256, 2) void k_emf_adj(Geo g, MParm m, Box eb, cons
     edge_emf_adj_s<RM, BORIS>(m, in, aE[ED * N3 + c], o);
     const double v[16] = {o.sv[0], o.sv[1], o.sv[2], o.sl1, o.sr1, o.anq1[0], o.anq1[1], o.anq1[2], o.anq1[3],
                           o.sl2, o.sr2, o.anq2[0], o.anq2[1], o.anq2[2], o.anq2[3], o.aDc};
-    for (int q = 0; q < 16; q++) sc[q * N3 + c] = v[q];
+    for (int q = 0; q < 16; q++)
+        if (q != 3 && q != 4 && q != 9 && q != 10) sc[q * N3 + c] = v[q];
+    // the stencil scalar pairs interleaved over planes 3-4 and 9-10
+    *(double2 *)(sc + 3 * N3 + 2 * c) = make_double2(o.sl1, o.sr1);
+    *(double2 *)(sc + 9 * N3 + 2 * c) = make_double2(o.sl2, o.sr2);
 }
 
 // Edge EMF transpose, pass 2: one thread per face position x (every
@@ -853,13 +858,15 @@ __global__ __launch_bounds__(256) void k_emf_adj_gather(Geo g, Box fb, EdgeBoxes
             if (inside(eb.e[EDa], e3[0], e3[1], e3[2])) {
                 const long e = c - (n - 4) * st[Da];
                 for (int d = 0; d < 3; d++) sv[d] += W8(n) * sa[d * N3 + e];
-                sh += stencil_adj<RM>(n, sa[3 * N3 + e], sa[4 * N3 + e]);
+                const double2 lr = *(const double2 *)(sa + 3 * N3 + 2 * e);
+                sh += stencil_adj<RM>(n, lr.x, lr.y);
             }
             int e4[3] = {i, j, k};
             e4[Db] -= n - 4;
             if (inside(eb.e[EDb], e4[0], e4[1], e4[2])) {
                 const long e = c - (n - 4) * st[Db];
-                sh += stencil_adj<RM>(n, sb[9 * N3 + e], sb[10 * N3 + e]);
+                const double2 lr = *(const double2 *)(sb + 9 * N3 + 2 * e);
+                sh += stencil_adj<RM>(n, lr.x, lr.y);
             }
         }
 #pragma unroll
