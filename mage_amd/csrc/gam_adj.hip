@@ -654,7 +654,7 @@ __global__ __launch_bounds__(256, 2) void k_flux_adj_mhd(Geo g, MParm m, Box fb,
         // nuke: gF -= Ca (U(c) - U(c - s)) where the mean speed is large
         const double vL = sqrt(WL[1] * WL[1] + WL[2] * WL[2] + WL[3] * WL[3]);
         const double vR = sqrt(WR[1] * WR[1] + WR[2] * WR[2] + WR[3] * WR[3]);
-        const double vn = sqrt(vL * vL + vR * vR) * 0.70710678118654752440084436210484903928483593768847;
+        const double vn = sqrt(vL * vL + vR * vR) / GAM_SQRT2_SP;
         if (vn >= 1.5 * m.Ca)
             for (int v = 0; v < 5; v++) {
                 AADD(aHG + v * N3 + c, -m.Ca * aF[v]);

@@ -13,6 +13,10 @@
 
 #define GAM_TINY 1.0e-12
 #define GAM_HUGE 1.0e+12
+// sqrt(2.0) of the Boris "nuke" threshold (stress.F90:405): the intrinsic of a
+// default-real argument returns default real, so the reference divides by the
+// single-precision sqrt(2)
+#define GAM_SQRT2_SP 1.41421353816986083984375
 
 struct Geo {
     int P;          // i pitch

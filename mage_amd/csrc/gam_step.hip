@@ -581,7 +581,7 @@ __device__ __forceinline__ void face_flux_lr(const Geo &g, const MParm &m,
     if (BORIS) {   // "nuclear option" (stress.F90:399-417)
         double vL = sqrt(WL[1] * WL[1] + WL[2] * WL[2] + WL[3] * WL[3]);
         double vR = sqrt(WR[1] * WR[1] + WR[2] * WR[2] + WR[3] * WR[3]);
-        double vn = sqrt(vL * vL + vR * vR) * 0.70710678118654752440084436210484903928483593768847;
+        double vn = sqrt(vL * vL + vR * vR) / GAM_SQRT2_SP;
         if (vn >= 1.5 * m.Ca) {
 #pragma unroll
             for (int v = 0; v < 5; v++) {

@@ -11,9 +11,15 @@
  * the reference tree (rgorby/MAGE).
  *
  * Literal conventions (documented choices):
- *  - Fortran default-real literals are taken at double precision, i.e. the
- *    Intel "-fpconstant" reading (cmake/compilers.cmake:84).  Only
- *    sqrt(2.0) in the Boris "nuke" threshold (stress.F90:405) is affected.
+ *  - Fortran default-real literals that meet double-precision operands are
+ *    taken at double precision (the Intel "-fpconstant" reading,
+ *    cmake/compilers.cmake:84); every such literal on the path is exact in
+ *    single precision anyway.
+ *  - sqrt(2.0) in the Boris "nuke" threshold (stress.F90:405) is an
+ *    intrinsic of a default-real argument, so its result is default real
+ *    (single precision) under both the GNU and the Intel flags of the
+ *    reference (-fpconstant does not change an intrinsic's result kind):
+ *    the divisor is the single-precision sqrt(2), 1.41421353816986083984375.
  *  - norm2() is evaluated as sqrt(x*x+y*y+z*z) (no scaling).
  *  - x**2.0 is evaluated as x*x (the correctly rounded square).
  */
@@ -26,6 +32,7 @@
 #define HUGEV 1.0e+12          /* kdefs.F90:20 */
 #define PI (4.0 * atan(1.0))   /* kdefs.F90:18 */
 #define CLIM 1.5               /* stress.F90:25 */
+#define SQRT2_SP 1.41421353816986083984375   /* sqrt(2.0), default real (stress.F90:405) */
 #define MINMACH 1.0            /* wind.F90:18 */
 
 enum { DEN = 0, MOMX = 1, MOMY = 2, MOMZ = 3, ENERGY = 4 };
@@ -1015,7 +1022,7 @@ static void face_flux(const orc_dims *D, const orc_model *M, const orc_grid *G,
     if (M->doBoris) {  /* doNuke (stress.F90:399-417) */
         double vL = sqrt(PrimL[1] * PrimL[1] + PrimL[2] * PrimL[2] + PrimL[3] * PrimL[3]);
         double vR = sqrt(PrimR[1] * PrimR[1] + PrimR[2] * PrimR[2] + PrimR[3] * PrimR[3]);
-        double Vn = sqrt(vL * vL + vR * vR) / sqrt(2.0);
+        double Vn = sqrt(vL * vL + vR * vR) / SQRT2_SP;
         if (Vn >= CLIM * M->Ca) {
             for (int v = 0; v < 5; v++) {
                 double dCon = ConB[v][4] - ConB[v][3];
@@ -1578,4 +1585,38 @@ void orc_add_dipole_b0(const orc_dims *D, orc_grid *G, double M0)
                                           + fs[cK + nc * (2 + 3 * x)] - fs[c + nc * (2 + 3 * x)]) / G->volume[c];
             }
     free(fs);
+}
+
+/* ------------------------------------------------------ pins
+ * Thin wrappers that expose static pieces of the step to the oracle's own
+ * pin tests (tests/test_oracle_pins.py).  They add no arithmetic. */
+void orc_cell_boris(const orc_model *M, double *U2, const double *U0, const double *B2,
+                    const double *B0, const double *dGasH, const double *dGasM, double dt)
+{
+    cell_boris(M, U2, U0, B2, B0, dGasH, dGasM, M->Ca, dt);
+}
+
+void orc_wind_interp(const orc_model *M, double t, double *out8)
+{   /* out8 = rho, P, V(3), B(3) */
+    wind_interp(M, t, &out8[0], &out8[1], &out8[2], &out8[5]);
+}
+
+void orc_wind_front(const orc_model *M, double t, double *vFr)
+{
+    wind_front(M, t, vFr);
+}
+
+void orc_wind_at(const orc_model *M, const double *vFr, const double *xyz, double t, double *out8)
+{
+    wind_at(M, vFr, xyz, t, &out8[0], &out8[1], &out8[2], &out8[5]);
+}
+
+double orc_wind_wgt(const orc_model *M, const double *vFr, const double *xyz, double t)
+{
+    return wind_wgt(M, vFr, xyz, t);
+}
+
+double orc_ramp_down(double r, double rC, double lC)
+{
+    return ramp_down(r, rC, lC);
 }

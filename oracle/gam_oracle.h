@@ -131,6 +131,14 @@ int    orc_init_run(const orc_dims *D, orc_model *M, const orc_grid *G,
                     orc_state *S, orc_state *oS);
 double orc_divb(const orc_dims *D, const orc_model *M, const orc_grid *G,
                 const orc_state *S, double *divout);
+/* Static pieces of the step exposed for the oracle's pin tests */
+void   orc_cell_boris(const orc_model *M, double *U2, const double *U0, const double *B2,
+                      const double *B0, const double *dGasH, const double *dGasM, double dt);
+void   orc_wind_interp(const orc_model *M, double t, double *out8);
+void   orc_wind_front(const orc_model *M, double t, double *vFr);
+void   orc_wind_at(const orc_model *M, const double *vFr, const double *xyz, double t, double *out8);
+double orc_wind_wgt(const orc_model *M, const double *vFr, const double *xyz, double t);
+double orc_ramp_down(double r, double rC, double lC);
 /* Access to solver intermediates for staged comparisons */
 double *orc_solver_array(orc_solver *W, int which);
 enum { ORC_W_GFLX = 0, ORC_W_MFLX = 1, ORC_W_VF = 2, ORC_W_DGASH = 3,

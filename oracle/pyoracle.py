@@ -95,6 +95,14 @@ def lib():
         L.orc_divb.argtypes = [C.POINTER(Dims), C.POINTER(Model), C.POINTER(Grid),
                                C.POINTER(State), DP]
         L.orc_divb.restype = C.c_double
+        L.orc_cell_boris.argtypes = [C.POINTER(Model), DP, DP, DP, DP, DP, DP, C.c_double]
+        L.orc_wind_interp.argtypes = [C.POINTER(Model), C.c_double, DP]
+        L.orc_wind_front.argtypes = [C.POINTER(Model), C.c_double, DP]
+        L.orc_wind_at.argtypes = [C.POINTER(Model), DP, DP, C.c_double, DP]
+        L.orc_wind_wgt.argtypes = [C.POINTER(Model), DP, DP, C.c_double]
+        L.orc_wind_wgt.restype = C.c_double
+        L.orc_ramp_down.argtypes = [C.c_double] * 3
+        L.orc_ramp_down.restype = C.c_double
         _lib = L
     return _lib
 
