@@ -1,19 +1,25 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X GAMERA MHD step (FP64 cell-updates/s).
 
-Workload (one GPU): the 192x192x256 warped log-r spherical shell with the
-dipole background, Boris correction (Ca=10), 7th-order upwind + PDM
-reconstruction and the inflow/outflow outer boundary, stepped with the full
-stepGamera sequence (AdvanceMHD, CalcDT, EnforceBCs, bFlux2Fld).  Inputs are
-synthetic and resident in HBM before the timed region.
+Headline workload (one GPU): the 256x256x384 warped log-r spherical shell
+(25.2M cells) with the dipole background, Boris correction (Ca=10),
+7th-order upwind + PDM reconstruction and the inflow/outflow outer
+boundary, stepped with the full stepGamera sequence (AdvanceMHD, CalcDT,
+EnforceBCs, bFlux2Fld).  Inputs are synthetic and resident in HBM before
+the timed region.
 
 Prints ONE JSON line.  value = Nip*Njp*Nkp*steps / seconds (the kZCs
 definition of output.F90:69-70, times 1e3), timed with hipEvents on the
 library's stream around exactly --steps steps after --warmup steps.
+Beside it: the adjoint over 100 more steps of the same run, the
+192x192x256 shell, the 64^3 box, the 32-member ensemble, the 96x96x128
+adjoint, and the CPU baseline (the oracle's speed build on the host cores,
+on a few steps of the headline workload).
 """
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -28,12 +34,14 @@ STEP_BYTES_PER_CELL = 8 * (19 + 14 + 79 + 24)
 # Algorithmic words of one interface-flux launch, per face: stencil inputs
 # counted once per cell (half-state 5, limiter state 5, half B 3, limiter B
 # 3, volume 1), face data (Tf 9, area 1, half-state flux 1, fcB0 3,
-# bFlux0 1) and outputs (gas 5, magnetic 3): 40 words = 320 B.
-FLUX_BYTES_PER_FACE = 8 * 40
-# rocprofv3 FETCH_SIZE/WRITE_SIZE per dispatch of the flux kernel on this
-# workload, with the copy-kernel calibration (written by the profiling run
-# recorded in that file).
-PMC_SUMMARY = os.path.join(ROOT, "profiles", "shell_192x192x256_step_kernels.md")
+# bFlux0 1) and outputs (gas 5, magnetic 3, face velocity 3): 43 words =
+# 344 B.
+FLUX_BYTES_PER_FACE = 8 * 43
+# rocprofv3 summaries of the headline workload: FETCH_SIZE/WRITE_SIZE per
+# dispatch with the copy-kernel calibration, and the FP64 VALU instruction
+# counts per dispatch (written by the profiling runs recorded in the files).
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "shell_256x256x384_step_kernels.md")
+FLOP_SUMMARY = os.path.join(ROOT, "profiles", "shell_256x256x384_fp64_flops.md")
 
 
 def log(*a):
@@ -61,23 +69,40 @@ def setup_shell(gamhip, pb, N, rmeth="7UP", full_grid=False):
     return g, p, grid, st, wind
 
 
-def flux_traffic(path=PMC_SUMMARY):
-    """Calibrated HBM bytes per interface-flux dispatch from the committed
-    PMC summary (mean over the three directions of the launch shapes the
-    step ran; shapes launched only while autotuning have few calls), or
-    None without one."""
+def _summary_rows(path, prefix, col):
+    """Column col of the table rows of a committed profile summary whose
+    kernel name starts with prefix (launch shapes that only the autotuner
+    ran have few calls and are skipped)."""
     if not os.path.exists(path):
-        return None
+        return []
     vals = []
     for line in open(path):
         cols = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cols) >= 7 and cols[0].startswith("`k_flux"):
+        if len(cols) > col and cols[0].startswith(prefix):
             try:
-                if int(cols[1]) >= 50:
-                    vals.append(float(cols[5]))
+                if int(cols[1]) >= 10:
+                    vals.append(float(cols[col]))
             except ValueError:
                 pass
+    return vals
+
+
+def flux_traffic(path=PMC_SUMMARY):
+    """Calibrated HBM bytes per interface-flux dispatch from the committed
+    PMC summary (mean over the three directions), or None without one."""
+    vals = _summary_rows(path, "`k_flux", 5)
     return sum(vals) / len(vals) if vals else None
+
+
+def step_flops(path=FLOP_SUMMARY):
+    """FP64 FLOP of one step from the committed FP64-instruction summary
+    (its "FLOP per step" line), or None without one."""
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("FP64 FLOP per step:"):
+            return float(line.split(":")[1].split()[0])
+    return None
 
 
 def setup_member(gamhip, pb, spec, rmeth="7UP"):
@@ -125,45 +150,46 @@ def ensemble_gain(gamhip, pb, n=32, steps=100, warmup=2, tune=True):
             "distinct_plans": len({tuple(sorted(p.items())) for p in plans}) if plans else 1}
 
 
-def full_size(gamhip, pb, N=(256, 256, 384), steps=500, warmup=5, tune=True, adj_steps=100, adj_ckpt=10):
-    """Forward steps of the full-size shell (same physics as the 192x192x256
-    workload), HIP-event timed, against the step's algorithmic bytes; then
-    the adjoint over adj_steps more steps with checkpoints every adj_ckpt
-    (sqrt(adj_steps) checkpoints), objective the pressure integral over
-    3 <= r <= 6."""
-    import numpy as np
+def shell_extra(gamhip, pb, N=(192, 192, 256), steps=200, warmup=5, tune=True):
+    """The 192x192x256 shell (same physics as the headline workload),
+    HIP-event timed, against the step's algorithmic bytes."""
     g, p, grid, st, wind = setup_shell(gamhip, pb, N)
     del st, grid
     plan = g.autotune(reps=2) if tune else g.get_plan()
     g.run(warmup)
     ms = g.time_steps(steps)
+    g.close()
     cells = N[0] * N[1] * N[2]
     value = cells * steps / (ms * 1e-3)
-    adj = None
-    if adj_steps:
-        gr = g.get_grid(fields=("xyzcc", "volume"))
-        r = np.sqrt(np.sum(gr["xyzcc"] ** 2, axis=-1))
-        w = np.asfortranarray(gr["volume"] * ((r >= 3.0) & (r <= 6.0)))
-        del gr, r
-        try:
-            a = gamhip.GamAdjoint(g, adj_steps, adj_ckpt)
-            a.set_weights(w)
-            J = a.forward()
-            a.backward()
-            f_ms, b_ms = a.times()
-            a.close()
-            fwd = ms / steps * adj_steps
-            adj = {"steps": adj_steps, "ckpt_every": adj_ckpt, "J": J, "forward_plain_ms": fwd,
-                   "forward_recorded_ms": f_ms, "reverse_ms": b_ms, "adjoint_over_forward": b_ms / fwd}
-        except gamhip.GamError as e:
-            adj = {"steps": adj_steps, "ckpt_every": adj_ckpt, "error": str(e)}
-    g.close()
-    return {"workload": "%dx%dx%d warped shell (25.2M cells), dipole B0, inflow, 7UP, Boris, %d forward steps"
-                        % (N + (steps,)),
+    return {"workload": "%dx%dx%d warped shell, dipole B0, inflow, 7UP, Boris, %d steps" % (N + (steps,)),
             "cells": cells, "steps": steps, "ms_per_step": ms / steps, "value": value,
             "unit": "cell-updates/s", "launch_plan": plan,
-            "step_frac_of_hbm_roofline": STEP_BYTES_PER_CELL * value / 1e9 / HBM_PEAK_GBS,
-            "adjoint": adj}
+            "step_frac_of_hbm_roofline": STEP_BYTES_PER_CELL * value / 1e9 / HBM_PEAK_GBS}
+
+
+def run_adjoint(gamhip, g, adj_steps=100, adj_ckpt=10, ms_per_step=None):
+    """The adjoint over adj_steps more steps of a prepared shell context,
+    checkpoints every adj_ckpt (sqrt(adj_steps) checkpoints), objective the
+    pressure integral over 3 <= r <= 6."""
+    import numpy as np
+    gr = g.get_grid(fields=("xyzcc", "volume"))
+    r = np.sqrt(np.sum(gr["xyzcc"] ** 2, axis=-1))
+    w = np.asfortranarray(gr["volume"] * ((r >= 3.0) & (r <= 6.0)))
+    del gr, r
+    try:
+        a = gamhip.GamAdjoint(g, adj_steps, adj_ckpt)
+        taped, overlap = a.tape_info()
+        a.set_weights(w)
+        J = a.forward()
+        a.backward()
+        f_ms, b_ms = a.times()
+        a.close()
+        fwd = ms_per_step * adj_steps
+        return {"steps": adj_steps, "ckpt_every": adj_ckpt, "J": J, "forward_plain_ms": fwd,
+                "forward_recorded_ms": f_ms, "reverse_ms": b_ms, "adjoint_over_forward": b_ms / fwd,
+                "taped_steps_per_interval": taped, "overlapped_recompute": overlap}
+    except gamhip.GamError as e:
+        return {"steps": adj_steps, "ckpt_every": adj_ckpt, "error": str(e)}
 
 
 def adjoint_cost(gamhip, pb, N=(96, 96, 128), steps=200, ckpt=20, nwind=200):
@@ -322,16 +348,38 @@ def adjoint_fd_check(gamhip, pb, ndir=64, N=(96, 96, 128), steps=200, ckpt=20, n
     return out
 
 
-def cpu_baseline(gamhip, pb, min_seconds=12.0, max_steps=400):
-    """Time the oracle (C/OpenMP restatement of the reference step) on a
-    96x96x128 sample of the same shell workload."""
+def _cpu_model():
+    try:
+        out = subprocess.run(["lscpu"], stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True).stdout
+    except OSError:
+        return None
+    info = {}
+    for line in out.splitlines():
+        if ":" in line:
+            k, v = line.split(":", 1)
+            info[k.strip()] = v.strip()
+    return {k: info.get(k) for k in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core")}
+
+
+def cpu_baseline(gamhip, pb, g, N, min_seconds=12.0, max_steps=20):
+    """Time the oracle's speed build (C/OpenMP restatement of the reference
+    step, -O3 AVX2/FMA, no fast-math) on a few steps of the headline
+    workload: the grid and background of context g, the workload's
+    initial state, OMP_PROC_BIND=close and OMP_PLACES=cores on the host
+    cores this process may use."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    os.environ["OMP_NUM_THREADS"] = str(threads)
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
-    N = (96, 96, 128)
-    g, p, grid, st, wind = setup_shell(gamhip, pb, N, full_grid=True)
-    g.close()
-    o = po.Oracle(*N, p, grid, wind=wind)
+    grid = g.get_grid()
+    grid.update(g.get_background())
+    p = pb.shell_params("7UP")
+    st = pb.shell_state(*N, grid["xyzcc"])
+    o = po.Oracle(*N, p, grid, wind=pb.constant_wind(), fast=True)
     o.set_state(st)
+    del st
     o.init_run()
     o.step()                      # warm-up (first touch of the work arrays)
     n = 0
@@ -343,29 +391,30 @@ def cpu_baseline(gamhip, pb, min_seconds=12.0, max_steps=400):
             break
     sec = time.perf_counter() - t0
     cells = N[0] * N[1] * N[2]
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
     return {"value": cells * n / sec, "unit": "cell-updates/s", "cores": threads, "kind": "port",
-            "sample": "oracle (C/OpenMP restatement of stepGamera), %d steps of the %dx%dx%d shell "
-                      "with the same physics, %.1f s" % (n, N[0], N[1], N[2], sec)}
+            "cpu": _cpu_model(),
+            "omp": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")},
+            "sample": "oracle speed build (C/OpenMP restatement of stepGamera, -O3 -march=x86-64-v3, no "
+                      "fast-math), %d steps of the %dx%dx%d headline shell after 1 warm-up step, %.1f s"
+                      % (n, N[0], N[1], N[2], sec)}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--grid", type=str, default="192x192x256")
+    ap.add_argument("--grid", type=str, default="256x256x384")
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--repeats", type=int, default=5, help="timed loops of --steps steps (value = the first)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-box", action="store_true")
     ap.add_argument("--no-ens", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
-    ap.add_argument("--no-full", action="store_true")
-    ap.add_argument("--no-adj", action="store_true")
+    ap.add_argument("--no-c2", action="store_true", help="skip the 192x192x256 shell line")
+    ap.add_argument("--no-adj", action="store_true", help="skip both adjoint lines")
     ap.add_argument("--adj-fd", type=int, default=0,
-                    help="central-difference check of the adjoint workload with this many directions")
-    ap.add_argument("--full-steps", type=int, default=500)
+                    help="central-difference check of the 96x96x128 adjoint with this many directions")
     ap.add_argument("--ens-members", type=int, default=32)
     ap.add_argument("--ens-steps", type=int, default=100)
     args = ap.parse_args()
@@ -380,7 +429,8 @@ def main():
     cells = N[0] * N[1] * N[2]
     t0 = time.time()
     g, p, grid, st, wind = setup_shell(gamhip, pb, N)
-    log("setup %.1f s" % (time.time() - t0))
+    del grid, st
+    log("setup %s %.1f s" % (args.grid, time.time() - t0))
     plan = g.get_plan()
     if not args.no_tune:
         t0 = time.time()
@@ -417,7 +467,33 @@ def main():
     mhd_value = cells / (adv_ms * 1e-3)
     t, dt, ts = g.get_time()
     log("kernel ms/launch: " + ", ".join("%s=%.3f" % (k, v["ms_per_launch"] or 0) for k, v in kernels.items()))
+    flop = step_flops()
+    fp64 = None
+    if flop:
+        fp64 = {"flop_per_step": flop, "achieved": flop / (ms_step * 1e-3) / 1e12, "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": flop / (ms_step * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+                "flop_per_cell_update": flop / cells,
+                "source": os.path.relpath(FLOP_SUMMARY, ROOT)}
+
+    adj5 = None
+    if not args.no_adj:
+        t1 = time.time()
+        adj5 = run_adjoint(gamhip, g, 100, 10, ms_per_step=repeat["median_ms_per_step"])
+        log("adjoint over 100 more steps: %s (%.1f s)" % (adj5, time.time() - t1))
+
+    cpu = None
+    if not args.no_cpu:
+        t1 = time.time()
+        cpu = cpu_baseline(gamhip, pb, g, N)
+        log("cpu baseline %.4e cell-updates/s on %d threads (%.1f s)" % (cpu["value"], cpu["cores"], time.time() - t1))
     g.close()
+
+    c2 = None
+    if not args.no_c2:
+        t1 = time.time()
+        c2 = shell_extra(gamhip, pb, tune=not args.no_tune)
+        log("192x192x256 shell: %.3f ms/step, %.4e cell-updates/s (%.1f s)"
+            % (c2["ms_per_step"], c2["value"], time.time() - t1))
 
     # The plain-step workload: 64^3 periodic box, 100 steps (8CENT, no B0)
     box = None
@@ -438,16 +514,6 @@ def main():
                "step_frac_of_hbm_roofline": 8 * (19 + 14 + 79) * 64 ** 3 * 100 / (msb * 1e-3) / 1e9 / HBM_PEAK_GBS}
         log("box 64^3: %.3f ms/step, %.4e cell-updates/s" % (box["ms_per_step"], box["value"]))
 
-    full = None
-    if not args.no_full:
-        t1 = time.time()
-        full = full_size(gamhip, pb, steps=args.full_steps, tune=not args.no_tune)
-        log("full-size shell: %.3f ms/step, %.4e cell-updates/s (%.1f s)"
-            % (full["ms_per_step"], full["value"], time.time() - t1))
-        if full["adjoint"] and "reverse_ms" in full["adjoint"]:
-            log("full-size adjoint over %d steps: reverse %.1f ms, ratio %.2f"
-                % (full["adjoint"]["steps"], full["adjoint"]["reverse_ms"], full["adjoint"]["adjoint_over_forward"]))
-
     ens = None
     if not args.no_ens:
         t1 = time.time()
@@ -467,12 +533,6 @@ def main():
             adjc["fd_check"] = adjoint_fd_check(gamhip, pb, ndir=args.adj_fd)
             log("adjoint fd check: %s (%.1f s)" % (adjc["fd_check"], time.time() - t1))
 
-    cpu = None
-    if not args.no_cpu:
-        t1 = time.time()
-        cpu = cpu_baseline(gamhip, pb)
-        log("cpu baseline %.4e cell-updates/s on %d threads (%.1f s)" % (cpu["value"], cpu["cores"], time.time() - t1))
-
     out = {
         "metric": "FP64 MHD cell-updates/s vs host CPU; ensemble gain; adjoint/fwd cost",
         "value": value,
@@ -488,7 +548,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": "curvilinear single run: %dx%dx%d warped log-r spherical shell r=2.5..30, "
+            "workload": "full-size forward: %dx%dx%d warped log-r spherical shell r=2.5..30, "
                         "theta 0.03pi..0.97pi, phi periodic; dipole B0 |B0|=1 at r=2.5; rho=1, "
                         "p=1e-3 seeded; inflow rho=1 vx=-0.4 Bz=-0.05; 7UP+PDM, Boris Ca=10, "
                         "full stepGamera" % N,
@@ -499,7 +559,7 @@ def main():
         "launch_plan": plan,
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_flux (one interface direction per launch)",
+            "kernel": "k_flux (interface fluxes and face velocities of one face direction per launch)",
             "achieved": flux_gbs,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -516,9 +576,11 @@ def main():
             "unit": "GB/s",
             "frac": step_gbs / HBM_PEAK_GBS,
         },
+        "fp64": fp64,
         "mhd_cell_updates_per_s": mhd_value,
+        "adjoint_full_size": adj5,
+        "shell_192x192x256": c2,
         "box_64cubed": box,
-        "shell_256x256x384": full,
         "ensemble": ens,
         "adjoint": adjc,
         "kernels": kernels,

@@ -50,6 +50,18 @@ typedef struct gam_adj gam_adj;
 /* An adjoint run of nsteps steps of ctx with checkpoints every ckpt_every
  * (1 <= ckpt_every <= nsteps). */
 int gam_adj_create(gam_adj **out, gam_ctx *ctx, int nsteps, int ckpt_every);
+/* Same, with the tape budget set by the caller: at most max_taped steps of
+ * an interval keep their forward intermediates on the tape (max_taped < 0:
+ * as many as HBM allows, as gam_adj_create; 0: every adjoint step
+ * recomputes its predictor, face velocities and fluxes), and overlap = 0
+ * keeps a single tape pair (the interval recompute then runs on the
+ * context's stream before the interval's adjoint steps).  Gradients do not
+ * depend on either choice; large grids take these paths when HBM is short. */
+int gam_adj_create_budget(gam_adj **out, gam_ctx *ctx, int nsteps, int ckpt_every,
+                          int max_taped, int overlap);
+/* Steps of an interval whose intermediates are taped, and whether the
+ * reverse sweep overlaps the interval recompute (second tape pair). */
+int gam_adj_tape_info(gam_adj *a, int *taped_steps, int *overlap);
 int gam_adj_destroy(gam_adj *a);
 /* Objective weights w(Ni,Nj,Nk) (Fortran layout, ghosts included). */
 int gam_adj_set_weights(gam_adj *a, const double *w);
@@ -82,6 +94,14 @@ int gam_adj_replay(gam_adj *a, const double *Gas, const double *oGas, const doub
  * function of time and is not differentiated.  The series must be the one
  * the forward run used.  GAM_E_STATE without an inflow BC. */
 int gam_adj_wind_grad(gam_adj *a, double *gQ, double *gB);
+/* Time at which the start oState's ghost cells follow its interior through
+ * the boundary conditions, in the adjoint and in gam_adj_replay:
+ * at_state_time = 0 (default) at oState%time; 1 at State%time, which is
+ * what gam_init_run produces (oState = State after the boundary pass,
+ * init.F90:230-244), so a forward that starts with gam_init_run is
+ * differentiated exactly.  The two differ only for the time-dependent
+ * inflow BC.  Takes effect at the next gam_adj_forward. */
+int gam_adj_set_ostate_bc_time(gam_adj *a, int at_state_time);
 /* Milliseconds of the last gam_adj_forward and gam_adj_backward calls
  * (HIP events on the context's stream). */
 int gam_adj_times(gam_adj *a, double *fwd_ms, double *bwd_ms);

@@ -13,7 +13,9 @@
  *    Where the reference prints and stops (step.F90:145-151,
  *    init.F90:817-826, bcs.F90:196-206) these calls return an error code.
  *  - Host arrays are the reference's own Fortran arrays: column-major, i
- *    fastest, ghost lower bounds isg=1-nG.  Cell arrays are
+ *    fastest, ghost lower bounds isg=1-nG.  The state, grid, background and
+ *    adjoint-gradient arrays may also be device memory of the context's GPU
+ *    in that same layout (the copies use hipMemcpyDefault).  Cell arrays are
  *    (Ni,Nj,Nk[,comp...]) with Ni=Nip+2nG, face/edge arrays are
  *    (Ni+1,Nj+1,Nk+1[,comp...]) exactly as allocated by gamutils.F90:169-201
  *    (allocState/allocGridVec) and init.F90:793-1072 (Corners2Grid).
@@ -238,7 +240,10 @@ typedef struct gam_ens gam_ens;
  * fix which of its two state buffers is current; GAM_E_STATE otherwise).
  * The recorded launches keep each member's grid, background, inflow series
  * and launch plans as they were at creation: create the ensemble again
- * after changing any of them. */
+ * after changing any of them (gam_ens_run returns GAM_E_STATE once a
+ * member's gam_set_grid, gam_grid_from_corners, gam_set_background,
+ * gam_add_dipole_b0, gam_set_wind, gam_set_plan or gam_autotune has run
+ * after gam_ens_create). */
 int gam_ens_create(gam_ens **out, int n, gam_ctx *const *members);
 /* nsteps x stepGamera of every member; synchronises at the end and returns
  * the first member's CalcDT failure code, if any (gam_last_error names the

@@ -44,6 +44,7 @@ struct gam_ctx {
     hipStream_t stream;
     double *st[2];        // State / oState buffers (S_NPL planes each)
     int cur;              // which of st[] is State
+    unsigned gen;         // bumped whenever recorded launch arguments change
     double *H;            // half state + primitive planes
     double *met;          // metric planes
     double *bg;           // background planes (+9 scratch)
@@ -73,6 +74,7 @@ struct gam_ctx {
 
 static void graphs_drop(gam_ctx *c)
 {
+    c->gen++;   // ensembles recorded with the old arguments are stale
     if (!c->gx[0] && !c->gx[1]) return;
     hipStreamSynchronize(c->stream);   // no replay may be in flight
     for (int q = 0; q < 2; q++)
@@ -246,13 +248,17 @@ extern "C" int gam_destroy(gam_ctx *c)
 
 // ------------------------------------------------------------ transfers
 // Upload ncomp consecutive Fortran planes of (ni,nj,nk) into device planes.
+// The Fortran arrays may sit in host memory or in device memory of the
+// context's GPU (hipMemcpyDefault lets the runtime pick the direction), so
+// a caller holding its state on the GPU (torch tensors) moves nothing over
+// PCIe.
 static int upload(gam_ctx *c, const double *src, int face, int ncomp, double *dst)
 {
     const Geo &g = c->g;
     const int ni = g.Ni + face, nj = g.Nj + face, nk = g.Nk + face;
     const size_t n = (size_t)ni * nj * nk;
     for (int q = 0; q < ncomp; q++) {
-        HIPCHK(hipMemcpyAsync(c->stage, src + q * n, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->stage, src + q * n, n * sizeof(double), hipMemcpyDefault, c->stream));
         launch_repack(c->stream, 1, g, ni, nj, nk, c->stage, dst + (size_t)q * g.N3);
         HIPCHK(hipGetLastError());
     }
@@ -268,7 +274,7 @@ static int download(gam_ctx *c, const double *src, int face, int ncomp, double *
     for (int q = 0; q < ncomp; q++) {
         launch_repack(c->stream, 0, g, ni, nj, nk, src + (size_t)q * g.N3, c->stage);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(dst + q * n, c->stage, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(dst + q * n, c->stage, n * sizeof(double), hipMemcpyDefault, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
     }
     return GAM_OK;
@@ -582,12 +588,25 @@ static void enq_advance(gam_ctx *c, int finmode)
     tune_mark(c, TS_PREDICT);
     prof_end(c, KG_PREDICT);
 
+    // interface fluxes (CalcStress), with Mom2Face of the same faces; the
+    // EMFs do not depend on the fluxes, so they run after them
+    for (int d = 0; d < 3; d++) {
+        Box fb = act_box(g);
+        if (d == 0) fb.i1 += 1; else if (d == 1) fb.j1 += 1; else fb.k1 += 1;
+        prof_begin(c, KG_FLUX);
+        tune_mark(c, TS_FLUX_I + d);
+        launch_flux(s, d, c->plan.v[TS_FLUX_I + d], g, c->mp, fb, c->H, S, c->met, c->bg, c->gflx, c->mflx,
+                    c->mp.doMHD ? c->vf + (long)3 * d * N3 : nullptr);
+        tune_mark(c, TS_FLUX_I + d);
+        prof_end(c, KG_FLUX);
+    }
     Box eb[3];
     edge_boxes(g, eb);
     if (c->mp.doMHD) {
-        // Mom2Face of every face direction (ranges of fields.F90:119-175)
+        // Mom2Face on the rest of the ranges the EMF stencils read
+        // (fields.F90:119-175)
         prof_begin(c, KG_MOM2FACE);
-        launch_vface(s, g, c->mp, c->H, c->met, c->vf);
+        launch_vface_rim(s, g, c->H, c->met, c->vf);
         prof_end(c, KG_MOM2FACE);
         for (int eD = 0; eD < 3; eD++) {
             // edge direction eD takes the face velocities along dT1
@@ -599,15 +618,6 @@ static void enq_advance(gam_ctx *c, int finmode)
             tune_mark(c, TS_EMF_I + eD);
             prof_end(c, KG_EMF);
         }
-    }
-    for (int d = 0; d < 3; d++) {
-        Box fb = act_box(g);
-        if (d == 0) fb.i1 += 1; else if (d == 1) fb.j1 += 1; else fb.k1 += 1;
-        prof_begin(c, KG_FLUX);
-        tune_mark(c, TS_FLUX_I + d);
-        launch_flux(s, d, c->plan.v[TS_FLUX_I + d], g, c->mp, fb, c->H, S, c->met, c->bg, c->gflx, c->mflx);
-        tune_mark(c, TS_FLUX_I + d);
-        prof_end(c, KG_FLUX);
     }
 
     prof_begin(c, KG_UPDATE);
@@ -890,6 +900,7 @@ struct gam_ens {
     int n;
     gam_ctx **m;
     int *cur0;                       // each member's State buffer at recording
+    unsigned *gen0;                  // each member's launch-argument generation at recording
     int parity;                      // steps taken mod 2 since recording
     hipStream_t stream;
     unsigned char *dev;              // headers and argument records
@@ -910,6 +921,7 @@ extern "C" int gam_ens_destroy(gam_ens *e)
     if (e->stream) hipStreamDestroy(e->stream);
     free(e->m);
     free(e->cur0);
+    free(e->gen0);
     e->ph[0].~vector<EnsPhase>();
     e->ph[1].~vector<EnsPhase>();
     free(e);
@@ -947,7 +959,11 @@ extern "C" int gam_ens_create(gam_ens **out, int n, gam_ctx *const *members)
     e->n = n;
     e->m = (gam_ctx **)calloc(n, sizeof(gam_ctx *));
     e->cur0 = (int *)calloc(n, sizeof(int));
-    if (!e->m || !e->cur0) { gam_ens_destroy(e); return fail(GAM_E_NOMEM, "gam_ens_create: host allocation failed"); }
+    e->gen0 = (unsigned *)calloc(n, sizeof(unsigned));
+    if (!e->m || !e->cur0 || !e->gen0) {
+        gam_ens_destroy(e);
+        return fail(GAM_E_NOMEM, "gam_ens_create: host allocation failed");
+    }
 
     // record two steps of every member (enq_step swaps its buffers twice)
     std::vector<std::vector<EnsLaunch>> rec[2];
@@ -957,6 +973,7 @@ extern "C" int gam_ens_create(gam_ens **out, int n, gam_ctx *const *members)
         gam_ctx *c = members[q];
         e->m[q] = c;
         e->cur0[q] = c->cur;
+        e->gen0[q] = c->gen;
         const int prof = c->prof;
         c->prof = 0;
         for (int r = 0; r < 2; r++) {
@@ -1066,6 +1083,9 @@ static int ens_enqueue(gam_ens *e, int nsteps)
     int q0 = -1;
     for (int q = 0; q < e->n; q++) {
         gam_ctx *c = e->m[q];
+        if (c->gen != e->gen0[q])
+            return fail(GAM_E_STATE, "gam_ens_run: member %d's grid, background, inflow series or launch plan "
+                        "changed since gam_ens_create (create the ensemble again)", q);
         const int rel = c->cur ^ e->cur0[q];
         if (q0 < 0) q0 = rel;
         if (rel != q0)
@@ -1286,6 +1306,7 @@ struct gam_adj {
     double *tape;            // K + 2 State buffers of one interval
     double *itape;           // Kt steps' forward intermediates (IT_NPL planes each), or null
     int Kt;                  // steps of an interval whose intermediates are taped (<= K)
+    int ostAtState;          // start oState's ghosts follow the BCs at State%time
     // a second pair of interval tapes and a stream of its own: the reverse
     // sweep recomputes the next interval into one pair while the adjoint
     // steps of the current interval read the other (null: no overlap)
@@ -1350,6 +1371,12 @@ extern "C" int gam_adj_destroy(gam_adj *a)
 
 extern "C" int gam_adj_create(gam_adj **out, gam_ctx *c, int nsteps, int K)
 {
+    return gam_adj_create_budget(out, c, nsteps, K, -1, 1);
+}
+
+extern "C" int gam_adj_create_budget(gam_adj **out, gam_ctx *c, int nsteps, int K, int max_taped,
+                                     int overlap)
+{
     if (!out) return fail(GAM_E_ARG, "gam_adj_create: NULL argument");
     *out = nullptr;
     int rc = check_ready(c, "gam_adj_create");
@@ -1402,13 +1429,14 @@ extern "C" int gam_adj_create(gam_adj **out, gam_ctx *c, int nsteps, int K)
     // (as many steps of an interval as fit, 4 GiB kept free)
     const size_t slot = (size_t)IT_NPL * pl, spare = (size_t)4 << 30;
     a->Kt = (e == hipSuccess && mfree > spare) ? (int)std::min<size_t>((size_t)K, (mfree - spare) / slot) : 0;
+    if (max_taped >= 0) a->Kt = std::min(a->Kt, max_taped);
     const size_t itb = (size_t)a->Kt * slot;
     if (a->Kt > 0) {
         e = hipMalloc((void **)&a->itape, itb);
         if (e == hipSuccess) e = hipMemset(a->itape, 0, itb);
         // and, where HBM allows, the second pair for the overlapped sweep
         const size_t t2 = (size_t)(K + 2) * CK_NPL * pl;
-        if (e == hipSuccess && a->Kt == K && a->nck > 1 && itb + t2 + spare < mfree - itb) {
+        if (e == hipSuccess && overlap && a->Kt == K && a->nck > 1 && itb + t2 + spare < mfree - itb) {
             e = hipMalloc((void **)&a->itape2, itb);
             if (e == hipSuccess) e = hipMemset(a->itape2, 0, itb);
             if (e == hipSuccess) e = hipMalloc((void **)&a->tape2, t2);
@@ -1522,10 +1550,11 @@ extern "C" int gam_adj_forward(gam_adj *a, double *J)
     int rc = adj_objective(a, S_cur(c), a->lam, J);
     if (rc) return rc;
     HIPCHK(hipEventElapsedTime(&a->fwd_ms, a->ev[0], a->ev[1]));
-    // the start oState's boundary values are those of its own time
+    // the start oState's boundary values are those of its own time, or of
+    // State's when the caller's forward starts from gam_init_run
     Clock ck;
     HIPCHK(hipMemcpy(&ck, a->clkh, sizeof(Clock), hipMemcpyDeviceToHost));
-    ck.t = ck.ost;
+    if (!a->ostAtState) ck.t = ck.ost;
     HIPCHK(hipMemcpy(a->oclk, &ck, sizeof(Clock), hipMemcpyHostToDevice));
     if ((rc = read_clock(c, &ck))) return rc;
     a->recorded = 1;
@@ -1576,13 +1605,13 @@ static void adj_step(gam_adj *a, const StepIT *it, const double *Sn, const doubl
     if (!it) {
         launch_predict(s, c->plan.v[TS_PREDICT], g, c->mp, face_box(g), clk, Sn, On, c->met, c->H);
         if (c->mp.doMHD) {
-            launch_vface(s, g, c->mp, c->H, c->met, c->vf);
             for (int d = 0; d < 3; d++) {
                 Box fb = act_box(g);
                 if (d == 0) fb.i1 += 1; else if (d == 1) fb.j1 += 1; else fb.k1 += 1;
                 launch_flux(s, d, c->plan.v[TS_FLUX_I + d], g, c->mp, fb, c->H, Sn, c->met, c->bg, c->gflx,
-                            c->mflx);
+                            c->mflx, c->vf + (long)3 * d * N3);
             }
+            launch_vface_rim(s, g, c->H, c->met, c->vf);
         }
     }
     if (c->mp.doMHD) {
@@ -1781,6 +1810,21 @@ extern "C" int gam_adj_wind_grad(gam_adj *a, double *gQ, double *gB)
         for (int q = 0; q < WIND_COPIES; q++) v += hb[q * nb + e];
         gB[e] = v;
     }
+    return GAM_OK;
+}
+
+extern "C" int gam_adj_set_ostate_bc_time(gam_adj *a, int at_state_time)
+{
+    if (!a) return fail(GAM_E_ARG, "gam_adj_set_ostate_bc_time: NULL argument");
+    a->ostAtState = at_state_time ? 1 : 0;
+    return GAM_OK;
+}
+
+extern "C" int gam_adj_tape_info(gam_adj *a, int *taped_steps, int *overlap)
+{
+    if (!a) return fail(GAM_E_ARG, "gam_adj_tape_info: NULL argument");
+    if (taped_steps) *taped_steps = a->itape ? a->Kt : 0;
+    if (overlap) *overlap = a->itape2 ? 1 : 0;
     return GAM_OK;
 }
 

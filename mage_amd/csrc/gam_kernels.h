@@ -69,14 +69,16 @@ enum {
 
 void launch_predict(hipStream_t s, int ax, const Geo &g, const MParm &m, const Box &fb, const Clock *clk,
                     const double *S, const double *O, const double *met, double *H);
-// Mom2Face of the three face directions: Vf planes [3*dT1 + comp]
-void launch_vface(hipStream_t s, const Geo &g, const MParm &m, const double *H, const double *met,
-                  double *Vf);
+// Mom2Face on the faces the EMF stencils read past the interface-flux faces
+// (the flux kernels write the rest): Vf planes [3*dT1 + comp]
+void launch_vface_rim(hipStream_t s, const Geo &g, const double *H, const double *met, double *Vf);
 void launch_emf(hipStream_t s, int eD, int ax, const Geo &g, const MParm &m, const Box &b, const Clock *clk,
                 const double *H, const double *S, const double *Vf, const double *met,
                 const double *bg, double *E);
+// Interface fluxes of direction d; with vf (MHD) also Mom2Face of the same
+// faces into vf's three planes of direction d
 void launch_flux(hipStream_t s, int d, int form, const Geo &g, const MParm &m, const Box &b, const double *H,
-                 const double *S, const double *met, const double *bg, double *gf, double *mf);
+                 const double *S, const double *met, const double *bg, double *gf, double *mf, double *vf);
 void launch_update(hipStream_t s, int ax, const Geo &g, const MParm &m, const Box &fb, const Box &ctb,
                    const Box *eboxes, const Box &ab, const Box &dtb, Clock *clk, const double *So,
                    double *Sn, const double *gf, const double *mf, const double *met, const double *bg);

@@ -149,6 +149,15 @@ __device__ __forceinline__ void recon_lr_w(double pdmb, double rVi, const double
     }
 }
 
+// Mom2Face (fields.F90:235-285) of one face from the volume-weighted
+// velocity stencil qv[n] = V_n v_n and the stencil's volume interpolant
+// Vi (rVi = 1/Vi): <V v>_8 / max(<V>_8, TINY).
+__device__ __forceinline__ double mom2face(const double *qv, double Vi, double rVi)
+{
+    const double r = Vi > GAM_TINY ? rVi : 1.0 / GAM_TINY;
+    return wsum8([&](int n) { return qv[n]; }) * r;
+}
+
 // ---------------------------------------------------------- fluxes
 // One side (s = -1 left, +1 right) of the Lyon gas-kinetic flux,
 // GasKinFlux (stress.F90:432-511).  T = face triad (N, T1, T2).

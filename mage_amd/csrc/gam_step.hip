@@ -3,9 +3,10 @@
 // Kernel sequence of one stepGamera (gamapp.F90:73-113):
 //   k_predict    Predictor: AB2 face fluxes, half-step B and gas, plus the
 //                primitives used by the flux sweeps
-//   k_vface      face velocities of all three face directions
+//   k_flux       Lyon gas/magnetic fluxes, one per direction, and the face
+//                velocities (Mom2Face) of the same faces
+//   k_vface_rim  face velocities the EMF stencils read past the flux faces
 //   k_emf        edge EMFs
-//   k_flux       Lyon gas/magnetic fluxes, one per direction
 //   k_update     CT face-flux update, cell B of the new state, flux
 //                divergence, Reynolds, Maxwell/Boris and the CFL
 //                min-reduction of the new state
@@ -284,136 +285,48 @@ struct k_predict {
 };
 
 // ====================================================================
-// Column tiles for kernels that march along k: a workgroup owns 256
-// consecutive (i, j) columns of the plane (i fastest) over kc consecutive k,
-// so k-direction stencils slide through registers instead of being re-read.
-// Same XCD-aware numbering as Tile: column chunk fastest, then k-chunk.
-// kc is 32 where that still gives ~1024 workgroups (4 per CU), shorter on
-// small grids so that every CU gets work (each chunk re-reads 7 planes of
-// its window, so short chunks cost reads, not results).
+// The rest of Mom2Face.  The interface-flux kernels write the face
+// velocities of their own faces; the EMF stencils (GetCornerV, 8 faces
+// along dT2) also reach 4 faces past them on each side:
+//   I faces at j in js-4..js-1 and je+1..je+4 (k-edges, dT2 = J),
+//   J faces at k in ks-4..ks-1 and ke+1..ke+4 (i-edges, dT2 = K),
+//   K faces at i in is-4..is-1 and ie+1..ie+4 (j-edges, dT2 = I).
+// One launch covers the six slabs; slab q is face direction q / 2.
 // ====================================================================
-struct ColTile {
-    Box b;
-    int ni, nj;       // extents along i and j
-    int nc, nk, nb;   // 256-column chunks of the (i, j) plane, k chunks, total
-    int kc;           // k extent of a chunk
+struct RimSet {
+    Tile t[6];
+    int start[7];    // first workgroup of each slab
 };
 
-static inline ColTile coltile_of(const Box &b)
-{
-    ColTile t;
-    t.b = b;
-    t.ni = b.i1 - b.i0 + 1;
-    t.nj = b.j1 - b.j0 + 1;
-    t.nc = (t.ni * t.nj + BX * BY - 1) / (BX * BY);
-    const int nkc = b.k1 - b.k0 + 1;
-    t.kc = 32;
-    while (t.kc > 4 && t.nc * ((nkc + t.kc - 1) / t.kc) < 1024) t.kc >>= 1;
-    t.nk = (nkc + t.kc - 1) / t.kc;
-    t.nb = t.nc * t.nk;
-    return t;
-}
-static inline dim3 cgrid(const ColTile &t) { return dim3((unsigned)(((t.nb + 7) / 8) * 8)); }
-
-// Column of this thread: (i, j) and the k range [k0, k1]; false if none.
-__device__ __forceinline__ bool col_point(const ColTile &t, int w, int &i, int &j, int &k0, int &k1)
-{
-    const int per = (t.nb + 7) >> 3;
-    const int L = (w & 7) * per + (w >> 3);
-    if (L >= t.nb) return false;
-    const int ch = L % t.nc;
-    const int t2 = L / t.nc;
-    const unsigned p = (unsigned)ch * (BX * BY) + threadIdx.x + BX * threadIdx.y;
-    if (p >= (unsigned)(t.ni * t.nj)) return false;
-    const int a = (int)(p / (unsigned)t.ni);
-    i = t.b.i0 + (int)(p - (unsigned)a * t.ni);
-    j = t.b.j0 + a;
-    k0 = t.b.k0 + t2 * t.kc;
-    k1 = min(k0 + t.kc - 1, t.b.k1);
-    return true;
-}
-
-// ====================================================================
-// Mom2Face (fields.F90:235-285) for all three face directions in one pass:
-// Vf[3*dT1 + c] = <V * rho v / max(rho, dFloor)>_8 / <V>_8 along dT1, on
-// the face ranges of CalcElecField (fields.F90:119-175): dT1 = I on
-// i in is..ie+1, J on j in js..je+1, K on k in ks..ke+1 (other indices over
-// all cells).  The velocities rho*v/max(rho,dFloor) are the half-state
-// primitives the predictor stored.  The k stencil slides along the column.
-// ====================================================================
-struct k_vface {
-    __device__ __forceinline__ void operator()(const uint3 bi, Geo g, MParm m,
-                                               ColTile t, const double *__restrict__ H,
+struct k_vface_rim {
+    __device__ __forceinline__ void operator()(const uint3 bi, Geo g, RimSet r,
+                                               const double *__restrict__ H,
                                                const double *__restrict__ met,
                                                double *__restrict__ Vf) const
     {
-        int i, j, k0, k1;
-        if (!col_point(t, (int)bi.x, i, j, k0, k1)) return;
+        const int w = (int)bi.x;
+        int q = 0;
+#pragma unroll
+        for (int p = 1; p < 6; p++) q += (w >= r.start[p]) ? 1 : 0;
+        int i, j, k;
+        if (!tile_point(r.t[q], w - r.start[q], i, j, k)) return;
+        const int d = q >> 1;
         const long N3 = g.N3;
+        const long c = IX(g, i, j, k);
+        const long s = d == 0 ? 1L : (d == 1 ? (long)g.P : g.PP);
         const double *vol = met + (long)M_VOL * N3;
         const double *wv = H + (long)(H_WH + 1) * N3;   // V * velocity
-        // k window: slot n holds cell k-4+n (zero outside the cell range)
-        double zv[8], zx[8], zy[8], zz[8];
+        double v8[8];
 #pragma unroll
-        for (int n = 0; n < 7; n++) {
-            const int kk = k0 - 4 + n;
-            const bool in = kk >= g.ksg && kk <= g.keg;
-            const long q = IX(g, i, j, in ? kk : g.ksg);
-            zv[n] = in ? vol[q] : 0.0;
-            zx[n] = in ? wv[q] : 0.0;
-            zy[n] = in ? wv[N3 + q] : 0.0;
-            zz[n] = in ? wv[2 * N3 + q] : 0.0;
-        }
-        const bool iI = i >= g.is && i <= g.ie + 1;
-        const bool iJ = j >= g.js && j <= g.je + 1;
-#pragma unroll 1
-        for (int kl = k0; kl <= k1; kl++) {
-            int k = kl;
-            asm volatile("" : "+v"(k));   // one address base per iteration
-            {
-                const int kk = k + 3;
-                const bool in = kk <= g.keg;
-                long q = IX(g, i, j, in ? kk : g.keg);
-                asm volatile("" : "+v"(q));
-                zv[7] = in ? vol[q] : 0.0;
-                zx[7] = in ? wv[q] : 0.0;
-                zy[7] = in ? wv[N3 + q] : 0.0;
-                zz[7] = in ? wv[2 * N3 + q] : 0.0;
-            }
-            long c = IX(g, i, j, k);
-            asm volatile("" : "+v"(c));
-            if (k >= g.ks && k <= g.ke + 1) {
-                const double r = 1.0 / dmax(wsum8([&](int n) { return zv[n]; }), GAM_TINY);
-                Vf[(6 + 0) * N3 + c] = wsum8([&](int n) { return zx[n]; }) * r;
-                Vf[(6 + 1) * N3 + c] = wsum8([&](int n) { return zy[n]; }) * r;
-                Vf[(6 + 2) * N3 + c] = wsum8([&](int n) { return zz[n]; }) * r;
-            }
-            if (iI) {
-                double v8[8], x8[8], y8[8], z8[8];
+        for (int n = 0; n < 8; n++) v8[n] = vol[c + (n - 4) * s];
+        const double Vi = wsum8([&](int n) { return v8[n]; });
+        const double rVi = 1.0 / Vi;
 #pragma unroll
-                for (int n = 0; n < 8; n++) {
-                    const long q = c + (n - 4);
-                    v8[n] = vol[q]; x8[n] = wv[q]; y8[n] = wv[N3 + q]; z8[n] = wv[2 * N3 + q];
-                }
-                const double r = 1.0 / dmax(wsum8([&](int n) { return v8[n]; }), GAM_TINY);
-                Vf[0 * N3 + c] = wsum8([&](int n) { return x8[n]; }) * r;
-                Vf[1 * N3 + c] = wsum8([&](int n) { return y8[n]; }) * r;
-                Vf[2 * N3 + c] = wsum8([&](int n) { return z8[n]; }) * r;
-            }
-            if (iJ) {
-                double v8[8], x8[8], y8[8], z8[8];
+        for (int x = 0; x < 3; x++) {
+            double q8[8];
 #pragma unroll
-                for (int n = 0; n < 8; n++) {
-                    const long q = c + (long)(n - 4) * g.P;
-                    v8[n] = vol[q]; x8[n] = wv[q]; y8[n] = wv[N3 + q]; z8[n] = wv[2 * N3 + q];
-                }
-                const double r = 1.0 / dmax(wsum8([&](int n) { return v8[n]; }), GAM_TINY);
-                Vf[3 * N3 + c] = wsum8([&](int n) { return x8[n]; }) * r;
-                Vf[4 * N3 + c] = wsum8([&](int n) { return y8[n]; }) * r;
-                Vf[5 * N3 + c] = wsum8([&](int n) { return z8[n]; }) * r;
-            }
-#pragma unroll
-            for (int n = 0; n < 7; n++) { zv[n] = zv[n + 1]; zx[n] = zx[n + 1]; zy[n] = zy[n + 1]; zz[n] = zz[n + 1]; }
+            for (int n = 0; n < 8; n++) q8[n] = wv[x * N3 + c + (n - 4) * s];
+            Vf[(3 * d + x) * N3 + c] = mom2face(q8, Vi, rVi);
         }
     }
 };
@@ -607,7 +520,8 @@ struct k_flux {
                                                const double *__restrict__ met,
                                                const double *__restrict__ bg,
                                                double *__restrict__ gflx,
-                                               double *__restrict__ mflx) const
+                                               double *__restrict__ mflx,
+                                               double *__restrict__ Vf) const
     {
         TILE_IJK(b);
         const long N3 = g.N3;
@@ -628,7 +542,8 @@ struct k_flux {
         double vol[8];
 #pragma unroll
         for (int n = 0; n < 8; n++) vol[n] = LD(met + M_VOL * N3, n);
-        const double rVi = 1.0 / wsum8([&](int n) { return vol[n]; });
+        const double Vi = wsum8([&](int n) { return vol[n]; });
+        const double rVi = 1.0 / Vi;
 
         // L/R primitives (BlockStateLRs, recon.F90:39-93); Wv holds V*W
         const double *Wv = H + H_WH * N3;
@@ -642,6 +557,8 @@ struct k_flux {
 #pragma unroll
             for (int n = 0; n < 4; n++) nq[n] = LD(Wl + v * N3, n + 2);
             recon_lr_w<RM>(m.pdmb, rVi, nq, qv, WL[v], WR[v]);
+            // Mom2Face of this face from the same velocity stencil
+            if (v >= 1 && v <= 3 && Vf) stp(Vf + (v - 1) * N3, o[4], mom2face(qv, Vi, rVi));
         }
         auto recon_b = [&](double *BL, double *BR) {
             const double *Bh = H + H_B * N3;
@@ -690,7 +607,8 @@ struct k_flux_s {
                                                const double *__restrict__ met,
                                                const double *__restrict__ bg,
                                                double *__restrict__ gflx,
-                                               double *__restrict__ mflx) const
+                                               double *__restrict__ mflx,
+                                               double *__restrict__ Vf) const
     {
         __shared__ double vs[SR8][BX];
         __shared__ double buf[2][SRW][BX];
@@ -761,7 +679,8 @@ struct k_flux_s {
             store(0, reg);
         }
         __syncthreads();
-        const double rVi = 1.0 / wsum8([&](int n) { return vs[ty + n][tx]; });
+        const double Vi = wsum8([&](int n) { return vs[ty + n][tx]; });
+        const double rVi = 1.0 / Vi;
         double WL[5], WR[5], BL[3] = {0.0, 0.0, 0.0}, BR[3] = {0.0, 0.0, 0.0};
         // kept a loop: unrolled, the compiler hoists later stages' loads and
         // the register count doubles
@@ -781,6 +700,8 @@ struct k_flux_s {
             for (int n = 0; n < 4; n++) nq[n] = buf[cur][SR8 + ty + n][tx];
             if (st < 5) {
                 recon_lr_w<RM>(m.pdmb, rVi, nq, q, WL[st], WR[st]);
+                // Mom2Face of this face from the same velocity stencil
+                if (st >= 1 && st <= 3 && Vf && act) Vf[(st - 1) * N3 + at(i, a)] = mom2face(q, Vi, rVi);
             } else {
                 // cell B is not volume-weighted in H: weight it here (LDS volumes)
 #pragma unroll
@@ -1492,12 +1413,24 @@ void launch_predict(hipStream_t s, int ax, const Geo &g, const MParm &m, const B
     launch_f<k_predict>(tgrid(t), dim3(BX, BY), s, g, m, t, clk, S, O, met, H);
 }
 
-void launch_vface(hipStream_t s, const Geo &g, const MParm &m, const double *H, const double *met,
-                  double *Vf)
+void launch_vface_rim(hipStream_t s, const Geo &g, const double *H, const double *met, double *Vf)
 {
-    const Box b = {g.isg, g.ieg, g.jsg, g.jeg, g.ksg, g.keg};
-    const ColTile t = coltile_of(b);
-    launch_f<k_vface>(cgrid(t), dim3(BX, BY), s, g, m, t, H, met, Vf);
+    const Box bx[6] = {
+        {g.is, g.ie + 1, g.js - 4, g.js - 1, g.ks, g.ke},      // I faces, j rim
+        {g.is, g.ie + 1, g.je + 1, g.je + 4, g.ks, g.ke},
+        {g.is, g.ie, g.js, g.je + 1, g.ks - 4, g.ks - 1},      // J faces, k rim
+        {g.is, g.ie, g.js, g.je + 1, g.ke + 1, g.ke + 4},
+        {g.is - 4, g.is - 1, g.js, g.je, g.ks, g.ke + 1},      // K faces, i rim
+        {g.ie + 1, g.ie + 4, g.js, g.je, g.ks, g.ke + 1}};
+    RimSet r;
+    int tot = 0;
+    for (int q = 0; q < 6; q++) {
+        r.t[q] = tile_of(bx[q], q < 4 ? 2 : 1);
+        r.start[q] = tot;
+        tot += (int)tgrid(r.t[q]).x;
+    }
+    r.start[6] = tot;
+    launch_f<k_vface_rim>(dim3((unsigned)tot), dim3(BX, BY), s, g, r, H, met, Vf);
 }
 
 template <int ED, int RM, bool BG, bool BO>
@@ -1538,7 +1471,7 @@ void launch_emf(hipStream_t s, int eD, int ax, const Geo &g, const MParm &m, con
 
 template <int D, int RM, bool BG, bool BO>
 static void flux_t(hipStream_t s, int form, const Geo &g, const MParm &m, const Box &b, const double *H,
-                   const double *S, const double *met, const double *bg, double *gf, double *mf)
+                   const double *S, const double *met, const double *bg, double *gf, double *mf, double *vf)
 {
     if constexpr (D != 0) {
         if (form == PLAN_LDS) {
@@ -1551,37 +1484,37 @@ static void flux_t(hipStream_t s, int form, const Geo &g, const MParm &m, const 
             t.nta = (t.a1 - t.a0 + BY) / BY;
             t.nb = t.nti * t.nta * (t.b1 - t.b0 + 1);
             launch_f<k_flux_s<D, RM, BG, BO>>(dim3((unsigned)(((t.nb + 7) / 8) * 8)), dim3(BX, BY), s, g, m,
-                                              t, H, S, met, bg, gf, mf);
+                                              t, H, S, met, bg, gf, mf, vf);
             return;
         }
     }
     const Tile t = tile_of(b, form == PLAN_LDS ? 1 : form);
-    launch_f<k_flux<D, RM, BG, BO>>(tgrid(t), dim3(BX, BY), s, g, m, t, H, S, met, bg, gf, mf);
+    launch_f<k_flux<D, RM, BG, BO>>(tgrid(t), dim3(BX, BY), s, g, m, t, H, S, met, bg, gf, mf, vf);
 }
 
 template <int D>
 static void flux_d(hipStream_t s, int form, const Geo &g, const MParm &m, const Box &b, const double *H,
-                   const double *S, const double *met, const double *bg, double *gf, double *mf)
+                   const double *S, const double *met, const double *bg, double *gf, double *mf, double *vf)
 {
     const int sel = (m.rmeth ? 4 : 0) + (m.doBG ? 2 : 0) + (m.doBoris ? 1 : 0);
     switch (sel) {
-    case 0: flux_t<D, 0, false, false>(s, form, g, m, b, H, S, met, bg, gf, mf); break;
-    case 1: flux_t<D, 0, false, true>(s, form, g, m, b, H, S, met, bg, gf, mf); break;
-    case 2: flux_t<D, 0, true, false>(s, form, g, m, b, H, S, met, bg, gf, mf); break;
-    case 3: flux_t<D, 0, true, true>(s, form, g, m, b, H, S, met, bg, gf, mf); break;
-    case 4: flux_t<D, 1, false, false>(s, form, g, m, b, H, S, met, bg, gf, mf); break;
-    case 5: flux_t<D, 1, false, true>(s, form, g, m, b, H, S, met, bg, gf, mf); break;
-    case 6: flux_t<D, 1, true, false>(s, form, g, m, b, H, S, met, bg, gf, mf); break;
-    default: flux_t<D, 1, true, true>(s, form, g, m, b, H, S, met, bg, gf, mf); break;
+    case 0: flux_t<D, 0, false, false>(s, form, g, m, b, H, S, met, bg, gf, mf, vf); break;
+    case 1: flux_t<D, 0, false, true>(s, form, g, m, b, H, S, met, bg, gf, mf, vf); break;
+    case 2: flux_t<D, 0, true, false>(s, form, g, m, b, H, S, met, bg, gf, mf, vf); break;
+    case 3: flux_t<D, 0, true, true>(s, form, g, m, b, H, S, met, bg, gf, mf, vf); break;
+    case 4: flux_t<D, 1, false, false>(s, form, g, m, b, H, S, met, bg, gf, mf, vf); break;
+    case 5: flux_t<D, 1, false, true>(s, form, g, m, b, H, S, met, bg, gf, mf, vf); break;
+    case 6: flux_t<D, 1, true, false>(s, form, g, m, b, H, S, met, bg, gf, mf, vf); break;
+    default: flux_t<D, 1, true, true>(s, form, g, m, b, H, S, met, bg, gf, mf, vf); break;
     }
 }
 
 void launch_flux(hipStream_t s, int d, int form, const Geo &g, const MParm &m, const Box &b, const double *H,
-                 const double *S, const double *met, const double *bg, double *gf, double *mf)
+                 const double *S, const double *met, const double *bg, double *gf, double *mf, double *vf)
 {
-    if (d == 0) flux_d<0>(s, form, g, m, b, H, S, met, bg, gf, mf);
-    else if (d == 1) flux_d<1>(s, form, g, m, b, H, S, met, bg, gf, mf);
-    else flux_d<2>(s, form, g, m, b, H, S, met, bg, gf, mf);
+    if (d == 0) flux_d<0>(s, form, g, m, b, H, S, met, bg, gf, mf, vf);
+    else if (d == 1) flux_d<1>(s, form, g, m, b, H, S, met, bg, gf, mf, vf);
+    else flux_d<2>(s, form, g, m, b, H, S, met, bg, gf, mf, vf);
 }
 
 void launch_update(hipStream_t s, int ax, const Geo &g, const MParm &m, const Box &fb, const Box &ctb,

@@ -219,6 +219,24 @@ module gamhip
             type(c_ptr), value :: ctx
             integer(c_int), value :: nsteps, ckpt_every
         end function
+        integer(c_int) function gam_adj_create_budget(adj, ctx, nsteps, ckpt_every, max_taped, overlap) &
+                bind(C, name="gam_adj_create_budget")
+            import :: c_int, c_ptr
+            type(c_ptr) :: adj
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nsteps, ckpt_every, max_taped, overlap
+        end function
+        integer(c_int) function gam_adj_set_ostate_bc_time(adj, at_state_time) &
+                bind(C, name="gam_adj_set_ostate_bc_time")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: adj
+            integer(c_int), value :: at_state_time
+        end function
+        integer(c_int) function gam_adj_tape_info(adj, taped_steps, overlap) bind(C, name="gam_adj_tape_info")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: adj
+            integer(c_int) :: taped_steps, overlap
+        end function
         integer(c_int) function gam_adj_destroy(adj) bind(C, name="gam_adj_destroy")
             import :: c_int, c_ptr
             type(c_ptr), value :: adj

@@ -159,6 +159,7 @@ class GamHip:
         self.Ni, self.Nj, self.Nk = Nip + 8, Njp + 8, Nkp + 8
         self.params = dict(params)
         self.ctx = C.c_void_p()
+        self.wind = None
         d = Dims(Nip, Njp, Nkp, 4)
         m = model_struct(params)
         self._chk(self.L.gam_create(C.byref(self.ctx), C.byref(d), C.byref(m)))
@@ -240,6 +241,7 @@ class GamHip:
         self._chk(self.L.gam_set_wind(self.ctx, len(tW), _p(tW), _p(Q), _p(B),
                                       float(wind.get("Bx0", 0.0)), float(wind.get("ByC", 0.0)),
                                       float(wind.get("BzC", 0.0)), _p(xyzW)))
+        self.wind = dict(wind)
 
     # ---- state
     def put_state(self, st, which=GAM_STATE):
@@ -403,13 +405,18 @@ class GamEnsemble:
 
 # ---------------------------------------------------------------- adjoint
 # Every symbol include/gamadj.h declares (exported by libgamhip)
-ADJ_EXPORTS = ("gam_adj_create", "gam_adj_destroy", "gam_adj_set_weights", "gam_adj_forward",
-               "gam_adj_backward", "gam_adj_replay", "gam_adj_wind_grad", "gam_adj_times")
+ADJ_EXPORTS = ("gam_adj_create", "gam_adj_create_budget", "gam_adj_tape_info", "gam_adj_set_ostate_bc_time",
+               "gam_adj_destroy",
+               "gam_adj_set_weights", "gam_adj_forward", "gam_adj_backward", "gam_adj_replay",
+               "gam_adj_wind_grad", "gam_adj_times")
 
 
 def _load_adj(L):
     vp = C.c_void_p
     L.gam_adj_create.argtypes = [C.POINTER(vp), vp, C.c_int, C.c_int]
+    L.gam_adj_create_budget.argtypes = [C.POINTER(vp), vp, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.gam_adj_tape_info.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.gam_adj_set_ostate_bc_time.argtypes = [vp, C.c_int]
     L.gam_adj_destroy.argtypes = [vp]
     L.gam_adj_set_weights.argtypes = [vp, DP]
     L.gam_adj_forward.argtypes = [vp, DP]
@@ -426,11 +433,30 @@ class GamAdjoint:
     State/oState checkpoints every ckpt_every steps; wind_grad gives the
     gradient with respect to the inflow series."""
 
-    def __init__(self, g, nsteps, ckpt_every):
+    def __init__(self, g, nsteps, ckpt_every, max_taped=None, overlap=True):
+        """max_taped / overlap: the tape budget (gam_adj_create_budget);
+        by default as many taped steps as HBM allows, with the overlapped
+        sweep where a second tape pair fits."""
         self.L = _load_adj(load())
         self.g = g
         self.adj = C.c_void_p()
-        self._chk(self.L.gam_adj_create(C.byref(self.adj), g.ctx, int(nsteps), int(ckpt_every)))
+        if max_taped is None and overlap:
+            self._chk(self.L.gam_adj_create(C.byref(self.adj), g.ctx, int(nsteps), int(ckpt_every)))
+        else:
+            self._chk(self.L.gam_adj_create_budget(C.byref(self.adj), g.ctx, int(nsteps), int(ckpt_every),
+                                                   -1 if max_taped is None else int(max_taped),
+                                                   1 if overlap else 0))
+
+    def set_ostate_bc_time(self, at_state_time):
+        """oState's ghosts at State%time (True, what init_run produces) or
+        at oState%time (False, the default); next forward on."""
+        self._chk(self.L.gam_adj_set_ostate_bc_time(self.adj, 1 if at_state_time else 0))
+
+    def tape_info(self):
+        """(steps of an interval with taped intermediates, overlapped sweep)"""
+        kt, ov = C.c_int(), C.c_int()
+        self._chk(self.L.gam_adj_tape_info(self.adj, C.byref(kt), C.byref(ov)))
+        return kt.value, bool(ov.value)
 
     def _chk(self, rc):
         if rc != GAM_OK:

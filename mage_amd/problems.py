@@ -238,6 +238,36 @@ def shell_state(Nip, Njp, Nkp, xyzcc, gamma=5.0 / 3.0, seed=SEED0, p0=1.0e-3):
     return st
 
 
+def shell_mhd_state(Nip, Njp, Nkp, xyzcc, xyz, gamma=5.0 / 3.0, seed=SEED0, p0=0.05, vamp=0.3,
+                    aamp=0.3):
+    """Shell IC with every MHD term active: rho = 1 + 0.2*f1, p = p0*(1 +
+    0.5*f2), v = vamp * (8 seeded modes)/8 per component, residual B from a
+    seeded vector potential (8 modes of amplitude aamp on the 60-wide box)
+    through edge line integrals, so div(dB) vanishes to round-off.  The f are
+    seeded smooth fields in [-1, 1]; |v| spans ~0.05-0.3 across the shell."""
+    rng = SplitMix64(seed + 2)
+    rmodes = fourier_modes(rng, 8)
+    pmodes = fourier_modes(rng, 8)
+    vmodes = [fourier_modes(rng, 8) for _ in range(3)]
+    amodes = fourier_modes(rng, 8)
+    st = empty_state(Nip, Njp, Nkp)
+    X, Y, Z = xyzcc[..., 0], xyzcc[..., 1], xyzcc[..., 2]
+    rho = 1.0 + 0.2 * _modesum(rmodes, X, Y, Z, scale=60.0) / 8.0
+    p = p0 * (1.0 + 0.5 * _modesum(pmodes, X, Y, Z, scale=60.0) / 8.0)
+    v = [vamp * _modesum(vm, X, Y, Z, scale=60.0) / 8.0 for vm in vmodes]
+    st["Gas"][...] = prim2con(gamma, rho, v[0], v[1], v[2], p)
+
+    def A(x, y, z):
+        out = [np.zeros(np.broadcast(x, y, z).shape) for _ in range(3)]
+        for k, ph, d in amodes:
+            c = aamp * np.cos(2 * math.pi * (k[0] * x + k[1] * y + k[2] * z) / 60.0 + ph)
+            for q in range(3):
+                out[q] += d[q] * c
+        return out
+    st["magFlux"][...] = vecpot_fluxes(xyz, A)
+    return st
+
+
 def constant_wind(rho=1.0, vx=-0.4, p=1.0e-3, bz=-0.05):
     """A two-sample constant inflow series covering every time the runs reach."""
     tW = np.array([-1.0e3, 1.0e3])

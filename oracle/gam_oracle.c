@@ -953,6 +953,17 @@ static void calc_elec_field(const orc_dims *D, const orc_model *M, const orc_gri
 }
 
 /* ------------------------------------------------------ stress */
+/* Faces on which the nuke term fired since the last orc_nuke_count(1)
+ * (instrumentation for tests: shows that a case exercises stress.F90:399-417). */
+static long nuke_faces = 0;
+
+long orc_nuke_count(int reset)
+{
+    long n = nuke_faces;
+    if (reset) nuke_faces = 0;
+    return n;
+}
+
 static void face_flux(const orc_dims *D, const orc_model *M, const orc_grid *G,
                       const orc_state *nS, const orc_state *hS, int i, int j, int k,
                       int dN, double *gFo, double *mFo)
@@ -1024,6 +1035,8 @@ static void face_flux(const orc_dims *D, const orc_model *M, const orc_grid *G,
         double vR = sqrt(PrimR[1] * PrimR[1] + PrimR[2] * PrimR[2] + PrimR[3] * PrimR[3]);
         double Vn = sqrt(vL * vL + vR * vR) / SQRT2_SP;
         if (Vn >= CLIM * M->Ca) {
+#pragma omp atomic
+            nuke_faces++;
             for (int v = 0; v < 5; v++) {
                 double dCon = ConB[v][4] - ConB[v][3];
                 gF[v] = gF[v] - M->Ca * dCon;

@@ -141,6 +141,7 @@ double orc_wind_wgt(const orc_model *M, const double *vFr, const double *xyz, do
 double orc_ramp_down(double r, double rC, double lC);
 /* Access to solver intermediates for staged comparisons */
 double *orc_solver_array(orc_solver *W, int which);
+long   orc_nuke_count(int reset);
 enum { ORC_W_GFLX = 0, ORC_W_MFLX = 1, ORC_W_VF = 2, ORC_W_DGASH = 3,
        ORC_W_DGASM = 4, ORC_W_HF_GAS = 5, ORC_W_HF_MAGFLUX = 6,
        ORC_W_HF_BXYZ = 7 };

@@ -12,6 +12,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
+# the same source built for speed (-O3, AVX2/FMA): the CPU baseline only
+LIB_FAST = os.path.join(HERE, "liboracle_fast.so")
 
 BC = {"periodic": 0, "zerograd": 1, "sphin": 2, "sphout": 3, "wind": 4}
 DP = C.POINTER(C.c_double)
@@ -50,15 +52,16 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB):
+def lib(fast=False):
+    """The strict parity build, or with fast=True the speed build."""
+    path = LIB_FAST if fast else LIB
+    if path not in _libs:
+        if not os.path.exists(path):
             build()
-        L = C.CDLL(LIB)
+        L = C.CDLL(path)
         vp = C.c_void_p
         L.orc_dims_init.argtypes = [C.POINTER(Dims), C.c_int, C.c_int, C.c_int]
         L.orc_cell_c2p.argtypes = [C.POINTER(Model), DP, DP]
@@ -103,8 +106,15 @@ def lib():
         L.orc_wind_wgt.restype = C.c_double
         L.orc_ramp_down.argtypes = [C.c_double] * 3
         L.orc_ramp_down.restype = C.c_double
-        _lib = L
-    return _lib
+        L.orc_nuke_count.argtypes = [C.c_int]
+        L.orc_nuke_count.restype = C.c_long
+        _libs[path] = L
+    return _libs[path]
+
+
+def nuke_count(reset=True):
+    """Faces on which the oracle's nuke term fired since the last reset."""
+    return int(lib().orc_nuke_count(1 if reset else 0))
 
 
 def ptr(a):
@@ -144,8 +154,8 @@ def make_model(p, wind=None):
 class Oracle:
     """One oracle run: dims, model, grid arrays and the State/oState pair."""
 
-    def __init__(self, Nip, Njp, Nkp, params, grid, wind=None):
-        self.L = lib()
+    def __init__(self, Nip, Njp, Nkp, params, grid, wind=None, fast=False):
+        self.L = lib(fast)
         self.D = Dims()
         self.L.orc_dims_init(C.byref(self.D), Nip, Njp, Nkp)
         self.M = make_model(params, wind)

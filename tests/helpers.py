@@ -38,7 +38,10 @@ def box_case(N=(16, 16, 16), rmeth="7UP", seed=pb.SEED0, gpu=True, **over):
     return g, o, grid, st, p
 
 
-def shell_case(N=(32, 32, 32), rmeth="7UP", seed=pb.SEED0, wind=None, **over):
+def shell_case(N=(32, 32, 32), rmeth="7UP", seed=pb.SEED0, wind=None, state="quiet", **over):
+    """The warped shell with the dipole, Boris and the inflow BC.  state
+    "quiet" is the benchmark IC (v = 0, no residual B); "mhd" has seeded
+    flow and a seeded residual field (problems.shell_mhd_state)."""
     Nip, Njp, Nkp = N
     xyz = pb.shell_corners(Nip, Njp, Nkp, seed=seed)
     p = pb.shell_params(rmeth)
@@ -51,7 +54,10 @@ def shell_case(N=(32, 32, 32), rmeth="7UP", seed=pb.SEED0, wind=None, **over):
     grid = g.get_grid()
     grid.update(g.get_background())
     grid["xyz"] = xyz
-    st = pb.shell_state(Nip, Njp, Nkp, grid["xyzcc"], seed=seed)
+    if state == "mhd":
+        st = pb.shell_mhd_state(Nip, Njp, Nkp, grid["xyzcc"], xyz, seed=seed)
+    else:
+        st = pb.shell_state(Nip, Njp, Nkp, grid["xyzcc"], seed=seed)
     o = po.Oracle(Nip, Njp, Nkp, p, grid, wind=wind)
     o.set_state(st)
     g.put_state(st)

@@ -462,3 +462,66 @@ def test_torch_autograd_bridge():
     assert best <= 1e-4
     obj.close()
     g.close()
+
+
+def test_torch_bridge_inflow_shell_and_reuse():
+    """StepObjective with the inflow series as an input, on the MHD shell
+    with a time-varying inflow: the gradients with respect
+    to gas0 and to the series match central differences of the bridge's own
+    forward (obj itself, no replay), and J = obj(a) + obj(b) before one
+    backward gives each call its own gradient."""
+    import torch
+    from mage_amd.autograd import StepObjective
+    N = (24, 16, 24)
+    g = gamhip.GamHip(*N, pb.shell_params("7UP"))
+    xyz = pb.shell_corners(*N)
+    g.grid_from_corners(xyz)
+    g.add_dipole_b0(pb.SHELL_M0)
+    wind = _wind_series(seed=5)
+    g.set_wind(wind)
+    xyzcc = g.get_grid(fields=("xyzcc",))["xyzcc"]
+    g.put_state(pb.shell_mhd_state(*N, xyzcc, xyz, p0=0.1))
+    g.init_run()
+    _generic(g, 23)
+    g.run(2)
+    base = g.get_state()["Gas"]
+    obj = StepObjective(g, 4, 2, _outer_weights(g), wind=True)
+    Q0 = torch.from_numpy(np.ascontiguousarray(wind["Q"]))
+    B0 = torch.from_numpy(np.ascontiguousarray(wind["B"]))
+    gas0 = torch.from_numpy(np.ascontiguousarray(base)).requires_grad_(True)
+    Q = Q0.clone().requires_grad_(True)
+    B = B0.clone().requires_grad_(True)
+    J = obj(gas0, Q, B)
+    J.backward()
+    ggas, gQ, gB = gas0.grad.numpy(), Q.grad.numpy(), B.grad.numpy()
+    assert np.all(ggas[:4] == 0.0)
+    rng = np.random.default_rng(29)
+    d = _directions(base.shape, 1, 31)[0] * np.max(np.abs(base), axis=(0, 1, 2)) * 1e-2
+    dQ = rng.standard_normal(wind["Q"].shape) * np.max(np.abs(wind["Q"]), axis=0) * 1e-2
+    dQ[:, 1:4] = 0.0      # keep the lag (a step function of the velocity) in place
+    dB = np.zeros(wind["B"].shape)
+    dB[:, 1:] = rng.standard_normal((len(wind["tW"]), 2)) * 1e-4
+    td, tQ, tB = (torch.from_numpy(np.ascontiguousarray(x)) for x in (d, dQ, dB))
+    with torch.no_grad():
+        for name, ad, pert in (("gas", np.sum(ggas * d), lambda e: (gas0 + e * td, Q0, B0)),
+                               ("series", np.sum(gQ * dQ) + np.sum(gB * dB),
+                                lambda e: (gas0, Q0 + e * tQ, B0 + e * tB))):
+            best = None
+            for e in (1e-2, 1e-3, 1e-4, 1e-5):
+                fd = (obj(*pert(e)).item() - obj(*pert(-e)).item()) / (2 * e)
+                rel = abs(fd - ad) / max(abs(fd), abs(ad))
+                best = rel if best is None else min(best, rel)
+            print("torch bridge shell inflow %s: <grad, d> %.8e best rel %.2e" % (name, ad, best), flush=True)
+            assert best <= 1e-5, (name, best)
+    # two calls recorded before one backward: each keeps its own gradient
+    a = gas0.detach().clone().requires_grad_(True)
+    b = (gas0.detach() + 0.5 * td).requires_grad_(True)
+    (obj(a, Q0, B0) + obj(b, Q0, B0)).backward()
+    gb = b.grad.clone()
+    b2 = b.detach().clone().requires_grad_(True)
+    obj(b2, Q0, B0).backward()
+    assert torch.allclose(a.grad, gas0.grad, rtol=1e-12, atol=0.0)
+    assert torch.allclose(gb, b2.grad, rtol=1e-12, atol=0.0)
+    assert not torch.allclose(a.grad, gb, rtol=1e-6, atol=0.0)
+    obj.close()
+    g.close()

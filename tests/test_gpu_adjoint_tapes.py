@@ -1,0 +1,68 @@
+"""The adjoint's memory paths give the gradients of the full-tape path.
+
+Large grids (the 256x256x384 shell) run the reverse sweep with only part
+of an interval's forward intermediates taped (the other steps recompute
+their predictor, face velocities and fluxes inside the adjoint step) and
+without the second tape pair (the interval recompute then runs before the
+interval's adjoint steps, not beside them).  gam_adj_create_budget forces
+those paths on a small grid: the MHD shell with the dipole, Boris, 7UP and
+a time-varying inflow, 7 steps with checkpoints every 3 (intervals of 3,
+3 and 1 steps).  Every path must reproduce the full-tape, overlapped
+gradients (State and oState gas and face fluxes, inflow series) to 1e-12
+relative; only the series gradient's atomic adds may land in another
+order.
+"""
+import numpy as np
+import pytest
+
+from mage_amd import gamhip, problems as pb
+
+pytestmark = pytest.mark.gpu
+
+
+def _case():
+    N = (24, 16, 24)
+    g = gamhip.GamHip(*N, pb.shell_params("7UP"))
+    xyz = pb.shell_corners(*N)
+    g.grid_from_corners(xyz)
+    g.add_dipole_b0(pb.SHELL_M0)
+    wind = pb.varying_wind(24, t0=-60.0, t1=60.0)
+    g.set_wind(wind)
+    xyzcc = g.get_grid(fields=("xyzcc",))["xyzcc"]
+    g.put_state(pb.shell_mhd_state(*N, xyzcc, xyz, p0=0.1))
+    g.init_run()
+    g.run(2)
+    grid = g.get_grid(fields=("xyzcc", "volume"))
+    r = np.sqrt(np.sum(grid["xyzcc"] ** 2, axis=-1))
+    w = np.asfortranarray(grid["volume"] * (r >= 12.0))
+    return g, w, len(wind["tW"])
+
+
+def _grads(g, w, nw, **budget):
+    adj = gamhip.GamAdjoint(g, 7, 3, **budget)
+    info = adj.tape_info()
+    adj.set_weights(w)
+    J = adj.forward()
+    out = list(adj.backward(fluxes=True)) + list(adj.wind_grad(nw))
+    adj.close()
+    return J, out, info
+
+
+def test_adjoint_tape_budgets_match_full_tape():
+    print("adjoint tape budgets: 24x16x24 MHD shell, 7 steps, ckpt 3", flush=True)
+    g, w, nw = _case()
+    J0, ref, info0 = _grads(g, w, nw)
+    assert info0 == (3, True), info0
+    names = ("State gas", "oState gas", "State flux", "oState flux", "series Q", "series B")
+    for a in ref:
+        assert np.max(np.abs(a)) > 0.0
+    for budget, want in (({"overlap": False}, (3, False)), ({"max_taped": 2}, (2, False)),
+                         ({"max_taped": 0, "overlap": False}, (0, False))):
+        print("adjoint tape budget %s" % budget, flush=True)
+        J, got, info = _grads(g, w, nw, **budget)
+        assert info == want, (budget, info)
+        assert J == J0
+        for nm, a, b in zip(names, got, ref):
+            err = np.max(np.abs(a - b)) / np.max(np.abs(b))
+            assert err <= 1e-12, (budget, nm, err)
+    g.close()

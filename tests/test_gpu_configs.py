@@ -1,0 +1,175 @@
+"""The workload configurations at their real sizes, on the GPU.
+
+* 192x192x256 shell (the single-run workload): one step from the same
+  state on the device and in the oracle (<= 1e-12 relative L-infinity),
+  then 200 device steps with |div B| at round-off and finite, positive
+  density and pressure.
+* The 32-member ensemble at its real sizes (48-96 x 48-96 x 64-128 shells,
+  own warp, IC and inflow each): 100 steps in the batched graph against
+  the same members stepped alone, and the smallest member against the
+  oracle for 3 steps.
+* The adjoint workload (96x96x128 shell, 200 steps, checkpoints every 20,
+  pressure integral over 3 <= r <= 6): central differences of frozen-dt
+  replays on a time-budgeted subset of its 64 seeded directions.
+"""
+import time
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+from helpers import rel_linf
+from mage_amd import gamhip, problems as pb
+
+pytestmark = pytest.mark.gpu
+
+
+def _shell(N, seed=pb.SEED0, wind=None, vx=-0.4, bz=-0.05, full_grid=False, p0=1e-3):
+    g = gamhip.GamHip(*N, pb.shell_params("7UP"))
+    xyz = pb.shell_corners(*N, seed=seed)
+    g.grid_from_corners(xyz)
+    g.add_dipole_b0(pb.SHELL_M0)
+    wind = wind or pb.constant_wind(vx=vx, bz=bz)
+    g.set_wind(wind)
+    if full_grid:
+        grid = g.get_grid()
+        grid.update(g.get_background())
+        grid["xyz"] = xyz
+    else:
+        grid = g.get_grid(fields=("xyzcc", "volume"))
+    st = pb.shell_state(*N, grid["xyzcc"], seed=seed, p0=p0)
+    g.put_state(st)
+    return g, grid, st, wind
+
+
+def test_shell_192x192x256_step_and_200_steps():
+    N = (192, 192, 256)
+    t0 = time.time()
+    print("C2 192x192x256: set-up", flush=True)
+    g, grid, st, wind = _shell(N, full_grid=True)
+    o = po.Oracle(*N, pb.shell_params("7UP"), grid, wind=wind)
+    o.set_state(st)
+    del st
+    dtg = g.init_run()
+    o.init_run()
+    assert abs(dtg - o.M.dt) <= 1e-14 * o.M.dt
+    g.step()
+    o.step()
+    out = g.get_state()
+    e = {k: rel_linf(out[k], o.st[k]) for k in ("Gas", "magFlux", "Bxyz", "Efld")}
+    print("C2 one step: rel Linf %s (%.0f s)" % ({k: "%.2e" % v for k, v in e.items()}, time.time() - t0),
+          flush=True)
+    assert max(e.values()) <= 1e-12, e
+    del o, out
+    g.run(199)
+    s = g.get_state()
+    act = (slice(4, 4 + N[0]), slice(4, 4 + N[1]), slice(4, 4 + N[2]))
+    gas = s["Gas"][act]
+    assert np.all(np.isfinite(s["Gas"])) and np.all(np.isfinite(s["magFlux"]))
+    rho = gas[..., 0]
+    ke = 0.5 * np.sum(gas[..., 1:4] ** 2, axis=-1) / rho
+    P = (5.0 / 3.0 - 1.0) * (gas[..., 4] - ke)
+    assert rho.min() > 0.0 and P.min() > 0.0, (rho.min(), P.min())
+    scale = float(np.sum(np.abs(grid["bFlux0"][act])) + np.sum(np.abs(s["magFlux"][act])))
+    divb = g.divb()
+    t, dt, ts = g.get_time()
+    print("C2 200 steps: t %.4f, |div B| %.2e of %.2e, rho min %.3e, P min %.3e"
+          % (t, divb, scale, rho.min(), P.min()), flush=True)
+    assert ts == 200
+    assert divb <= 1e-12 * scale
+    g.close()
+
+
+def _member(spec):
+    g, grid, st, wind = _shell(spec["N"], seed=spec["seed"], vx=spec["vx"], bz=spec["bz"])
+    g.init_run()
+    return g
+
+
+def test_ensemble_32_members_real_sizes():
+    specs = pb.ensemble_members(n=32)
+    print("C3 32 members, %d cells: set-up" % sum(np.prod(s["N"]) for s in specs), flush=True)
+    A = [_member(s) for s in specs]
+    B = [_member(s) for s in specs]
+    for g in A:
+        g.autotune(reps=2)     # members with different launch plans share launches by shape
+    ens = gamhip.GamEnsemble(A)
+    ens.run(100)
+    ens.close()
+    worst = 0.0
+    for m, (a, b) in enumerate(zip(A, B)):
+        b.set_plan(a.get_plan())
+        b.run(100)
+        sa, sb = a.get_state(), b.get_state()
+        e = max(rel_linf(sa[k], sb[k]) for k in ("Gas", "magFlux", "Bxyz"))
+        ta, tb = a.get_time(), b.get_time()
+        assert ta[2] == tb[2] == 100 and abs(ta[0] - tb[0]) <= 1e-12 * tb[0], (m, ta, tb)
+        worst = max(worst, e)
+        a.close()
+        b.close()
+    print("C3 ensemble vs alone after 100 steps: worst rel Linf %.2e" % worst, flush=True)
+    assert worst <= 1e-10
+
+    # the smallest member against the oracle
+    spec = min(specs, key=lambda s: np.prod(s["N"]))
+    g, grid, st, wind = _shell(spec["N"], seed=spec["seed"], vx=spec["vx"], bz=spec["bz"], full_grid=True)
+    o = po.Oracle(*spec["N"], pb.shell_params("7UP"), grid, wind=wind)
+    o.set_state(st)
+    g.init_run()
+    o.init_run()
+    for q in range(3):
+        g.step()
+        o.step()
+        out = g.get_state()
+        e = {k: rel_linf(out[k], o.st[k]) for k in ("Gas", "magFlux", "Bxyz", "Efld")}
+        assert max(e.values()) <= 1e-12, (spec["N"], q, e)
+    print("C3 member %s vs oracle, 3 steps: %s" % (spec["N"], {k: "%.1e" % v for k, v in e.items()}), flush=True)
+    g.close()
+
+
+def test_adjoint_96x96x128_200_steps_directions():
+    N = (96, 96, 128)
+    nw = 200
+    print("C4 96x96x128 adjoint, 200 steps, ckpt 20", flush=True)
+    wind = pb.varying_wind(nw)
+    g, grid, st, _ = _shell(N, wind=wind, p0=0.1)
+    g.init_run()
+    g.run(2)
+    r = np.sqrt(np.sum(grid["xyzcc"] ** 2, axis=-1))
+    w = np.asfortranarray(grid["volume"] * ((r >= 3.0) & (r <= 6.0)))
+    S0 = g.get_state(gamhip.GAM_STATE)["Gas"].copy(order="F")
+    O0 = g.get_state(gamhip.GAM_OSTATE)["Gas"].copy(order="F")
+    adj = gamhip.GamAdjoint(g, 200, 20)
+    adj.set_weights(w)
+    J = adj.forward()
+    gS, gO = adj.backward()
+    assert adj.replay(S0, O0) == pytest.approx(J, rel=1e-14, abs=0.0)
+    rng = np.random.default_rng(31337)
+    x = np.linspace(0, 1, S0.shape[0])[:, None, None]
+    y = np.linspace(0, 1, S0.shape[1])[None, :, None]
+    z = np.linspace(0, 1, S0.shape[2])[None, None, :]
+    t0 = time.time()
+    results = []
+    while len(results) < 8 and time.time() - t0 < 60.0:
+        d = np.zeros(S0.shape, order="F")
+        for v in range(5):
+            k = rng.integers(1, 4, 3)
+            ph = rng.uniform(0, 2 * np.pi, 3)
+            d[..., v] = (np.sin(2 * np.pi * k[0] * x + ph[0]) * np.sin(2 * np.pi * k[1] * y + ph[1])
+                         * np.sin(2 * np.pi * k[2] * z + ph[2]))
+        d *= np.max(np.abs(S0), axis=(0, 1, 2)) * 1e-2
+        ad = float(np.sum(gS * d))
+        gd = float(np.sum(np.abs(gS * d)))
+        best = None
+        for e in (1e-4, 1e-5):
+            fd = (adj.replay(S0 + e * d, O0) - adj.replay(S0 - e * d, O0)) / (2 * e)
+            q = (abs(fd - ad) / max(abs(fd), abs(ad)), abs(fd - ad) / gd)
+            best = q if best is None or q[0] < best[0] else best
+        print("C4 direction %d: <g,d> %.8e rel %.2e of sum|g d| %.2e" % (len(results), ad, best[0], best[1]),
+              flush=True)
+        results.append(best)
+    adj.close()
+    g.close()
+    assert len(results) >= 4
+    # 200 steps of a limiter-switching flow: measured of sum|g d| 1e-7 .. 5e-6
+    assert max(r[1] for r in results) <= 1e-5, results

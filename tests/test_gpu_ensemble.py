@@ -105,3 +105,31 @@ def test_ensemble_of_one_matches_plain_run():
     ens.close()
     x.close()
     y.close()
+
+
+def test_ensemble_refuses_members_changed_after_creation():
+    """A member whose inflow series (or grid, background, plan) changes after
+    the ensemble was recorded makes gam_ens_run refuse, instead of replaying
+    launches that point at freed or stale device memory."""
+    N = (16, 16, 24)
+    gs = []
+    for q in range(2):
+        g = gamhip.GamHip(*N, pb.shell_params("7UP"))
+        g.grid_from_corners(pb.shell_corners(*N, seed=pb.SEED0 + q))
+        g.add_dipole_b0(pb.SHELL_M0)
+        g.set_wind(pb.constant_wind())
+        g.put_state(pb.shell_state(*N, g.get_grid(fields=("xyzcc",))["xyzcc"]))
+        g.init_run()
+        gs.append(g)
+    ens = gamhip.GamEnsemble(gs)
+    ens.run(2)
+    gs[1].set_wind(pb.varying_wind(8))       # a different series length reallocates it
+    with pytest.raises(gamhip.GamError) as ei:
+        ens.run(2)
+    assert ei.value.code == -3    # GAM_E_STATE
+    ens.close()
+    ens = gamhip.GamEnsemble(gs)              # recorded again: runs
+    ens.run(2)
+    ens.close()
+    for g in gs:
+        g.close()

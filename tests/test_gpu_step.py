@@ -213,3 +213,37 @@ def test_calc_dt_matches_and_flags_failure():
         g.calc_dt()
     assert ei.value.code == -4
     g.close()
+
+
+def test_state_transfer_accepts_device_pointers():
+    """gam_put_state / gam_get_state take Fortran arrays in device memory of
+    the context's GPU as well as host arrays (hipMemcpyDefault): a caller on
+    the library's HIP runtime round-trips a state without host copies."""
+    import ctypes as C
+    g, o, grid, st, p = box_case((16, 16, 16), "7UP")
+    g.init_run()
+    hip = C.CDLL("libamdhip64.so.7")          # the runtime libgamhip is linked against
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipFree.argtypes = [C.c_void_p]
+    ref = g.get_state()
+    gas = np.asfortranarray(ref["Gas"] * 1.25)
+    nb = gas.nbytes
+    dptr = C.c_void_p()
+    assert hip.hipMalloc(C.byref(dptr), nb) == 0
+    try:
+        assert hip.hipMemcpy(dptr, gas.ctypes.data, nb, 4) == 0          # hipMemcpyDefault
+        DP = C.POINTER(C.c_double)
+        dp = C.cast(dptr, DP)
+        assert g.L.gam_put_state(g.ctx, gamhip.GAM_OSTATE, dp, None, None, None, C.c_double(0.0)) == 0
+        back = g.get_state(gamhip.GAM_OSTATE)["Gas"]
+        assert np.array_equal(back, gas)
+        assert hip.hipMemcpy(dptr, np.zeros_like(gas).ctypes.data, nb, 4) == 0
+        t = C.c_double()
+        assert g.L.gam_get_state(g.ctx, gamhip.GAM_STATE, dp, None, None, None, C.byref(t)) == 0
+        out = np.zeros_like(gas)
+        assert hip.hipMemcpy(out.ctypes.data, dptr, nb, 4) == 0
+        assert np.array_equal(out, ref["Gas"])
+    finally:
+        hip.hipFree(dptr)
+    g.close()
