@@ -596,7 +596,7 @@ static void enq_advance(gam_ctx *c, int finmode)
         prof_begin(c, KG_FLUX);
         tune_mark(c, TS_FLUX_I + d);
         launch_flux(s, d, c->plan.v[TS_FLUX_I + d], g, c->mp, fb, c->H, S, c->met, c->bg, c->gflx, c->mflx,
-                    c->mp.doMHD ? c->vf + (long)3 * d * N3 : nullptr);
+                    c->vf + (long)3 * d * N3);
         tune_mark(c, TS_FLUX_I + d);
         prof_end(c, KG_FLUX);
     }

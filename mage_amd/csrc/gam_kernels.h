@@ -75,8 +75,8 @@ void launch_vface_rim(hipStream_t s, const Geo &g, const double *H, const double
 void launch_emf(hipStream_t s, int eD, int ax, const Geo &g, const MParm &m, const Box &b, const Clock *clk,
                 const double *H, const double *S, const double *Vf, const double *met,
                 const double *bg, double *E);
-// Interface fluxes of direction d; with vf (MHD) also Mom2Face of the same
-// faces into vf's three planes of direction d
+// Interface fluxes of direction d, and Mom2Face of the same faces into
+// vf's three planes of direction d
 void launch_flux(hipStream_t s, int d, int form, const Geo &g, const MParm &m, const Box &b, const double *H,
                  const double *S, const double *met, const double *bg, double *gf, double *mf, double *vf);
 void launch_update(hipStream_t s, int ax, const Geo &g, const MParm &m, const Box &fb, const Box &ctb,

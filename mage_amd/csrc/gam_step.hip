@@ -558,7 +558,9 @@ struct k_flux {
             for (int n = 0; n < 4; n++) nq[n] = LD(Wl + v * N3, n + 2);
             recon_lr_w<RM>(m.pdmb, rVi, nq, qv, WL[v], WR[v]);
             // Mom2Face of this face from the same velocity stencil
-            if (v >= 1 && v <= 3 && Vf) stp(Vf + (v - 1) * N3, o[4], mom2face(qv, Vi, rVi));
+            // (written without a condition on Vf: a branch here keeps the
+            // stencils alive and costs ~30 VGPRs)
+            if (v >= 1 && v <= 3) stp(Vf + (v - 1) * N3, o[4], mom2face(qv, Vi, rVi));
         }
         auto recon_b = [&](double *BL, double *BR) {
             const double *Bh = H + H_B * N3;
@@ -701,7 +703,7 @@ struct k_flux_s {
             if (st < 5) {
                 recon_lr_w<RM>(m.pdmb, rVi, nq, q, WL[st], WR[st]);
                 // Mom2Face of this face from the same velocity stencil
-                if (st >= 1 && st <= 3 && Vf && act) Vf[(st - 1) * N3 + at(i, a)] = mom2face(q, Vi, rVi);
+                if (st >= 1 && st <= 3 && act) Vf[(st - 1) * N3 + at(i, a)] = mom2face(q, Vi, rVi);
             } else {
                 // cell B is not volume-weighted in H: weight it here (LDS volumes)
 #pragma unroll

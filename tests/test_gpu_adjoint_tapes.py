@@ -38,7 +38,19 @@ def _case():
     return g, w, len(wind["tW"])
 
 
-def _grads(g, w, nw, **budget):
+def _snapshot(g):
+    return g.get_state(gamhip.GAM_STATE), g.get_state(gamhip.GAM_OSTATE), g.get_time()
+
+
+def _restore(g, snap):
+    S, O, (t, dt, ts) = snap
+    g.put_state(S, gamhip.GAM_STATE)
+    g.put_state(O, gamhip.GAM_OSTATE)
+    g.set_time(t, dt, ts)
+
+
+def _grads(g, w, nw, snap, **budget):
+    _restore(g, snap)       # every run starts from the same state
     adj = gamhip.GamAdjoint(g, 7, 3, **budget)
     info = adj.tape_info()
     adj.set_weights(w)
@@ -51,7 +63,8 @@ def _grads(g, w, nw, **budget):
 def test_adjoint_tape_budgets_match_full_tape():
     print("adjoint tape budgets: 24x16x24 MHD shell, 7 steps, ckpt 3", flush=True)
     g, w, nw = _case()
-    J0, ref, info0 = _grads(g, w, nw)
+    snap = _snapshot(g)
+    J0, ref, info0 = _grads(g, w, nw, snap)
     assert info0 == (3, True), info0
     names = ("State gas", "oState gas", "State flux", "oState flux", "series Q", "series B")
     for a in ref:
@@ -59,7 +72,7 @@ def test_adjoint_tape_budgets_match_full_tape():
     for budget, want in (({"overlap": False}, (3, False)), ({"max_taped": 2}, (2, False)),
                          ({"max_taped": 0, "overlap": False}, (0, False))):
         print("adjoint tape budget %s" % budget, flush=True)
-        J, got, info = _grads(g, w, nw, **budget)
+        J, got, info = _grads(g, w, nw, snap, **budget)
         assert info == want, (budget, info)
         assert J == J0
         for nm, a, b in zip(names, got, ref):
