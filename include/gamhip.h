@@ -178,6 +178,25 @@ int gam_sync(gam_ctx *ctx);
  * (DivB, gridutils.F90:717-770). */
 int gam_divb(gam_ctx *ctx, double *totDivB);
 
+/* gam_divb without a context and without any device allocation.  No
+ * counterpart in the reference; it is for callers that cannot allocate
+ * (code under HIP graph capture, tight loops, harnesses that forbid
+ * hipMalloc).  magFlux: (Ni+1,Nj+1,Nk+1,3) face fluxes and bFlux0: NULL or
+ * the background's (Ni,Nj,Nk,3) face fluxes, both DEVICE memory of the
+ * current GPU in the reference's Fortran layout; *totDivB (device memory)
+ * receives the total |div B| over the active cells.  Kernels only, on
+ * `stream` (a hipStream_t; NULL = the null stream), asynchronous.  The
+ * per-block partial sums live in a fixed 16 KiB device array that each
+ * GPU holds for the life of the process; every call overwrites all of it,
+ * so nothing carries over between calls, but calls that overlap on one GPU
+ * (two streams, two host threads) race on it.  The first call on a device
+ * looks up that array's address (not allowed under graph capture); later
+ * calls may be captured.  GAM_E_ARG for NULL arguments, nG outside 1..16,
+ * or more than GAM_DIVB_NOALLOC_MAX_CELLS active cells. */
+#define GAM_DIVB_NOALLOC_MAX_CELLS 2147483647L
+int gam_divb_noalloc(const gam_dims *dims, const double *magFlux, const double *bFlux0,
+                     void *stream, double *totDivB);
+
 /* hipEvent-timed loop of nsteps x stepGamera on the context's stream (as
  * gam_run, graphs included).  ms = elapsed milliseconds between the events. */
 int gam_time_steps(gam_ctx *ctx, int nsteps, double *ms);

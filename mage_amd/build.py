@@ -13,7 +13,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libgamhip.so")
-SOURCES = ["gam_api.hip", "gam_step.hip", "gam_grid.hip", "gam_adj.hip"]
+SOURCES = ["gam_api.hip", "gam_step.hip", "gam_grid.hip", "gam_adj.hip", "gam_noalloc.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

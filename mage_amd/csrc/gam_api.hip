@@ -25,6 +25,13 @@ static int fail(int code, const char *fmt, ...)
     return code;
 }
 
+// The error path of the library's other sources: sets gam_last_error()'s
+// message ("msg: detail") and returns code.
+__attribute__((visibility("hidden"))) int gam_fail_(int code, const char *msg, const char *detail)
+{
+    return fail(code, detail && detail[0] ? "%s: %s" : "%s", msg, detail);
+}
+
 #define HIPCHK(call)                                                                  \
     do {                                                                              \
         hipError_t e_ = (call);                                                       \

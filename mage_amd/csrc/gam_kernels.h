@@ -67,6 +67,9 @@ enum {
     FIN_INIT = 16      // oState%time = State%time - dt (init.F90:238-244)
 };
 
+// Sets the message gam_last_error() returns ("msg: detail") and returns code.
+int gam_fail_(int code, const char *msg, const char *detail);
+
 void launch_predict(hipStream_t s, int ax, const Geo &g, const MParm &m, const Box &fb, const Clock *clk,
                     const double *S, const double *O, const double *met, double *H);
 // Mom2Face on the faces the EMF stencils read past the interface-flux faces

@@ -34,6 +34,7 @@ EXPORTS = (
     "gam_kernel_times", "gam_kernel_group_name", "gam_get_solver",
     "gam_autotune", "gam_get_plan", "gam_set_plan", "gam_tune_slot_name", "gam_tune_candidates",
     "gam_ens_create", "gam_ens_run", "gam_ens_time_steps", "gam_ens_launches_per_step", "gam_ens_destroy",
+    "gam_divb_noalloc",
 )
 
 DP = C.POINTER(C.c_double)

@@ -115,3 +115,20 @@ def test_adjoint_header_symbols_exported():
     assert L.gam_adj_backward(None, None, None, None, None) == -1
     assert L.gam_adj_replay(None, None, None, None, None, C.byref(J)) == -1
     assert L.gam_adj_destroy(None) == 0
+
+
+def test_divb_noalloc_argument_checks():
+    """gam_divb_noalloc refuses NULL arguments, bad ghost widths and grids
+    beyond its maximum before touching a GPU."""
+    L = gamhip.load()
+    DP = C.POINTER(C.c_double)
+    L.gam_divb_noalloc.argtypes = [C.POINTER(gamhip.Dims), DP, DP, C.c_void_p, DP]
+    x = (C.c_double * 1)()
+    px = C.cast(x, DP)
+    d = gamhip.Dims(16, 16, 16, 4)
+    assert L.gam_divb_noalloc(None, px, None, None, px) == -1
+    assert L.gam_divb_noalloc(C.byref(d), None, None, None, px) == -1
+    assert L.gam_divb_noalloc(C.byref(d), px, None, None, None) == -1
+    assert L.gam_divb_noalloc(C.byref(gamhip.Dims(16, 16, 16, 0)), px, None, None, px) == -1
+    assert L.gam_divb_noalloc(C.byref(gamhip.Dims(2048, 2048, 1024, 4)), px, None, None, px) == -1
+    assert "maximum" in L.gam_last_error().decode()
