@@ -80,7 +80,7 @@ def _summary_rows(path, prefix, col):
         cols = [c.strip() for c in line.strip().strip("|").split("|")]
         if len(cols) > col and cols[0].startswith(prefix):
             try:
-                if int(cols[1]) >= 10:
+                if int(cols[1]) >= 50:
                     vals.append(float(cols[col]))
             except ValueError:
                 pass
