@@ -46,11 +46,12 @@ void set_launch_recorder(std::vector<EnsLaunch> *r);
 
 // Launch plan: the shape of every tunable launch of the step.  Values
 // 1, 2, 3 are the Tile row orders (ax1: j fastest, k fastest, j-slabs);
-// PLAN_LDS selects the LDS-staged form of the j/k interface fluxes (a
-// 64 x 4 face tile with its stencil halo rows in LDS).
+// PLAN_LDS and PLAN_LDS8 select the LDS-staged form of the j/k interface
+// fluxes (a 64 x 4 or 64 x 8 face tile with its stencil halo rows in LDS).
 enum { TS_PREDICT = 0, TS_EMF_I, TS_EMF_J, TS_EMF_K, TS_FLUX_I, TS_FLUX_J, TS_FLUX_K, TS_UPDATE,
        TS_COUNT };
 #define PLAN_LDS 4
+#define PLAN_LDS8 5
 struct Plan {
     int v[TS_COUNT];
 };

@@ -19,6 +19,7 @@
 // buffer and writes every point of the other.
 #include "gam_phys.h"
 #include "gam_kernels.h"
+#include <type_traits>
 
 #define BX 64
 #define BY 4
@@ -121,8 +122,12 @@ static inline dim3 grid_of(const Box &b)
 // own launch) and each with its own argument record.  A table after the
 // header names the member of every run of 8 workgroups.
 // ====================================================================
+// Threads per workgroup of functor F: F::NT where it declares one, else 256.
+template <class F, class = void> struct nthreads { static constexpr int v = BX * BY; };
+template <class F> struct nthreads<F, std::void_t<decltype(F::NT)>> { static constexpr int v = F::NT; };
+
 template <class F, class... A>
-__global__ __launch_bounds__(256) void k_one(A... a)
+__global__ __launch_bounds__(nthreads<F>::v) void k_one(A... a)
 {
     F{}(make_uint3(blockIdx.x, blockIdx.y, blockIdx.z), a...);
 }
@@ -157,7 +162,7 @@ __device__ __forceinline__ void pcall(const Pack<A0, A...> &p, uint3 bi, const B
 }
 
 template <class F, class... A>
-__global__ __launch_bounds__(256) void k_ens(const EnsHdr *__restrict__ h,
+__global__ __launch_bounds__(nthreads<F>::v) void k_ens(const EnsHdr *__restrict__ h,
                                              const Pack<A...> *__restrict__ rec)
 {
     const unsigned w = blockIdx.x;
@@ -584,25 +589,27 @@ struct k_flux {
 
 // ====================================================================
 // Interface fluxes along j or k (D = 1, 2) with the stencil planes staged
-// in LDS.  A workgroup owns 64 (i) x 4 (along D) faces; the rows of the
-// 8-point stencil (faces' cells a-4 .. a+3, 11 rows for the 4 face rows)
-// and of the 4-point limiter stencil (a-2 .. a+1, 7 rows) of one variable
-// at a time go to LDS with coalesced loads, double-buffered so the next
-// variable's rows are in flight while the current one is reconstructed.
-// Each row is loaded once per workgroup instead of once per stencil point.
+// in LDS.  A workgroup owns 64 (i) x FH (along D) faces, FH = 4 or 8 (a
+// launch-plan choice: the taller tile loads FH + 7 rows for FH face rows,
+// fewer halo rows per face, with twice the waves per barrier); the rows
+// of the 8-point stencil (faces' cells a-4 .. a+FH+2) and of the 4-point
+// limiter stencil (a-2 .. a+FH) of one variable at a time go to LDS with
+// coalesced loads, double-buffered so the next variable's rows are in
+// flight while the current one is reconstructed.  Each row is loaded once
+// per workgroup instead of once per stencil point.
 // ====================================================================
-#define SR8 (BY + 7)          // rows of an 8-point stage
-#define SR4 (BY + 3)          // rows of a 4-point stage
-#define SRW (SR8 + SR4)       // rows of one variable's stage
-#define SLD ((SRW * BX + BX * BY - 1) / (BX * BY))   // loads per thread per stage
-
 struct FluxTile {
     int i0, i1, a0, a1, b0, b1;   // faces along i, along D, along the third axis
     int nti, nta, nb;
 };
 
-template <int D, int RM, bool BG, bool BORIS>
+template <int D, int RM, bool BG, bool BORIS, int FH = BY>
 struct k_flux_s {
+    static constexpr int NT = BX * FH;                         // threads
+    static constexpr int SR8 = FH + 7;                         // rows of an 8-point stage
+    static constexpr int SR4 = FH + 3;                         // rows of a 4-point stage
+    static constexpr int SRW = SR8 + SR4;                      // rows of one variable's stage
+    static constexpr int SLD = (SRW * BX + NT - 1) / NT;       // loads per thread per stage
     __device__ __forceinline__ void operator()(const uint3 bi, Geo g, MParm m,
                                                FluxTile t, const double *__restrict__ H,
                                                const double *__restrict__ S,
@@ -623,7 +630,7 @@ struct k_flux_s {
         const int ta = r % t.nta;
         const int bb = t.b0 + r / t.nta;
         const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + BX * ty;
-        const int i0 = t.i0 + ti * BX, a0 = t.a0 + ta * BY;
+        const int i0 = t.i0 + ti * BX, a0 = t.a0 + ta * FH;
         const int i = i0 + tx, a = a0 + ty;
         const bool act = i <= t.i1 && a <= t.a1;
         const long N3 = g.N3;
@@ -638,7 +645,7 @@ struct k_flux_s {
         auto fetch = [&](const double *p8, const double *p4, double *reg) {
 #pragma unroll
             for (int q = 0; q < SLD; q++) {
-                const int e = tid + q * BX * BY;
+                const int e = tid + q * NT;
                 const int row = e / BX, col = e % BX;
                 const int ii = i0 + col;
                 double v = 0.0;
@@ -653,14 +660,14 @@ struct k_flux_s {
         auto store = [&](int bsel, const double *reg) {
 #pragma unroll
             for (int q = 0; q < SLD; q++) {
-                const int e = tid + q * BX * BY;
+                const int e = tid + q * NT;
                 if (e < SRW * BX) buf[bsel][e / BX][e % BX] = reg[q];
             }
         };
         // volumes (8-point rows) and the first stage
         {
             const double *pv = met + M_VOL * N3;
-            for (int e = tid; e < SR8 * BX; e += BX * BY) {
+            for (int e = tid; e < SR8 * BX; e += NT) {
                 const int row = e / BX, col = e % BX;
                 const int ii = i0 + col, aa = a0 - 4 + row;
                 vs[row][col] = (ii <= t.i1 && aa <= aTop) ? pv[at(ii, aa)] : 0.0;
@@ -1404,7 +1411,10 @@ int plan_candidates(int slot, int *vals)
     if (slot < 0 || slot >= TS_COUNT) return 0;
     const int maxax = (slot == TS_FLUX_I) ? 2 : 3;
     for (int a = 1; a <= maxax; a++) vals[n++] = a;
-    if (slot == TS_FLUX_J || slot == TS_FLUX_K) vals[n++] = PLAN_LDS;
+    if (slot == TS_FLUX_J || slot == TS_FLUX_K) {
+        vals[n++] = PLAN_LDS;
+        vals[n++] = PLAN_LDS8;
+    }
     return n;
 }
 
@@ -1476,21 +1486,25 @@ static void flux_t(hipStream_t s, int form, const Geo &g, const MParm &m, const 
                    const double *S, const double *met, const double *bg, double *gf, double *mf, double *vf)
 {
     if constexpr (D != 0) {
-        if (form == PLAN_LDS) {
+        if (form == PLAN_LDS || form == PLAN_LDS8) {
+            const int fh = form == PLAN_LDS ? BY : 8;
             FluxTile t;
             t.i0 = b.i0;
             t.i1 = b.i1;
             if (D == 1) { t.a0 = b.j0; t.a1 = b.j1; t.b0 = b.k0; t.b1 = b.k1; }
             else { t.a0 = b.k0; t.a1 = b.k1; t.b0 = b.j0; t.b1 = b.j1; }
             t.nti = (t.i1 - t.i0 + BX) / BX;
-            t.nta = (t.a1 - t.a0 + BY) / BY;
+            t.nta = (t.a1 - t.a0 + fh) / fh;
             t.nb = t.nti * t.nta * (t.b1 - t.b0 + 1);
-            launch_f<k_flux_s<D, RM, BG, BO>>(dim3((unsigned)(((t.nb + 7) / 8) * 8)), dim3(BX, BY), s, g, m,
-                                              t, H, S, met, bg, gf, mf, vf);
+            const dim3 grid((unsigned)(((t.nb + 7) / 8) * 8));
+            if (fh == 8)
+                launch_f<k_flux_s<D, RM, BG, BO, 8>>(grid, dim3(BX, 8), s, g, m, t, H, S, met, bg, gf, mf, vf);
+            else
+                launch_f<k_flux_s<D, RM, BG, BO>>(grid, dim3(BX, BY), s, g, m, t, H, S, met, bg, gf, mf, vf);
             return;
         }
     }
-    const Tile t = tile_of(b, form == PLAN_LDS ? 1 : form);
+    const Tile t = tile_of(b, form >= PLAN_LDS ? 1 : form);
     launch_f<k_flux<D, RM, BG, BO>>(tgrid(t), dim3(BX, BY), s, g, m, t, H, S, met, bg, gf, mf, vf);
 }
 
