@@ -603,7 +603,7 @@ struct FluxTile {
     int nti, nta, nb;
 };
 
-template <int D, int RM, bool BG, bool BORIS, int FH = BY>
+template <int D, int RM, bool BG, bool BORIS, int FH = BY, bool DMA = false>
 struct k_flux_s {
     static constexpr int NT = BX * FH;                         // threads
     static constexpr int SR8 = FH + 7;                         // rows of an 8-point stage
@@ -664,6 +664,26 @@ struct k_flux_s {
                 if (e < SRW * BX) buf[bsel][e / BX][e % BX] = reg[q];
             }
         };
+        // DMA form: the stage's rows go straight from global memory to LDS
+        // (global_load_lds, 16 B per lane): one instruction moves two rows,
+        // lanes 0-31 the even row and 32-63 the odd one, into consecutive
+        // LDS rows.  Columns past t.i1 read the tile's first pair and rows
+        // past aTop the row aTop instead (valid cells; only the lanes of
+        // faces outside the box read those LDS entries).
+        auto dma = [&](const double *p8, const double *p4, int bsel) {
+            const int wv = tid >> 6, ln = tid & 63;
+            for (int q = wv; q < SRW / 2; q += NT / 64) {
+                const int row = 2 * q + (ln >> 5);
+                int ii = i0 + 2 * (ln & 31);
+                if (ii > t.i1) ii = i0;
+                int aa = row < SR8 ? a0 - 4 + row : a0 - 2 + (row - SR8);
+                if (aa > aTop) aa = aTop;
+                const double *src = (row < SR8 ? p8 : p4) + at(ii, aa);
+                __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)src,
+                                                 (__attribute__((address_space(3))) void *)&buf[bsel][2 * q][0],
+                                                 16, 0, 0);
+            }
+        };
         // volumes (8-point rows) and the first stage
         {
             const double *pv = met + M_VOL * N3;
@@ -680,12 +700,17 @@ struct k_flux_s {
             if (st < 5) { p8 = Wv + st * N3; p4 = Wl + st * N3; }
             else { p8 = Bh + (st - 5) * N3; p4 = Bl + (st - 5) * N3; }
         };
-        double reg[SLD];
+        double reg[DMA ? 1 : SLD];
         {
             const double *p8, *p4;
             planes(0, p8, p4);
-            fetch(p8, p4, reg);
-            store(0, reg);
+            if (DMA) {
+                dma(p8, p4, 0);
+                __builtin_amdgcn_s_waitcnt(0);
+            } else {
+                fetch(p8, p4, reg);
+                store(0, reg);
+            }
         }
         __syncthreads();
         const double Vi = wsum8([&](int n) { return vs[ty + n][tx]; });
@@ -697,16 +722,29 @@ struct k_flux_s {
         for (int st = 0; st < 8; st++) {
             if (st >= nst) break;
             const int cur = st & 1;
-            if (st + 1 < nst) {
+            if (!DMA && st + 1 < nst) {
                 const double *p8, *p4;
                 planes(st + 1, p8, p4);
                 fetch(p8, p4, reg);
             }
-            double q[8], nq[4];
+            double q[8], nq[4], vq[8];
 #pragma unroll
             for (int n = 0; n < 8; n++) q[n] = buf[cur][ty + n][tx];
 #pragma unroll
             for (int n = 0; n < 4; n++) nq[n] = buf[cur][SR8 + ty + n][tx];
+            if (DMA) {
+                // this stage's LDS reads come first, then the next stage's
+                // DMA, so that no LDS read waits behind the DMA in flight
+                if (st >= 5) {
+#pragma unroll
+                    for (int n = 0; n < 8; n++) vq[n] = vs[ty + n][tx];
+                }
+                if (st + 1 < nst) {
+                    const double *p8, *p4;
+                    planes(st + 1, p8, p4);
+                    dma(p8, p4, cur ^ 1);
+                }
+            }
             if (st < 5) {
                 recon_lr_w<RM>(m.pdmb, rVi, nq, q, WL[st], WR[st]);
                 // Mom2Face of this face from the same velocity stencil
@@ -714,10 +752,11 @@ struct k_flux_s {
             } else {
                 // cell B is not volume-weighted in H: weight it here (LDS volumes)
 #pragma unroll
-                for (int n = 0; n < 8; n++) q[n] = vs[ty + n][tx] * q[n];
+                for (int n = 0; n < 8; n++) q[n] = (DMA ? vq[n] : vs[ty + n][tx]) * q[n];
                 recon_lr_w<RM>(m.pdmb, rVi, nq, q, BL[st - 5], BR[st - 5]);
             }
-            if (st + 1 < nst) store(cur ^ 1, reg);
+            if (DMA) __builtin_amdgcn_s_waitcnt(0);
+            else if (st + 1 < nst) store(cur ^ 1, reg);
             __syncthreads();
         }
         if (act) {
@@ -1399,8 +1438,8 @@ Plan default_plan()
     p.v[TS_EMF_J] = 2;    // along i and k
     p.v[TS_EMF_K] = 1;    // along i and j
     p.v[TS_FLUX_I] = 1;
-    p.v[TS_FLUX_J] = PLAN_LDS;
-    p.v[TS_FLUX_K] = PLAN_LDS;
+    p.v[TS_FLUX_J] = PLAN_LDSDMA;
+    p.v[TS_FLUX_K] = PLAN_LDSDMA;
     p.v[TS_UPDATE] = 3;
     return p;
 }
@@ -1414,6 +1453,7 @@ int plan_candidates(int slot, int *vals)
     if (slot == TS_FLUX_J || slot == TS_FLUX_K) {
         vals[n++] = PLAN_LDS;
         vals[n++] = PLAN_LDS8;
+        vals[n++] = PLAN_LDSDMA;
     }
     return n;
 }
@@ -1486,8 +1526,8 @@ static void flux_t(hipStream_t s, int form, const Geo &g, const MParm &m, const 
                    const double *S, const double *met, const double *bg, double *gf, double *mf, double *vf)
 {
     if constexpr (D != 0) {
-        if (form == PLAN_LDS || form == PLAN_LDS8) {
-            const int fh = form == PLAN_LDS ? BY : 8;
+        if (form == PLAN_LDS || form == PLAN_LDS8 || form == PLAN_LDSDMA) {
+            const int fh = form == PLAN_LDS8 ? 8 : BY;
             FluxTile t;
             t.i0 = b.i0;
             t.i1 = b.i1;
@@ -1499,6 +1539,9 @@ static void flux_t(hipStream_t s, int form, const Geo &g, const MParm &m, const 
             const dim3 grid((unsigned)(((t.nb + 7) / 8) * 8));
             if (fh == 8)
                 launch_f<k_flux_s<D, RM, BG, BO, 8>>(grid, dim3(BX, 8), s, g, m, t, H, S, met, bg, gf, mf, vf);
+            else if (form == PLAN_LDSDMA)
+                launch_f<k_flux_s<D, RM, BG, BO, BY, true>>(grid, dim3(BX, BY), s, g, m, t, H, S, met, bg, gf, mf,
+                                                            vf);
             else
                 launch_f<k_flux_s<D, RM, BG, BO>>(grid, dim3(BX, BY), s, g, m, t, H, S, met, bg, gf, mf, vf);
             return;
