@@ -164,9 +164,13 @@ int gam_step(gam_ctx *ctx, double *dt_next);
  * the first CalcDT failure code seen, if any.  Pairs of steps are replayed
  * as one hipGraph of two steps, captured on first use for each State
  * buffer (a step is kernels only, so a replay launches exactly what
- * gam_step would: the results are bit-identical).  Changing the grid,
- * background, inflow series or launch plan drops the captured graphs;
- * with profiling on, steps are launched one by one. */
+ * gam_step would: the results are bit-identical).  Inside the graph the
+ * j and k interface fluxes and the EMFs that take their face velocities
+ * are branches beside the i-direction ones, which the device may run
+ * side by side (GAM_CONCURRENT=0 in the environment at gam_create keeps
+ * the graph a single chain).  Changing the grid, background, inflow
+ * series or launch plan drops the captured graphs; with profiling on,
+ * steps are launched one by one. */
 int gam_run(gam_ctx *ctx, int nsteps);
 /* gamAdvanceModel (gamtypessub.F90:74-90): one stepGamera, then more while
  * Model%t < t0 + dt.  *nsteps (may be NULL) receives the step count.

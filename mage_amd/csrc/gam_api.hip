@@ -77,6 +77,13 @@ struct gam_ctx {
     // two steps captured as one graph, from State buffer 0 / 1 (null: not
     // captured yet or dropped because a launch argument changed)
     hipGraphExec_t gx[2];
+    // Fork/join of the independent flux and EMF launches inside captured
+    // step graphs: two more streams (default priority) and the events that
+    // order them; fork = 1 while a capture may branch.
+    int concurrent;          // branch the captured graphs (default 1)
+    int fork;
+    hipStream_t aux[2];
+    hipEvent_t fev[6];
 };
 
 static void graphs_drop(gam_ctx *c)
@@ -198,6 +205,12 @@ extern "C" int gam_create(gam_ctx **out, const gam_dims *d, const gam_model *m)
         if (e == hipSuccess) e = hipMemset((ptr), 0, (size_t)(nplanes) * pl); \
     }
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    for (int q = 0; e == hipSuccess && q < 2; q++) e = hipStreamCreateWithFlags(&c->aux[q], hipStreamNonBlocking);
+    for (int q = 0; e == hipSuccess && q < 6; q++) e = hipEventCreateWithFlags(&c->fev[q], hipEventDisableTiming);
+    {
+        const char *cv = getenv("GAM_CONCURRENT");
+        c->concurrent = (cv && cv[0] == '0') ? 0 : 1;
+    }
     ALLOC(c->st[0], S_NPL);
     ALLOC(c->st[1], S_NPL);
     ALLOC(c->H, H_NPL);
@@ -247,6 +260,8 @@ extern "C" int gam_destroy(gam_ctx *c)
     if (c->clk) hipFree(c->clk);
     for (int q = 0; q < (int)(sizeof(c->ev) / sizeof(c->ev[0])); q++) if (c->ev[q]) hipEventDestroy(c->ev[q]);
     for (int q = 0; q < 2 * GAM_TUNE_MAXREPS; q++) if (c->tev[q]) hipEventDestroy(c->tev[q]);
+    for (int q = 0; q < 2; q++) if (c->aux[q]) hipStreamDestroy(c->aux[q]);
+    for (int q = 0; q < 6; q++) if (c->fev[q]) hipEventDestroy(c->fev[q]);
     if (c->stream) hipStreamDestroy(c->stream);
     c->pend.~vector<int>();
     free(c);
@@ -596,13 +611,26 @@ static void enq_advance(gam_ctx *c, int finmode)
     prof_end(c, KG_PREDICT);
 
     // interface fluxes (CalcStress), with Mom2Face of the same faces; the
-    // EMFs do not depend on the fluxes, so they run after them
+    // EMFs do not depend on the fluxes, so they run after them.  Inside a
+    // captured graph (fork) the j and k fluxes and the EMFs that take their
+    // face velocities run on two more streams: the graph then has three
+    // branches between the predictor and the update, which the device may
+    // overlap.  The launches and their arguments are the same either way.
+    const bool fork = c->fork && c->mp.doMHD;
+    hipStream_t fs[3] = {s, s, s};   // stream of the flux of direction d and of its EMF
+    if (fork) {
+        hipEventRecord(c->fev[0], s);
+        for (int q = 0; q < 2; q++) {
+            hipStreamWaitEvent(c->aux[q], c->fev[0], 0);
+            fs[1 + q] = c->aux[q];
+        }
+    }
     for (int d = 0; d < 3; d++) {
         Box fb = act_box(g);
         if (d == 0) fb.i1 += 1; else if (d == 1) fb.j1 += 1; else fb.k1 += 1;
         prof_begin(c, KG_FLUX);
         tune_mark(c, TS_FLUX_I + d);
-        launch_flux(s, d, c->plan.v[TS_FLUX_I + d], g, c->mp, fb, c->H, S, c->met, c->bg, c->gflx, c->mflx,
+        launch_flux(fs[d], d, c->plan.v[TS_FLUX_I + d], g, c->mp, fb, c->H, S, c->met, c->bg, c->gflx, c->mflx,
                     c->vf + (long)3 * d * N3);
         tune_mark(c, TS_FLUX_I + d);
         prof_end(c, KG_FLUX);
@@ -615,16 +643,25 @@ static void enq_advance(gam_ctx *c, int finmode)
         prof_begin(c, KG_MOM2FACE);
         launch_vface_rim(s, g, c->H, c->met, c->vf);
         prof_end(c, KG_MOM2FACE);
+        if (fork) {
+            hipEventRecord(c->fev[1], s);
+            for (int q = 0; q < 2; q++) hipStreamWaitEvent(c->aux[q], c->fev[1], 0);
+        }
         for (int eD = 0; eD < 3; eD++) {
             // edge direction eD takes the face velocities along dT1
             const int dT1 = (eD == 0) ? 1 : (eD == 1) ? 2 : 0;
             prof_begin(c, KG_EMF);
             tune_mark(c, TS_EMF_I + eD);
-            launch_emf(s, eD, c->plan.v[TS_EMF_I + eD], g, c->mp, eb[eD], c->clk, c->H, S,
+            launch_emf(fs[dT1], eD, c->plan.v[TS_EMF_I + eD], g, c->mp, eb[eD], c->clk, c->H, S,
                        c->vf + (long)3 * dT1 * N3, c->met, c->bg, Sn + S_E * N3);
             tune_mark(c, TS_EMF_I + eD);
             prof_end(c, KG_EMF);
         }
+        if (fork)
+            for (int q = 0; q < 2; q++) {
+                hipEventRecord(c->fev[2 + q], c->aux[q]);
+                hipStreamWaitEvent(s, c->fev[2 + q], 0);
+            }
     }
 
     prof_begin(c, KG_UPDATE);
@@ -664,8 +701,10 @@ static int enq_steps(gam_ctx *c, int nsteps)
         if (!c->gx[p]) {
             hipGraph_t gr = nullptr;
             HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            c->fork = c->concurrent;
             enq_step(c);
             enq_step(c);
+            c->fork = 0;
             hipError_t e = hipStreamEndCapture(c->stream, &gr);
             c->cur = p;
             if (e == hipSuccess) e = hipGraphInstantiate(&c->gx[p], gr, nullptr, nullptr, 0);
