@@ -76,7 +76,13 @@ int gam_adj_forward(gam_adj *a, double *J);
  * the face fluxes: each start state's ghost cells follow from its interior
  * through the boundary conditions and its cell B from its face fluxes
  * (bFlux2Fld), so those adjoints are folded in (the ghost entries are 0).
- * The context is left at the final state of the forward run. */
+ * The start is taken in gam_adj_replay's order (active cell B, the
+ * boundary pass, every cell's B), the form stepGamera leaves State in
+ * after each step.  The reference's set-up (and gam_init_run) forms ghost
+ * B by the boundary rules instead (init.F90:200-244); gGas does not depend
+ * on the difference, gMF does: for gMF to be the gradient of the run,
+ * start it from that form (gam_init_run, then gam_bflux2fld of State and
+ * of oState).  The context is left at the final state of the forward run. */
 int gam_adj_backward(gam_adj *a, double *gGas, double *gGasO, double *gMF, double *gMFO);
 /* Forward from the recorded start with the given State Gas (and oState
  * Gas, State and oState magFlux; NULL keeps the recorded array), ghost

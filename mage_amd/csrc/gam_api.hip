@@ -1782,14 +1782,22 @@ extern "C" int gam_adj_backward(gam_adj *a, double *gGas, double *gGasO, double 
     a->lastTaped = (a->nck == 1);
     c->prof = prof;
     // the start states' ghost cells follow from their interiors through the
-    // boundary conditions, and their cell B from the face fluxes: fold those
-    // adjoints into the interior gas and face fluxes
+    // boundary conditions, and their cell B from the face fluxes (in the
+    // order of gam_adj_replay: active cell B, the boundary pass, every
+    // cell's B): fold those adjoints into the interior gas and face fluxes.
+    // The boundary pass reads active cells' B (the inflow's fast-speed fix,
+    // the sphere walls' copies), so its transposes leave adjoints on active
+    // cell B, which a second cell-B transpose folds into the face fluxes.
     if (c->mp.doMHD) {
         launch_cellb_adj(s, g, 1, lA, c->met);
         launch_cellb_adj(s, g, 1, lB, c->met);
     }
     adj_bcs(a, lA, a->ck, a->clkh);
     adj_bcs(a, lB, a->ck + (size_t)CK_NPL * N3, a->oclk);
+    if (c->mp.doMHD) {
+        launch_cellb_adj(s, g, 1, lA, c->met);
+        launch_cellb_adj(s, g, 1, lB, c->met);
+    }
     HIPCHK(hipEventRecord(a->ev[3], s));
     HIPCHK(hipEventSynchronize(a->ev[3]));
     HIPCHK(hipEventElapsedTime(&a->bwd_ms, a->ev[2], a->ev[3]));

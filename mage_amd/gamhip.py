@@ -477,7 +477,10 @@ class GamAdjoint:
 
     def backward(self, fluxes=False):
         """Gradients with respect to the starting State and oState gas, and
-        with fluxes=True also their magFlux: (gS, gO) or (gS, gO, gMF, gMFO)."""
+        with fluxes=True also their magFlux: (gS, gO) or (gS, gO, gMF, gMFO).
+        gMF is the gradient of a run whose start has every cell's B formed
+        from the face fluxes after the boundary pass (after init_run, call
+        bflux2fld on GAM_STATE and GAM_OSTATE)."""
         shp = self.g.cshape[:3] + (5,)
         gS = np.zeros(shp, order="F")
         gO = np.zeros(shp, order="F")

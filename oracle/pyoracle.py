@@ -212,6 +212,13 @@ class Oracle:
     def enforce_bcs(self):
         self.L.orc_enforce_bcs(C.byref(self.D), C.byref(self.M), C.byref(self.G), C.byref(self.S))
 
+    def recompute_cell_b(self):
+        """Cell B of State and oState from their own face fluxes over every
+        cell, ghosts included (bFlux2Fld after the boundary pass)."""
+        for st in (self.st, self.ost):
+            self.L.orc_bflux2fld(C.byref(self.D), C.byref(self.M), C.byref(self.G), ptr(st["magFlux"]),
+                                 ptr(st["Bxyz"]))
+
     def divb(self):
         return self.L.orc_divb(C.byref(self.D), C.byref(self.M), C.byref(self.G), C.byref(self.S), DP())
 
