@@ -226,8 +226,8 @@ int gam_get_solver(gam_ctx *ctx, int which, double *out);
  * launch shapes that give the same results: the row order of the
  * 256-point chunk tiling (1 = j fastest, 2 = k fastest, 3 = slabs of 8
  * j-rows) and, for the j and k fluxes, the LDS-staged face tile of
- * 64 x 4 (4) or 64 x 8 (5) faces, or the 64 x 4 tile whose rows reach LDS
- * by direct global-to-LDS loads (6).
+ * 64 x 4 (4) or 64 x 8 (5) faces, or those tiles with their rows brought
+ * to LDS by direct global-to-LDS loads (6: 64 x 4, 7: 64 x 8).
  * gam_autotune times every candidate of every slot on the context's own
  * grid (reps steps per candidate, hipEvents around that slot's launch),
  * keeps the fastest (the current shape unless another is more than 1 %
@@ -240,7 +240,7 @@ int gam_autotune(gam_ctx *ctx, int reps);             /* 1 <= reps <= GAM_TUNE_M
 int gam_get_plan(gam_ctx *ctx, int *plan);            /* GAM_TUNE_SLOTS values */
 int gam_set_plan(gam_ctx *ctx, const int *plan);      /* GAM_E_ARG if a value is not a candidate */
 const char *gam_tune_slot_name(int slot);              /* "" outside 0..GAM_TUNE_SLOTS-1 */
-/* Candidate values of a slot (up to 6); returns their number, 0 for a bad slot. */
+/* Candidate values of a slot (up to 7); returns their number, 0 for a bad slot. */
 int gam_tune_candidates(int slot, int *values);
 
 /* ---------------------------------------------------------------------

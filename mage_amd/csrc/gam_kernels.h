@@ -48,12 +48,14 @@ void set_launch_recorder(std::vector<EnsLaunch> *r);
 // 1, 2, 3 are the Tile row orders (ax1: j fastest, k fastest, j-slabs);
 // PLAN_LDS and PLAN_LDS8 select the LDS-staged form of the j/k interface
 // fluxes (a 64 x 4 or 64 x 8 face tile with its stencil halo rows in LDS);
-// PLAN_LDSDMA the 64 x 4 tile whose rows reach LDS by global_load_lds.
+// PLAN_LDSDMA and PLAN_LDS8DMA the 64 x 4 and 64 x 8 tiles whose rows reach
+// LDS by global_load_lds.
 enum { TS_PREDICT = 0, TS_EMF_I, TS_EMF_J, TS_EMF_K, TS_FLUX_I, TS_FLUX_J, TS_FLUX_K, TS_UPDATE,
        TS_COUNT };
 #define PLAN_LDS 4
 #define PLAN_LDS8 5
 #define PLAN_LDSDMA 6
+#define PLAN_LDS8DMA 7
 struct Plan {
     int v[TS_COUNT];
 };

@@ -1454,6 +1454,7 @@ int plan_candidates(int slot, int *vals)
         vals[n++] = PLAN_LDS;
         vals[n++] = PLAN_LDS8;
         vals[n++] = PLAN_LDSDMA;
+        vals[n++] = PLAN_LDS8DMA;
     }
     return n;
 }
@@ -1526,8 +1527,8 @@ static void flux_t(hipStream_t s, int form, const Geo &g, const MParm &m, const 
                    const double *S, const double *met, const double *bg, double *gf, double *mf, double *vf)
 {
     if constexpr (D != 0) {
-        if (form == PLAN_LDS || form == PLAN_LDS8 || form == PLAN_LDSDMA) {
-            const int fh = form == PLAN_LDS8 ? 8 : BY;
+        if (form >= PLAN_LDS && form <= PLAN_LDS8DMA) {
+            const int fh = (form == PLAN_LDS8 || form == PLAN_LDS8DMA) ? 8 : BY;
             FluxTile t;
             t.i0 = b.i0;
             t.i1 = b.i1;
@@ -1537,7 +1538,9 @@ static void flux_t(hipStream_t s, int form, const Geo &g, const MParm &m, const 
             t.nta = (t.a1 - t.a0 + fh) / fh;
             t.nb = t.nti * t.nta * (t.b1 - t.b0 + 1);
             const dim3 grid((unsigned)(((t.nb + 7) / 8) * 8));
-            if (fh == 8)
+            if (form == PLAN_LDS8DMA)
+                launch_f<k_flux_s<D, RM, BG, BO, 8, true>>(grid, dim3(BX, 8), s, g, m, t, H, S, met, bg, gf, mf, vf);
+            else if (fh == 8)
                 launch_f<k_flux_s<D, RM, BG, BO, 8>>(grid, dim3(BX, 8), s, g, m, t, H, S, met, bg, gf, mf, vf);
             else if (form == PLAN_LDSDMA)
                 launch_f<k_flux_s<D, RM, BG, BO, BY, true>>(grid, dim3(BX, BY), s, g, m, t, H, S, met, bg, gf, mf,

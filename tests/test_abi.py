@@ -89,7 +89,7 @@ def test_plan_entry_points_without_gpu():
     v = (C.c_int * 8)()
     assert L.gam_tune_candidates(0, v) == 3 and list(v[:3]) == [1, 2, 3]
     assert L.gam_tune_candidates(4, v) == 2
-    assert L.gam_tune_candidates(5, v) == 6 and list(v[3:6]) == [4, 5, 6]
+    assert L.gam_tune_candidates(5, v) == 7 and list(v[3:7]) == [4, 5, 6, 7]
     assert L.gam_tune_candidates(8, v) == 0
     plan = (C.c_int * 8)()
     assert L.gam_get_plan(None, plan) == -1
