@@ -24,9 +24,13 @@
  * (P the CellC2P pressure), w a weight array (Ni,Nj,Nk) given by the
  * caller (e.g. cell volume times a region mask).
  *
- * dt: each step keeps the dt of the recorded forward run (the CFL minimum
- * is not differentiated); gam_adj_replay runs the forward with those dt
- * values, which is what finite-difference checks of the gradient need.
+ * dt: by default each step keeps the dt of the recorded forward run (the
+ * CFL minimum is not differentiated) and gam_adj_replay runs the forward
+ * with those dt values.  gam_adj_set_dt_mode turns on the dt
+ * sensitivities: dJ/d(dt_n) of every step (gam_adj_dt_grad), and, in mode
+ * 2, the gradient of the run in which each dt after the first is CalcDT's
+ * minimum of the state before it (step.F90:58-101), as in the reference's
+ * step loop; gam_adj_replay_dt replays with a given or a free dt sequence.
  *
  * Memory: checkpoints of State and oState every ckpt_every steps, and the
  * states of one checkpoint interval, recomputed forward before its
@@ -108,6 +112,34 @@ int gam_adj_wind_grad(gam_adj *a, double *gQ, double *gB);
  * differentiated exactly.  The two differ only for the time-dependent
  * inflow BC.  Takes effect at the next gam_adj_forward. */
 int gam_adj_set_ostate_bc_time(gam_adj *a, int at_state_time);
+/* dt sensitivities of the next gam_adj_backward (no counterpart in the
+ * reference): mode 0 (default) none, the dt of every step held at the
+ * recorded value; 1: also dJ/d(dt_n) for n = 0..nsteps-1, each with the
+ * other steps' dt held (the partial derivatives through the update, the
+ * constrained transport, the EMF's vDiff cap CFL*edge/dt and the AB2
+ * predictor weight dt_n / (2 dt_(n-1))); 2: dt_(n+1) = CalcDT of the state
+ * step n leaves before its boundary pass (the minimum of CellDT over
+ * is-1..ie+1 x js-1..je+1 x ks-1..ke+1, gridutils.F90:49-106), so the
+ * gradients are those of the run whose dt follows the state, and
+ * dJ/d(dt_n) is the total derivative (dt_0 is the caller's input).  A dt
+ * the recorded run capped (tFin, 10*dt0) does not follow the state.  Mode
+ * 2 takes effect at the next gam_adj_forward (its steps record CalcDT's
+ * argmin cell and values before the boundary pass, the state that exists
+ * only inside the step); modes 0 and 1 at the next gam_adj_backward.  The
+ * first call with mode > 0 allocates 2 planes of the grid. */
+int gam_adj_set_dt_mode(gam_adj *a, int mode);
+/* After gam_adj_backward with the dt sensitivities on: g[n] = dJ/d(dt_n),
+ * n = 0..nsteps-1. */
+int gam_adj_dt_grad(gam_adj *a, double *g);
+/* dt of each step of the recorded run, dts[n], n = 0..nsteps-1. */
+int gam_adj_step_dt(gam_adj *a, double *dts);
+/* gam_adj_replay with the dt of step n = dts[n] (nsteps values, each >= 0;
+ * a 0 takes CalcDT's dt of the state before that step, or the recorded dt
+ * for the first step), or, with dts NULL, the recorded dt for the first
+ * step and CalcDT's for each later one (the forward the dt mode 2
+ * gradients belong to). */
+int gam_adj_replay_dt(gam_adj *a, const double *Gas, const double *oGas, const double *MF,
+                      const double *oMF, const double *dts, double *J);
 /* Milliseconds of the last gam_adj_forward and gam_adj_backward calls
  * (HIP events on the context's stream). */
 int gam_adj_times(gam_adj *a, double *fwd_ms, double *bwd_ms);

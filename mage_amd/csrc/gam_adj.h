@@ -467,11 +467,12 @@ __device__ __forceinline__ void maxwell_adj(const MParm &m, const double *U, con
 // PredictCell (gas part of k_predict) at one cell: Q = State gas, oQ =
 // oState gas, w = AB2 weight, vol = cell volume.  Adjoints of the
 // predictor's outputs: aHG (half-state conserved, H_GAS), aWH (V * half
-// primitives, H_WH), aWN (limiter primitives, H_WN).  Adds into aQ, aoQ.
+// primitives, H_WH), aWN (limiter primitives, H_WN).  Adds into aQ, aoQ,
+// and, where aw is given, the adjoint of the AB2 weight w into *aw.
 __device__ __forceinline__ void predict_cell_adj(const MParm &m, const double *Q, const double *oQ,
                                                  double w, double vol, const double *aHG,
                                                  const double *aWH, const double *aWN, double *aQ,
-                                                 double *aoQ)
+                                                 double *aoQ, double *aw = nullptr)
 {
     double W[5], oW[5], pW[5], pQ[5];
     c2p_cell(m, oQ, oW);
@@ -494,6 +495,8 @@ __device__ __forceinline__ void predict_cell_adj(const MParm &m, const double *Q
         aW[v] = (1.0 + w) * apW[v];
         aoW[v] = -w * apW[v];
     }
+    if (aw)
+        for (int v = 0; v < 5; v++) *aw += (W[v] - oW[v]) * apW[v];
     c2p_cell_adj(m, Q, aW, aQ);
     c2p_cell_adj(m, oQ, aoW, aoQ);
     if (m.doLFM) c2p_block_adj(m, Q, aWN, aQ);
@@ -560,15 +563,18 @@ __device__ __forceinline__ double edge_emf(const MParm &m, const EdgeIn &in, dou
 // reconstructions' stencil scalars and limiter adjoints (recon_lr_w_adj_s)
 // for the mH/A and mL/A values; aDc the corner density's adjoint (each D4
 // gets aDc / 4).
+// E is the edge's EMF (times the edge length, as k_emf stores it) and acap
+// the adjoint of the vDiff cap (nonzero only where the cap binds).
 struct EdgeAdjS {
     double sv[3], sl1, sr1, anq1[4], sl2, sr2, anq2[4], aDc;
+    double E, acap;
 };
 
 template <int RM, bool BORIS>
 __device__ __forceinline__ void edge_emf_adj_s(const MParm &m, const EdgeIn &in, double aE, EdgeAdjS &o)
 {
     double l1, r1, l2, r2;
-    edge_emf<RM, BORIS>(m, in, &l1, &r1, &l2, &r2);
+    o.E = edge_emf<RM, BORIS>(m, in, &l1, &r1, &l2, &r2);
     // forward pieces again
     const double rdA = 1.0 / dmax(wsum8([&](int n) { return in.A1[n]; }), GAM_TINY);
     double vc[3];
@@ -594,6 +600,7 @@ __device__ __forceinline__ void edge_emf_adj_s(const MParm &m, const EdgeIn &in,
     double av1 = -b2 * ae, av2 = b1 * ae, ab1 = v2 * ae, ab2 = -v1 * ae;
     const double avDiff = m.Vd0 * Jd * ae, aJd = m.Vd0 * vDiff * ae;
     const double avs = (vsum < in.cap) ? avDiff : 0.0;
+    o.acap = (vsum < in.cap) ? 0.0 : avDiff;
     double au = avs;
     if (BORIS) {
         const double c2 = m.Ca * m.Ca + u * u;
@@ -652,6 +659,70 @@ __device__ __forceinline__ void edge_emf_adj(const MParm &m, const EdgeIn &in, d
         out.mL2[n] += t.anq2[n] / dmax(in.A2[n + 2], GAM_TINY);
         out.D4[n] += 0.25 * t.aDc;
     }
+}
+
+// ---------------------------------------------------------- CellDT
+// cell_dt (CellDT, gridutils.F90:49-106) on conserved U and total cell B:
+// adds adt * d(cell_dt)/dU into aU (5) and adt * d(cell_dt)/dB into aB (3).
+template <bool BORIS>
+__device__ __forceinline__ void cell_dt_adj(const MParm &m, const double *U, double bx, double by,
+                                            double bz, double dl, double adt, double *aU, double *aB)
+{
+    const double rho = U[DEN];
+    const double ir = 1.0 / rho;
+    const double v[3] = {U[MOMX] * ir, U[MOMY] * ir, U[MOMZ] * ir};
+    const double mv = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    const double ke = 0.5 * rho * (mv * mv);
+    const double p = (m.gamma - 1) * (U[ENER] - ke);
+    const double cs = sqrt(m.gamma * p * ir);
+    double u = 0.0, va = 0.0, nb = 0.0;
+    if (m.doMHD) {
+        nb = sqrt(bx * bx + by * by + bz * bz);
+        u = nb / sqrt(rho);
+        va = u;
+        if (BORIS) va = m.Ca * u / sqrt(m.Ca * m.Ca + u * u);
+    }
+    const double f = sqrt(cs * cs + va * va);
+    const double vcfl = mv + f + 0.0;
+    const double dt = m.CFL * dl / vcfl;
+    // reverse
+    const double avc = -dt / vcfl * adt;
+    double amv = avc, arho = 0.0, air = 0.0;
+    const double af = avc;
+    double acs = 0.0, ava = 0.0;
+    if (f > 0.0) { acs = af * cs / f; ava = af * va / f; }
+    if (m.doMHD) {
+        double au = ava;
+        if (BORIS) {
+            const double c2 = m.Ca * m.Ca + u * u;
+            au = ava * m.Ca * m.Ca * m.Ca / (c2 * sqrt(c2));
+        }
+        const double anb = au / sqrt(rho);
+        arho += -0.5 * u / rho * au;
+        if (nb > 0.0) {
+            aB[0] += bx / nb * anb;
+            aB[1] += by / nb * anb;
+            aB[2] += bz / nb * anb;
+        }
+    }
+    double ap = 0.0;
+    if (cs > 0.0) {
+        const double t = acs * m.gamma / (2.0 * cs);
+        ap = t * ir;
+        air += t * p;
+    }
+    aU[ENER] += (m.gamma - 1) * ap;
+    const double ake = -(m.gamma - 1) * ap;
+    arho += 0.5 * (mv * mv) * ake;
+    amv += rho * mv * ake;
+    if (mv > 0.0)
+        for (int x = 0; x < 3; x++) {
+            const double avx = v[x] / mv * amv;
+            aU[1 + x] += ir * avx;
+            air += U[1 + x] * avx;
+        }
+    arho += -ir * ir * air;
+    aU[DEN] += arho;
 }
 
 // CellBoris (the Boris branch of k_update), single fluid.  Inputs: oU =

@@ -16,8 +16,14 @@
 // gathers over a stencil (the hydrodynamic flux, cell B, the boundary
 // scatter of the inflow and outer-wall BCs) scatter with FP64 atomic adds
 // (hardware global_atomic_add_f64); the rest gather.  dt is the forward
-// run's value at each step (the CFL minimum is held fixed, as the
-// finite-difference checks also do).
+// run's value at each step; with the dt sensitivities on, the update, EMF
+// and predictor transposes also write each point's share of dJ/d(dt) of
+// their step into two planes (pdt: terms in the step's dt; pw: terms in
+// the predictor's AB2 weight), which k_dt_part / k_dt_fold reduce into
+// one value per step.  CalcDT's minimum (step.F90:58-101) is recorded in
+// the forward steps (k_dtmin_part / k_dtmin_record: the argmin cell and its
+// values before the boundary pass) and k_dt_inject carries the next step's
+// value back through it.
 #include "gam_adj.h"
 #include "gam_kernels.h"
 
@@ -145,17 +151,31 @@ __global__ void k_bc_adj(Geo g, Box t, double *__restrict__ aS)
 // Transpose of Sn = So + dtM * dH (active cells), Sn = So (other cells):
 // aSo += aSn everywhere; the adjoint of the active faces' fluxes is
 // dtM (aSn/V of the cell above the face - aSn/V of the cell below).
+// With pdt, active cells also write d/d(dtM) of their update: aSn . dH.
 __global__ __launch_bounds__(256) void k_update_adj(Geo g, Box fb, Box ab, const Clock *clk,
                                                     const double *__restrict__ aSn,
                                                     double *__restrict__ aSo,
                                                     double *__restrict__ ag,
-                                                    const double *__restrict__ met)
+                                                    const double *__restrict__ met,
+                                                    const double *__restrict__ gflx,
+                                                    double *__restrict__ pdt)
 {
     POINT_IN(fb);
     const long N3 = g.N3;
     const long c = IX(g, i, j, k);
     if (i <= g.ieg && j <= g.jeg && k <= g.keg)
         for (int v = 0; v < 5; v++) aSo[v * N3 + c] += aSn[v * N3 + c];
+    if (pdt && inside(ab, i, j, k)) {
+        const double idV = 1.0 / met[M_VOL * N3 + c];
+        double sdt = 0.0;
+        for (int v = 0; v < 5; v++) {
+            const double *gv = gflx + (long)v * N3;
+            const double dH = (gv[c] - gv[c + 1] + gv[5 * N3 + c] - gv[5 * N3 + c + g.P] + gv[10 * N3 + c] -
+                               gv[10 * N3 + c + g.PP]) * idV;
+            sdt += aSn[v * N3 + c] * dH;
+        }
+        pdt[c] = sdt;
+    }
     const double dtM = clk->dt;
     const long st[3] = {1, g.P, g.PP};
     for (int d = 0; d < 3; d++) {
@@ -229,21 +249,23 @@ __global__ __launch_bounds__(256) void k_predict_adj(Geo g, MParm m, Box cb, con
                                                      const double *__restrict__ aWH,
                                                      const double *__restrict__ aWN,
                                                      double *__restrict__ aS,
-                                                     double *__restrict__ aO)
+                                                     double *__restrict__ aO,
+                                                     double *__restrict__ pw)
 {
     POINT_IN(cb);
     const long N3 = g.N3;
     const long c = IX(g, i, j, k);
     const double w = ab2_weight(clk);
     double Q[5], oQ[5], aHG[5] = {0, 0, 0, 0, 0}, awh[5], awn[5], aQ[5] = {0, 0, 0, 0, 0},
-           aoQ[5] = {0, 0, 0, 0, 0};
+           aoQ[5] = {0, 0, 0, 0, 0}, aw = 0.0;
     for (int v = 0; v < 5; v++) {
         Q[v] = S[(S_GAS + v) * N3 + c];
         oQ[v] = O[(S_GAS + v) * N3 + c];
         awh[v] = aWH[v * N3 + c];
         awn[v] = aWN[v * N3 + c];
     }
-    predict_cell_adj(m, Q, oQ, w, met[M_VOL * N3 + c], aHG, awh, awn, aQ, aoQ);
+    predict_cell_adj(m, Q, oQ, w, met[M_VOL * N3 + c], aHG, awh, awn, aQ, aoQ, pw ? &aw : nullptr);
+    if (pw) pw[c] = aw;
     for (int v = 0; v < 5; v++) {
         aS[v * N3 + c] += aQ[v];
         aO[v * N3 + c] = aoQ[v];
@@ -259,6 +281,209 @@ __global__ void k_set_dt(Clock *clk, const Clock *rec)
     c.dt = rec->dt;
     c.dtA = rec->dtA;
     *clk = c;
+}
+
+// dt = dtA = dts[n] (replays with a given dt sequence)
+__global__ void k_set_dt_val(Clock *clk, const double *dts, int n)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    Clock c = *clk;
+    c.dt = dts[n];
+    c.dtA = dts[n];
+    *clk = c;
+}
+
+// ------------------------------------------------------------ dt sensitivities
+// d(objective)/d(dt) of a step.  Its dt enters the update (dt dH, dt dM and
+// the CT term dt curl E), the EMF's vDiff cap (CFL edge / dt) and the
+// predictor's AB2 weight w = dt / (2 odt), with odt = State%time -
+// oState%time, the previous step's dt.  The transposes leave each point's
+// share in two planes: pdt (terms in the step's dt) and pw (terms in w).
+// k_dt_part sums them per 64 x 4 block of the face box; k_dt_fold (one
+// block, a fixed order: deterministic) folds the sums into mu.
+__global__ __launch_bounds__(256) void k_dt_part(Geo g, Box b, const double *__restrict__ pdt,
+                                                 const double *__restrict__ pw, double *__restrict__ part)
+{
+    __shared__ double r0[ABX * ABY], r1[ABX * ABY];
+    const int tid = threadIdx.x + ABX * threadIdx.y;
+    const int i = b.i0 + (int)(blockIdx.x * ABX + threadIdx.x);
+    const int j = b.j0 + (int)(blockIdx.y * ABY + threadIdx.y);
+    const int k = b.k0 + (int)blockIdx.z;
+    double x = 0.0, y = 0.0;
+    if (i <= b.i1 && j <= b.j1) {
+        const long c = IX(g, i, j, k);
+        x = pdt[c];
+        y = pw[c];
+    }
+    r0[tid] = x;
+    r1[tid] = y;
+    __syncthreads();
+    for (int h = ABX * ABY / 2; h > 0; h >>= 1) {
+        if (tid < h) {
+            r0[tid] += r0[tid + h];
+            r1[tid] += r1[tid + h];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const long q = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+        part[2 * q] = r0[0];
+        part[2 * q + 1] = r1[0];
+    }
+}
+
+// mu[n] += the step's direct terms + the weight's share through dtA;
+// mu[n - 1] += the weight's share through odt.  On a first step (oState%time
+// = State%time) odt is the step's own dt, as in k_predict, and w = 1/2.
+__global__ __launch_bounds__(256) void k_dt_fold(const double *__restrict__ part, long np,
+                                                 const Clock *clk, int n, double *__restrict__ mu)
+{
+    __shared__ double r0[256], r1[256];
+    const int tid = threadIdx.x;
+    double x = 0.0, y = 0.0;
+    for (long q = tid; q < np; q += 256) {
+        x += part[2 * q];
+        y += part[2 * q + 1];
+    }
+    r0[tid] = x;
+    r1[tid] = y;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (tid < h) {
+            r0[tid] += r0[tid + h];
+            r1[tid] += r1[tid + h];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const Clock c = *clk;
+        const double st = c.st;
+        double ost = c.ost;
+        const bool first = fabs(ost - st) < GAM_TINY;
+        if (first) ost = st - c.dt;
+        const double odt = st - ost;
+        const double w = (0.5 * c.dtA) / odt;
+        const double aw = r1[0];
+        double m = r0[0] + aw * (0.5 / odt);   // dw/d(dtA) = 1 / (2 odt)
+        if (first) m += aw * (-w / odt);       // dw/d(odt) = -w / odt
+        mu[n] += m;
+        if (!first && n > 0) mu[n - 1] += aw * (-w / odt);
+    }
+}
+
+// CalcDT's minimum (step.F90:58-101) over the DT box of the state the
+// update has just written, before the boundary pass (run inside the
+// recorded and recomputed forward steps, where that state exists): per
+// 64 x 4 block the smallest CellDT and its cell (the lowest index on ties),
+// keeping only positive finite values as the forward's reduction does.
+__device__ __forceinline__ void dt_cell_in(const Geo &g, const MParm &m, long c, const double *S, const double *met,
+                                           const double *bg, double *U, double *B, double &dl)
+{
+    const long N3 = g.N3;
+    for (int v = 0; v < 5; v++) U[v] = S[(S_GAS + v) * N3 + c];
+    for (int x = 0; x < 3; x++) B[x] = m.doMHD ? S[(S_B + x) * N3 + c] + (m.doBG ? bg[(B_B0 + x) * N3 + c] : 0.0) : 0.0;
+    dl = dmin(dmin(met[M_DI * N3 + c], met[M_DJ * N3 + c]), met[M_DK * N3 + c]);
+}
+
+__device__ __forceinline__ void min_pair(double *rv, double *ri, int a, int b)
+{
+    if (ri[b] >= 0.0 && (ri[a] < 0.0 || rv[b] < rv[a] || (rv[b] == rv[a] && ri[b] < ri[a]))) {
+        rv[a] = rv[b];
+        ri[a] = ri[b];
+    }
+}
+
+template <bool BORIS>
+__global__ __launch_bounds__(256) void k_dtmin_part(Geo g, MParm m, Box b, const double *__restrict__ S,
+                                                    const double *__restrict__ met,
+                                                    const double *__restrict__ bg, double *__restrict__ part)
+{
+    __shared__ double rv[ABX * ABY], ri[ABX * ABY];
+    const int tid = threadIdx.x + ABX * threadIdx.y;
+    const int i = b.i0 + (int)(blockIdx.x * ABX + threadIdx.x);
+    const int j = b.j0 + (int)(blockIdx.y * ABY + threadIdx.y);
+    const int k = b.k0 + (int)blockIdx.z;
+    double v = GAM_HUGE, ix = -1.0;
+    if (i <= b.i1 && j <= b.j1) {
+        const long c = IX(g, i, j, k);
+        double U[5], B[3], dl;
+        dt_cell_in(g, m, c, S, met, bg, U, B, dl);
+        const double d = cell_dt<BORIS>(m, U, B[0], B[1], B[2], dl);
+        if (d > 0.0 && d < GAM_HUGE) {
+            v = d;
+            ix = (double)c;
+        }
+    }
+    rv[tid] = v;
+    ri[tid] = ix;
+    __syncthreads();
+    for (int h = ABX * ABY / 2; h > 0; h >>= 1) {
+        if (tid < h) min_pair(rv, ri, tid, tid + h);
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const long q = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+        part[2 * q] = rv[0];
+        part[2 * q + 1] = ri[0];
+    }
+}
+
+// One block: the argmin over the blocks.  If its CellDT is the dt CalcDT
+// has just set (clk, after the step's time bookkeeping; not capped by tFin
+// or 10 dt0, where dt does not follow the state), rec = {cell, gas (5),
+// total cell B (3), dl, 1}; otherwise rec[10] = 0.
+template <bool BORIS>
+__global__ __launch_bounds__(256) void k_dtmin_record(Geo g, MParm m, const double *__restrict__ part, long np,
+                                                      const Clock *clk, const double *__restrict__ S,
+                                                      const double *__restrict__ met,
+                                                      const double *__restrict__ bg, double *__restrict__ rec)
+{
+    __shared__ double rv[256], ri[256];
+    const int tid = threadIdx.x;
+    rv[tid] = GAM_HUGE;
+    ri[tid] = -1.0;
+    for (long q = tid; q < np; q += 256) {
+        const double pv = part[2 * q], pi = part[2 * q + 1];
+        if (pi >= 0.0 && (ri[tid] < 0.0 || pv < rv[tid] || (pv == rv[tid] && pi < ri[tid]))) {
+            rv[tid] = pv;
+            ri[tid] = pi;
+        }
+    }
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (tid < h) min_pair(rv, ri, tid, tid + h);
+        __syncthreads();
+    }
+    if (tid != 0) return;
+    const double dtn = clk->dt;
+    rec[10] = 0.0;
+    if (ri[0] < 0.0 || !(fabs(rv[0] - dtn) <= 1e-12 * dtn)) return;
+    const long c = (long)ri[0];
+    double U[5], B[3], dl;
+    dt_cell_in(g, m, c, S, met, bg, U, B, dl);
+    rec[0] = (double)c;
+    for (int v = 0; v < 5; v++) rec[1 + v] = U[v];
+    for (int x = 0; x < 3; x++) rec[6 + x] = B[x];
+    rec[9] = dl;
+    rec[10] = 1.0;
+}
+
+// CalcDT transpose: lam += mu * d(CellDT)/d(gas, cell B) at the recorded
+// argmin (the adjoint of the update's output before the boundary pass).
+template <bool BORIS>
+__global__ void k_dt_inject(Geo g, MParm m, const double *__restrict__ rec, const double *__restrict__ mu,
+                            double *__restrict__ lam)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double a = *mu;
+    if (rec[10] == 0.0 || a == 0.0) return;
+    const long c = (long)rec[0];
+    double aU[5] = {0, 0, 0, 0, 0}, aB[3] = {0, 0, 0};
+    cell_dt_adj<BORIS>(m, rec + 1, rec[6], rec[7], rec[8], rec[9], a, aU, aB);
+    const long N3 = g.N3;
+    for (int v = 0; v < 5; v++) lam[(S_GAS + v) * N3 + c] += aU[v];
+    if (m.doMHD)
+        for (int x = 0; x < 3; x++) lam[(S_B + x) * N3 + c] += aB[x];
 }
 
 // ============================================================ MHD
@@ -467,7 +692,8 @@ __global__ __launch_bounds__(256) void k_update_adj_cells(Geo g, MParm m, Box cb
                                                           double *__restrict__ aSn,
                                                           double *__restrict__ aSo,
                                                           double *__restrict__ adHo,
-                                                          double *__restrict__ adMo)
+                                                          double *__restrict__ adMo,
+                                                          double *__restrict__ pdt)
 {
     POINT_IN(cb);
     const long N3 = g.N3;
@@ -519,6 +745,14 @@ __global__ __launch_bounds__(256) void k_update_adj_cells(Geo g, MParm m, Box cb
         adHo[v * N3 + c] = adH[v];
     }
     for (int x = 0; x < 3; x++) adMo[x * N3 + c] = adm[x];
+    if (pdt) {
+        // the cell's update depends on dt only through dt dH and dt dM, so
+        // dt d/d(dt) = dH . adH + dM . adM
+        double sdt = 0.0;
+        for (int v = 0; v < 5; v++) sdt += dH[v] * adH[v];
+        for (int x = 0; x < 3; x++) sdt += dM[x] * adm[x];
+        pdt[c] = sdt / dt;
+    }
 }
 
 // Update transpose, part 2 (faces): adjoints of the active faces' gas and
@@ -777,7 +1011,8 @@ __global__ __launch_bounds__(256, 2) void k_emf_adj(Geo g, MParm m, Box eb, cons
                                                  const double *__restrict__ met,
                                                  const double *__restrict__ bg,
                                                  const double *__restrict__ aE,
-                                                 double *__restrict__ sc)
+                                                 double *__restrict__ sc,
+                                                 double *__restrict__ pdt)
 {
     POINT_IN(eb);
     constexpr int T1 = (ED == 0) ? 1 : (ED == 1) ? 2 : 0;
@@ -815,7 +1050,12 @@ __global__ __launch_bounds__(256, 2) void k_emf_adj(Geo g, MParm m, Box eb, cons
     in.edge = met[(M_EDGE + ED) * N3 + c];
     in.cap = m.CFL * in.edge / clk->dt;
     EdgeAdjS o;
-    edge_emf_adj_s<RM, BORIS>(m, in, aE[ED * N3 + c], o);
+    const double aEc = aE[ED * N3 + c];
+    edge_emf_adj_s<RM, BORIS>(m, in, aEc, o);
+    // dt of this edge's terms: constrained transport (new fluxes = old -
+    // dt C E with C the circulation, and aE = -dt C^T aF, so the flux
+    // adjoints' share is E . aE / dt) and the vDiff cap CFL edge / dt
+    if (pdt) pdt[c] += (o.E * aEc - o.acap * in.cap) / clk->dt;
     const double v[16] = {o.sv[0], o.sv[1], o.sv[2], o.sl1, o.sr1, o.anq1[0], o.anq1[1], o.anq1[2], o.anq1[3],
                           o.sl2, o.sr2, o.anq2[0], o.anq2[1], o.anq2[2], o.anq2[3], o.aDc};
     for (int q = 0; q < 16; q++)
@@ -939,7 +1179,10 @@ __global__ __launch_bounds__(256) void k_predict_mf_adj(Geo g, Box fb, const Clo
                                                         const double *__restrict__ aHB,
                                                         const double *__restrict__ aHMF,
                                                         double *__restrict__ aS,
-                                                        double *__restrict__ aO)
+                                                        double *__restrict__ aO,
+                                                        const double *__restrict__ S,
+                                                        const double *__restrict__ O,
+                                                        double *__restrict__ pw)
 {
     POINT_IN(fb);
     const long N3 = g.N3;
@@ -965,6 +1208,7 @@ __global__ __launch_bounds__(256) void k_predict_mf_adj(Geo g, Box fb, const Clo
         }
         aS[(S_MF + d) * N3 + c] += (1.0 + w) * a;
         aO[(S_MF + d) * N3 + c] = -w * a;
+        if (pw) pw[c] += (S[(S_MF + d) * N3 + c] - O[(S_MF + d) * N3 + c]) * a;
     }
 }
 
@@ -978,13 +1222,14 @@ __global__ __launch_bounds__(256) void k_predict_adj_mhd(Geo g, MParm m, Box cb,
                                                          const double *__restrict__ aWH,
                                                          const double *__restrict__ aWN,
                                                          double *__restrict__ aS,
-                                                         double *__restrict__ aO)
+                                                         double *__restrict__ aO,
+                                                         double *__restrict__ pw)
 {
     POINT_IN(cb);
     const long N3 = g.N3;
     const long c = IX(g, i, j, k);
     const double w = ab2_weight(clk);
-    double Q[5], oQ[5], ahg[5], awh[5], awn[5], aQ[5] = {0, 0, 0, 0, 0}, aoQ[5] = {0, 0, 0, 0, 0};
+    double Q[5], oQ[5], ahg[5], awh[5], awn[5], aQ[5] = {0, 0, 0, 0, 0}, aoQ[5] = {0, 0, 0, 0, 0}, aw = 0.0;
     for (int v = 0; v < 5; v++) {
         Q[v] = S[(S_GAS + v) * N3 + c];
         oQ[v] = O[(S_GAS + v) * N3 + c];
@@ -992,7 +1237,8 @@ __global__ __launch_bounds__(256) void k_predict_adj_mhd(Geo g, MParm m, Box cb,
         awh[v] = aWH[v * N3 + c];
         awn[v] = aWN[v * N3 + c];
     }
-    predict_cell_adj(m, Q, oQ, w, met[M_VOL * N3 + c], ahg, awh, awn, aQ, aoQ);
+    predict_cell_adj(m, Q, oQ, w, met[M_VOL * N3 + c], ahg, awh, awn, aQ, aoQ, pw ? &aw : nullptr);
+    if (pw) pw[c] = aw;
     for (int v = 0; v < 5; v++) {
         aS[v * N3 + c] += aQ[v];
         aO[v * N3 + c] = aoQ[v];
@@ -1039,10 +1285,10 @@ void launch_bc_adj(hipStream_t s, int side, int kind, const Geo &g, double *aS)
 }
 
 void launch_update_adj(hipStream_t s, const Geo &g, const Box &ab, const Clock *clk, const double *aSn,
-                       double *aSo, double *ag, const double *met)
+                       double *aSo, double *ag, const double *met, const double *gflx, double *pdt)
 {
     const Box fb = {g.isg, g.ieg + 1, g.jsg, g.jeg + 1, g.ksg, g.keg + 1};
-    hipLaunchKernelGGL(k_update_adj, agrid(fb), dim3(ABT), 0, s, g, fb, ab, clk, aSn, aSo, ag, met);
+    hipLaunchKernelGGL(k_update_adj, agrid(fb), dim3(ABT), 0, s, g, fb, ab, clk, aSn, aSo, ag, met, gflx, pdt);
 }
 
 template <int D>
@@ -1063,16 +1309,65 @@ void launch_flux_adj(hipStream_t s, int d, const Geo &g, const MParm &m, const B
 
 void launch_predict_adj(hipStream_t s, const Geo &g, const MParm &m, const Clock *clk, const double *S,
                         const double *O, const double *met, const double *aWH, const double *aWN,
-                        double *aS, double *aO)
+                        double *aS, double *aO, double *pw)
 {
     const Box cb = {g.isg, g.ieg, g.jsg, g.jeg, g.ksg, g.keg};
     hipLaunchKernelGGL(k_predict_adj, agrid(cb), dim3(ABT), 0, s, g, m, cb, clk, S, O, met, aWH, aWN,
-                       aS, aO);
+                       aS, aO, pw);
 }
 
 void launch_set_dt(hipStream_t s, Clock *clk, const Clock *rec)
 {
     hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(64), 0, s, clk, rec);
+}
+
+void launch_set_dt_val(hipStream_t s, Clock *clk, const double *dts, int n)
+{
+    hipLaunchKernelGGL(k_set_dt_val, dim3(1), dim3(64), 0, s, clk, dts, n);
+}
+
+static inline Box face_points(const Geo &g) { return Box{g.isg, g.ieg + 1, g.jsg, g.jeg + 1, g.ksg, g.keg + 1}; }
+
+long dt_part_blocks(const Geo &g)
+{
+    const dim3 gr = agrid2(face_points(g));
+    return (long)gr.x * gr.y * gr.z;
+}
+
+void launch_dt_fold(hipStream_t s, const Geo &g, const double *pdt, const double *pw, double *part,
+                    const Clock *clk, int n, double *mu)
+{
+    const Box fb = face_points(g);
+    const dim3 gr = agrid2(fb);
+    hipLaunchKernelGGL(k_dt_part, gr, dim3(ABX, ABY), 0, s, g, fb, pdt, pw, part);
+    hipLaunchKernelGGL(k_dt_fold, dim3(1), dim3(256), 0, s, part, (long)gr.x * gr.y * gr.z, clk, n, mu);
+}
+
+long dt_record_blocks(const Geo &g, const Box &dtb)
+{
+    const dim3 gr = agrid2(dtb);
+    return (long)gr.x * gr.y * gr.z;
+}
+
+void launch_dt_record(hipStream_t s, const Geo &g, const MParm &m, const Box &dtb, const double *S,
+                      const double *met, const double *bg, const Clock *clk, double *part, double *rec)
+{
+    const dim3 gr = agrid2(dtb);
+    const long np = (long)gr.x * gr.y * gr.z;
+    if (m.doBoris) {
+        hipLaunchKernelGGL(k_dtmin_part<true>, gr, dim3(ABX, ABY), 0, s, g, m, dtb, S, met, bg, part);
+        hipLaunchKernelGGL(k_dtmin_record<true>, dim3(1), dim3(256), 0, s, g, m, part, np, clk, S, met, bg, rec);
+    } else {
+        hipLaunchKernelGGL(k_dtmin_part<false>, gr, dim3(ABX, ABY), 0, s, g, m, dtb, S, met, bg, part);
+        hipLaunchKernelGGL(k_dtmin_record<false>, dim3(1), dim3(256), 0, s, g, m, part, np, clk, S, met, bg, rec);
+    }
+}
+
+void launch_dt_inject(hipStream_t s, const Geo &g, const MParm &m, const double *rec, const double *mu,
+                      double *lam)
+{
+    if (m.doBoris) hipLaunchKernelGGL(k_dt_inject<true>, dim3(1), dim3(64), 0, s, g, m, rec, mu, lam);
+    else hipLaunchKernelGGL(k_dt_inject<false>, dim3(1), dim3(64), 0, s, g, m, rec, mu, lam);
 }
 
 // ------------------------------------------------------------ MHD launchers
@@ -1140,15 +1435,15 @@ void launch_update_adj_mhd(hipStream_t s, const Geo &g, const MParm &m, const Bo
                            const Box *eboxes, const Clock *clk, const double *So, const double *Sn1,
                            const double *gflx, const double *mflx, const double *met, const double *bg,
                            double *aSn, double *aSo, double *aUr, double *adM, double *ag, double *amf,
-                           double *aE)
+                           double *aE, double *pdt)
 {
     const Box cb = cells_of(g), fb = faces_of(g);
     if (m.doBoris)
         hipLaunchKernelGGL(k_update_adj_cells<true>, agrid(cb), dim3(ABT), 0, s, g, m, cb, ab, clk, So, Sn1,
-                           gflx, mflx, met, bg, m.doBG, aSn, aSo, aUr, adM);
+                           gflx, mflx, met, bg, m.doBG, aSn, aSo, aUr, adM, pdt);
     else
         hipLaunchKernelGGL(k_update_adj_cells<false>, agrid(cb), dim3(ABT), 0, s, g, m, cb, ab, clk, So, Sn1,
-                           gflx, mflx, met, bg, m.doBG, aSn, aSo, aUr, adM);
+                           gflx, mflx, met, bg, m.doBG, aSn, aSo, aUr, adM, pdt);
     // the update's new cell B from its new face fluxes (every cell)
     launch_cellb_adj(s, g, 1, aSn, met);
     EdgeBoxes eb;
@@ -1206,19 +1501,19 @@ void launch_recon_adj_gather(hipStream_t s, const Geo &g, const MParm &m, const 
 template <int ED, int RM, bool BG, bool BO>
 static void emf_adj_t(hipStream_t s, const Geo &g, const MParm &m, const Box &eb, const Clock *clk,
                       const double *H, const double *S, const double *Vf, const double *met, const double *bg,
-                      const double *aE, double *sc)
+                      const double *aE, double *sc, double *pdt)
 {
     hipLaunchKernelGGL((k_emf_adj<ED, RM, BG, BO>), agrid(eb), dim3(ABT), 0, s, g, m, eb, clk, H, S, Vf, met,
-                       bg, aE, sc + (size_t)16 * ED * g.N3);
+                       bg, aE, sc + (size_t)16 * ED * g.N3, pdt);
 }
 
 template <int ED>
 static void emf_adj_d(hipStream_t s, const Geo &g, const MParm &m, const Box &eb, const Clock *clk,
                       const double *H, const double *S, const double *Vf, const double *met, const double *bg,
-                      const double *aE, double *sc)
+                      const double *aE, double *sc, double *pdt)
 {
     const int sel = (m.rmeth ? 4 : 0) + (m.doBG ? 2 : 0) + (m.doBoris ? 1 : 0);
-#define EMFADJ(RM, BG, BO) emf_adj_t<ED, RM, BG, BO>(s, g, m, eb, clk, H, S, Vf, met, bg, aE, sc)
+#define EMFADJ(RM, BG, BO) emf_adj_t<ED, RM, BG, BO>(s, g, m, eb, clk, H, S, Vf, met, bg, aE, sc, pdt)
     switch (sel) {
     case 0: EMFADJ(0, false, false); break;
     case 1: EMFADJ(0, false, true); break;
@@ -1234,11 +1529,11 @@ static void emf_adj_d(hipStream_t s, const Geo &g, const MParm &m, const Box &eb
 
 void launch_emf_adj(hipStream_t s, int ed, const Geo &g, const MParm &m, const Box &eb, const Clock *clk,
                     const double *H, const double *S, const double *Vf, const double *met, const double *bg,
-                    const double *aE, double *sc)
+                    const double *aE, double *sc, double *pdt)
 {
-    if (ed == 0) emf_adj_d<0>(s, g, m, eb, clk, H, S, Vf, met, bg, aE, sc);
-    else if (ed == 1) emf_adj_d<1>(s, g, m, eb, clk, H, S, Vf, met, bg, aE, sc);
-    else emf_adj_d<2>(s, g, m, eb, clk, H, S, Vf, met, bg, aE, sc);
+    if (ed == 0) emf_adj_d<0>(s, g, m, eb, clk, H, S, Vf, met, bg, aE, sc, pdt);
+    else if (ed == 1) emf_adj_d<1>(s, g, m, eb, clk, H, S, Vf, met, bg, aE, sc, pdt);
+    else emf_adj_d<2>(s, g, m, eb, clk, H, S, Vf, met, bg, aE, sc, pdt);
 }
 
 void launch_emf_adj_gather(hipStream_t s, const Geo &g, const MParm &m, const Box *eboxes, const double *sc,
@@ -1263,12 +1558,13 @@ void launch_vface_rv(hipStream_t s, const Geo &g, const double *met, double *rv)
 
 void launch_predict_adj_mhd(hipStream_t s, const Geo &g, const MParm &m, const Clock *clk, const double *S,
                             const double *O, const double *met, const double *aHG, const double *aWH,
-                            const double *aWN, double *aHB, double *aHMF, double *aS, double *aO)
+                            const double *aWN, double *aHB, double *aHMF, double *aS, double *aO, double *pw)
 {
     const Box cb = cells_of(g), fb = faces_of(g);
     hipLaunchKernelGGL(k_predict_adj_mhd, agrid(cb), dim3(ABT), 0, s, g, m, cb, clk, S, O, met, aHG, aWH,
-                       aWN, aS, aO);
-    hipLaunchKernelGGL(k_predict_mf_adj, agrid(fb), dim3(ABT), 0, s, g, fb, clk, met, aHB, aHMF, aS, aO);
+                       aWN, aS, aO, pw);
+    hipLaunchKernelGGL(k_predict_mf_adj, agrid(fb), dim3(ABT), 0, s, g, fb, clk, met, aHB, aHMF, aS, aO, S, O,
+                       pw);
 }
 
 // ============================================================ inflow BC

@@ -84,6 +84,9 @@ struct gam_ctx {
     int fork;
     hipStream_t aux[2];
     hipEvent_t fev[6];
+    // set by the adjoint's forward steps with the dt sensitivities through
+    // CalcDT: record each step's CalcDT argmin into dtrec (DT_REC doubles)
+    double *dtrec, *dtrec_part;
 };
 
 static void graphs_drop(gam_ctx *c)
@@ -680,6 +683,8 @@ static void enq_step(gam_ctx *c)
 {
     enq_advance(c, FIN_ADVANCE | FIN_MODEL | FIN_CALCDT | FIN_SETDT);
     double *S = S_cur(c);
+    if (c->dtrec)
+        launch_dt_record(c->stream, c->g, c->mp, dt_box(c->g), S, c->met, c->bg, c->clk, c->dtrec_part, c->dtrec);
     prof_begin(c, KG_BC);
     enq_bcs(c, S, false);
     prof_end(c, KG_BC);
@@ -1375,7 +1380,25 @@ struct gam_adj {
     double *gwq, *gwb;       // adjoints of the inflow series Q (nw,5), B (nw,3), WIND_COPIES copies each
     hipEvent_t ev[4];
     float fwd_ms, bwd_ms;
+    // dt sensitivities (gam_adj_set_dt_mode): 0 off, 1 dJ/d(dt_n) with the
+    // other steps' dt held, 2 also through CalcDT (each dt follows the state)
+    int dtMode, dtDone;
+    double *mu;              // dJ/d(dt_n), n = 0..nsteps
+    double *pdt;             // the transposes' per-point shares: pdt, then pw (2 planes)
+    double *dtpart;          // block partial sums (2 per block of the face box)
+    double *dts;             // a replay's dt sequence (nsteps)
+    double *dtrec;           // each step's CalcDT argmin record (DT_REC doubles; mode 2)
+    double *dtrpart;         // block partials of the forward steps' argmin scans
+    int dtrecOK;             // the recorded forward wrote dtrec
 };
+
+// Step n of a forward run records its CalcDT argmin (dt mode 2).
+static void dt_rec_use(gam_adj *a, int n)
+{
+    gam_ctx *c = a->c;
+    c->dtrec = a->dtMode == 2 ? a->dtrec + (size_t)n * DT_REC : nullptr;
+    c->dtrec_part = a->dtrpart;
+}
 
 static StepIT it_slot(gam_adj *a, int q, int buf = 0)
 {
@@ -1401,7 +1424,7 @@ extern "C" int gam_adj_destroy(gam_adj *a)
     if (a->c && a->c->stream) hipStreamSynchronize(a->c->stream);
     double *bufs[] = {a->ck, a->fin, a->tape, a->itape, a->tape2, a->itape2, a->lam, a->ag, a->amf, a->awh, a->awn, a->ahg, a->ahmf,
                       a->ahb, a->avf, a->aur, a->adm, a->ae, a->wgt, a->part, a->gwq, a->gwb, a->rv,
-                      a->fsc};
+                      a->fsc, a->mu, a->pdt, a->dtpart, a->dts, a->dtrec, a->dtrpart};
     for (double *b : bufs) if (b) hipFree(b);
     if (a->clkh) hipFree(a->clkh);
     if (a->oclk) hipFree(a->oclk);
@@ -1581,9 +1604,12 @@ extern "C" int gam_adj_forward(gam_adj *a, double *J)
                                   hipMemcpyDeviceToDevice, c->stream));
             it_use(a, q, 0, own);
         }
+        dt_rec_use(a, n);
         enq_step(c);
         HIPCHK(hipMemcpyAsync(a->clkh + n + 1, c->clk, sizeof(Clock), hipMemcpyDeviceToDevice, c->stream));
     }
+    c->dtrec = nullptr;
+    a->dtrecOK = a->dtMode == 2;
     c->H = own.H; c->vf = own.vf; c->gflx = own.gflx; c->mflx = own.mflx;
     HIPCHK(hipMemcpyAsync(a->tape + (size_t)(1 + a->nsteps - sL) * CK_NPL * N3, S_cur(c), ckb,
                           hipMemcpyDeviceToDevice, c->stream));
@@ -1635,7 +1661,10 @@ static void adj_bcs(gam_adj *a, double *l, const double *S, const Clock *bclk)
 // out.  Sn, On = the step's input State and oState, Sn1 its output State;
 // clk its clock, bclk the clock after it (its boundary pass's time).
 // it: the step's taped forward intermediates, or null to recompute them.
-static void adj_step(gam_adj *a, const StepIT *it, const double *Sn, const double *On, const double *Sn1,
+// n: the step's index (its dt sensitivity goes to mu[n], the predictor's
+// share through odt to mu[n - 1]; with dt mode 2, mu[n + 1] goes back
+// through the CalcDT that ends step n).
+static void adj_step(gam_adj *a, int n, const StepIT *it, const double *Sn, const double *On, const double *Sn1,
                      const Clock *clk, const Clock *bclk, double *lA, double *lB, double *lC)
 {
     gam_ctx *c = a->c;
@@ -1643,6 +1672,13 @@ static void adj_step(gam_adj *a, const StepIT *it, const double *Sn, const doubl
     const long N3 = g.N3;
     hipStream_t s = c->stream;
     const size_t pl = (size_t)N3 * sizeof(double);
+    double *pdt = a->dtMode ? a->pdt : nullptr, *pw = a->dtMode ? a->pdt + N3 : nullptr;
+    if (pdt) hipMemsetAsync(pdt, 0, 2 * pl, s);
+    // CalcDT at the end of step n (before its boundary pass) gave dt[n + 1]
+    auto dt_inject = [&]() {
+        if (a->dtMode == 2 && n + 1 < a->nsteps)
+            launch_dt_inject(s, g, c->mp, a->dtrec + (size_t)n * DT_REC, a->mu + n + 1, lA);
+    };
     Box eb[3];
     edge_boxes(g, eb);
     const double *H = it ? it->H : c->H, *vf = it ? it->vf : c->vf;
@@ -1650,28 +1686,31 @@ static void adj_step(gam_adj *a, const StepIT *it, const double *Sn, const doubl
     // the forward intermediates the transposes read
     if (!it) {
         launch_predict(s, c->plan.v[TS_PREDICT], g, c->mp, face_box(g), clk, Sn, On, c->met, c->H);
-        if (c->mp.doMHD) {
+        // the hydrodynamic transposes read no face fluxes, but the dt share
+        // of the update does
+        if (c->mp.doMHD || pdt) {
             for (int d = 0; d < 3; d++) {
                 Box fb = act_box(g);
                 if (d == 0) fb.i1 += 1; else if (d == 1) fb.j1 += 1; else fb.k1 += 1;
                 launch_flux(s, d, c->plan.v[TS_FLUX_I + d], g, c->mp, fb, c->H, Sn, c->met, c->bg, c->gflx,
                             c->mflx, c->vf + (long)3 * d * N3);
             }
-            launch_vface_rim(s, g, c->H, c->met, c->vf);
+            if (c->mp.doMHD) launch_vface_rim(s, g, c->H, c->met, c->vf);
         }
     }
     if (c->mp.doMHD) {
         // ghost bFlux2Fld, boundary pass, update (with its cell B)
         launch_cellb_adj(s, g, 0, lA, c->met);
         adj_bcs(a, lA, Sn1, bclk);
+        dt_inject();
         const Box ctb = mkbox(g.is, g.ie + 1, g.js, g.je + 1, g.ks, g.ke + 1);
         launch_update_adj_mhd(s, g, c->mp, act_box(g), ctb, eb, clk, Sn, Sn1, gflx, mflx, c->met, c->bg,
-                              lA, lB, a->aur, a->adm, a->ag, a->amf, a->ae);
+                              lA, lB, a->aur, a->adm, a->ag, a->amf, a->ae, pdt);
         // the EMF transpose first: its gather writes aVf, aHMF and aHG whole,
         // the flux transpose adds into aHMF and aHG, and its gather writes
         // aWH, aWN and aHB whole (no plane needs zeroing)
         for (int ed = 0; ed < 3; ed++)
-            launch_emf_adj(s, ed, g, c->mp, eb[ed], clk, H, Sn, vf, c->met, c->bg, a->ae, a->esc);
+            launch_emf_adj(s, ed, g, c->mp, eb[ed], clk, H, Sn, vf, c->met, c->bg, a->ae, a->esc, pdt);
         launch_emf_adj_gather(s, g, c->mp, eb, a->esc, c->met, a->avf, a->ahmf, a->ahg, lB + S_MF * N3);
         for (int d = 0; d < 3; d++) {
             Box fb = act_box(g);
@@ -1681,10 +1720,12 @@ static void adj_step(gam_adj *a, const StepIT *it, const double *Sn, const doubl
         }
         launch_recon_adj_gather(s, g, c->mp, a->fsc, a->rv, a->avf, c->met, a->awh, a->awn, a->ahb,
                                 lB + S_B * N3);
-        launch_predict_adj_mhd(s, g, c->mp, clk, Sn, On, c->met, a->ahg, a->awh, a->awn, a->ahb, a->ahmf, lB, lC);
+        launch_predict_adj_mhd(s, g, c->mp, clk, Sn, On, c->met, a->ahg, a->awh, a->awn, a->ahb, a->ahmf, lB, lC,
+                               pw);
     } else {
         adj_bcs(a, lA, Sn1, bclk);
-        launch_update_adj(s, g, act_box(g), clk, lA, lB, a->ag, c->met);
+        dt_inject();
+        launch_update_adj(s, g, act_box(g), clk, lA, lB, a->ag, c->met, gflx, pdt);
         hipMemsetAsync(a->awh, 0, 5 * pl, s);
         hipMemsetAsync(a->awn, 0, 5 * pl, s);
         for (int d = 0; d < 3; d++) {
@@ -1692,8 +1733,9 @@ static void adj_step(gam_adj *a, const StepIT *it, const double *Sn, const doubl
             if (d == 0) fb.i1 += 1; else if (d == 1) fb.j1 += 1; else fb.k1 += 1;
             launch_flux_adj(s, d, g, c->mp, fb, H, c->met, a->ag, a->awh, a->awn);
         }
-        launch_predict_adj(s, g, c->mp, clk, Sn, On, c->met, a->awh, a->awn, lB, lC);
+        launch_predict_adj(s, g, c->mp, clk, Sn, On, c->met, a->awh, a->awn, lB, lC, pw);
     }
+    if (pdt) launch_dt_fold(s, g, pdt, pw, a->dtpart, clk, n, a->mu);
 }
 
 extern "C" int gam_adj_backward(gam_adj *a, double *gGas, double *gGasO, double *gMF, double *gMFO)
@@ -1708,9 +1750,13 @@ extern "C" int gam_adj_backward(gam_adj *a, double *gGas, double *gGasO, double 
     double *lA = a->lam, *lB = a->lam + ADJ_SET * N3, *lC = a->lam + 2 * ADJ_SET * N3;
     if (a->nw && a->nw != c->wd.n)
         return fail(GAM_E_STATE, "gam_adj_backward: the inflow series changed length since gam_adj_create");
+    if (a->dtMode == 2 && !a->dtrecOK)
+        return fail(GAM_E_STATE, "gam_adj_backward: dt mode 2 needs a gam_adj_forward run in that mode");
     HIPCHK(hipStreamSynchronize(s));
     HIPCHK(hipEventRecord(a->ev[2], s));
     HIPCHK(hipMemsetAsync(a->lam, 0, 3 * ADJ_SET * N3 * sizeof(double), s));
+    if (a->dtMode) HIPCHK(hipMemsetAsync(a->mu, 0, (size_t)(a->nsteps + 1) * sizeof(double), s));
+    a->dtDone = 0;
     if (a->nw) {
         HIPCHK(hipMemsetAsync(a->gwq, 0, (size_t)WIND_COPIES * a->nw * 5 * sizeof(double), s));
         HIPCHK(hipMemsetAsync(a->gwb, 0, (size_t)WIND_COPIES * a->nw * 3 * sizeof(double), s));
@@ -1738,9 +1784,11 @@ extern "C" int gam_adj_backward(gam_adj *a, double *gGas, double *gGasO, double 
                 r = fail(GAM_E_HIP, "gam_adj_backward: tape copy failed");
             if (!r && q < s1 - s0) {
                 it_use(a, q, buf, own);   // a taped step writes its intermediates into its slot
+                dt_rec_use(a, s0 + q);
                 enq_step(c);
             }
         }
+        c->dtrec = nullptr;
         if (!r && ovl && hipEventRecord(a->evR[buf], c->stream)) r = fail(GAM_E_HIP, "gam_adj_backward: event");
         c->H = own.H; c->vf = own.vf; c->gflx = own.gflx; c->mflx = own.mflx;
         c->stream = s;
@@ -1768,7 +1816,7 @@ extern "C" int gam_adj_backward(gam_adj *a, double *gGas, double *gGasO, double 
             const int q = n - s0;
             const bool tpd = it_taped(a, q);
             const StepIT its = tpd ? it_slot(a, q, buf) : StepIT{c->H, c->vf, c->gflx, c->mflx};
-            adj_step(a, tpd ? &its : nullptr, tp + (size_t)(1 + q) * CK_NPL * N3, tp + (size_t)q * CK_NPL * N3,
+            adj_step(a, n, tpd ? &its : nullptr, tp + (size_t)(1 + q) * CK_NPL * N3, tp + (size_t)q * CK_NPL * N3,
                      tp + (size_t)(2 + q) * CK_NPL * N3, a->clkh + n, a->clkh + n + 1, lA, lB, lC);
             double *t = lA;
             lA = lB;
@@ -1801,6 +1849,7 @@ extern "C" int gam_adj_backward(gam_adj *a, double *gGas, double *gGasO, double 
     HIPCHK(hipEventRecord(a->ev[3], s));
     HIPCHK(hipEventSynchronize(a->ev[3]));
     HIPCHK(hipEventElapsedTime(&a->bwd_ms, a->ev[2], a->ev[3]));
+    a->dtDone = a->dtMode;
     if ((rc = download(c, lA, 0, 5, gGas))) return rc;
     if (gGasO && (rc = download(c, lB, 0, 5, gGasO))) return rc;
     if (gMF && (rc = download(c, lA + S_MF * N3, 1, 3, gMF))) return rc;
@@ -1808,13 +1857,19 @@ extern "C" int gam_adj_backward(gam_adj *a, double *gGas, double *gGasO, double 
     return adj_restore_final(a);
 }
 
-extern "C" int gam_adj_replay(gam_adj *a, const double *Gas, const double *oGas, const double *MF,
-                              const double *oMF, double *J)
+// Replay from the recorded start with the given start arrays; dt of each
+// step: mode 0 the recorded one, 1 dts[n] (host array, nsteps values; 0
+// takes CalcDT's dt for that step, the recorded one for the first step),
+// 2 the recorded one for the first step and CalcDT's for the others.
+static int adj_replay(gam_adj *a, const double *Gas, const double *oGas, const double *MF, const double *oMF,
+                      int mode, const double *dts, double *J)
 {
-    if (!a || !Gas) return fail(GAM_E_ARG, "gam_adj_replay: NULL argument");
-    if (!a->recorded) return fail(GAM_E_STATE, "gam_adj_replay: no recorded forward run");
     gam_ctx *c = a->c;
     const long N3 = c->g.N3;
+    if (mode == 1) {
+        if (!a->dts) HIPCHK(hipMalloc((void **)&a->dts, (size_t)a->nsteps * sizeof(double)));
+        HIPCHK(hipMemcpy(a->dts, dts, (size_t)a->nsteps * sizeof(double), hipMemcpyHostToDevice));
+    }
     int rc = adj_put(a, 0, a->ck, CK_NPL);
     if (rc) return rc;
     if ((rc = upload(c, Gas, 0, 5, S_cur(c) + S_GAS * N3))) return rc;
@@ -1834,13 +1889,78 @@ extern "C" int gam_adj_replay(gam_adj *a, const double *Gas, const double *oGas,
     const int prof = c->prof;
     c->prof = 0;
     for (int n = 0; n < a->nsteps; n++) {
+        if (mode == 1 && dts[n] > 0.0) launch_set_dt_val(c->stream, c->clk, a->dts, n);
         enq_step(c);
-        launch_set_dt(c->stream, c->clk, a->clkh + n + 1);
+        if (mode == 0) launch_set_dt(c->stream, c->clk, a->clkh + n + 1);
     }
     c->prof = prof;
     HIPCHK(hipGetLastError());
     if ((rc = adj_objective(a, S_cur(c), a->lam + 2 * ADJ_SET * N3, J))) return rc;
     return adj_restore_final(a);
+}
+
+extern "C" int gam_adj_replay(gam_adj *a, const double *Gas, const double *oGas, const double *MF,
+                              const double *oMF, double *J)
+{
+    if (!a || !Gas) return fail(GAM_E_ARG, "gam_adj_replay: NULL argument");
+    if (!a->recorded) return fail(GAM_E_STATE, "gam_adj_replay: no recorded forward run");
+    return adj_replay(a, Gas, oGas, MF, oMF, 0, nullptr, J);
+}
+
+extern "C" int gam_adj_replay_dt(gam_adj *a, const double *Gas, const double *oGas, const double *MF,
+                                 const double *oMF, const double *dts, double *J)
+{
+    if (!a || !Gas) return fail(GAM_E_ARG, "gam_adj_replay_dt: NULL argument");
+    if (!a->recorded) return fail(GAM_E_STATE, "gam_adj_replay_dt: no recorded forward run");
+    if (dts)
+        for (int n = 0; n < a->nsteps; n++)
+            if (!(dts[n] >= 0.0)) return fail(GAM_E_ARG, "gam_adj_replay_dt: dts[%d] is negative or NaN", n);
+    return adj_replay(a, Gas, oGas, MF, oMF, dts ? 1 : 2, dts, J);
+}
+
+extern "C" int gam_adj_set_dt_mode(gam_adj *a, int mode)
+{
+    if (!a) return fail(GAM_E_ARG, "gam_adj_set_dt_mode: NULL argument");
+    if (mode < 0 || mode > 2) return fail(GAM_E_ARG, "gam_adj_set_dt_mode: mode must be 0, 1 or 2");
+    if (mode && !a->pdt) {
+        const size_t pl = (size_t)a->c->g.N3 * sizeof(double);
+        hipError_t e = hipMalloc((void **)&a->pdt, 2 * pl);
+        if (e == hipSuccess) e = hipMalloc((void **)&a->mu, (size_t)(a->nsteps + 1) * sizeof(double));
+        if (e == hipSuccess) e = hipMalloc((void **)&a->dtpart, 2 * (size_t)dt_part_blocks(a->c->g) * sizeof(double));
+        if (e == hipSuccess) e = hipMalloc((void **)&a->dtrec, (size_t)a->nsteps * DT_REC * sizeof(double));
+        if (e == hipSuccess)
+            e = hipMalloc((void **)&a->dtrpart, 2 * (size_t)dt_record_blocks(a->c->g, dt_box(a->c->g)) * sizeof(double));
+        if (e != hipSuccess) {
+            for (double **b : {&a->pdt, &a->mu, &a->dtpart, &a->dtrec, &a->dtrpart}) {
+                if (*b) hipFree(*b);
+                *b = nullptr;
+            }
+            return fail(GAM_E_NOMEM, "gam_adj_set_dt_mode: %s", hipGetErrorString(e));
+        }
+    }
+    if (mode == 2 && a->dtMode != 2) a->dtrecOK = 0;   // the argmins are recorded by the next forward
+    a->dtMode = mode;
+    a->dtDone = 0;
+    return GAM_OK;
+}
+
+extern "C" int gam_adj_step_dt(gam_adj *a, double *dts)
+{
+    if (!a || !dts) return fail(GAM_E_ARG, "gam_adj_step_dt: NULL argument");
+    if (!a->recorded) return fail(GAM_E_STATE, "gam_adj_step_dt: no recorded forward run");
+    std::vector<Clock> h(a->nsteps);
+    HIPCHK(hipMemcpy(h.data(), a->clkh, (size_t)a->nsteps * sizeof(Clock), hipMemcpyDeviceToHost));
+    for (int n = 0; n < a->nsteps; n++) dts[n] = h[n].dtA;
+    return GAM_OK;
+}
+
+extern "C" int gam_adj_dt_grad(gam_adj *a, double *g)
+{
+    if (!a || !g) return fail(GAM_E_ARG, "gam_adj_dt_grad: NULL argument");
+    if (!a->dtDone) return fail(GAM_E_STATE, "gam_adj_dt_grad: no backward sweep with the dt sensitivities on");
+    HIPCHK(hipStreamSynchronize(a->c->stream));
+    HIPCHK(hipMemcpy(g, a->mu, (size_t)a->nsteps * sizeof(double), hipMemcpyDeviceToHost));
+    return GAM_OK;
 }
 
 extern "C" int gam_adj_wind_grad(gam_adj *a, double *gQ, double *gB)

@@ -116,14 +116,36 @@ void launch_dipole_b0(hipStream_t s, const Geo &g, double M0, const double *met,
 void launch_adj_objective(hipStream_t s, const Geo &g, const MParm &m, const double *S,
                           const double *wgt, double *aS, double *part, int *nparts);
 void launch_bc_adj(hipStream_t s, int side, int kind, const Geo &g, double *aS);
+// pdt (may be null): the dt-sensitivity plane (d/d(dt) shares, see
+// launch_dt_fold); pw: the AB2-weight plane
 void launch_update_adj(hipStream_t s, const Geo &g, const Box &ab, const Clock *clk, const double *aSn,
-                       double *aSo, double *ag, const double *met);
+                       double *aSo, double *ag, const double *met, const double *gflx, double *pdt);
 void launch_flux_adj(hipStream_t s, int d, const Geo &g, const MParm &m, const Box &fb, const double *H,
                      const double *met, const double *ag, double *aWH, double *aWN);
 void launch_predict_adj(hipStream_t s, const Geo &g, const MParm &m, const Clock *clk, const double *S,
                         const double *O, const double *met, const double *aWH, const double *aWN,
-                        double *aS, double *aO);
+                        double *aS, double *aO, double *pw);
 void launch_set_dt(hipStream_t s, Clock *clk, const Clock *rec);
+// dt = dtA = dts[n] on the device clock (replays with a given dt sequence)
+void launch_set_dt_val(hipStream_t s, Clock *clk, const double *dts, int n);
+// Reduce one adjoint step's dt planes (pdt, pw over the face box) into
+// mu[n] (and mu[n-1] through the AB2 weight's odt); part holds 2 doubles
+// per 64 x 4 block of the face box (dt_part_blocks).
+long dt_part_blocks(const Geo &g);
+void launch_dt_fold(hipStream_t s, const Geo &g, const double *pdt, const double *pw, double *part,
+                    const Clock *clk, int n, double *mu);
+// CalcDT's argmin over the DT box of S (the state the update has just
+// written, before the boundary pass), recorded with its values as
+// DT_REC doubles {cell, gas 5, total cell B 3, dl, valid} when its CellDT
+// is the dt clk holds (valid = 0 where tFin or 10 dt0 capped it); part: 2
+// doubles per 64 x 4 block of dtb (dt_record_blocks)
+#define DT_REC 12
+long dt_record_blocks(const Geo &g, const Box &dtb);
+void launch_dt_record(hipStream_t s, const Geo &g, const MParm &m, const Box &dtb, const double *S,
+                      const double *met, const double *bg, const Clock *clk, double *part, double *rec);
+// CalcDT transpose: lam += mu * d(CellDT)/d(gas, cell B) at rec's cell
+void launch_dt_inject(hipStream_t s, const Geo &g, const MParm &m, const double *rec, const double *mu,
+                      double *lam);
 void launch_cellb_adj(hipStream_t s, const Geo &g, int all, double *aS, const double *met);
 void launch_bc_periodic_adj(hipStream_t s, int side, const Geo &g, double *aS);
 void launch_bc_mhd_adj(hipStream_t s, int side, int kind, const Geo &g, double *aS, const double *met);
@@ -136,7 +158,7 @@ void launch_update_adj_mhd(hipStream_t s, const Geo &g, const MParm &m, const Bo
                            const Box *eboxes, const Clock *clk, const double *So, const double *Sn1,
                            const double *gflx, const double *mflx, const double *met, const double *bg,
                            double *aSn, double *aSo, double *aUr, double *adM, double *ag, double *amf,
-                           double *aE);
+                           double *aE, double *pdt);
 // pass 1 of the flux transpose: per face, the stencil scalars and limiter
 // adjoints of its 8 reconstructions into sc (48 planes: per variable the
 // pair (sl, sr) interleaved over two planes, then the four limiter
@@ -152,7 +174,7 @@ void launch_recon_adj_gather(hipStream_t s, const Geo &g, const MParm &m, const 
 // pass 1 of the edge EMF transpose: 16 planes per edge direction into sc
 void launch_emf_adj(hipStream_t s, int ed, const Geo &g, const MParm &m, const Box &eb, const Clock *clk,
                     const double *H, const double *S, const double *Vf, const double *met, const double *bg,
-                    const double *aE, double *sc);
+                    const double *aE, double *sc, double *pdt);
 // pass 2: writes aVf, aHMF and aHG at every point (adds the limiter parts
 // to aSSMF with the LFM limiter)
 void launch_emf_adj_gather(hipStream_t s, const Geo &g, const MParm &m, const Box *eboxes, const double *sc,
@@ -160,4 +182,4 @@ void launch_emf_adj_gather(hipStream_t s, const Geo &g, const MParm &m, const Bo
 void launch_vface_rv(hipStream_t s, const Geo &g, const double *met, double *rv);
 void launch_predict_adj_mhd(hipStream_t s, const Geo &g, const MParm &m, const Clock *clk, const double *S,
                             const double *O, const double *met, const double *aHG, const double *aWH,
-                            const double *aWN, double *aHB, double *aHMF, double *aS, double *aO);
+                            const double *aWN, double *aHB, double *aHMF, double *aS, double *aO, double *pw);

@@ -259,6 +259,24 @@ module gamhip
             type(c_ptr), value :: adj, Gas, oGas, MF, oMF
             real(c_double) :: J
         end function
+        integer(c_int) function gam_adj_replay_dt(adj, Gas, oGas, MF, oMF, dts, J) bind(C, name="gam_adj_replay_dt")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: adj, Gas, oGas, MF, oMF, dts
+            real(c_double) :: J
+        end function
+        integer(c_int) function gam_adj_set_dt_mode(adj, mode) bind(C, name="gam_adj_set_dt_mode")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: adj
+            integer(c_int), value :: mode
+        end function
+        integer(c_int) function gam_adj_dt_grad(adj, g) bind(C, name="gam_adj_dt_grad")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: adj, g
+        end function
+        integer(c_int) function gam_adj_step_dt(adj, dts) bind(C, name="gam_adj_step_dt")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: adj, dts
+        end function
         integer(c_int) function gam_adj_wind_grad(adj, gQ, gB) bind(C, name="gam_adj_wind_grad")
             import :: c_int, c_ptr
             type(c_ptr), value :: adj, gQ, gB

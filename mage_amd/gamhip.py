@@ -409,7 +409,8 @@ class GamEnsemble:
 ADJ_EXPORTS = ("gam_adj_create", "gam_adj_create_budget", "gam_adj_tape_info", "gam_adj_set_ostate_bc_time",
                "gam_adj_destroy",
                "gam_adj_set_weights", "gam_adj_forward", "gam_adj_backward", "gam_adj_replay",
-               "gam_adj_wind_grad", "gam_adj_times")
+               "gam_adj_wind_grad", "gam_adj_times", "gam_adj_set_dt_mode", "gam_adj_dt_grad",
+               "gam_adj_replay_dt", "gam_adj_step_dt")
 
 
 def _load_adj(L):
@@ -425,6 +426,10 @@ def _load_adj(L):
     L.gam_adj_replay.argtypes = [vp, DP, DP, DP, DP, DP]
     L.gam_adj_wind_grad.argtypes = [vp, DP, DP]
     L.gam_adj_times.argtypes = [vp, DP, DP]
+    L.gam_adj_set_dt_mode.argtypes = [vp, C.c_int]
+    L.gam_adj_dt_grad.argtypes = [vp, DP]
+    L.gam_adj_replay_dt.argtypes = [vp, DP, DP, DP, DP, DP, DP]
+    L.gam_adj_step_dt.argtypes = [vp, DP]
     return L
 
 
@@ -497,6 +502,33 @@ class GamAdjoint:
         J = C.c_double()
         arr = [None if x is None else np.asfortranarray(x, dtype=np.float64) for x in (Gas, oGas, MF, oMF)]
         self._chk(self.L.gam_adj_replay(self.adj, *[_p(x) for x in arr], C.byref(J)))
+        return J.value
+
+    def set_dt_mode(self, mode):
+        """0: dt of every step held (default); 1: also dJ/d(dt_n) with the
+        other steps' dt held; 2: dt after the first step follows CalcDT of
+        the state (gradients of that run, dJ/d(dt_n) total)."""
+        self._chk(self.L.gam_adj_set_dt_mode(self.adj, int(mode)))
+
+    def dt_grad(self, nsteps):
+        """dJ/d(dt_n), n = 0..nsteps-1, of the last backward."""
+        g = np.zeros(nsteps)
+        self._chk(self.L.gam_adj_dt_grad(self.adj, _p(g)))
+        return g
+
+    def step_dt(self, nsteps):
+        """dt of each step of the recorded run."""
+        d = np.zeros(nsteps)
+        self._chk(self.L.gam_adj_step_dt(self.adj, _p(d)))
+        return d
+
+    def replay_dt(self, Gas, oGas=None, MF=None, oMF=None, dts=None):
+        """replay with the dt of step n = dts[n], or (dts None) the recorded
+        first dt and CalcDT's after it."""
+        J = C.c_double()
+        arr = [None if x is None else np.asfortranarray(x, dtype=np.float64) for x in (Gas, oGas, MF, oMF)]
+        d = None if dts is None else np.ascontiguousarray(dts, dtype=np.float64)
+        self._chk(self.L.gam_adj_replay_dt(self.adj, *[_p(x) for x in arr], _p(d), C.byref(J)))
         return J.value
 
     def wind_grad(self, n):

@@ -227,6 +227,74 @@ void a_pred(double g, int lfm, double w, double vol, const double *x, const doub
     for (int n = 0; n < 10; n++) ax[n] = 0;
     predict_cell_adj(m, x, x + 5, w, vol, a, a + 5, a + 10, ax, ax + 5);
 }
+// the predictor with its AB2 weight as an input: x = Q[5], oQ[5], w
+void f_predw(double g, int lfm, double vol, const double *x, double *y) { f_pred(g, lfm, x[10], vol, x, y); }
+void a_predw(double g, int lfm, double vol, const double *x, const double *a, double *ax)
+{
+    MParm m = mp(g, 1e-12, 1e-12, lfm);
+    for (int n = 0; n < 11; n++) ax[n] = 0;
+    predict_cell_adj(m, x, x + 5, x[10], vol, a, a + 5, a + 10, ax, ax + 5, ax + 10);
+}
+// CellDT: x = U[5], B[3] (total cell B), dl = 0.7
+static MParm mcdt(int mhd)
+{
+    MParm m = mp(5.0 / 3.0, 1e-12, 1e-12, 1); m.doMHD = mhd; m.CFL = 0.3; m.Ca = 0.8;
+    return m;
+}
+void f_cdt(int mhd, int boris, const double *x, double *y)
+{
+    MParm m = mcdt(mhd);
+    y[0] = boris ? cell_dt<true>(m, x, x[5], x[6], x[7], 0.7) : cell_dt<false>(m, x, x[5], x[6], x[7], 0.7);
+}
+void a_cdt(int mhd, int boris, const double *x, const double *a, double *ax)
+{
+    MParm m = mcdt(mhd);
+    for (int n = 0; n < 8; n++) ax[n] = 0;
+    if (boris) cell_dt_adj<true>(m, x, x[5], x[6], x[7], 0.7, a[0], ax, ax + 5);
+    else cell_dt_adj<false>(m, x, x[5], x[6], x[7], 0.7, a[0], ax, ax + 5);
+}
+// the edge EMF's value and dE/d(cap) (aE = 1) from its transpose
+void a_emfcap(int rm, int boris, const double *geo, const double *x, double *out)
+{
+    MParm m = mp(5.0 / 3.0, 1e-12, 1e-12, 1); m.pdmb = 0.75; m.Vd0 = 0.5; m.Ca = 2.0;
+    EdgeIn in; edge_fill(in, geo, x);
+    EdgeAdjS o;
+    if (rm) { if (boris) edge_emf_adj_s<1, true>(m, in, 1.0, o); else edge_emf_adj_s<1, false>(m, in, 1.0, o); }
+    else { if (boris) edge_emf_adj_s<0, true>(m, in, 1.0, o); else edge_emf_adj_s<0, false>(m, in, 1.0, o); }
+    out[0] = o.E; out[1] = o.acap;
+}
+// the update's dt share as the kernels form it: (dH . adH + dM . adM) / dt
+// from the transposes, for a cell updated with dtM = dt = t.
+// Maxwell: x = oU[5], dH[5], dM[3]; U' = Maxwell(oU + t dH, dM, t)
+void f_maxt(double g, double t, const double *x, double *y)
+{
+    double u[8];
+    for (int v = 0; v < 5; v++) u[v] = x[v] + t * x[5 + v];
+    for (int q = 0; q < 3; q++) u[5 + q] = x[10 + q];
+    f_maxw(g, t, u, y);
+}
+double d_maxt(double g, double t, const double *x, const double *a)
+{
+    double u[8], au[8];
+    for (int v = 0; v < 5; v++) u[v] = x[v] + t * x[5 + v];
+    for (int q = 0; q < 3; q++) u[5 + q] = x[10 + q];
+    a_maxw(g, t, u, a, au);
+    double s = 0.0;
+    for (int v = 0; v < 5; v++) s += x[5 + v] * (t * au[v]);   // adH = dtM aU
+    for (int q = 0; q < 3; q++) s += x[10 + q] * au[5 + q];
+    return s / t;
+}
+// Boris: x as f_boris
+void f_bort(double t, const double *x, double *y) { f_boris(t, t, x, y); }
+double d_bort(double t, const double *x, const double *a)
+{
+    double ax[19];
+    a_boris(t, t, x, a, ax);
+    double s = 0.0;
+    for (int v = 0; v < 5; v++) s += x[5 + v] * ax[5 + v];
+    for (int q = 0; q < 3; q++) s += x[10 + q] * ax[10 + q];
+    return s / t;
+}
 }
 '''
 
@@ -471,6 +539,107 @@ def test_inflow_weight_direction_adjoint(lib):
         y = np.zeros(1)
         fwd(np.array([-0.5, 0.0, 0.0]), y)
         assert 0.0 <= y[0] <= 1.0
+
+
+@pytest.mark.parametrize("lfm", [0, 1])
+def test_predictor_weight_adjoint(lib, lfm):
+    """The AB2 weight's adjoint (the predictor's share of dJ/d(dt))."""
+    rng = np.random.default_rng(47 + lfm)
+    g = 5.0 / 3.0
+
+    def x0(r):
+        d = r.uniform(0.3, 2.0)
+        v = r.standard_normal(3) * 0.5
+        Q = np.array([d, d * v[0], d * v[1], d * v[2], 0.5 * d * v @ v + r.uniform(0.2, 2.0) * 1.5])
+        return np.concatenate([Q, Q * (1 + 0.05 * r.standard_normal(5)), [r.uniform(0.2, 0.8)]])
+    fwd = lambda x, y: lib.f_predw(C.c_double(g), lfm, C.c_double(1.3), _p(x), _p(y))
+    adj = lambda x, a, ax: lib.a_predw(C.c_double(g), lfm, C.c_double(1.3), _p(x), _p(a), _p(ax))
+    _check(fwd, adj, x0, 15, rng, 1e-6, True)
+
+
+@pytest.mark.parametrize("mhd,boris", [(0, 0), (1, 0), (1, 1)])
+def test_cell_dt_adjoint(lib, mhd, boris):
+    """CellDT (sound, Alfven with the Boris limit, flow speed) against its
+    transpose: the CalcDT share of the dt-following gradient."""
+    rng = np.random.default_rng(53 + mhd + 2 * boris)
+
+    def x0(r):
+        d = r.uniform(0.3, 2.0)
+        v = r.standard_normal(3) * 0.5
+        U = [d, d * v[0], d * v[1], d * v[2], 0.5 * d * v @ v + r.uniform(0.1, 2.0) * 1.5]
+        return np.concatenate([U, r.standard_normal(3)])
+    fwd = lambda x, y: lib.f_cdt(mhd, boris, _p(x), _p(y))
+    adj = lambda x, a, ax: lib.a_cdt(mhd, boris, _p(x), _p(a), _p(ax))
+    _check(fwd, adj, x0, 1, rng, 1e-6, True)
+
+
+@pytest.mark.parametrize("rm,boris", [(1, 0), (0, 1)])
+def test_edge_emf_cap_derivative(lib, rm, boris):
+    """The EMF's value as its transpose recomputes it, and dE/d(cap) where
+    the vDiff cap CFL edge / dt binds (0 where it does not)."""
+    rng = np.random.default_rng(59 + rm)
+    t1 = _triad(rng)
+    lib.f_emf.argtypes = None
+    nbind = 0
+    for q in range(200):
+        cap = rng.uniform(0.02, 3.0)
+        geo = np.concatenate([rng.uniform(0.8, 1.2, 16), t1[3:6], t1[6:9], [1.0, 0.1, 0.05, 0.9],
+                              [0.05, -0.02, 0.3, cap]])
+        V = rng.standard_normal(24) * 0.3
+        b1 = np.cumsum(rng.standard_normal(8)) * 0.2 + 1.0
+        b2 = np.cumsum(rng.standard_normal(8)) * 0.2 - 0.5
+        x = np.concatenate([V, b1, b1[2:6] + 0.02 * rng.standard_normal(4), b2,
+                            b2[2:6] + 0.02 * rng.standard_normal(4), rng.uniform(0.5, 1.5, 4)])
+        out = np.zeros(2)
+        lib.a_emfcap(rm, boris, _p(geo), _p(x), _p(out))
+        y = np.zeros(1)
+        lib.f_emf(rm, boris, _p(geo), _p(x), _p(y))
+        assert out[0] == y[0]
+        e = 1e-7 * cap
+        gp, gm = geo.copy(), geo.copy()
+        gp[29] += e
+        gm[29] -= e
+        yp, ym = np.zeros(1), np.zeros(1)
+        lib.f_emf(rm, boris, _p(gp), _p(x), _p(yp))
+        lib.f_emf(rm, boris, _p(gm), _p(x), _p(ym))
+        fd = (yp[0] - ym[0]) / (2 * e)
+        if out[1] != 0.0:
+            nbind += 1
+        # central-difference rounding: ~1e-16 |E| / e
+        tol = 1e-6 * max(abs(fd), abs(out[1])) + 4e-16 * max(abs(y[0]), 1.0) / e
+        assert abs(fd - out[1]) <= tol, (q, fd, out[1])
+    assert 20 <= nbind <= 180, nbind   # both sides of the cap were sampled
+
+
+@pytest.mark.parametrize("kind", ["maxwell", "boris"])
+def test_update_dt_share(lib, kind):
+    """The update transposes' dt share, (dH . adH + dM . adM) / dt, equals
+    the derivative of the updated gas along dt (the update depends on dt
+    only through dt dH and dt dM)."""
+    rng = np.random.default_rng(61)
+    lib.d_maxt.restype = C.c_double
+    lib.d_bort.restype = C.c_double
+    for q in range(100):
+        d = rng.uniform(0.5, 2.0)
+        v = rng.standard_normal(3) * 0.3
+        oU = [d, d * v[0], d * v[1], d * v[2], 0.5 * d * v @ v + rng.uniform(0.5, 2.0) * 1.5]
+        x = np.concatenate([oU, rng.standard_normal(5) * 0.1, rng.standard_normal(3) * 0.2,
+                            rng.standard_normal(3), rng.standard_normal(3)])
+        a = rng.standard_normal(5)
+        t = rng.uniform(0.01, 0.1)
+        e = 1e-6 * t
+        yp, ym = np.zeros(5), np.zeros(5)
+        if kind == "maxwell":
+            lib.f_maxt(C.c_double(5.0 / 3.0), C.c_double(t + e), _p(x), _p(yp))
+            lib.f_maxt(C.c_double(5.0 / 3.0), C.c_double(t - e), _p(x), _p(ym))
+            ad = lib.d_maxt(C.c_double(5.0 / 3.0), C.c_double(t), _p(x), _p(a))
+        else:
+            lib.f_bort(C.c_double(t + e), _p(x), _p(yp))
+            lib.f_bort(C.c_double(t - e), _p(x), _p(ym))
+            ad = lib.d_bort(C.c_double(t), _p(x), _p(a))
+        fd = a @ (yp - ym) / (2 * e)
+        scale = np.sum(np.abs(a * (yp - ym))) / (2 * e)
+        assert abs(fd - ad) <= 1e-7 * max(scale, 1e-300), (q, fd, ad)
 
 
 def test_adjoint_catches_a_planted_sign_error(lib):
