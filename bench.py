@@ -4,7 +4,7 @@
 Headline workload (one GPU): the 256x256x384 warped log-r spherical shell
 (25.2M cells) with the dipole background, Boris correction (Ca=10),
 7th-order upwind + PDM reconstruction and the inflow/outflow outer
-boundary, stepped with the full stepGamera sequence (AdvanceMHD, CalcDT,
+boundary fed by a time-varying seeded inflow series, stepped with the full stepGamera sequence (AdvanceMHD, CalcDT,
 EnforceBCs, bFlux2Fld).  Inputs are synthetic and resident in HBM before
 the timed region.
 
@@ -48,14 +48,14 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def setup_shell(gamhip, pb, N, rmeth="7UP", full_grid=False):
+def setup_shell(gamhip, pb, N, rmeth="7UP", full_grid=False, wind=None):
     Nip, Njp, Nkp = N
     p = pb.shell_params(rmeth)
     g = gamhip.GamHip(Nip, Njp, Nkp, p)
     xyz = pb.shell_corners(Nip, Njp, Nkp)
     g.grid_from_corners(xyz)
     g.add_dipole_b0(pb.SHELL_M0)
-    wind = pb.constant_wind()
+    wind = wind or pb.constant_wind()
     g.set_wind(wind)
     if full_grid:
         grid = g.get_grid()
@@ -361,7 +361,7 @@ def _cpu_model():
     return {k: info.get(k) for k in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core")}
 
 
-def cpu_baseline(gamhip, pb, g, N, min_seconds=12.0, max_steps=20):
+def cpu_baseline(gamhip, pb, g, N, wind, min_seconds=12.0, max_steps=20):
     """Time the oracle's speed build (C/OpenMP restatement of the reference
     step, -O3 AVX2/FMA, no fast-math) on a few steps of the headline
     workload: the grid and background of context g, the workload's
@@ -377,7 +377,7 @@ def cpu_baseline(gamhip, pb, g, N, min_seconds=12.0, max_steps=20):
     grid.update(g.get_background())
     p = pb.shell_params("7UP")
     st = pb.shell_state(*N, grid["xyzcc"])
-    o = po.Oracle(*N, p, grid, wind=pb.constant_wind(), fast=True)
+    o = po.Oracle(*N, p, grid, wind=wind, fast=True)
     o.set_state(st)
     del st
     o.init_run()
@@ -428,7 +428,8 @@ def main():
     N = tuple(int(x) for x in args.grid.split("x"))
     cells = N[0] * N[1] * N[2]
     t0 = time.time()
-    g, p, grid, st, wind = setup_shell(gamhip, pb, N)
+    # the full-size workload's time-varying seeded inflow series
+    g, p, grid, st, wind = setup_shell(gamhip, pb, N, wind=pb.varying_wind(200))
     del grid, st
     log("setup %s %.1f s" % (args.grid, time.time() - t0))
     plan = g.get_plan()
@@ -484,7 +485,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         t1 = time.time()
-        cpu = cpu_baseline(gamhip, pb, g, N)
+        cpu = cpu_baseline(gamhip, pb, g, N, wind)
         log("cpu baseline %.4e cell-updates/s on %d threads (%.1f s)" % (cpu["value"], cpu["cores"], time.time() - t1))
     g.close()
 
@@ -550,7 +551,8 @@ def main():
         "config": {
             "workload": "full-size forward: %dx%dx%d warped log-r spherical shell r=2.5..30, "
                         "theta 0.03pi..0.97pi, phi periodic; dipole B0 |B0|=1 at r=2.5; rho=1, "
-                        "p=1e-3 seeded; inflow rho=1 vx=-0.4 Bz=-0.05; 7UP+PDM, Boris Ca=10, "
+                        "p=1e-3 seeded; time-varying seeded inflow series (200 samples around rho=1 "
+                        "vx=-0.4 Bz=-0.05); 7UP+PDM, Boris Ca=10, "
                         "full stepGamera" % N,
             "grid": list(N),
             "cells": cells,
