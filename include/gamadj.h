@@ -121,9 +121,12 @@ int gam_adj_set_ostate_bc_time(gam_adj *a, int at_state_time);
  * step n leaves before its boundary pass (the minimum of CellDT over
  * is-1..ie+1 x js-1..je+1 x ks-1..ke+1, gridutils.F90:49-106), so the
  * gradients are those of the run whose dt follows the state, and
- * dJ/d(dt_n) is the total derivative (dt_0 is the caller's input).  A dt
- * the recorded run capped (tFin, 10*dt0) does not follow the state.  Mode
- * 2 takes effect at the next gam_adj_forward (its steps record CalcDT's
+ * dJ/d(dt_n) is the total derivative (dt_0 is the caller's input); 3: as
+ * 2, and dt_0 is CalcDT of the start State as gam_adj_forward finds it
+ * (after its boundary pass, as gam_init_run computes it), so the
+ * gradients are those of a run that starts from gam_init_run.  A dt
+ * the recorded run capped (tFin, 10*dt0) does not follow the state.  Modes
+ * 2 and 3 take effect at the next gam_adj_forward (its steps record CalcDT's
  * argmin cell and values before the boundary pass, the state that exists
  * only inside the step); modes 0 and 1 at the next gam_adj_backward.  The
  * first call with mode > 0 allocates 2 planes of the grid. */
@@ -134,10 +137,11 @@ int gam_adj_dt_grad(gam_adj *a, double *g);
 /* dt of each step of the recorded run, dts[n], n = 0..nsteps-1. */
 int gam_adj_step_dt(gam_adj *a, double *dts);
 /* gam_adj_replay with the dt of step n = dts[n] (nsteps values, each >= 0;
- * a 0 takes CalcDT's dt of the state before that step, or the recorded dt
- * for the first step), or, with dts NULL, the recorded dt for the first
- * step and CalcDT's for each later one (the forward the dt mode 2
- * gradients belong to). */
+ * a 0 takes CalcDT's dt: of the state the step before left, or, for the
+ * first step, of the start after its boundary pass, as gam_init_run does),
+ * or, with dts NULL, the recorded dt for the first step and CalcDT's for
+ * each later one (the forward the dt mode 2 gradients belong to; all zeros
+ * is the forward of mode 3). */
 int gam_adj_replay_dt(gam_adj *a, const double *Gas, const double *oGas, const double *MF,
                       const double *oMF, const double *dts, double *J);
 /* Milliseconds of the last gam_adj_forward and gam_adj_backward calls

@@ -16,7 +16,11 @@ dt = CalcDT), steps ``nsteps`` times recording checkpoints every
 adjoint; because State and oState both start from ``gas0``, the gradient
 is the sum of the two adjoints, taken with respect to the active cells
 (ghost cells follow through the boundary conditions; the gradient's ghost
-entries are 0).  Each step keeps the dt of the forward run.
+entries are 0).  By default each step keeps the dt of the forward run (the
+gradient of the forward with that dt sequence held).  With
+``dt_follows=True`` the adjoint also differentiates every dt: the set-up's
+CalcDT of gas0 and the CalcDT after each step (gam_adj_set_dt_mode 3), so
+the gradient is that of this forward, whose dt follows the state.
 
 With ``wind=True`` the call takes the inflow series too, ``Q`` (n, 5)
 primitives and ``B`` (n, 3), as given to ``gam_set_wind``, and the backward
@@ -72,7 +76,7 @@ class _StepFn(torch.autograd.Function):
 
 
 class StepObjective:
-    def __init__(self, g, nsteps, ckpt_every, weights, wind=False):
+    def __init__(self, g, nsteps, ckpt_every, weights, wind=False, dt_follows=False):
         self.g = g
         self.template = g.get_state(gamhip.GAM_STATE)
         self.t0 = self.template["time"]
@@ -80,6 +84,8 @@ class StepObjective:
         self.adj = gamhip.GamAdjoint(g, nsteps, ckpt_every)
         self.adj.set_weights(weights)
         self.adj.set_ostate_bc_time(True)
+        if dt_follows:
+            self.adj.set_dt_mode(3)
         self.wind = bool(wind)
         self.windspec = None
         if self.wind:

@@ -1381,22 +1381,24 @@ struct gam_adj {
     hipEvent_t ev[4];
     float fwd_ms, bwd_ms;
     // dt sensitivities (gam_adj_set_dt_mode): 0 off, 1 dJ/d(dt_n) with the
-    // other steps' dt held, 2 also through CalcDT (each dt follows the state)
+    // other steps' dt held, 2 also through CalcDT (each dt follows the state),
+    // 3 the first dt too (CalcDT of the start State)
     int dtMode, dtDone;
     double *mu;              // dJ/d(dt_n), n = 0..nsteps
     double *pdt;             // the transposes' per-point shares: pdt, then pw (2 planes)
     double *dtpart;          // block partial sums (2 per block of the face box)
     double *dts;             // a replay's dt sequence (nsteps)
-    double *dtrec;           // each step's CalcDT argmin record (DT_REC doubles; mode 2)
+    double *dtrec;           // each step's CalcDT argmin record (DT_REC doubles; mode >= 2),
+                             // then the start State's (mode 3)
     double *dtrpart;         // block partials of the forward steps' argmin scans
-    int dtrecOK;             // the recorded forward wrote dtrec
+    int dtrecMode;           // dt mode of the recorded forward
 };
 
 // Step n of a forward run records its CalcDT argmin (dt mode 2).
 static void dt_rec_use(gam_adj *a, int n)
 {
     gam_ctx *c = a->c;
-    c->dtrec = a->dtMode == 2 ? a->dtrec + (size_t)n * DT_REC : nullptr;
+    c->dtrec = a->dtMode >= 2 ? a->dtrec + (size_t)n * DT_REC : nullptr;
     c->dtrec_part = a->dtrpart;
 }
 
@@ -1590,6 +1592,10 @@ extern "C" int gam_adj_forward(gam_adj *a, double *J)
     // the last interval's states (and intermediates) go straight onto the
     // tapes, so the reverse sweep starts without recomputing it
     const int sL = (a->nck - 1) * a->K;
+    // mode 3: the first dt is CalcDT of the start State as it is now
+    if (a->dtMode == 3)
+        launch_dt_record(c->stream, c->g, c->mp, dt_box(c->g), S_cur(c), c->met, c->bg, c->clk, a->dtrpart,
+                         a->dtrec + (size_t)a->nsteps * DT_REC);
     const StepIT own = {c->H, c->vf, c->gflx, c->mflx};
     for (int n = 0; n < a->nsteps; n++) {
         if (n % a->K == 0) {
@@ -1609,7 +1615,7 @@ extern "C" int gam_adj_forward(gam_adj *a, double *J)
         HIPCHK(hipMemcpyAsync(a->clkh + n + 1, c->clk, sizeof(Clock), hipMemcpyDeviceToDevice, c->stream));
     }
     c->dtrec = nullptr;
-    a->dtrecOK = a->dtMode == 2;
+    a->dtrecMode = a->dtMode;
     c->H = own.H; c->vf = own.vf; c->gflx = own.gflx; c->mflx = own.mflx;
     HIPCHK(hipMemcpyAsync(a->tape + (size_t)(1 + a->nsteps - sL) * CK_NPL * N3, S_cur(c), ckb,
                           hipMemcpyDeviceToDevice, c->stream));
@@ -1676,7 +1682,7 @@ static void adj_step(gam_adj *a, int n, const StepIT *it, const double *Sn, cons
     if (pdt) hipMemsetAsync(pdt, 0, 2 * pl, s);
     // CalcDT at the end of step n (before its boundary pass) gave dt[n + 1]
     auto dt_inject = [&]() {
-        if (a->dtMode == 2 && n + 1 < a->nsteps)
+        if (a->dtMode >= 2 && n + 1 < a->nsteps)
             launch_dt_inject(s, g, c->mp, a->dtrec + (size_t)n * DT_REC, a->mu + n + 1, lA);
     };
     Box eb[3];
@@ -1750,8 +1756,8 @@ extern "C" int gam_adj_backward(gam_adj *a, double *gGas, double *gGasO, double 
     double *lA = a->lam, *lB = a->lam + ADJ_SET * N3, *lC = a->lam + 2 * ADJ_SET * N3;
     if (a->nw && a->nw != c->wd.n)
         return fail(GAM_E_STATE, "gam_adj_backward: the inflow series changed length since gam_adj_create");
-    if (a->dtMode == 2 && !a->dtrecOK)
-        return fail(GAM_E_STATE, "gam_adj_backward: dt mode 2 needs a gam_adj_forward run in that mode");
+    if (a->dtMode >= 2 && a->dtrecMode != a->dtMode)
+        return fail(GAM_E_STATE, "gam_adj_backward: dt mode %d needs a gam_adj_forward run in that mode", a->dtMode);
     HIPCHK(hipStreamSynchronize(s));
     HIPCHK(hipEventRecord(a->ev[2], s));
     HIPCHK(hipMemsetAsync(a->lam, 0, 3 * ADJ_SET * N3 * sizeof(double), s));
@@ -1829,6 +1835,9 @@ extern "C" int gam_adj_backward(gam_adj *a, double *gGas, double *gGasO, double 
     // pair 0 still holds the last interval only if no other went onto it
     a->lastTaped = (a->nck == 1);
     c->prof = prof;
+    // mode 3: the first dt was CalcDT of the start State (after its boundary
+    // pass): its share goes to that state before its ghosts are folded
+    if (a->dtMode == 3) launch_dt_inject(s, g, c->mp, a->dtrec + (size_t)a->nsteps * DT_REC, a->mu, lA);
     // the start states' ghost cells follow from their interiors through the
     // boundary conditions, and their cell B from the face fluxes (in the
     // order of gam_adj_replay: active cell B, the boundary pass, every
@@ -1859,8 +1868,9 @@ extern "C" int gam_adj_backward(gam_adj *a, double *gGas, double *gGasO, double 
 
 // Replay from the recorded start with the given start arrays; dt of each
 // step: mode 0 the recorded one, 1 dts[n] (host array, nsteps values; 0
-// takes CalcDT's dt for that step, the recorded one for the first step),
-// 2 the recorded one for the first step and CalcDT's for the others.
+// takes CalcDT's dt: of the state the step before left, or for the first
+// step of the start after its boundary pass, as gam_init_run does), 2 the
+// recorded one for the first step and CalcDT's for the others.
 static int adj_replay(gam_adj *a, const double *Gas, const double *oGas, const double *MF, const double *oMF,
                       int mode, const double *dts, double *J)
 {
@@ -1885,6 +1895,10 @@ static int adj_replay(gam_adj *a, const double *Gas, const double *oGas, const d
     if (c->mp.doMHD) {       // cell B from the face fluxes
         enq_bflux2fld(c, S_cur(c));
         enq_bflux2fld(c, S_old(c));
+    }
+    if (mode == 1 && dts[0] == 0.0) {   // the first dt: CalcDT of the start, as gam_init_run
+        launch_calc_dt(c->stream, c->g, c->mp, dt_box(c->g), c->clk, S_cur(c), c->met, c->bg);
+        launch_finalize(c->stream, c->mp, c->clk, FIN_CALCDT | FIN_SETDT | FIN_INIT);
     }
     const int prof = c->prof;
     c->prof = 0;
@@ -1921,13 +1935,13 @@ extern "C" int gam_adj_replay_dt(gam_adj *a, const double *Gas, const double *oG
 extern "C" int gam_adj_set_dt_mode(gam_adj *a, int mode)
 {
     if (!a) return fail(GAM_E_ARG, "gam_adj_set_dt_mode: NULL argument");
-    if (mode < 0 || mode > 2) return fail(GAM_E_ARG, "gam_adj_set_dt_mode: mode must be 0, 1 or 2");
+    if (mode < 0 || mode > 3) return fail(GAM_E_ARG, "gam_adj_set_dt_mode: mode must be 0 .. 3");
     if (mode && !a->pdt) {
         const size_t pl = (size_t)a->c->g.N3 * sizeof(double);
         hipError_t e = hipMalloc((void **)&a->pdt, 2 * pl);
         if (e == hipSuccess) e = hipMalloc((void **)&a->mu, (size_t)(a->nsteps + 1) * sizeof(double));
         if (e == hipSuccess) e = hipMalloc((void **)&a->dtpart, 2 * (size_t)dt_part_blocks(a->c->g) * sizeof(double));
-        if (e == hipSuccess) e = hipMalloc((void **)&a->dtrec, (size_t)a->nsteps * DT_REC * sizeof(double));
+        if (e == hipSuccess) e = hipMalloc((void **)&a->dtrec, (size_t)(a->nsteps + 1) * DT_REC * sizeof(double));
         if (e == hipSuccess)
             e = hipMalloc((void **)&a->dtrpart, 2 * (size_t)dt_record_blocks(a->c->g, dt_box(a->c->g)) * sizeof(double));
         if (e != hipSuccess) {
@@ -1938,8 +1952,7 @@ extern "C" int gam_adj_set_dt_mode(gam_adj *a, int mode)
             return fail(GAM_E_NOMEM, "gam_adj_set_dt_mode: %s", hipGetErrorString(e));
         }
     }
-    if (mode == 2 && a->dtMode != 2) a->dtrecOK = 0;   // the argmins are recorded by the next forward
-    a->dtMode = mode;
+    a->dtMode = mode;   // modes 2 and 3 take the argmins the next forward records
     a->dtDone = 0;
     return GAM_OK;
 }

@@ -507,7 +507,9 @@ class GamAdjoint:
     def set_dt_mode(self, mode):
         """0: dt of every step held (default); 1: also dJ/d(dt_n) with the
         other steps' dt held; 2: dt after the first step follows CalcDT of
-        the state (gradients of that run, dJ/d(dt_n) total)."""
+        the state (gradients of that run, dJ/d(dt_n) total); 3: the first
+        dt too (CalcDT of the start State, as after init_run).  Modes 2
+        and 3 take effect at the next forward."""
         self._chk(self.L.gam_adj_set_dt_mode(self.adj, int(mode)))
 
     def dt_grad(self, nsteps):
@@ -523,8 +525,9 @@ class GamAdjoint:
         return d
 
     def replay_dt(self, Gas, oGas=None, MF=None, oMF=None, dts=None):
-        """replay with the dt of step n = dts[n], or (dts None) the recorded
-        first dt and CalcDT's after it."""
+        """replay with the dt of step n = dts[n] (0: CalcDT's, for the
+        first step of the start after its boundary pass), or (dts None) the
+        recorded first dt and CalcDT's after it."""
         J = C.c_double()
         arr = [None if x is None else np.asfortranarray(x, dtype=np.float64) for x in (Gas, oGas, MF, oMF)]
         d = None if dts is None else np.ascontiguousarray(dts, dtype=np.float64)

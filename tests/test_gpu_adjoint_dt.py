@@ -16,6 +16,12 @@ the caller's input in both.  The mode-0 gradient (every dt held) is
 checked against the same free-dt differences, and must miss them: the dt
 path carries a real share of the derivative.
 
+Mode 3: mode 2 with the first dt also CalcDT of the start State (after
+its boundary pass, as gam_init_run computes it): against replays whose
+every dt is CalcDT's, and, through the torch bridge (StepObjective with
+dt_follows=True, whose forward runs gam_init_run), against central
+differences of the bridge's own forward.
+
 Cases: the MHD shell (seeded flow and residual field, dipole background,
 Boris, 7UP, time-varying inflow; J = pressure integral over r >= 12), and
 the hydrodynamic and the MHD periodic boxes (J = a smoothly weighted
@@ -227,4 +233,86 @@ def test_dt_following_gradient_matches_free_dt_differences(case):
     # the held-dt gradient is not the gradient of this run
     assert max(gap) >= 10.0, gap
     adj.close()
+    g.close()
+
+
+def _directional(adj, grad, gas0, d, replay, gd_floor=1e-7):
+    """Best (relative, of sum |g d|) agreement of <grad, d> with central
+    differences of replay over a few steps."""
+    a = float(np.sum(grad * d))
+    gd = float(np.sum(np.abs(grad * d)))
+    best = None
+    for e in (1e-3, 1e-4, 1e-5, 1e-6):
+        fd = (replay(gas0 + e * d) - replay(gas0 - e * d)) / (2 * e)
+        c = (abs(fd - a) / max(abs(fd), abs(a)), abs(fd - a) / gd)
+        best = c if best is None or c[0] < best[0] else best
+    return a, best
+
+
+@pytest.mark.parametrize("case", ["shell", "box"])
+def test_dt_from_the_start_state(case):
+    g, o, grid, st, p, w, N, wind = _case(case)
+    print("dt mode 3, %s %s, %d steps" % (case, N, NSTEPS), flush=True)
+    gas0 = np.asfortranarray(st["Gas"])
+    g.init_run()
+    if p["doMHD"]:
+        # the replay's start form: every cell's B from the face fluxes; the
+        # first dt is CalcDT of that state
+        g.bflux2fld(gamhip.GAM_STATE)
+        g.bflux2fld(gamhip.GAM_OSTATE)
+        t, _, ts = g.get_time()
+        g.set_time(t, g.calc_dt(), ts)
+    adj = gamhip.GamAdjoint(g, NSTEPS, 2)
+    adj.set_ostate_bc_time(True)
+    adj.set_weights(w)
+    adj.set_dt_mode(3)
+    J = adj.forward()
+    gS, gO = adj.backward()
+    grad3 = gS + gO
+    zeros = np.zeros(NSTEPS)
+    assert adj.replay_dt(gas0, gas0, None, None, zeros) == pytest.approx(J, rel=1e-13)
+    rng = np.random.default_rng({"shell": 787, "box": 788}[case])
+    scale = np.max(np.abs(gas0), axis=(0, 1, 2))
+    for q in range(3):
+        d = _smooth(gas0.shape, rng, scale)
+        a, best = _directional(adj, grad3, gas0, d, lambda x: adj.replay_dt(x, x, None, None, zeros))
+        print("%s mode 3 direction %d: adjoint %.12e, best rel vs all-CalcDT FD %.2e (%.1e of sum|g d|)"
+              % (case, q, a, best[0], best[1]), flush=True)
+        assert best[0] <= 1e-6 or best[1] <= 1e-7, (case, q, best)
+    adj.close()
+    g.close()
+
+
+def test_torch_bridge_with_dt_following_the_state():
+    import torch
+    from mage_amd.autograd import StepObjective
+    g, o, grid, st, p, w, N, wind = _case("box")
+    g.init_run()
+    base = np.asfortranarray(st["Gas"])
+    obj = StepObjective(g, NSTEPS, 2, w, dt_follows=True)
+    held = StepObjective(g, NSTEPS, 2, w)   # made from the same start
+    gas0 = torch.from_numpy(np.ascontiguousarray(base)).requires_grad_(True)
+    J = obj(gas0)
+    J.backward()
+    grad = gas0.grad.numpy()
+    gas1 = torch.from_numpy(np.ascontiguousarray(base)).requires_grad_(True)
+    held(gas1).backward()
+    grad0 = gas1.grad.numpy()
+
+    def J_of(x):
+        with torch.no_grad():
+            return float(obj(torch.from_numpy(np.ascontiguousarray(x))))
+
+    rng = np.random.default_rng(799)
+    scale = np.max(np.abs(base), axis=(0, 1, 2))
+    for q in range(2):
+        d = _smooth(base.shape, rng, scale)
+        a, best = _directional(None, grad, base, d, J_of)
+        a0, best0 = _directional(None, grad0, base, d, J_of)
+        print("bridge dt_follows direction %d: <grad, d> %.12e best rel %.2e (%.1e of sum|g d|); "
+              "held-dt gradient %.2e" % (q, a, best[0], best[1], best0[0]), flush=True)
+        assert best[0] <= 1e-6 or best[1] <= 1e-7, (q, best)
+        assert best0[0] > 10 * best[0]
+    obj.close()
+    held.close()
     g.close()
