@@ -413,6 +413,7 @@ def main():
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-c2", action="store_true", help="skip the 192x192x256 shell line")
     ap.add_argument("--no-adj", action="store_true", help="skip both adjoint lines")
+    ap.add_argument("--no-c4-adj", action="store_true", help="skip the 96x96x128 adjoint line")
     ap.add_argument("--adj-fd", type=int, default=0,
                     help="central-difference check of the 96x96x128 adjoint with this many directions")
     ap.add_argument("--ens-members", type=int, default=32)
@@ -523,7 +524,7 @@ def main():
             % (ens["members"], ens["sequential_ms"], ens["batched_ms"], ens["gain"], time.time() - t1))
 
     adjc = None
-    if not args.no_adj:
+    if not args.no_adj and not args.no_c4_adj:
         t1 = time.time()
         adjc = adjoint_cost(gamhip, pb)
         log("adjoint: forward %.1f ms (recorded %.1f), reverse %.1f ms, ratio %.2f (%.1f s)"

@@ -54,6 +54,47 @@ static inline dim3 agrid(const Box &b)
     const int i = (b).i0 + (int)(p_ - (unsigned)(j - (b).j0) * (unsigned)ni_);              \
     const int k = (b).k0 + (int)blockIdx.y;
 
+// Point kernels whose stencils run along j and k: the box's rows (the
+// points of one (j, k) along i) in slabs of RBA j-rows, j fastest within a
+// slab, then k, then the next slab, cut into 256-point chunks numbered as
+// in POINT_IN (one contiguous run per XCD).  The rows at j +- 4 and at
+// k +- 4 of a point are then at most 4 RBA rows away in the order the
+// workgroups run, so the 8-point stencils of neighbouring workgroups meet
+// in L2; with the plane-by-plane order of POINT_IN the k neighbours lie a
+// whole (i, j) plane of workgroups away.
+#define RBA 8
+static inline dim3 agrid_slab(const Box &b)
+{
+    const long pts = (long)(b.i1 - b.i0 + 1) * (b.j1 - b.j0 + 1) * (b.k1 - b.k0 + 1);
+    const long nc = (pts + ABT - 1) / ABT;
+    return dim3((unsigned)(((nc + 7) / 8) * 8));
+}
+
+__device__ __forceinline__ void slab_row(const Box &b, unsigned nj, unsigned R, int &j, int &k)
+{
+    const unsigned nk = (unsigned)(b.k1 - b.k0 + 1);
+    const unsigned slab = RBA * nk, nfull = nj / RBA, sb = R / slab;
+    if (sb < nfull) {
+        const unsigned r = R - sb * slab;
+        k = b.k0 + (int)(r / RBA);
+        j = b.j0 + (int)(sb * RBA + r % RBA);
+    } else {
+        const unsigned r = R - nfull * slab, rj = nj - nfull * RBA;
+        k = b.k0 + (int)(r / rj);
+        j = b.j0 + (int)(nfull * RBA + r % rj);
+    }
+}
+
+#define POINT_SLAB(b)                                                                            \
+    const unsigned ni_ = (unsigned)((b).i1 - (b).i0 + 1), nj_ = (unsigned)((b).j1 - (b).j0 + 1); \
+    const unsigned L_ = (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);                \
+    const unsigned p_ = L_ * ABT + threadIdx.x;                                                  \
+    if (p_ >= ni_ * nj_ * (unsigned)((b).k1 - (b).k0 + 1)) return;                               \
+    const unsigned R_ = p_ / ni_;                                                                \
+    const int i = (b).i0 + (int)(p_ - R_ * ni_);                                                 \
+    int j, k;                                                                                    \
+    slab_row((b), nj_, R_, j, k);
+
 // the objective's block-sum reduction keeps plain 64 x 4 blocks
 static inline dim3 agrid2(const Box &b)
 {
@@ -818,7 +859,7 @@ __global__ __launch_bounds__(256, 2) void k_flux_adj_mhd(Geo g, MParm m, Box fb,
                                                       double *__restrict__ aHMF,
                                                       double *__restrict__ aHG)
 {
-    POINT_IN(fb);
+    POINT_SLAB(fb);
     const long N3 = g.N3;
     const long c = IX(g, i, j, k);
     const long s = STRIDE<D>(g);
@@ -939,7 +980,7 @@ __global__ __launch_bounds__(256) void k_recon_adj_gather(Geo g, Box cb, const d
                                                           double *__restrict__ aWH, double *__restrict__ aWN,
                                                           double *__restrict__ aHB, double *__restrict__ aSSB)
 {
-    POINT_IN(cb);
+    POINT_SLAB(cb);
     const long N3 = g.N3;
     const long c = IX(g, i, j, k);
     double gh[8], gn[8], av[3] = {0.0, 0.0, 0.0};
@@ -1014,7 +1055,7 @@ __global__ __launch_bounds__(256, 2) void k_emf_adj(Geo g, MParm m, Box eb, cons
                                                  double *__restrict__ sc,
                                                  double *__restrict__ pdt)
 {
-    POINT_IN(eb);
+    POINT_SLAB(eb);
     constexpr int T1 = (ED == 0) ? 1 : (ED == 1) ? 2 : 0;
     constexpr int T2 = (ED == 0) ? 2 : (ED == 1) ? 0 : 1;
     constexpr int E1 = (ED == 0) ? 1 : 0;
@@ -1081,7 +1122,7 @@ __global__ __launch_bounds__(256) void k_emf_adj_gather(Geo g, Box fb, EdgeBoxes
                                                         double *__restrict__ aVf, double *__restrict__ aHMF,
                                                         double *__restrict__ aHG, double *__restrict__ aSSMF)
 {
-    POINT_IN(fb);
+    POINT_SLAB(fb);
     const long N3 = g.N3;
     const long c = IX(g, i, j, k);
     const long st[3] = {1, (long)g.P, g.PP};
@@ -1458,7 +1499,7 @@ static void flux_adj_mhd_r(hipStream_t s, const Geo &g, const MParm &m, const Bo
                            const double *amf, double *sc, double *aHMF, double *aHG)
 {
 #define FLUXADJ(BG, BO)                                                                                      \
-    hipLaunchKernelGGL((k_flux_adj_mhd<D, RM, BG, BO>), agrid(fb), dim3(ABT), 0, s, g, m, fb, H, S, met,      \
+    hipLaunchKernelGGL((k_flux_adj_mhd<D, RM, BG, BO>), agrid_slab(fb), dim3(ABT), 0, s, g, m, fb, H, S, met,      \
                        bg, ag, amf, sc, aHMF, aHG)
     if (m.doBG && m.doBoris) FLUXADJ(true, true);
     else if (m.doBG) FLUXADJ(true, false);
@@ -1491,10 +1532,10 @@ void launch_recon_adj_gather(hipStream_t s, const Geo &g, const MParm &m, const 
 {
     const Box cb = cells_of(g);
     if (m.rmeth)
-        hipLaunchKernelGGL(k_recon_adj_gather<1>, agrid(cb), dim3(ABT), 0, s, g, cb, sc, rv, aVf, met,
+        hipLaunchKernelGGL(k_recon_adj_gather<1>, agrid_slab(cb), dim3(ABT), 0, s, g, cb, sc, rv, aVf, met,
                            m.doLFM ? 1 : 0, aWH, aWN, aHB, aSSB);
     else
-        hipLaunchKernelGGL(k_recon_adj_gather<0>, agrid(cb), dim3(ABT), 0, s, g, cb, sc, rv, aVf, met,
+        hipLaunchKernelGGL(k_recon_adj_gather<0>, agrid_slab(cb), dim3(ABT), 0, s, g, cb, sc, rv, aVf, met,
                            m.doLFM ? 1 : 0, aWH, aWN, aHB, aSSB);
 }
 
@@ -1503,7 +1544,7 @@ static void emf_adj_t(hipStream_t s, const Geo &g, const MParm &m, const Box &eb
                       const double *H, const double *S, const double *Vf, const double *met, const double *bg,
                       const double *aE, double *sc, double *pdt)
 {
-    hipLaunchKernelGGL((k_emf_adj<ED, RM, BG, BO>), agrid(eb), dim3(ABT), 0, s, g, m, eb, clk, H, S, Vf, met,
+    hipLaunchKernelGGL((k_emf_adj<ED, RM, BG, BO>), agrid_slab(eb), dim3(ABT), 0, s, g, m, eb, clk, H, S, Vf, met,
                        bg, aE, sc + (size_t)16 * ED * g.N3, pdt);
 }
 
@@ -1543,10 +1584,10 @@ void launch_emf_adj_gather(hipStream_t s, const Geo &g, const MParm &m, const Bo
     EdgeBoxes eb;
     for (int d = 0; d < 3; d++) eb.e[d] = eboxes[d];
     if (m.rmeth)
-        hipLaunchKernelGGL(k_emf_adj_gather<1>, agrid(fb), dim3(ABT), 0, s, g, fb, eb, sc, met,
+        hipLaunchKernelGGL(k_emf_adj_gather<1>, agrid_slab(fb), dim3(ABT), 0, s, g, fb, eb, sc, met,
                            m.doLFM ? 1 : 0, aVf, aHMF, aHG, aSSMF);
     else
-        hipLaunchKernelGGL(k_emf_adj_gather<0>, agrid(fb), dim3(ABT), 0, s, g, fb, eb, sc, met,
+        hipLaunchKernelGGL(k_emf_adj_gather<0>, agrid_slab(fb), dim3(ABT), 0, s, g, fb, eb, sc, met,
                            m.doLFM ? 1 : 0, aVf, aHMF, aHG, aSSMF);
 }
 
