@@ -223,9 +223,10 @@ int gam_get_solver(gam_ctx *ctx, int which, double *out);
  * fixed at compile time, vecLen in kdefs.F90).  The step's tunable
  * launches ("slots": predictor, the EMF of each edge direction, the
  * interface flux of each face direction, the update) each have a few
- * launch shapes that give the same results: the row order of the
- * 256-point chunk tiling (1 = j fastest, 2 = k fastest, 3 = slabs of 8
- * j-rows) and, for the j and k fluxes, the LDS-staged face tile of
+ * launch shapes that give the same results: the chunk tiling of the box's
+ * points, value = row order (1 = j fastest, 2 = k fastest, 3 = slabs of 8
+ * j-rows) + 8 x chunk extent (0: 256 points per workgroup, 1: 512,
+ * 2: 128), and, for the j and k fluxes, the LDS-staged face tile of
  * 64 x 4 (4) or 64 x 8 (5) faces, or those tiles with their rows brought
  * to LDS by direct global-to-LDS loads (6: 64 x 4, 7: 64 x 8).
  * gam_autotune times every candidate of every slot on the context's own
@@ -240,7 +241,9 @@ int gam_autotune(gam_ctx *ctx, int reps);             /* 1 <= reps <= GAM_TUNE_M
 int gam_get_plan(gam_ctx *ctx, int *plan);            /* GAM_TUNE_SLOTS values */
 int gam_set_plan(gam_ctx *ctx, const int *plan);      /* GAM_E_ARG if a value is not a candidate */
 const char *gam_tune_slot_name(int slot);              /* "" outside 0..GAM_TUNE_SLOTS-1 */
-/* Candidate values of a slot (up to 7); returns their number, 0 for a bad slot. */
+/* Candidate values of a slot (up to GAM_TUNE_MAXCAND); returns their number,
+ * 0 for a bad slot. */
+#define GAM_TUNE_MAXCAND 16
 int gam_tune_candidates(int slot, int *values);
 
 /* ---------------------------------------------------------------------

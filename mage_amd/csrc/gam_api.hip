@@ -1219,7 +1219,7 @@ extern "C" const char *gam_tune_slot_name(int slot)
 
 extern "C" int gam_tune_candidates(int slot, int *values)
 {
-    int v[8];
+    int v[GAM_TUNE_MAXCAND];
     const int n = plan_candidates(slot, v);
     if (values) for (int q = 0; q < n; q++) values[q] = v[q];
     return n;
@@ -1237,7 +1237,7 @@ extern "C" int gam_set_plan(gam_ctx *c, const int *plan)
     if (!c || !plan) return fail(GAM_E_ARG, "gam_set_plan: NULL argument");
     graphs_drop(c);   // launch arguments change
     for (int q = 0; q < TS_COUNT; q++) {
-        int v[8];
+        int v[GAM_TUNE_MAXCAND];
         const int n = plan_candidates(q, v);
         bool ok = false;
         for (int r = 0; r < n; r++) ok = ok || v[r] == plan[q];
@@ -1295,7 +1295,7 @@ extern "C" int gam_autotune(gam_ctx *c, int reps)
     // the current shape is timed first and another replaces it only when
     // more than 1 % faster, so timing noise does not flip near-equal shapes
     for (int slot = 0; slot < TS_COUNT && rc == GAM_OK; slot++) {
-        int v[8];
+        int v[GAM_TUNE_MAXCAND];
         const int n = plan_candidates(slot, v);
         int bestv = c->plan.v[slot];
         double best = 0.0;

@@ -52,6 +52,7 @@ struct Tile {
     int ax1;
     int ni, nj, nk;   // extents
     int nb;           // chunks in all
+    int ch;           // points per chunk (= threads per workgroup)
 };
 
 __device__ __forceinline__ bool tile_point(const Tile &t, int w, int &i, int &j, int &k)
@@ -60,7 +61,7 @@ __device__ __forceinline__ bool tile_point(const Tile &t, int w, int &i, int &j,
     const int L = (w & 7) * per + (w >> 3);
     i = j = k = 0x40000000;
     if (L >= t.nb) return false;
-    const unsigned p = (unsigned)L * (BX * BY) + threadIdx.x + BX * threadIdx.y;
+    const unsigned p = (unsigned)L * (unsigned)t.ch + threadIdx.x + BX * threadIdx.y;
     if (p >= (unsigned)t.ni * (unsigned)(t.nj * t.nk)) return false;
     const unsigned R = p / (unsigned)t.ni;
     i = t.b.i0 + (int)(p - R * (unsigned)t.ni);
@@ -93,7 +94,7 @@ __device__ __forceinline__ bool tile_point(const Tile &t, int w, int &i, int &j,
     int i, j, k;                     \
     if (!tile_point((t), (int)bi.x, i, j, k)) return;
 
-static inline Tile tile_of(const Box &b, int ax1)
+static inline Tile tile_of(const Box &b, int ax1, int ch = BX * BY)
 {
     Tile t;
     t.b = b;
@@ -101,10 +102,16 @@ static inline Tile tile_of(const Box &b, int ax1)
     t.ni = b.i1 - b.i0 + 1;
     t.nj = b.j1 - b.j0 + 1;
     t.nk = b.k1 - b.k0 + 1;
+    t.ch = ch;
     const long pts = (long)t.ni * t.nj * t.nk;
-    t.nb = (int)((pts + BX * BY - 1) / (BX * BY));
+    t.nb = (int)((pts + ch - 1) / ch);
     return t;
 }
+
+// A launch plan value of a chunk-tiled slot: the row order in its low 3
+// bits, the chunk size (threads per workgroup) above them.
+static inline int plan_ax(int v) { return v & 7; }
+static inline int plan_chunk(int v) { return (v >> 3) == 1 ? 512 : (v >> 3) == 2 ? 128 : BX * BY; }
 static inline dim3 tgrid(const Tile &t) { return dim3((unsigned)(((t.nb + 7) / 8) * 8)); }
 
 static inline dim3 grid_of(const Box &b)
@@ -215,7 +222,9 @@ static void launch_f(dim3 grid, dim3 block, hipStream_t s, A... a)
 // the limiter state n.
 // Domain: all face points isg..ieg+1 (x) jsg..jeg+1 (x) ksg..keg+1.
 // ====================================================================
+template <int NTH = BX * BY>
 struct k_predict {
+    static constexpr int NT = NTH;
     __device__ __forceinline__ void operator()(const uint3 bi, Geo g, MParm m,
                                                Tile fb, const Clock *clk,
                                                const double *__restrict__ S,
@@ -367,8 +376,9 @@ __device__ __forceinline__ void edge_lr(const Geo &g, const MParm &m, long c,
     bR = r * fA;
 }
 
-template <int ED, int RM, bool BG, bool BORIS>
+template <int ED, int RM, bool BG, bool BORIS, int NTH = BX * BY>
 struct k_emf {
+    static constexpr int NT = NTH;
     __device__ __forceinline__ void operator()(const uint3 bi, Geo g, MParm m,
                                                Tile b, const Clock *clk,
                                                const double *__restrict__ H,
@@ -517,8 +527,9 @@ __device__ __forceinline__ void face_flux_lr(const Geo &g, const MParm &m,
 // stencil point read from global memory (used along i, where the 8 points
 // are contiguous and merge into wide loads)
 // ====================================================================
-template <int D, int RM, bool BG, bool BORIS>
+template <int D, int RM, bool BG, bool BORIS, int NTH = BX * BY>
 struct k_flux {
+    static constexpr int NT = NTH;
     __device__ __forceinline__ void operator()(const uint3 bi, Geo g, MParm m,
                                                Tile b, const double *__restrict__ H,
                                                const double *__restrict__ S,
@@ -808,7 +819,7 @@ __device__ __forceinline__ double ct_face(const Geo &g, const EdgeBoxes &eb, con
 // ====================================================================
 __device__ __forceinline__ void block_min_dt(double v, Clock *clk)
 {
-    __shared__ double red[256 / 64];
+    __shared__ double red[512 / 64];   // up to 8 waves
     for (int off = 32; off > 0; off >>= 1) v = dmin(v, __shfl_xor(v, off, 64));
     const int tid = threadIdx.x + threadIdx.y * blockDim.x;
     if ((tid & 63) == 0) red[tid >> 6] = v;
@@ -834,8 +845,9 @@ __device__ __forceinline__ void block_min_dt(double v, Clock *clk)
 //    (step.F90:91-101).
 // Every thread reaches the block reduction.
 // ====================================================================
-template <bool BG, bool BORIS>
+template <bool BG, bool BORIS, int NTH = BX * BY>
 struct k_update {
+    static constexpr int NT = NTH;
     __device__ __forceinline__ void operator()(const uint3 bi, Geo g, MParm m,
                                                Tile ft, Box ctb, EdgeBoxes eb, Box ab, Box dtb,
                                                Clock *clk, const double *__restrict__ So,
@@ -1449,7 +1461,9 @@ int plan_candidates(int slot, int *vals)
     int n = 0;
     if (slot < 0 || slot >= TS_COUNT) return 0;
     const int maxax = (slot == TS_FLUX_I) ? 2 : 3;
-    for (int a = 1; a <= maxax; a++) vals[n++] = a;
+    // row order x chunk size (256, 512, 128 points per workgroup)
+    for (int c = 0; c < 3; c++)
+        for (int a = 1; a <= maxax; a++) vals[n++] = a + 8 * c;
     if (slot == TS_FLUX_J || slot == TS_FLUX_K) {
         vals[n++] = PLAN_LDS;
         vals[n++] = PLAN_LDS8;
@@ -1459,11 +1473,14 @@ int plan_candidates(int slot, int *vals)
     return n;
 }
 
-void launch_predict(hipStream_t s, int ax, const Geo &g, const MParm &m, const Box &fb, const Clock *clk,
+void launch_predict(hipStream_t s, int plan, const Geo &g, const MParm &m, const Box &fb, const Clock *clk,
                     const double *S, const double *O, const double *met, double *H)
 {
-    const Tile t = tile_of(fb, ax);
-    launch_f<k_predict>(tgrid(t), dim3(BX, BY), s, g, m, t, clk, S, O, met, H);
+    const int ch = plan_chunk(plan);
+    const Tile t = tile_of(fb, plan_ax(plan), ch);
+    if (ch == 512) launch_f<k_predict<512>>(tgrid(t), dim3(BX, 8), s, g, m, t, clk, S, O, met, H);
+    else if (ch == 128) launch_f<k_predict<128>>(tgrid(t), dim3(BX, 2), s, g, m, t, clk, S, O, met, H);
+    else launch_f<k_predict<>>(tgrid(t), dim3(BX, BY), s, g, m, t, clk, S, O, met, H);
 }
 
 void launch_vface_rim(hipStream_t s, const Geo &g, const double *H, const double *met, double *Vf)
@@ -1487,12 +1504,18 @@ void launch_vface_rim(hipStream_t s, const Geo &g, const double *H, const double
 }
 
 template <int ED, int RM, bool BG, bool BO>
-static void emf_t(hipStream_t s, int ax, const Geo &g, const MParm &m, const Box &b, const Clock *clk,
+static void emf_t(hipStream_t s, int plan, const Geo &g, const MParm &m, const Box &b, const Clock *clk,
                   const double *H, const double *S, const double *Vf, const double *met,
                   const double *bg, double *E)
 {
-    const Tile t = tile_of(b, ax);
-    launch_f<k_emf<ED, RM, BG, BO>>(tgrid(t), dim3(BX, BY), s, g, m, t, clk, H, S, Vf, met, bg, E);
+    const int ch = plan_chunk(plan);
+    const Tile t = tile_of(b, plan_ax(plan), ch);
+    if (ch == 512)
+        launch_f<k_emf<ED, RM, BG, BO, 512>>(tgrid(t), dim3(BX, 8), s, g, m, t, clk, H, S, Vf, met, bg, E);
+    else if (ch == 128)
+        launch_f<k_emf<ED, RM, BG, BO, 128>>(tgrid(t), dim3(BX, 2), s, g, m, t, clk, H, S, Vf, met, bg, E);
+    else
+        launch_f<k_emf<ED, RM, BG, BO>>(tgrid(t), dim3(BX, BY), s, g, m, t, clk, H, S, Vf, met, bg, E);
 }
 
 template <int ED>
@@ -1550,8 +1573,14 @@ static void flux_t(hipStream_t s, int form, const Geo &g, const MParm &m, const 
             return;
         }
     }
-    const Tile t = tile_of(b, form >= PLAN_LDS ? 1 : form);
-    launch_f<k_flux<D, RM, BG, BO>>(tgrid(t), dim3(BX, BY), s, g, m, t, H, S, met, bg, gf, mf, vf);
+    const int ch = plan_chunk(form);
+    const Tile t = tile_of(b, plan_ax(form), ch);
+    if (ch == 512)
+        launch_f<k_flux<D, RM, BG, BO, 512>>(tgrid(t), dim3(BX, 8), s, g, m, t, H, S, met, bg, gf, mf, vf);
+    else if (ch == 128)
+        launch_f<k_flux<D, RM, BG, BO, 128>>(tgrid(t), dim3(BX, 2), s, g, m, t, H, S, met, bg, gf, mf, vf);
+    else
+        launch_f<k_flux<D, RM, BG, BO>>(tgrid(t), dim3(BX, BY), s, g, m, t, H, S, met, bg, gf, mf, vf);
 }
 
 template <int D>
@@ -1579,19 +1608,36 @@ void launch_flux(hipStream_t s, int d, int form, const Geo &g, const MParm &m, c
     else flux_d<2>(s, form, g, m, b, H, S, met, bg, gf, mf, vf);
 }
 
-void launch_update(hipStream_t s, int ax, const Geo &g, const MParm &m, const Box &fb, const Box &ctb,
+template <bool BG, bool BO>
+static void update_t(hipStream_t s, int plan, const Geo &g, const MParm &m, const Box &fb, const Box &ctb,
+                     const EdgeBoxes &eb, const Box &ab, const Box &dtb, Clock *clk, const double *So,
+                     double *Sn, const double *gf, const double *mf, const double *met, const double *bg)
+{
+    const int ch = plan_chunk(plan);
+    const Tile t = tile_of(fb, plan_ax(plan), ch);
+    if (ch == 512)
+        launch_f<k_update<BG, BO, 512>>(tgrid(t), dim3(BX, 8), s, g, m, t, ctb, eb, ab, dtb, clk, So, Sn, gf, mf,
+                                        met, bg);
+    else if (ch == 128)
+        launch_f<k_update<BG, BO, 128>>(tgrid(t), dim3(BX, 2), s, g, m, t, ctb, eb, ab, dtb, clk, So, Sn, gf, mf,
+                                        met, bg);
+    else
+        launch_f<k_update<BG, BO>>(tgrid(t), dim3(BX, BY), s, g, m, t, ctb, eb, ab, dtb, clk, So, Sn, gf, mf, met,
+                                   bg);
+}
+
+void launch_update(hipStream_t s, int plan, const Geo &g, const MParm &m, const Box &fb, const Box &ctb,
                    const Box *eboxes, const Box &ab, const Box &dtb, Clock *clk, const double *So,
                    double *Sn, const double *gf, const double *mf, const double *met, const double *bg)
 {
     EdgeBoxes eb;
     for (int d = 0; d < 3; d++) eb.e[d] = eboxes[d];
     const int sel = (m.doBG ? 2 : 0) + (m.doBoris ? 1 : 0);
-    const Tile t = tile_of(fb, ax);
     switch (sel) {
-    case 0: launch_f<k_update<false, false>>(tgrid(t), dim3(BX, BY), s, g, m, t, ctb, eb, ab, dtb, clk, So, Sn, gf, mf, met, bg); break;
-    case 1: launch_f<k_update<false, true>>(tgrid(t), dim3(BX, BY), s, g, m, t, ctb, eb, ab, dtb, clk, So, Sn, gf, mf, met, bg); break;
-    case 2: launch_f<k_update<true, false>>(tgrid(t), dim3(BX, BY), s, g, m, t, ctb, eb, ab, dtb, clk, So, Sn, gf, mf, met, bg); break;
-    default: launch_f<k_update<true, true>>(tgrid(t), dim3(BX, BY), s, g, m, t, ctb, eb, ab, dtb, clk, So, Sn, gf, mf, met, bg); break;
+    case 0: update_t<false, false>(s, plan, g, m, fb, ctb, eb, ab, dtb, clk, So, Sn, gf, mf, met, bg); break;
+    case 1: update_t<false, true>(s, plan, g, m, fb, ctb, eb, ab, dtb, clk, So, Sn, gf, mf, met, bg); break;
+    case 2: update_t<true, false>(s, plan, g, m, fb, ctb, eb, ab, dtb, clk, So, Sn, gf, mf, met, bg); break;
+    default: update_t<true, true>(s, plan, g, m, fb, ctb, eb, ab, dtb, clk, So, Sn, gf, mf, met, bg); break;
     }
 }
 

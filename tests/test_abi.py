@@ -86,10 +86,11 @@ def test_plan_entry_points_without_gpu():
     names = [L.gam_tune_slot_name(q).decode() for q in range(gamhip.TUNE_SLOTS)]
     assert names == ["predict", "emf_i", "emf_j", "emf_k", "flux_i", "flux_j", "flux_k", "update"]
     assert L.gam_tune_slot_name(gamhip.TUNE_SLOTS).decode() == ""
-    v = (C.c_int * 8)()
-    assert L.gam_tune_candidates(0, v) == 3 and list(v[:3]) == [1, 2, 3]
-    assert L.gam_tune_candidates(4, v) == 2
-    assert L.gam_tune_candidates(5, v) == 7 and list(v[3:7]) == [4, 5, 6, 7]
+    v = (C.c_int * 16)()
+    # row order (1, 2, 3) + 8 x chunk extent (256, 512, 128 points)
+    assert L.gam_tune_candidates(0, v) == 9 and list(v[:9]) == [1, 2, 3, 9, 10, 11, 17, 18, 19]
+    assert L.gam_tune_candidates(4, v) == 6 and list(v[:6]) == [1, 2, 9, 10, 17, 18]
+    assert L.gam_tune_candidates(5, v) == 13 and list(v[9:13]) == [4, 5, 6, 7]
     assert L.gam_tune_candidates(8, v) == 0
     plan = (C.c_int * 8)()
     assert L.gam_get_plan(None, plan) == -1

@@ -22,7 +22,7 @@ def _diff(a, b):
 def _candidates(slot):
     L = gamhip.load()
     import ctypes as C
-    v = (C.c_int * 8)()
+    v = (C.c_int * 16)()
     n = L.gam_tune_candidates(slot, v)
     return [v[q] for q in range(n)]
 
