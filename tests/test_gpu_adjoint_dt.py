@@ -172,9 +172,14 @@ def test_dt_following_gradient_matches_free_dt_differences(case):
     grad2 = gS + gO
     mu = adj.dt_grad(NSTEPS)
     dts = adj.step_dt(NSTEPS)
+    nw = len(wind["tW"]) if wind is not None else 0
+    if nw:
+        gQ2, gB2 = adj.wind_grad(nw)
     adj.set_dt_mode(0)
     gS0, gO0 = adj.backward()
     grad0 = gS0 + gO0
+    if nw:
+        gQ0, gB0 = adj.wind_grad(nw)
     # the free-dt replay of the unperturbed start is the recorded run
     assert adj.replay_dt(gas0, gas0, mf0, mf0) == pytest.approx(J, rel=1e-13)
     print("%s: dt_n %s, dJ/d(dt_n) %s" % (case, np.array2string(dts, precision=6),
@@ -186,8 +191,8 @@ def test_dt_following_gradient_matches_free_dt_differences(case):
         if p["doMHD"]:
             oo.recompute_cell_b()
 
-    def J_oracle(gas):
-        oo = po.Oracle(*N, p, grid, wind=wind) if wind is not None else po.Oracle(*N, p, grid)
+    def J_oracle(gas, wnd=wind):
+        oo = po.Oracle(*N, p, grid, wind=wnd) if wnd is not None else po.Oracle(*N, p, grid)
         s = dict(st)
         s["Gas"] = gas
         oo.set_state(s)
@@ -232,6 +237,36 @@ def test_dt_following_gradient_matches_free_dt_differences(case):
         gap.append(best0 / max(best2[0], 1e-16))
     # the held-dt gradient is not the gradient of this run
     assert max(gap) >= 10.0, gap
+    # the inflow series (density and pressure columns, By and Bz; the
+    # velocity moves the lag, a step function)
+    for q in range(2 if nw else 0):
+        dQ = np.zeros(wind["Q"].shape, order="F")
+        dQ[:, 0] = rng.standard_normal(nw) * 1e-2
+        dQ[:, 4] = rng.standard_normal(nw) * 1e-5
+        dB = np.zeros(wind["B"].shape, order="F")
+        dB[:, 1:] = rng.standard_normal((nw, 2)) * 1e-3
+        a2 = float(np.sum(gQ2 * dQ) + np.sum(gB2 * dB))
+        a0 = float(np.sum(gQ0 * dQ) + np.sum(gB0 * dB))
+        best2 = bestO = None
+        for e in (1e-2, 1e-3, 1e-4, 1e-5):
+            wp, wm = dict(wind), dict(wind)
+            wp["Q"], wm["Q"] = wind["Q"] + e * dQ, wind["Q"] - e * dQ
+            wp["B"], wm["B"] = wind["B"] + e * dB, wind["B"] - e * dB
+            g.set_wind(wp)
+            Jp = adj.replay_dt(gas0, gas0, mf0, mf0)
+            g.set_wind(wm)
+            Jm = adj.replay_dt(gas0, gas0, mf0, mf0)
+            g.set_wind(wind)
+            fd = (Jp - Jm) / (2 * e)
+            r2 = abs(fd - a2) / max(abs(fd), abs(a2))
+            best2 = r2 if best2 is None else min(best2, r2)
+            if e >= 1e-4:
+                fo = (J_oracle(gas0, wp) - J_oracle(gas0, wm)) / (2 * e)
+                ro = abs(fo - a2) / max(abs(fo), abs(a2))
+                bestO = ro if bestO is None else min(bestO, ro)
+        print("%s series direction %d: adjoint %.12e (dt held %.12e), best rel vs free-dt FD %.2e, vs oracle "
+              "FD %.2e" % (case, q, a2, a0, best2, bestO), flush=True)
+        assert best2 <= 1e-6 and bestO <= 1e-6, (case, q, best2, bestO)
     adj.close()
     g.close()
 
