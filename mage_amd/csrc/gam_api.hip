@@ -622,9 +622,9 @@ static void enq_advance(gam_ctx *c, int finmode)
     const bool fork = c->fork && c->mp.doMHD;
     hipStream_t fs[3] = {s, s, s};   // stream of the flux of direction d and of its EMF
     if (fork) {
-        hipEventRecord(c->fev[0], s);
+        order_record(s, c->fev[0], 0);
         for (int q = 0; q < 2; q++) {
-            hipStreamWaitEvent(c->aux[q], c->fev[0], 0);
+            order_wait(c->aux[q], c->fev[0], 0);
             fs[1 + q] = c->aux[q];
         }
     }
@@ -647,8 +647,8 @@ static void enq_advance(gam_ctx *c, int finmode)
         launch_vface_rim(s, g, c->H, c->met, c->vf);
         prof_end(c, KG_MOM2FACE);
         if (fork) {
-            hipEventRecord(c->fev[1], s);
-            for (int q = 0; q < 2; q++) hipStreamWaitEvent(c->aux[q], c->fev[1], 0);
+            order_record(s, c->fev[1], 1);
+            for (int q = 0; q < 2; q++) order_wait(c->aux[q], c->fev[1], 1);
         }
         for (int eD = 0; eD < 3; eD++) {
             // edge direction eD takes the face velocities along dT1
@@ -662,8 +662,8 @@ static void enq_advance(gam_ctx *c, int finmode)
         }
         if (fork)
             for (int q = 0; q < 2; q++) {
-                hipEventRecord(c->fev[2 + q], c->aux[q]);
-                hipStreamWaitEvent(s, c->fev[2 + q], 0);
+                order_record(c->aux[q], c->fev[2 + q], 2 + q);
+                order_wait(s, c->fev[2 + q], 2 + q);
             }
     }
 
@@ -939,12 +939,16 @@ extern "C" const char *gam_kernel_group_name(int grp)
 // launches, twice: once from each of its State/oState roles.  Launch p of
 // all members becomes one batched launch (k_ens) over the members'
 // concatenated grids with a device table of their argument records; the
-// batched launches of one step are captured as a hipGraph per role.
+// batched launches of one step are captured as a hipGraph per role, with
+// the members' fork/join of the flux and EMF branches (the same streams
+// and events at the same places, so the graph has the same branches).
 struct EnsPhase {
     void (*launch)(hipStream_t, const void *, const void *, unsigned, dim3);
     dim3 block;
     unsigned nwg;
     size_t hdr_off, rec_off;   // byte offsets into gam_ens::dev
+    int lane;                  // 0 the main stream, 1 and 2 the branch streams
+    int mark, ev;              // EnsLaunch::mark and ::ev; mark 0 is a launch
 };
 
 struct gam_ens {
@@ -954,6 +958,8 @@ struct gam_ens {
     unsigned *gen0;                  // each member's launch-argument generation at recording
     int parity;                      // steps taken mod 2 since recording
     hipStream_t stream;
+    hipStream_t aux[2];              // branch streams of the capture
+    hipEvent_t ev[4];                // fork/join events of the capture
     unsigned char *dev;              // headers and argument records
     std::vector<EnsPhase> ph[2];
     hipGraph_t graph[2];
@@ -970,6 +976,8 @@ extern "C" int gam_ens_destroy(gam_ens *e)
     }
     if (e->dev) hipFree(e->dev);
     if (e->stream) hipStreamDestroy(e->stream);
+    for (int q = 0; q < 2; q++) if (e->aux[q]) hipStreamDestroy(e->aux[q]);
+    for (int q = 0; q < 4; q++) if (e->ev[q]) hipEventDestroy(e->ev[q]);
     free(e->m);
     free(e->cur0);
     free(e->gen0);
@@ -1027,22 +1035,33 @@ extern "C" int gam_ens_create(gam_ens **out, int n, gam_ctx *const *members)
         e->gen0[q] = c->gen;
         const int prof = c->prof;
         c->prof = 0;
+        c->fork = c->concurrent;
         for (int r = 0; r < 2; r++) {
             set_launch_recorder(&rec[r][q]);
             enq_step(c);
         }
         set_launch_recorder(nullptr);
+        c->fork = 0;
         c->prof = prof;
     }
     // every member issues the same number of launches per step; at each
     // position the members are grouped by kernel (their launch plans may
     // pick different shapes), one batched launch per group
+    auto lane_of = [&](int q, hipStream_t st) {
+        return st == members[q]->aux[0] ? 1 : st == members[q]->aux[1] ? 2 : 0;
+    };
     for (int r = 0; r < 2; r++)
-        for (int q = 1; q < n; q++)
-            if (rec[r][q].size() != rec[r][0].size()) {
+        for (int q = 1; q < n; q++) {
+            bool same = rec[r][q].size() == rec[r][0].size();
+            for (size_t p = 0; same && p < rec[r][0].size(); p++) {
+                const EnsLaunch &A = rec[r][0][p], &B = rec[r][q][p];
+                same = A.mark == B.mark && A.ev == B.ev && lane_of(0, A.stream) == lane_of(q, B.stream);
+            }
+            if (!same) {
                 gam_ens_destroy(e);
                 return fail(GAM_E_ARG, "gam_ens_create: member %d issues a different kernel sequence", q);
             }
+        }
     // device layout: per batched launch an EnsHdr, its owner table and the
     // group's argument records, each 256-aligned
     std::vector<unsigned char> host;
@@ -1054,6 +1073,15 @@ extern "C" int gam_ens_create(gam_ens **out, int n, gam_ctx *const *members)
     };
     for (int r = 0; r < 2; r++) {
         for (size_t p = 0; p < rec[r][0].size(); p++) {
+            const int lane = lane_of(0, rec[r][0][p].stream);
+            if (rec[r][0][p].mark) {
+                EnsPhase ph = {};
+                ph.lane = lane;
+                ph.mark = rec[r][0][p].mark;
+                ph.ev = rec[r][0][p].ev;
+                e->ph[r].push_back(ph);
+                continue;
+            }
             std::vector<std::vector<int>> groups;
             for (int q = 0; q < n; q++) {
                 const EnsLaunch &L = rec[r][q][p];
@@ -1090,7 +1118,8 @@ extern "C" int gam_ens_create(gam_ens **out, int n, gam_ctx *const *members)
                 for (int q = 0; q < gn; q++)
                     for (unsigned w = h.wg0[q]; w < h.wg0[q + 1]; w += 8) own[w / 8] = (unsigned short)q;
                 const EnsLaunch &F = rec[r][grp[0]][p];
-                EnsPhase ph;
+                EnsPhase ph = {};
+                ph.lane = lane;
                 ph.launch = F.launch;
                 ph.block = F.block;
                 ph.nwg = (unsigned)tot;
@@ -1109,14 +1138,22 @@ extern "C" int gam_ens_create(gam_ens **out, int n, gam_ctx *const *members)
         }
     }
     hipError_t err = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+    for (int q = 0; err == hipSuccess && q < 2; q++) err = hipStreamCreateWithFlags(&e->aux[q], hipStreamNonBlocking);
+    for (int q = 0; err == hipSuccess && q < 4; q++) err = hipEventCreateWithFlags(&e->ev[q], hipEventDisableTiming);
     if (err == hipSuccess) err = hipMalloc((void **)&e->dev, host.size());
     if (err == hipSuccess) err = hipMemcpy(e->dev, host.data(), host.size(), hipMemcpyHostToDevice);
     for (int r = 0; err == hipSuccess && r < 2; r++) {
         err = hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal);
         if (err != hipSuccess) break;
-        for (const EnsPhase &ph : e->ph[r])
-            ph.launch(e->stream, e->dev + ph.hdr_off, e->dev + ph.rec_off, ph.nwg, ph.block);
-        err = hipStreamEndCapture(e->stream, &e->graph[r]);
+        const hipStream_t ls[3] = {e->stream, e->aux[0], e->aux[1]};
+        for (const EnsPhase &ph : e->ph[r]) {
+            if (ph.mark == 1) err = hipEventRecord(e->ev[ph.ev], ls[ph.lane]);
+            else if (ph.mark == 2) err = hipStreamWaitEvent(ls[ph.lane], e->ev[ph.ev], 0);
+            else ph.launch(ls[ph.lane], e->dev + ph.hdr_off, e->dev + ph.rec_off, ph.nwg, ph.block);
+            if (err != hipSuccess) break;
+        }
+        const hipError_t ec = hipStreamEndCapture(e->stream, &e->graph[r]);
+        if (err == hipSuccess) err = ec;
         if (err == hipSuccess) err = hipGraphInstantiate(&e->exec[r], e->graph[r], nullptr, nullptr, 0);
     }
     if (err != hipSuccess) {
@@ -1202,7 +1239,9 @@ extern "C" int gam_ens_time_steps(gam_ens *e, int nsteps, double *ms)
 extern "C" int gam_ens_launches_per_step(gam_ens *e, int *n)
 {
     if (!e || !n) return fail(GAM_E_ARG, "gam_ens_launches_per_step: NULL argument");
-    *n = (int)e->ph[0].size();
+    int k = 0;
+    for (const EnsPhase &ph : e->ph[0]) k += ph.mark == 0;
+    *n = k;
     return GAM_OK;
 }
 

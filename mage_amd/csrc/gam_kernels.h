@@ -39,10 +39,18 @@ struct EnsLaunch {
     void (*launch)(hipStream_t s, const void *hdr, const void *rec, unsigned nwg, dim3 block);
     dim3 grid, block;
     std::vector<unsigned char> rec;
+    hipStream_t stream;   // the stream the launch or marker was issued to
+    int mark;             // 0 launch; 1 event ev recorded on stream; 2 stream waits for event ev
+    int ev;
 };
 // While r is set, the step kernels that this thread would launch are
 // appended to *r instead (nothing is issued); nullptr restores launching.
 void set_launch_recorder(std::vector<EnsLaunch> *r);
+// Stream order of a branching step: hipEventRecord(e, s) and
+// hipStreamWaitEvent(s, e) at once, or, while a recorder is set, appended
+// to it as markers naming the event by its index ev.
+void order_record(hipStream_t s, hipEvent_t e, int ev);
+void order_wait(hipStream_t s, hipEvent_t e, int ev);
 
 // Launch plan: the shape of every tunable launch of the step.  Values
 // 1, 2, 3 are the Tile row orders (ax1: j fastest, k fastest, j-slabs);

@@ -208,10 +208,36 @@ static void launch_f(dim3 grid, dim3 block, hipStream_t s, A... a)
         e.block = block;
         e.rec.resize(sizeof(p));
         memcpy(e.rec.data(), &p, sizeof(p));
+        e.stream = s;
+        e.mark = 0;
+        e.ev = -1;
         g_rec->push_back(std::move(e));
         return;
     }
     hipLaunchKernelGGL((k_one<F, A...>), grid, block, 0, s, a...);
+}
+
+static void order_mark(hipStream_t s, int mark, int ev)
+{
+    EnsLaunch e;
+    e.launch = nullptr;
+    e.grid = e.block = dim3(0, 0, 0);
+    e.stream = s;
+    e.mark = mark;
+    e.ev = ev;
+    g_rec->push_back(std::move(e));
+}
+
+void order_record(hipStream_t s, hipEvent_t e, int ev)
+{
+    if (g_rec) order_mark(s, 1, ev);
+    else hipEventRecord(e, s);
+}
+
+void order_wait(hipStream_t s, hipEvent_t e, int ev)
+{
+    if (g_rec) order_mark(s, 2, ev);
+    else hipStreamWaitEvent(s, e, 0);
 }
 
 
