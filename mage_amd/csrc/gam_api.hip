@@ -87,6 +87,10 @@ struct gam_ctx {
     // set by the adjoint's forward steps with the dt sensitivities through
     // CalcDT: record each step's CalcDT argmin into dtrec (DT_REC doubles)
     double *dtrec, *dtrec_part;
+    // bumped by everything that may write the forward intermediates (H, vf,
+    // gflx, mflx): steps, step and ensemble graph launches, the adjoint's
+    // recomputes
+    unsigned long itgen;
 };
 
 static void graphs_drop(gam_ctx *c)
@@ -601,6 +605,7 @@ static void tune_mark(gam_ctx *c, int slot)
 // AdvanceMHD on the device (mhdgroup.F90:16-76); swaps State/oState.
 static void enq_advance(gam_ctx *c, int finmode)
 {
+    c->itgen++;
     const Geo &g = c->g;
     const long N3 = g.N3;
     hipStream_t s = c->stream;
@@ -679,6 +684,26 @@ static void enq_advance(gam_ctx *c, int finmode)
     c->cur = 1 - c->cur;
 }
 
+// The forward intermediates of one step as enq_advance computes them
+// (predictor, interface fluxes with their face velocities, the Mom2Face
+// rim), without the rest of the step.
+static void enq_intermediates(gam_ctx *c)
+{
+    c->itgen++;
+    const Geo &g = c->g;
+    const long N3 = g.N3;
+    hipStream_t s = c->stream;
+    double *S = S_cur(c), *O = S_old(c);
+    launch_predict(s, c->plan.v[TS_PREDICT], g, c->mp, face_box(g), c->clk, S, O, c->met, c->H);
+    for (int d = 0; d < 3; d++) {
+        Box fb = act_box(g);
+        if (d == 0) fb.i1 += 1; else if (d == 1) fb.j1 += 1; else fb.k1 += 1;
+        launch_flux(s, d, c->plan.v[TS_FLUX_I + d], g, c->mp, fb, c->H, S, c->met, c->bg, c->gflx, c->mflx,
+                    c->vf + (long)3 * d * N3);
+    }
+    if (c->mp.doMHD) launch_vface_rim(s, g, c->H, c->met, c->vf);
+}
+
 static void enq_step(gam_ctx *c)
 {
     enq_advance(c, FIN_ADVANCE | FIN_MODEL | FIN_CALCDT | FIN_SETDT);
@@ -720,6 +745,7 @@ static int enq_steps(gam_ctx *c, int nsteps)
             }
         }
         HIPCHK(hipGraphLaunch(c->gx[p], c->stream));   // two steps: the parity comes back to p
+        c->itgen++;
     }
     for (; n < nsteps; n++) enq_step(c);
     return GAM_OK;
@@ -1185,6 +1211,7 @@ static int ens_enqueue(gam_ens *e, int nsteps)
         HIPCHK(hipGraphLaunch(e->exec[e->parity], e->stream));
         e->parity ^= 1;
     }
+    for (int q = 0; q < e->n; q++) e->m[q]->itgen++;
     for (int q = 0; q < e->n; q++) e->m[q]->cur = e->cur0[q] ^ e->parity;
     return GAM_OK;
 }
@@ -1389,6 +1416,8 @@ struct gam_adj {
     int cur0;                // State buffer at the start of the run
     int haveW, recorded;
     int lastTaped;           // the tapes hold the last interval from gam_adj_forward
+    int ownLast;             // and the context's own intermediates its last step,
+    unsigned long ownGen;    // while the context's itgen is still ownGen
     double *ck;              // nck x (State, oState), CK_NPL planes each
     double *fin;             // State, oState at the end
     Clock *clkh;             // clock before step n, n = 0..nsteps
@@ -1659,6 +1688,8 @@ extern "C" int gam_adj_forward(gam_adj *a, double *J)
     HIPCHK(hipMemcpyAsync(a->tape + (size_t)(1 + a->nsteps - sL) * CK_NPL * N3, S_cur(c), ckb,
                           hipMemcpyDeviceToDevice, c->stream));
     a->lastTaped = 1;
+    a->ownLast = !it_taped(a, a->nsteps - 1 - sL);
+    a->ownGen = c->itgen;
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(a->ev[1], c->stream));
     c->prof = prof;
@@ -1730,6 +1761,7 @@ static void adj_step(gam_adj *a, int n, const StepIT *it, const double *Sn, cons
     const double *gflx = it ? it->gflx : c->gflx, *mflx = it ? it->mflx : c->mflx;
     // the forward intermediates the transposes read
     if (!it) {
+        c->itgen++;
         launch_predict(s, c->plan.v[TS_PREDICT], g, c->mp, face_box(g), clk, Sn, On, c->met, c->H);
         // the hydrodynamic transposes read no face fluxes, but the dt share
         // of the update does
@@ -1813,6 +1845,11 @@ extern "C" int gam_adj_backward(gam_adj *a, double *gGas, double *gGasO, double 
     // Interval seg recomputed from its checkpoint onto tape pair buf: on the
     // second stream when the sweep overlaps, else on the context's stream.
     const bool ovl = a->tape2 != nullptr;
+    // Step whose forward intermediates the context's own buffers hold: the
+    // last step of an interval recomputed without overlap, when it is not
+    // taped.  Its adjoint step comes right after that recompute and uses
+    // them instead of recomputing its predictor, face velocities and fluxes.
+    int ownStep = -1;
     auto recompute = [&](int seg, int buf) -> int {
         const int s0 = seg * a->K, s1 = (s0 + a->K < a->nsteps) ? s0 + a->K : a->nsteps;
         const double *ck = a->ck + (size_t)seg * 2 * CK_NPL * N3;
@@ -1823,24 +1860,36 @@ extern "C" int gam_adj_backward(gam_adj *a, double *gGas, double *gGasO, double 
         // tape[0] = oState at s0, tape[1 + q] = State at step s0 + q, q = 0..s1-s0
         if (!r && hipMemcpyAsync(tp, ck + (size_t)CK_NPL * N3, bytes, hipMemcpyDeviceToDevice, c->stream))
             r = fail(GAM_E_HIP, "gam_adj_backward: tape copy failed");
-        for (int q = 0; !r && q <= s1 - s0; q++) {
+        for (int q = 0; !r && q < s1 - s0; q++) {
             if (hipMemcpyAsync(tp + (size_t)(1 + q) * CK_NPL * N3, S_cur(c), bytes, hipMemcpyDeviceToDevice,
                                c->stream))
                 r = fail(GAM_E_HIP, "gam_adj_backward: tape copy failed");
-            if (!r && q < s1 - s0) {
-                it_use(a, q, buf, own);   // a taped step writes its intermediates into its slot
+            if (r) break;
+            it_use(a, q, buf, own);   // a taped step writes its intermediates into its slot
+            if (q + 1 < s1 - s0) {
                 dt_rec_use(a, s0 + q);
                 enq_step(c);
+            } else {
+                // the last step: its adjoint step reads only its intermediates
+                // (its dt record is the forward run's), and the State it
+                // would give is the next checkpoint's, or the final one
+                enq_intermediates(c);
             }
         }
+        const double *nxt = seg + 1 < a->nck ? a->ck + (size_t)(seg + 1) * 2 * CK_NPL * N3 : a->fin;
+        if (!r && hipMemcpyAsync(tp + (size_t)(1 + s1 - s0) * CK_NPL * N3, nxt, bytes, hipMemcpyDeviceToDevice,
+                                 c->stream))
+            r = fail(GAM_E_HIP, "gam_adj_backward: tape copy failed");
         c->dtrec = nullptr;
         if (!r && ovl && hipEventRecord(a->evR[buf], c->stream)) r = fail(GAM_E_HIP, "gam_adj_backward: event");
         c->H = own.H; c->vf = own.vf; c->gflx = own.gflx; c->mflx = own.mflx;
         c->stream = s;
+        ownStep = (!ovl && !it_taped(a, s1 - s0 - 1)) ? s1 - 1 : -1;
         return r;
     };
     // the last interval: from the recorded forward run's tapes, or recomputed
     const bool reuse = a->lastTaped;
+    if (reuse && !ovl && a->ownLast && c->itgen == a->ownGen) ownStep = a->nsteps - 1;
     if (!reuse && (rc = recompute(a->nck - 1, 0))) return rc;
     for (int seg = a->nck - 1; seg >= 0; seg--) {
         const int s0 = seg * a->K, s1 = (s0 + a->K < a->nsteps) ? s0 + a->K : a->nsteps;
@@ -1859,10 +1908,12 @@ extern "C" int gam_adj_backward(gam_adj *a, double *gGas, double *gGasO, double 
         const double *tp = buf ? a->tape2 : a->tape;
         for (int n = s1 - 1; n >= s0; n--) {
             const int q = n - s0;
-            const bool tpd = it_taped(a, q);
+            const bool tpd = it_taped(a, q), own = !tpd && n == ownStep;
             const StepIT its = tpd ? it_slot(a, q, buf) : StepIT{c->H, c->vf, c->gflx, c->mflx};
-            adj_step(a, n, tpd ? &its : nullptr, tp + (size_t)(1 + q) * CK_NPL * N3, tp + (size_t)q * CK_NPL * N3,
-                     tp + (size_t)(2 + q) * CK_NPL * N3, a->clkh + n, a->clkh + n + 1, lA, lB, lC);
+            adj_step(a, n, (tpd || own) ? &its : nullptr, tp + (size_t)(1 + q) * CK_NPL * N3,
+                     tp + (size_t)q * CK_NPL * N3, tp + (size_t)(2 + q) * CK_NPL * N3, a->clkh + n, a->clkh + n + 1,
+                     lA, lB, lC);
+            ownStep = -1;
             double *t = lA;
             lA = lB;
             lB = lC;

@@ -79,3 +79,25 @@ def test_adjoint_tape_budgets_match_full_tape():
             err = np.max(np.abs(a - b)) / np.max(np.abs(b))
             assert err <= 1e-12, (budget, nm, err)
     g.close()
+
+
+def test_adjoint_after_the_context_stepped_again():
+    """Steps of the context between gam_adj_forward and gam_adj_backward
+    rewrite its own forward intermediates; the backward must not take them
+    for the recorded last step's (it recomputes them): the gradients equal
+    those of a backward right after the forward."""
+    print("adjoint after further steps: 24x16x24 MHD shell, 7 steps, ckpt 3, no taped steps", flush=True)
+    g, w, nw = _case()
+    snap = _snapshot(g)
+    J0, ref, _ = _grads(g, w, nw, snap, max_taped=0, overlap=False)
+    _restore(g, snap)
+    adj = gamhip.GamAdjoint(g, 7, 3, max_taped=0, overlap=False)
+    adj.set_weights(w)
+    J = adj.forward()
+    g.run(3)
+    got = list(adj.backward(fluxes=True)) + list(adj.wind_grad(nw))
+    adj.close()
+    assert J == J0
+    for a, b in zip(got, ref):
+        assert np.max(np.abs(a - b)) <= 1e-12 * np.max(np.abs(b))
+    g.close()
