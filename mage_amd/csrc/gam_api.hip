@@ -91,6 +91,10 @@ struct gam_ctx {
     // gflx, mflx): steps, step and ensemble graph launches, the adjoint's
     // recomputes
     unsigned long itgen;
+    // when set, the next step writes its new State there instead of over
+    // oState's buffer, and that buffer becomes State (the adjoint's interval
+    // recompute steps through its tape this way)
+    double *snx;
 };
 
 static void graphs_drop(gam_ctx *c)
@@ -610,7 +614,7 @@ static void enq_advance(gam_ctx *c, int finmode)
     const long N3 = g.N3;
     hipStream_t s = c->stream;
     double *S = S_cur(c), *O = S_old(c);
-    double *Sn = O;   // becomes State
+    double *Sn = c->snx ? c->snx : O;   // becomes State
 
     prof_begin(c, KG_PREDICT);
     tune_mark(c, TS_PREDICT);
@@ -681,6 +685,10 @@ static void enq_advance(gam_ctx *c, int finmode)
     prof_end(c, KG_UPDATE);
 
     launch_finalize(s, c->mp, c->clk, finmode);
+    if (c->snx) {
+        c->st[1 - c->cur] = c->snx;
+        c->snx = nullptr;
+    }
     c->cur = 1 - c->cur;
 }
 
@@ -1856,17 +1864,26 @@ extern "C" int gam_adj_backward(gam_adj *a, double *gGas, double *gGasO, double 
         double *tp = buf ? a->tape2 : a->tape;
         const StepIT own = {c->H, c->vf, c->gflx, c->mflx};
         c->stream = ovl ? a->rs : s;
-        int r = adj_put(a, s0, ck, CK_NPL);
-        // tape[0] = oState at s0, tape[1 + q] = State at step s0 + q, q = 0..s1-s0
-        if (!r && hipMemcpyAsync(tp, ck + (size_t)CK_NPL * N3, bytes, hipMemcpyDeviceToDevice, c->stream))
+        // tape[0] = oState at s0, tape[1 + q] = State at step s0 + q, q = 0..s1-s0.
+        // The steps run on the tape itself: step q reads State from slot
+        // 1 + q and oState from slot q and writes its new State into slot
+        // 2 + q (its 3 EMF planes reach past the slot's 11 into the first
+        // planes of slot 3 + q, which nothing reads before the next step
+        // overwrites them), so no State is copied onto the tape.
+        auto slot = [&](int k) { return tp + (size_t)k * CK_NPL * N3; };
+        double *keep[2] = {c->st[0], c->st[1]};
+        c->cur = a->cur0 ^ (s0 & 1);
+        int r = GAM_OK;
+        if (hipMemcpyAsync(slot(1), ck, bytes, hipMemcpyDeviceToDevice, c->stream) ||
+            hipMemcpyAsync(slot(0), ck + (size_t)CK_NPL * N3, bytes, hipMemcpyDeviceToDevice, c->stream) ||
+            hipMemcpyAsync(c->clk, a->clkh + s0, sizeof(Clock), hipMemcpyDeviceToDevice, c->stream))
             r = fail(GAM_E_HIP, "gam_adj_backward: tape copy failed");
+        c->st[c->cur] = slot(1);
+        c->st[1 - c->cur] = slot(0);
         for (int q = 0; !r && q < s1 - s0; q++) {
-            if (hipMemcpyAsync(tp + (size_t)(1 + q) * CK_NPL * N3, S_cur(c), bytes, hipMemcpyDeviceToDevice,
-                               c->stream))
-                r = fail(GAM_E_HIP, "gam_adj_backward: tape copy failed");
-            if (r) break;
             it_use(a, q, buf, own);   // a taped step writes its intermediates into its slot
             if (q + 1 < s1 - s0) {
+                c->snx = slot(2 + q);
                 dt_rec_use(a, s0 + q);
                 enq_step(c);
             } else {
@@ -1876,9 +1893,11 @@ extern "C" int gam_adj_backward(gam_adj *a, double *gGas, double *gGasO, double 
                 enq_intermediates(c);
             }
         }
+        c->snx = nullptr;
+        c->st[0] = keep[0];
+        c->st[1] = keep[1];
         const double *nxt = seg + 1 < a->nck ? a->ck + (size_t)(seg + 1) * 2 * CK_NPL * N3 : a->fin;
-        if (!r && hipMemcpyAsync(tp + (size_t)(1 + s1 - s0) * CK_NPL * N3, nxt, bytes, hipMemcpyDeviceToDevice,
-                                 c->stream))
+        if (!r && hipMemcpyAsync(slot(1 + s1 - s0), nxt, bytes, hipMemcpyDeviceToDevice, c->stream))
             r = fail(GAM_E_HIP, "gam_adj_backward: tape copy failed");
         c->dtrec = nullptr;
         if (!r && ovl && hipEventRecord(a->evR[buf], c->stream)) r = fail(GAM_E_HIP, "gam_adj_backward: event");
