@@ -232,7 +232,9 @@ int gam_get_solver(gam_ctx *ctx, int which, double *out);
  * gam_autotune times every candidate of every slot on the context's own
  * grid (reps steps per candidate, hipEvents around that slot's launch),
  * keeps the fastest (the current shape unless another is more than 1 %
- * faster), and restores State, oState, time and dt as they were; the
+ * faster), keeps the resulting plan only if 4 x reps whole graph-replayed
+ * steps also run more than 1 % faster with it than with the plan it
+ * started from, and restores State, oState, time and dt as they were; the
  * solver intermediates (gam_get_solver) are then those of a tuning step.
  * ------------------------------------------------------------------- */
 #define GAM_TUNE_SLOTS 8
