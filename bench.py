@@ -414,6 +414,7 @@ def main():
     ap.add_argument("--no-c2", action="store_true", help="skip the 192x192x256 shell line")
     ap.add_argument("--no-adj", action="store_true", help="skip both adjoint lines")
     ap.add_argument("--no-c4-adj", action="store_true", help="skip the 96x96x128 adjoint line")
+    ap.add_argument("--no-full-adj", action="store_true", help="skip the full-size adjoint line")
     ap.add_argument("--adj-fd", type=int, default=0,
                     help="central-difference check of the 96x96x128 adjoint with this many directions")
     ap.add_argument("--ens-members", type=int, default=32)
@@ -478,7 +479,7 @@ def main():
                 "source": os.path.relpath(FLOP_SUMMARY, ROOT)}
 
     adj5 = None
-    if not args.no_adj:
+    if not args.no_adj and not args.no_full_adj:
         t1 = time.time()
         adj5 = run_adjoint(gamhip, g, 100, 10, ms_per_step=repeat["median_ms_per_step"])
         log("adjoint over 100 more steps: %s (%.1f s)" % (adj5, time.time() - t1))
