@@ -41,8 +41,13 @@ def _diff(a, b):
     return max(rel_linf(sa[k], sb[k]) for k in KEYS)
 
 
-def test_ensemble_matches_members_stepped_alone():
-    print("ensemble sizes", [s["N"] for s in SPECS], flush=True)
+@pytest.mark.parametrize("concurrent", ["1", "0"])
+def test_ensemble_matches_members_stepped_alone(concurrent, monkeypatch):
+    """concurrent: GAM_CONCURRENT at context creation; "1" (the default)
+    records the members' flux/EMF branches and the batched graph keeps
+    them, "0" records a single chain."""
+    monkeypatch.setenv("GAM_CONCURRENT", concurrent)
+    print("ensemble sizes", [s["N"] for s in SPECS], "concurrent", concurrent, flush=True)
     batched = [member(s) for s in SPECS]
     alone = [member(s) for s in SPECS]
     ens = gamhip.GamEnsemble(batched)
