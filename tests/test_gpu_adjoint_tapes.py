@@ -101,3 +101,33 @@ def test_adjoint_after_the_context_stepped_again():
     for a, b in zip(got, ref):
         assert np.max(np.abs(a - b)) <= 1e-12 * np.max(np.abs(b))
     g.close()
+
+
+def _grads_dt(g, w, nw, snap, **budget):
+    _restore(g, snap)
+    adj = gamhip.GamAdjoint(g, 7, 3, **budget)
+    adj.set_weights(w)
+    adj.set_dt_mode(3)
+    J = adj.forward()
+    out = list(adj.backward(fluxes=True)) + list(adj.wind_grad(nw)) + [adj.dt_grad(7)]
+    adj.close()
+    return J, out
+
+
+def test_dt_following_adjoint_on_every_tape_path():
+    """With every dt following CalcDT (mode 3), the reduced tape paths (the
+    interval's last step up to its fluxes only, taking its CalcDT record
+    from the forward run; its intermediates from the context's buffers
+    when untaped) give the full-tape gradients, dt sensitivities included."""
+    print("dt-following adjoint on the tape paths: 24x16x24 MHD shell, 7 steps, ckpt 3", flush=True)
+    g, w, nw = _case()
+    snap = _snapshot(g)
+    J0, ref = _grads_dt(g, w, nw, snap)
+    assert np.max(np.abs(ref[-1])) > 0.0
+    for budget in ({"overlap": False}, {"max_taped": 2}, {"max_taped": 0, "overlap": False}):
+        print("adjoint tape budget %s, dt mode 3" % budget, flush=True)
+        J, got = _grads_dt(g, w, nw, snap, **budget)
+        assert J == J0
+        for a, b in zip(got, ref):
+            assert np.max(np.abs(a - b)) <= 1e-12 * np.max(np.abs(b)), budget
+    g.close()
